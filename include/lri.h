@@ -1,0 +1,180 @@
+/*
+ * lri.h -- C ABI of liblri_hip.so, the MI355X (gfx950) engine for the per-step
+ * hot path of dynamical low-rank approximation (DLRA) on data problems.
+ *
+ * The reference (LowRankIntegrators.jl) is pure Julia; its "operator API" for
+ * this path is multiple dispatch on (problem, algorithm).  Each entry point
+ * below replaces one piece of it; the Julia file:line it stands in for is
+ * given with every declaration.
+ *
+ * Conventions
+ *  - Every function returns an int status: LRI_OK (0) or a negative LRI_E*.
+ *    No C++ exception crosses the ABI.  lri_last_error() gives the message.
+ *  - A context is single-threaded.  Step calls are asynchronous and ordered on
+ *    the context's HIP stream; device-side conditions (a non-positive-definite
+ *    Gram matrix, non-finite values) are latched on the device and returned by
+ *    the next synchronising call (lri_synchronize, lri_get_state,
+ *    lri_get_rank, lri_step_bug_ra).
+ *  - Matrices crossing the ABI are column-major (Julia's layout) with an
+ *    explicit leading dimension.  U is n x r, S is r x r, V is m x r, and
+ *    Y = U * S * V'.  The state is held in FP64 on the device.
+ *  - Snapshots are n x m column-major arrays A_k.  Steps take snapshot
+ *    indices k_prev, k_next and use the increment dA = A[k_next] - A[k_prev],
+ *    formed on the fly (it is never materialised).
+ */
+#ifndef LRI_H
+#define LRI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lri_ctx lri_ctx;
+
+/* status codes */
+#define LRI_OK 0
+#define LRI_EINVAL (-1)      /* bad argument (the reference raises AssertionError/DimensionMismatch) */
+#define LRI_EHIP (-2)        /* a HIP runtime call failed */
+#define LRI_EOOM (-3)        /* device allocation failed */
+#define LRI_EBREAKDOWN (-4)  /* a Gram matrix in CholeskyQR2 was numerically singular */
+#define LRI_ENONFINITE (-5)  /* NaN/Inf reached a Gram matrix (e.g. NaN in a snapshot) */
+#define LRI_ENOTSUP (-6)     /* combination not supported by this build */
+
+/* snapshot storage types */
+#define LRI_F64 0
+#define LRI_F32 1
+#define LRI_BF16 2
+
+/* arithmetic of the n x m x r products (everything else is always FP64) */
+#define LRI_PREC_F64 0   /* FP64 products; F32/BF16 snapshots are decoded exactly */
+#define LRI_PREC_MIXED 1 /* BF16 snapshots: BF16 MFMA with hi/lo split (3 products), FP32 accumulate;
+                            F32 snapshots: FP32 MFMA; F64 snapshots: FP64 (as LRI_PREC_F64) */
+
+/* projector-splitting orders (projector_splitting.jl:1-3) */
+#define LRI_PRIMAL 0
+#define LRI_DUAL 1
+#define LRI_STRANG 2
+
+/* limits of this build */
+#define LRI_MAX_RANK 64 /* state rank capacity; the rank-adaptive step works on 2r <= 128 */
+
+/* ---- library / context ------------------------------------------------- */
+
+/* Version as 10000*major + 100*minor + patch. */
+int lri_version(void);
+
+/* Static text for a status code. */
+const char* lri_status_string(int status);
+
+/* Message of the last failure on ctx; with ctx == NULL, the last failure of
+ * lri_create on the calling thread. Never NULL. */
+const char* lri_last_error(const lri_ctx* ctx);
+
+/* Create a context on HIP device `device` for n x m data and ranks up to
+ * rcap (1 <= rcap <= LRI_MAX_RANK, rcap <= min(n, m)).  Allocates all work
+ * buffers once (the reference allocates its caches in alg_cache,
+ * projector_splitting.jl:87-105, unconventional.jl:86-107,
+ * rank_adaptive_unconventional.jl:106-131, and again on every rank change,
+ * :133-169; here rank changes reuse the same buffers). */
+int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap);
+
+/* Free the context and everything it owns (bound snapshots are borrowed and
+ * are not freed).  NULL is accepted. */
+int lri_destroy(lri_ctx* ctx);
+
+/* Select the product arithmetic (LRI_PREC_F64 / LRI_PREC_MIXED). */
+int lri_set_precision(lri_ctx* ctx, int precision);
+
+/* ---- state (the SVDLikeRepresentation u of DLRIntegrator, primitives.jl:54-63) */
+
+/* Copy Y = U*S*V' (host, column-major FP64) into the device state.  Replaces
+ * `u = deepcopy(prob.u0)` in init (projector_splitting.jl:110,
+ * unconventional.jl:112, rank_adaptive_unconventional.jl:97).
+ * U must have orthonormal columns (as the reference assumes). */
+int lri_set_state(lri_ctx* ctx, int64_t r, const double* U, int64_t ldu, const double* S,
+                  int64_t lds, const double* V, int64_t ldv);
+
+/* Synchronise, then copy the state to host (the `deepcopy(integrator.u)` of
+ * update_sol!, primitives.jl:82-90).  Any pointer may be NULL to skip that
+ * factor.  *r_out receives the current rank. */
+int lri_get_state(lri_ctx* ctx, int64_t* r_out, double* U, int64_t ldu, double* S, int64_t lds,
+                  double* V, int64_t ldv);
+
+/* Current rank (synchronising). */
+int lri_get_rank(lri_ctx* ctx, int64_t* r_out);
+
+/* ---- data (update_data!, data_integrator.jl:22-28) --------------------- */
+
+/* Allocate `count` library-owned snapshot slots of type dtype (ld = n). */
+int lri_snapshots_reserve(lri_ctx* ctx, int64_t count, int dtype);
+
+/* Copy one host snapshot (column-major, leading dimension ld >= n, same
+ * dtype as reserved) into slot k.  Stream-ordered with the steps. */
+int lri_snapshot_upload(lri_ctx* ctx, int64_t k, const void* host, int64_t ld);
+
+/* Use caller-owned device arrays as snapshots (borrowed; the caller keeps
+ * them alive).  ld >= n is the column stride in elements. */
+int lri_snapshots_bind(lri_ctx* ctx, const void* const* dev_ptrs, int64_t count, int dtype,
+                       int64_t ld);
+
+/* ---- steps ------------------------------------------------------------- */
+
+/* One projector-splitting (KSL) step with dA = A[k_next] - A[k_prev].
+ * order LRI_PRIMAL: primal_LT_step! (projector_splitting.jl:117-152,191-196)
+ * order LRI_DUAL:   dual_LT_step!   (projector_splitting.jl:154-189,198-203)
+ * (LRI_STRANG needs a midpoint snapshot: use lri_step_ksl_strang.) */
+int lri_step_ksl(lri_ctx* ctx, int order, int64_t k_prev, int64_t k_next);
+
+/* Strang KSL step: primal half step on A[k_mid]-A[k_prev], then dual half
+ * step on A[k_next]-A[k_mid] (projector_splitting.jl:205-211). */
+int lri_step_ksl_strang(lri_ctx* ctx, int64_t k_prev, int64_t k_mid, int64_t k_next);
+
+/* Unconventional (BUG) step (unconventional.jl:121-164). */
+int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next);
+
+/* Rank-adaptive BUG step (rank_adaptive_unconventional.jl:171-233).  The new
+ * rank min(rmax, truncate_to_tolerance(sigma, tol)) -- further clipped to the
+ * context capacity -- is written to *r_new (synchronising). */
+int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax,
+                    int64_t* r_new);
+
+/* Wait for all queued work; returns a latched device-side status, if any. */
+int lri_synchronize(lri_ctx* ctx);
+
+/* The context's HIP stream (hipStream_t), for callers that order their own
+ * work (e.g. snapshot generation) with the steps. */
+int lri_get_stream(lri_ctx* ctx, void** stream);
+
+/* ---- measurement ------------------------------------------------------- */
+
+/* Kernel families timed by lri_kernel_timing_read. */
+#define LRI_KERNEL_PASS_N 0 /* dA * B, reduction over the m columns */
+#define LRI_KERNEL_PASS_T 1 /* dA' * B, reduction over the n rows */
+#define LRI_KERNEL_SMALL 2  /* all FP64 n x r / r x r kernels of a step */
+#define LRI_KERNEL_COUNT 3
+
+/* Enable (1) or disable (0) HIP-event timing of every launch of the
+ * families above on the context's stream; enabling resets the totals. */
+int lri_kernel_timing(lri_ctx* ctx, int enable);
+
+/* Synchronise and return the summed duration (ms) and launch count of a
+ * kernel family since timing was enabled. */
+int lri_kernel_timing_read(lri_ctx* ctx, int family, double* total_ms, int64_t* launches);
+
+/* ---- diagnostics --------------------------------------------------------- */
+
+/* Run one matrix-core instruction on device 0 with the caller's operands and
+ * return its raw per-lane results, so a test can check the operand/result
+ * lane layouts the kernels assume.  which: 0 = FP64 16x16x4 (in: 64 + 64
+ * doubles, out: 256 doubles), 1 = BF16 32x32x16 (in: 512 + 512 bf16 bit
+ * patterns, out: 1024 floats), 2 = FP32 32x32x2 (in: 64 + 64 floats, out:
+ * 1024 floats).  Results are ordered [lane][register]. */
+int lri_selftest_mfma(int which, const void* a, const void* b, void* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LRI_H */

@@ -1,0 +1,378 @@
+"""Host-side mirror of the reference's operator API for the data-driven DLRA path.
+
+Same names, argument meaning and error behaviour as LowRankIntegrators.jl
+(src/primitives.jl, src/integrators/*.jl); every step runs in liblri_hip.so on
+the GPU.  Julia's ``step!`` is spelled ``step`` here.
+
+    prob = MatrixDataProblem(y, u0, tspan)           # y: t -> matrix, or a list of snapshots
+    sol  = solve(prob, ProjectorSplitting(PrimalLieTrotter()), dt)
+    sol.Y[-1].U, sol.Y[-1].S, sol.Y[-1].V, sol.t
+
+Extensions beyond the reference (keyword-only): ``precision`` ("f64" or
+"mixed"), ``snapshot_dtype`` ("f64", "f32", "bf16"), ``save_everystep``,
+``device``.  ``MatrixDEProblem`` and the custom K/L/S right-hand-side hooks are
+part of the reference's DE path, which this engine does not run: they raise
+NotImplementedError.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Any, Callable, List, Optional, Sequence, Union
+
+import numpy as np
+
+from . import native
+
+# ----------------------------------------------------------------------------
+# low-rank types (LowRankArithmetic.jl names)
+# ----------------------------------------------------------------------------
+
+
+@dataclass
+class SVDLikeRepresentation:
+    """Y = U * S * V' (LowRankArithmetic ``SVDLikeRepresentation``), host FP64 arrays."""
+
+    U: np.ndarray
+    S: np.ndarray
+    V: np.ndarray
+
+    @property
+    def rank(self) -> int:
+        return int(self.S.shape[0])
+
+    @property
+    def shape(self):
+        return (self.U.shape[0], self.V.shape[0])
+
+    def matrix(self) -> np.ndarray:
+        return self.U @ self.S @ self.V.T
+
+    def copy(self) -> "SVDLikeRepresentation":
+        return SVDLikeRepresentation(self.U.copy(), self.S.copy(), self.V.copy())
+
+
+@dataclass
+class TwoFactorRepresentation:
+    """Y = U * Z' (LowRankArithmetic ``TwoFactorRepresentation``)."""
+
+    U: np.ndarray
+    Z: np.ndarray
+
+    def matrix(self) -> np.ndarray:
+        return self.U @ self.Z.T
+
+
+def Matrix(u) -> np.ndarray:
+    """Dense matrix of a low-rank representation (Julia ``Matrix(u)``)."""
+    return u.matrix()
+
+
+def rank(u) -> int:
+    return u.rank
+
+
+def truncated_svd(A: np.ndarray, r: Optional[int] = None, tol: Optional[float] = None) -> SVDLikeRepresentation:
+    """Rank-r (or tolerance) truncated SVD of a dense host matrix (initial data, LAPACK)."""
+    U, s, Vt = np.linalg.svd(np.asarray(A, dtype=np.float64), full_matrices=False)
+    if r is None:
+        if tol is None:
+            raise ValueError("truncated_svd needs r or tol")
+        tail = np.sqrt(np.concatenate([np.cumsum(s[::-1] ** 2)[::-1], [0.0]]))
+        r = next(rho for rho in range(1, len(s) + 1) if tail[rho] <= tol) if len(s) else 0
+    r = int(min(r, len(s)))
+    return SVDLikeRepresentation(U[:, :r].copy(), np.diag(s[:r]), Vt[:r].T.copy())
+
+
+# ----------------------------------------------------------------------------
+# problems (primitives.jl:13-41)
+# ----------------------------------------------------------------------------
+
+
+class AbstractDLRProblem:
+    pass
+
+
+@dataclass
+class MatrixDataProblem(AbstractDLRProblem):
+    """Track a low-rank factorisation of data y(t) (primitives.jl:23-30).
+
+    ``y`` is a callable t -> (n x m) array or a sequence of snapshots (host
+    arrays, or device tensors exposing ``data_ptr()``, all of one dtype and
+    column-major / Fortran-ordered).  For a sequence the time span defaults to
+    ``(1, len(y))`` and the step is the integer index step 1.
+    """
+
+    y: Union[Callable[[float], Any], Sequence[Any]]
+    u0: SVDLikeRepresentation
+    tspan: Optional[tuple] = None
+
+    def __post_init__(self):
+        if self.tspan is None:
+            if callable(self.y):
+                raise TypeError("MatrixDataProblem(y::Function, u0) needs a tspan")
+            self.tspan = (1, len(self.y))
+
+
+class MatrixDEProblem(AbstractDLRProblem):
+    """The reference's matrix-ODE problem (primitives.jl:13-17): not run by this engine."""
+
+    def __init__(self, f, u0, tspan):
+        self.f, self.u0, self.tspan = f, u0, tspan
+
+
+class MatrixHybridProblem(AbstractDLRProblem):
+    """The reference's hybrid problem (primitives.jl:36-41, Greedy integrator only): not run here."""
+
+    def __init__(self, y, f, u0, tspan):
+        self.y, self.f, self.u0, self.tspan = y, f, u0, tspan
+
+
+# ----------------------------------------------------------------------------
+# algorithms (projector_splitting.jl:1-41, unconventional.jl:1-21,
+#             rank_adaptive_unconventional.jl:1-23)
+# ----------------------------------------------------------------------------
+
+
+class PrimalLieTrotter:
+    pass
+
+
+class DualLieTrotter:
+    pass
+
+
+class Strang:
+    pass
+
+
+class AbstractDLRAlgorithm:
+    pass
+
+
+def _no_hooks(kw):
+    hooks = [k for k in ("S_rhs", "L_rhs", "K_rhs") if kw.get(k) is not None]
+    if hooks:
+        raise NotImplementedError(f"custom {', '.join(hooks)} hooks belong to the DE path, which this engine does not run")
+
+
+class ProjectorSplitting(AbstractDLRAlgorithm):
+    """KSL projector-splitting integrator (Lubich-Oseledets 2014)."""
+
+    def __init__(self, order=None, *, S_rhs=None, L_rhs=None, K_rhs=None, S_kwargs=None, L_kwargs=None,
+                 K_kwargs=None, S_alg=None, L_alg=None, K_alg=None):
+        _no_hooks(dict(S_rhs=S_rhs, L_rhs=L_rhs, K_rhs=K_rhs))
+        self.order = PrimalLieTrotter() if order is None else order
+        if not isinstance(self.order, (PrimalLieTrotter, DualLieTrotter, Strang)):
+            raise TypeError(f"unknown projector-splitting order {self.order!r}")
+
+
+class UnconventionalAlgorithm(AbstractDLRAlgorithm):
+    """Unconventional (BUG) integrator (Ceruti-Lubich 2021)."""
+
+    def __init__(self, *, S_rhs=None, L_rhs=None, K_rhs=None, S_kwargs=None, L_kwargs=None, K_kwargs=None,
+                 S_alg=None, L_alg=None, K_alg=None):
+        _no_hooks(dict(S_rhs=S_rhs, L_rhs=L_rhs, K_rhs=K_rhs))
+
+
+class RankAdaptiveUnconventionalAlgorithm(AbstractDLRAlgorithm):
+    """Rank-adaptive BUG integrator (Ceruti-Kusch-Lubich); rank capped at ``native.MAX_RANK``."""
+
+    def __init__(self, tol, *, S_rhs=None, L_rhs=None, K_rhs=None, S_kwargs=None, L_kwargs=None,
+                 K_kwargs=None, S_alg=None, L_alg=None, K_alg=None, rmax=2 ** 62):
+        _no_hooks(dict(S_rhs=S_rhs, L_rhs=L_rhs, K_rhs=K_rhs))
+        self.tol = float(tol)
+        self.rmax = int(rmax)
+
+
+# ----------------------------------------------------------------------------
+# solution / integrator (primitives.jl:46-104)
+# ----------------------------------------------------------------------------
+
+
+@dataclass
+class DLRSolution:
+    Y: List[Any]
+    t: List[Any]
+
+
+def _is_int(x) -> bool:
+    return isinstance(x, (int, np.integer)) and not isinstance(x, bool)
+
+
+def init_sol(dt, t0, tf, u0) -> DLRSolution:
+    """``init_sol`` (primitives.jl:92-104)."""
+    if _is_int(dt) and _is_int(t0) and _is_int(tf):
+        steps = list(range(t0, tf + 1, dt))
+        return DLRSolution([None] * len(steps), steps)
+    n = int(math.floor((tf - t0) / dt)) + 1
+    return DLRSolution([None] * n, list(np.linspace(t0, tf, n)))
+
+
+def _encode(a, dtype: str) -> np.ndarray:
+    a = np.asarray(a, dtype=np.float64)
+    if dtype == "f64":
+        return np.asfortranarray(a)
+    if dtype == "f32":
+        return np.asfortranarray(a, dtype=np.float32)
+    f = np.asfortranarray(a, dtype=np.float32)
+    b = f.view(np.uint32).astype(np.uint64)
+    b = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) & 0xFFFF
+    return np.asfortranarray(b.astype(np.uint16))
+
+
+class DLRIntegrator:
+    """``DLRIntegrator`` (primitives.jl:54-63) holding a device context."""
+
+    def __init__(self, prob, alg, dt, ctx, u, sol, *, snapshot_dtype, save_everystep):
+        self.prob = prob
+        self.alg = alg
+        self.dt = dt
+        self.t = prob.tspan[0]
+        self.iter = 0
+        self.ctx = ctx
+        self.u = u
+        self.sol = sol
+        self.probType = type(prob)
+        self.snapshot_dtype = snapshot_dtype
+        self.save_everystep = save_everystep
+        self._array_data = not callable(prob.y)
+        self._slot_of_t = 0  # function data: ring slot (of 3) holding y(t)
+        self.rank = u.rank
+
+    def refresh_u(self):
+        U, S, V = self.ctx.get_state()
+        self.u = SVDLikeRepresentation(U, S, V)
+        return self.u
+
+
+def _device_ptr(x):
+    dp = getattr(x, "data_ptr", None)
+    return dp() if callable(dp) else None
+
+
+def init(prob, alg, dt, *, precision: str = "f64", snapshot_dtype: Optional[str] = None,
+         save_everystep: bool = True, device: int = 0) -> DLRIntegrator:
+    """``init(prob, alg, dt)`` (projector_splitting.jl:107-115 and the BUG/RA counterparts)."""
+    if isinstance(prob, (MatrixDEProblem, MatrixHybridProblem)):
+        raise NotImplementedError(f"{type(prob).__name__}: only MatrixDataProblem runs on this engine")
+    if not isinstance(alg, (ProjectorSplitting, UnconventionalAlgorithm, RankAdaptiveUnconventionalAlgorithm)):
+        raise TypeError(f"unsupported algorithm {alg!r}")
+    t0, tf = prob.tspan
+    if not tf > t0:
+        raise AssertionError("Integration in reverse time direction is not supported")
+    u0 = prob.u0
+    n, m = u0.U.shape[0], u0.V.shape[0]
+    r = u0.rank
+    if isinstance(alg, RankAdaptiveUnconventionalAlgorithm):
+        rcap = int(min(alg.rmax, native.MAX_RANK, n, m))
+        if r > rcap:
+            raise ValueError(f"initial rank {r} exceeds the capacity {rcap}")
+    else:
+        rcap = r
+    if r > native.MAX_RANK:
+        raise ValueError(f"rank {r} exceeds LRI_MAX_RANK = {native.MAX_RANK}")
+    ctx = native.Context(n, m, rcap, device)
+    ctx.set_precision(native.PREC_MIXED if precision == "mixed" else native.PREC_F64)
+    ctx.set_state(u0.U, u0.S, u0.V)
+
+    # data: device tensors are bound as borrowed snapshots, host data is uploaded
+    y = prob.y
+    if callable(y):
+        sdt = snapshot_dtype or "f64"
+        ctx.reserve(3, native.DTYPES[sdt])
+        ctx.upload(0, _encode(y(t0), sdt))
+    else:
+        ptr0 = _device_ptr(y[0])
+        if ptr0 is not None:
+            sdt = snapshot_dtype or {"torch.float64": "f64", "torch.float32": "f32",
+                                     "torch.bfloat16": "bf16"}[str(y[0].dtype)]
+            ld = int(y[0].stride(1)) if y[0].dim() == 2 else n
+            ctx.bind([_device_ptr(a) for a in y], native.DTYPES[sdt], ld, owners=y)
+        else:
+            sdt = snapshot_dtype or "f64"
+            ctx.reserve(len(y), native.DTYPES[sdt])
+            for k, a in enumerate(y):
+                ctx.upload(k, _encode(a, sdt))
+    sol = init_sol(dt, t0, tf, u0)
+    sol.Y[0] = u0.copy()
+    return DLRIntegrator(prob, alg, dt, ctx, u0.copy(), sol, snapshot_dtype=sdt, save_everystep=save_everystep)
+
+
+def step(integrator: DLRIntegrator, alg=None, dt=None) -> None:
+    """``step!(integrator, alg, dt)``: one step on the device, then t += dt, iter += 1."""
+    alg = integrator.alg if alg is None else alg
+    dt = integrator.dt if dt is None else dt
+    ctx = integrator.ctx
+    t = integrator.t
+    if integrator._array_data:
+        if isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang):
+            # update_data!(x, y::AbstractArray, t::Int, dt::Int) has no method for dt/2 (data_integrator.jl:26)
+            raise TypeError("array-form data has no half-step snapshot: Strang needs function-form data")
+        if not (_is_int(t) and _is_int(dt)):
+            raise TypeError("array-form data needs integer t and dt")
+        kp, kn = t - 1, t + dt - 1
+        if kn >= len(integrator.prob.y):
+            raise IndexError(f"snapshot index {kn + 1} beyond the data (length {len(integrator.prob.y)})")
+        _dispatch(integrator, alg, kp, kn, None)
+    else:
+        cur = integrator._slot_of_t
+        free = [s for s in range(3) if s != cur]
+        if isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang):
+            ctx.upload(free[0], _encode(integrator.prob.y(t + dt / 2), integrator.snapshot_dtype))
+            ctx.upload(free[1], _encode(integrator.prob.y(t + dt / 2 + dt / 2), integrator.snapshot_dtype))
+            ctx.step_ksl_strang(cur, free[0], free[1])
+            integrator._slot_of_t = free[1]
+        else:
+            ctx.upload(free[0], _encode(integrator.prob.y(t + dt), integrator.snapshot_dtype))
+            _dispatch(integrator, alg, cur, free[0], None)
+            integrator._slot_of_t = free[0]
+    integrator.t = t + dt
+    integrator.iter += 1
+
+
+def _dispatch(integrator, alg, kp, kn, _):
+    ctx = integrator.ctx
+    if isinstance(alg, ProjectorSplitting):
+        ctx.step_ksl(native.PRIMAL if isinstance(alg.order, PrimalLieTrotter) else native.DUAL, kp, kn)
+    elif isinstance(alg, UnconventionalAlgorithm):
+        ctx.step_bug(kp, kn)
+    else:
+        r_new = ctx.step_bug_ra(kp, kn, alg.tol, alg.rmax)
+        if r_new != integrator.rank:
+            print(f"rank adjusted: new rank = {r_new}")  # rank_adaptive_unconventional.jl:230
+            integrator.rank = r_new
+
+
+def update_sol(integrator: DLRIntegrator, force: bool = False) -> None:
+    """``update_sol!`` (primitives.jl:82-90): store a host copy of u and t."""
+    if not (integrator.save_everystep or force):
+        return
+    u = integrator.refresh_u().copy()
+    k = integrator.iter
+    if k <= len(integrator.sol.Y) - 1:
+        integrator.sol.Y[k] = u
+        integrator.sol.t[k] = integrator.t
+    else:
+        integrator.sol.Y.append(u)
+        integrator.sol.t.append(integrator.t)
+
+
+def solve(prob, alg, dt=None, **kw) -> DLRSolution:
+    """``solve(prob, alg, dt)`` / ``solve(prob::MatrixDataProblem, alg)`` (primitives.jl:68-80)."""
+    if dt is None:
+        if not isinstance(prob, MatrixDataProblem) or callable(prob.y):
+            raise AssertionError("If the data is not provided as array, integration stepsize needs to be specified")
+        dt = 1
+    integ = init(prob, alg, dt, **kw)
+    try:
+        tf, t0 = prob.tspan[1], prob.tspan[0]
+        T = tf - t0
+        while (tf - integ.t) / T > 1e-8:
+            step(integ, alg, dt)
+            last = not ((tf - integ.t) / T > 1e-8)
+            update_sol(integ, force=last)
+        integ.ctx.synchronize()
+        return integ.sol
+    finally:
+        integ.ctx.close()

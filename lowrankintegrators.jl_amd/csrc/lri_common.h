@@ -1,0 +1,48 @@
+// Shared definitions of the gfx950 kernels and the engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
+typedef __attribute__((ext_vector_type(16))) float f32x16_t;
+typedef __attribute__((ext_vector_type(4))) double f64x4_t;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4_t;
+
+namespace lri {
+
+// Device-side status bits latched into ctx->flags (read at synchronising calls).
+enum : int { FLAG_BREAKDOWN = 1, FLAG_NONFINITE = 2 };
+
+constexpr int kMaxRank = 64;       // state rank capacity
+constexpr int kMaxAug = 2 * kMaxRank;  // rank-adaptive augmented width
+
+// ---------------------------------------------------------------- snapshot decode
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
+__device__ __forceinline__ double snap_f64(const double* p, int64_t k) { return p[k]; }
+__device__ __forceinline__ double snap_f64(const float* p, int64_t k) { return double(p[k]); }
+__device__ __forceinline__ double snap_f64(const uint16_t* p, int64_t k) { return double(bf16_to_f32(p[k])); }
+
+// Round-to-nearest-even float -> bfloat16 bits (finite inputs).
+__device__ __forceinline__ uint32_t f32_to_bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// Split x into hi + lo, both bfloat16 (returned as bit patterns); |x - hi - lo| <= 2^-18 |x|.
+__device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) {
+  hi = f32_to_bf16_rne(x);
+  float r = x - __uint_as_float(hi << 16);
+  lo = f32_to_bf16_rne(r);
+}
+
+// ---------------------------------------------------------------- packed operand layouts
+// FP64 products take the B operand row-major (k x rb, rb a multiple of 16);
+// MIXED products take column-major bf16 hi/lo (or f32) panels, c-major with a
+// leading dimension padded to a multiple of 64.
+inline int64_t pad_to(int64_t x, int64_t q) { return (x + q - 1) / q * q; }
+
+}  // namespace lri

@@ -1,0 +1,1025 @@
+// Context, device buffers and step orchestration behind the C ABI (include/lri.h).
+//
+// One context = one problem size (n x m, rank capacity rcap) on one GPU, one
+// HIP stream.  All buffers are allocated once at lri_create; a step is a fixed
+// sequence of stream-ordered launches with no host synchronisation (except the
+// rank read-back of the rank-adaptive step).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/lri.h"
+#include "lri_common.h"
+#include "lri_kernels.h"
+
+using namespace lri;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Failure {
+  int status;
+  std::string msg;
+};
+
+#define HIPCHECK(call)                                                                   \
+  do {                                                                                   \
+    hipError_t _e = (call);                                                              \
+    if (_e != hipSuccess)                                                                \
+      throw Failure{_e == hipErrorOutOfMemory ? LRI_EOOM : LRI_EHIP,                     \
+                    std::string(#call) + ": " + hipGetErrorString(_e)};                  \
+  } while (0)
+
+#define REQUIRE(cond, msg)                                   \
+  do {                                                       \
+    if (!(cond)) throw Failure{LRI_EINVAL, std::string(msg)}; \
+  } while (0)
+
+__global__ void k_set_diag(double* S, int lds, int rcap, const double* sigma, int r) {
+  for (int e = threadIdx.x; e < rcap * rcap; e += blockDim.x) {
+    int i = e % rcap, j = e / rcap;
+    S[i + j * lds] = (i == j && i < r) ? sigma[i] : 0.0;
+  }
+}
+
+struct Timed {
+  int family;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct lri_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t n = 0, m = 0, rcap = 0, r = 0;
+  int prec = LRI_PREC_F64;
+  // FP64 state and work (column-major)
+  double *U = nullptr, *V = nullptr, *S = nullptr;  // n x rcap, m x rcap, rcap x rcap
+  double *U2 = nullptr, *V2 = nullptr;              // BUG: new bases before the swap
+  double *K = nullptr, *K2 = nullptr, *P = nullptr;  // n x 2rcap
+  double *L = nullptr, *L2 = nullptr, *Q = nullptr;  // m x 2rcap
+  double *Uhat = nullptr, *Vhat = nullptr;           // n x 2rcap, m x 2rcap
+  // packed B operands: N side (rows m) and T side (rows n)
+  double *pkN_f64 = nullptr, *pkT_f64 = nullptr;
+  float *pkN_f32 = nullptr, *pkT_f32 = nullptr;
+  uint16_t *pkN_hi = nullptr, *pkN_lo = nullptr, *pkT_hi = nullptr, *pkT_lo = nullptr;
+  int64_t ldm_pk = 0, ldn_pk = 0;
+  // packed-panel validity: which pass mode and rb they hold (-1: stale)
+  int pkN_mode = -1, pkN_rb = 0, pkT_mode = -1, pkT_rb = 0;
+  // split partials
+  void *partN = nullptr, *partT = nullptr;
+  size_t part_bytes = 0;
+  // reductions
+  double* gpart = nullptr;
+  unsigned* counter = nullptr;
+  int* flags = nullptr;
+  SmallState* small = nullptr;
+  double* svd_work = nullptr;
+  // snapshots
+  std::vector<const void*> snaps;
+  std::vector<void*> owned;
+  int sdtype = -1;
+  int64_t sld = 0;
+  // timing
+  bool timing = false;
+  std::vector<Timed> pending;
+  std::vector<hipEvent_t> pool;
+  double tot_ms[LRI_KERNEL_COUNT] = {0, 0, 0};
+  int64_t launches[LRI_KERNEL_COUNT] = {0, 0, 0};
+  std::string err;
+
+  // ------------------------------------------------------------------ helpers
+  int pass_mode() const {
+    if (prec == LRI_PREC_F64 || sdtype == LRI_F64) return PASS_F64;
+    return sdtype == LRI_BF16 ? PASS_BF16X3 : PASS_F32;
+  }
+  static int rb_for(int mode, int64_t cols) {
+    if (mode == PASS_F64) {
+      if (cols <= 16) return 16;
+      if (cols <= 32) return 32;
+      if (cols <= 48) return 48;
+      if (cols <= 64) return 64;
+      return 128;
+    }
+    if (cols <= 32) return 32;
+    if (cols <= 64) return 64;
+    return 128;
+  }
+
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  template <class F>
+  void timed(int family, F&& f) {
+    if (!timing) {
+      HIPCHECK(f());
+      return;
+    }
+    Timed t{family, ev(), ev()};
+    HIPCHECK(hipEventRecord(t.a, stream));
+    HIPCHECK(f());
+    HIPCHECK(hipEventRecord(t.b, stream));
+    pending.push_back(t);
+  }
+  void drain_timing() {
+    for (auto& t : pending) {
+      float ms = 0.f;
+      HIPCHECK(hipEventSynchronize(t.b));
+      HIPCHECK(hipEventElapsedTime(&ms, t.a, t.b));
+      tot_ms[t.family] += ms;
+      launches[t.family] += 1;
+      pool.push_back(t.a);
+      pool.push_back(t.b);
+    }
+    pending.clear();
+  }
+
+  template <typename PT>
+  void rows(RowArgs<PT> A) {
+    A.epi.small = small;
+    A.epi.counter = counter;
+    A.epi.flags = flags;
+    A.gpart = gpart;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_rows<PT>(A, stream); });
+  }
+
+  // Pack X (rows x cols FP64, ld ldx) as the B operand of a pass.
+  void pack(const double* X, int64_t ldx, int64_t rows_, int64_t cols, bool forN) {
+    const int mode = pass_mode();
+    const int rb = rb_for(mode, cols);
+    RowArgs<double> A;
+    A.rows = rows_;
+    A.r_out = (int)cols;
+    A.base = X;
+    A.ld_base = ldx;
+    set_pack_outputs(A, forN, mode, rb);
+    rows(A);
+  }
+  template <typename PT>
+  void set_pack_outputs(RowArgs<PT>& A, bool forN, int mode, int rb) {
+    A.pk_cols = rb;
+    if (mode == PASS_F64) {
+      A.pk_f64 = forN ? pkN_f64 : pkT_f64;
+    } else if (mode == PASS_F32) {
+      A.pk_f32 = forN ? pkN_f32 : pkT_f32;
+    } else {
+      A.pk_hi = forN ? pkN_hi : pkT_hi;
+      A.pk_lo = forN ? pkN_lo : pkT_lo;
+      A.ld_pk = forN ? ldm_pk : ldn_pk;
+    }
+    if (forN) {
+      pkN_mode = mode;
+      pkN_rb = rb;
+    } else {
+      pkT_mode = mode;
+      pkT_rb = rb;
+    }
+  }
+  void ensure_packed(bool forN) {
+    const int mode = pass_mode();
+    const int rb = rb_for(mode, r);
+    if (forN && (pkN_mode != mode || pkN_rb != rb)) pack(V, m, m, r, true);
+    if (!forN && (pkT_mode != mode || pkT_rb != rb)) pack(U, n, n, r, false);
+  }
+
+  // The product passes over dA = A[kn] - A[kp] with the currently packed B.
+  int passN(int64_t kp, int64_t kn, int64_t cols, void* part) {
+    const int mode = pass_mode();
+    PassArgs a;
+    a.anext = snaps[kn];
+    a.aprev = snaps[kp];
+    a.dtype = sdtype;
+    a.lda = sld;
+    a.n = n;
+    a.m = m;
+    a.rb = rb_for(mode, cols);
+    REQUIRE(pkN_mode == mode && pkN_rb == a.rb, "internal: stale packed V panel");
+    if (mode == PASS_F64) a.B = pkN_f64;
+    else if (mode == PASS_F32) a.B = pkN_f32;
+    else {
+      a.B = pkN_hi;
+      a.B2 = pkN_lo;
+      a.ldb = ldm_pk;
+    }
+    a.part = part;
+    int64_t need = plan_pass(mode, false, n, m, a.rb, &a.splits, &a.chunk);
+    REQUIRE((size_t)need * (mode == PASS_F64 ? 8 : 4) <= part_bytes, "internal: partial buffer too small");
+    timed(LRI_KERNEL_PASS_N, [&] { return launch_passN(mode, a, stream); });
+    return a.splits;
+  }
+  int passT(int64_t kp, int64_t kn, int64_t cols, void* part) {
+    const int mode = pass_mode();
+    PassArgs a;
+    a.anext = snaps[kn];
+    a.aprev = snaps[kp];
+    a.dtype = sdtype;
+    a.lda = sld;
+    a.n = n;
+    a.m = m;
+    a.rb = rb_for(mode, cols);
+    REQUIRE(pkT_mode == mode && pkT_rb == a.rb, "internal: stale packed U panel");
+    if (mode == PASS_F64) a.B = pkT_f64;
+    else if (mode == PASS_F32) a.B = pkT_f32;
+    else {
+      a.B = pkT_hi;
+      a.B2 = pkT_lo;
+      a.ldb = ldn_pk;
+    }
+    a.part = part;
+    int64_t need = plan_pass(mode, true, m, n, a.rb, &a.splits, &a.chunk);
+    REQUIRE((size_t)need * (mode == PASS_F64 ? 8 : 4) <= part_bytes, "internal: partial buffer too small");
+    timed(LRI_KERNEL_PASS_T, [&] { return launch_passT(mode, a, stream); });
+    return a.splits;
+  }
+
+  // rows kernel reading split partials of the current pass mode
+  template <class F>
+  void with_parts(void* part, F&& f) {
+    if (pass_mode() == PASS_F64) {
+      RowArgs<double> A;
+      A.parts = (const double*)part;
+      f(A);
+    } else {
+      RowArgs<float> A;
+      A.parts = (const float*)part;
+      f(A);
+    }
+  }
+
+  // CholeskyQR2 of X (rows x r, ld rows) given the first kernel already
+  // produced X and EPI_CHOL1 (R1 in small).  Writes Q to `out` (ld rows),
+  // Rtot in small; optional packed output and cross reduction with W.
+  void cholqr_tail(double* X, double* tmp, int64_t rows_, int rr, double* out, bool pk_forN, const double* W,
+                   int rw, EpiArgs final_epi, int sfinal) {
+    RowArgs<double> B;
+    B.rows = rows_;
+    B.r_out = rr;
+    B.r_in = rr;
+    B.X = X;
+    B.ldx = rows_;
+    B.Y = small->R1inv;
+    B.ldy = rr;
+    B.out = tmp;
+    B.ldo = rows_;
+    B.reduce = 1;
+    B.epi.kind = EPI_CHOL2;
+    B.epi.breakdown_is_error = final_epi.breakdown_is_error;
+    B.epi.S = sfinal ? S : nullptr;
+    B.epi.ld_s = (int)rcap;
+    B.epi.transpose = sfinal == 2;
+    rows(B);
+    RowArgs<double> C;
+    C.rows = rows_;
+    C.r_out = rr;
+    C.r_in = rr;
+    C.X = tmp;
+    C.ldx = rows_;
+    C.Y = small->R2inv;
+    C.ldy = rr;
+    C.out = out;
+    C.ldo = rows_;
+    set_pack_outputs(C, pk_forN, pass_mode(), rb_for(pass_mode(), rr));
+    if (final_epi.kind != EPI_NONE || W) {
+      C.reduce = 1;
+      C.W = W;
+      C.ldw = rows_;
+      C.rw = rw;
+      C.epi = final_epi;
+    }
+    rows(C);
+  }
+
+  void check_step(int64_t kp, int64_t kn) {
+    REQUIRE(r >= 1, "state not set (call lri_set_state)");
+    REQUIRE(!snaps.empty(), "no snapshots (call lri_snapshots_reserve/upload or lri_snapshots_bind)");
+    REQUIRE(kp >= 0 && kp < (int64_t)snaps.size() && kn >= 0 && kn < (int64_t)snaps.size(),
+            "snapshot index out of range");
+    REQUIRE(snaps[kp] && snaps[kn], "snapshot slot is empty");
+  }
+
+  // ------------------------------------------------------------------ steps
+  // primal_LT_step! (projector_splitting.jl:129-152)
+  void ksl_primal(int64_t kp, int64_t kn) {
+    const int rr = (int)r;
+    ensure_packed(true);
+    const int sN = passN(kp, kn, rr, partN);
+    with_parts(partN, [&](auto A) {  // K = U S + dA V ; P = dA V ; Gram -> R1
+      A.rows = n;
+      A.nsplit = sN;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.r_in = rr;
+      A.sum_out = P;
+      A.ld_sum = n;
+      A.X = U;
+      A.ldx = n;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.out = K;
+      A.ldo = n;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs e;
+    e.kind = EPI_STILDE;  // S~ = R - U1' P
+    cholqr_tail(K, K2, n, rr, U, false, P, rr, e, 0);
+    const int sT = passT(kp, kn, rr, partT);
+    with_parts(partT, [&](auto A) {  // L = V S~' + dA' U1 ; Gram -> R1
+      A.rows = m;
+      A.nsplit = sT;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = V;
+      A.ldx = m;
+      A.Y = small->St;
+      A.ldy = rr;
+      A.transY = 1;
+      A.out = L;
+      A.ldo = m;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs none;
+    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 2);  // V1, S = R'
+  }
+
+  // dual_LT_step! (projector_splitting.jl:166-189)
+  void ksl_dual(int64_t kp, int64_t kn) {
+    const int rr = (int)r;
+    ensure_packed(false);
+    const int sT = passT(kp, kn, rr, partT);
+    with_parts(partT, [&](auto A) {  // L = V S' + dA' U ; Q = dA' U
+      A.rows = m;
+      A.nsplit = sT;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.r_in = rr;
+      A.sum_out = Q;
+      A.ld_sum = m;
+      A.X = V;
+      A.ldx = m;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.transY = 1;
+      A.out = L;
+      A.ldo = m;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs e;
+    e.kind = EPI_STILDE;  // S~ = (R - V1' Q)' = R' - U' dA V1
+    e.transpose = 1;
+    cholqr_tail(L, L2, m, rr, V, true, Q, rr, e, 0);
+    const int sN = passN(kp, kn, rr, partN);
+    with_parts(partN, [&](auto A) {  // K = U S~ + dA V1
+      A.rows = n;
+      A.nsplit = sN;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = U;
+      A.ldx = n;
+      A.Y = small->St;
+      A.ldy = rr;
+      A.out = K;
+      A.ldo = n;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs none;
+    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 1);  // U1, S = R
+  }
+
+  // unconventional_step! (unconventional.jl:133-157)
+  void bug(int64_t kp, int64_t kn) {
+    const int rr = (int)r;
+    ensure_packed(true);
+    ensure_packed(false);
+    const int sN = passN(kp, kn, rr, partN);  // dA V0
+    const int sT = passT(kp, kn, rr, partT);  // dA' U0
+    with_parts(partN, [&](auto A) {
+      A.rows = n;
+      A.nsplit = sN;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = U;
+      A.ldx = n;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.out = K;
+      A.ldo = n;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs eM;
+    eM.kind = EPI_BUG_M;  // M = U1' U0
+    cholqr_tail(K, K2, n, rr, U2, false, U, rr, eM, 0);
+    with_parts(partT, [&](auto A) {
+      A.rows = m;
+      A.nsplit = sT;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = V;
+      A.ldx = m;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.transY = 1;
+      A.out = L;
+      A.ldo = m;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs eN;
+    eN.kind = EPI_BUG_N;  // N = V1' V0, Sinit = M S0 N'
+    eN.S = S;
+    eN.ld_s = (int)rcap;
+    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0);
+    const int sN2 = passN(kp, kn, rr, partN);  // dA V1
+    with_parts(partN, [&](auto A) {  // S1 = Sinit + U1' dA V1
+      A.rows = n;
+      A.nsplit = sN2;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.reduce = 1;
+      A.W = U2;
+      A.ldw = n;
+      A.rw = rr;
+      A.epi.kind = EPI_BUG_S;
+      A.epi.S = S;
+      A.epi.ld_s = (int)rcap;
+      rows(A);
+    });
+    std::swap(U, U2);
+    std::swap(V, V2);
+  }
+
+  // Orthonormal completion of span(X) against an orthonormal B: Q_perp with
+  // [B | Q_perp] orthonormal (dependent directions become zero columns).
+  // X = first kernel's output (rows x rr), block Gram-Schmidt twice, then CholeskyQR2.
+  void complete_basis(double* X, const double* Bm, int64_t rows_, int rr, double* t1, double* t2,
+                      double* qout, EpiArgs fin) {
+    // pass 1: C = X' B
+    {
+      RowArgs<double> A;
+      A.rows = rows_;
+      A.r_out = rr;
+      A.base = X;
+      A.ld_base = rows_;
+      A.reduce = 1;
+      A.W = Bm;
+      A.ldw = rows_;
+      A.rw = rr;
+      A.epi.kind = EPI_STORE;  // C -> small->Ctmp
+      rows(A);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+      // t = src - B (B' src) = src - B * C'   (C = src' B)
+      RowArgs<double> A;
+      A.rows = rows_;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.base = pass == 0 ? X : t1;
+      A.ld_base = rows_;
+      A.X = Bm;
+      A.ldx = rows_;
+      A.Y = small->Ctmp;
+      A.ldy = rr;
+      A.transY = 1;
+      A.xy_sign = -1.0;
+      A.out = pass == 0 ? t1 : t2;
+      A.ldo = rows_;
+      A.reduce = 1;
+      if (pass == 0) {
+        A.W = Bm;
+        A.ldw = rows_;
+        A.rw = rr;
+        A.epi.kind = EPI_STORE;
+      } else {
+        A.epi.kind = EPI_CHOL1;
+        A.epi.breakdown_is_error = 0;
+      }
+      rows(A);
+    }
+    // CholeskyQR2 tail on t2 -> qout, cross with B -> fin (EPI_RA_M / EPI_RA_N)
+    fin.breakdown_is_error = 0;
+    RowArgs<double> B2;
+    B2.rows = rows_;
+    B2.r_out = rr;
+    B2.r_in = rr;
+    B2.X = t2;
+    B2.ldx = rows_;
+    B2.Y = small->R1inv;
+    B2.ldy = rr;
+    B2.out = t1;
+    B2.ldo = rows_;
+    B2.reduce = 1;
+    B2.epi.kind = EPI_CHOL2;
+    B2.epi.breakdown_is_error = 0;
+    rows(B2);
+    RowArgs<double> C;
+    C.rows = rows_;
+    C.r_out = rr;
+    C.r_in = rr;
+    C.X = t1;
+    C.ldx = rows_;
+    C.Y = small->R2inv;
+    C.ldy = rr;
+    C.out = qout;
+    C.ldo = rows_;
+    C.reduce = 1;
+    C.W = Bm;
+    C.ldw = rows_;
+    C.rw = rr;
+    C.epi = fin;
+    rows(C);
+  }
+
+  // rankadaptive_unconventional_step! (rank_adaptive_unconventional.jl:194-233)
+  int64_t bug_ra(int64_t kp, int64_t kn, double tol, int64_t rmax) {
+    const int rr = (int)r;
+    const int k2 = 2 * rr;
+    ensure_packed(true);
+    ensure_packed(false);
+    const int sN = passN(kp, kn, rr, partN);  // dA V0
+    const int sT = passT(kp, kn, rr, partT);  // dA' U0
+    // K = U S + dA V  -> K ; Uhat = [U | Q_perp(K)]
+    with_parts(partN, [&](auto A) {
+      A.rows = n;
+      A.nsplit = sN;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = U;
+      A.ldx = n;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.out = K;
+      A.ldo = n;
+      rows(A);
+    });
+    HIPCHECK(hipMemcpyAsync(Uhat, U, sizeof(double) * n * rr, hipMemcpyDeviceToDevice, stream));
+    EpiArgs eM;
+    eM.kind = EPI_RA_M;
+    complete_basis(K, U, n, rr, K2, K2 + n * rcap, Uhat + n * rr, eM);
+    with_parts(partT, [&](auto A) {
+      A.rows = m;
+      A.nsplit = sT;
+      A.rb = rb_for(pass_mode(), rr);
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = V;
+      A.ldx = m;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.transY = 1;
+      A.out = L;
+      A.ldo = m;
+      rows(A);
+    });
+    HIPCHECK(hipMemcpyAsync(Vhat, V, sizeof(double) * m * rr, hipMemcpyDeviceToDevice, stream));
+    EpiArgs eN;
+    eN.kind = EPI_RA_N;
+    eN.S = S;
+    eN.ld_s = (int)rcap;
+    complete_basis(L, V, m, rr, L2, L2 + m * rcap, Vhat + m * rr, eN);
+    // Shat = M S N' + Uhat' dA Vhat
+    pack(Vhat, m, m, k2, true);
+    const int sN2 = passN(kp, kn, k2, partN);
+    with_parts(partN, [&](auto A) {
+      A.rows = n;
+      A.nsplit = sN2;
+      A.rb = rb_for(pass_mode(), k2);
+      A.r_out = k2;
+      A.reduce = 1;
+      A.W = Uhat;
+      A.ldw = n;
+      A.rw = k2;
+      A.epi.kind = EPI_RA_S;
+      rows(A);
+    });
+    SvdArgs sv;
+    sv.S = small->Shat;
+    sv.lds = k2;
+    sv.k = k2;
+    sv.U = small->Ush;
+    sv.V = small->Vsh;
+    sv.sigma = small->sigma;
+    sv.r_out = &small->r_new;
+    sv.tol = tol;
+    sv.rmax = std::min<int64_t>(rmax, rcap);
+    sv.work = svd_work;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_jacobi_svd(sv, stream); });
+    int r1 = 0;
+    HIPCHECK(hipMemcpyAsync(&r1, &small->r_new, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    REQUIRE(r1 >= 1 && r1 <= rcap, "internal: rank-adaptive step returned an invalid rank");
+    // U = Uhat Ush[:, :r1], V = Vhat Vsh[:, :r1], S = diag(sigma[:r1])
+    for (int side = 0; side < 2; ++side) {
+      RowArgs<double> A;
+      A.rows = side == 0 ? n : m;
+      A.r_out = r1;
+      A.r_in = k2;
+      A.X = side == 0 ? Uhat : Vhat;
+      A.ldx = A.rows;
+      A.Y = side == 0 ? small->Ush : small->Vsh;
+      A.ldy = k2;
+      A.out = side == 0 ? U : V;
+      A.ldo = A.rows;
+      set_pack_outputs(A, side == 1, pass_mode(), rb_for(pass_mode(), r1));
+      rows(A);
+    }
+    hipLaunchKernelGGL(k_set_diag, dim3(1), dim3(256), 0, stream, S, (int)rcap, (int)rcap, small->sigma, r1);
+    HIPCHECK(hipGetLastError());
+    r = r1;
+    return r1;
+  }
+
+  int read_flags() {
+    int f = 0;
+    HIPCHECK(hipStreamSynchronize(stream));
+    HIPCHECK(hipMemcpy(&f, flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (f) HIPCHECK(hipMemset(flags, 0, sizeof(int)));
+    if (timing) drain_timing();
+    if (f & FLAG_NONFINITE) return LRI_ENONFINITE;
+    if (f & FLAG_BREAKDOWN) return LRI_EBREAKDOWN;
+    return LRI_OK;
+  }
+};
+
+namespace {
+
+template <class F>
+int guarded(lri_ctx* ctx, F&& f) {
+  if (!ctx) return LRI_EINVAL;
+  try {
+    HIPCHECK(hipSetDevice(ctx->device));
+    int st = f();
+    if (st != LRI_OK && ctx->err.empty()) ctx->err = lri_status_string(st);
+    return st;
+  } catch (const Failure& e) {
+    ctx->err = e.msg;
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    ctx->err = "host allocation failed";
+    return LRI_EOOM;
+  } catch (...) {
+    ctx->err = "unexpected C++ exception";
+    return LRI_EHIP;
+  }
+}
+
+size_t dtype_size(int dt) { return dt == LRI_F64 ? 8 : (dt == LRI_F32 ? 4 : 2); }
+
+void free_ctx(lri_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->owned) (void)hipFree(p);
+  void* bufs[] = {c->U,      c->V,      c->S,      c->U2,      c->V2,      c->K,      c->K2,     c->P,
+                  c->L,      c->L2,     c->Q,      c->Uhat,    c->Vhat,    c->pkN_f64, c->pkT_f64, c->pkN_f32,
+                  c->pkT_f32, c->pkN_hi, c->pkN_lo, c->pkT_hi,  c->pkT_lo,  c->partN,  c->partT,  c->gpart,
+                  c->counter, c->flags,  c->small,  c->svd_work};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  for (auto& t : c->pending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lri_version(void) { return 100; }
+
+const char* lri_status_string(int s) {
+  switch (s) {
+    case LRI_OK: return "ok";
+    case LRI_EINVAL: return "invalid argument";
+    case LRI_EHIP: return "HIP runtime error";
+    case LRI_EOOM: return "out of device memory";
+    case LRI_EBREAKDOWN: return "numerically singular Gram matrix in CholeskyQR2";
+    case LRI_ENONFINITE: return "non-finite values reached a Gram matrix";
+    case LRI_ENOTSUP: return "not supported";
+    default: return "unknown status";
+  }
+}
+
+const char* lri_last_error(const lri_ctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->err.c_str();
+}
+
+int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
+  g_create_error.clear();
+  if (!out) {
+    g_create_error = "out is NULL";
+    return LRI_EINVAL;
+  }
+  *out = nullptr;
+  if (n < 1 || m < 1 || rcap < 1 || rcap > LRI_MAX_RANK || rcap > n || rcap > m) {
+    g_create_error = "need n, m >= 1 and 1 <= rcap <= min(n, m, LRI_MAX_RANK)";
+    return LRI_EINVAL;
+  }
+  lri_ctx* c = nullptr;
+  try {
+    c = new lri_ctx();
+    c->device = device;
+    c->n = n;
+    c->m = m;
+    c->rcap = rcap;
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    auto alloc = [&](void** p, size_t bytes) {
+      HIPCHECK(hipMalloc(p, bytes));
+      HIPCHECK(hipMemset(*p, 0, bytes));
+    };
+    const size_t d = sizeof(double);
+    const int64_t r2 = 2 * rcap;
+    alloc((void**)&c->U, d * n * rcap);
+    alloc((void**)&c->V, d * m * rcap);
+    alloc((void**)&c->S, d * rcap * rcap);
+    alloc((void**)&c->U2, d * n * rcap);
+    alloc((void**)&c->V2, d * m * rcap);
+    alloc((void**)&c->K, d * n * r2);
+    alloc((void**)&c->K2, d * n * r2);
+    alloc((void**)&c->P, d * n * r2);
+    alloc((void**)&c->L, d * m * r2);
+    alloc((void**)&c->L2, d * m * r2);
+    alloc((void**)&c->Q, d * m * r2);
+    alloc((void**)&c->Uhat, d * n * r2);
+    alloc((void**)&c->Vhat, d * m * r2);
+    c->ldm_pk = pad_to(m, 64);
+    c->ldn_pk = pad_to(n, 64);
+    const int64_t rbmax = 128;
+    alloc((void**)&c->pkN_f64, d * m * rbmax);
+    alloc((void**)&c->pkT_f64, d * n * rbmax);
+    alloc((void**)&c->pkN_f32, 4 * m * rbmax);
+    alloc((void**)&c->pkT_f32, 4 * n * rbmax);
+    alloc((void**)&c->pkN_hi, 2 * c->ldm_pk * rbmax);
+    alloc((void**)&c->pkN_lo, 2 * c->ldm_pk * rbmax);
+    alloc((void**)&c->pkT_hi, 2 * c->ldn_pk * rbmax);
+    alloc((void**)&c->pkT_lo, 2 * c->ldn_pk * rbmax);
+    // partial buffers: the largest plan over modes, directions and widths
+    size_t pb = 0;
+    for (int mode = 0; mode < 3; ++mode)
+      for (int rb : {16, 32, 48, 64, 128}) {
+        if (mode != PASS_F64 && (rb == 16 || rb == 48)) continue;
+        int s;
+        int64_t ch;
+        size_t es = mode == PASS_F64 ? 8 : 4;
+        pb = std::max(pb, es * (size_t)plan_pass(mode, false, n, m, rb, &s, &ch));
+        pb = std::max(pb, es * (size_t)plan_pass(mode, true, m, n, rb, &s, &ch));
+      }
+    c->part_bytes = pb;
+    alloc(&c->partN, pb);
+    alloc(&c->partT, pb);
+    alloc((void**)&c->gpart, d * 64 * kMaxAug * kMaxAug);
+    alloc((void**)&c->counter, 64);
+    alloc((void**)&c->flags, 64);
+    alloc((void**)&c->small, sizeof(SmallState));
+    alloc((void**)&c->svd_work, d * kMaxAug * kMaxAug);
+    HIPCHECK(hipDeviceSynchronize());
+  } catch (const Failure& e) {
+    g_create_error = e.msg;
+    free_ctx(c);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_create_error = "host allocation failed";
+    free_ctx(c);
+    return LRI_EOOM;
+  }
+  *out = c;
+  return LRI_OK;
+}
+
+int lri_destroy(lri_ctx* ctx) {
+  free_ctx(ctx);
+  return LRI_OK;
+}
+
+int lri_set_precision(lri_ctx* ctx, int precision) {
+  return guarded(ctx, [&] {
+    REQUIRE(precision == LRI_PREC_F64 || precision == LRI_PREC_MIXED, "precision must be LRI_PREC_F64 or LRI_PREC_MIXED");
+    ctx->prec = precision;
+    return LRI_OK;
+  });
+}
+
+int lri_set_state(lri_ctx* ctx, int64_t r, const double* U, int64_t ldu, const double* S, int64_t lds,
+                  const double* V, int64_t ldv) {
+  return guarded(ctx, [&] {
+    REQUIRE(r >= 1 && r <= ctx->rcap, "rank must be in [1, rcap]");
+    REQUIRE(U && S && V, "U, S and V are required");
+    REQUIRE(ldu >= ctx->n && ldv >= ctx->m && lds >= r, "leading dimension too small");
+    const size_t d = sizeof(double);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy2D(ctx->U, d * ctx->n, U, d * ldu, d * ctx->n, r, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy2D(ctx->V, d * ctx->m, V, d * ldv, d * ctx->m, r, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(ctx->S, 0, d * ctx->rcap * ctx->rcap));
+    HIPCHECK(hipMemcpy2D(ctx->S, d * ctx->rcap, S, d * lds, d * r, r, hipMemcpyHostToDevice));
+    ctx->r = r;
+    ctx->pkN_mode = ctx->pkT_mode = -1;
+    return LRI_OK;
+  });
+}
+
+int lri_get_state(lri_ctx* ctx, int64_t* r_out, double* U, int64_t ldu, double* S, int64_t lds, double* V,
+                  int64_t ldv) {
+  return guarded(ctx, [&] {
+    int st = ctx->read_flags();
+    const int64_t r = ctx->r;
+    const size_t d = sizeof(double);
+    if (r_out) *r_out = r;
+    if (r >= 1) {
+      if (U) {
+        REQUIRE(ldu >= ctx->n, "ldu too small");
+        HIPCHECK(hipMemcpy2D(U, d * ldu, ctx->U, d * ctx->n, d * ctx->n, r, hipMemcpyDeviceToHost));
+      }
+      if (V) {
+        REQUIRE(ldv >= ctx->m, "ldv too small");
+        HIPCHECK(hipMemcpy2D(V, d * ldv, ctx->V, d * ctx->m, d * ctx->m, r, hipMemcpyDeviceToHost));
+      }
+      if (S) {
+        REQUIRE(lds >= r, "lds too small");
+        HIPCHECK(hipMemcpy2D(S, d * lds, ctx->S, d * ctx->rcap, d * r, r, hipMemcpyDeviceToHost));
+      }
+    }
+    return st;
+  });
+}
+
+int lri_get_rank(lri_ctx* ctx, int64_t* r_out) {
+  return guarded(ctx, [&] {
+    REQUIRE(r_out, "r_out is NULL");
+    int st = ctx->read_flags();
+    *r_out = ctx->r;
+    return st;
+  });
+}
+
+int lri_snapshots_reserve(lri_ctx* ctx, int64_t count, int dtype) {
+  return guarded(ctx, [&] {
+    REQUIRE(count >= 1, "count must be >= 1");
+    REQUIRE(dtype == LRI_F64 || dtype == LRI_F32 || dtype == LRI_BF16, "unknown dtype");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (void* p : ctx->owned) HIPCHECK(hipFree(p));
+    ctx->owned.clear();
+    ctx->snaps.clear();
+    ctx->sdtype = dtype;
+    ctx->sld = ctx->n;
+    const size_t bytes = dtype_size(dtype) * ctx->n * ctx->m;
+    for (int64_t k = 0; k < count; ++k) {
+      void* p = nullptr;
+      HIPCHECK(hipMalloc(&p, bytes));
+      ctx->owned.push_back(p);
+      ctx->snaps.push_back(p);
+    }
+    ctx->pkN_mode = ctx->pkT_mode = -1;
+    return LRI_OK;
+  });
+}
+
+int lri_snapshot_upload(lri_ctx* ctx, int64_t k, const void* host, int64_t ld) {
+  return guarded(ctx, [&] {
+    REQUIRE(!ctx->owned.empty(), "no library-owned snapshot slots (call lri_snapshots_reserve)");
+    REQUIRE(k >= 0 && k < (int64_t)ctx->owned.size(), "slot out of range");
+    REQUIRE(host && ld >= ctx->n, "bad host pointer or leading dimension");
+    const size_t es = dtype_size(ctx->sdtype);
+    HIPCHECK(hipMemcpy2DAsync(ctx->owned[k], es * ctx->n, host, es * ld, es * ctx->n, ctx->m,
+                              hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return LRI_OK;
+  });
+}
+
+int lri_snapshots_bind(lri_ctx* ctx, const void* const* dev_ptrs, int64_t count, int dtype, int64_t ld) {
+  return guarded(ctx, [&] {
+    REQUIRE(dev_ptrs && count >= 1, "need count >= 1 device pointers");
+    REQUIRE(dtype == LRI_F64 || dtype == LRI_F32 || dtype == LRI_BF16, "unknown dtype");
+    REQUIRE(ld >= ctx->n, "ld must be >= n");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (void* p : ctx->owned) HIPCHECK(hipFree(p));
+    ctx->owned.clear();
+    ctx->snaps.assign(dev_ptrs, dev_ptrs + count);
+    for (auto p : ctx->snaps) REQUIRE(p != nullptr, "NULL snapshot pointer");
+    ctx->sdtype = dtype;
+    ctx->sld = ld;
+    ctx->pkN_mode = ctx->pkT_mode = -1;
+    return LRI_OK;
+  });
+}
+
+int lri_step_ksl(lri_ctx* ctx, int order, int64_t k_prev, int64_t k_next) {
+  return guarded(ctx, [&] {
+    REQUIRE(order == LRI_PRIMAL || order == LRI_DUAL,
+            "lri_step_ksl takes LRI_PRIMAL or LRI_DUAL (Strang: lri_step_ksl_strang)");
+    ctx->check_step(k_prev, k_next);
+    if (order == LRI_PRIMAL) ctx->ksl_primal(k_prev, k_next);
+    else ctx->ksl_dual(k_prev, k_next);
+    return LRI_OK;
+  });
+}
+
+int lri_step_ksl_strang(lri_ctx* ctx, int64_t k_prev, int64_t k_mid, int64_t k_next) {
+  return guarded(ctx, [&] {
+    ctx->check_step(k_prev, k_mid);
+    ctx->check_step(k_mid, k_next);
+    ctx->ksl_primal(k_prev, k_mid);
+    ctx->ksl_dual(k_mid, k_next);
+    return LRI_OK;
+  });
+}
+
+int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next) {
+  return guarded(ctx, [&] {
+    ctx->check_step(k_prev, k_next);
+    ctx->bug(k_prev, k_next);
+    return LRI_OK;
+  });
+}
+
+int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax, int64_t* r_new) {
+  return guarded(ctx, [&] {
+    ctx->check_step(k_prev, k_next);
+    REQUIRE(tol >= 0.0, "tol must be >= 0");
+    REQUIRE(rmax >= 1, "rmax must be >= 1");
+    int64_t r1 = ctx->bug_ra(k_prev, k_next, tol, rmax);
+    if (r_new) *r_new = r1;
+    return ctx->read_flags();
+  });
+}
+
+int lri_synchronize(lri_ctx* ctx) {
+  return guarded(ctx, [&] { return ctx->read_flags(); });
+}
+
+int lri_get_stream(lri_ctx* ctx, void** stream) {
+  return guarded(ctx, [&] {
+    REQUIRE(stream, "stream is NULL");
+    *stream = (void*)ctx->stream;
+    return LRI_OK;
+  });
+}
+
+int lri_kernel_timing(lri_ctx* ctx, int enable) {
+  return guarded(ctx, [&] {
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->drain_timing();
+    ctx->timing = enable != 0;
+    for (int f = 0; f < LRI_KERNEL_COUNT; ++f) {
+      ctx->tot_ms[f] = 0.0;
+      ctx->launches[f] = 0;
+    }
+    return LRI_OK;
+  });
+}
+
+int lri_kernel_timing_read(lri_ctx* ctx, int family, double* total_ms, int64_t* launches) {
+  return guarded(ctx, [&] {
+    REQUIRE(family >= 0 && family < LRI_KERNEL_COUNT, "unknown kernel family");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->drain_timing();
+    if (total_ms) *total_ms = ctx->tot_ms[family];
+    if (launches) *launches = ctx->launches[family];
+    return LRI_OK;
+  });
+}
+
+int lri_selftest_mfma(int which, const void* a, const void* b, void* out) {
+  if (which < 0 || which > 2 || !a || !b || !out) return LRI_EINVAL;
+  hipError_t e = selftest_mfma(which, a, b, out);
+  if (e != hipSuccess) {
+    g_create_error = std::string("selftest: ") + hipGetErrorString(e);
+    return LRI_EHIP;
+  }
+  return LRI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+// Kernel argument blocks and launchers shared by the engine and the kernels.
+#pragma once
+
+#include "lri_common.h"
+
+namespace lri {
+
+// r x r (up to 2r x 2r) FP64 matrices of one step, device-resident.
+struct SmallState {
+  double R1[kMaxRank * kMaxRank];      // first CholeskyQR factor
+  double R1inv[kMaxRank * kMaxRank];
+  double R2inv[kMaxRank * kMaxRank];   // second CholeskyQR factor, inverted
+  double Rtot[kMaxRank * kMaxRank];    // R = R2 * R1
+  double St[kMaxRank * kMaxRank];      // S~ of the KSL S sub-step
+  double M[kMaxAug * kMaxRank];        // Galerkin U1' U0 (RA: 2r x r)
+  double N[kMaxAug * kMaxRank];        // Galerkin V1' V0
+  double Sinit[kMaxAug * kMaxAug];     // M S0 N'
+  double Shat[kMaxAug * kMaxAug];      // RA core before the SVD
+  double Ush[kMaxAug * kMaxAug];       // its left singular vectors
+  double Vsh[kMaxAug * kMaxAug];       // its right singular vectors
+  double sigma[kMaxAug];
+  double Ctmp[kMaxAug * kMaxAug];      // scratch reduction result (EPI_STORE)
+  int r_new;
+  int ndropped;
+};
+
+enum EpiKind {
+  EPI_NONE = 0,
+  EPI_CHOL1,   // C = Gram -> R1, R1^{-1}
+  EPI_CHOL2,   // C = Gram -> R2^{-1}, Rtot = R2 R1; if S: S = Rtot (transpose: Rtot')
+  EPI_STILDE,  // St = Rtot - C (transpose: St = (Rtot - C)')
+  EPI_SFINAL,  // S = Rtot (transpose: Rtot')
+  EPI_BUG_M,   // M = C
+  EPI_BUG_N,   // N = C, Sinit = M S0 N'
+  EPI_BUG_S,   // S = Sinit + C'
+  EPI_RA_M,    // M = [I; C]
+  EPI_RA_N,    // N = [I; C], Sinit = M S0 N'
+  EPI_RA_S,    // Shat = Sinit + C'
+  EPI_STORE,   // Ctmp = C
+};
+
+struct EpiArgs {
+  int kind = EPI_NONE;
+  int transpose = 0;
+  int breakdown_is_error = 1;
+  SmallState* small = nullptr;
+  unsigned* counter = nullptr;  // zero on entry, reset to zero by the last workgroup
+  int* flags = nullptr;
+  double* S = nullptr;          // state S (EPI_SFINAL / EPI_BUG_* / EPI_RA_N)
+  int ld_s = 0;
+};
+
+// Row kernel (see lri_rowops.hip).
+template <typename PT>
+struct RowArgs {
+  int64_t rows = 0;
+  int r_out = 0, r_in = 0;
+  // sum of split partials part[s][c][i]
+  const PT* parts = nullptr;
+  int nsplit = 0, rb = 0;
+  double* sum_out = nullptr;
+  int64_t ld_sum = 0;
+  // + base_sign * base
+  const double* base = nullptr;
+  int64_t ld_base = 0;
+  double base_sign = 1.0;
+  // + xy_sign * X * op(Y)
+  const double* X = nullptr;
+  int64_t ldx = 0;
+  const double* Y = nullptr;
+  int ldy = 0, transY = 0;
+  double xy_sign = 1.0;
+  // outputs
+  double* out = nullptr;
+  int64_t ldo = 0;
+  double* out2 = nullptr;
+  int64_t ldo2 = 0;
+  double* pk_f64 = nullptr;   // row-major [i][pk_cols]
+  float* pk_f32 = nullptr;    // row-major [i][pk_cols]
+  uint16_t* pk_hi = nullptr;  // column-major [c][ld_pk]
+  uint16_t* pk_lo = nullptr;
+  int64_t ld_pk = 0;
+  int pk_cols = 0;
+  // reduction C = out' * W  (W == nullptr: out' * out)
+  int reduce = 0;
+  const double* W = nullptr;
+  int64_t ldw = 0;
+  int rw = 0;
+  double* gpart = nullptr;    // [grid][r_out * rw]
+  EpiArgs epi;
+  int max_blocks = 64;
+  // filled by the launcher
+  int tr = 0, ri_pad = 0, ro_pad = 0, epi_doubles = 0;
+};
+
+template <typename PT>
+hipError_t launch_rows(RowArgs<PT> A, hipStream_t st);
+
+struct SvdArgs {
+  const double* S = nullptr;  // k x k, ld lds
+  int lds = 0;
+  int k = 0;
+  double* U = nullptr;  // k x k (ld k), columns by descending sigma
+  double* V = nullptr;
+  double* sigma = nullptr;
+  int* r_out = nullptr;
+  double tol = 0.0;
+  int64_t rmax = 0;
+  double* work = nullptr;  // global ke x ke scratch, used when it does not fit in LDS
+};
+hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st);
+
+// Product passes (see lri_tsmm.hip).
+enum PassMode { PASS_F64 = 0, PASS_BF16X3 = 1, PASS_F32 = 2 };
+
+struct PassArgs {
+  const void* anext = nullptr;  // A[k_next], column-major, ld lda
+  const void* aprev = nullptr;  // A[k_prev]
+  int dtype = 0;                // LRI_F64 / LRI_F32 / LRI_BF16
+  int64_t lda = 0;
+  int64_t n = 0, m = 0;
+  int rb = 0;                   // output columns (multiple of 16 for F64, 32 otherwise)
+  // B operand: F64 row-major [k][rb]; F32 row-major [k][rb]; BF16X3 column-major hi/lo [c][ldb]
+  const void* B = nullptr;
+  const void* B2 = nullptr;
+  int64_t ldb = 0;
+  void* part = nullptr;         // [splits][rb][out_rows], double (F64) or float
+  int splits = 0;
+  int64_t chunk = 0;            // reduction-range length per split
+};
+
+// Choose splits/chunk for a pass whose output has `out_rows` rows and whose
+// reduction runs over `red_len`; returns the partial buffer element count.
+int64_t plan_pass(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int* splits,
+                  int64_t* chunk);
+hipError_t launch_passN(int mode, const PassArgs& a, hipStream_t st);
+hipError_t launch_passT(int mode, const PassArgs& a, hipStream_t st);
+
+hipError_t selftest_mfma(int which, const void* a, const void* b, void* out);
+
+}  // namespace lri

@@ -1,0 +1,224 @@
+"""ctypes binding of liblri_hip.so (declarations in include/lri.h).
+
+The library is loaded from this package directory; there is no fallback: if it
+is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblri_hip.so")
+
+# status codes (include/lri.h)
+OK, EINVAL, EHIP, EOOM, EBREAKDOWN, ENONFINITE, ENOTSUP = 0, -1, -2, -3, -4, -5, -6
+F64, F32, BF16 = 0, 1, 2
+PREC_F64, PREC_MIXED = 0, 1
+PRIMAL, DUAL, STRANG = 0, 1, 2
+KERNEL_PASS_N, KERNEL_PASS_T, KERNEL_SMALL = 0, 1, 2
+MAX_RANK = 64
+
+DTYPES = {"f64": F64, "f32": F32, "bf16": BF16}
+
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "lri_version": (_int, []),
+    "lri_status_string": (ctypes.c_char_p, [_int]),
+    "lri_last_error": (ctypes.c_char_p, [_vp]),
+    "lri_create": (_int, [ctypes.POINTER(_vp), _int, _i64, _i64, _i64]),
+    "lri_destroy": (_int, [_vp]),
+    "lri_set_precision": (_int, [_vp, _int]),
+    "lri_set_state": (_int, [_vp, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
+    "lri_get_state": (_int, [_vp, ctypes.POINTER(_i64), _dp, _i64, _dp, _i64, _dp, _i64]),
+    "lri_get_rank": (_int, [_vp, ctypes.POINTER(_i64)]),
+    "lri_snapshots_reserve": (_int, [_vp, _i64, _int]),
+    "lri_snapshot_upload": (_int, [_vp, _i64, _vp, _i64]),
+    "lri_snapshots_bind": (_int, [_vp, ctypes.POINTER(_vp), _i64, _int, _i64]),
+    "lri_step_ksl": (_int, [_vp, _int, _i64, _i64]),
+    "lri_step_ksl_strang": (_int, [_vp, _i64, _i64, _i64]),
+    "lri_step_bug": (_int, [_vp, _i64, _i64]),
+    "lri_step_bug_ra": (_int, [_vp, _i64, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
+    "lri_synchronize": (_int, [_vp]),
+    "lri_get_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "lri_kernel_timing": (_int, [_vp, _int]),
+    "lri_kernel_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)]),
+    "lri_selftest_mfma": (_int, [_int, _vp, _vp, _vp]),
+}
+
+_LIB = None
+
+
+class LRIError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+class BreakdownError(LRIError):
+    pass
+
+
+class NonFiniteError(LRIError):
+    pass
+
+
+def lib():
+    """Load (once) and return the library with typed entry points."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status, ctx=None):
+    if status == OK:
+        return
+    L = lib()
+    msg = L.lri_last_error(ctx).decode(errors="replace") if L.lri_last_error(ctx) else ""
+    if not msg:
+        msg = L.lri_status_string(status).decode()
+    if status == EBREAKDOWN:
+        raise BreakdownError(status, msg)
+    if status == ENONFINITE:
+        raise NonFiniteError(status, msg)
+    if status == EINVAL:
+        raise ValueError(msg)
+    raise LRIError(status, msg)
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """Owning wrapper of an lri_ctx (one problem size on one device)."""
+
+    def __init__(self, n: int, m: int, rcap: int, device: int = 0):
+        L = lib()
+        h = _vp()
+        st = L.lri_create(ctypes.byref(h), int(device), int(n), int(m), int(rcap))
+        check(st, None)
+        self.h = h
+        self.n, self.m, self.rcap = int(n), int(m), int(rcap)
+        self._bound = None  # keeps borrowed snapshot owners alive
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lri_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state
+    def set_state(self, U, S, V):
+        U = np.asfortranarray(U, dtype=np.float64)
+        S = np.asfortranarray(S, dtype=np.float64)
+        V = np.asfortranarray(V, dtype=np.float64)
+        r = S.shape[0]
+        if U.shape != (self.n, r) or V.shape != (self.m, r) or S.shape != (r, r):
+            raise ValueError(f"shape mismatch: U {U.shape}, S {S.shape}, V {V.shape} for n={self.n}, m={self.m}")
+        check(lib().lri_set_state(self.h, r, _dptr(U), self.n, _dptr(S), r, _dptr(V), self.m), self.h)
+
+    def rank(self) -> int:
+        r = _i64()
+        check(lib().lri_get_rank(self.h, ctypes.byref(r)), self.h)
+        return int(r.value)
+
+    def get_state(self):
+        r = _i64()
+        check(lib().lri_get_rank(self.h, ctypes.byref(r)), self.h)
+        r = int(r.value)
+        U = np.zeros((self.n, r), order="F")
+        S = np.zeros((r, r), order="F")
+        V = np.zeros((self.m, r), order="F")
+        r2 = _i64()
+        check(lib().lri_get_state(self.h, ctypes.byref(r2), _dptr(U), self.n, _dptr(S), r, _dptr(V), self.m), self.h)
+        return U, S, V
+
+    def set_precision(self, prec: int):
+        check(lib().lri_set_precision(self.h, int(prec)), self.h)
+
+    # -- data
+    def reserve(self, count: int, dtype: int):
+        check(lib().lri_snapshots_reserve(self.h, int(count), int(dtype)), self.h)
+        self.dtype = dtype
+
+    def upload(self, k: int, host: np.ndarray):
+        """Upload a host snapshot (already encoded in the reserved dtype)."""
+        a = np.asfortranarray(host)
+        check(lib().lri_snapshot_upload(self.h, int(k), a.ctypes.data_as(_vp), a.shape[0]), self.h)
+
+    def bind(self, ptrs, dtype: int, ld: int, owners=None):
+        arr = (_vp * len(ptrs))(*[_vp(int(p)) for p in ptrs])
+        check(lib().lri_snapshots_bind(self.h, arr, len(ptrs), int(dtype), int(ld)), self.h)
+        self._bound = owners
+        self.dtype = dtype
+
+    # -- steps
+    def step_ksl(self, order: int, kp: int, kn: int):
+        check(lib().lri_step_ksl(self.h, int(order), int(kp), int(kn)), self.h)
+
+    def step_ksl_strang(self, kp: int, km: int, kn: int):
+        check(lib().lri_step_ksl_strang(self.h, int(kp), int(km), int(kn)), self.h)
+
+    def step_bug(self, kp: int, kn: int):
+        check(lib().lri_step_bug(self.h, int(kp), int(kn)), self.h)
+
+    def step_bug_ra(self, kp: int, kn: int, tol: float, rmax: int) -> int:
+        r = _i64()
+        check(lib().lri_step_bug_ra(self.h, int(kp), int(kn), float(tol), int(rmax), ctypes.byref(r)), self.h)
+        return int(r.value)
+
+    def synchronize(self):
+        check(lib().lri_synchronize(self.h), self.h)
+
+    def stream(self) -> int:
+        s = _vp()
+        check(lib().lri_get_stream(self.h, ctypes.byref(s)), self.h)
+        return int(s.value or 0)
+
+    # -- timing
+    def timing(self, enable: bool):
+        check(lib().lri_kernel_timing(self.h, 1 if enable else 0), self.h)
+
+    def timing_read(self, family: int):
+        ms = ctypes.c_double()
+        cnt = _i64()
+        check(lib().lri_kernel_timing_read(self.h, int(family), ctypes.byref(ms), ctypes.byref(cnt)), self.h)
+        return float(ms.value), int(cnt.value)
+
+
+def selftest_mfma(which: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Raw per-lane results of one matrix-core instruction (see lri_selftest_mfma)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    out = np.zeros(256, np.float64) if which == 0 else np.zeros(1024, np.float32)
+    check(lib().lri_selftest_mfma(int(which), a.ctypes.data_as(_vp), b.ctypes.data_as(_vp),
+                                  out.ctypes.data_as(_vp)), None)
+    return out

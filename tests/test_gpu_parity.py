@@ -1,0 +1,312 @@
+"""Parity of the HIP engine with the CPU oracle (needs an MI355X).
+
+Tolerances:
+* FP64 products: relative Frobenius error on Y <= 1e-10 against the oracle on
+  identical inputs; subspaces compared through projectors (<= 1e-9).
+* Mixed precision: BF16 snapshots (hi/lo-split BF16 matrix cores, FP32
+  accumulation) and F32 snapshots (FP32 matrix cores) against the FP64 engine
+  on the same decoded snapshots: <= 1e-6.
+"""
+import functools
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import dlra
+from oracle.data import bf16_bits, bf16_round, random_factors, rotating_data
+from oracle.metrics import orth_defect, proj_dist, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@functools.lru_cache(maxsize=None)
+def cfg0_data():
+    d = rotating_data(256, 256, 8, noise=1e-3, seed=0)
+    return d, d.initial()
+
+
+# ------------------------------------------------------------------ lane layouts
+
+
+def test_mfma_layouts(lri):
+    rng = np.random.default_rng(0)
+    # FP64 16x16x4: A[i][k] at lane 16k+i, B[k][j] at lane 16k+j, D[4*(l/16)+v][l%16]
+    A = rng.integers(-4, 5, (16, 4)).astype(np.float64)
+    B = rng.integers(-4, 5, (4, 16)).astype(np.float64)
+    a = np.array([A[l % 16, l // 16] for l in range(64)])
+    b = np.array([B[l // 16, l % 16] for l in range(64)])
+    out = lri.native.selftest_mfma(0, a, b).reshape(64, 4)
+    D = A @ B
+    got = np.zeros((16, 16))
+    for l in range(64):
+        for v in range(4):
+            got[4 * (l // 16) + v, l % 16] = out[l, v]
+    assert np.array_equal(got, D)
+    # BF16 32x32x16: A[i][8*(l/32)+e] at lane l (i = l%32), B[8*(l/32)+e][l%32];
+    # D[8*(v/4)+4*(l/32)+(v%4)][l%32]
+    A = rng.integers(-4, 5, (32, 16)).astype(np.float64)
+    B = rng.integers(-4, 5, (16, 32)).astype(np.float64)
+    a = np.zeros((64, 8), np.uint16)
+    b = np.zeros((64, 8), np.uint16)
+    for l in range(64):
+        for e in range(8):
+            a[l, e] = bf16_bits(np.array([A[l % 32, 8 * (l // 32) + e]]))[0]
+            b[l, e] = bf16_bits(np.array([B[8 * (l // 32) + e, l % 32]]))[0]
+    out = lri.native.selftest_mfma(1, a, b).reshape(64, 16)
+    D = A @ B
+    got = np.zeros((32, 32))
+    for l in range(64):
+        for v in range(16):
+            got[8 * (v // 4) + 4 * (l // 32) + (v % 4), l % 32] = out[l, v]
+    assert np.array_equal(got, D)
+    # FP32 32x32x2
+    A = rng.integers(-4, 5, (32, 2)).astype(np.float32)
+    B = rng.integers(-4, 5, (2, 32)).astype(np.float32)
+    a = np.array([A[l % 32, l // 32] for l in range(64)], np.float32)
+    b = np.array([B[l // 32, l % 32] for l in range(64)], np.float32)
+    out = lri.native.selftest_mfma(2, a, b).reshape(64, 16)
+    got = np.zeros((32, 32))
+    for l in range(64):
+        for v in range(16):
+            got[8 * (v // 4) + 4 * (l // 32) + (v % 4), l % 32] = out[l, v]
+    assert np.array_equal(got, A.astype(np.float64) @ B.astype(np.float64))
+
+
+# ------------------------------------------------------------------ FP64 parity, one step
+
+
+def one_step_case(seed, n, m, r, inc=1e-2):
+    rng = np.random.default_rng(seed)
+    u = random_factors(rng, n, m, r, decay=0.2)
+    A0 = rng.standard_normal((n, m))
+    A1 = A0 + inc * rng.standard_normal((n, m))
+    return u, A0, A1
+
+
+def run_native_step(lri, kind, u, snaps, dtype="f64", precision="f64", tol=1e-8, rmax=64, rcap=None):
+    n, m = u.U.shape[0], u.V.shape[0]
+    rcap = rcap or (min(64, n, m) if kind == "ra" else u.rank)
+    with lri.native.Context(n, m, rcap) as ctx:
+        ctx.set_precision(lri.native.PREC_MIXED if precision == "mixed" else lri.native.PREC_F64)
+        ctx.set_state(u.U, u.S, u.V)
+        ctx.reserve(len(snaps), lri.native.DTYPES[dtype])
+        for k, a in enumerate(snaps):
+            ctx.upload(k, lri.api._encode(a, dtype))
+        if kind == "primal":
+            ctx.step_ksl(lri.native.PRIMAL, 0, 1)
+        elif kind == "dual":
+            ctx.step_ksl(lri.native.DUAL, 0, 1)
+        elif kind == "strang":
+            ctx.step_ksl_strang(0, 1, 2)
+        elif kind == "bug":
+            ctx.step_bug(0, 1)
+        elif kind == "ra":
+            ctx.step_bug_ra(0, 1, tol, rmax)
+        return ctx.get_state()
+
+
+ORACLE = {
+    "primal": dlra.ksl_primal_step,
+    "dual": dlra.ksl_dual_step,
+    "bug": dlra.bug_step,
+}
+
+
+@pytest.mark.parametrize("kind", ["primal", "dual", "bug"])
+@pytest.mark.parametrize("shape", [(256, 256, 8), (200, 136, 5), (37, 53, 3), (1000, 700, 32), (300, 300, 64)])
+def test_one_step_fp64(lri, kind, shape):
+    n, m, r = shape
+    say(f"case: one {kind} step n={n} m={m} r={r} FP64")
+    u, A0, A1 = one_step_case(1, n, m, r)
+    U, S, V = run_native_step(lri, kind, u, [A0, A1])
+    Uo, So, Vo = ORACLE[kind](u.U, u.S, u.V, A1 - A0)
+    assert rel_err(U, S, V, Uo, So, Vo) <= FP64_TOL
+    assert proj_dist(U, Uo) <= 1e-9 and proj_dist(V, Vo) <= 1e-9
+    assert orth_defect(U) < 1e-13 and orth_defect(V) < 1e-13
+    if kind == "primal":
+        assert np.abs(np.triu(S, 1)).max() == 0.0  # S = R' lower triangular
+    if kind == "dual":
+        assert np.abs(np.tril(S, -1)).max() == 0.0
+
+
+def test_one_step_strang_fp64(lri):
+    u, A0, A1 = one_step_case(2, 300, 240, 12)
+    Am = 0.5 * (A0 + A1) + 1e-3 * np.random.default_rng(5).standard_normal(A0.shape)
+    U, S, V = run_native_step(lri, "strang", u, [A0, Am, A1])
+    Uo, So, Vo = dlra.ksl_primal_step(u.U, u.S, u.V, Am - A0)
+    Uo, So, Vo = dlra.ksl_dual_step(Uo, So, Vo, A1 - Am)
+    assert rel_err(U, S, V, Uo, So, Vo) <= FP64_TOL
+
+
+@pytest.mark.parametrize("shape,tol", [((256, 256, 8), 1e-3), ((200, 150, 12), 2e-2), ((64, 96, 32), 1e-6)])
+def test_one_step_rank_adaptive_fp64(lri, shape, tol):
+    n, m, r = shape
+    say(f"case: one RA step n={n} m={m} r={r} tol={tol}")
+    u, A0, A1 = one_step_case(3, n, m, r, inc=1e-2)
+    U, S, V = run_native_step(lri, "ra", u, [A0, A1], tol=tol, rmax=64)
+    Uo, So, Vo, r1 = dlra.rank_adaptive_bug_step(u.U, u.S, u.V, A1 - A0, tol, 64)
+    assert S.shape[0] == r1
+    assert rel_err(U, S, V, Uo, So, Vo) <= 1e-9
+    assert np.allclose(np.diag(S), np.diag(So), rtol=1e-10, atol=1e-14)
+
+
+def test_rank_adaptive_zero_increment_keeps_rank(lri):
+    u, A0, _ = one_step_case(4, 128, 96, 6)
+    U, S, V = run_native_step(lri, "ra", u, [A0, A0.copy()], tol=1e-10, rmax=64)
+    assert S.shape[0] == 6
+    assert rel_err(U, S, V, u.U, u.S, u.V) <= 1e-12
+
+
+def test_rank_adaptive_rmax(lri):
+    u, A0, A1 = one_step_case(5, 128, 96, 6, inc=1e-1)
+    U, S, V = run_native_step(lri, "ra", u, [A0, A1], tol=1e-14, rmax=9)
+    assert S.shape[0] == 9
+
+
+# ------------------------------------------------------------------ full solves vs fixture
+
+
+GOLDEN_ALGS = {
+    "ksl_primal": lambda lri: lri.ProjectorSplitting(lri.PrimalLieTrotter()),
+    "ksl_dual": lambda lri: lri.ProjectorSplitting(lri.DualLieTrotter()),
+    "ksl_strang": lambda lri: lri.ProjectorSplitting(lri.Strang()),
+    "bug": lambda lri: lri.UnconventionalAlgorithm(),
+    "ra_bug": lambda lri: lri.RankAdaptiveUnconventionalAlgorithm(2e-2, rmax=16),
+}
+
+
+@pytest.mark.parametrize("name", list(GOLDEN_ALGS))
+def test_cfg0_solve_matches_fixture(lri, name):
+    """Workload: n=m=256, r=8, t in [0,1], dt=1e-2 (100 steps), function-form data, FP64."""
+    import os
+
+    say(f"case: cfg0 solve {name}")
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg0_golden.npz"))
+    d, u0 = cfg0_data()
+    prob = lri.MatrixDataProblem(d, lri.SVDLikeRepresentation(u0.U, u0.S, u0.V), (0.0, 1.0))
+    sol = lri.solve(prob, GOLDEN_ALGS[name](lri), 1e-2)
+    assert len(sol.Y) == 101 and sol.t[-1] == z[f"{name}_t"][-1]
+    yf = sol.Y[-1]
+    assert rel_err(yf.U, yf.S, yf.V, z[f"{name}_U"], z[f"{name}_S"], z[f"{name}_V"]) <= FP64_TOL
+    norms = np.array([np.linalg.norm(y.S) for y in sol.Y])
+    assert np.allclose(norms, z[f"{name}_norms"], rtol=1e-11, atol=0)
+    assert [y.rank for y in sol.Y] == list(z[f"{name}_ranks"])
+
+
+def test_array_data_matches_function_data(lri):
+    """The reference's own consistency test (test/data_driven_approximation.jl:27-31) on the engine."""
+    d, u0 = cfg0_data()
+    y0 = lri.SVDLikeRepresentation(u0.U, u0.S, u0.V)
+    snaps = [d(0.01 * k) for k in range(101)]
+    for alg in (lri.ProjectorSplitting(lri.PrimalLieTrotter()), lri.UnconventionalAlgorithm()):
+        a = lri.solve(lri.MatrixDataProblem(d, y0, (0.0, 1.0)), alg, 1e-2).Y[-1]
+        b = lri.solve(lri.MatrixDataProblem(snaps, y0), alg).Y[-1]
+        A, B = a.matrix(), b.matrix()
+        assert np.all(np.abs(A - B) <= np.sqrt(np.finfo(float).eps) * np.maximum(np.abs(A), np.abs(B)))
+
+
+# ------------------------------------------------------------------ mixed precision
+
+
+def rotating_snaps(n, r, steps, seed, dtype):
+    d = rotating_data(n, n, r, noise=1e-3, seed=seed)
+    u0 = d.initial()
+    ts = np.arange(steps + 1) / 199.0
+    snaps = [d(t) for t in ts]
+    if dtype == "bf16":
+        snaps = [bf16_round(a) for a in snaps]
+    elif dtype == "f32":
+        snaps = [a.astype(np.float32).astype(np.float64) for a in snaps]
+    return u0, snaps
+
+
+def run_solve(lri, u0, snaps, kind, precision, dtype):
+    n, m = u0.U.shape[0], u0.V.shape[0]
+    with lri.native.Context(n, m, u0.rank) as ctx:
+        ctx.set_precision(lri.native.PREC_MIXED if precision == "mixed" else lri.native.PREC_F64)
+        ctx.set_state(u0.U, u0.S, u0.V)
+        ctx.reserve(len(snaps), lri.native.DTYPES[dtype])
+        for k, a in enumerate(snaps):
+            ctx.upload(k, lri.api._encode(a, dtype))
+        for k in range(len(snaps) - 1):
+            if kind == "primal":
+                ctx.step_ksl(lri.native.PRIMAL, k, k + 1)
+            elif kind == "dual":
+                ctx.step_ksl(lri.native.DUAL, k, k + 1)
+            else:
+                ctx.step_bug(k, k + 1)
+        return ctx.get_state()
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f32"])
+@pytest.mark.parametrize("kind", ["primal", "dual", "bug"])
+def test_mixed_precision_error(lri, dtype, kind):
+    say(f"case: mixed {kind} {dtype} n=1024 r=32 20 steps")
+    u0, snaps = rotating_snaps(1024, 32, 20, seed=1, dtype=dtype)
+    ref = run_solve(lri, u0, snaps, kind, "f64", dtype)
+    mix = run_solve(lri, u0, snaps, kind, "mixed", dtype)
+    err = rel_err(*mix, *ref)
+    say(f"  rel err mixed vs FP64 = {err:.3e}")
+    assert err <= 1e-6
+    # the FP64 engine on decoded snapshots agrees with the CPU oracle
+    if kind == "primal" and dtype == "bf16":
+        U, S, V = u0.U, u0.S, u0.V
+        for k in range(len(snaps) - 1):
+            U, S, V = dlra.ksl_primal_step(U, S, V, snaps[k + 1] - snaps[k])
+        assert rel_err(*ref, U, S, V) <= FP64_TOL
+
+
+def test_mixed_precision_ragged(lri):
+    """Sizes that are not multiples of the tiles, BF16 vector and scalar load paths."""
+    rng = np.random.default_rng(7)
+    for (n, m, r) in [(200, 136, 5), (517, 389, 40), (64, 1000, 33)]:
+        u = random_factors(rng, n, m, r, decay=0.1)
+        A0 = bf16_round(rng.standard_normal((n, m)))
+        A1 = bf16_round(A0 + 1e-2 * rng.standard_normal((n, m)))
+        ref = run_solve(lri, u, [A0, A1], "bug", "f64", "bf16")
+        mix = run_solve(lri, u, [A0, A1], "bug", "mixed", "bf16")
+        assert rel_err(*mix, *ref) <= 1e-6
+
+
+# ------------------------------------------------------------------ errors, determinism
+
+
+def test_nan_snapshot_reports_nonfinite(lri):
+    u, A0, A1 = one_step_case(6, 64, 64, 4)
+    A1[3, 5] = np.nan
+    with pytest.raises(lri.native.NonFiniteError):
+        run_native_step(lri, "primal", u, [A0, A1])
+
+
+def test_bad_indices_and_missing_state(lri):
+    with lri.native.Context(32, 32, 4) as ctx:
+        with pytest.raises(ValueError):
+            ctx.step_ksl(lri.native.PRIMAL, 0, 1)  # no state
+        rng = np.random.default_rng(0)
+        u = random_factors(rng, 32, 32, 4)
+        ctx.set_state(u.U, u.S, u.V)
+        with pytest.raises(ValueError):
+            ctx.step_ksl(lri.native.PRIMAL, 0, 1)  # no snapshots
+        ctx.reserve(2, lri.native.F64)
+        with pytest.raises(ValueError):
+            ctx.step_ksl(lri.native.PRIMAL, 0, 2)
+        with pytest.raises(ValueError):
+            ctx.step_ksl(lri.native.STRANG, 0, 1)
+        with pytest.raises(ValueError):
+            ctx.set_state(u.U[:, :3], u.S[:3, :3], u.V)
+
+
+def test_bitwise_deterministic(lri):
+    u0, snaps = rotating_snaps(512, 16, 5, seed=2, dtype="bf16")
+    a = run_solve(lri, u0, snaps, "primal", "mixed", "bf16")
+    b = run_solve(lri, u0, snaps, "primal", "mixed", "bf16")
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)

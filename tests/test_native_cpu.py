@@ -1,0 +1,84 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol that
+include/lri.h declares, and validates arguments before touching a device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lri.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lri_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for required in ("lri_create", "lri_destroy", "lri_set_state", "lri_get_state", "lri_snapshots_bind",
+                     "lri_snapshots_reserve", "lri_snapshot_upload", "lri_step_ksl", "lri_step_ksl_strang",
+                     "lri_step_bug", "lri_step_bug_ra", "lri_last_error"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(native):
+    for name in declared_functions():
+        assert hasattr(native, name), f"{name} declared in lri.h but not exported"
+
+
+def test_binding_covers_the_header(lri):
+    assert set(declared_functions()) == set(lri.native.SIGNATURES)
+
+
+def test_status_strings(native):
+    for s in (0, -1, -2, -3, -4, -5, -6):
+        assert native.lri_status_string(s)
+    assert native.lri_version() >= 100
+
+
+def test_create_validates_before_the_device(native):
+    h = ctypes.c_void_p()
+    assert native.lri_create(ctypes.byref(h), 0, 0, 16, 4) == -1
+    assert b"rcap" in native.lri_last_error(None)
+    assert native.lri_create(ctypes.byref(h), 0, 16, 16, 65) == -1
+    assert native.lri_create(ctypes.byref(h), 0, 8, 16, 9) == -1
+    assert native.lri_create(None, 0, 8, 16, 4) == -1
+
+
+def test_null_context_is_rejected(native):
+    assert native.lri_destroy(None) == 0
+    assert native.lri_synchronize(None) == -1
+    assert native.lri_step_ksl(None, 0, 0, 1) == -1
+
+
+def test_api_rejects_out_of_scope_problems(lri):
+    u0 = lri.SVDLikeRepresentation(np.eye(4, 2), np.eye(2), np.eye(4, 2))
+    with pytest.raises(NotImplementedError):
+        lri.init(lri.MatrixDEProblem(lambda y, t: y, u0, (0.0, 1.0)), lri.UnconventionalAlgorithm(), 0.1)
+    with pytest.raises(NotImplementedError):
+        lri.ProjectorSplitting(lri.PrimalLieTrotter(), K_rhs=lambda *a: None)
+    with pytest.raises(AssertionError):
+        lri.solve(lri.MatrixDataProblem(lambda t: np.zeros((4, 4)), u0, (0.0, 1.0)), lri.UnconventionalAlgorithm())
+    with pytest.raises(AssertionError):
+        lri.init(lri.MatrixDataProblem(lambda t: np.zeros((4, 4)), u0, (1.0, 0.0)), lri.UnconventionalAlgorithm(), 0.1)
+
+
+def test_init_sol_slots(lri):
+    s = lri.api.init_sol(1e-2, 0.0, 1.0, None)
+    assert len(s.Y) == 101
+    s = lri.api.init_sol(1, 1, 11, None)
+    assert s.t == list(range(1, 12))
+
+
+def test_bf16_encoding_round_to_nearest_even(lri):
+    x = np.array([[1.0, 1.00390625, 1.01171875, -3.14159, 0.0]])
+    bits = lri.api._encode(x, "bf16")
+    back = (bits.astype(np.uint32) << 16).view(np.float32)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: even mantissa -> 1.0
+    assert back[0, 0] == 1.0 and back[0, 1] == 1.0
+    assert abs(back[0, 3] - np.float32(-3.140625)) == 0
+    assert back[0, 4] == 0.0

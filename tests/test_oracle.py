@@ -1,0 +1,254 @@
+"""Pins for the CPU oracle (no GPU).
+
+The reference (Julia) cannot run here, and its own tests hold no golden
+numbers for this path, so the oracle is pinned by what the mathematics fixes:
+
+* closed forms of one step, brute-forced with explicit n x n projectors whose
+  bases come from an SVD (not from the QR the oracle uses);
+* the exactness theorems of KSL and BUG on exactly rank-r data;
+* invariants (orthonormality, S shapes, zero increment, rank bounds);
+* agreement of the reference-order ("faithful", 3 passes) and 2-pass modes;
+* the reference's own consistency test, function vs snapshot data
+  (test/data_driven_approximation.jl:1-32);
+* the driver's step count and t accumulation (src/primitives.jl:68-104).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import dlra
+from oracle.data import rotating_data, random_factors, skew
+from oracle.lra import SVDLike, truncate_to_tolerance, truncated_svd
+from oracle.metrics import frob_diff, frob_norm, orth_defect, proj_dist, rel_err
+
+
+def orth_svd(A, k):
+    """Orthonormal basis of the range of A via SVD (independent of QR)."""
+    U, _, _ = np.linalg.svd(A, full_matrices=False)
+    return U[:, :k]
+
+
+def case(seed, n=40, m=30, r=4, inc=1e-2):
+    rng = np.random.default_rng(seed)
+    u = random_factors(rng, n, m, r, decay=0.3)
+    dA = inc * rng.standard_normal((n, m))
+    return u, dA
+
+
+# --------------------------------------------------------------- metrics
+
+
+def test_metrics_against_dense():
+    rng = np.random.default_rng(0)
+    a = random_factors(rng, 50, 40, 5)
+    b = SVDLike(a.U.copy(), a.S + 1e-13 * rng.standard_normal((5, 5)), a.V.copy())
+    dense = np.linalg.norm(a.to_matrix() - b.to_matrix())
+    assert abs(frob_diff(a.U, a.S, a.V, b.U, b.S, b.V) - dense) <= 1e-3 * dense + 1e-28
+    assert abs(frob_norm(a.U, a.S, a.V) - np.linalg.norm(a.to_matrix())) < 1e-14
+    c = random_factors(rng, 50, 40, 5)
+    dense = np.linalg.norm(a.to_matrix() - c.to_matrix())
+    assert abs(frob_diff(a.U, a.S, a.V, c.U, c.S, c.V) - dense) < 1e-13
+    Pa, Pc = a.U @ a.U.T, c.U @ c.U.T
+    assert abs(proj_dist(a.U, c.U) - np.linalg.norm(Pa - Pc)) < 1e-13
+    # a rotated basis spans the same space: distance ~ 0
+    Qr, _ = np.linalg.qr(rng.standard_normal((5, 5)))
+    assert proj_dist(a.U, a.U @ Qr) < 1e-14
+
+
+def test_truncated_svd_eckart_young():
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((30, 20))
+    s = np.linalg.svd(A, compute_uv=False)
+    for r in (1, 5, 19):
+        t = truncated_svd(A, r=r)
+        assert t.rank == r
+        assert abs(np.linalg.norm(A - t.to_matrix()) - np.sqrt(np.sum(s[r:] ** 2))) < 1e-12
+
+
+def test_truncate_to_tolerance_rule():
+    s = np.array([1.0, 0.5, 0.1, 1e-3, 1e-6, 1e-9])
+    for tol in (1e-12, 1e-8, 1e-5, 2e-3, 0.2, 10.0):
+        rho = truncate_to_tolerance(s, tol)
+        assert 1 <= rho <= len(s)
+        assert np.sqrt(np.sum(s[rho:] ** 2)) <= tol or rho == len(s)
+        if rho > 1:  # minimal: dropping one more would exceed tol
+            assert np.sqrt(np.sum(s[rho - 1 :] ** 2)) > tol
+    assert truncate_to_tolerance(s, 1e-12) == 6
+    assert truncate_to_tolerance(s, 10.0) == 1
+
+
+# --------------------------------------------------------------- closed forms
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_ksl_primal_closed_form(seed):
+    u, dA = case(seed)
+    Y0 = u.to_matrix()
+    U1, S1, V1 = dlra.ksl_primal_step(u.U, u.S, u.V, dA)
+    Ub = orth_svd(u.U @ u.S + dA @ u.V, u.rank)
+    PU, PV = Ub @ Ub.T, u.V @ u.V.T
+    Yb = Y0 + dA @ PV - PU @ dA @ PV + PU @ dA
+    assert np.linalg.norm(U1 @ S1 @ V1.T - Yb) / np.linalg.norm(Yb) < 1e-13
+    assert np.allclose(S1, np.tril(S1), atol=0)  # S = R' is lower triangular
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_ksl_dual_closed_form(seed):
+    u, dA = case(seed)
+    Y0 = u.to_matrix()
+    U1, S1, V1 = dlra.ksl_dual_step(u.U, u.S, u.V, dA)
+    Vb = orth_svd(u.V @ u.S.T + dA.T @ u.U, u.rank)
+    PU, PV = u.U @ u.U.T, Vb @ Vb.T
+    Yb = Y0 + PU @ dA - PU @ dA @ PV + dA @ PV
+    assert np.linalg.norm(U1 @ S1 @ V1.T - Yb) / np.linalg.norm(Yb) < 1e-13
+    assert np.allclose(S1, np.triu(S1), atol=0)  # S = R is upper triangular
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_bug_closed_form(seed):
+    u, dA = case(seed)
+    Y0 = u.to_matrix()
+    U1, S1, V1 = dlra.bug_step(u.U, u.S, u.V, dA)
+    Ub = orth_svd(u.U @ u.S + dA @ u.V, u.rank)
+    Vb = orth_svd(u.V @ u.S.T + dA.T @ u.U, u.rank)
+    Yb = Ub @ Ub.T @ (Y0 + dA) @ Vb @ Vb.T
+    assert np.linalg.norm(U1 @ S1 @ V1.T - Yb) / np.linalg.norm(Yb) < 1e-13
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_rank_adaptive_closed_form(seed):
+    u, dA = case(seed, inc=1e-2)
+    Y0 = u.to_matrix()
+    r = u.rank
+    tol = 5e-3
+    U1, S1, V1, r1 = dlra.rank_adaptive_bug_step(u.U, u.S, u.V, dA, tol, 2**62)
+    Uh = orth_svd(np.hstack([u.U @ u.S + dA @ u.V, u.U]), 2 * r)
+    Vh = orth_svd(np.hstack([u.V @ u.S.T + dA.T @ u.U, u.V]), 2 * r)
+    Z = Uh @ Uh.T @ (Y0 + dA) @ Vh @ Vh.T
+    Uz, sz, Vzt = np.linalg.svd(Z)
+    rho = truncate_to_tolerance(sz[: 2 * r], tol)
+    assert r1 == rho
+    Yb = Uz[:, :rho] @ np.diag(sz[:rho]) @ Vzt[:rho]
+    assert np.linalg.norm(U1 @ S1 @ V1.T - Yb) / np.linalg.norm(Yb) < 1e-12
+    assert np.allclose(S1, np.diag(np.diag(S1)), atol=0)
+    assert np.all(np.diff(np.diag(S1)) <= 0)
+
+
+def test_rank_adaptive_rank_bounds():
+    u, dA = case(3, inc=1e-1)
+    r = u.rank
+    for rmax in (1, 3, 6, 2**62):
+        *_, r1 = dlra.rank_adaptive_bug_step(u.U, u.S, u.V, dA, 1e-14, rmax)
+        assert 1 <= r1 <= min(2 * r, rmax)
+
+
+# --------------------------------------------------------------- invariants
+
+
+@pytest.mark.parametrize("step", [dlra.ksl_primal_step, dlra.ksl_dual_step, dlra.bug_step])
+def test_zero_increment_and_orthonormality(step):
+    u, dA = case(4)
+    U1, S1, V1 = step(u.U, u.S, u.V, np.zeros_like(dA))
+    assert rel_err(U1, S1, V1, u.U, u.S, u.V) < 1e-14
+    U1, S1, V1 = step(u.U, u.S, u.V, dA)
+    assert orth_defect(U1) < 1e-14 and orth_defect(V1) < 1e-14
+
+
+def test_rank_adaptive_zero_increment():
+    u, dA = case(5)
+    U1, S1, V1, r1 = dlra.rank_adaptive_bug_step(u.U, u.S, u.V, np.zeros_like(dA), 1e-10, 2**62)
+    assert r1 == u.rank
+    assert rel_err(U1, S1, V1, u.U, u.S, u.V) < 1e-13
+
+
+@pytest.mark.parametrize(
+    "step", [dlra.ksl_primal_step, dlra.ksl_dual_step, dlra.bug_step]
+)
+def test_faithful_matches_efficient(step):
+    u, dA = case(6)
+    a = step(u.U, u.S, u.V, dA, mode="faithful")
+    b = step(u.U, u.S, u.V, dA, mode="efficient")
+    assert rel_err(*a, *b) < 1e-13
+
+
+# --------------------------------------------------------------- exactness
+
+
+@pytest.mark.parametrize(
+    "alg",
+    [dlra.ProjectorSplitting(dlra.PrimalLieTrotter()), dlra.ProjectorSplitting(dlra.DualLieTrotter()),
+     dlra.ProjectorSplitting(dlra.Strang()), dlra.UnconventionalAlgorithm()],
+    ids=["ksl_primal", "ksl_dual", "ksl_strang", "bug"],
+)
+def test_exactness_on_rank_r_data(alg):
+    """Lubich-Oseledets 2014 / Ceruti-Lubich 2021: rank-r data is reproduced exactly."""
+    d = rotating_data(48, 40, 4, noise=0.0, seed=7)
+    sol = dlra.solve(dlra.MatrixDataProblem(d, d.initial(), (0.0, 1.0)), alg, 1e-2)
+    Yf = sol.Y[-1]
+    exact = d(sol.t[-1])
+    assert np.linalg.norm(Yf.to_matrix() - exact) / np.linalg.norm(exact) < 1e-12
+
+
+def test_rank_adaptive_exactness_keeps_rank():
+    d = rotating_data(48, 40, 4, noise=0.0, seed=8)
+    alg = dlra.RankAdaptiveUnconventionalAlgorithm(1e-10, rmax=20)
+    sol = dlra.solve(dlra.MatrixDataProblem(d, d.initial(), (0.0, 1.0)), alg, 1e-2)
+    Yf = sol.Y[-1]
+    assert Yf.rank == 4
+    exact = d(sol.t[-1])
+    assert np.linalg.norm(Yf.to_matrix() - exact) / np.linalg.norm(exact) < 1e-11
+
+
+# --------------------------------------------------------------- driver
+
+
+def test_driver_step_count_and_time():
+    d = rotating_data(16, 12, 2, seed=9)
+    sol = dlra.solve(dlra.MatrixDataProblem(d, d.initial(), (0.0, 1.0)), dlra.UnconventionalAlgorithm(), 1e-2)
+    assert len(sol.Y) == 101 and len(sol.t) == 101
+    t = 0.0
+    for _ in range(100):
+        t += 1e-2
+    assert sol.t[-1] == t == 1.0000000000000007
+    snaps = d.snapshots(np.linspace(0, 1, 11))
+    sol = dlra.solve(dlra.MatrixDataProblem(snaps, d.initial()), dlra.UnconventionalAlgorithm())
+    assert sol.t == list(range(1, 12)) and len(sol.Y) == 11
+
+
+def test_driver_errors():
+    d = rotating_data(16, 12, 2, seed=9)
+    with pytest.raises(AssertionError):
+        dlra.solve(dlra.MatrixDataProblem(d, d.initial(), (1.0, 0.0)), dlra.UnconventionalAlgorithm(), 1e-2)
+    with pytest.raises(AssertionError):
+        dlra.solve(dlra.MatrixDataProblem(d, d.initial(), (0.0, 1.0)), dlra.UnconventionalAlgorithm())
+    snaps = d.snapshots(np.linspace(0, 1, 5))
+    with pytest.raises(TypeError):  # no array method for the half-step index (data_integrator.jl:26)
+        dlra.solve(dlra.MatrixDataProblem(snaps, d.initial()), dlra.ProjectorSplitting(dlra.Strang()))
+
+
+def test_reference_consistency_function_vs_snapshots():
+    """Restatement of test/data_driven_approximation.jl:1-32 (seeded, N reduced to 60)."""
+    N = 60
+    rng = np.random.default_rng(11)
+    Ws = []
+    for _ in range(2):
+        W = np.triu(rng.standard_normal((N, N)), 1)
+        Ws.append(W - W.T)
+    D = np.diag(2.0 ** -np.arange(1, N + 1))
+    import scipy.linalg as sla
+
+    def Y(t):
+        return sla.expm(t * Ws[0]) @ (math.exp(t) * D) @ sla.expm(t * Ws[1])
+
+    X0 = truncated_svd(Y(0.0), tol=1e-4)
+    prob = dlra.MatrixDataProblem(Y, X0, (0.0, 1.0))
+    data = [Y(0.01 * k) for k in range(101)]
+    dprob = dlra.MatrixDataProblem(data, X0)
+    for alg in (dlra.UnconventionalAlgorithm(), dlra.ProjectorSplitting(dlra.PrimalLieTrotter()),
+                dlra.ProjectorSplitting(dlra.DualLieTrotter()),
+                dlra.RankAdaptiveUnconventionalAlgorithm(1e-8, rmax=20)):
+        a = dlra.solve(prob, alg, 1e-2).Y[-1].to_matrix()
+        b = dlra.solve(dprob, alg).Y[-1].to_matrix()
+        # Julia's elementwise isapprox, rtol = sqrt(eps), atol = 0
+        assert np.all(np.abs(a - b) <= np.sqrt(np.finfo(float).eps) * np.maximum(np.abs(a), np.abs(b)))
