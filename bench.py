@@ -1,0 +1,359 @@
+"""Benchmark: steps/s of the mixed-precision primal KSL data integrator on one MI355X.
+
+Workload (the headline configuration): projector-splitting KSL, primal
+Lie-Trotter, on a MatrixDataProblem with n = m = 16384, rank r = 32 and 200
+snapshots (199 steps per sweep over the data), snapshots stored in BF16 in HBM,
+products on BF16 matrix cores with a hi/lo split (FP32 accumulation), Gram,
+Cholesky and the S step in FP64.
+
+Data (synthetic, generated on the GPU from one seed):
+    A_k = Q1(t_k) (A1 + e^{t_k} A2) Q2(t_k)',  Qi(t) = expm(t Ti),  t_k = k/199
+with Ti skew-symmetric (upper triangle N(0, 1/n)), A1 rank 32 with singular
+values 2^-i, A2 = 1e-3 N(0,1).  The rotations are advanced by Ei = expm(dt Ti)
+(Taylor series in FP32, ||dt Ti|| ~ 0.01) and the noise term by
+C_{k+1} = E1 C_k E2' in FP32 GEMMs; the snapshots are rounded to BF16.
+Y0 is a rank-32 randomized SVD of A_0 (FP64, 2 power iterations).
+
+Reported beside the headline value:
+* the FP64 form (FP64 snapshots, FP64 products) timed on a ring of resident
+  FP64 snapshots, and the speed-up of the mixed form over it;
+* the FP32 form (FP32 snapshots, FP32 matrix cores) timed the same way;
+* the relative Frobenius error of the mixed result after one full sweep vs the
+  FP64 engine on the same decoded snapshots (and FP32 vs FP64 on its ring);
+* roofline of the dominant pass kernel from HIP events on the engine's stream;
+* the CPU oracle (NumPy/OpenBLAS FP64, reference operation order) on a bounded
+  sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "steps/s at n=m=16384,r=32: mixed vs FP64 @≤1e-6 rel err; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------- data
+
+
+def skew_t(torch, gen, n, device):
+    W = torch.randn(n, n, generator=gen, device=device, dtype=torch.float32) / math.sqrt(n)
+    W = torch.triu(W, 1)
+    return W - W.t()
+
+
+def expm_small(torch, X, order=10):
+    """exp(X) by a Taylor series, for ||X|| << 1 (here ~0.01, truncation error < 1e-25)."""
+    n = X.shape[0]
+    E = torch.eye(n, device=X.device, dtype=X.dtype)
+    term = torch.eye(n, device=X.device, dtype=X.dtype)
+    for k in range(1, order + 1):
+        term = (term @ X) / k
+        E += term
+        if float(term.abs().max()) < 1e-30:
+            break
+    return E
+
+
+def generate_snapshots(torch, n, r, count, seed, device):
+    """BF16 snapshots A_k (column-major n x n: stored as row-major A_k')."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    dt = 1.0 / (count - 1)
+    T1 = skew_t(torch, gen, n, device)
+    T2 = skew_t(torch, gen, n, device)
+    Ua, _ = torch.linalg.qr(torch.randn(n, r, generator=gen, device=device, dtype=torch.float64))
+    Va, _ = torch.linalg.qr(torch.randn(n, r, generator=gen, device=device, dtype=torch.float64))
+    sig = 2.0 ** -torch.arange(r, device=device, dtype=torch.float64)
+    D = 1e-3 * torch.randn(n, n, generator=gen, device=device, dtype=torch.float32)  # C_0' = A2'
+    E1 = expm_small(torch, dt * T1)
+    E2 = expm_small(torch, dt * T2)
+    del T1, T2
+    E1d, E2d = E1.double(), E2.double()
+    snaps = []
+    tmp = torch.empty_like(D)
+    for k in range(count):
+        t = k * dt
+        # A_k' = Va_k diag(sig) Ua_k' + e^t D_k,  D_k = C_k'
+        lowrank = ((Va * sig) @ Ua.t()).float()
+        A_t = lowrank.add_(D, alpha=math.exp(t))
+        snaps.append(A_t.to(torch.bfloat16))
+        del A_t, lowrank
+        if k + 1 < count:
+            # D_{k+1} = (E1 C_k E2')' = E2 D_k E1'
+            torch.matmul(E2, D, out=tmp)
+            torch.matmul(tmp, E1.t(), out=D)
+            Ua = E1d @ Ua
+            Va = E2d @ Va
+        if (k + 1) % 25 == 0 or k + 1 == count:
+            torch.cuda.synchronize()
+            log(f"generated {k + 1}/{count} snapshots")
+    del D, tmp, E1, E2, E1d, E2d
+    torch.cuda.empty_cache()
+    return snaps
+
+
+def randomized_svd(torch, A_t, r, seed, p=32, q=2):
+    """Rank-r SVD of A (given A' row-major, i.e. A column-major) in FP64."""
+    gen = torch.Generator(device=A_t.device)
+    gen.manual_seed(seed + 1)
+    A = A_t.double().t()  # view of A
+    m = A.shape[1]
+    Om = torch.randn(m, r + p, generator=gen, device=A_t.device, dtype=torch.float64)
+    Y = A @ Om
+    Qy, _ = torch.linalg.qr(Y)
+    for _ in range(q):
+        Z, _ = torch.linalg.qr(A.t() @ Qy)
+        Qy, _ = torch.linalg.qr(A @ Z)
+    B = Qy.t() @ A
+    Ub, s, Vh = torch.linalg.svd(B, full_matrices=False)
+    U = (Qy @ Ub[:, :r]).cpu().numpy()
+    V = Vh[:r].t().cpu().numpy()
+    S = np.diag(s[:r].cpu().numpy())
+    del A
+    return U, S, V
+
+
+# ---------------------------------------------------------------------------- runs
+
+
+def run_steps(ctx, lri, nsteps, npairs, order_primal=True):
+    for i in range(nsteps):
+        k = i % npairs
+        ctx.step_ksl(lri.native.PRIMAL, k, k + 1)
+
+
+def timed_steps(torch, ctx, lri, nsteps, npairs):
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_steps(ctx, lri, nsteps, npairs)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(n, r, steps=3):
+    """Oracle (reference operation order, 3 passes per step) on a bounded sample."""
+    from oracle import dlra
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    rng = np.random.default_rng(0)
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    S = np.diag(2.0 ** -np.arange(r))
+    A = np.asfortranarray(1e-3 * rng.standard_normal((n, n)))
+    ring = [A]
+    for _ in range(2):
+        ring.append(np.asfortranarray(ring[-1] + 1e-5 * rng.standard_normal((n, n))))
+    # warm-up step (untimed), then `steps` timed steps
+    U, S, V = dlra.ksl_primal_step(U, S, V, ring[1] - ring[0], mode="faithful")
+    t0 = time.perf_counter()
+    for i in range(steps):
+        a, b = ring[(i + 1) % 3], ring[(i + 2) % 3]
+        U, S, V = dlra.ksl_primal_step(U, S, V, b - a, mode="faithful")
+    dt = time.perf_counter() - t0
+    return {
+        "value": steps / dt,
+        "unit": "steps/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"{steps} primal KSL steps (+1 warm-up) of the FP64 NumPy oracle in the reference's operation "
+                  f"order (3 passes over the increment per step) at n=m={n}, r={r} on a ring of 3 resident FP64 "
+                  f"snapshots; OpenBLAS threads={threads}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=995)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--rank", type=int, default=32)
+    ap.add_argument("--snapshots", type=int, default=200)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--ring", type=int, default=8, help="FP64/FP32 ring length (snapshots)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP64/FP32 forms")
+    ap.add_argument("--no-check", action="store_true", help="skip the error check")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU: multi-GPU runs are not part of this build "
+                                   "(the design for row sharding over RCCL is documented, not built)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__
+
+    lri = __graft_entry__.load_package()
+    lri.native.lib()
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    n = m = args.n
+    r = args.rank
+    N = args.snapshots
+    npairs = N - 1
+
+    log(f"generating {N} BF16 snapshots n=m={n} on the GPU")
+    tg = time.perf_counter()
+    snaps = generate_snapshots(torch, n, r, N, args.seed, dev)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - tg
+    log(f"generation {gen_s:.1f} s; randomized SVD of A_0")
+    U0, S0, V0 = randomized_svd(torch, snaps[0], r, args.seed)
+
+    ctx = lri.native.Context(n, m, r, 0)
+    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, n, owners=snaps)
+    ctx.set_precision(lri.native.PREC_MIXED)
+    ctx.set_state(U0, S0, V0)
+
+    log(f"warm-up {args.warmup} steps")
+    run_steps(ctx, lri, args.warmup, npairs)
+    ctx.set_state(U0, S0, V0)
+    ctx.timing(True)
+    log(f"timed {args.steps} mixed-precision steps")
+    el = timed_steps(torch, ctx, lri, args.steps, npairs)
+    value = args.steps / el
+    fam = {}
+    for name, f in (("pass_N", lri.native.KERNEL_PASS_N), ("pass_T", lri.native.KERNEL_PASS_T),
+                    ("small", lri.native.KERNEL_SMALL)):
+        ms, cnt = ctx.timing_read(f)
+        fam[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / max(cnt, 1)}
+    ctx.timing(False)
+    # untimed-event check of the event overhead
+    ctx.set_state(U0, S0, V0)
+    el_noev = timed_steps(torch, ctx, lri, min(args.steps, 2 * npairs), npairs)
+    value_noev = min(args.steps, 2 * npairs) / el_noev
+    log(f"mixed: {value:.1f} steps/s ({1e3 * el / args.steps:.3f} ms/step); without kernel events {value_noev:.1f}")
+
+    dom = max(("pass_N", "pass_T"), key=lambda k: fam[k]["avg_ms"])
+    bytes_per_launch = 2.0 * n * m * 2  # both BF16 snapshots streamed once
+    achieved = bytes_per_launch / (fam[dom]["avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pass_traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get(dom)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "bytes_per_launch": bytes_per_launch}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"KSL primal Lie-Trotter, MatrixDataProblem, n=m={n}, r={r}, {N} BF16 snapshots "
+                               f"resident in HBM ({npairs} steps per sweep), BF16x3 matrix-core products with FP32 "
+                               f"accumulation, FP64 CholeskyQR2/S-step",
+                   "n": n, "m": m, "r": r, "snapshots": N, "snapshot_dtype": "bf16", "precision": "mixed",
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "kernels": fam,
+        "value_without_kernel_events": value_noev,
+        "generation_s": gen_s,
+    }
+
+    if not args.no_check:
+        log("error check: one sweep mixed vs FP64 products on the same BF16 snapshots")
+        ctx.set_precision(lri.native.PREC_MIXED)
+        ctx.set_state(U0, S0, V0)
+        run_steps(ctx, lri, npairs, npairs)
+        Ym = ctx.get_state()
+        ctx.set_precision(lri.native.PREC_F64)
+        ctx.set_state(U0, S0, V0)
+        t0 = time.perf_counter()
+        run_steps(ctx, lri, npairs, npairs)
+        Yd = ctx.get_state()
+        f64_on_bf16 = npairs / (time.perf_counter() - t0)
+        from oracle.metrics import rel_err
+
+        out["rel_err_vs_fp64"] = rel_err(*Ym, *Yd)
+        out["fp64_products_on_bf16_steps_per_s"] = f64_on_bf16
+        log(f"rel err mixed vs FP64 after {npairs} steps: {out['rel_err_vs_fp64']:.3e}")
+    ctx.close()
+
+    if not args.no_forms:
+        R = min(args.ring, N)
+        for form, tdt, ldt, prec in (("fp64", torch.float64, lri.native.F64, lri.native.PREC_F64),
+                                     ("fp32", torch.float32, lri.native.F32, lri.native.PREC_MIXED)):
+            ring = [snaps[k].to(tdt) for k in range(R)]
+            c2 = lri.native.Context(n, m, r, 0)
+            c2.bind([a.data_ptr() for a in ring], ldt, n, owners=ring)
+            c2.set_precision(prec)
+            c2.set_state(U0, S0, V0)
+            run_steps(c2, lri, 4, R - 1)
+            c2.set_state(U0, S0, V0)
+            ns = 10 * (R - 1)
+            c2.timing(True)
+            el2 = timed_steps(torch, c2, lri, ns, R - 1)
+            pN = c2.timing_read(lri.native.KERNEL_PASS_N)
+            pT = c2.timing_read(lri.native.KERNEL_PASS_T)
+            c2.timing(False)
+            out[f"{form}_steps_per_s"] = ns / el2
+            esz = 8 if form == "fp64" else 4
+            avg = max(pN[0] / max(pN[1], 1), pT[0] / max(pT[1], 1))
+            out[f"{form}_pass_GBs"] = 2.0 * n * m * esz / (avg * 1e-3) / 1e9
+            if form == "fp32" and not args.no_check:
+                c2.set_state(U0, S0, V0)
+                run_steps(c2, lri, R - 1, R - 1)
+                Yf = c2.get_state()
+                c2.set_precision(lri.native.PREC_F64)
+                c2.set_state(U0, S0, V0)
+                run_steps(c2, lri, R - 1, R - 1)
+                Yd = c2.get_state()
+                from oracle.metrics import rel_err
+
+                out["fp32_rel_err_vs_fp64"] = rel_err(*Yf, *Yd)
+            c2.close()
+            del ring
+            torch.cuda.empty_cache()
+            log(f"{form} form: {out[f'{form}_steps_per_s']:.1f} steps/s")
+        out["speedup_vs_fp64"] = value / out["fp64_steps_per_s"]
+
+    del snaps
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        log("CPU baseline (oracle) on a bounded sample")
+        out["cpu_baseline"] = cpu_baseline(n, r)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

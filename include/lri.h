@@ -173,6 +173,13 @@ int lri_kernel_timing_read(lri_ctx* ctx, int family, double* total_ms, int64_t* 
  * 1024 floats).  Results are ordered [lane][register]. */
 int lri_selftest_mfma(int which, const void* a, const void* b, void* out);
 
+/* One product pass with the current state and precision, reduced to FP64 and
+ * copied to host (synchronising): which = 0 gives dA * V (n x r, ld ldo >= n),
+ * which = 1 gives dA' * U (m x r, ld ldo >= m), dA = A[k_next] - A[k_prev].
+ * These are the n x m x r products of the data sub-integrator
+ * (data_integrator.jl:14-16) that every step is built from. */
+int lri_debug_product(lri_ctx* ctx, int which, int64_t k_prev, int64_t k_next, double* out, int64_t ldo);
+
 #ifdef __cplusplus
 }
 #endif

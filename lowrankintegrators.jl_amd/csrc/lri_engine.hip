@@ -1012,6 +1012,34 @@ int lri_kernel_timing_read(lri_ctx* ctx, int family, double* total_ms, int64_t* 
   });
 }
 
+int lri_debug_product(lri_ctx* ctx, int which, int64_t k_prev, int64_t k_next, double* out, int64_t ldo) {
+  return guarded(ctx, [&] {
+    ctx->check_step(k_prev, k_next);
+    REQUIRE(which == 0 || which == 1, "which must be 0 (dA V) or 1 (dA' U)");
+    REQUIRE(out != nullptr, "out is NULL");
+    const int rr = (int)ctx->r;
+    const int64_t rows = which == 0 ? ctx->n : ctx->m;
+    REQUIRE(ldo >= rows, "ldo too small");
+    ctx->ensure_packed(which == 0);
+    void* part = which == 0 ? ctx->partN : ctx->partT;
+    const int ns = which == 0 ? ctx->passN(k_prev, k_next, rr, part) : ctx->passT(k_prev, k_next, rr, part);
+    double* dst = which == 0 ? ctx->P : ctx->Q;
+    ctx->with_parts(part, [&](auto A) {
+      A.rows = rows;
+      A.nsplit = ns;
+      A.rb = lri_ctx::rb_for(ctx->pass_mode(), rr);
+      A.r_out = rr;
+      A.out = dst;
+      A.ldo = rows;
+      ctx->rows(A);
+    });
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(hipMemcpy2D(out, sizeof(double) * ldo, dst, sizeof(double) * rows, sizeof(double) * rows, rr,
+                         hipMemcpyDeviceToHost));
+    return ctx->read_flags();
+  });
+}
+
 int lri_selftest_mfma(int which, const void* a, const void* b, void* out) {
   if (which < 0 || which > 2 || !a || !b || !out) return LRI_EINVAL;
   hipError_t e = selftest_mfma(which, a, b, out);

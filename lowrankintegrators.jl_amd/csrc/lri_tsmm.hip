@@ -17,7 +17,7 @@
 //
 // Matrix-core instruction layouts used below (checked by lri_selftest_mfma):
 //  f64 16x16x4:   A lane l -> A[l%16][l/16]; B lane l -> B[l/16][l%16];
-//                 D lane l, reg v -> D[4*(l/16)+v][l%16]
+//                 D lane l, reg v -> D[4*v+l/16][l%16]
 //  bf16 32x32x16: A lane l, elem e -> A[l%32][8*(l/32)+e]; B -> B[8*(l/32)+e][l%32];
 //                 D lane l, reg v -> D[8*(v/4)+4*(l/32)+(v%4)][l%32]
 //  f32 32x32x2:   A lane l -> A[l%32][l/32]; B lane l -> B[l/32][l%32]; D as bf16.
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_f64(const T* __restrict_
     for (int c = 0; c < CB; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t ii = i0 + 16 * t + 4 * lk + v;
+        const int64_t ii = i0 + 16 * t + 4 * v + lk;
         if (ii < n) part[((int64_t)s * rb + c * 16 + lr) * n + ii] = acc[t][c][v];
       }
 }
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict_
     for (int c = 0; c < CB; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t jj = j0 + 16 * t + 4 * lk + v;
+        const int64_t jj = j0 + 16 * t + 4 * v + lk;
         if (jj < m) part[((int64_t)s * rb + c * 16 + lr) * m + jj] = acc[t][c][v];
       }
 }

@@ -51,6 +51,7 @@ SIGNATURES = {
     "lri_kernel_timing": (_int, [_vp, _int]),
     "lri_kernel_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)]),
     "lri_selftest_mfma": (_int, [_int, _vp, _vp, _vp]),
+    "lri_debug_product": (_int, [_vp, _int, _i64, _i64, _dp, _i64]),
 }
 
 _LIB = None
@@ -202,6 +203,14 @@ class Context:
         s = _vp()
         check(lib().lri_get_stream(self.h, ctypes.byref(s)), self.h)
         return int(s.value or 0)
+
+    def product(self, which: int, kp: int, kn: int) -> np.ndarray:
+        """dA @ V (which=0, n x r) or dA.T @ U (which=1, m x r) with the current state."""
+        r = self.rank()
+        rows = self.n if which == 0 else self.m
+        out = np.zeros((rows, r), order="F")
+        check(lib().lri_debug_product(self.h, int(which), int(kp), int(kn), _dptr(out), rows), self.h)
+        return out
 
     # -- timing
     def timing(self, enable: bool):

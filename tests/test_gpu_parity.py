@@ -38,7 +38,7 @@ def cfg0_data():
 
 def test_mfma_layouts(lri):
     rng = np.random.default_rng(0)
-    # FP64 16x16x4: A[i][k] at lane 16k+i, B[k][j] at lane 16k+j, D[4*(l/16)+v][l%16]
+    # FP64 16x16x4: A[i][k] at lane 16k+i, B[k][j] at lane 16k+j, D[4*v+l/16][l%16]
     A = rng.integers(-4, 5, (16, 4)).astype(np.float64)
     B = rng.integers(-4, 5, (4, 16)).astype(np.float64)
     a = np.array([A[l % 16, l // 16] for l in range(64)])
@@ -48,7 +48,7 @@ def test_mfma_layouts(lri):
     got = np.zeros((16, 16))
     for l in range(64):
         for v in range(4):
-            got[4 * (l // 16) + v, l % 16] = out[l, v]
+            got[4 * v + l // 16, l % 16] = out[l, v]
     assert np.array_equal(got, D)
     # BF16 32x32x16: A[i][8*(l/32)+e] at lane l (i = l%32), B[8*(l/32)+e][l%32];
     # D[8*(v/4)+4*(l/32)+(v%4)][l%32]
@@ -78,6 +78,73 @@ def test_mfma_layouts(lri):
         for v in range(16):
             got[8 * (v // 4) + 4 * (l // 32) + (v % 4), l % 32] = out[l, v]
     assert np.array_equal(got, A.astype(np.float64) @ B.astype(np.float64))
+
+
+def test_mfma_layout_decode(lri):
+    """Decode where each (row, col) of D lands: A[i][0] = 1, A[i][1] = i, B[0][j] = 64 j, B[1][j] = 1,
+    so D[i][j] = 64 j + i.  Prints the observed map of the first lanes (diagnostic for the test above)."""
+    res = {}
+    # FP64 16x16x4
+    A = np.zeros((16, 4)); A[:, 0] = 1; A[:, 1] = np.arange(16)
+    B = np.zeros((4, 16)); B[0, :] = 64 * np.arange(16); B[1, :] = 1
+    a = np.array([A[l % 16, l // 16] for l in range(64)])
+    b = np.array([B[l // 16, l % 16] for l in range(64)])
+    out = lri.native.selftest_mfma(0, a, b).reshape(64, 4)
+    bad = [(l, v, int(out[l, v]) % 64, int(out[l, v]) // 64) for l in range(64) for v in range(4)
+           if (int(out[l, v]) % 64, int(out[l, v]) // 64) != (4 * v + l // 16, l % 16)]
+    res["f64"] = bad[:8]
+    # BF16 32x32x16
+    A = np.zeros((32, 16)); A[:, 0] = 1; A[:, 1] = np.arange(32)
+    B = np.zeros((16, 32)); B[0, :] = 64 * np.arange(32); B[1, :] = 1
+    a = np.zeros((64, 8), np.uint16); b = np.zeros((64, 8), np.uint16)
+    for l in range(64):
+        for e in range(8):
+            a[l, e] = bf16_bits(np.array([A[l % 32, 8 * (l // 32) + e]]))[0]
+            b[l, e] = bf16_bits(np.array([B[8 * (l // 32) + e, l % 32]]))[0]
+    out = lri.native.selftest_mfma(1, a, b).reshape(64, 16)
+    bad = [(l, v, int(out[l, v]) % 64, int(out[l, v]) // 64) for l in range(64) for v in range(16)
+           if (int(out[l, v]) % 64, int(out[l, v]) // 64) != (8 * (v // 4) + 4 * (l // 32) + (v % 4), l % 32)]
+    res["bf16"] = bad[:8]
+    say(f"layout mismatches (lane, reg, row, col): {res}")
+    assert not res["f64"] and not res["bf16"]
+
+
+# ------------------------------------------------------------------ the product passes in isolation
+
+
+@pytest.mark.parametrize("dtype,precision,tol", [("f64", "f64", 1e-13), ("bf16", "f64", 1e-13),
+                                                 ("f32", "f64", 1e-13), ("bf16", "mixed", 2e-5),
+                                                 ("f32", "mixed", 2e-6)])
+@pytest.mark.parametrize("shape", [(256, 256, 8), (1000, 700, 32), (37, 53, 3), (512, 384, 64)])
+def test_products(lri, dtype, precision, tol, shape):
+    n, m, r = shape
+    say(f"case: products n={n} m={m} r={r} {dtype} {precision}")
+    u, A0, A1 = one_step_case(9, n, m, r)
+    if dtype == "bf16":
+        A0, A1 = bf16_round(A0), bf16_round(A1)
+    elif dtype == "f32":
+        A0, A1 = A0.astype(np.float32).astype(np.float64), A1.astype(np.float32).astype(np.float64)
+    with lri.native.Context(n, m, r) as ctx:
+        ctx.set_precision(lri.native.PREC_MIXED if precision == "mixed" else lri.native.PREC_F64)
+        ctx.set_state(u.U, u.S, u.V)
+        ctx.reserve(2, lri.native.DTYPES[dtype])
+        ctx.upload(0, lri.api._encode(A0, dtype))
+        ctx.upload(1, lri.api._encode(A1, dtype))
+        P = ctx.product(0, 0, 1)
+        Q = ctx.product(1, 0, 1)
+    dA = A1 - A0
+    Pr, Qr = dA @ u.V, dA.T @ u.U
+    eP = np.linalg.norm(P - Pr) / np.linalg.norm(Pr)
+    eQ = np.linalg.norm(Q - Qr) / np.linalg.norm(Qr)
+    say(f"  rel err dA V = {eP:.2e}, dA' U = {eQ:.2e}")
+    assert eP <= tol and eQ <= tol
+
+
+def test_zero_increment_steps(lri):
+    u, A0, _ = one_step_case(8, 96, 80, 6)
+    for kind in ("primal", "dual", "bug"):
+        U, S, V = run_native_step(lri, kind, u, [A0, A0.copy()])
+        assert rel_err(U, S, V, u.U, u.S, u.V) <= 1e-13, kind
 
 
 # ------------------------------------------------------------------ FP64 parity, one step
@@ -265,12 +332,29 @@ def test_mixed_precision_error(lri, dtype, kind):
 
 
 def test_mixed_precision_ragged(lri):
-    """Sizes that are not multiples of the tiles, BF16 vector and scalar load paths."""
+    """Sizes that are not multiples of the tiles (BF16 vector and scalar load paths): the products to
+    2e-5 relative (three BF16 products per term drop terms below 2^-17), and one BUG step on data whose
+    increment is 1e-3 of the data to 1e-6."""
     rng = np.random.default_rng(7)
     for (n, m, r) in [(200, 136, 5), (517, 389, 40), (64, 1000, 33)]:
-        u = random_factors(rng, n, m, r, decay=0.1)
-        A0 = bf16_round(rng.standard_normal((n, m)))
-        A1 = bf16_round(A0 + 1e-2 * rng.standard_normal((n, m)))
+        say(f"case: ragged mixed n={n} m={m} r={r}")
+        X = rng.standard_normal((n, r)) @ np.diag(2.0 ** -np.arange(r)) @ rng.standard_normal((r, m))
+        A0 = bf16_round(X + 1e-3 * rng.standard_normal((n, m)))
+        A1 = bf16_round(A0 + 1e-3 * rng.standard_normal((n, m)))
+        U, s, Vt = np.linalg.svd(A0, full_matrices=False)
+        from oracle.lra import SVDLike
+
+        u = SVDLike(U[:, :r].copy(), np.diag(s[:r]), Vt[:r].T.copy())
+        with lri.native.Context(n, m, r) as ctx:
+            ctx.set_precision(lri.native.PREC_MIXED)
+            ctx.set_state(u.U, u.S, u.V)
+            ctx.reserve(2, lri.native.BF16)
+            ctx.upload(0, lri.api._encode(A0, "bf16"))
+            ctx.upload(1, lri.api._encode(A1, "bf16"))
+            P, Q = ctx.product(0, 0, 1), ctx.product(1, 0, 1)
+        dA = A1 - A0
+        assert np.linalg.norm(P - dA @ u.V) <= 2e-5 * np.linalg.norm(dA @ u.V)
+        assert np.linalg.norm(Q - dA.T @ u.U) <= 2e-5 * np.linalg.norm(dA.T @ u.U)
         ref = run_solve(lri, u, [A0, A1], "bug", "f64", "bf16")
         mix = run_solve(lri, u, [A0, A1], "bug", "mixed", "bf16")
         assert rel_err(*mix, *ref) <= 1e-6
