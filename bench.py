@@ -197,6 +197,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP64/FP32 forms")
     ap.add_argument("--no-check", action="store_true", help="skip the error check")
+    ap.add_argument("--no-events", action="store_true",
+                    help="no HIP-event kernel timing in the timed region (for external profilers)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -234,7 +236,7 @@ def main():
     log(f"warm-up {args.warmup} steps")
     run_steps(ctx, lri, args.warmup, npairs)
     ctx.set_state(U0, S0, V0)
-    ctx.timing(True)
+    ctx.timing(not args.no_events)
     log(f"timed {args.steps} mixed-precision steps")
     el = timed_steps(torch, ctx, lri, args.steps, npairs)
     value = args.steps / el

@@ -77,7 +77,6 @@ struct lri_ctx {
   size_t part_bytes = 0;
   // reductions
   double* gpart = nullptr;
-  unsigned* counter = nullptr;
   int* flags = nullptr;
   SmallState* small = nullptr;
   double* svd_work = nullptr;
@@ -150,7 +149,6 @@ struct lri_ctx {
   template <typename PT>
   void rows(RowArgs<PT> A) {
     A.epi.small = small;
-    A.epi.counter = counter;
     A.epi.flags = flags;
     A.gpart = gpart;
     timed(LRI_KERNEL_SMALL, [&] { return launch_rows<PT>(A, stream); });
@@ -178,7 +176,7 @@ struct lri_ctx {
     } else {
       A.pk_hi = forN ? pkN_hi : pkT_hi;
       A.pk_lo = forN ? pkN_lo : pkT_lo;
-      A.ld_pk = forN ? ldm_pk : ldn_pk;
+      A.pk_layout = forN ? 0 : 1;
     }
     if (forN) {
       pkN_mode = mode;
@@ -700,7 +698,7 @@ void free_ctx(lri_ctx* c) {
   void* bufs[] = {c->U,      c->V,      c->S,      c->U2,      c->V2,      c->K,      c->K2,     c->P,
                   c->L,      c->L2,     c->Q,      c->Uhat,    c->Vhat,    c->pkN_f64, c->pkT_f64, c->pkN_f32,
                   c->pkT_f32, c->pkN_hi, c->pkN_lo, c->pkT_hi,  c->pkT_lo,  c->partN,  c->partT,  c->gpart,
-                  c->counter, c->flags,  c->small,  c->svd_work};
+                  c->flags,  c->small,  c->svd_work};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   for (auto& t : c->pending) {
@@ -800,8 +798,7 @@ int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
     c->part_bytes = pb;
     alloc(&c->partN, pb);
     alloc(&c->partT, pb);
-    alloc((void**)&c->gpart, d * 64 * kMaxAug * kMaxAug);
-    alloc((void**)&c->counter, 64);
+    alloc((void**)&c->gpart, d * 256 * kMaxAug * kMaxAug);
     alloc((void**)&c->flags, 64);
     alloc((void**)&c->small, sizeof(SmallState));
     alloc((void**)&c->svd_work, d * kMaxAug * kMaxAug);

@@ -20,6 +20,7 @@ struct SmallState {
   double Vsh[kMaxAug * kMaxAug];       // its right singular vectors
   double sigma[kMaxAug];
   double Ctmp[kMaxAug * kMaxAug];      // scratch reduction result (EPI_STORE)
+  double Cred[kMaxAug * kMaxAug];      // the reduced C handed from k_reduce to k_epilogue
   int r_new;
   int ndropped;
 };
@@ -44,7 +45,6 @@ struct EpiArgs {
   int transpose = 0;
   int breakdown_is_error = 1;
   SmallState* small = nullptr;
-  unsigned* counter = nullptr;  // zero on entry, reset to zero by the last workgroup
   int* flags = nullptr;
   double* S = nullptr;          // state S (EPI_SFINAL / EPI_BUG_* / EPI_RA_N)
   int ld_s = 0;
@@ -77,9 +77,9 @@ struct RowArgs {
   int64_t ldo2 = 0;
   double* pk_f64 = nullptr;   // row-major [i][pk_cols]
   float* pk_f32 = nullptr;    // row-major [i][pk_cols]
-  uint16_t* pk_hi = nullptr;  // column-major [c][ld_pk]
+  uint16_t* pk_hi = nullptr;  // fragment order of the BF16 passes (see pk_offset)
   uint16_t* pk_lo = nullptr;
-  int64_t ld_pk = 0;
+  int pk_layout = 0;          // 0: pass-N order (16-row blocks), 1: pass-T order (64-row blocks)
   int pk_cols = 0;
   // reduction C = out' * W  (W == nullptr: out' * out)
   int reduce = 0;
@@ -88,7 +88,7 @@ struct RowArgs {
   int rw = 0;
   double* gpart = nullptr;    // [grid][r_out * rw]
   EpiArgs epi;
-  int max_blocks = 64;
+  int max_blocks = 256;
   // filled by the launcher
   int tr = 0, ri_pad = 0, ro_pad = 0, epi_doubles = 0;
 };
@@ -120,7 +120,7 @@ struct PassArgs {
   int64_t lda = 0;
   int64_t n = 0, m = 0;
   int rb = 0;                   // output columns (multiple of 16 for F64, 32 otherwise)
-  // B operand: F64 row-major [k][rb]; F32 row-major [k][rb]; BF16X3 column-major hi/lo [c][ldb]
+  // B operand: F64 row-major [k][rb]; F32 row-major [k][rb]; BF16X3 hi/lo panels in fragment order
   const void* B = nullptr;
   const void* B2 = nullptr;
   int64_t ldb = 0;
@@ -137,5 +137,18 @@ hipError_t launch_passN(int mode, const PassArgs& a, hipStream_t st);
 hipError_t launch_passT(int mode, const PassArgs& a, hipStream_t st);
 
 hipError_t selftest_mfma(int which, const void* a, const void* b, void* out);
+
+// Element (row i, column c) of a BF16 hi/lo panel with rb columns, in the
+// order the passes load their B fragments: pass N reads 8 consecutive rows
+// for (16-row block, half h, column c); pass T reads 8 consecutive rows for
+// (64-row block, k-step q, half h, column c).
+__host__ __device__ inline int64_t pk_offset(int layout, int64_t i, int c, int rb) {
+  if (layout == 0) {
+    const int r = (int)(i & 15);
+    return ((((i >> 4) * 2 + (r >> 3)) * rb) + c) * 8 + (r & 7);
+  }
+  const int r = (int)(i & 63);
+  return (((((i >> 6) * 4 + ((r & 31) >> 3)) * 2 + (r >> 5)) * rb) + c) * 8 + (r & 7);
+}
 
 }  // namespace lri

@@ -26,78 +26,96 @@ constexpr double kEps = 2.220446049250313e-16;
 // small dense routines on LDS-resident matrices (column-major, ld = k)
 // ----------------------------------------------------------------------------
 
-// Upper Cholesky G = R'R of the k x k symmetric matrix G (upper triangle read,
-// G is overwritten).  A pivot <= tau = 10 eps max_i G(i,i) marks column p as
-// numerically dependent: dropped[p] = 1, R(p,p) = 1 and row p of R is zero, so
-// that the inverse computed by triu_inv_drop gives a zero column p in Q.
-// Returns the number of dropped columns; sets *nonfinite on NaN/Inf.
-__device__ int chol_drop(double* G, double* R, int k, unsigned char* dropped, int* nonfinite) {
-  __shared__ double s_tau, s_piv;
-  __shared__ int s_nd, s_nf, s_keep;
-  const int t = threadIdx.x;
-  if (t == 0) {
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Upper Cholesky G = R'R of the k x k symmetric matrix G (k <= 64; upper
+// triangle read, G overwritten), computed by wave 0 alone: lane j owns column
+// j of the trailing update, so no workgroup barrier sits on the pivot chain.
+// A pivot <= tau = 10 eps max_i G(i,i) marks column p as numerically
+// dependent: dropped[p] = 1, R(p,p) = 1 and row p of R is zero, so that the
+// inverse from triu_inv_drop gives a zero column p in Q.  Returns the number
+// of dropped columns; sets *nonfinite on NaN/Inf.  Called by all threads.
+__device__ int chol_drop(double* __restrict__ G, double* __restrict__ R, int k, unsigned char* dropped,
+                          int* nonfinite) {
+  __shared__ int s_nd, s_nf;
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
     double md = 0.0;
     int nf = 0;
-    for (int i = 0; i < k; ++i) {
-      double d = G[i + i * k];
+    for (int i = lane; i < k; i += 64) {
+      const double d = G[i + i * k];
       if (!isfinite(d)) nf = 1;
       md = fmax(md, d);
     }
-    s_tau = 10.0 * kEps * md;
-    s_nd = 0;
-    s_nf = nf;
-  }
-  for (int e = t; e < k * k; e += blockDim.x) R[e] = 0.0;
-  __syncthreads();
-  for (int p = 0; p < k; ++p) {
-    if (t == 0) {
-      double d = G[p + p * k];
-      int keep = (d > s_tau) && isfinite(d);
-      if (!isfinite(d)) s_nf = 1;
-      s_keep = keep;
-      s_piv = keep ? sqrt(d) : 1.0;
-      s_nd += keep ? 0 : 1;
-      dropped[p] = keep ? 0 : 1;
-      R[p + p * k] = s_piv;
+    for (int off = 32; off > 0; off >>= 1) {
+      md = fmax(md, __shfl_xor(md, off));
+      nf |= __shfl_xor(nf, off);
     }
-    __syncthreads();
-    const double piv = s_piv;
-    const int keep = s_keep;
-    for (int j = p + 1 + t; j < k; j += blockDim.x) R[p + j * k] = keep ? G[p + j * k] / piv : 0.0;
-    __syncthreads();
-    if (keep) {
-      const int rem = k - p - 1;
-      for (int e = t; e < rem * rem; e += blockDim.x) {
-        int i = p + 1 + e % rem, j = p + 1 + e / rem;
-        if (i <= j) G[i + j * k] -= R[p + i * k] * R[p + j * k];
+    const double tau = 10.0 * kEps * md;
+    for (int e = lane; e < k * k; e += 64) R[e] = 0.0;
+    wave_lds_sync();
+    int nd = 0;
+    const int j = lane;  // the column this lane updates
+    for (int p = 0; p < k; ++p) {
+      const double d = G[p + p * k];
+      const bool keep = (d > tau) && isfinite(d);
+      if (!isfinite(d)) nf = 1;
+      const double piv = keep ? sqrt(d) : 1.0;
+      const double inv = 1.0 / piv;
+      nd += keep ? 0 : 1;
+      double rpj = 0.0;
+      if (j > p && j < k) {
+        rpj = keep ? G[p + j * k] * inv : 0.0;
+        R[p + j * k] = rpj;
       }
+      if (lane == 0) {
+        R[p + p * k] = piv;
+        dropped[p] = keep ? 0 : 1;
+      }
+      wave_lds_sync();
+      if (keep && j > p && j < k) {
+        for (int i = p + 1; i <= j; ++i) G[i + j * k] -= R[p + i * k] * rpj;
+      }
+      wave_lds_sync();
     }
-    __syncthreads();
+    if (lane == 0) {
+      s_nd = nd;
+      s_nf = nf;
+    }
   }
-  if (t == 0 && s_nf) *nonfinite = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_nf) *nonfinite = 1;
   __syncthreads();
   return s_nd;
 }
 
-// Rinv = R^{-1} for upper-triangular R, with dropped columns/rows zeroed.
-__device__ void triu_inv_drop(const double* R, double* Rinv, int k, const unsigned char* dropped) {
-  const int t = threadIdx.x;
-  for (int e = t; e < k * k; e += blockDim.x) Rinv[e] = 0.0;
-  __syncthreads();
-  for (int j = t; j < k; j += blockDim.x) {
-    if (dropped[j]) continue;
-    for (int i = j; i >= 0; --i) {
-      if (dropped[i]) continue;
-      double s = (i == j) ? 1.0 : 0.0;
-      for (int l = i + 1; l <= j; ++l) s -= R[i + l * k] * Rinv[l + j * k];
-      Rinv[i + j * k] = s / R[i + i * k];
+// Rinv = R^{-1} for upper-triangular R (k <= 64), dropped columns/rows zeroed;
+// wave 0, lane j solves column j by back substitution.  Called by all threads.
+__device__ void triu_inv_drop(const double* __restrict__ R, double* __restrict__ Rinv, int k,
+                              const unsigned char* dropped) {
+  if (threadIdx.x < 64) {
+    const int j = threadIdx.x;
+    for (int e = j; e < k * k; e += 64) Rinv[e] = 0.0;
+    wave_lds_sync();
+    if (j < k && !dropped[j]) {
+      for (int i = j; i >= 0; --i) {
+        if (dropped[i]) continue;
+        double s = (i == j) ? 1.0 : 0.0;
+        for (int l = i + 1; l <= j; ++l) s -= R[i + l * k] * Rinv[l + j * k];
+        Rinv[i + j * k] = s / R[i + i * k];
+      }
     }
   }
   __syncthreads();
 }
 
 // C = op(A) * op(B), k x k, column-major.
-__device__ void small_gemm(const double* A, int tA, const double* B, int tB, double* C, int k) {
+__device__ void small_gemm(const double* __restrict__ A, int tA, const double* __restrict__ B, int tB,
+                           double* __restrict__ C, int k) {
   for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
     int i = e % k, j = e / k;
     double s = 0.0;
@@ -115,7 +133,7 @@ __host__ __device__ inline int epi_scratch_doubles(int kind, int a, int b) {
     case EPI_BUG_N:
       return 2 * k * k;
     case EPI_RA_N:
-      return 4 * k * k;
+      return 2 * k * k;
     default:
       return 0;
   }
@@ -144,9 +162,13 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
           st->R1inv[e] = Rinv[e];
         }
       } else {
-        for (int e = t; e < k * k; e += blockDim.x) st->R2inv[e] = Rinv[e];
-        // Rtot = R2 * R1 (R1 was written by an earlier launch on this stream)
-        small_gemm(R, 0, st->R1, 0, st->Rtot, k);
+        // Rtot = R2 * R1 (R1 was written by an earlier launch on this stream); C is free now
+        for (int e = t; e < k * k; e += blockDim.x) {
+          st->R2inv[e] = Rinv[e];
+          C[e] = st->R1[e];
+        }
+        __syncthreads();
+        small_gemm(R, 0, C, 0, st->Rtot, k);
         if (E.S) {
           __syncthreads();
           for (int e = t; e < k * k; e += blockDim.x) {
@@ -185,17 +207,20 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
       break;
     }
     case EPI_BUG_N: {
+      // N = C; Sinit = M (S0 N') with S0, then M, staged in the first scratch matrix
       const int k = a;
-      double* S0 = scr;
+      double* W = scr;
       double* T = scr + k * k;
       for (int e = t; e < k * k; e += blockDim.x) {
         st->N[e] = C[e];
-        S0[e] = E.S[(e % k) + (e / k) * E.ld_s];
+        W[e] = E.S[(e % k) + (e / k) * E.ld_s];
       }
       __syncthreads();
-      small_gemm(st->M, 0, S0, 0, T, k);
+      small_gemm(W, 0, C, 1, T, k);  // T = S0 N'
       __syncthreads();
-      small_gemm(T, 0, C, 1, st->Sinit, k);
+      for (int e = t; e < k * k; e += blockDim.x) W[e] = st->M[e];
+      __syncthreads();
+      small_gemm(W, 0, T, 0, st->Sinit, k);
       break;
     }
     case EPI_BUG_S: {
@@ -349,41 +374,52 @@ __device__ void red_flush(const RedPlan& p, int a, int b, double (&acc)[PER][16]
   }
 }
 
-// The last-arriving workgroup sums the partials (workgroup order) into C and
-// runs the epilogue; it then re-arms the counter for the next launch.
-__device__ void finish_reduction(const EpiArgs& E, int a, int b, const double* gpart_all, double* C,
-                                 double* scr) {
-  __shared__ int s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(E.counter, 1u);
-    s_last = (prev == gridDim.x - 1);
+// Partial sums of the per-workgroup reductions: workgroup b of k_reduce sums
+// entries [64b, 64b + 64) of the nb partials (each a*b doubles) into C; thread
+// (entry e, quarter g) adds partials g, g+4, ... with 16 loads in flight, and the
+// four quarters are combined in a fixed order (deterministic).
+__global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ gpart, int nb, int ab,
+                                                     double* __restrict__ C) {
+  __shared__ double q4[4][64];
+  const int el = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + el;
+  double s = 0.0;
+  if (e < ab) {
+    int k = g;
+    for (; k + 4 * 15 < nb; k += 4 * 16) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = gpart[(int64_t)(k + 4 * u) * ab + e];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) s += v[u];
+    }
+    for (; k < nb; k += 4) s += gpart[(int64_t)k * ab + e];
   }
+  q4[g][el] = s;
   __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  const int nb = gridDim.x;
-  for (int e = threadIdx.x; e < a * b; e += kThreads) {
-    double s = 0.0;
-    for (int blk = 0; blk < nb; ++blk) s += __builtin_nontemporal_load(&gpart_all[(int64_t)blk * a * b + e]);
-    C[e] = s;
-  }
+  if (g == 0 && e < ab) C[e] = ((q4[0][el] + q4[1][el]) + q4[2][el]) + q4[3][el];
+}
+
+// One workgroup: C (a x b, from k_reduce) into LDS, then the epilogue.
+__global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* __restrict__ Cg, int a, int b) {
+  extern __shared__ double lds[];
+  double* C = lds;
+  double* scr = lds + a * b;
+  for (int e = threadIdx.x; e < a * b; e += kThreads) C[e] = Cg[e];
   __syncthreads();
   run_epilogue(E, C, a, b, scr);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    atomicExch(E.counter, 0u);
-  }
 }
 
 // ----------------------------------------------------------------------------
 // the row kernel
 //   out[i, c] = (parts ? sum_s part[s][c][i] : 0) + base_sign * base[i, c]
 //             + xy_sign * sum_d X[i, d] * op(Y)(d, c)
-// LDS: Y (r_in x r_out) | tile_out (tr x a4) | tile_w (tr x b4) | group partials,
-//      and after the tile loop, C (a x b) + epilogue scratch in the tile area.
+// Thread t of a tile handles row t % rpb and the 8 columns 8*(t / rpb) + 0..7
+// (tpr = 256 / rpb threads per row), so every wave reads and writes 64
+// consecutive rows of one column.  Y sits in LDS as [c][d] (ld r_in + 1) and is
+// read as a broadcast; the X row is loaded 16 values at a time.
+// LDS: Ys | tile_out (rpb x a4) | tile_w (rpb x b4) | group partials, and after
+// the tile loop, C (a x b) + epilogue scratch reuse everything past Ys.
 // ----------------------------------------------------------------------------
 
 template <typename PT, int PER>
@@ -392,17 +428,20 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
   const int t = threadIdx.x;
   const int ro = A.r_out, ri = A.r_in;
   const int rw = A.W ? A.rw : ro;
+  const int rpb = A.tr;
+  const int lrow = t % rpb, cg = t / rpb, tpr = kThreads / rpb;
+  const int ldy = ri + 1;
   double* Ys = lds;
-  double* area = Ys + A.ri_pad * A.ro_pad;
+  double* area = Ys + (A.X ? ro * ldy : 0);
   const RedPlan p = red_plan(ro, rw);
   double* tout = area;
-  double* tw = tout + A.tr * p.a4;
-  double* red = tw + (A.W ? A.tr * p.b4 : 0);
+  double* tw = tout + rpb * p.a4;
+  double* red = tw + (A.W ? rpb * p.b4 : 0);
 
   if (A.X) {
     for (int e = t; e < ri * ro; e += kThreads) {
-      int d = e % ri, c = e / ri;
-      Ys[d + c * ri] = A.transY ? A.Y[c + d * A.ldy] : A.Y[d + c * A.ldy];
+      const int d = e % ri, c = e / ri;
+      Ys[c * ldy + d] = A.transY ? A.Y[c + d * A.ldy] : A.Y[d + c * A.ldy];
     }
   }
   __syncthreads();
@@ -413,56 +452,101 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[q][u] = 0.0;
 
-  const int64_t ntiles = (A.rows + A.tr - 1) / A.tr;
+  const int64_t ntiles = (A.rows + rpb - 1) / rpb;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t row0 = tile * A.tr;
-    const int nrows = (int)min((int64_t)A.tr, A.rows - row0);
-    if (t < A.tr) {
-      const int64_t i = row0 + t;
-      const bool ok = t < nrows;
-      for (int c = 0; c < ro; ++c) {
-        double v = 0.0;
-        if (ok) {
-          if (A.parts) {
-            for (int s = 0; s < A.nsplit; ++s) v += (double)A.parts[((int64_t)s * A.rb + c) * A.rows + i];
-            if (A.sum_out) A.sum_out[i + c * A.ld_sum] = v;
+    const int64_t row0 = tile * rpb;
+    const int nrows = (int)min((int64_t)rpb, A.rows - row0);
+    const int64_t i = row0 + lrow;
+    const bool ok = lrow < nrows;
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = 0.0;
+    if (ok) {
+      if (A.parts) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int c = 8 * cg + q;
+          if (c < ro) {
+            double sum = 0.0;
+            int sp = 0;
+            for (; sp + 4 <= A.nsplit; sp += 4) {
+              double w[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) w[u] = (double)A.parts[((int64_t)(sp + u) * A.rb + c) * A.rows + i];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) sum += w[u];
+            }
+            for (; sp < A.nsplit; ++sp) sum += (double)A.parts[((int64_t)sp * A.rb + c) * A.rows + i];
+            v[q] = sum;
+            if (A.sum_out) A.sum_out[i + c * A.ld_sum] = sum;
           }
-          if (A.base) v += A.base_sign * A.base[i + c * A.ld_base];
-          if (A.X) {
-            double s = 0.0;
-            for (int d = 0; d < ri; ++d) s += A.X[i + d * A.ldx] * Ys[d + c * ri];
-            v += A.xy_sign * s;
+        }
+      }
+      if (A.base) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int c = 8 * cg + q;
+          if (c < ro) v[q] += A.base_sign * A.base[i + c * A.ld_base];
+        }
+      }
+      if (A.X) {
+        double xy[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) xy[q] = 0.0;
+        for (int d0 = 0; d0 < ri; d0 += 32) {
+          double x[32];
+#pragma unroll
+          for (int e = 0; e < 32; ++e) x[e] = (d0 + e < ri) ? A.X[i + (d0 + e) * A.ldx] : 0.0;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int c = min(8 * cg + q, ro - 1);
+            const double* yc = Ys + c * ldy + d0;
+            double sacc = 0.0;
+#pragma unroll
+            for (int e = 0; e < 32; ++e)
+              if (d0 + e < ri) sacc += x[e] * yc[e];
+            xy[q] += sacc;
           }
-          if (A.out) A.out[i + c * A.ldo] = v;
-          if (A.out2) A.out2[i + c * A.ldo2] = v;
-          if (A.pk_f64) A.pk_f64[i * A.pk_cols + c] = v;
-          if (A.pk_f32) A.pk_f32[i * A.pk_cols + c] = (float)v;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] += A.xy_sign * xy[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int c = 8 * cg + q;
+        if (c < ro) {
+          if (A.out) A.out[i + c * A.ldo] = v[q];
+          if (A.out2) A.out2[i + c * A.ldo2] = v[q];
+          if (A.pk_f64) A.pk_f64[i * A.pk_cols + c] = v[q];
+          if (A.pk_f32) A.pk_f32[i * A.pk_cols + c] = (float)v[q];
           if (A.pk_hi) {
             uint32_t hi, lo;
-            split_bf16((float)v, hi, lo);
-            A.pk_hi[c * A.ld_pk + i] = (uint16_t)hi;
-            A.pk_lo[c * A.ld_pk + i] = (uint16_t)lo;
-          }
-        }
-        if (A.reduce) tout[t * p.a4 + c] = ok ? v : 0.0;
-      }
-      if (ok) {  // zero the packed columns beyond the current rank
-        for (int c = ro; c < A.pk_cols; ++c) {
-          if (A.pk_f64) A.pk_f64[i * A.pk_cols + c] = 0.0;
-          if (A.pk_f32) A.pk_f32[i * A.pk_cols + c] = 0.0f;
-          if (A.pk_hi) {
-            A.pk_hi[c * A.ld_pk + i] = 0;
-            A.pk_lo[c * A.ld_pk + i] = 0;
+            split_bf16((float)v[q], hi, lo);
+            const int64_t off = pk_offset(A.pk_layout, i, c, A.pk_cols);
+            A.pk_hi[off] = (uint16_t)hi;
+            A.pk_lo[off] = (uint16_t)lo;
           }
         }
       }
-      if (A.reduce) {
-        for (int c = ro; c < p.a4; ++c) tout[t * p.a4 + c] = 0.0;
-        if (A.W)
-          for (int c = 0; c < p.b4; ++c) tw[t * p.b4 + c] = (ok && c < rw) ? A.W[i + c * A.ldw] : 0.0;
+      // zero the packed columns beyond the current rank
+      for (int c = ro + cg; c < A.pk_cols; c += tpr) {
+        if (A.pk_f64) A.pk_f64[i * A.pk_cols + c] = 0.0;
+        if (A.pk_f32) A.pk_f32[i * A.pk_cols + c] = 0.0f;
+        if (A.pk_hi) {
+          const int64_t off = pk_offset(A.pk_layout, i, c, A.pk_cols);
+          A.pk_hi[off] = 0;
+          A.pk_lo[off] = 0;
+        }
       }
     }
     if (A.reduce) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int c = 8 * cg + q;
+        if (c < p.a4) tout[lrow * p.a4 + c] = (ok && c < ro) ? v[q] : 0.0;
+      }
+      if (A.W)
+        for (int c = cg; c < p.b4; c += tpr) tw[lrow * p.b4 + c] = (ok && c < rw) ? A.W[i + c * A.ldw] : 0.0;
       __syncthreads();
       red_accum<PER>(p, tout, A.W ? tw : tout, nrows, acc);
       __syncthreads();
@@ -471,27 +555,25 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
   if (!A.reduce) return;
   double* gpart = A.gpart + (int64_t)blockIdx.x * ro * rw;
   red_flush<PER>(p, ro, rw, acc, red, gpart);
-  __syncthreads();
-  double* C = area;
-  double* scr = area + ro * rw;
-  finish_reduction(A.epi, ro, rw, A.gpart, C, scr);
 }
 
 template <typename PT>
 hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
   const int rw = A.W ? A.rw : A.r_out;
   const RedPlan p = red_plan(A.r_out, rw);
-  const int width = p.a4 + (A.W ? p.b4 : 0);
-  int tr = 256;
-  while (tr > 16 && (size_t)tr * width * 8 > 65536) tr /= 2;
-  A.tr = tr;
-  A.ri_pad = A.X ? A.r_in : 0;
-  A.ro_pad = A.X ? A.r_out : 0;
-  size_t body = (size_t)tr * width + (p.groups > 1 ? (size_t)p.groups * p.a4 * p.b4 : 0);
-  size_t epi = A.reduce ? (size_t)A.r_out * rw + epi_scratch_doubles(A.epi.kind, A.r_out, rw) : 0;
-  size_t lds = sizeof(double) * ((size_t)A.ri_pad * A.ro_pad + (body > epi ? body : epi));
-  if (lds > 150 * 1024) return hipErrorInvalidValue;  // leaves room for the static shared variables
-  const int64_t ntiles = (A.rows + tr - 1) / tr;
+  // threads per row: a power of two covering r_out in groups of 8
+  int tpr = 1;
+  while (8 * tpr < A.r_out) tpr *= 2;
+  if (tpr > 16) return hipErrorInvalidValue;
+  const int rpb = kThreads / tpr;
+  A.tr = rpb;
+  if (p.a4 > 8 * tpr) return hipErrorInvalidValue;
+  const size_t ys = A.X ? (size_t)A.r_out * (A.r_in + 1) : 0;
+  const size_t body = (size_t)rpb * (p.a4 + (A.W ? p.b4 : 0)) + (p.groups > 1 ? (size_t)p.groups * p.a4 * p.b4 : 0);
+  const size_t lds = sizeof(double) * (ys + body);
+  const size_t epi_lds = sizeof(double) * ((size_t)A.r_out * rw + epi_scratch_doubles(A.epi.kind, A.r_out, rw));
+  if (lds > 150 * 1024 || epi_lds > 150 * 1024) return hipErrorInvalidValue;  // room for static shared variables
+  const int64_t ntiles = (A.rows + rpb - 1) / rpb;
   int grid = (int)(ntiles < A.max_blocks ? ntiles : A.max_blocks);
   if (grid < 1) grid = 1;
   if (p.per <= 1) {
@@ -501,6 +583,15 @@ hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
   } else {
     return hipErrorInvalidValue;
   }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !A.reduce) return e;
+  const int ab = A.r_out * rw;
+  hipLaunchKernelGGL(k_reduce, dim3((ab + 63) / 64), dim3(kThreads), 0, st, (const double*)A.gpart, grid, ab,
+                     A.epi.small->Cred);
+  e = hipGetLastError();
+  if (e != hipSuccess || A.epi.kind == EPI_NONE) return e;
+  hipLaunchKernelGGL(k_epilogue, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)A.epi.small->Cred,
+                     A.r_out, rw);
   return hipGetLastError();
 }
 

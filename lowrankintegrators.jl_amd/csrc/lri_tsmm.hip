@@ -44,45 +44,123 @@ __device__ __forceinline__ f32x16_t mfma_f32(float a, float b, f32x16_t c) {
 __device__ __forceinline__ bf16x8_t as_bf16x8(u32x4_t v) { return __builtin_bit_cast(bf16x8_t, v); }
 
 // ============================================================================
+// Work decomposition shared by all passes
+//
+// grid.x = output tiles (rows of dA for pass N, columns for pass T),
+// grid.y = splits of the reduction range.  The 4 waves of a workgroup share
+// the output tile; wave w streams the w-th contiguous quarter of the split's
+// range, prefetching the next iteration's operands while the current one is
+// multiplied.  The four accumulators are summed through LDS in a fixed order
+// (deterministic) and wave 0 writes the split's partial.
+// ============================================================================
+
+struct WaveRange {
+  int64_t b, e;
+};
+
+// [lo, hi) cut into 4 contiguous pieces whose lengths are multiples of q (the last one is shorter).
+__device__ __forceinline__ WaveRange wave_range(int64_t lo, int64_t hi, int wave, int64_t q) {
+  const int64_t per = ((hi - lo + 3) / 4 + q - 1) / q * q;
+  const int64_t b = min(hi, lo + wave * per);
+  return {b, min(hi, b + per)};
+}
+
+// Sum the accumulators of waves 1..3 into wave 0 (LDS red: 3 * NV * 64 values).
+template <typename T, int NV>
+__device__ __forceinline__ void wg_sum(T (&acc)[NV], T* red, int wave, int lane) {
+  __syncthreads();
+  if (wave > 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) red[((wave - 1) * NV + v) * 64 + lane] = acc[v];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      acc[v] = ((acc[v] + red[v * 64 + lane]) + red[(NV + v) * 64 + lane]) + red[(2 * NV + v) * 64 + lane];
+  }
+}
+
+// Order LDS stores of one wave before its later loads (and vice versa).
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ============================================================================
 // FP64 passes (v_mfma_f64_16x16x4_f64); snapshots of any type decoded exactly
 // ============================================================================
 
-// pass N: wave tile = 16*TM rows x all rb columns, k-steps of 4 columns.
+template <typename T>
+__device__ __forceinline__ double ldiff(const T* __restrict__ an, const T* __restrict__ ap, int64_t off) {
+  return snap_f64(an, off) - snap_f64(ap, off);
+}
+
+// pass N: tile = 16*TM rows; an iteration covers 16 columns (4 k-steps of 4).
 template <int TM, int CB, typename T>
 __global__ __launch_bounds__(kPassThreads) void k_passN_f64(const T* __restrict__ an, const T* __restrict__ ap,
                                                             int64_t lda, int64_t n, int64_t m,
                                                             const double* __restrict__ B, int rb, int64_t jchunk,
                                                             double* __restrict__ part) {
+  constexpr int NV = TM * CB * 4;
+  __shared__ double red[3 * NV * 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int64_t i0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * TM);
-  if (i0 >= n) return;
+  const int64_t i0 = (int64_t)blockIdx.x * (16 * TM);
   const int s = blockIdx.y;
   const int64_t jb = (int64_t)s * jchunk;
   const int64_t je = min(m, jb + jchunk);
+  const WaveRange R = wave_range(jb, je, wave, 16);
   f64x4_t acc[TM][CB];
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c) acc[t][c] = f64x4_t{0.0, 0.0, 0.0, 0.0};
-  for (int64_t j = jb; j < je; j += 4) {
-    const int64_t jj = j + lk;
-    const bool jok = jj < je;
-    double b[CB];
+  double a[4][TM], b[4][CB], na[4][TM], nb[4][CB];
+  auto load = [&](int64_t j, double (&xa)[4][TM], double (&xb)[4][CB]) {
 #pragma unroll
-    for (int c = 0; c < CB; ++c) b[c] = jok ? B[jj * rb + c * 16 + lr] : 0.0;
+    for (int u = 0; u < 4; ++u) {
+      const int64_t jj = j + 4 * u + lk;
+      const bool jok = jj < je;
 #pragma unroll
-    for (int t = 0; t < TM; ++t) {
-      const int64_t ii = i0 + 16 * t + lr;
-      double a = 0.0;
-      if (jok && ii < n) {
-        const int64_t off = jj * lda + ii;
-        a = snap_f64(an, off) - snap_f64(ap, off);
+      for (int c = 0; c < CB; ++c) xb[u][c] = jok ? B[jj * rb + c * 16 + lr] : 0.0;
+#pragma unroll
+      for (int t = 0; t < TM; ++t) {
+        const int64_t ii = i0 + 16 * t + lr;
+        xa[u][t] = (jok && ii < n) ? ldiff(an, ap, jj * lda + ii) : 0.0;
       }
+    }
+  };
+  if (R.b < R.e) load(R.b, a, b);
+  for (int64_t j = R.b; j < R.e; j += 16) {
+    const bool more = j + 16 < R.e;
+    if (more) load(j + 16, na, nb);
 #pragma unroll
-      for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f64(a, b[c], acc[t][c]);
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int t = 0; t < TM; ++t)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f64(a[u][t], b[u][c], acc[t][c]);
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int t = 0; t < TM; ++t) a[u][t] = na[u][t];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) b[u][c] = nb[u][c];
+      }
     }
   }
+  double flat[NV];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) flat[(t * CB + c) * 4 + v] = acc[t][c][v];
+  wg_sum<double, NV>(flat, red, wave, lane);
+  if (wave != 0) return;
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
@@ -90,57 +168,79 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_f64(const T* __restrict_
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t ii = i0 + 16 * t + 4 * v + lk;
-        if (ii < n) part[((int64_t)s * rb + c * 16 + lr) * n + ii] = acc[t][c][v];
+        if (ii < n) part[((int64_t)s * rb + c * 16 + lr) * n + ii] = flat[(t * CB + c) * 4 + v];
       }
 }
 
-// pass T: wave tile = 16*TJ columns of dA x all rb columns; each lane owns 4
-// consecutive rows of its column per iteration (16 rows per iteration).
+// pass T: tile = 16*TJ columns of dA; an iteration covers 16 rows, each lane
+// owning 4 consecutive rows of its column.
 template <int TJ, int CB, typename T>
 __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict__ an, const T* __restrict__ ap,
                                                             int64_t lda, int64_t n, int64_t m,
                                                             const double* __restrict__ B, int rb, int64_t ichunk,
                                                             double* __restrict__ part) {
+  constexpr int NV = TJ * CB * 4;
+  __shared__ double red[3 * NV * 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int64_t j0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * TJ);
-  if (j0 >= m) return;
+  const int64_t j0 = (int64_t)blockIdx.x * (16 * TJ);
   const int s = blockIdx.y;
   const int64_t ib = (int64_t)s * ichunk;
   const int64_t ie = min(n, ib + ichunk);
+  const WaveRange R = wave_range(ib, ie, wave, 16);
   f64x4_t acc[TJ][CB];
 #pragma unroll
   for (int t = 0; t < TJ; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c) acc[t][c] = f64x4_t{0.0, 0.0, 0.0, 0.0};
-  for (int64_t i = ib; i < ie; i += 16) {
-    double a[TJ][4];
+  double a[TJ][4], b[4][CB], na[TJ][4], nb[4][CB];
+  auto load = [&](int64_t i, double (&xa)[TJ][4], double (&xb)[4][CB]) {
 #pragma unroll
     for (int t = 0; t < TJ; ++t) {
       const int64_t jj = j0 + 16 * t + lr;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int64_t ii = i + 4 * lk + e;
-        double v = 0.0;
-        if (jj < m && ii < ie) {
-          const int64_t off = jj * lda + ii;
-          v = snap_f64(an, off) - snap_f64(ap, off);
-        }
-        a[t][e] = v;
+        xa[t][e] = (jj < m && ii < ie) ? ldiff(an, ap, jj * lda + ii) : 0.0;
       }
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int64_t ii = i + 4 * lk + e;
-      double b[CB];
 #pragma unroll
-      for (int c = 0; c < CB; ++c) b[c] = ii < ie ? B[ii * rb + c * 16 + lr] : 0.0;
+      for (int c = 0; c < CB; ++c) xb[e][c] = ii < ie ? B[ii * rb + c * 16 + lr] : 0.0;
+    }
+  };
+  if (R.b < R.e) load(R.b, a, b);
+  for (int64_t i = R.b; i < R.e; i += 16) {
+    const bool more = i + 16 < R.e;
+    if (more) load(i + 16, na, nb);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
 #pragma unroll
       for (int t = 0; t < TJ; ++t)
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f64(a[t][e], b[c], acc[t][c]);
+        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f64(a[t][e], b[e][c], acc[t][c]);
+    if (more) {
+#pragma unroll
+      for (int t = 0; t < TJ; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[t][e] = na[t][e];
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) b[e][c] = nb[e][c];
     }
   }
+  double flat[NV];
+#pragma unroll
+  for (int t = 0; t < TJ; ++t)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) flat[(t * CB + c) * 4 + v] = acc[t][c][v];
+  wg_sum<double, NV>(flat, red, wave, lane);
+  if (wave != 0) return;
 #pragma unroll
   for (int t = 0; t < TJ; ++t)
 #pragma unroll
@@ -148,57 +248,78 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict_
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t jj = j0 + 16 * t + 4 * v + lk;
-        if (jj < m) part[((int64_t)s * rb + c * 16 + lr) * m + jj] = acc[t][c][v];
+        if (jj < m) part[((int64_t)s * rb + c * 16 + lr) * m + jj] = flat[(t * CB + c) * 4 + v];
       }
 }
 
 // ============================================================================
-// FP32 passes (v_mfma_f32_32x32x2_f32) for F32 (or BF16) snapshots
+// FP32 passes (v_mfma_f32_32x32x2_f32) for F32 snapshots
 // ============================================================================
 
-template <typename T>
-__device__ __forceinline__ float snap_f32(const T* p, int64_t k);
-template <>
-__device__ __forceinline__ float snap_f32<float>(const float* p, int64_t k) { return p[k]; }
-template <>
-__device__ __forceinline__ float snap_f32<uint16_t>(const uint16_t* p, int64_t k) { return bf16_to_f32(p[k]); }
-
-// pass N: wave tile = 32*TM rows; k-steps of 2 columns.
-template <int TM, int CB, typename T>
-__global__ __launch_bounds__(kPassThreads) void k_passN_f32(const T* __restrict__ an, const T* __restrict__ ap,
-                                                            int64_t lda, int64_t n, int64_t m,
-                                                            const float* __restrict__ B, int rb, int64_t jchunk,
-                                                            float* __restrict__ part) {
+// pass N: tile = 32*TM rows; an iteration covers 16 columns (8 k-steps of 2).
+template <int TM, int CB>
+__global__ __launch_bounds__(kPassThreads) void k_passN_f32(const float* __restrict__ an,
+                                                            const float* __restrict__ ap, int64_t lda, int64_t n,
+                                                            int64_t m, const float* __restrict__ B, int rb,
+                                                            int64_t jchunk, float* __restrict__ part) {
+  constexpr int NV = TM * CB * 16;
+  __shared__ float red[3 * NV * 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 31, lk = lane >> 5;
-  const int64_t i0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * TM);
-  if (i0 >= n) return;
+  const int64_t i0 = (int64_t)blockIdx.x * (32 * TM);
   const int s = blockIdx.y;
   const int64_t jb = (int64_t)s * jchunk;
   const int64_t je = min(m, jb + jchunk);
+  const WaveRange R = wave_range(jb, je, wave, 16);
   f32x16_t acc[TM][CB];
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c) acc[t][c] = f32x16_t{};
-  for (int64_t j = jb; j < je; j += 2) {
-    const int64_t jj = j + lk;
-    const bool jok = jj < je;
-    float b[CB];
+  float a[8][TM], b[8][CB], na[8][TM], nb[8][CB];
+  auto load = [&](int64_t j, float (&xa)[8][TM], float (&xb)[8][CB]) {
 #pragma unroll
-    for (int c = 0; c < CB; ++c) b[c] = jok ? B[jj * rb + c * 32 + lr] : 0.0f;
+    for (int u = 0; u < 8; ++u) {
+      const int64_t jj = j + 2 * u + lk;
+      const bool jok = jj < je;
 #pragma unroll
-    for (int t = 0; t < TM; ++t) {
-      const int64_t ii = i0 + 32 * t + lr;
-      float a = 0.0f;
-      if (jok && ii < n) {
-        const int64_t off = jj * lda + ii;
-        a = snap_f32(an, off) - snap_f32(ap, off);
+      for (int c = 0; c < CB; ++c) xb[u][c] = jok ? B[jj * rb + c * 32 + lr] : 0.0f;
+#pragma unroll
+      for (int t = 0; t < TM; ++t) {
+        const int64_t ii = i0 + 32 * t + lr;
+        xa[u][t] = (jok && ii < n) ? an[jj * lda + ii] - ap[jj * lda + ii] : 0.0f;
       }
+    }
+  };
+  if (R.b < R.e) load(R.b, a, b);
+  for (int64_t j = R.b; j < R.e; j += 16) {
+    const bool more = j + 16 < R.e;
+    if (more) load(j + 16, na, nb);
 #pragma unroll
-      for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f32(a, b[c], acc[t][c]);
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int t = 0; t < TM; ++t)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f32(a[u][t], b[u][c], acc[t][c]);
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+#pragma unroll
+        for (int t = 0; t < TM; ++t) a[u][t] = na[u][t];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) b[u][c] = nb[u][c];
+      }
     }
   }
+  float flat[NV];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) flat[(t * CB + c) * 16 + v] = acc[t][c][v];
+  wg_sum<float, NV>(flat, red, wave, lane);
+  if (wave != 0) return;
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
@@ -206,56 +327,79 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_f32(const T* __restrict_
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int64_t ii = i0 + 32 * t + 8 * (v / 4) + 4 * lk + (v % 4);
-        if (ii < n) part[((int64_t)s * rb + c * 32 + lr) * n + ii] = acc[t][c][v];
+        if (ii < n) part[((int64_t)s * rb + c * 32 + lr) * n + ii] = flat[(t * CB + c) * 16 + v];
       }
 }
 
-// pass T: wave tile = 32*TJ columns; each lane owns 4 consecutive rows per iteration (8 rows per iteration).
-template <int TJ, int CB, typename T>
-__global__ __launch_bounds__(kPassThreads) void k_passT_f32(const T* __restrict__ an, const T* __restrict__ ap,
-                                                            int64_t lda, int64_t n, int64_t m,
-                                                            const float* __restrict__ B, int rb, int64_t ichunk,
-                                                            float* __restrict__ part) {
+// pass T: tile = 32*TJ columns; an iteration covers 16 rows (2 groups of 8,
+// each lane owning 4 consecutive rows of its column per group).
+template <int TJ, int CB>
+__global__ __launch_bounds__(kPassThreads) void k_passT_f32(const float* __restrict__ an,
+                                                            const float* __restrict__ ap, int64_t lda, int64_t n,
+                                                            int64_t m, const float* __restrict__ B, int rb,
+                                                            int64_t ichunk, float* __restrict__ part) {
+  constexpr int NV = TJ * CB * 16;
+  __shared__ float red[3 * NV * 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 31, lk = lane >> 5;
-  const int64_t j0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * TJ);
-  if (j0 >= m) return;
+  const int64_t j0 = (int64_t)blockIdx.x * (32 * TJ);
   const int s = blockIdx.y;
   const int64_t ib = (int64_t)s * ichunk;
   const int64_t ie = min(n, ib + ichunk);
+  const WaveRange R = wave_range(ib, ie, wave, 16);
   f32x16_t acc[TJ][CB];
 #pragma unroll
   for (int t = 0; t < TJ; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c) acc[t][c] = f32x16_t{};
-  for (int64_t i = ib; i < ie; i += 8) {
-    float a[TJ][4];
+  float a[TJ][8], b[8][CB], na[TJ][8], nb[8][CB];
+  auto load = [&](int64_t i, float (&xa)[TJ][8], float (&xb)[8][CB]) {
 #pragma unroll
     for (int t = 0; t < TJ; ++t) {
       const int64_t jj = j0 + 32 * t + lr;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t ii = i + 4 * lk + e;
-        float v = 0.0f;
-        if (jj < m && ii < ie) {
-          const int64_t off = jj * lda + ii;
-          v = snap_f32(an, off) - snap_f32(ap, off);
-        }
-        a[t][e] = v;
+      for (int e = 0; e < 8; ++e) {
+        const int64_t ii = i + 8 * (e / 4) + 4 * lk + (e % 4);
+        xa[t][e] = (jj < m && ii < ie) ? an[jj * lda + ii] - ap[jj * lda + ii] : 0.0f;
       }
     }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int64_t ii = i + 4 * lk + e;
-      float b[CB];
+    for (int e = 0; e < 8; ++e) {
+      const int64_t ii = i + 8 * (e / 4) + 4 * lk + (e % 4);
 #pragma unroll
-      for (int c = 0; c < CB; ++c) b[c] = ii < ie ? B[ii * rb + c * 32 + lr] : 0.0f;
+      for (int c = 0; c < CB; ++c) xb[e][c] = ii < ie ? B[ii * rb + c * 32 + lr] : 0.0f;
+    }
+  };
+  if (R.b < R.e) load(R.b, a, b);
+  for (int64_t i = R.b; i < R.e; i += 16) {
+    const bool more = i + 16 < R.e;
+    if (more) load(i + 16, na, nb);
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
 #pragma unroll
       for (int t = 0; t < TJ; ++t)
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f32(a[t][e], b[c], acc[t][c]);
+        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f32(a[t][e], b[e][c], acc[t][c]);
+    if (more) {
+#pragma unroll
+      for (int t = 0; t < TJ; ++t)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[t][e] = na[t][e];
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) b[e][c] = nb[e][c];
     }
   }
+  float flat[NV];
+#pragma unroll
+  for (int t = 0; t < TJ; ++t)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) flat[(t * CB + c) * 16 + v] = acc[t][c][v];
+  wg_sum<float, NV>(flat, red, wave, lane);
+  if (wave != 0) return;
 #pragma unroll
   for (int t = 0; t < TJ; ++t)
 #pragma unroll
@@ -263,113 +407,172 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f32(const T* __restrict_
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int64_t jj = j0 + 32 * t + 8 * (v / 4) + 4 * lk + (v % 4);
-        if (jj < m) part[((int64_t)s * rb + c * 32 + lr) * m + jj] = acc[t][c][v];
+        if (jj < m) part[((int64_t)s * rb + c * 32 + lr) * m + jj] = flat[(t * CB + c) * 16 + v];
       }
 }
 
 // ============================================================================
 // BF16x3 passes (v_mfma_f32_32x32x16_bf16), BF16 snapshots.
 // dA = A_next - A_prev is formed exactly in FP32, split dA = a_hi + a_lo into
-// two BF16 values, the B panel is stored as b_hi + b_lo, and each product is
-// a_hi*b_hi + a_hi*b_lo + a_lo*b_hi (the a_lo*b_lo term is below 2^-16
-// relative), accumulated in FP32.
+// two BF16 values (v_cvt_pk_bf16_f32, round to nearest even); the B panel is
+// stored as b_hi + b_lo, and each product is a_hi*b_hi + a_hi*b_lo + a_lo*b_hi
+// (the a_lo*b_lo term is below 2^-16 relative), accumulated in FP32.
 // ============================================================================
 
-// Load 8 consecutive bf16 of a column (rows r0..r0+7, masked against rend)
-// from both snapshots and return the FP32 differences.
-__device__ __forceinline__ void load8_diff(const uint16_t* __restrict__ an, const uint16_t* __restrict__ ap,
-                                           int64_t off, int64_t r0, int64_t rend, bool colok, bool vec,
-                                           float (&d)[8]) {
-  if (colok && vec && r0 + 8 <= rend) {
-    const u32x4_t x = *reinterpret_cast<const u32x4_t*>(an + off);
-    const u32x4_t y = *reinterpret_cast<const u32x4_t*>(ap + off);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      d[2 * q] = __uint_as_float(x[q] << 16) - __uint_as_float(y[q] << 16);
-      d[2 * q + 1] = __uint_as_float(x[q] & 0xFFFF0000u) - __uint_as_float(y[q] & 0xFFFF0000u);
-    }
-  } else {
+typedef __attribute__((ext_vector_type(2))) float f32x2_t;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2_t;
+
+// 8 consecutive bf16 of a column starting at off (rows r0..r0+7 masked against rend).
+__device__ __forceinline__ u32x4_t load_raw8(const uint16_t* __restrict__ p, int64_t off, int64_t r0, int64_t rend,
+                                             bool colok, bool vec) {
+  if (colok && vec && r0 + 8 <= rend) return __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p + off));
+  u32x4_t v = {0u, 0u, 0u, 0u};
+  if (colok) {
 #pragma unroll
     for (int e = 0; e < 8; ++e)
-      d[e] = (colok && r0 + e < rend) ? bf16_to_f32(an[off + e]) - bf16_to_f32(ap[off + e]) : 0.0f;
+      if (r0 + e < rend) v[e / 2] |= uint32_t(p[off + e]) << (16 * (e % 2));
   }
+  return v;
 }
 
-__device__ __forceinline__ void split8(const float (&d)[8], u32x4_t& hi, u32x4_t& lo) {
+// hi/lo bf16 split of the FP32 differences of two packed bf16x8 vectors.
+__device__ __forceinline__ void diff_split8(u32x4_t x, u32x4_t y, u32x4_t& hi, u32x4_t& lo) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    uint32_t h0, l0, h1, l1;
-    split_bf16(d[2 * q], h0, l0);
-    split_bf16(d[2 * q + 1], h1, l1);
-    hi[q] = h0 | (h1 << 16);
-    lo[q] = l0 | (l1 << 16);
+    f32x2_t d;
+    d[0] = __uint_as_float(x[q] << 16) - __uint_as_float(y[q] << 16);
+    d[1] = __uint_as_float(x[q] & 0xFFFF0000u) - __uint_as_float(y[q] & 0xFFFF0000u);
+    const bf16x2_t h = __builtin_convertvector(d, bf16x2_t);
+    const f32x2_t hf = __builtin_convertvector(h, f32x2_t);
+    const f32x2_t r = d - hf;
+    const bf16x2_t l = __builtin_convertvector(r, bf16x2_t);
+    hi[q] = __builtin_bit_cast(uint32_t, h);
+    lo[q] = __builtin_bit_cast(uint32_t, l);
   }
 }
 
-// pass T: wave tile = 32*TJ columns of dA; each lane owns 32 consecutive rows
-// of its column per iteration (64 rows per iteration, 4 k-steps of 16).
-template <int TJ, int CB>
+// pass T: tile = 32 columns of dA per workgroup; an iteration covers 64 rows.
+// Loads are fully coalesced (8 lanes read the 128 contiguous bytes of one
+// column, a wave 8 columns per instruction); the hi/lo differences are staged
+// column-major in the wave's LDS region (column pitch 64 + 8 bf16 keeps the
+// 16-byte fragment reads conflict-free) and read back as lane (column lc,
+// half h): rows 32h + 8q .. 32h + 8q + 7 for k-step q.
+// B panel (U) in fragment order: for rows block b (64 rows), k-step q, half h,
+// column c: 8 consecutive bf16 at ((((b*4 + q)*2 + h)*rb + c)*8).
+constexpr int kTPitch = 72;
+
+template <int CB>
 __global__ __launch_bounds__(kPassThreads) void k_passT_bf16x3(const uint16_t* __restrict__ an,
                                                                const uint16_t* __restrict__ ap, int64_t lda,
                                                                int64_t n, int64_t m, const uint16_t* __restrict__ Bhi,
                                                                const uint16_t* __restrict__ Blo, int64_t ldb,
                                                                int rb, int64_t ichunk, float* __restrict__ part,
                                                                int vec) {
+  constexpr int NV = CB * 16;
+  constexpr int STAGE = 4 * 2 * 32 * kTPitch * 2;  // bytes
+  constexpr int RED = 3 * NV * 64 * 4;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE > RED ? STAGE : RED];
+  (void)ldb;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lc = lane & 31, h = lane >> 5;
-  const int64_t j0 = ((int64_t)blockIdx.x * 4 + wave) * (32 * TJ);
-  if (j0 >= m) return;
+  const int64_t j0 = (int64_t)blockIdx.x * 32;
   const int s = blockIdx.y;
   const int64_t ib = (int64_t)s * ichunk;
   const int64_t ie = min(n, ib + ichunk);
-  f32x16_t acc[TJ][CB];
+  const WaveRange R = wave_range(ib, ie, wave, 64);
+  const bool v = vec != 0;
+  uint16_t* th = reinterpret_cast<uint16_t*>(smem) + (wave * 2 + 0) * 32 * kTPitch;
+  uint16_t* tl = reinterpret_cast<uint16_t*>(smem) + (wave * 2 + 1) * 32 * kTPitch;
+  f32x16_t acc[CB];
 #pragma unroll
-  for (int t = 0; t < TJ; ++t)
+  for (int c = 0; c < CB; ++c) acc[c] = f32x16_t{};
+  // load lane mapping: instruction p covers columns 8p .. 8p+7; lane -> column 8p + lane/8, rows 8*(lane%8)
+  const int lcol = lane >> 3, loct = lane & 7;
+  u32x4_t xn[4], xp[4], nxn[4], nxp[4];
+  auto load = [&](int64_t i, u32x4_t (&A1)[4], u32x4_t (&A0)[4]) {
 #pragma unroll
-    for (int c = 0; c < CB; ++c) acc[t][c] = f32x16_t{};
-  for (int64_t i = ib; i < ie; i += 64) {
+    for (int p = 0; p < 4; ++p) {
+      const int64_t jj = j0 + 8 * p + lcol;
+      const int64_t r0 = i + 8 * loct;
+      A1[p] = load_raw8(an, jj * lda + r0, r0, ie, jj < m, v);
+      A0[p] = load_raw8(ap, jj * lda + r0, r0, ie, jj < m, v);
+    }
+  };
+  if (R.b < R.e) load(R.b, xn, xp);
+  for (int64_t i = R.b; i < R.e; i += 64) {
+    const bool more = i + 64 < R.e;
+    if (more) load(i + 64, nxn, nxp);
+    // B fragments of this 64-row block (L2-resident panel)
+    u32x4_t bh[4][CB], bl[4][CB];
+    const int64_t blk = i >> 6;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t r0 = i + 32 * h + 8 * q;
-      u32x4_t bh[CB], bl[CB];
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
-        const int64_t boff = (int64_t)(c * 32 + lc) * ldb + r0;
-        bh[c] = *reinterpret_cast<const u32x4_t*>(Bhi + boff);
-        bl[c] = *reinterpret_cast<const u32x4_t*>(Blo + boff);
+        const int64_t off = ((((blk * 4 + q) * 2 + h) * rb) + c * 32 + lc) * 8;
+        bh[q][c] = *reinterpret_cast<const u32x4_t*>(Bhi + off);
+        bl[q][c] = *reinterpret_cast<const u32x4_t*>(Blo + off);
       }
 #pragma unroll
-      for (int t = 0; t < TJ; ++t) {
-        const int64_t jj = j0 + 32 * t + lc;
-        float d[8];
-        load8_diff(an, ap, jj * lda + r0, r0, ie, jj < m, vec, d);
-        u32x4_t ah, al;
-        split8(d, ah, al);
+    for (int p = 0; p < 4; ++p) {
+      u32x4_t hv, lv;
+      diff_split8(xn[p], xp[p], hv, lv);
+      const int col = 8 * p + lcol;
+      *reinterpret_cast<u32x4_t*>(th + col * kTPitch + 8 * loct) = hv;
+      *reinterpret_cast<u32x4_t*>(tl + col * kTPitch + 8 * loct) = lv;
+    }
+    wave_lds_fence();
 #pragma unroll
-        for (int c = 0; c < CB; ++c) {
-          acc[t][c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bh[c]), acc[t][c]);
-          acc[t][c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bl[c]), acc[t][c]);
-          acc[t][c] = mfma_bf16(as_bf16x8(al), as_bf16x8(bh[c]), acc[t][c]);
-        }
+    for (int q = 0; q < 4; ++q) {
+      const u32x4_t ah = *reinterpret_cast<const u32x4_t*>(th + lc * kTPitch + 32 * h + 8 * q);
+      const u32x4_t al = *reinterpret_cast<const u32x4_t*>(tl + lc * kTPitch + 32 * h + 8 * q);
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        acc[c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bh[q][c]), acc[c]);
+        acc[c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bl[q][c]), acc[c]);
+        acc[c] = mfma_bf16(as_bf16x8(al), as_bf16x8(bh[q][c]), acc[c]);
+      }
+    }
+    wave_lds_fence();
+    if (more) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        xn[p] = nxn[p];
+        xp[p] = nxp[p];
       }
     }
   }
+  float flat[NV];
 #pragma unroll
-  for (int t = 0; t < TJ; ++t)
+  for (int c = 0; c < CB; ++c)
 #pragma unroll
-    for (int c = 0; c < CB; ++c)
+    for (int w = 0; w < 16; ++w) flat[c * 16 + w] = acc[c][w];
+  wg_sum<float, NV>(flat, reinterpret_cast<float*>(smem), wave, lane);
+  if (wave != 0) return;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t jj = j0 + 32 * t + 8 * (v / 4) + 4 * h + (v % 4);
-        if (jj < m) part[((int64_t)s * rb + c * 32 + lc) * m + jj] = acc[t][c][v];
-      }
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      const int64_t jj = j0 + 8 * (w / 4) + 4 * h + (w % 4);
+      if (jj < m) part[((int64_t)s * rb + c * 32 + lc) * m + jj] = flat[c * 16 + w];
+    }
 }
 
-// pass N: the operand needs 8 consecutive columns per lane, but memory runs
-// along rows, so each wave stages a (32*TM rows x 16 columns) hi/lo tile
-// through its own LDS region: coalesced 16-byte loads along the columns,
-// stored column-major, then read back as 8 column-consecutive bf16 per lane.
-constexpr int kNPad = 4;  // LDS column stride = rows + 4 (bf16) keeps the two lane halves on different banks
+// pass N: the operand needs 8 consecutive columns per lane but memory runs
+// along rows, so each wave stages a (32*TM rows x 16 columns) hi/lo tile in
+// its own LDS region: coalesced 16-byte loads down the columns, stored
+// column-major, read back as 8 column-consecutive bf16 per lane.
+// B panel (V) in fragment order: for the 16-row block b, half h, column c:
+// 8 consecutive bf16 at (((b*2 + h)*rb + c)*8).
+constexpr int kNPad = 4;  // LDS column stride = rows + 4 (bf16): the two lane halves hit different banks
+
+template <int TM, int CB>
+struct PassNLds {
+  static constexpr int LDC = 32 * TM + kNPad;
+  static constexpr int STAGE_BYTES = 4 * 2 * 16 * LDC * 2;
+  static constexpr int RED_BYTES = 3 * TM * CB * 16 * 64 * 4;
+  static constexpr int BYTES = STAGE_BYTES > RED_BYTES ? STAGE_BYTES : RED_BYTES;
+};
 
 template <int TM, int CB>
 __global__ __launch_bounds__(kPassThreads) void k_passN_bf16x3(const uint16_t* __restrict__ an,
@@ -378,83 +581,111 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_bf16x3(const uint16_t* _
                                                                const uint16_t* __restrict__ Blo, int64_t ldb,
                                                                int rb, int64_t jchunk, float* __restrict__ part,
                                                                int vec) {
-  constexpr int ROWS = 32 * TM;
-  constexpr int LDC = ROWS + kNPad;  // column stride in the LDS tile (bf16 elements)
-  __shared__ uint16_t tile[4][2][16 * LDC];
+  using L = PassNLds<TM, CB>;
+  constexpr int LDC = L::LDC;
+  constexpr int NV = TM * CB * 16;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[L::BYTES];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lc = lane & 31, h = lane >> 5;
-  const int64_t i0 = ((int64_t)blockIdx.x * 4 + wave) * ROWS;
-  const bool active = i0 < n;
+  const int64_t i0 = (int64_t)blockIdx.x * (32 * TM);
   const int s = blockIdx.y;
   const int64_t jb = (int64_t)s * jchunk;
   const int64_t je = min(m, jb + jchunk);
-  uint16_t* th = tile[wave][0];
-  uint16_t* tl = tile[wave][1];
+  const WaveRange R = wave_range(jb, je, wave, 16);
+  const bool v = vec != 0;
+  uint16_t* th = reinterpret_cast<uint16_t*>(smem) + (wave * 2 + 0) * 16 * LDC;
+  uint16_t* tl = reinterpret_cast<uint16_t*>(smem) + (wave * 2 + 1) * 16 * LDC;
   f32x16_t acc[TM][CB];
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c) acc[t][c] = f32x16_t{};
-  for (int64_t j = jb; j < je; j += 16) {
-    // stage: TM 8-row chunks per lane; chunk b -> column b / (4*TM), row octet b % (4*TM)
-    if (active) {
+  // staging chunk q of a k-step: b = q*64 + lane -> column b / (4*TM), row octet b % (4*TM)
+  u32x4_t xn[TM], xp[TM], bh[CB], bl[CB], nxn[TM], nxp[TM], nbh[CB], nbl[CB];
+  auto load = [&](int64_t j, u32x4_t (&A1)[TM], u32x4_t (&A0)[TM], u32x4_t (&H)[CB], u32x4_t (&Lo)[CB]) {
 #pragma unroll
-      for (int q = 0; q < TM; ++q) {
-        const int b = q * 64 + lane;
-        const int col = b / (4 * TM), oct = b % (4 * TM);
-        const int64_t jj = j + col;
-        const int64_t r0 = i0 + 8 * oct;
-        float d[8];
-        load8_diff(an, ap, jj * lda + r0, r0, n, jj < je, vec, d);
-        u32x4_t hv, lv;
-        split8(d, hv, lv);
-        uint32_t* ph = reinterpret_cast<uint32_t*>(th + col * LDC + 8 * oct);
-        uint32_t* pl = reinterpret_cast<uint32_t*>(tl + col * LDC + 8 * oct);
-        // LDC*2 bytes is 8-byte aligned: two 8-byte stores per 16-byte chunk
-        reinterpret_cast<uint2*>(ph)[0] = uint2{hv[0], hv[1]};
-        reinterpret_cast<uint2*>(ph)[1] = uint2{hv[2], hv[3]};
-        reinterpret_cast<uint2*>(pl)[0] = uint2{lv[0], lv[1]};
-        reinterpret_cast<uint2*>(pl)[1] = uint2{lv[2], lv[3]};
-      }
+    for (int q = 0; q < TM; ++q) {
+      const int b = q * 64 + lane;
+      const int col = b / (4 * TM), oct = b % (4 * TM);
+      const int64_t jj = j + col;
+      const int64_t r0 = i0 + 8 * oct;
+      A1[q] = load_raw8(an, jj * lda + r0, r0, n, jj < je, v);
+      A0[q] = load_raw8(ap, jj * lda + r0, r0, n, jj < je, v);
     }
-    __syncthreads();
-    if (active) {
-      u32x4_t bh[CB], bl[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const int64_t boff = ((((j >> 4) * 2 + h) * rb) + c * 32 + lc) * 8;
+      H[c] = *reinterpret_cast<const u32x4_t*>(Bhi + boff);
+      Lo[c] = *reinterpret_cast<const u32x4_t*>(Blo + boff);
+    }
+  };
+  if (R.b < R.e) load(R.b, xn, xp, bh, bl);
+  for (int64_t j = R.b; j < R.e; j += 16) {
+    const bool more = j + 16 < R.e;
+    if (more) load(j + 16, nxn, nxp, nbh, nbl);
+#pragma unroll
+    for (int q = 0; q < TM; ++q) {
+      const int b = q * 64 + lane;
+      const int col = b / (4 * TM), oct = b % (4 * TM);
+      u32x4_t hv, lv;
+      diff_split8(xn[q], xp[q], hv, lv);
+      // LDC*2 bytes is 8-byte aligned: two 8-byte stores per 16-byte chunk
+      uint2* ph = reinterpret_cast<uint2*>(th + col * LDC + 8 * oct);
+      uint2* pl = reinterpret_cast<uint2*>(tl + col * LDC + 8 * oct);
+      ph[0] = uint2{hv[0], hv[1]};
+      ph[1] = uint2{hv[2], hv[3]};
+      pl[0] = uint2{lv[0], lv[1]};
+      pl[1] = uint2{lv[2], lv[3]};
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int t = 0; t < TM; ++t) {
+      const int row = 32 * t + lc;
+      u32x4_t ah, al;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c0 = 8 * h + 2 * q;
+        ah[q] = uint32_t(th[c0 * LDC + row]) | (uint32_t(th[(c0 + 1) * LDC + row]) << 16);
+        al[q] = uint32_t(tl[c0 * LDC + row]) | (uint32_t(tl[(c0 + 1) * LDC + row]) << 16);
+      }
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
-        const int64_t boff = (int64_t)(c * 32 + lc) * ldb + j + 8 * h;
-        bh[c] = *reinterpret_cast<const u32x4_t*>(Bhi + boff);
-        bl[c] = *reinterpret_cast<const u32x4_t*>(Blo + boff);
-      }
-#pragma unroll
-      for (int t = 0; t < TM; ++t) {
-        const int row = 32 * t + lc;
-        u32x4_t ah, al;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int c0 = 8 * h + 2 * q;
-          ah[q] = uint32_t(th[c0 * LDC + row]) | (uint32_t(th[(c0 + 1) * LDC + row]) << 16);
-          al[q] = uint32_t(tl[c0 * LDC + row]) | (uint32_t(tl[(c0 + 1) * LDC + row]) << 16);
-        }
-#pragma unroll
-        for (int c = 0; c < CB; ++c) {
-          acc[t][c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bh[c]), acc[t][c]);
-          acc[t][c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bl[c]), acc[t][c]);
-          acc[t][c] = mfma_bf16(as_bf16x8(al), as_bf16x8(bh[c]), acc[t][c]);
-        }
+        acc[t][c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bh[c]), acc[t][c]);
+        acc[t][c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bl[c]), acc[t][c]);
+        acc[t][c] = mfma_bf16(as_bf16x8(al), as_bf16x8(bh[c]), acc[t][c]);
       }
     }
-    __syncthreads();
+    wave_lds_fence();
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < TM; ++q) {
+        xn[q] = nxn[q];
+        xp[q] = nxp[q];
+      }
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        bh[c] = nbh[c];
+        bl[c] = nbl[c];
+      }
+    }
   }
-  if (!active) return;
+  float flat[NV];
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t ii = i0 + 32 * t + 8 * (v / 4) + 4 * h + (v % 4);
-        if (ii < n) part[((int64_t)s * rb + c * 32 + lc) * n + ii] = acc[t][c][v];
+      for (int w = 0; w < 16; ++w) flat[(t * CB + c) * 16 + w] = acc[t][c][w];
+  wg_sum<float, NV>(flat, reinterpret_cast<float*>(smem), wave, lane);
+  if (wave != 0) return;
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int w = 0; w < 16; ++w) {
+        const int64_t ii = i0 + 32 * t + 8 * (w / 4) + 4 * h + (w % 4);
+        if (ii < n) part[((int64_t)s * rb + c * 32 + lc) * n + ii] = flat[(t * CB + c) * 16 + w];
       }
 }
 
@@ -462,9 +693,9 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_bf16x3(const uint16_t* _
 // planning and launch
 // ============================================================================
 
-// Tile height per wave (rows of dA for pass N, columns of dA for pass T), in
-// units of the instruction's M (16 for FP64, 32 otherwise).  Accumulators are
-// tile x (rb / N) blocks, kept at <= 8 (FP64) or 4 (FP32) blocks per wave.
+// Tile height (output rows per workgroup) in units of the instruction's M
+// (16 for FP64, 32 otherwise); accumulators are kept at <= 8 (FP64) or 4
+// (FP32) blocks per wave.
 static int tile_units(int mode, bool transposed, int rb) {
   if (mode == PASS_F64) {
     int cb = rb / 16;
@@ -472,27 +703,27 @@ static int tile_units(int mode, bool transposed, int rb) {
   }
   int cb = rb / 32;
   if (mode == PASS_F32) return cb <= 1 ? 2 : 1;
-  if (transposed) return cb <= 1 ? 2 : 1;
+  if (transposed) return 1;
   return cb <= 1 ? 4 : (cb == 2 ? 2 : 1);
 }
 
 static int wg_span(int mode, bool transposed, int rb) {
-  return 4 * (mode == PASS_F64 ? 16 : 32) * tile_units(mode, transposed, rb);
+  return (mode == PASS_F64 ? 16 : 32) * tile_units(mode, transposed, rb);
 }
 
 int64_t plan_pass(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int* splits,
                   int64_t* chunk) {
   const int span = wg_span(mode, transposed, rb);
-  const int64_t wgs = (out_rows + span - 1) / span;
-  const int64_t quantum = 64;  // reduction chunks are multiples of 64 (alignment of the bf16 tiles)
-  // aim for ~2048 workgroups, chunks of at least 256 along the reduction
-  int64_t want = (2048 + wgs - 1) / wgs;
-  int64_t maxs = (red_len + 255) / 256;
+  const int64_t tiles = (out_rows + span - 1) / span;
+  const int64_t quantum = 256;  // 4 waves x the largest per-wave iteration (64 rows)
+  // aim for ~512 workgroups (2048 waves) with at least 1024 of the reduction range per split
+  int64_t want = (512 + tiles - 1) / tiles;
+  const int64_t maxs = (red_len + 1023) / 1024;
   if (want > maxs) want = maxs;
   if (want < 1) want = 1;
   int64_t ch = (red_len + want - 1) / want;
   ch = (ch + quantum - 1) / quantum * quantum;
-  int64_t s = (red_len + ch - 1) / ch;
+  const int64_t s = (red_len + ch - 1) / ch;
   *splits = (int)s;
   *chunk = ch;
   return s * rb * out_rows;
@@ -505,7 +736,7 @@ static bool vec_ok(const PassArgs& a) {
 template <int TM, int CB>
 static hipError_t run_pass(int mode, bool transposed, const PassArgs& a, hipStream_t st) {
   const int64_t out_rows = transposed ? a.m : a.n;
-  const int span = 4 * (mode == PASS_F64 ? 16 : 32) * TM;
+  const int span = (mode == PASS_F64 ? 16 : 32) * TM;
   dim3 grid((unsigned)((out_rows + span - 1) / span), (unsigned)a.splits);
   dim3 blk(kPassThreads);
   if (mode == PASS_F64) {
@@ -527,27 +758,18 @@ static hipError_t run_pass(int mode, bool transposed, const PassArgs& a, hipStre
     }
 #undef LRI_F64_LAUNCH
   } else if (mode == PASS_F32) {
-    const float* B = (const float*)a.B;
-    float* P = (float*)a.part;
-#define LRI_F32_LAUNCH(T)                                                                                    \
-  if (transposed)                                                                                            \
-    hipLaunchKernelGGL((k_passT_f32<TM, CB, T>), grid, blk, 0, st, (const T*)a.anext, (const T*)a.aprev,      \
-                       a.lda, a.n, a.m, B, a.rb, a.chunk, P);                                                \
-  else                                                                                                       \
-    hipLaunchKernelGGL((k_passN_f32<TM, CB, T>), grid, blk, 0, st, (const T*)a.anext, (const T*)a.aprev,      \
-                       a.lda, a.n, a.m, B, a.rb, a.chunk, P);
-    if (a.dtype == LRI_F32) {
-      LRI_F32_LAUNCH(float)
-    } else {
-      LRI_F32_LAUNCH(uint16_t)
-    }
-#undef LRI_F32_LAUNCH
+    if (transposed)
+      hipLaunchKernelGGL((k_passT_f32<TM, CB>), grid, blk, 0, st, (const float*)a.anext, (const float*)a.aprev,
+                         a.lda, a.n, a.m, (const float*)a.B, a.rb, a.chunk, (float*)a.part);
+    else
+      hipLaunchKernelGGL((k_passN_f32<TM, CB>), grid, blk, 0, st, (const float*)a.anext, (const float*)a.aprev,
+                         a.lda, a.n, a.m, (const float*)a.B, a.rb, a.chunk, (float*)a.part);
   } else {
     const uint16_t* an = (const uint16_t*)a.anext;
     const uint16_t* ap = (const uint16_t*)a.aprev;
     const int v = vec_ok(a) ? 1 : 0;
     if (transposed)
-      hipLaunchKernelGGL((k_passT_bf16x3<TM, CB>), grid, blk, 0, st, an, ap, a.lda, a.n, a.m,
+      hipLaunchKernelGGL((k_passT_bf16x3<CB>), grid, blk, 0, st, an, ap, a.lda, a.n, a.m,
                          (const uint16_t*)a.B, (const uint16_t*)a.B2, a.ldb, a.rb, a.chunk, (float*)a.part, v);
     else
       hipLaunchKernelGGL((k_passN_bf16x3<TM, CB>), grid, blk, 0, st, an, ap, a.lda, a.n, a.m,
@@ -558,7 +780,7 @@ static hipError_t run_pass(int mode, bool transposed, const PassArgs& a, hipStre
 
 static hipError_t launch_pass(int mode, bool transposed, const PassArgs& a, hipStream_t st) {
   if (mode == PASS_BF16X3 && a.dtype != LRI_BF16) return hipErrorInvalidValue;
-  if (mode == PASS_F32 && a.dtype == LRI_F64) return hipErrorInvalidValue;
+  if (mode == PASS_F32 && a.dtype != LRI_F32) return hipErrorInvalidValue;
   const int tu = tile_units(mode, transposed, a.rb);
   if (mode == PASS_F64) {
     switch (a.rb) {
@@ -571,7 +793,9 @@ static hipError_t launch_pass(int mode, bool transposed, const PassArgs& a, hipS
     }
   }
   switch (a.rb) {
-    case 32: return tu == 4 ? run_pass<4, 1>(mode, transposed, a, st) : run_pass<2, 1>(mode, transposed, a, st);
+    case 32:
+      return tu == 4 ? run_pass<4, 1>(mode, transposed, a, st)
+                     : (tu == 2 ? run_pass<2, 1>(mode, transposed, a, st) : run_pass<1, 1>(mode, transposed, a, st));
     case 64: return tu == 2 ? run_pass<2, 2>(mode, transposed, a, st) : run_pass<1, 2>(mode, transposed, a, st);
     case 128: return run_pass<1, 4>(mode, transposed, a, st);
     default: return hipErrorInvalidValue;
