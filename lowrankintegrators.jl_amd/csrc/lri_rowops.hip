@@ -32,57 +32,86 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Upper Cholesky G = R'R of the k x k symmetric matrix G (k <= 64; upper
-// triangle read, G overwritten), computed by wave 0 alone: lane j owns column
-// j of the trailing update, so no workgroup barrier sits on the pivot chain.
-// A pivot <= tau = 10 eps max_i G(i,i) marks column p as numerically
-// dependent: dropped[p] = 1, R(p,p) = 1 and row p of R is zero, so that the
-// inverse from triu_inv_drop gives a zero column p in Q.  Returns the number
-// of dropped columns; sets *nonfinite on NaN/Inf.  Called by all threads.
-__device__ int chol_drop(double* __restrict__ G, double* __restrict__ R, int k, unsigned char* dropped,
-                          int* nonfinite) {
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+// Upper Cholesky G = R'R (k <= K <= 64) and R^{-1}, computed by wave 0 alone
+// with every column in registers: lane j holds column j of the trailing
+// matrix in a window g[t] = G(p + t, j) that shifts by one row per pivot, so
+// all register indices are static; pivots and multipliers are broadcast with
+// readlane.  A pivot <= tau = 10 eps max_i G(i,i) marks column p as
+// numerically dependent: dropped[p] = 1, R(p,p) = 1, row p of R is zero and
+// column p of R^{-1} is zero (so Q gets a zero column there).
+// R and Rinv are written to LDS (column-major, ld k).  Returns the number of
+// dropped columns; sets *nonfinite on NaN/Inf.  Called by all threads.
+template <int K>
+__device__ int chol_inv_wave(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv, int k,
+                             unsigned char* dropped, int* nonfinite) {
   __shared__ int s_nd, s_nf;
   if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
+    const int j = threadIdx.x;
     double md = 0.0;
     int nf = 0;
-    for (int i = lane; i < k; i += 64) {
-      const double d = G[i + i * k];
-      if (!isfinite(d)) nf = 1;
-      md = fmax(md, d);
+    if (j < k) {
+      const double dj = G[j + j * k];
+      nf = !isfinite(dj);
+      md = dj;
     }
     for (int off = 32; off > 0; off >>= 1) {
       md = fmax(md, __shfl_xor(md, off));
       nf |= __shfl_xor(nf, off);
     }
     const double tau = 10.0 * kEps * md;
-    for (int e = lane; e < k * k; e += 64) R[e] = 0.0;
-    wave_lds_sync();
+    double g[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) g[t] = (t < k && j < k && t <= j) ? G[t + j * k] : 0.0;
     int nd = 0;
-    const int j = lane;  // the column this lane updates
     for (int p = 0; p < k; ++p) {
-      const double d = G[p + p * k];
-      const bool keep = (d > tau) && isfinite(d);
-      if (!isfinite(d)) nf = 1;
+      const double d = readlane_f64(g[0], p);
+      const bool fin = isfinite(d);
+      const bool keep = (d > tau) && fin;
+      nf |= !fin;
       const double piv = keep ? sqrt(d) : 1.0;
       const double inv = 1.0 / piv;
       nd += keep ? 0 : 1;
-      double rpj = 0.0;
-      if (j > p && j < k) {
-        rpj = keep ? G[p + j * k] * inv : 0.0;
-        R[p + j * k] = rpj;
+      const double rpj = (keep && j > p && j < k) ? g[0] * inv : 0.0;
+      if (j < k) R[p + j * k] = (j == p) ? piv : (j > p ? rpj : 0.0);
+      if (j == p) dropped[p] = keep ? 0 : 1;
+#pragma unroll
+      for (int t = 1; t < K; ++t) {
+        if (p + t < k) {
+          const double rpi = readlane_f64(rpj, p + t);
+          if (p + t <= j) g[t - 1] = g[t] - rpi * rpj;
+          else g[t - 1] = g[t];
+        } else {
+          g[t - 1] = g[t];
+        }
       }
-      if (lane == 0) {
-        R[p + p * k] = piv;
-        dropped[p] = keep ? 0 : 1;
-      }
-      wave_lds_sync();
-      if (keep && j > p && j < k) {
-        for (int i = p + 1; i <= j; ++i) G[i + j * k] -= R[p + i * k] * rpj;
-      }
-      wave_lds_sync();
+      g[K - 1] = 0.0;
     }
-    if (lane == 0) {
+    wave_lds_sync();
+    // R^{-1}, column j: back substitution over i = k-1 .. 0 with the known
+    // entries x(i+1+t) in the window y[t] (shifted right by one per row)
+    double y[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) y[t] = 0.0;
+    for (int i = k - 1; i >= 0; --i) {
+      double sacc = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int t = 0; t < K; ++t)
+        if (i + 1 + t < k) sacc -= R[i + (i + 1 + t) * k] * y[t];
+      const bool di = dropped[i] != 0;
+      const double xi = (di || j >= k || (j < k && dropped[j]) || i > j) ? 0.0 : sacc / R[i + i * k];
+      if (j < k) Rinv[i + j * k] = xi;
+#pragma unroll
+      for (int t = K - 1; t > 0; --t) y[t] = y[t - 1];
+      y[0] = xi;
+    }
+    if (j == 0) {
       s_nd = nd;
       s_nf = nf;
     }
@@ -93,24 +122,10 @@ __device__ int chol_drop(double* __restrict__ G, double* __restrict__ R, int k, 
   return s_nd;
 }
 
-// Rinv = R^{-1} for upper-triangular R (k <= 64), dropped columns/rows zeroed;
-// wave 0, lane j solves column j by back substitution.  Called by all threads.
-__device__ void triu_inv_drop(const double* __restrict__ R, double* __restrict__ Rinv, int k,
-                              const unsigned char* dropped) {
-  if (threadIdx.x < 64) {
-    const int j = threadIdx.x;
-    for (int e = j; e < k * k; e += 64) Rinv[e] = 0.0;
-    wave_lds_sync();
-    if (j < k && !dropped[j]) {
-      for (int i = j; i >= 0; --i) {
-        if (dropped[i]) continue;
-        double s = (i == j) ? 1.0 : 0.0;
-        for (int l = i + 1; l <= j; ++l) s -= R[i + l * k] * Rinv[l + j * k];
-        Rinv[i + j * k] = s / R[i + i * k];
-      }
-    }
-  }
-  __syncthreads();
+__device__ int chol_inv(const double* G, double* R, double* Rinv, int k, unsigned char* dropped, int* nonfinite) {
+  if (k <= 16) return chol_inv_wave<16>(G, R, Rinv, k, dropped, nonfinite);
+  if (k <= 32) return chol_inv_wave<32>(G, R, Rinv, k, dropped, nonfinite);
+  return chol_inv_wave<64>(G, R, Rinv, k, dropped, nonfinite);
 }
 
 // C = op(A) * op(B), k x k, column-major.
@@ -151,11 +166,58 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
     case EPI_CHOL2: {
       const int k = a;
       __shared__ unsigned char dropped[kMaxAug];
+      __shared__ int s_big;
       double* R = scr;
       double* Rinv = scr + k * k;
-      int nf = 0;
-      int nd = chol_drop(C, R, k, dropped, &nf);
-      triu_inv_drop(R, Rinv, k, dropped);
+      int nf = 0, nd = 0;
+      // CholeskyQR2's second Gram matrix is I + E with |E| ~ eps cond^2: when
+      // max|E| <= 1e-6 (and the first pass dropped nothing), R = I + F with
+      // F = F1 + F2, F1 = upper(E) (diagonal halved), F2 = upper(-F1'F1)
+      // (diagonal halved), and R^{-1} = I - F + F^2, all exact to O(|E|^3).
+      bool pert = false;
+      if (E.kind == EPI_CHOL2 && st->ndropped == 0) {
+        if (t == 0) s_big = 0;
+        __syncthreads();
+        double mx = 0.0;
+        for (int e = t; e < k * k; e += blockDim.x) {
+          const int i = e % k, jj = e / k;
+          const double v = C[e] - (i == jj ? 1.0 : 0.0);
+          mx = fmax(mx, isfinite(v) ? fabs(v) : 1e300);
+        }
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        if ((t & 63) == 0 && mx > 1e-6) atomicOr(&s_big, 1);  // any wave over the bound -> full Cholesky
+        __syncthreads();
+        pert = (s_big == 0);
+      }
+      if (pert) {
+        double* F1 = Rinv;  // scratch: F1, then F1'F1 into R
+        for (int e = t; e < k * k; e += blockDim.x) {
+          const int i = e % k, jj = e / k;
+          const double v = C[e] - (i == jj ? 1.0 : 0.0);
+          F1[e] = i < jj ? v : (i == jj ? 0.5 * v : 0.0);
+        }
+        __syncthreads();
+        small_gemm(F1, 1, F1, 0, R, k);  // R <- F1' F1
+        __syncthreads();
+        for (int e = t; e < k * k; e += blockDim.x) {  // C <- F = F1 + upper(-F1'F1) (diag halved)
+          const int i = e % k, jj = e / k;
+          const double f2 = i < jj ? -R[e] : (i == jj ? -0.5 * R[e] : 0.0);
+          C[e] = F1[e] + f2;
+        }
+        __syncthreads();
+        small_gemm(C, 0, C, 0, Rinv, k);  // Rinv <- F^2
+        __syncthreads();
+        for (int e = t; e < k * k; e += blockDim.x) {
+          const int i = e % k, jj = e / k;
+          const double id = i == jj ? 1.0 : 0.0;
+          R[e] = id + C[e];
+          Rinv[e] = id - C[e] + Rinv[e];
+        }
+        if (t < k) dropped[t] = 0;
+        __syncthreads();
+      } else {
+        nd = chol_inv(C, R, Rinv, k, dropped, &nf);
+      }
       if (E.kind == EPI_CHOL1) {
         for (int e = t; e < k * k; e += blockDim.x) {
           st->R1[e] = R[e];
@@ -172,15 +234,15 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
         if (E.S) {
           __syncthreads();
           for (int e = t; e < k * k; e += blockDim.x) {
-            int i = e % k, j = e / k;
-            E.S[i + j * E.ld_s] = E.transpose ? st->Rtot[j + i * k] : st->Rtot[e];
+            int i = e % k, jj = e / k;
+            E.S[i + jj * E.ld_s] = E.transpose ? st->Rtot[jj + i * k] : st->Rtot[e];
           }
         }
       }
       if (t == 0) {
         if (nf) atomicOr(E.flags, FLAG_NONFINITE);
         if (nd && E.breakdown_is_error) atomicOr(E.flags, FLAG_BREAKDOWN);
-        st->ndropped = nd;
+        if (E.kind == EPI_CHOL1) st->ndropped = nd;
       }
       break;
     }
