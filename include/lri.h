@@ -140,6 +140,43 @@ int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next);
 int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax,
                     int64_t* r_new);
 
+/* ---- matrix ODE (MatrixDEProblem, primitives.jl:13-17) --------------------
+ *
+ * The reference evaluates a user closure F(Y, t) on low-rank arguments inside
+ * its K/L/S right-hand sides (projector_splitting.jl:52-80,
+ * unconventional.jl:51-79, rank_adaptive_unconventional.jl:59-86).  A closure
+ * cannot run on the device, so the operator is given as a descriptor:
+ *
+ *     F(Y) = A*Y + Y*B' + G*H'
+ *
+ * with A (n x n) and B (m x m) symmetric tridiagonal -- diagonal a_diag[i],
+ * a_off[i] coupling rows i and i+1, and when a_periodic also a_off[n-1]
+ * coupling rows n-1 and 0 (periodic arrays need n >= 3) -- and a forcing G*H'
+ * of rank q <= LRI_MAX_FORCE (G n x q, H m x q; q = 0 for none).  Arrays are
+ * host FP64 and are copied; a_off has n-1 entries (n when periodic).
+ * The sub-problems are integrated with `substeps` classical RK4 steps of size
+ * dt/substeps (the reference's default is adaptive Tsit5).  The sub-problem
+ * columns are limited to n, m <= LRI_DE_MAX_ROWS. */
+#define LRI_MAX_FORCE 4
+#define LRI_DE_MAX_ROWS 16384
+
+int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* a_off, int a_periodic,
+                              const double* b_diag, const double* b_off, int b_periodic, int64_t q,
+                              const double* G, int64_t ldg, const double* H, int64_t ldh);
+
+/* One KSL step of the matrix ODE over [t, t+dt]: order LRI_PRIMAL
+ * (projector_splitting.jl:129-152,191-196), LRI_DUAL (:166-189,198-203) or
+ * LRI_STRANG (primal over dt/2, then dual over dt/2, :205-211; each half with
+ * `substeps` RK4 steps). */
+int lri_step_de_ksl(lri_ctx* ctx, int order, double dt, int64_t substeps);
+
+/* One BUG step of the matrix ODE (unconventional.jl:133-164). */
+int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps);
+
+/* One rank-adaptive BUG step of the matrix ODE (rank_adaptive_unconventional.jl:171-233);
+ * truncation and *r_new as lri_step_bug_ra (synchronising). */
+int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, int64_t rmax, int64_t* r_new);
+
 /* Wait for all queued work; returns a latched device-side status, if any. */
 int lri_synchronize(lri_ctx* ctx);
 
@@ -153,11 +190,14 @@ int lri_get_stream(lri_ctx* ctx, void** stream);
 #define LRI_KERNEL_PASS_N 0 /* dA * B, reduction over the m columns */
 #define LRI_KERNEL_PASS_T 1 /* dA' * B, reduction over the n rows */
 #define LRI_KERNEL_SMALL 2  /* all FP64 n x r / r x r kernels of a step */
-#define LRI_KERNEL_COUNT 3
+#define LRI_KERNEL_DE 3     /* RK4 sub-step integration of the matrix-ODE columns */
+#define LRI_KERNEL_COUNT 4
 
-/* Enable (1) or disable (0) HIP-event timing of every launch of the
- * families above on the context's stream; enabling resets the totals. */
-int lri_kernel_timing(lri_ctx* ctx, int enable);
+/* HIP-event timing on the context's stream: mode 0 disables it, mode 1 times
+ * every launch of the families above, mode 2 only the product passes
+ * (LRI_KERNEL_PASS_N/T, two events per pass, the least intrusive).  Enabling
+ * resets the totals. */
+int lri_kernel_timing(lri_ctx* ctx, int mode);
 
 /* Synchronise and return the summed duration (ms) and launch count of a
  * kernel family since timing was enabled. */

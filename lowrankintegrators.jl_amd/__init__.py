@@ -1,7 +1,8 @@
 """MI355X (gfx950) engine for the per-step DLRA hot path of LowRankIntegrators.jl.
 
 The steps of the projector-splitting (KSL), unconventional (BUG) and
-rank-adaptive BUG integrators on data problems run in ``liblri_hip.so``
+rank-adaptive BUG integrators on data problems, and on matrix ODEs with a
+tridiagonal Lyapunov-type right-hand side, run in ``liblri_hip.so``
 (hand-written HIP kernels behind the C ABI of ``include/lri.h``); this package
 is the host side: a ctypes binding (``native``) and a mirror of the
 reference's Julia API (``api``).
@@ -11,6 +12,7 @@ from .api import (  # noqa: F401
     DLRIntegrator,
     DLRSolution,
     DualLieTrotter,
+    LyapunovOperator,
     Matrix,
     MatrixDataProblem,
     MatrixDEProblem,
@@ -20,10 +22,13 @@ from .api import (  # noqa: F401
     RankAdaptiveUnconventionalAlgorithm,
     Strang,
     SVDLikeRepresentation,
+    SymTridiagonal,
     TwoFactorRepresentation,
     UnconventionalAlgorithm,
     init,
+    laplacian_1d,
     rank,
+    rk4_substeps,
     solve,
     step,
     truncated_svd,

@@ -8,10 +8,18 @@ the GPU.  Julia's ``step!`` is spelled ``step`` here.
     sol  = solve(prob, ProjectorSplitting(PrimalLieTrotter()), dt)
     sol.Y[-1].U, sol.Y[-1].S, sol.Y[-1].V, sol.t
 
+Matrix ODEs run too when F is given as an operator descriptor (a Python
+closure cannot run on the device):
+
+    F    = LyapunovOperator(laplacian_1d(n), laplacian_1d(m), G, H)   # A Y + Y B' + G H'
+    prob = MatrixDEProblem(F, u0, (0.0, 0.05))
+    sol  = solve(prob, UnconventionalAlgorithm(), 1e-4)          # RK4 sub-steps
+
 Extensions beyond the reference (keyword-only): ``precision`` ("f64" or
 "mixed"), ``snapshot_dtype`` ("f64", "f32", "bf16"), ``save_everystep``,
-``device``.  ``MatrixDEProblem`` and the custom K/L/S right-hand-side hooks are
-part of the reference's DE path, which this engine does not run: they raise
+``device``, and for matrix ODEs ``substeps`` (RK4 steps per sub-problem; by
+default the smallest count with h * rho <= 2.5 for the Gershgorin bound rho of
+the operator).  The custom K/L/S right-hand-side hooks raise
 NotImplementedError.
 """
 from __future__ import annotations
@@ -114,8 +122,89 @@ class MatrixDataProblem(AbstractDLRProblem):
             self.tspan = (1, len(self.y))
 
 
+class SymTridiagonal:
+    """Symmetric tridiagonal n x n matrix: ``d`` on the diagonal, ``e[i]`` coupling rows
+    i and i+1 (n - 1 entries), plus ``e[n-1]`` coupling rows n-1 and 0 when periodic."""
+
+    def __init__(self, d, e, periodic: bool = False):
+        self.d = np.asarray(d, dtype=np.float64).reshape(-1).copy()
+        self.e = np.asarray(e, dtype=np.float64).reshape(-1).copy()
+        self.periodic = bool(periodic)
+        n = self.d.shape[0]
+        if self.e.shape[0] != (n if self.periodic else n - 1):
+            raise ValueError(f"a {'periodic ' if periodic else ''}{n} x {n} tridiagonal needs "
+                             f"{n if periodic else n - 1} off-diagonal entries")
+        if self.periodic and n < 3:
+            raise ValueError("a periodic tridiagonal needs n >= 3")
+
+    @property
+    def n(self) -> int:
+        return self.d.shape[0]
+
+    def dense(self) -> np.ndarray:
+        n = self.n
+        A = np.diag(self.d)
+        A[np.arange(n - 1), np.arange(1, n)] = self.e[: n - 1]
+        A[np.arange(1, n), np.arange(n - 1)] = self.e[: n - 1]
+        if self.periodic:
+            A[n - 1, 0] += self.e[n - 1]
+            A[0, n - 1] += self.e[n - 1]
+        return A
+
+    def gershgorin(self) -> float:
+        """Upper bound of the spectral radius."""
+        n = self.n
+        a = np.abs(self.e)
+        up = np.zeros(n)
+        dn = np.zeros(n)
+        dn[: n - 1] = a[: n - 1]
+        up[1:] = a[: n - 1]
+        if self.periodic:
+            dn[n - 1] += a[n - 1]
+            up[0] += a[n - 1]
+        return float(np.max(np.abs(self.d) + up + dn))
+
+
+def laplacian_1d(n: int, scale: Optional[float] = None, periodic: bool = False) -> SymTridiagonal:
+    """scale * tridiag(1, -2, 1) (default scale n^2), Dirichlet or periodic boundaries."""
+    s = float(n * n) if scale is None else float(scale)
+    return SymTridiagonal(-2.0 * s * np.ones(n), s * np.ones(n if periodic else n - 1), periodic)
+
+
+class LyapunovOperator:
+    """F(Y, t) = A Y + Y B' + G H' with A, B ``SymTridiagonal`` and G (n x q), H (m x q), q <= 4.
+
+    The device form of a matrix-ODE right-hand side.  Calling it evaluates F on a
+    dense host matrix (for small checks)."""
+
+    def __init__(self, A: SymTridiagonal, B: SymTridiagonal, G=None, H=None):
+        self.A, self.B = A, B
+        if (G is None) != (H is None):
+            raise ValueError("give both G and H, or neither")
+        if G is None:
+            self.G = np.zeros((A.n, 0))
+            self.H = np.zeros((B.n, 0))
+        else:
+            self.G = np.asarray(G, dtype=np.float64).reshape(A.n, -1)
+            self.H = np.asarray(H, dtype=np.float64).reshape(B.n, -1)
+            if self.G.shape[1] != self.H.shape[1] or self.G.shape[1] > native.MAX_FORCE:
+                raise ValueError(f"G and H need the same number q <= {native.MAX_FORCE} of columns")
+
+    def __call__(self, Y, t=0.0):
+        return self.A.dense() @ Y + Y @ self.B.dense().T + self.G @ self.H.T
+
+    def spectral_bound(self) -> float:
+        return self.A.gershgorin() + self.B.gershgorin()
+
+
+def rk4_substeps(rho: float, dt: float, cap: float = 2.5) -> int:
+    """Fewest RK4 steps with h * rho <= cap (RK4's real stability interval is [-2.78, 0])."""
+    return max(1, int(math.ceil(abs(dt) * rho / cap)))
+
+
 class MatrixDEProblem(AbstractDLRProblem):
-    """The reference's matrix-ODE problem (primitives.jl:13-17): not run by this engine."""
+    """Matrix ODE dY/dt = f(Y, t) (primitives.jl:13-17).  On this engine ``f`` is a
+    ``LyapunovOperator``."""
 
     def __init__(self, f, u0, tspan):
         self.f, self.u0, self.tspan = f, u0, tspan
@@ -236,7 +325,7 @@ class DLRIntegrator:
         self.probType = type(prob)
         self.snapshot_dtype = snapshot_dtype
         self.save_everystep = save_everystep
-        self._array_data = not callable(prob.y)
+        self._array_data = isinstance(prob, MatrixDataProblem) and not callable(prob.y)
         self._slot_of_t = 0  # function data: ring slot (of 3) holding y(t)
         self.rank = u.rank
 
@@ -252,10 +341,12 @@ def _device_ptr(x):
 
 
 def init(prob, alg, dt, *, precision: str = "f64", snapshot_dtype: Optional[str] = None,
-         save_everystep: bool = True, device: int = 0) -> DLRIntegrator:
+         save_everystep: bool = True, device: int = 0, substeps: Optional[int] = None) -> DLRIntegrator:
     """``init(prob, alg, dt)`` (projector_splitting.jl:107-115 and the BUG/RA counterparts)."""
-    if isinstance(prob, (MatrixDEProblem, MatrixHybridProblem)):
-        raise NotImplementedError(f"{type(prob).__name__}: only MatrixDataProblem runs on this engine")
+    if isinstance(prob, MatrixHybridProblem):
+        raise NotImplementedError("MatrixHybridProblem belongs to the Greedy integrator, which this engine does not run")
+    if isinstance(prob, MatrixDEProblem):
+        return _init_de(prob, alg, dt, save_everystep=save_everystep, device=device, substeps=substeps)
     if not isinstance(alg, (ProjectorSplitting, UnconventionalAlgorithm, RankAdaptiveUnconventionalAlgorithm)):
         raise TypeError(f"unsupported algorithm {alg!r}")
     t0, tf = prob.tspan
@@ -299,13 +390,69 @@ def init(prob, alg, dt, *, precision: str = "f64", snapshot_dtype: Optional[str]
     return DLRIntegrator(prob, alg, dt, ctx, u0.copy(), sol, snapshot_dtype=sdt, save_everystep=save_everystep)
 
 
+def _init_de(prob, alg, dt, *, save_everystep, device, substeps) -> DLRIntegrator:
+    """``init`` for a matrix ODE: the operator descriptor is copied to the device."""
+    F = prob.f
+    if not isinstance(F, LyapunovOperator):
+        raise NotImplementedError("a Python callable cannot run on the device: give the right-hand side as a "
+                                  "LyapunovOperator (A Y + Y B' + G H')")
+    if not isinstance(alg, (ProjectorSplitting, UnconventionalAlgorithm, RankAdaptiveUnconventionalAlgorithm)):
+        raise TypeError(f"unsupported algorithm {alg!r}")
+    t0, tf = prob.tspan
+    if not tf > t0:
+        raise AssertionError("Integration in reverse time direction is not supported")
+    u0 = prob.u0
+    n, m = u0.U.shape[0], u0.V.shape[0]
+    if (F.A.n, F.B.n) != (n, m):
+        raise ValueError(f"operator is {F.A.n} x {F.B.n}, state is {n} x {m}")
+    r = u0.rank
+    if isinstance(alg, RankAdaptiveUnconventionalAlgorithm):
+        rcap = int(min(alg.rmax, native.MAX_RANK, n, m))
+        if r > rcap:
+            raise ValueError(f"initial rank {r} exceeds the capacity {rcap}")
+    else:
+        rcap = r
+    if r > native.MAX_RANK:
+        raise ValueError(f"rank {r} exceeds LRI_MAX_RANK = {native.MAX_RANK}")
+    ctx = native.Context(n, m, rcap, device)
+    ctx.set_state(u0.U, u0.S, u0.V)
+    q = F.G.shape[1]
+    ctx.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic,
+                              F.G if q else None, F.H if q else None)
+    sol = init_sol(dt, t0, tf, u0)
+    sol.Y[0] = u0.copy()
+    integ = DLRIntegrator(prob, alg, dt, ctx, u0.copy(), sol, snapshot_dtype=None, save_everystep=save_everystep)
+    integ.substeps = int(substeps) if substeps is not None else None
+    return integ
+
+
+def _step_de(integrator, alg, dt) -> None:
+    ctx = integrator.ctx
+    N = integrator.substeps
+    if N is None:
+        half = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang)
+        N = rk4_substeps(integrator.prob.f.spectral_bound(), dt / 2 if half else dt)
+    if isinstance(alg, ProjectorSplitting):
+        order = {PrimalLieTrotter: native.PRIMAL, DualLieTrotter: native.DUAL, Strang: native.STRANG}[type(alg.order)]
+        ctx.step_de_ksl(order, dt, N)
+    elif isinstance(alg, UnconventionalAlgorithm):
+        ctx.step_de_bug(dt, N)
+    else:
+        r_new = ctx.step_de_bug_ra(dt, N, alg.tol, alg.rmax)
+        if r_new != integrator.rank:
+            print(f"rank adjusted: new rank = {r_new}")  # rank_adaptive_unconventional.jl:230
+            integrator.rank = r_new
+
+
 def step(integrator: DLRIntegrator, alg=None, dt=None) -> None:
     """``step!(integrator, alg, dt)``: one step on the device, then t += dt, iter += 1."""
     alg = integrator.alg if alg is None else alg
     dt = integrator.dt if dt is None else dt
     ctx = integrator.ctx
     t = integrator.t
-    if integrator._array_data:
+    if isinstance(integrator.prob, MatrixDEProblem):
+        _step_de(integrator, alg, dt)
+    elif integrator._array_data:
         if isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang):
             # update_data!(x, y::AbstractArray, t::Int, dt::Int) has no method for dt/2 (data_integrator.jl:26)
             raise TypeError("array-form data has no half-step snapshot: Strang needs function-form data")

@@ -1,0 +1,326 @@
+// Matrix-ODE (MatrixDEProblem) sub-steps for the operator
+//
+//     F(Y) = A Y + Y B' + sum_k g_k h_k'
+//
+// with A (n x n) and B (m x m) symmetric tridiagonal (optionally periodic) and a
+// rank-q forcing.  With U, V fixed during a sub-step the reference's default
+// right-hand sides (projector_splitting.jl:52-80, unconventional.jl:51-79,
+// rank_adaptive_unconventional.jl:59-86) become linear:
+//
+//     K' = A K + K Bt + g (V'h)',   Bt = V'BV          (K step)
+//     L' = B L + L At + h (U'g)',   At = U'AU          (L step)
+//     S' = sgn (At S + S Bt + (U'g)(V'h)')             (S step; sgn = -1 in KSL)
+//
+// Bt is symmetric, so with Bt = W diag(beta) W' the columns of Kbar = K W
+// decouple: kbar_j' = (A + beta_j) kbar_j + g (h'V W)_j.  Classical RK4 commutes
+// with a constant orthogonal change of variables, so N RK4 steps on Kbar
+// followed by K = Kbar W' equal N RK4 steps on K, which is what a fixed-step
+// RK4 sub-integrator does in the reference's K/L/S sub-problems.  Each column
+// then runs all of its RK4 stages inside one workgroup.  The S step is
+// diagonal in the product eigenbasis and its N RK4 steps are applied in closed
+// form entry by entry.
+#include "lri_common.h"
+#include "lri_kernels.h"
+
+namespace lri {
+
+// ----------------------------------------------------------------------------
+// RK4 on independent columns.  Workgroup j owns column j; thread t owns rows
+// [t*RPT, t*RPT + RPT) in registers and only the first and last of them go
+// through LDS (double-buffered by stage, so one barrier per stage).  A stage
+// is evaluated in ascending row order in place: row u needs the old values of
+// rows u-1 (kept in `prev`) and u+1 (not yet overwritten).
+// REG keeps the row coefficients in registers; otherwise they are re-read from
+// the (cache-resident) operator arrays at each stage.
+// ----------------------------------------------------------------------------
+
+template <int RPT, bool REG>
+__global__ __launch_bounds__(1024) void k_rk4_columns(RK4ColArgs P) {
+  __shared__ double s_first[2][1024], s_last[2][1024];
+  const int t = threadIdx.x;
+  const int j = blockIdx.x;
+  const int64_t n = P.n;
+  const int64_t r0 = (int64_t)t * RPT;
+  const int last_t = (int)((n - 1) / RPT);
+  const bool active = t <= last_t;
+  const int nlast = (int)(n - (int64_t)last_t * RPT);
+  const int nmine = !active ? 0 : (t == last_t ? nlast : RPT);
+  const int tn = (t == last_t) ? 0 : t + 1;  // holder of the row after my last row
+  const int tp = (t == 0) ? last_t : t - 1;  // holder of the row before my first row
+  const bool has_dn = active && (t < last_t || P.periodic);
+  const bool has_up = active && (t > 0 || P.periodic);
+  const double mu = P.mu[j];
+  const double h = P.h, h2 = 0.5 * h, h6 = h / 6.0;
+  double cf[kMaxForce];
+#pragma unroll
+  for (int k = 0; k < kMaxForce; ++k) cf[k] = k < P.q ? P.coef[k + j * P.q] : 0.0;
+  // e of the row above my first row
+  double eup = 0.0;
+  if (has_up) eup = r0 > 0 ? P.st_e[r0 - 1] : P.st_e[n - 1];
+
+  // row pointers of this thread (dereferenced only for its own rows)
+  const double* pd = P.st_d + (active ? r0 : 0);
+  const double* pe = P.st_e + (active ? r0 : 0);
+  const double* pg = P.g ? P.g + (active ? r0 : 0) : nullptr;
+  // e of the last row: zero unless periodic
+  const bool e_last = P.periodic != 0;
+  auto load_coef = [&](int u, double& dd, double& ee, double& ff) {
+    if (u < nmine) {
+      dd = pd[u] + mu;
+      ee = (r0 + u + 1 < n || e_last) ? pe[u] : 0.0;
+      double s = 0.0;
+      for (int k = 0; k < P.q; ++k) s += pg[u + k * P.ldg] * cf[k];
+      ff = s;
+    } else {
+      dd = ee = ff = 0.0;
+    }
+  };
+
+  double d[REG ? RPT : 1], e[REG ? RPT : 1], f[REG ? RPT : 1];
+  if (REG) {
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) load_coef(u, d[REG ? u : 0], e[REG ? u : 0], f[REG ? u : 0]);
+  }
+
+  double y[RPT], ks[RPT], x[RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    y[u] = u < nmine ? P.X[r0 + u + (int64_t)j * P.ldx] : 0.0;
+    x[u] = y[u];
+    ks[u] = 0.0;
+  }
+  int buf = 0;
+  for (int64_t s = 0; s < P.nsub; ++s) {
+#pragma unroll 1
+    for (int stage = 0; stage < 4; ++stage) {
+      // stage input x = y + c_stage * k, accumulated weight w_stage
+      const double cs = stage < 2 ? h2 : h;
+      const double ws = (stage == 0) ? 1.0 : 2.0;
+      s_first[buf][t] = x[0];
+      double xl = x[RPT - 1];
+#pragma unroll
+      for (int u = 0; u < RPT; ++u)
+        if (u == nmine - 1) xl = x[u];
+      s_last[buf][t] = xl;
+      __syncthreads();
+      const double above = has_up ? s_last[buf][tp] : 0.0;
+      const double below = has_dn ? s_first[buf][tn] : 0.0;
+      buf ^= 1;
+      double prev = above, e_prev = eup;
+#pragma unroll
+      for (int u = 0; u < RPT; ++u) {
+        double dd, ee, ff;
+        if (REG) {
+          dd = d[REG ? u : 0];
+          ee = e[REG ? u : 0];
+          ff = f[REG ? u : 0];
+        } else {
+          load_coef(u, dd, ee, ff);
+        }
+        const bool real = u < nmine;
+        const double xu = x[u];
+        const double xd = (u + 1 < nmine) ? x[u + 1 < RPT ? u + 1 : RPT - 1] : below;
+        const double k = real ? dd * xu + e_prev * prev + ee * xd + ff : 0.0;
+        prev = xu;
+        e_prev = ee;
+        if (stage < 3) {
+          ks[u] = stage == 0 ? k : ks[u] + ws * k;
+          x[u] = y[u] + cs * k;
+        } else {
+          y[u] = y[u] + h6 * (ks[u] + k);
+          x[u] = y[u];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RPT; ++u)
+    if (u < nmine) P.Y[r0 + u + (int64_t)j * P.ldy] = y[u];
+}
+
+hipError_t launch_rk4_columns(const RK4ColArgs& P, hipStream_t st) {
+  if (P.ncols < 1 || P.nsub < 1) return hipSuccess;
+  if (P.n < 1 || P.n > 16384 || P.q < 0 || P.q > kMaxForce) return hipErrorInvalidValue;
+  int rpt = 4;
+  while ((int64_t)rpt * 1024 < P.n) rpt *= 2;
+  int nt = (int)((P.n + rpt - 1) / rpt);
+  nt = (nt + 63) & ~63;
+  const dim3 grid((unsigned)P.ncols), blk((unsigned)nt);
+  if (rpt == 4)
+    hipLaunchKernelGGL((k_rk4_columns<4, true>), grid, blk, 0, st, P);
+  else if (rpt == 8)
+    hipLaunchKernelGGL((k_rk4_columns<8, true>), grid, blk, 0, st, P);
+  else
+    hipLaunchKernelGGL((k_rk4_columns<16, false>), grid, blk, 0, st, P);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// CW = op(C) W (k x k) and coef = FX W (q x k), one workgroup
+// ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_de_prep(const double* __restrict__ C, int ldc, int transC,
+                                                 const double* __restrict__ W, const double* __restrict__ FX, int k,
+                                                 int q, double* __restrict__ CW, double* __restrict__ coef) {
+  const int t = threadIdx.x;
+  for (int e = t; e < k * k; e += blockDim.x) {
+    const int i = e % k, jj = e / k;
+    double s = 0.0;
+    for (int c = 0; c < k; ++c) s += (transC ? C[c + i * ldc] : C[i + c * ldc]) * W[c + jj * k];
+    CW[e] = s;
+  }
+  for (int e = t; e < q * k; e += blockDim.x) {
+    const int qq = e % q, jj = e / q;
+    double s = 0.0;
+    for (int c = 0; c < k; ++c) s += FX[qq + c * q] * W[c + jj * k];
+    coef[e] = s;
+  }
+}
+
+hipError_t launch_de_prep(const double* C, int ldc, int transC, const double* W, const double* FX, int k, int q,
+                          double* CW, double* coef, hipStream_t st) {
+  if (k < 1 || k > kMaxAug || q < 0 || q > kMaxForce) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_de_prep, dim3(1), dim3(256), 0, st, C, ldc, transC, W, FX, k, q, CW, coef);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// S sub-step in the product eigenbasis, one workgroup (k, l <= 128):
+//   Sbar = Wa' S0 Wb,  cbar = (Wa' U'g)(Wb' V'h)',
+//   s_ij' = sgn ((alpha_i + beta_j) s_ij + cbar_ij)  for N RK4 steps of size h,
+//   S1 = Wa Sbar Wb'.
+// One RK4 step on y' = a y + b maps y -> R y + h Pz b with z = h a,
+// R = 1 + z + z^2/2 + z^3/6 + z^4/24, Pz = 1 + z/2 + z^2/6 + z^3/24; N steps are
+// the N-fold composition of this affine map, formed by repeated squaring.
+// ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_s_substep(SStepArgs P) {
+  const int k = P.k, l = P.l, q = P.q, t = threadIdx.x;
+  double* Sb = P.work;       // k x l
+  double* T = Sb + k * l;    // k x l
+  // T = S0 Wb
+  for (int e = t; e < k * l; e += blockDim.x) {
+    const int i = e % k, jj = e / k;
+    double s = 0.0;
+    for (int c = 0; c < l; ++c)
+      s += (P.transS0 ? P.S0[c + i * P.lds0] : P.S0[i + c * P.lds0]) * P.Wb[c + jj * l];
+    T[e] = s;
+  }
+  __syncthreads();
+  // Sbar = Wa' T, then the RK4 flow per entry
+  for (int e = t; e < k * l; e += blockDim.x) {
+    const int i = e % k, jj = e / k;
+    double s = 0.0;
+    for (int c = 0; c < k; ++c) s += P.Wa[c + i * k] * T[c + jj * k];
+    double cb = 0.0;
+    for (int qq = 0; qq < q; ++qq) {
+      double ga = 0.0, hb = 0.0;
+      for (int c = 0; c < k; ++c) ga += P.Wa[c + i * k] * P.gaT[qq + c * q];
+      for (int c = 0; c < l; ++c) hb += P.Wb[c + jj * l] * P.hbT[qq + c * q];
+      cb += ga * hb;
+    }
+    const double a = P.sgn * (P.alpha[i] + P.beta[jj]);
+    const double b = P.sgn * cb;
+    const double z = P.h * a;
+    const double R = 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0 + z / 24.0)));
+    const double Pz = 1.0 + z * (0.5 + z * (1.0 / 6.0 + z / 24.0));
+    double Rm = R, Qm = P.h * Pz * b;  // one step: y -> Rm y + Qm
+    double Ra = 1.0, Qa = 0.0;         // accumulated map
+    for (int64_t nn = P.nsub; nn > 0; nn >>= 1) {
+      if (nn & 1) {
+        Qa = Rm * Qa + Qm;
+        Ra = Rm * Ra;
+      }
+      Qm = Rm * Qm + Qm;
+      Rm = Rm * Rm;
+    }
+    Sb[e] = Ra * s + Qa;
+  }
+  __syncthreads();
+  // T = Sbar Wb'
+  for (int e = t; e < k * l; e += blockDim.x) {
+    const int i = e % k, jj = e / k;
+    double s = 0.0;
+    for (int c = 0; c < l; ++c) s += Sb[i + c * k] * P.Wb[jj + c * l];
+    T[e] = s;
+  }
+  __syncthreads();
+  // S1 = Wa T
+  for (int e = t; e < k * l; e += blockDim.x) {
+    const int i = e % k, jj = e / k;
+    double s = 0.0;
+    for (int c = 0; c < k; ++c) s += P.Wa[i + c * k] * T[c + jj * k];
+    P.S1[i + jj * P.lds1] = s;
+  }
+}
+
+hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st) {
+  if (P.k < 1 || P.l < 1 || P.k > kMaxAug || P.l > kMaxAug || P.q < 0 || P.q > kMaxForce || !P.work)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_s_substep, dim3(1), dim3(256), 0, st, P);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// Symmetric k x k eigen-decomposition (k <= 128) as the one-sided Jacobi SVD of
+// M + sigma I with sigma = max_i sum_j |M_ij| >= spectral radius, so the
+// shifted matrix is positive semidefinite: its right singular vectors are
+// eigenvectors of M and its singular values minus sigma the eigenvalues.
+// ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_shift_sym(const double* __restrict__ M, int ldm, int k,
+                                                   double* __restrict__ out, double* __restrict__ shift) {
+  __shared__ double s_max[256];
+  const int t = threadIdx.x;
+  double mx = 0.0;
+  for (int i = t; i < k; i += blockDim.x) {
+    double s = 0.0;
+    for (int jj = 0; jj < k; ++jj) s += fabs(M[i + jj * ldm]);
+    mx = fmax(mx, s);
+  }
+  s_max[t] = mx;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (t < w) s_max[t] = fmax(s_max[t], s_max[t + w]);
+    __syncthreads();
+  }
+  const double sg = s_max[0];
+  for (int e = t; e < k * k; e += blockDim.x) {
+    const int i = e % k, jj = e / k;
+    // symmetrised: the Galerkin products are symmetric up to rounding
+    const double v = 0.5 * (M[i + jj * ldm] + M[jj + i * ldm]);
+    out[e] = v + (i == jj ? sg : 0.0);
+  }
+  if (t == 0) *shift = sg;
+}
+
+__global__ void k_unshift(double* __restrict__ ev, int k, const double* __restrict__ shift) {
+  for (int i = threadIdx.x; i < k; i += blockDim.x) ev[i] -= *shift;
+}
+
+hipError_t launch_sym_eig(const double* M, int ldm, int k, double* W, double* ev, double* work, double* Uscr,
+                          double* svd_work, int* r_scr, hipStream_t st) {
+  if (k < 1 || k > kMaxAug) return hipErrorInvalidValue;
+  double* shifted = work;
+  double* shift = work + (size_t)k * k;
+  hipLaunchKernelGGL(k_shift_sym, dim3(1), dim3(256), 0, st, M, ldm, k, shifted, shift);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  SvdArgs sv;
+  sv.S = shifted;
+  sv.lds = k;
+  sv.k = k;
+  sv.U = Uscr;
+  sv.V = W;
+  sv.sigma = ev;
+  sv.r_out = r_scr;
+  sv.tol = 0.0;
+  sv.rmax = k;
+  sv.work = svd_work;
+  e = launch_jacobi_svd(sv, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_unshift, dim3(1), dim3(128), 0, st, ev, k, (const double*)shift);
+  return hipGetLastError();
+}
+
+}  // namespace lri

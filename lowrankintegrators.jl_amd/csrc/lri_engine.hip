@@ -5,6 +5,7 @@
 // sequence of stream-ordered launches with no host synchronisation (except the
 // rank read-back of the rank-adaptive step).
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -52,6 +53,25 @@ struct Timed {
   hipEvent_t a, b;
 };
 
+// small FP64 matrices of a matrix-ODE step, device-resident (k <= 2 rcap)
+struct DeSmall {
+  double Ga[kMaxAug * kMaxAug];  // U'AU (n side)
+  double Gb[kMaxAug * kMaxAug];  // V'BV (m side)
+  double Wa[kMaxAug * kMaxAug];  // eigenvectors of Ga
+  double Wb[kMaxAug * kMaxAug];
+  double alpha[kMaxAug];         // eigenvalues of Ga
+  double beta[kMaxAug];
+  double gaT[kMaxForce * kMaxAug];  // G'U (q x k)
+  double hbT[kMaxForce * kMaxAug];  // H'V (q x k)
+  double CW[kMaxAug * kMaxAug];
+  double coef[kMaxForce * kMaxAug];
+  double Snew[kMaxAug * kMaxAug];
+  double ework[kMaxAug * kMaxAug + 2];
+  double Uscr[kMaxAug * kMaxAug];
+  double swork[2 * kMaxAug * kMaxAug];
+  int rscr;
+};
+
 }  // namespace
 
 struct lri_ctx {
@@ -87,11 +107,19 @@ struct lri_ctx {
   int64_t sld = 0;
   // timing
   bool timing = false;
+  bool timing_passes_only = false;
   std::vector<Timed> pending;
   std::vector<hipEvent_t> pool;
-  double tot_ms[LRI_KERNEL_COUNT] = {0, 0, 0};
-  int64_t launches[LRI_KERNEL_COUNT] = {0, 0, 0};
+  double tot_ms[LRI_KERNEL_COUNT] = {};
+  int64_t launches[LRI_KERNEL_COUNT] = {};
   std::string err;
+  // matrix-ODE operator F(Y) = A Y + Y B' + G H' (allocated on first use)
+  bool op_set = false;
+  double *op_ad = nullptr, *op_ae = nullptr, *op_bd = nullptr, *op_be = nullptr;
+  double *op_G = nullptr, *op_H = nullptr;
+  int op_aper = 0, op_bper = 0, op_q = 0;
+  DeSmall* de = nullptr;
+  bool pack_enabled = true;
 
   // ------------------------------------------------------------------ helpers
   int pass_mode() const {
@@ -123,7 +151,7 @@ struct lri_ctx {
   }
   template <class F>
   void timed(int family, F&& f) {
-    if (!timing) {
+    if (!timing || (timing_passes_only && family != LRI_KERNEL_PASS_N && family != LRI_KERNEL_PASS_T)) {
       HIPCHECK(f());
       return;
     }
@@ -168,6 +196,7 @@ struct lri_ctx {
   }
   template <typename PT>
   void set_pack_outputs(RowArgs<PT>& A, bool forN, int mode, int rb) {
+    if (!pack_enabled) return;
     A.pk_cols = rb;
     if (mode == PASS_F64) {
       A.pk_f64 = forN ? pkN_f64 : pkT_f64;
@@ -617,6 +646,12 @@ struct lri_ctx {
       A.epi.kind = EPI_RA_S;
       rows(A);
     });
+    return ra_finish(k2, tol, rmax);
+  }
+
+  // SVD of the 2r x 2r core small->Shat, truncation, and the rotation of Uhat,
+  // Vhat into the new state (rank_adaptive_unconventional.jl:222-231).
+  int64_t ra_finish(int k2, double tol, int64_t rmax) {
     SvdArgs sv;
     sv.S = small->Shat;
     sv.lds = k2;
@@ -652,6 +687,228 @@ struct lri_ctx {
     HIPCHECK(hipGetLastError());
     r = r1;
     return r1;
+  }
+
+  // ------------------------------------------------------------ matrix ODE
+  // Galerkin data of an orthonormal basis X (rows x k) on side A (n rows,
+  // nside) or B (m rows): Cg = X' T X (k x k) and FXT = F' X (q x k), with T the
+  // side's tridiagonal operator and F its forcing factor (G or H).
+  void de_galerkin(const double* X, bool nside, int k, double* Cg, double* FXT) {
+    const int64_t rows_ = nside ? n : m;
+    RowArgs<double> A;
+    A.rows = rows_;
+    A.r_out = k;
+    A.base = X;
+    A.ld_base = rows_;
+    A.st_d = nside ? op_ad : op_bd;
+    A.st_e = nside ? op_ae : op_be;
+    A.st_periodic = nside ? op_aper : op_bper;
+    A.reduce = 1;
+    A.W = X;
+    A.ldw = rows_;
+    A.rw = k;
+    A.epi.kind = EPI_COPY;
+    A.epi.dst = Cg;
+    rows(A);
+    if (op_q > 0) {
+      RowArgs<double> B;
+      B.rows = rows_;
+      B.r_out = op_q;
+      B.base = nside ? op_G : op_H;
+      B.ld_base = rows_;
+      B.reduce = 1;
+      B.W = X;
+      B.ldw = rows_;
+      B.rw = k;
+      B.epi.kind = EPI_COPY;
+      B.epi.dst = FXT;
+      rows(B);
+    }
+  }
+  void de_eig(const double* Cg, int k, double* W, double* ev) {
+    timed(LRI_KERNEL_SMALL,
+          [&] { return launch_sym_eig(Cg, k, k, W, ev, de->ework, de->Uscr, svd_work, &de->rscr, stream); });
+  }
+  // One K-type sub-problem on side `nside` (rows n, operator A, forcing G; or
+  // rows m, operator B, forcing H):  Z' = T Z + Z Co + F FXo,  Z0 = X0 op(C0),
+  // where Co = Wo diag(ev) Wo' is the other side's Galerkin matrix and FXo its
+  // forcing projection (q x k).  Writes Z(t + dt) to `out` with the reduction
+  // `epi` on its Gram matrix (EPI_NONE: no reduction).
+  void de_substep(const double* X0, bool nside, int k, const double* C0, int ldc, int transC, const double* Wo,
+                  const double* ev, const double* FXo, double h, int64_t nsub, double* out, double* scratch,
+                  EpiArgs epi) {
+    const int64_t rows_ = nside ? n : m;
+    timed(LRI_KERNEL_SMALL,
+          [&] { return launch_de_prep(C0, ldc, transC, Wo, FXo, k, op_q, de->CW, de->coef, stream); });
+    RowArgs<double> A;  // Zbar0 = X0 op(C0) Wo
+    A.rows = rows_;
+    A.r_out = k;
+    A.r_in = k;
+    A.X = X0;
+    A.ldx = rows_;
+    A.Y = de->CW;
+    A.ldy = k;
+    A.out = scratch;
+    A.ldo = rows_;
+    rows(A);
+    RK4ColArgs P;
+    P.n = rows_;
+    P.ncols = k;
+    P.X = scratch;
+    P.ldx = rows_;
+    P.Y = scratch;
+    P.ldy = rows_;
+    P.st_d = nside ? op_ad : op_bd;
+    P.st_e = nside ? op_ae : op_be;
+    P.periodic = nside ? op_aper : op_bper;
+    P.mu = ev;
+    P.g = nside ? op_G : op_H;
+    P.ldg = rows_;
+    P.coef = de->coef;
+    P.q = op_q;
+    P.h = h;
+    P.nsub = nsub;
+    timed(LRI_KERNEL_DE, [&] { return launch_rk4_columns(P, stream); });
+    RowArgs<double> B;  // Z = Zbar Wo'
+    B.rows = rows_;
+    B.r_out = k;
+    B.r_in = k;
+    B.X = scratch;
+    B.ldx = rows_;
+    B.Y = Wo;
+    B.ldy = k;
+    B.transY = 1;
+    B.out = out;
+    B.ldo = rows_;
+    if (epi.kind != EPI_NONE) {
+      B.reduce = 1;
+      B.epi = epi;
+    }
+    rows(B);
+  }
+  void de_sstep(int k, int l, const double* S0, int lds0, int transS0, double* S1, int lds1, double sgn, double h,
+                int64_t nsub) {
+    SStepArgs P;
+    P.k = k;
+    P.l = l;
+    P.q = op_q;
+    P.S0 = S0;
+    P.lds0 = lds0;
+    P.transS0 = transS0;
+    P.S1 = S1;
+    P.lds1 = lds1;
+    P.Wa = de->Wa;
+    P.alpha = de->alpha;
+    P.Wb = de->Wb;
+    P.beta = de->beta;
+    P.gaT = de->gaT;
+    P.hbT = de->hbT;
+    P.sgn = sgn;
+    P.h = h;
+    P.nsub = nsub;
+    P.work = de->swork;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_s_substep(P, stream); });
+  }
+
+  // primal_LT_step! with the DE sub-problems (projector_splitting.jl:52-80,129-152)
+  void de_ksl_primal(double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    EpiArgs chol, none;
+    chol.kind = EPI_CHOL1;
+    de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
+    de_eig(de->Gb, rr, de->Wb, de->beta);
+    de_substep(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, chol);  // K
+    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 0);                                   // U1, R
+    de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U1'AU1
+    de_eig(de->Ga, rr, de->Wa, de->alpha);
+    de_sstep(rr, rr, small->Rtot, rr, 0, de->Snew, rr, -1.0, h, nsub);  // S~
+    de_substep(V, false, rr, de->Snew, rr, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, chol);  // L
+    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 2);  // V1, S = R'
+  }
+
+  // dual_LT_step! with the DE sub-problems (projector_splitting.jl:52-80,166-189)
+  void de_ksl_dual(double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    EpiArgs chol, none;
+    chol.kind = EPI_CHOL1;
+    de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U0'AU0
+    de_eig(de->Ga, rr, de->Wa, de->alpha);
+    de_substep(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, chol);  // L
+    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 0);                                     // V1, R
+    de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V1'BV1
+    de_eig(de->Gb, rr, de->Wb, de->beta);
+    de_sstep(rr, rr, small->Rtot, rr, 1, de->Snew, rr, -1.0, h, nsub);  // S~ from R'
+    de_substep(U, true, rr, de->Snew, rr, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, chol);  // K
+    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 1);  // U1, S = R
+  }
+
+  // unconventional_step! with the DE sub-problems (unconventional.jl:51-79,133-157)
+  void de_bug(double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    EpiArgs chol;
+    chol.kind = EPI_CHOL1;
+    de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
+    de_eig(de->Gb, rr, de->Wb, de->beta);
+    de_substep(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, chol);
+    EpiArgs eM;
+    eM.kind = EPI_BUG_M;  // M = U1' U0
+    cholqr_tail(K, K2, n, rr, U2, false, U, rr, eM, 0);
+    de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U0'AU0
+    de_eig(de->Ga, rr, de->Wa, de->alpha);
+    de_substep(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, chol);
+    EpiArgs eN;
+    eN.kind = EPI_BUG_N;  // N = V1' V0, Sinit = M S0 N'
+    eN.S = S;
+    eN.ld_s = (int)rcap;
+    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0);
+    de_galerkin(U2, true, rr, de->Ga, de->gaT);  // At = U1'AU1
+    de_eig(de->Ga, rr, de->Wa, de->alpha);
+    de_galerkin(V2, false, rr, de->Gb, de->hbT);  // Bt = V1'BV1
+    de_eig(de->Gb, rr, de->Wb, de->beta);
+    de_sstep(rr, rr, small->Sinit, rr, 0, S, (int)rcap, 1.0, h, nsub);
+    std::swap(U, U2);
+    std::swap(V, V2);
+  }
+
+  // rankadaptive_unconventional_step! with the DE sub-problems
+  // (rank_adaptive_unconventional.jl:59-86,194-233)
+  int64_t de_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax) {
+    const int rr = (int)r;
+    const int k2 = 2 * rr;
+    const double h = dt / (double)nsub;
+    EpiArgs none;
+    de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
+    de_eig(de->Gb, rr, de->Wb, de->beta);
+    de_substep(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, none);
+    de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U0'AU0
+    de_eig(de->Ga, rr, de->Wa, de->alpha);
+    de_substep(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, none);
+    HIPCHECK(hipMemcpyAsync(Uhat, U, sizeof(double) * n * rr, hipMemcpyDeviceToDevice, stream));
+    EpiArgs eM;
+    eM.kind = EPI_RA_M;
+    complete_basis(K, U, n, rr, K2, K2 + n * rcap, Uhat + n * rr, eM);
+    HIPCHECK(hipMemcpyAsync(Vhat, V, sizeof(double) * m * rr, hipMemcpyDeviceToDevice, stream));
+    EpiArgs eN;
+    eN.kind = EPI_RA_N;  // Sinit = M S0 N' (2r x 2r)
+    eN.S = S;
+    eN.ld_s = (int)rcap;
+    complete_basis(L, V, m, rr, L2, L2 + m * rcap, Vhat + m * rr, eN);
+    de_galerkin(Uhat, true, k2, de->Ga, de->gaT);
+    de_eig(de->Ga, k2, de->Wa, de->alpha);
+    de_galerkin(Vhat, false, k2, de->Gb, de->hbT);
+    de_eig(de->Gb, k2, de->Wb, de->beta);
+    de_sstep(k2, k2, small->Sinit, k2, 0, small->Shat, k2, 1.0, h, nsub);
+    return ra_finish(k2, tol, rmax);
+  }
+
+  void check_de(double dt, int64_t nsub) {
+    REQUIRE(r >= 1, "state not set (call lri_set_state)");
+    REQUIRE(op_set, "no operator (call lri_set_operator_lyapunov)");
+    REQUIRE(dt > 0.0 && std::isfinite(dt), "dt must be positive and finite");
+    REQUIRE(nsub >= 1, "substeps must be >= 1");
   }
 
   int read_flags() {
@@ -698,7 +955,8 @@ void free_ctx(lri_ctx* c) {
   void* bufs[] = {c->U,      c->V,      c->S,      c->U2,      c->V2,      c->K,      c->K2,     c->P,
                   c->L,      c->L2,     c->Q,      c->Uhat,    c->Vhat,    c->pkN_f64, c->pkT_f64, c->pkN_f32,
                   c->pkT_f32, c->pkN_hi, c->pkN_lo, c->pkT_hi,  c->pkT_lo,  c->partN,  c->partT,  c->gpart,
-                  c->flags,  c->small,  c->svd_work};
+                  c->flags,  c->small,  c->svd_work, c->op_ad,  c->op_ae,   c->op_bd,   c->op_be,  c->op_G,
+                  c->op_H,   c->de};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   for (auto& t : c->pending) {
@@ -973,6 +1231,108 @@ int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, in
   });
 }
 
+namespace {
+
+// A matrix-ODE step does not pack the data-path panels, and it leaves the ones
+// packed before stale.
+struct DeScope {
+  lri_ctx* c;
+  explicit DeScope(lri_ctx* c_) : c(c_) { c->pack_enabled = false; }
+  ~DeScope() {
+    c->pack_enabled = true;
+    c->pkN_mode = c->pkT_mode = -1;
+  }
+};
+
+}  // namespace
+
+int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* a_off, int a_periodic,
+                              const double* b_diag, const double* b_off, int b_periodic, int64_t q,
+                              const double* G, int64_t ldg, const double* H, int64_t ldh) {
+  return guarded(ctx, [&] {
+    REQUIRE(a_diag && b_diag, "a_diag and b_diag are required");
+    REQUIRE((a_off || ctx->n == 1) && (b_off || ctx->m == 1), "a_off and b_off are required");
+    REQUIRE(q >= 0 && q <= LRI_MAX_FORCE, "forcing rank q must be in [0, LRI_MAX_FORCE]");
+    REQUIRE(q == 0 || (G && H && ldg >= ctx->n && ldh >= ctx->m), "G, H or their leading dimensions");
+    REQUIRE(!a_periodic || ctx->n >= 3, "a periodic operator needs n >= 3");
+    REQUIRE(!b_periodic || ctx->m >= 3, "a periodic operator needs m >= 3");
+    REQUIRE(ctx->n <= LRI_DE_MAX_ROWS && ctx->m <= LRI_DE_MAX_ROWS, "matrix ODE needs n, m <= LRI_DE_MAX_ROWS");
+    const size_t d = sizeof(double);
+    const int64_t n = ctx->n, m = ctx->m;
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    auto alloc = [&](double** p, size_t count) {
+      if (!*p) HIPCHECK(hipMalloc((void**)p, d * count));
+      HIPCHECK(hipMemset(*p, 0, d * count));
+    };
+    alloc(&ctx->op_ad, n);
+    alloc(&ctx->op_ae, n);
+    alloc(&ctx->op_bd, m);
+    alloc(&ctx->op_be, m);
+    alloc(&ctx->op_G, n * LRI_MAX_FORCE);
+    alloc(&ctx->op_H, m * LRI_MAX_FORCE);
+    if (!ctx->de) {
+      HIPCHECK(hipMalloc((void**)&ctx->de, sizeof(DeSmall)));
+      HIPCHECK(hipMemset(ctx->de, 0, sizeof(DeSmall)));
+    }
+    HIPCHECK(hipMemcpy(ctx->op_ad, a_diag, d * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(ctx->op_bd, b_diag, d * m, hipMemcpyHostToDevice));
+    // a_off: n entries when periodic, n - 1 otherwise (the last stays zero)
+    const int64_t na = a_periodic ? n : n - 1, nb = b_periodic ? m : m - 1;
+    if (na > 0) HIPCHECK(hipMemcpy(ctx->op_ae, a_off, d * na, hipMemcpyHostToDevice));
+    if (nb > 0) HIPCHECK(hipMemcpy(ctx->op_be, b_off, d * nb, hipMemcpyHostToDevice));
+    if (q > 0) {
+      HIPCHECK(hipMemcpy2D(ctx->op_G, d * n, G, d * ldg, d * n, q, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy2D(ctx->op_H, d * m, H, d * ldh, d * m, q, hipMemcpyHostToDevice));
+    }
+    ctx->op_aper = a_periodic != 0;
+    ctx->op_bper = b_periodic != 0;
+    ctx->op_q = (int)q;
+    ctx->op_set = true;
+    return LRI_OK;
+  });
+}
+
+int lri_step_de_ksl(lri_ctx* ctx, int order, double dt, int64_t substeps) {
+  return guarded(ctx, [&] {
+    REQUIRE(order == LRI_PRIMAL || order == LRI_DUAL || order == LRI_STRANG, "unknown order");
+    ctx->check_de(dt, substeps);
+    DeScope scope(ctx);
+    if (order == LRI_PRIMAL) {
+      ctx->de_ksl_primal(dt, substeps);
+    } else if (order == LRI_DUAL) {
+      ctx->de_ksl_dual(dt, substeps);
+    } else {
+      ctx->de_ksl_primal(0.5 * dt, substeps);
+      ctx->de_ksl_dual(0.5 * dt, substeps);
+    }
+    return LRI_OK;
+  });
+}
+
+int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps) {
+  return guarded(ctx, [&] {
+    ctx->check_de(dt, substeps);
+    DeScope scope(ctx);
+    ctx->de_bug(dt, substeps);
+    return LRI_OK;
+  });
+}
+
+int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, int64_t rmax, int64_t* r_new) {
+  return guarded(ctx, [&] {
+    ctx->check_de(dt, substeps);
+    REQUIRE(tol >= 0.0, "tol must be >= 0");
+    REQUIRE(rmax >= 1, "rmax must be >= 1");
+    int64_t r1 = 0;
+    {
+      DeScope scope(ctx);
+      r1 = ctx->de_bug_ra(dt, substeps, tol, rmax);
+    }
+    if (r_new) *r_new = r1;
+    return ctx->read_flags();
+  });
+}
+
 int lri_synchronize(lri_ctx* ctx) {
   return guarded(ctx, [&] { return ctx->read_flags(); });
 }
@@ -985,11 +1345,13 @@ int lri_get_stream(lri_ctx* ctx, void** stream) {
   });
 }
 
-int lri_kernel_timing(lri_ctx* ctx, int enable) {
+int lri_kernel_timing(lri_ctx* ctx, int mode) {
   return guarded(ctx, [&] {
+    REQUIRE(mode >= 0 && mode <= 2, "timing mode must be 0, 1 or 2");
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     ctx->drain_timing();
-    ctx->timing = enable != 0;
+    ctx->timing = mode != 0;
+    ctx->timing_passes_only = mode == 2;
     for (int f = 0; f < LRI_KERNEL_COUNT; ++f) {
       ctx->tot_ms[f] = 0.0;
       ctx->launches[f] = 0;

@@ -38,6 +38,7 @@ enum EpiKind {
   EPI_RA_N,    // N = [I; C], Sinit = M S0 N'
   EPI_RA_S,    // Shat = Sinit + C'
   EPI_STORE,   // Ctmp = C
+  EPI_COPY,    // dst = C (a x b, ld a)
 };
 
 struct EpiArgs {
@@ -48,6 +49,7 @@ struct EpiArgs {
   int* flags = nullptr;
   double* S = nullptr;          // state S (EPI_SFINAL / EPI_BUG_* / EPI_RA_N)
   int ld_s = 0;
+  double* dst = nullptr;        // EPI_COPY
 };
 
 // Row kernel (see lri_rowops.hip).
@@ -60,10 +62,15 @@ struct RowArgs {
   int nsplit = 0, rb = 0;
   double* sum_out = nullptr;
   int64_t ld_sum = 0;
-  // + base_sign * base
+  // + base_sign * op(base), op = identity or the symmetric tridiagonal stencil
+  // (st_d[i] on the diagonal, st_e[i] coupling rows i and i+1, st_e[rows-1]
+  // coupling the last and first row when st_periodic)
   const double* base = nullptr;
   int64_t ld_base = 0;
   double base_sign = 1.0;
+  const double* st_d = nullptr;
+  const double* st_e = nullptr;
+  int st_periodic = 0;
   // + xy_sign * X * op(Y)
   const double* X = nullptr;
   int64_t ldx = 0;
@@ -137,6 +144,60 @@ hipError_t launch_passN(int mode, const PassArgs& a, hipStream_t st);
 hipError_t launch_passT(int mode, const PassArgs& a, hipStream_t st);
 
 hipError_t selftest_mfma(int which, const void* a, const void* b, void* out);
+
+// Matrix-ODE sub-steps (see lri_de.hip).  The forcing has rank q <= kMaxForce.
+constexpr int kMaxForce = 4;
+
+// y_j' = (T + mu_j I) y_j + sum_k g_k coef[k + j q], T symmetric tridiagonal
+struct RK4ColArgs {
+  int64_t n = 0;
+  int ncols = 0;
+  const double* X = nullptr;  // initial columns, n x ncols (ld ldx)
+  int64_t ldx = 0;
+  double* Y = nullptr;        // result (may alias X)
+  int64_t ldy = 0;
+  const double* st_d = nullptr;
+  const double* st_e = nullptr;
+  int periodic = 0;
+  const double* mu = nullptr;    // ncols
+  const double* g = nullptr;     // n x q (ld ldg)
+  int64_t ldg = 0;
+  const double* coef = nullptr;  // q x ncols
+  int q = 0;
+  double h = 0.0;
+  int64_t nsub = 0;
+};
+hipError_t launch_rk4_columns(const RK4ColArgs& P, hipStream_t st);
+
+// Small preparation for a rotated sub-problem: CW = op(C) W (k x k), coef = FX W (q x k).
+hipError_t launch_de_prep(const double* C, int ldc, int transC, const double* W, const double* FX, int k, int q,
+                          double* CW, double* coef, hipStream_t st);
+
+// S' = sgn (At S + S Bt + ga hb') in the eigenbases At = Wa diag(alpha) Wa', Bt = Wb diag(beta) Wb'
+struct SStepArgs {
+  int k = 0, l = 0, q = 0;
+  const double* S0 = nullptr;
+  int lds0 = 0;
+  int transS0 = 0;              // start from S0'
+  double* S1 = nullptr;
+  int lds1 = 0;
+  const double* Wa = nullptr;   // k x k
+  const double* alpha = nullptr;
+  const double* Wb = nullptr;   // l x l
+  const double* beta = nullptr;
+  const double* gaT = nullptr;  // q x k: (G' U)
+  const double* hbT = nullptr;  // q x l: (H' V)
+  double sgn = 1.0;
+  double h = 0.0;
+  int64_t nsub = 0;
+  double* work = nullptr;       // 2 k l doubles
+};
+hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st);
+
+// Symmetric eigen-decomposition M = W diag(ev) W' of a k x k matrix (k <= kMaxAug).
+// work: k*k + 2 doubles; svd_work: kMaxAug^2 doubles; Uscr: k*k doubles; r_scr: one int.
+hipError_t launch_sym_eig(const double* M, int ldm, int k, double* W, double* ev, double* work, double* Uscr,
+                          double* svd_work, int* r_scr, hipStream_t st);
 
 // Element (row i, column c) of a BF16 hi/lo panel with rb columns, in the
 // order the passes load their B fragments: pass N reads 8 consecutive rows

@@ -39,19 +39,23 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
   return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
-// Upper Cholesky G = R'R (k <= K <= 64) and R^{-1}, computed by wave 0 alone
-// with every column in registers: lane j holds column j of the trailing
-// matrix in a window g[t] = G(p + t, j) that shifts by one row per pivot, so
-// all register indices are static; pivots and multipliers are broadcast with
-// readlane.  A pivot <= tau = 10 eps max_i G(i,i) marks column p as
+// Upper Cholesky G = R'R (k <= K <= 32) and R^{-1}, computed by wave 0 alone
+// with every column in registers.  Lane j holds column j of the trailing
+// matrix, g[t] = G(t, j), and column j of B = L^{-1} (L = R', B starts as I),
+// w[t]; the pivot loop is unrolled over K, so every register index is static
+// and step p touches only rows t > p.  One right-looking sweep yields R and
+// R^{-T} together, with no back substitution.  The pivot comes by readlane; the
+// multipliers R(p, t) are broadcast through an LDS row (double-buffered, and
+// zero past k).  A pivot <= tau = 10 eps max_i G(i,i) marks column p as
 // numerically dependent: dropped[p] = 1, R(p,p) = 1, row p of R is zero and
-// column p of R^{-1} is zero (so Q gets a zero column there).
+// row and column p of R^{-1} are zero (so Q gets a zero column there).
 // R and Rinv are written to LDS (column-major, ld k).  Returns the number of
 // dropped columns; sets *nonfinite on NaN/Inf.  Called by all threads.
 template <int K>
-__device__ int chol_inv_wave(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv, int k,
-                             unsigned char* dropped, int* nonfinite) {
+__device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv,
+                                 int k, unsigned char* dropped, int* nonfinite) {
   __shared__ int s_nd, s_nf;
+  __shared__ double s_r[2][64];  // R(p, c) by column c
   if (threadIdx.x < 64) {
     const int j = threadIdx.x;
     double md = 0.0;
@@ -66,50 +70,43 @@ __device__ int chol_inv_wave(const double* __restrict__ G, double* __restrict__ 
       nf |= __shfl_xor(nf, off);
     }
     const double tau = 10.0 * kEps * md;
-    double g[K];
+    double g[K], w[K];
 #pragma unroll
-    for (int t = 0; t < K; ++t) g[t] = (t < k && j < k && t <= j) ? G[t + j * k] : 0.0;
+    for (int t = 0; t < K; ++t) {
+      g[t] = (t < k && j < k && t <= j) ? G[t + j * k] : 0.0;
+      w[t] = (t == j && j < k) ? 1.0 : 0.0;
+    }
     int nd = 0;
-    for (int p = 0; p < k; ++p) {
-      const double d = readlane_f64(g[0], p);
-      const bool fin = isfinite(d);
-      const bool keep = (d > tau) && fin;
-      nf |= !fin;
-      const double piv = keep ? sqrt(d) : 1.0;
-      const double inv = 1.0 / piv;
-      nd += keep ? 0 : 1;
-      const double rpj = (keep && j > p && j < k) ? g[0] * inv : 0.0;
-      if (j < k) R[p + j * k] = (j == p) ? piv : (j > p ? rpj : 0.0);
-      if (j == p) dropped[p] = keep ? 0 : 1;
+    bool mydrop = false;
 #pragma unroll
-      for (int t = 1; t < K; ++t) {
-        if (p + t < k) {
-          const double rpi = readlane_f64(rpj, p + t);
-          if (p + t <= j) g[t - 1] = g[t] - rpi * rpj;
-          else g[t - 1] = g[t];
-        } else {
-          g[t - 1] = g[t];
+    for (int p = 0; p < K; ++p) {
+      if (p < k) {
+        const double d = readlane_f64(g[p], p);
+        const bool fin = isfinite(d);
+        const bool keep = (d > tau) && fin;
+        nf |= !fin;
+        const double piv = keep ? sqrt(d) : 1.0;
+        const double inv = 1.0 / piv;
+        nd += keep ? 0 : 1;
+        mydrop = mydrop || (j == p && !keep);
+        const double rpj = (keep && j > p && j < k) ? g[p] * inv : 0.0;  // R(p, j)
+        const double xpj = (keep && j < k) ? w[p] * inv : 0.0;           // (L^{-1})(p, j) = R^{-1}(j, p)
+        if (j < k) {
+          R[p + j * k] = (j == p) ? piv : (j > p ? rpj : 0.0);
+          Rinv[j + p * k] = mydrop ? 0.0 : xpj;
+        }
+        if (j == p) dropped[p] = keep ? 0 : 1;
+        s_r[p & 1][j] = rpj;
+        wave_lds_sync();
+        // rows t > p of column j: G -= R(p,t) R(p,j), B -= R(p,t) (L^{-1})(p,j)
+        // (entries of g below the diagonal are never read, so they need no mask)
+#pragma unroll
+        for (int t = p + 1; t < K; ++t) {
+          const double rpt = s_r[p & 1][t];
+          g[t] -= rpt * rpj;
+          w[t] -= rpt * xpj;
         }
       }
-      g[K - 1] = 0.0;
-    }
-    wave_lds_sync();
-    // R^{-1}, column j: back substitution over i = k-1 .. 0 with the known
-    // entries x(i+1+t) in the window y[t] (shifted right by one per row)
-    double y[K];
-#pragma unroll
-    for (int t = 0; t < K; ++t) y[t] = 0.0;
-    for (int i = k - 1; i >= 0; --i) {
-      double sacc = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-      for (int t = 0; t < K; ++t)
-        if (i + 1 + t < k) sacc -= R[i + (i + 1 + t) * k] * y[t];
-      const bool di = dropped[i] != 0;
-      const double xi = (di || j >= k || (j < k && dropped[j]) || i > j) ? 0.0 : sacc / R[i + i * k];
-      if (j < k) Rinv[i + j * k] = xi;
-#pragma unroll
-      for (int t = K - 1; t > 0; --t) y[t] = y[t - 1];
-      y[0] = xi;
     }
     if (j == 0) {
       s_nd = nd;
@@ -122,10 +119,86 @@ __device__ int chol_inv_wave(const double* __restrict__ G, double* __restrict__ 
   return s_nd;
 }
 
+// The same sweep for K = 64 with a runtime pivot loop (full unrolling would
+// not fit the register file): the windows g[t] = G(p + t, j) and w[t] shift by
+// one row per pivot so that register indices stay static, and the multipliers
+// are read from a zeroed LDS row of 2K entries by absolute column.
+template <int K>
+__device__ int chol_inv_window(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv,
+                               int k, unsigned char* dropped, int* nonfinite) {
+  __shared__ int s_nd, s_nf;
+  __shared__ double s_r[2 * K];  // R(p, c) by column c; [k, 2K) stays zero
+  if (threadIdx.x < 64) {
+    const int j = threadIdx.x;
+    for (int e = j; e < 2 * K; e += 64) s_r[e] = 0.0;
+    double md = 0.0;
+    int nf = 0;
+    if (j < k) {
+      const double dj = G[j + j * k];
+      nf = !isfinite(dj);
+      md = dj;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      md = fmax(md, __shfl_xor(md, off));
+      nf |= __shfl_xor(nf, off);
+    }
+    const double tau = 10.0 * kEps * md;
+    double g[K], w[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      g[t] = (t < k && j < k && t <= j) ? G[t + j * k] : 0.0;
+      w[t] = (t == j && j < k) ? 1.0 : 0.0;
+    }
+    int nd = 0;
+    bool mydrop = false;
+    wave_lds_sync();
+    for (int p = 0; p < k; ++p) {
+      const double d = readlane_f64(g[0], p);
+      const bool fin = isfinite(d);
+      const bool keep = (d > tau) && fin;
+      nf |= !fin;
+      const double piv = keep ? sqrt(d) : 1.0;
+      const double inv = 1.0 / piv;
+      nd += keep ? 0 : 1;
+      mydrop = mydrop || (j == p && !keep);
+      const double rpj = (keep && j > p && j < k) ? g[0] * inv : 0.0;
+      const double xpj = (keep && j < k) ? w[0] * inv : 0.0;
+      if (j < k) {
+        R[p + j * k] = (j == p) ? piv : (j > p ? rpj : 0.0);
+        Rinv[j + p * k] = mydrop ? 0.0 : xpj;
+      }
+      if (j == p) dropped[p] = keep ? 0 : 1;
+      if (j < 2 * K) s_r[j] = rpj;
+      wave_lds_sync();
+#pragma unroll
+      for (int t = 1; t < K; ++t) {
+        const double rpi = s_r[p + t];
+        g[t - 1] = g[t] - rpi * rpj;
+        w[t - 1] = w[t] - rpi * xpj;
+      }
+      g[K - 1] = 0.0;
+      w[K - 1] = 0.0;
+      wave_lds_sync();
+    }
+    if (j == 0) {
+      s_nd = nd;
+      s_nf = nf;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_nf) *nonfinite = 1;
+  __syncthreads();
+  return s_nd;
+}
+
+// CK: the Cholesky variant compiled into an epilogue kernel (16, 32: unrolled
+// for k <= CK; 64: runtime loop).  Each kernel instance carries one variant,
+// so the register allocation of one does not burden the others.
+template <int CK>
 __device__ int chol_inv(const double* G, double* R, double* Rinv, int k, unsigned char* dropped, int* nonfinite) {
-  if (k <= 16) return chol_inv_wave<16>(G, R, Rinv, k, dropped, nonfinite);
-  if (k <= 32) return chol_inv_wave<32>(G, R, Rinv, k, dropped, nonfinite);
-  return chol_inv_wave<64>(G, R, Rinv, k, dropped, nonfinite);
+  if (CK == 16) return chol_inv_unrolled<16>(G, R, Rinv, k, dropped, nonfinite);
+  if (CK == 32) return chol_inv_unrolled<32>(G, R, Rinv, k, dropped, nonfinite);
+  return chol_inv_window<64>(G, R, Rinv, k, dropped, nonfinite);
 }
 
 // C = op(A) * op(B), k x k, column-major.
@@ -158,6 +231,7 @@ __host__ __device__ inline int epi_scratch_doubles(int kind, int a, int b) {
 // epilogues run by the last workgroup on the reduced C (LDS, a x b)
 // ----------------------------------------------------------------------------
 
+template <int CK>
 __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* scr) {
   const int t = threadIdx.x;
   SmallState* st = E.small;
@@ -190,14 +264,16 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
         pert = (s_big == 0);
       }
       if (pert) {
-        double* F1 = Rinv;  // scratch: F1, then F1'F1 into R
+        double* F1 = Rinv;  // scratch: F1 and F1' (in C), then F1'F1 into R
         for (int e = t; e < k * k; e += blockDim.x) {
           const int i = e % k, jj = e / k;
           const double v = C[e] - (i == jj ? 1.0 : 0.0);
           F1[e] = i < jj ? v : (i == jj ? 0.5 * v : 0.0);
         }
         __syncthreads();
-        small_gemm(F1, 1, F1, 0, R, k);  // R <- F1' F1
+        for (int e = t; e < k * k; e += blockDim.x) C[e] = F1[(e / k) + (e % k) * k];
+        __syncthreads();
+        small_gemm(C, 0, F1, 0, R, k);  // R <- F1' F1
         __syncthreads();
         for (int e = t; e < k * k; e += blockDim.x) {  // C <- F = F1 + upper(-F1'F1) (diag halved)
           const int i = e % k, jj = e / k;
@@ -216,7 +292,7 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
         if (t < k) dropped[t] = 0;
         __syncthreads();
       } else {
-        nd = chol_inv(C, R, Rinv, k, dropped, &nf);
+        nd = chol_inv<CK>(C, R, Rinv, k, dropped, &nf);
       }
       if (E.kind == EPI_CHOL1) {
         for (int e = t; e < k * k; e += blockDim.x) {
@@ -321,6 +397,10 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
     }
     case EPI_STORE: {
       for (int e = t; e < a * b; e += blockDim.x) st->Ctmp[e] = C[e];
+      break;
+    }
+    case EPI_COPY: {
+      for (int e = t; e < a * b; e += blockDim.x) E.dst[e] = C[e];
       break;
     }
     case EPI_RA_S: {
@@ -463,13 +543,14 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ 
 }
 
 // One workgroup: C (a x b, from k_reduce) into LDS, then the epilogue.
+template <int CK>
 __global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* __restrict__ Cg, int a, int b) {
   extern __shared__ double lds[];
   double* C = lds;
   double* scr = lds + a * b;
   for (int e = threadIdx.x; e < a * b; e += kThreads) C[e] = Cg[e];
   __syncthreads();
-  run_epilogue(E, C, a, b, scr);
+  run_epilogue<CK>(E, C, a, b, scr);
 }
 
 // ----------------------------------------------------------------------------
@@ -500,26 +581,25 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
   double* tw = tout + rpb * p.a4;
   double* red = tw + (A.W ? rpb * p.b4 : 0);
 
-  if (A.X) {
-    for (int e = t; e < ri * ro; e += kThreads) {
-      const int d = e % ri, c = e / ri;
-      Ys[c * ldy + d] = A.transY ? A.Y[c + d * A.ldy] : A.Y[d + c * A.ldy];
-    }
-  }
-  __syncthreads();
-
   double acc[PER][16];
 #pragma unroll
   for (int q = 0; q < PER; ++q)
 #pragma unroll
     for (int u = 0; u < 16; ++u) acc[q][u] = 0.0;
 
+  bool staged = (A.X == nullptr);
   const int64_t ntiles = (A.rows + rpb - 1) / rpb;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * rpb;
     const int nrows = (int)min((int64_t)rpb, A.rows - row0);
     const int64_t i = row0 + lrow;
     const bool ok = lrow < nrows;
+    // issue this tile's global loads first: the first X chunk, the split partials, the base
+    double x0[32];
+    if (A.X) {
+#pragma unroll
+      for (int e = 0; e < 32; ++e) x0[e] = (ok && e < ri) ? A.X[i + e * A.ldx] : 0.0;
+    }
     double v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) v[q] = 0.0;
@@ -545,12 +625,44 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
         }
       }
       if (A.base) {
+        if (A.st_d) {
+          // neighbours of row i (with the periodic wrap when asked for)
+          const int64_t n = A.rows;
+          const bool hasdn = i + 1 < n || A.st_periodic;
+          const bool hasup = i > 0 || A.st_periodic;
+          const int64_t idn = i + 1 < n ? i + 1 : 0, iup = i > 0 ? i - 1 : n - 1;
+          const double dd = A.st_d[i];
+          const double edn = hasdn ? A.st_e[i] : 0.0, eup = hasup ? A.st_e[iup] : 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int c = 8 * cg + q;
-          if (c < ro) v[q] += A.base_sign * A.base[i + c * A.ld_base];
+          for (int q = 0; q < 8; ++q) {
+            const int c = 8 * cg + q;
+            if (c < ro) {
+              const double* col = A.base + c * A.ld_base;
+              double w = dd * col[i];
+              if (hasdn) w += edn * col[idn];
+              if (hasup) w += eup * col[iup];
+              v[q] += A.base_sign * w;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int c = 8 * cg + q;
+            if (c < ro) v[q] += A.base_sign * A.base[i + c * A.ld_base];
+          }
         }
       }
+    }
+    // Y -> LDS ([c][d], ld r_in + 1) once per workgroup, overlapping the loads above
+    if (!staged) {
+      for (int e = t; e < ri * ro; e += kThreads) {
+        const int d = e % ri, c = e / ri;
+        Ys[c * ldy + d] = A.transY ? A.Y[c + d * A.ldy] : A.Y[d + c * A.ldy];
+      }
+      __syncthreads();
+      staged = true;
+    }
+    if (ok) {
       if (A.X) {
         double xy[8];
 #pragma unroll
@@ -558,7 +670,8 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
         for (int d0 = 0; d0 < ri; d0 += 32) {
           double x[32];
 #pragma unroll
-          for (int e = 0; e < 32; ++e) x[e] = (d0 + e < ri) ? A.X[i + (d0 + e) * A.ldx] : 0.0;
+          for (int e = 0; e < 32; ++e)
+            x[e] = d0 == 0 ? x0[e] : ((d0 + e < ri) ? A.X[i + (d0 + e) * A.ldx] : 0.0);
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
             const int c = min(8 * cg + q, ro - 1);
@@ -652,8 +765,14 @@ hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
                      A.epi.small->Cred);
   e = hipGetLastError();
   if (e != hipSuccess || A.epi.kind == EPI_NONE) return e;
-  hipLaunchKernelGGL(k_epilogue, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)A.epi.small->Cred,
-                     A.r_out, rw);
+  const double* Cr = A.epi.small->Cred;
+  const bool chol = A.epi.kind == EPI_CHOL1 || A.epi.kind == EPI_CHOL2;
+  if (chol && A.r_out <= 16)
+    hipLaunchKernelGGL(k_epilogue<16>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, Cr, A.r_out, rw);
+  else if (chol && A.r_out <= 32)
+    hipLaunchKernelGGL(k_epilogue<32>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, Cr, A.r_out, rw);
+  else
+    hipLaunchKernelGGL(k_epilogue<64>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, Cr, A.r_out, rw);
   return hipGetLastError();
 }
 

@@ -92,61 +92,102 @@ __device__ __forceinline__ void wave_lds_fence() {
 // FP64 passes (v_mfma_f64_16x16x4_f64); snapshots of any type decoded exactly
 // ============================================================================
 
-template <typename T>
-__device__ __forceinline__ double ldiff(const T* __restrict__ an, const T* __restrict__ ap, int64_t off) {
-  return snap_f64(an, off) - snap_f64(ap, off);
+// N consecutive elements (N = 1, 2 or 4) of a column (rows r0 .. r0+N-1,
+// masked against rend), decoded exactly to FP64; vector loads when aligned and
+// in range.
+template <int N, typename T>
+__device__ __forceinline__ void load_rows_f64(const T* __restrict__ p, int64_t off, int64_t r0, int64_t rend,
+                                              bool ok, bool vec, double (&v)[N]) {
+  if (ok && vec && r0 + N <= rend) {
+    if constexpr (sizeof(T) == 8 && N == 4) {
+      const double2 x = *reinterpret_cast<const double2*>(p + off);
+      const double2 y = *reinterpret_cast<const double2*>(p + off + 2);
+      v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+      return;
+    } else if constexpr (sizeof(T) == 8 && N == 2) {
+      const double2 x = *reinterpret_cast<const double2*>(p + off);
+      v[0] = x.x; v[1] = x.y;
+      return;
+    } else if constexpr (sizeof(T) == 4 && N == 4) {
+      const float4 x = *reinterpret_cast<const float4*>(p + off);
+      v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+      return;
+    } else if constexpr (sizeof(T) == 4 && N == 2) {
+      const float2 x = *reinterpret_cast<const float2*>(p + off);
+      v[0] = x.x; v[1] = x.y;
+      return;
+    } else if constexpr (sizeof(T) == 2 && N == 4) {
+      const uint2 x = *reinterpret_cast<const uint2*>(p + off);
+      v[0] = __uint_as_float(x.x << 16); v[1] = __uint_as_float(x.x & 0xFFFF0000u);
+      v[2] = __uint_as_float(x.y << 16); v[3] = __uint_as_float(x.y & 0xFFFF0000u);
+      return;
+    } else if constexpr (sizeof(T) == 2 && N == 2) {
+      const uint32_t x = *reinterpret_cast<const uint32_t*>(p + off);
+      v[0] = __uint_as_float(x << 16); v[1] = __uint_as_float(x & 0xFFFF0000u);
+      return;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < N; ++e) v[e] = (ok && r0 + e < rend) ? snap_f64(p, off + e) : 0.0;
 }
 
-// pass N: tile = 16*TM rows; an iteration covers 16 columns (4 k-steps of 4).
+// pass N: tile = 16*TM rows.  Lane (lr = l%16, lk = l/16) owns rows
+// i0 + TM*lr + e (e < TM) and feeds them to TM instructions (row block e,
+// instruction row lr), so each lane streams TM consecutive rows of a column per
+// load; an iteration covers 8 columns (2 k-steps of 4).
 template <int TM, int CB, typename T>
 __global__ __launch_bounds__(kPassThreads) void k_passN_f64(const T* __restrict__ an, const T* __restrict__ ap,
                                                             int64_t lda, int64_t n, int64_t m,
                                                             const double* __restrict__ B, int rb, int64_t jchunk,
-                                                            double* __restrict__ part) {
+                                                            double* __restrict__ part, int vec) {
   constexpr int NV = TM * CB * 4;
   __shared__ double red[3 * NV * 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int64_t i0 = (int64_t)blockIdx.x * (16 * TM);
+  const int64_t r0 = i0 + TM * lr;
   const int s = blockIdx.y;
   const int64_t jb = (int64_t)s * jchunk;
   const int64_t je = min(m, jb + jchunk);
-  const WaveRange R = wave_range(jb, je, wave, 16);
+  const WaveRange R = wave_range(jb, je, wave, 8);
+  const bool v = vec != 0;
   f64x4_t acc[TM][CB];
 #pragma unroll
-  for (int t = 0; t < TM; ++t)
+  for (int e = 0; e < TM; ++e)
 #pragma unroll
-    for (int c = 0; c < CB; ++c) acc[t][c] = f64x4_t{0.0, 0.0, 0.0, 0.0};
-  double a[4][TM], b[4][CB], na[4][TM], nb[4][CB];
-  auto load = [&](int64_t j, double (&xa)[4][TM], double (&xb)[4][CB]) {
+    for (int c = 0; c < CB; ++c) acc[e][c] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+  double xa[2][TM], xp[2][TM], b[2][CB], nxa[2][TM], nxp[2][TM], nb[2][CB];
+  auto load = [&](int64_t j, double (&A1)[2][TM], double (&A0)[2][TM], double (&Bv)[2][CB]) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 2; ++u) {
       const int64_t jj = j + 4 * u + lk;
       const bool jok = jj < je;
+      load_rows_f64<TM>(an, jj * lda + r0, r0, n, jok, v, A1[u]);
+      load_rows_f64<TM>(ap, jj * lda + r0, r0, n, jok, v, A0[u]);
 #pragma unroll
-      for (int c = 0; c < CB; ++c) xb[u][c] = jok ? B[jj * rb + c * 16 + lr] : 0.0;
-#pragma unroll
-      for (int t = 0; t < TM; ++t) {
-        const int64_t ii = i0 + 16 * t + lr;
-        xa[u][t] = (jok && ii < n) ? ldiff(an, ap, jj * lda + ii) : 0.0;
-      }
+      for (int c = 0; c < CB; ++c) Bv[u][c] = jok ? B[jj * rb + c * 16 + lr] : 0.0;
     }
   };
-  if (R.b < R.e) load(R.b, a, b);
-  for (int64_t j = R.b; j < R.e; j += 16) {
-    const bool more = j + 16 < R.e;
-    if (more) load(j + 16, na, nb);
+  if (R.b < R.e) load(R.b, xa, xp, b);
+  for (int64_t j = R.b; j < R.e; j += 8) {
+    const bool more = j + 8 < R.e;
+    if (more) load(j + 8, nxa, nxp, nb);
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int t = 0; t < TM; ++t)
+      for (int e = 0; e < TM; ++e) {
+        const double d = xa[u][e] - xp[u][e];
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f64(a[u][t], b[u][c], acc[t][c]);
+        for (int c = 0; c < CB; ++c) acc[e][c] = mfma_f64(d, b[u][c], acc[e][c]);
+      }
     if (more) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 2; ++u) {
 #pragma unroll
-        for (int t = 0; t < TM; ++t) a[u][t] = na[u][t];
+        for (int e = 0; e < TM; ++e) {
+          xa[u][e] = nxa[u][e];
+          xp[u][e] = nxp[u][e];
+        }
 #pragma unroll
         for (int c = 0; c < CB; ++c) b[u][c] = nb[u][c];
       }
@@ -154,31 +195,32 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_f64(const T* __restrict_
   }
   double flat[NV];
 #pragma unroll
-  for (int t = 0; t < TM; ++t)
+  for (int e = 0; e < TM; ++e)
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) flat[(t * CB + c) * 4 + v] = acc[t][c][v];
+      for (int w = 0; w < 4; ++w) flat[(e * CB + c) * 4 + w] = acc[e][c][w];
   wg_sum<double, NV>(flat, red, wave, lane);
   if (wave != 0) return;
+  // instruction row 4w + lk of row block e is dA row i0 + TM*(4w + lk) + e
 #pragma unroll
-  for (int t = 0; t < TM; ++t)
+  for (int e = 0; e < TM; ++e)
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t ii = i0 + 16 * t + 4 * v + lk;
-        if (ii < n) part[((int64_t)s * rb + c * 16 + lr) * n + ii] = flat[(t * CB + c) * 4 + v];
+      for (int w = 0; w < 4; ++w) {
+        const int64_t ii = i0 + TM * (4 * w + lk) + e;
+        if (ii < n) part[((int64_t)s * rb + c * 16 + lr) * n + ii] = flat[(e * CB + c) * 4 + w];
       }
 }
 
 // pass T: tile = 16*TJ columns of dA; an iteration covers 16 rows, each lane
-// owning 4 consecutive rows of its column.
+// owning 4 consecutive rows of its column (one vector load per snapshot).
 template <int TJ, int CB, typename T>
 __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict__ an, const T* __restrict__ ap,
                                                             int64_t lda, int64_t n, int64_t m,
                                                             const double* __restrict__ B, int rb, int64_t ichunk,
-                                                            double* __restrict__ part) {
+                                                            double* __restrict__ part, int vec) {
   constexpr int NV = TJ * CB * 4;
   __shared__ double red[3 * NV * 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -188,44 +230,48 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict_
   const int64_t ib = (int64_t)s * ichunk;
   const int64_t ie = min(n, ib + ichunk);
   const WaveRange R = wave_range(ib, ie, wave, 16);
+  const bool v = vec != 0;
   f64x4_t acc[TJ][CB];
 #pragma unroll
   for (int t = 0; t < TJ; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c) acc[t][c] = f64x4_t{0.0, 0.0, 0.0, 0.0};
-  double a[TJ][4], b[4][CB], na[TJ][4], nb[4][CB];
-  auto load = [&](int64_t i, double (&xa)[TJ][4], double (&xb)[4][CB]) {
+  double xa[TJ][4], xp[TJ][4], b[4][CB], nxa[TJ][4], nxp[TJ][4], nb[4][CB];
+  auto load = [&](int64_t i, double (&A1)[TJ][4], double (&A0)[TJ][4], double (&Bv)[4][CB]) {
+    const int64_t r0 = i + 4 * lk;
 #pragma unroll
     for (int t = 0; t < TJ; ++t) {
       const int64_t jj = j0 + 16 * t + lr;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t ii = i + 4 * lk + e;
-        xa[t][e] = (jj < m && ii < ie) ? ldiff(an, ap, jj * lda + ii) : 0.0;
-      }
+      load_rows_f64<4>(an, jj * lda + r0, r0, ie, jj < m, v, A1[t]);
+      load_rows_f64<4>(ap, jj * lda + r0, r0, ie, jj < m, v, A0[t]);
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int64_t ii = i + 4 * lk + e;
+      const int64_t ii = r0 + e;
 #pragma unroll
-      for (int c = 0; c < CB; ++c) xb[e][c] = ii < ie ? B[ii * rb + c * 16 + lr] : 0.0;
+      for (int c = 0; c < CB; ++c) Bv[e][c] = ii < ie ? B[ii * rb + c * 16 + lr] : 0.0;
     }
   };
-  if (R.b < R.e) load(R.b, a, b);
+  if (R.b < R.e) load(R.b, xa, xp, b);
   for (int64_t i = R.b; i < R.e; i += 16) {
     const bool more = i + 16 < R.e;
-    if (more) load(i + 16, na, nb);
+    if (more) load(i + 16, nxa, nxp, nb);
 #pragma unroll
     for (int e = 0; e < 4; ++e)
 #pragma unroll
-      for (int t = 0; t < TJ; ++t)
+      for (int t = 0; t < TJ; ++t) {
+        const double d = xa[t][e] - xp[t][e];
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f64(a[t][e], b[e][c], acc[t][c]);
+        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f64(d, b[e][c], acc[t][c]);
+      }
     if (more) {
 #pragma unroll
       for (int t = 0; t < TJ; ++t)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) a[t][e] = na[t][e];
+        for (int e = 0; e < 4; ++e) {
+          xa[t][e] = nxa[t][e];
+          xp[t][e] = nxp[t][e];
+        }
 #pragma unroll
       for (int e = 0; e < 4; ++e)
 #pragma unroll
@@ -238,7 +284,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict_
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) flat[(t * CB + c) * 4 + v] = acc[t][c][v];
+      for (int w = 0; w < 4; ++w) flat[(t * CB + c) * 4 + w] = acc[t][c][w];
   wg_sum<double, NV>(flat, red, wave, lane);
   if (wave != 0) return;
 #pragma unroll
@@ -246,9 +292,9 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict_
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t jj = j0 + 16 * t + 4 * v + lk;
-        if (jj < m) part[((int64_t)s * rb + c * 16 + lr) * m + jj] = flat[(t * CB + c) * 4 + v];
+      for (int w = 0; w < 4; ++w) {
+        const int64_t jj = j0 + 16 * t + 4 * w + lk;
+        if (jj < m) part[((int64_t)s * rb + c * 16 + lr) * m + jj] = flat[(t * CB + c) * 4 + w];
       }
 }
 
@@ -729,6 +775,12 @@ int64_t plan_pass(int mode, bool transposed, int64_t out_rows, int64_t red_len, 
   return s * rb * out_rows;
 }
 
+// vector loads of 4 consecutive elements need 4-element-aligned columns
+static int vec_f64(const PassArgs& a, size_t esz) {
+  const size_t al = esz == 8 ? 16 : 4 * esz;
+  return (a.lda % 4 == 0) && ((uintptr_t)a.anext % al == 0) && ((uintptr_t)a.aprev % al == 0) ? 1 : 0;
+}
+
 static bool vec_ok(const PassArgs& a) {
   return (a.lda % 8 == 0) && ((uintptr_t)a.anext % 16 == 0) && ((uintptr_t)a.aprev % 16 == 0);
 }
@@ -745,10 +797,10 @@ static hipError_t run_pass(int mode, bool transposed, const PassArgs& a, hipStre
 #define LRI_F64_LAUNCH(T)                                                                                    \
   if (transposed)                                                                                            \
     hipLaunchKernelGGL((k_passT_f64<TM, CB, T>), grid, blk, 0, st, (const T*)a.anext, (const T*)a.aprev,      \
-                       a.lda, a.n, a.m, B, a.rb, a.chunk, P);                                                \
+                       a.lda, a.n, a.m, B, a.rb, a.chunk, P, vec_f64(a, sizeof(T)));                         \
   else                                                                                                       \
-    hipLaunchKernelGGL((k_passN_f64<TM, CB, T>), grid, blk, 0, st, (const T*)a.anext, (const T*)a.aprev,      \
-                       a.lda, a.n, a.m, B, a.rb, a.chunk, P);
+    hipLaunchKernelGGL((k_passN_f64<TM, CB, T>), grid, blk, 0, st, (const T*)a.anext, (const T*)a.aprev, a.lda,\
+                       a.n, a.m, B, a.rb, a.chunk, P, vec_f64(a, sizeof(T)));
     if (a.dtype == LRI_F64) {
       LRI_F64_LAUNCH(double)
     } else if (a.dtype == LRI_F32) {

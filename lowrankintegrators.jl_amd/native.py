@@ -18,8 +18,10 @@ OK, EINVAL, EHIP, EOOM, EBREAKDOWN, ENONFINITE, ENOTSUP = 0, -1, -2, -3, -4, -5,
 F64, F32, BF16 = 0, 1, 2
 PREC_F64, PREC_MIXED = 0, 1
 PRIMAL, DUAL, STRANG = 0, 1, 2
-KERNEL_PASS_N, KERNEL_PASS_T, KERNEL_SMALL = 0, 1, 2
+KERNEL_PASS_N, KERNEL_PASS_T, KERNEL_SMALL, KERNEL_DE = 0, 1, 2, 3
 MAX_RANK = 64
+MAX_FORCE = 4
+DE_MAX_ROWS = 16384
 
 DTYPES = {"f64": F64, "f32": F32, "bf16": BF16}
 
@@ -46,6 +48,10 @@ SIGNATURES = {
     "lri_step_ksl_strang": (_int, [_vp, _i64, _i64, _i64]),
     "lri_step_bug": (_int, [_vp, _i64, _i64]),
     "lri_step_bug_ra": (_int, [_vp, _i64, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
+    "lri_set_operator_lyapunov": (_int, [_vp, _dp, _dp, _int, _dp, _dp, _int, _i64, _dp, _i64, _dp, _i64]),
+    "lri_step_de_ksl": (_int, [_vp, _int, ctypes.c_double, _i64]),
+    "lri_step_de_bug": (_int, [_vp, ctypes.c_double, _i64]),
+    "lri_step_de_bug_ra": (_int, [_vp, ctypes.c_double, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
     "lri_synchronize": (_int, [_vp]),
     "lri_get_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "lri_kernel_timing": (_int, [_vp, _int]),
@@ -196,6 +202,51 @@ class Context:
         check(lib().lri_step_bug_ra(self.h, int(kp), int(kn), float(tol), int(rmax), ctypes.byref(r)), self.h)
         return int(r.value)
 
+    # -- matrix ODE
+    def set_operator_lyapunov(self, a_diag, a_off, a_periodic, b_diag, b_off, b_periodic, G=None, H=None):
+        """F(Y) = A Y + Y B' + G H' with A, B symmetric tridiagonal (diagonal, off-diagonal)."""
+        def vec(x, n, name):
+            x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+            if x.shape[0] != n:
+                raise ValueError(f"{name} must have {n} entries, got {x.shape[0]}")
+            return x
+
+        ad = vec(a_diag, self.n, "a_diag")
+        bd = vec(b_diag, self.m, "b_diag")
+        # off-diagonals: n - 1 entries, or n when periodic (padded to n either way)
+        ao = np.zeros(self.n)
+        bo = np.zeros(self.m)
+        a_off = np.asarray(a_off, dtype=np.float64).reshape(-1)
+        b_off = np.asarray(b_off, dtype=np.float64).reshape(-1)
+        na, nb = (self.n if a_periodic else self.n - 1), (self.m if b_periodic else self.m - 1)
+        if a_off.shape[0] != na or b_off.shape[0] != nb:
+            raise ValueError(f"off-diagonals need {na} and {nb} entries, got {a_off.shape[0]} and {b_off.shape[0]}")
+        ao[:na] = a_off
+        bo[:nb] = b_off
+        if G is None or H is None:
+            q, Gf, Hf = 0, np.zeros((self.n, 1), order="F"), np.zeros((self.m, 1), order="F")
+        else:
+            Gf = np.asfortranarray(np.asarray(G, dtype=np.float64).reshape(self.n, -1))
+            Hf = np.asfortranarray(np.asarray(H, dtype=np.float64).reshape(self.m, -1))
+            if Gf.shape[1] != Hf.shape[1]:
+                raise ValueError("G and H need the same number of columns")
+            q = Gf.shape[1]
+        check(lib().lri_set_operator_lyapunov(self.h, _dptr(ad), _dptr(ao), int(bool(a_periodic)), _dptr(bd), _dptr(bo),
+                                              int(bool(b_periodic)), int(q), _dptr(Gf), self.n, _dptr(Hf), self.m),
+              self.h)
+
+    def step_de_ksl(self, order: int, dt: float, substeps: int):
+        check(lib().lri_step_de_ksl(self.h, int(order), float(dt), int(substeps)), self.h)
+
+    def step_de_bug(self, dt: float, substeps: int):
+        check(lib().lri_step_de_bug(self.h, float(dt), int(substeps)), self.h)
+
+    def step_de_bug_ra(self, dt: float, substeps: int, tol: float, rmax: int) -> int:
+        r = _i64()
+        check(lib().lri_step_de_bug_ra(self.h, float(dt), int(substeps), float(tol), int(rmax), ctypes.byref(r)),
+              self.h)
+        return int(r.value)
+
     def synchronize(self):
         check(lib().lri_synchronize(self.h), self.h)
 
@@ -213,8 +264,9 @@ class Context:
         return out
 
     # -- timing
-    def timing(self, enable: bool):
-        check(lib().lri_kernel_timing(self.h, 1 if enable else 0), self.h)
+    def timing(self, mode):
+        """0/False: off; 1/True: every kernel family; 2: the two product passes only."""
+        check(lib().lri_kernel_timing(self.h, int(mode)), self.h)
 
     def timing_read(self, family: int):
         ms = ctypes.c_double()

@@ -1,0 +1,185 @@
+"""FP64 CPU restatement of the reference's matrix-ODE (MatrixDEProblem) step path.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Reference files followed:
+
+* DE sub-problem wiring, KSL ......... ``src/integrators/projector_splitting.jl:43-85,129-189``
+* DE sub-problem wiring, BUG ......... ``src/integrators/unconventional.jl:39-84,133-157``
+* DE sub-problem wiring, RA-BUG ...... ``src/integrators/rank_adaptive_unconventional.jl:47-92,194-233``
+* driver ............................. ``src/primitives.jl:68-80``
+
+The reference integrates each sub-problem (K, S, L) with an OrdinaryDiffEq
+integrator (default Tsit5, adaptive) stepped by exactly dt
+(``step!(integ, dt, true)``).  This restatement uses the classical fixed-step
+RK4 with N sub-steps of h = dt/N for every sub-problem (the ``RK4`` sub-step
+algorithm of the engine); Tsit5 itself is not restated ("parity unpinned" for
+Tsit5-driven results: that needs DifferentialEquations.jl).
+
+The right-hand sides are the reference's defaults:
+  K' = F(K V0', t) V0                     (projector_splitting.jl:53-54)
+  L' = F(U0 L', t)' U0                    (:73-74)
+  S' = -U1' F(U1 S V0', t) V0  (KSL, :63-64);  S' = U1' F(U1 S V1', t) V1 (BUG, unconventional.jl:72-73)
+F acts on dense n x m matrices here (small sizes only).
+
+Pinning (``tests/test_oracle_de.py``): the RK4 tableau on scalar linear ODEs
+(local error order 5, against the exact exponential); the sub-step flows
+against the matrix exponential of the linear sub-problem when F is linear; the
+closed forms of one step in the small-dt limit; exactness for F whose flow
+stays in a fixed rank-r manifold; agreement of the K/L/S sub-problems with the
+data path when F(Y, t) = dA/dt is constant in Y.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any, Callable, List
+
+import numpy as np
+
+from .dlra import DLRSolution, PrimalLieTrotter, DualLieTrotter, Strang, ProjectorSplitting, \
+    UnconventionalAlgorithm, RankAdaptiveUnconventionalAlgorithm, _init_sol_len, _qr
+from .lra import SVDLike, truncate_to_tolerance
+
+
+@dataclass
+class MatrixDEProblem:
+    """``MatrixDEProblem{f,u0,tspan}`` (primitives.jl:13-17): dY/dt = f(Y, t)."""
+
+    f: Callable[[np.ndarray, float], np.ndarray]
+    u0: SVDLike
+    tspan: tuple
+
+
+@dataclass
+class LyapunovOperator:
+    """F(Y, t) = A Y + Y B' + sum_k g_k h_k' with dense A (n x n), B (m x m), G = g h'."""
+
+    A: np.ndarray
+    B: np.ndarray
+    g: np.ndarray  # n x q
+    h: np.ndarray  # m x q
+
+    def __call__(self, Y, t=0.0):
+        return self.A @ Y + Y @ self.B.T + self.g @ self.h.T
+
+
+def dirichlet_laplacian(n: int, scale: float | None = None) -> np.ndarray:
+    """Tridiagonal [1, -2, 1] * scale (default n^2), homogeneous Dirichlet boundaries."""
+    s = float(n * n) if scale is None else scale
+    return s * (np.diag(-2.0 * np.ones(n)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1))
+
+
+def rk4_substeps(rho: float, dt: float, cap: float = 2.5) -> int:
+    """Fixed RK4 sub-steps with h*rho <= cap (RK4 is stable on [-2.78, 0] of the real axis)."""
+    return max(1, int(np.ceil(dt * rho / cap)))
+
+
+def rk4(f, y, t, dt, N):
+    """N classical RK4 steps of size dt/N for y' = f(y, t)."""
+    h = dt / N
+    for s in range(N):
+        ts = t + s * h
+        k1 = f(y, ts)
+        k2 = f(y + 0.5 * h * k1, ts + 0.5 * h)
+        k3 = f(y + 0.5 * h * k2, ts + 0.5 * h)
+        k4 = f(y + h * k3, ts + h)
+        y = y + (h / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+    return y
+
+
+# ----------------------------------------------------------------------------
+# steps
+# ----------------------------------------------------------------------------
+
+
+def _K_rhs(F, V):
+    return lambda K, t: F(K @ V.T, t) @ V
+
+
+def _L_rhs(F, U):
+    return lambda L, t: F(U @ L.T, t).T @ U
+
+
+def ksl_primal_de_step(F, U, S, V, t, dt, N):
+    """projector_splitting.jl:129-152 with the DE sub-problems (:52-80)."""
+    K = rk4(_K_rhs(F, V), U @ S, t, dt, N)
+    U1, R = _qr(K)
+    St = rk4(lambda S_, tt: -(U1.T @ F(U1 @ S_ @ V.T, tt) @ V), R, t, dt, N)
+    L = rk4(_L_rhs(F, U1), V @ St.T, t, dt, N)
+    V1, R2 = _qr(L)
+    return U1, R2.T.copy(), V1
+
+
+def ksl_dual_de_step(F, U, S, V, t, dt, N):
+    """projector_splitting.jl:166-189 with the DE sub-problems."""
+    L = rk4(_L_rhs(F, U), V @ S.T, t, dt, N)
+    V1, R = _qr(L)
+    St = rk4(lambda S_, tt: -(U.T @ F(U @ S_ @ V1.T, tt) @ V1), R.T.copy(), t, dt, N)
+    K = rk4(_K_rhs(F, V1), U @ St, t, dt, N)
+    U1, R2 = _qr(K)
+    return U1, R2, V1
+
+
+def bug_de_step(F, U, S, V, t, dt, N):
+    """unconventional.jl:133-157 with the DE sub-problems (:51-79)."""
+    K = rk4(_K_rhs(F, V), U @ S, t, dt, N)
+    U1, _ = _qr(K)
+    M = U1.T @ U
+    L = rk4(_L_rhs(F, U), V @ S.T, t, dt, N)
+    V1, _ = _qr(L)
+    Nn = V1.T @ V
+    S1 = rk4(lambda S_, tt: U1.T @ F(U1 @ S_ @ V1.T, tt) @ V1, M @ S @ Nn.T, t, dt, N)
+    return U1, S1, V1
+
+
+def rank_adaptive_bug_de_step(F, U, S, V, t, dt, N, tol, rmax):
+    """rank_adaptive_unconventional.jl:194-233 with the DE sub-problems (:59-86)."""
+    K = rk4(_K_rhs(F, V), U @ S, t, dt, N)
+    Uh, _ = _qr(np.hstack([K, U]))
+    M = Uh.T @ U
+    L = rk4(_L_rhs(F, U), V @ S.T, t, dt, N)
+    Vh, _ = _qr(np.hstack([L, V]))
+    Nn = Vh.T @ V
+    Sh = rk4(lambda S_, tt: Uh.T @ F(Uh @ S_ @ Vh.T, tt) @ Vh, M @ S @ Nn.T, t, dt, N)
+    Us, sig, Vst = np.linalg.svd(Sh)
+    r1 = int(min(rmax, truncate_to_tolerance(sig, tol)))
+    return Uh @ Us[:, :r1], np.diag(sig[:r1]), Vh @ Vst[:r1].T, r1
+
+
+def solve_de(prob: MatrixDEProblem, alg, dt, substeps: int) -> DLRSolution:
+    """``solve(prob::MatrixDEProblem, alg, dt)`` (primitives.jl:68-76) with RK4 sub-steps."""
+    t0, tf = prob.tspan
+    if not tf > t0:
+        raise AssertionError("Integration in reverse time direction is not supported")
+    u = prob.u0.copy()
+    Ys: List[Any] = [None] * _init_sol_len(dt, t0, tf)
+    ts: List[Any] = list(np.linspace(t0, tf, len(Ys)))
+    Ys[0] = prob.u0.copy()
+    t, it, T = t0, 0, tf - t0
+    F = prob.f
+    while (tf - t) / T > 1e-8:
+        U, S, V = u.U, u.S, u.V
+        if isinstance(alg, ProjectorSplitting):
+            if isinstance(alg.order, PrimalLieTrotter):
+                U, S, V = ksl_primal_de_step(F, U, S, V, t, dt, substeps)
+            elif isinstance(alg.order, DualLieTrotter):
+                U, S, V = ksl_dual_de_step(F, U, S, V, t, dt, substeps)
+            elif isinstance(alg.order, Strang):
+                U, S, V = ksl_primal_de_step(F, U, S, V, t, dt / 2, substeps)
+                U, S, V = ksl_dual_de_step(F, U, S, V, t + dt / 2, dt / 2, substeps)
+        elif isinstance(alg, UnconventionalAlgorithm):
+            U, S, V = bug_de_step(F, U, S, V, t, dt, substeps)
+        elif isinstance(alg, RankAdaptiveUnconventionalAlgorithm):
+            U, S, V, _ = rank_adaptive_bug_de_step(F, U, S, V, t, dt, substeps, alg.tol, alg.rmax)
+        else:
+            raise TypeError(f"unsupported algorithm {alg!r}")
+        u = SVDLike(U, S, V)
+        t = t + dt
+        it += 1
+        if it <= len(Ys) - 1:
+            Ys[it] = u.copy()
+            ts[it] = t
+        else:
+            Ys.append(u.copy())
+            ts.append(t)
+    return DLRSolution(Ys, ts)

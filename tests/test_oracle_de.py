@@ -1,0 +1,124 @@
+"""Pins for the DE-path oracle (no GPU)."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import dlra, dlra_de
+from oracle.data import random_factors
+from oracle.lra import SVDLike
+from oracle.metrics import rel_err
+
+
+def test_rk4_local_order():
+    lam = -3.0
+    errs = []
+    for h in (0.1, 0.05, 0.025):
+        y = dlra_de.rk4(lambda y, t: lam * y, np.array([1.0]), 0.0, h, 1)[0]
+        errs.append(abs(y - np.exp(lam * h)))
+    # local error ~ h^5: halving h divides the error by ~32
+    assert 25 < errs[0] / errs[1] < 40 and 25 < errs[1] / errs[2] < 40
+
+
+def test_rk4_time_argument():
+    # y' = t: exact for polynomial degree <= 4
+    y = dlra_de.rk4(lambda y, t: np.array([t ** 3]), np.array([0.0]), 1.0, 0.5, 3)[0]
+    assert abs(y - (1.5 ** 4 - 1.0) / 4) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["primal", "dual", "bug"])
+def test_constant_rhs_matches_data_path(name):
+    """F(Y, t) = C (constant): every sub-problem is linear in t, RK4 is exact, and the DE step
+    equals the data step with increment dt*C."""
+    rng = np.random.default_rng(0)
+    u = random_factors(rng, 30, 24, 4)
+    C = rng.standard_normal((30, 24))
+    dt = 0.05
+    F = lambda Y, t: C
+    de = {"primal": dlra_de.ksl_primal_de_step, "dual": dlra_de.ksl_dual_de_step, "bug": dlra_de.bug_de_step}[name]
+    da = {"primal": dlra.ksl_primal_step, "dual": dlra.ksl_dual_step, "bug": dlra.bug_step}[name]
+    a = de(F, u.U, u.S, u.V, 0.0, dt, 3)
+    b = da(u.U, u.S, u.V, dt * C)
+    assert rel_err(*a, *b) < 1e-13
+
+
+def test_constant_rhs_rank_adaptive_matches_data_path():
+    rng = np.random.default_rng(1)
+    u = random_factors(rng, 30, 24, 4)
+    C = rng.standard_normal((30, 24))
+    a = dlra_de.rank_adaptive_bug_de_step(lambda Y, t: C, u.U, u.S, u.V, 0.0, 0.05, 2, 1e-3, 20)
+    b = dlra.rank_adaptive_bug_step(u.U, u.S, u.V, 0.05 * C, 1e-3, 20)
+    assert a[3] == b[3]
+    assert rel_err(*a[:3], *b[:3]) < 1e-13
+
+
+def test_K_subproblem_against_matrix_exponential():
+    """K' = F(K V') V with F(Y) = A Y + Y B' + g h' is linear: compare RK4 to the exact flow."""
+    rng = np.random.default_rng(2)
+    n, m, r = 20, 16, 3
+    A = dlra_de.dirichlet_laplacian(n, 4.0)
+    B = dlra_de.dirichlet_laplacian(m, 3.0)
+    g, h = rng.standard_normal((n, 1)), rng.standard_normal((m, 1))
+    F = dlra_de.LyapunovOperator(A, B, g, h)
+    V, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    K0 = rng.standard_normal((n, r))
+    dt = 0.1
+    K = dlra_de.rk4(lambda K, t: F(K @ V.T, t) @ V, K0, 0.0, dt, 200)
+    # vec(K)' = (I (x) A + Bt (x) I) vec(K) + vec(g h' V),  Bt = V'BV
+    Bt = V.T @ B @ V
+    Op = np.kron(np.eye(r), A) + np.kron(Bt, np.eye(n))
+    c = (g @ h.T @ V).reshape(-1, order="F")
+    Aug = np.zeros((n * r + 1, n * r + 1))
+    Aug[: n * r, : n * r] = Op
+    Aug[: n * r, -1] = c
+    z = sla.expm(dt * Aug) @ np.concatenate([K0.reshape(-1, order="F"), [1.0]])
+    exact = z[: n * r].reshape(n, r, order="F")
+    assert np.linalg.norm(K - exact) / np.linalg.norm(exact) < 1e-10
+
+
+def test_laplacian_and_substeps():
+    L = dlra_de.dirichlet_laplacian(4)
+    assert np.allclose(L, 16 * np.array([[-2, 1, 0, 0], [1, -2, 1, 0], [0, 1, -2, 1], [0, 0, 1, -2]]))
+    assert dlra_de.rk4_substeps(4 * 4096 ** 2, 1e-4) == 2685
+    ev = np.linalg.eigvalsh(dlra_de.dirichlet_laplacian(64))
+    assert -ev.min() < 4 * 64 ** 2
+
+
+def test_solve_de_driver():
+    rng = np.random.default_rng(3)
+    n = 24
+    u = random_factors(rng, n, n, 3)
+    F = dlra_de.LyapunovOperator(dlra_de.dirichlet_laplacian(n, 1.0), dlra_de.dirichlet_laplacian(n, 1.0),
+                                 rng.standard_normal((n, 1)), rng.standard_normal((n, 1)))
+    prob = dlra_de.MatrixDEProblem(F, u, (0.0, 0.1))
+    sol = dlra_de.solve_de(prob, dlra.UnconventionalAlgorithm(), 0.01, 2)
+    assert len(sol.Y) == 11
+    # rank-r projection of the exact full solution is close for a smooth linear problem
+    Y = u.to_matrix()
+    for k in range(10):
+        Y = dlra_de.rk4(lambda X, t: F(X, t), Y, 0.01 * k, 0.01, 2)
+    yf = sol.Y[-1]
+    assert np.linalg.norm(yf.to_matrix() - Y) / np.linalg.norm(Y) < 0.5
+
+
+@pytest.mark.parametrize("name", ["primal", "dual", "strang", "bug"])
+def test_scalar_operator_closed_form(name):
+    """F(Y) = c Y: K and L flows forward, the KSL S flow backward, so one KSL
+    step scales Y by (R(z)^2 R(-z))^N and one BUG step by R(z)^N (z = h c, R the
+    RK4 stability polynomial).  A dropped minus sign in the KSL S step fails this."""
+    rng = np.random.default_rng(4)
+    u = random_factors(rng, 20, 16, 3)
+    c, dt, N = -4.5, 0.2, 7
+    F = lambda Y, t: c * Y
+    R = lambda z: 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+    z = dt / N * c
+    if name == "primal":
+        got, f = dlra_de.ksl_primal_de_step(F, u.U, u.S, u.V, 0.0, dt, N), (R(z) ** 2 * R(-z)) ** N
+    elif name == "dual":
+        got, f = dlra_de.ksl_dual_de_step(F, u.U, u.S, u.V, 0.0, dt, N), (R(z) ** 2 * R(-z)) ** N
+    elif name == "strang":
+        a = dlra_de.ksl_primal_de_step(F, u.U, u.S, u.V, 0.0, dt / 2, N)
+        got = dlra_de.ksl_dual_de_step(F, *a, dt / 2, dt / 2, N)
+        f = (R(z / 2) ** 2 * R(-z / 2)) ** (2 * N)
+    else:
+        got, f = dlra_de.bug_de_step(F, u.U, u.S, u.V, 0.0, dt, N), R(z) ** N
+    assert rel_err(*got, u.U, f * u.S, u.V) < 1e-13
