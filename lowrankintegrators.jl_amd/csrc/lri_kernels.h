@@ -97,7 +97,7 @@ struct RowArgs {
   EpiArgs epi;
   int max_blocks = 256;
   // filled by the launcher
-  int tr = 0, ri_pad = 0, ro_pad = 0, epi_doubles = 0;
+  int tr = 0;  // rows per tile
 };
 
 template <typename PT>

@@ -416,106 +416,6 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
   }
 }
 
-// ----------------------------------------------------------------------------
-// per-workgroup reduction C_blk += tile_out' * tile_w, register-blocked 4 x 4
-// ----------------------------------------------------------------------------
-
-struct RedPlan {
-  int a4, b4, nblk, groups, per;
-};
-
-__host__ __device__ inline RedPlan red_plan(int a, int b) {
-  RedPlan p;
-  p.a4 = (a + 3) & ~3;
-  p.b4 = (b + 3) & ~3;
-  p.nblk = (p.a4 / 4) * (p.b4 / 4);
-  if (p.nblk >= kThreads) {
-    p.groups = 1;
-    p.per = (p.nblk + kThreads - 1) / kThreads;
-  } else {
-    p.groups = kThreads / p.nblk;
-    p.per = 1;
-  }
-  return p;
-}
-
-template <int PER>
-__device__ __forceinline__ void red_accum(const RedPlan& p, const double* ta, const double* tb, int nrows,
-                                          double (&acc)[PER][16]) {
-  const int t = threadIdx.x;
-  const int na = p.a4 / 4;
-  if (p.groups > 1) {
-    const int blk = t % p.nblk, g = t / p.nblk;
-    if (g >= p.groups) return;
-    const int bi = (blk % na) * 4, bj = (blk / na) * 4;
-    for (int row = g; row < nrows; row += p.groups) {
-      double x[4], y[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        x[u] = ta[row * p.a4 + bi + u];
-        y[u] = tb[row * p.b4 + bj + u];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[0][u * 4 + v] += x[u] * y[v];
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int blk = t + q * kThreads;
-      if (blk < p.nblk) {
-        const int bi = (blk % na) * 4, bj = (blk / na) * 4;
-        for (int row = 0; row < nrows; ++row) {
-          double x[4], y[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            x[u] = ta[row * p.a4 + bi + u];
-            y[u] = tb[row * p.b4 + bj + u];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[q][u * 4 + v] += x[u] * y[v];
-        }
-      }
-    }
-  }
-}
-
-template <int PER>
-__device__ void red_flush(const RedPlan& p, int a, int b, double (&acc)[PER][16], double* red, double* gpart) {
-  const int t = threadIdx.x;
-  const int na = p.a4 / 4;
-  if (p.groups > 1) {
-    const int blk = t % p.nblk, g = t / p.nblk;
-    const int bi = (blk % na) * 4, bj = (blk / na) * 4;
-    if (g < p.groups)
-      for (int u = 0; u < 4; ++u)
-        for (int v = 0; v < 4; ++v) red[g * p.a4 * p.b4 + (bi + u) + (bj + v) * p.a4] = acc[0][u * 4 + v];
-    __syncthreads();
-    for (int e = t; e < a * b; e += kThreads) {
-      int i = e % a, j = e / a;
-      double s = 0.0;
-      for (int gg = 0; gg < p.groups; ++gg) s += red[gg * p.a4 * p.b4 + i + j * p.a4];
-      gpart[e] = s;
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int blk = t + q * kThreads;
-      if (blk < p.nblk) {
-        const int bi = (blk % na) * 4, bj = (blk / na) * 4;
-        for (int u = 0; u < 4; ++u)
-          for (int v = 0; v < 4; ++v) {
-            int i = bi + u, j = bj + v;
-            if (i < a && j < b) gpart[i + j * a] = acc[q][u * 4 + v];
-          }
-      }
-    }
-  }
-}
-
 // Partial sums of the per-workgroup reductions: workgroup b of k_reduce sums
 // entries [64b, 64b + 64) of the nb partials (each a*b doubles) into C; thread
 // (entry e, quarter g) adds partials g, g+4, ... with 16 loads in flight, and the
@@ -555,51 +455,103 @@ __global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* 
 
 // ----------------------------------------------------------------------------
 // the row kernel
-//   out[i, c] = (parts ? sum_s part[s][c][i] : 0) + base_sign * base[i, c]
+//   out[i, c] = (parts ? sum_s part[s][c][i] : 0) + base_sign * op(base)[i, c]
 //             + xy_sign * sum_d X[i, d] * op(Y)(d, c)
-// Thread t of a tile handles row t % rpb and the 8 columns 8*(t / rpb) + 0..7
-// (tpr = 256 / rpb threads per row), so every wave reads and writes 64
-// consecutive rows of one column.  Y sits in LDS as [c][d] (ld r_in + 1) and is
-// read as a broadcast; the X row is loaded 16 values at a time.
-// LDS: Ys | tile_out (rpb x a4) | tile_w (rpb x b4) | group partials, and after
-// the tile loop, C (a x b) + epilogue scratch reuse everything past Ys.
+// and optionally the reduction C = out' * W (W == nullptr: out' * out).
+// A tile has rpb rows.  For loads and stores thread t handles row t % rpb and
+// the columns 8*(t / rpb) + 0..7, so every wave reads and writes 64
+// consecutive rows of one column.  The tile then goes through LDS (Xt: the X
+// rows, Ot: the outputs, Wt: the W rows), where both products run on FP64
+// matrix cores (16x16x4): X*Y adds into Ot in place, and the reduction
+// accumulates 16x16 blocks of C in registers across tiles (block b belongs to
+// wave b % 4), written once per workgroup to gpart[blockIdx.x].
 // ----------------------------------------------------------------------------
 
-template <typename PT, int PER>
+struct RowsPlan {
+  int tpr, rpb;                // threads per row, rows per tile
+  int ri4, ro16, rw16;         // padded widths
+  int ldy, ldxt, ldot, ldwt;   // LDS leading dimensions (doubles; odd against bank conflicts)
+  int nxy, nred;               // 16x16 blocks of X*Y per tile, of the reduction
+  int ys_off, xt_off, ot_off, wt_off;
+  size_t lds_doubles;
+};
+
+// threads per row: enough for r_out in groups of 8, and more (fewer rows per
+// tile) while the LDS tile would exceed 96 KB (e.g. a narrow output reduced
+// against a wide W)
+__host__ __device__ inline RowsPlan rows_plan(int ro, int ri, int rw, bool hasX, bool hasW, bool reduce) {
+  RowsPlan p;
+  p.ri4 = (ri + 3) & ~3;
+  p.ro16 = (ro + 15) & ~15;
+  p.rw16 = hasW ? ((rw + 15) & ~15) : p.ro16;
+  p.ldy = p.ro16 + 1;
+  p.ldxt = p.ri4 + 1;
+  p.ldot = p.ro16 + 1;
+  p.ldwt = p.rw16 + 1;
+  int tpr = 1;
+  while (8 * tpr < ro) tpr *= 2;
+  for (;;) {
+    p.tpr = tpr;
+    p.rpb = kThreads / tpr;
+    int off = 0;
+    p.ys_off = off;
+    if (hasX) off += p.ri4 * p.ldy;
+    p.xt_off = off;
+    if (hasX) off += p.rpb * p.ldxt;
+    p.ot_off = off;
+    if (hasX || reduce) off += p.rpb * p.ldot;
+    p.wt_off = off;
+    if (hasW && reduce) off += p.rpb * p.ldwt;
+    p.lds_doubles = (size_t)off;
+    if (p.lds_doubles * sizeof(double) <= 96 * 1024 || tpr >= 16) break;
+    tpr *= 2;
+  }
+  p.nxy = hasX ? (p.rpb / 16) * (p.ro16 / 16) : 0;
+  p.nred = reduce ? (p.ro16 / 16) * (p.rw16 / 16) : 0;
+  return p;
+}
+
+template <typename PT, int NRB>
 __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
   extern __shared__ double lds[];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int ro = A.r_out, ri = A.r_in;
-  const int rw = A.W ? A.rw : ro;
-  const int rpb = A.tr;
-  const int lrow = t % rpb, cg = t / rpb, tpr = kThreads / rpb;
-  const int ldy = ri + 1;
-  double* Ys = lds;
-  double* area = Ys + (A.X ? ro * ldy : 0);
-  const RedPlan p = red_plan(ro, rw);
-  double* tout = area;
-  double* tw = tout + rpb * p.a4;
-  double* red = tw + (A.W ? rpb * p.b4 : 0);
+  const bool hasX = A.X != nullptr;
+  const bool hasW = A.W != nullptr;
+  const bool reduce = A.reduce != 0;
+  const int rw = hasW ? A.rw : ro;
+  const RowsPlan p = rows_plan(ro, ri, rw, hasX, hasW, reduce);
+  const int rpb = p.rpb, tpr = p.tpr;
+  const int lrow = t % rpb, cg = t / rpb;
+  double* Ys = lds + p.ys_off;
+  double* Xt = lds + p.xt_off;
+  double* Ot = lds + p.ot_off;
+  double* Wt = (hasW && reduce) ? lds + p.wt_off : Ot;
+  const int ldwt = (hasW && reduce) ? p.ldwt : p.ldot;
+  const bool use_tile = hasX || reduce;
+  const int lr = lane & 15, lk = lane >> 4;  // matrix-core lane coordinates
 
-  double acc[PER][16];
+  f64x4_t racc[NRB];
 #pragma unroll
-  for (int q = 0; q < PER; ++q)
-#pragma unroll
-    for (int u = 0; u < 16; ++u) acc[q][u] = 0.0;
+  for (int b = 0; b < NRB; ++b) racc[b] = f64x4_t{0.0, 0.0, 0.0, 0.0};
 
-  bool staged = (A.X == nullptr);
+  // xy_sign * op(Y) into LDS once, as Ys[d][c] (ri4 x ro16, zero padded); the
+  // first tile's barrier publishes it
+  if (hasX) {
+    for (int e = t; e < p.ri4 * p.ro16; e += kThreads) {
+      const int c = e % p.ro16, d = e / p.ro16;
+      double y = 0.0;
+      if (c < ro && d < ri) y = A.transY ? A.Y[c + d * A.ldy] : A.Y[d + c * A.ldy];
+      Ys[d * p.ldy + c] = A.xy_sign * y;
+    }
+  }
+
   const int64_t ntiles = (A.rows + rpb - 1) / rpb;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * rpb;
     const int nrows = (int)min((int64_t)rpb, A.rows - row0);
     const int64_t i = row0 + lrow;
     const bool ok = lrow < nrows;
-    // issue this tile's global loads first: the first X chunk, the split partials, the base
-    double x0[32];
-    if (A.X) {
-#pragma unroll
-      for (int e = 0; e < 32; ++e) x0[e] = (ok && e < ri) ? A.X[i + e * A.ldx] : 0.0;
-    }
     double v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) v[q] = 0.0;
@@ -653,39 +605,44 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
         }
       }
     }
-    // Y -> LDS ([c][d], ld r_in + 1) once per workgroup, overlapping the loads above
-    if (!staged) {
-      for (int e = t; e < ri * ro; e += kThreads) {
-        const int d = e % ri, c = e / ri;
-        Ys[c * ldy + d] = A.transY ? A.Y[c + d * A.ldy] : A.Y[d + c * A.ldy];
+    if (use_tile) {
+      // the tile into LDS: partial outputs, X rows, W rows (zero beyond the data)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int c = 8 * cg + q;
+        if (c < p.ro16) Ot[lrow * p.ldot + c] = (ok && c < ro) ? v[q] : 0.0;
       }
+      for (int c = 8 * tpr + cg; c < p.ro16; c += tpr) Ot[lrow * p.ldot + c] = 0.0;
+      if (hasX)
+        for (int d = cg; d < p.ri4; d += tpr) Xt[lrow * p.ldxt + d] = (ok && d < ri) ? A.X[i + d * A.ldx] : 0.0;
+      if (hasW && reduce)
+        for (int c = cg; c < p.rw16; c += tpr) Wt[lrow * p.ldwt + c] = (ok && c < rw) ? A.W[i + c * A.ldw] : 0.0;
       __syncthreads();
-      staged = true;
+      if (hasX) {
+        // Ot += Xt * Ys, one 16 x 16 output block per wave at a time
+        const int nrb = rpb / 16;
+        for (int blk = wave; blk < p.nxy; blk += 4) {
+          const int r0 = (blk % nrb) * 16, c0 = (blk / nrb) * 16;
+          f64x4_t acc;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[u] = Ot[(r0 + 4 * u + lk) * p.ldot + c0 + lr];
+          for (int d0 = 0; d0 < p.ri4; d0 += 4) {
+            const double a = Xt[(r0 + lr) * p.ldxt + d0 + lk];
+            const double b = Ys[(d0 + lk) * p.ldy + c0 + lr];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) Ot[(r0 + 4 * u + lk) * p.ldot + c0 + lr] = acc[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int c = 8 * cg + q;
+          if (c < ro) v[q] = Ot[lrow * p.ldot + c];
+        }
+      }
     }
     if (ok) {
-      if (A.X) {
-        double xy[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) xy[q] = 0.0;
-        for (int d0 = 0; d0 < ri; d0 += 32) {
-          double x[32];
-#pragma unroll
-          for (int e = 0; e < 32; ++e)
-            x[e] = d0 == 0 ? x0[e] : ((d0 + e < ri) ? A.X[i + (d0 + e) * A.ldx] : 0.0);
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const int c = min(8 * cg + q, ro - 1);
-            const double* yc = Ys + c * ldy + d0;
-            double sacc = 0.0;
-#pragma unroll
-            for (int e = 0; e < 32; ++e)
-              if (d0 + e < ri) sacc += x[e] * yc[e];
-            xy[q] += sacc;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] += A.xy_sign * xy[q];
-      }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int c = 8 * cg + q;
@@ -714,50 +671,68 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
         }
       }
     }
-    if (A.reduce) {
+    if (reduce) {
+      // C += Ot' * Wt over the tile's rows (rows past the data are zero)
+      const int nbi = p.ro16 / 16;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int c = 8 * cg + q;
-        if (c < p.a4) tout[lrow * p.a4 + c] = (ok && c < ro) ? v[q] : 0.0;
+      for (int b = 0; b < NRB; ++b) {
+        const int blk = wave + 4 * b;
+        if (blk < p.nred) {
+          const int c0 = (blk % nbi) * 16, j0 = (blk / nbi) * 16;
+          for (int k0 = 0; k0 < rpb; k0 += 4) {
+            const double a = Ot[(k0 + lk) * p.ldot + c0 + lr];
+            const double bb = Wt[(k0 + lk) * ldwt + j0 + lr];
+            racc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, racc[b], 0, 0, 0);
+          }
+        }
       }
-      if (A.W)
-        for (int c = cg; c < p.b4; c += tpr) tw[lrow * p.b4 + c] = (ok && c < rw) ? A.W[i + c * A.ldw] : 0.0;
-      __syncthreads();
-      red_accum<PER>(p, tout, A.W ? tw : tout, nrows, acc);
-      __syncthreads();
+    }
+    if (use_tile) __syncthreads();  // the next tile overwrites the LDS tile
+  }
+  if (!reduce) return;
+  double* gp = A.gpart + (int64_t)blockIdx.x * ro * rw;
+  const int nbi = p.ro16 / 16;
+#pragma unroll
+  for (int b = 0; b < NRB; ++b) {
+    const int blk = wave + 4 * b;
+    if (blk < p.nred) {
+      const int c0 = (blk % nbi) * 16, j0 = (blk / nbi) * 16;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int ci = c0 + 4 * u + lk, cj = j0 + lr;
+        if (ci < ro && cj < rw) gp[ci + cj * ro] = racc[b][u];
+      }
     }
   }
-  if (!A.reduce) return;
-  double* gpart = A.gpart + (int64_t)blockIdx.x * ro * rw;
-  red_flush<PER>(p, ro, rw, acc, red, gpart);
 }
 
 template <typename PT>
 hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
-  const int rw = A.W ? A.rw : A.r_out;
-  const RedPlan p = red_plan(A.r_out, rw);
-  // threads per row: a power of two covering r_out in groups of 8
-  int tpr = 1;
-  while (8 * tpr < A.r_out) tpr *= 2;
-  if (tpr > 16) return hipErrorInvalidValue;
-  const int rpb = kThreads / tpr;
-  A.tr = rpb;
-  if (p.a4 > 8 * tpr) return hipErrorInvalidValue;
-  const size_t ys = A.X ? (size_t)A.r_out * (A.r_in + 1) : 0;
-  const size_t body = (size_t)rpb * (p.a4 + (A.W ? p.b4 : 0)) + (p.groups > 1 ? (size_t)p.groups * p.a4 * p.b4 : 0);
-  const size_t lds = sizeof(double) * (ys + body);
+  const bool hasW = A.W != nullptr;
+  const int rw = hasW ? A.rw : A.r_out;
+  if (A.r_out < 1 || A.r_out > kMaxAug || rw < 1 || rw > kMaxAug || A.r_in < 0 || A.r_in > kMaxAug)
+    return hipErrorInvalidValue;
+  if (A.X && A.r_in < 1) return hipErrorInvalidValue;
+  const RowsPlan p = rows_plan(A.r_out, A.r_in, rw, A.X != nullptr, hasW, A.reduce != 0);
+  if (p.tpr > 16) return hipErrorInvalidValue;
+  A.tr = p.rpb;
+  const size_t lds = sizeof(double) * p.lds_doubles;
   const size_t epi_lds = sizeof(double) * ((size_t)A.r_out * rw + epi_scratch_doubles(A.epi.kind, A.r_out, rw));
   if (lds > 150 * 1024 || epi_lds > 150 * 1024) return hipErrorInvalidValue;  // room for static shared variables
-  const int64_t ntiles = (A.rows + rpb - 1) / rpb;
+  const int64_t ntiles = (A.rows + p.rpb - 1) / p.rpb;
   int grid = (int)(ntiles < A.max_blocks ? ntiles : A.max_blocks);
   if (grid < 1) grid = 1;
-  if (p.per <= 1) {
+  const int nrb = (p.nred + 3) / 4;
+  if (nrb <= 1)
     hipLaunchKernelGGL((k_rows<PT, 1>), dim3(grid), dim3(kThreads), lds, st, A);
-  } else if (p.per <= 4) {
+  else if (nrb <= 2)
+    hipLaunchKernelGGL((k_rows<PT, 2>), dim3(grid), dim3(kThreads), lds, st, A);
+  else if (nrb <= 4)
     hipLaunchKernelGGL((k_rows<PT, 4>), dim3(grid), dim3(kThreads), lds, st, A);
-  } else {
-    return hipErrorInvalidValue;
-  }
+  else if (nrb <= 8)
+    hipLaunchKernelGGL((k_rows<PT, 8>), dim3(grid), dim3(kThreads), lds, st, A);
+  else
+    hipLaunchKernelGGL((k_rows<PT, 16>), dim3(grid), dim3(kThreads), lds, st, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !A.reduce) return e;
   const int ab = A.r_out * rw;
