@@ -1,6 +1,7 @@
 /*
  * lri.h -- C ABI of liblri_hip.so, the MI355X (gfx950) engine for the per-step
- * hot path of dynamical low-rank approximation (DLRA) on data problems.
+ * hot path of dynamical low-rank approximation (DLRA) on data problems and on
+ * matrix ODEs with a tridiagonal Lyapunov-type right-hand side.
  *
  * The reference (LowRankIntegrators.jl) is pure Julia; its "operator API" for
  * this path is multiple dispatch on (problem, algorithm).  Each entry point

@@ -1,12 +1,12 @@
 // FP64 kernels on the thin n x r (m x r) factors and the r x r cores.
 //
-// k_rows is row-parallel over the tall dimension (one thread per row, tiles of
-// up to 256 rows, grid-stride over tiles) and may end in a global reduction
-// C = out' * W (or the Gram out' * out).  The reduction is finished by the last
-// workgroup to arrive, which sums the per-workgroup partials in workgroup order
-// (so results are bitwise reproducible) and runs a small epilogue on C in LDS:
-// a Cholesky factorisation for CholeskyQR2, the S-step update, the Galerkin
-// matrices of the BUG steps.
+// k_rows is row-parallel over the tall dimension (tiles of 16..256 rows,
+// grid-stride over tiles); its products run on FP64 matrix cores and it may
+// end in a global reduction C = out' * W (or the Gram out' * out), written as
+// one partial per workgroup.  k_reduce sums the partials in a fixed order (so
+// results are bitwise reproducible) and k_epilogue runs a small epilogue on C
+// in one workgroup: a Cholesky factorisation for CholeskyQR2, the S-step
+// update, the Galerkin matrices of the BUG steps.
 //
 // These replace the LAPACK/BLAS calls of the reference on the factors:
 // qr!/Matrix(Q) (projector_splitting.jl:137-138,149-150,174-175,186-187;
@@ -39,23 +39,44 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
   return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
+// 1/sqrt(d) for normal d > 0: the hardware estimate refined by two Newton
+// steps y <- y + y (1 - d y^2) / 2.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  double r = __builtin_fma(-d * y, y, 1.0);
+  y = __builtin_fma(0.5 * y, r, y);
+  r = __builtin_fma(-d * y, y, 1.0);
+  return __builtin_fma(0.5 * y, r, y);
+}
+
+// Exponent e (even) with max_diag * 2^-e in [0.5, 2): the Gram matrix is
+// scaled by 2^-e (exactly), so R scales by 2^(-e/2) and every pivot stays a
+// normal number; zero, NaN and Inf give 0.
+__device__ __forceinline__ int chol_scale_exp(double md) {
+  int ex = 0;
+  if (md > 0.0 && isfinite(md)) frexp(md, &ex);
+  return ex & ~1;
+}
+
 // Upper Cholesky G = R'R (k <= K <= 32) and R^{-1}, computed by wave 0 alone
 // with every column in registers.  Lane j holds column j of the trailing
 // matrix, g[t] = G(t, j), and column j of B = L^{-1} (L = R', B starts as I),
 // w[t]; the pivot loop is unrolled over K, so every register index is static
 // and step p touches only rows t > p.  One right-looking sweep yields R and
-// R^{-T} together, with no back substitution.  The pivot comes by readlane; the
-// multipliers R(p, t) are broadcast through an LDS row (double-buffered, and
-// zero past k).  A pivot <= tau = 10 eps max_i G(i,i) marks column p as
-// numerically dependent: dropped[p] = 1, R(p,p) = 1, row p of R is zero and
-// row and column p of R^{-1} are zero (so Q gets a zero column there).
-// R and Rinv are written to LDS (column-major, ld k).  Returns the number of
-// dropped columns; sets *nonfinite on NaN/Inf.  Called by all threads.
+// R^{-T} together, with no back substitution.  The multipliers R(p, t) are
+// broadcast with readlane from lane t (a constant lane per unrolled step), so a
+// step needs no LDS round trip; the pivot of step p+1 is taken as soon as
+// lane p+1 has applied step p to its own diagonal.  G is read by rows (it is
+// symmetric), which keeps the LDS reads free of bank conflicts.  A pivot <=
+// tau = 10 eps max_i G(i,i) marks column p as numerically dependent:
+// dropped[p] = 1, R(p,p) = 1, row p of R is zero and row and column p of
+// R^{-1} are zero (so Q gets a zero column there).  R and Rinv are written to
+// LDS (column-major, ld k).  Returns the number of dropped columns; sets
+// *nonfinite on NaN/Inf.  Called by all threads.
 template <int K>
 __device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv,
                                  int k, unsigned char* dropped, int* nonfinite) {
   __shared__ int s_nd, s_nf;
-  __shared__ double s_r[2][64];  // R(p, c) by column c
   if (threadIdx.x < 64) {
     const int j = threadIdx.x;
     double md = 0.0;
@@ -69,42 +90,43 @@ __device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restric
       md = fmax(md, __shfl_xor(md, off));
       nf |= __shfl_xor(nf, off);
     }
-    const double tau = 10.0 * kEps * md;
+    const int e2 = chol_scale_exp(md), eh = e2 / 2;
+    const double tau = 10.0 * kEps * ldexp(md, -e2);
     double g[K], w[K];
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-      g[t] = (t < k && j < k && t <= j) ? G[t + j * k] : 0.0;
+      g[t] = (t < k && j < k && t <= j) ? ldexp(G[j + t * k], -e2) : 0.0;
       w[t] = (t == j && j < k) ? 1.0 : 0.0;
     }
     int nd = 0;
     bool mydrop = false;
+    double dnext = readlane_f64(g[0], 0);
 #pragma unroll
     for (int p = 0; p < K; ++p) {
       if (p < k) {
-        const double d = readlane_f64(g[p], p);
+        const double d = dnext;
         const bool fin = isfinite(d);
         const bool keep = (d > tau) && fin;
         nf |= !fin;
-        const double piv = keep ? sqrt(d) : 1.0;
-        const double inv = 1.0 / piv;
+        const double inv = keep ? rsqrt_nr(d) : 1.0;
         nd += keep ? 0 : 1;
         mydrop = mydrop || (j == p && !keep);
         const double rpj = (keep && j > p && j < k) ? g[p] * inv : 0.0;  // R(p, j)
         const double xpj = (keep && j < k) ? w[p] * inv : 0.0;           // (L^{-1})(p, j) = R^{-1}(j, p)
+        if (p + 1 < K) dnext = readlane_f64(__builtin_fma(-rpj, rpj, g[p + 1]), p + 1);
         if (j < k) {
-          R[p + j * k] = (j == p) ? piv : (j > p ? rpj : 0.0);
-          Rinv[j + p * k] = mydrop ? 0.0 : xpj;
+          R[p + j * k] = (j == p) ? (keep ? ldexp(d * inv, eh) : 1.0) : (j > p ? ldexp(rpj, eh) : 0.0);
+          Rinv[j + p * k] = mydrop ? 0.0 : ldexp(xpj, -eh);
         }
         if (j == p) dropped[p] = keep ? 0 : 1;
-        s_r[p & 1][j] = rpj;
-        wave_lds_sync();
         // rows t > p of column j: G -= R(p,t) R(p,j), B -= R(p,t) (L^{-1})(p,j)
-        // (entries of g below the diagonal are never read, so they need no mask)
+        // (entries of g below the diagonal are never read, so they need no mask;
+        // lanes t >= k hold R(p,t) = 0)
 #pragma unroll
         for (int t = p + 1; t < K; ++t) {
-          const double rpt = s_r[p & 1][t];
-          g[t] -= rpt * rpj;
-          w[t] -= rpt * xpj;
+          const double rpt = readlane_f64(rpj, t);
+          g[t] = __builtin_fma(-rpt, rpj, g[t]);
+          w[t] = __builtin_fma(-rpt, xpj, w[t]);
         }
       }
     }
@@ -142,30 +164,32 @@ __device__ int chol_inv_window(const double* __restrict__ G, double* __restrict_
       md = fmax(md, __shfl_xor(md, off));
       nf |= __shfl_xor(nf, off);
     }
-    const double tau = 10.0 * kEps * md;
+    const int e2 = chol_scale_exp(md), eh = e2 / 2;
+    const double tau = 10.0 * kEps * ldexp(md, -e2);
     double g[K], w[K];
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-      g[t] = (t < k && j < k && t <= j) ? G[t + j * k] : 0.0;
+      g[t] = (t < k && j < k && t <= j) ? ldexp(G[j + t * k], -e2) : 0.0;  // G symmetric: read by rows
       w[t] = (t == j && j < k) ? 1.0 : 0.0;
     }
     int nd = 0;
     bool mydrop = false;
     wave_lds_sync();
+    double dnext = readlane_f64(g[0], 0);
     for (int p = 0; p < k; ++p) {
-      const double d = readlane_f64(g[0], p);
+      const double d = dnext;
       const bool fin = isfinite(d);
       const bool keep = (d > tau) && fin;
       nf |= !fin;
-      const double piv = keep ? sqrt(d) : 1.0;
-      const double inv = 1.0 / piv;
+      const double inv = keep ? rsqrt_nr(d) : 1.0;
       nd += keep ? 0 : 1;
       mydrop = mydrop || (j == p && !keep);
       const double rpj = (keep && j > p && j < k) ? g[0] * inv : 0.0;
       const double xpj = (keep && j < k) ? w[0] * inv : 0.0;
+      dnext = readlane_f64(__builtin_fma(-rpj, rpj, g[1]), (p + 1) & 63);
       if (j < k) {
-        R[p + j * k] = (j == p) ? piv : (j > p ? rpj : 0.0);
-        Rinv[j + p * k] = mydrop ? 0.0 : xpj;
+        R[p + j * k] = (j == p) ? (keep ? ldexp(d * inv, eh) : 1.0) : (j > p ? ldexp(rpj, eh) : 0.0);
+        Rinv[j + p * k] = mydrop ? 0.0 : ldexp(xpj, -eh);
       }
       if (j == p) dropped[p] = keep ? 0 : 1;
       if (j < 2 * K) s_r[j] = rpj;
@@ -173,8 +197,8 @@ __device__ int chol_inv_window(const double* __restrict__ G, double* __restrict_
 #pragma unroll
       for (int t = 1; t < K; ++t) {
         const double rpi = s_r[p + t];
-        g[t - 1] = g[t] - rpi * rpj;
-        w[t - 1] = w[t] - rpi * xpj;
+        g[t - 1] = __builtin_fma(-rpi, rpj, g[t]);
+        w[t - 1] = __builtin_fma(-rpi, xpj, w[t]);
       }
       g[K - 1] = 0.0;
       w[K - 1] = 0.0;
@@ -420,8 +444,8 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
 // entries [64b, 64b + 64) of the nb partials (each a*b doubles) into C; thread
 // (entry e, quarter g) adds partials g, g+4, ... with 16 loads in flight, and the
 // four quarters are combined in a fixed order (deterministic).
-__global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ gpart, int nb, int ab,
-                                                     double* __restrict__ C) {
+__device__ __forceinline__ void reduce_entries(const double* __restrict__ gpart, int nb, int ab,
+                                               double* __restrict__ C) {
   __shared__ double q4[4][64];
   const int el = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int e = blockIdx.x * 64 + el;
@@ -442,7 +466,14 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ 
   if (g == 0 && e < ab) C[e] = ((q4[0][el] + q4[1][el]) + q4[2][el]) + q4[3][el];
 }
 
-// One workgroup: C (a x b, from k_reduce) into LDS, then the epilogue.
+__global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ gpart, int nb, int ab,
+                                                     double* __restrict__ C) {
+  reduce_entries(gpart, nb, ab, C);
+}
+
+// One workgroup: C (a x b, from k_reduce) into LDS, then the epilogue.  (A
+// variant where the last k_reduce workgroup ran the epilogue after a
+// device-scope release/acquire measured slower than this second launch.)
 template <int CK>
 __global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* __restrict__ Cg, int a, int b) {
   extern __shared__ double lds[];
@@ -452,6 +483,7 @@ __global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* 
   __syncthreads();
   run_epilogue<CK>(E, C, a, b, scr);
 }
+
 
 // ----------------------------------------------------------------------------
 // the row kernel
@@ -535,17 +567,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
 #pragma unroll
   for (int b = 0; b < NRB; ++b) racc[b] = f64x4_t{0.0, 0.0, 0.0, 0.0};
 
-  // xy_sign * op(Y) into LDS once, as Ys[d][c] (ri4 x ro16, zero padded); the
-  // first tile's barrier publishes it
-  if (hasX) {
-    for (int e = t; e < p.ri4 * p.ro16; e += kThreads) {
-      const int c = e % p.ro16, d = e / p.ro16;
-      double y = 0.0;
-      if (c < ro && d < ri) y = A.transY ? A.Y[c + d * A.ldy] : A.Y[d + c * A.ldy];
-      Ys[d * p.ldy + c] = A.xy_sign * y;
-    }
-  }
-
+  bool ys_staged = !hasX;
   const int64_t ntiles = (A.rows + rpb - 1) / rpb;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * rpb;
@@ -605,6 +627,27 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
         }
       }
     }
+    // the first 16 of this thread's X values, loaded before Y is staged so that
+    // both loads are in flight together
+    double xr[16];
+    if (hasX) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int d = cg + u * tpr;
+        xr[u] = (ok && d < ri) ? A.X[i + d * A.ldx] : 0.0;
+      }
+    }
+    if (!ys_staged) {
+      // xy_sign * op(Y) into LDS once, as Ys[d][c] (ri4 x ro16, zero padded);
+      // the barrier below publishes it
+      for (int e = t; e < p.ri4 * p.ro16; e += kThreads) {
+        const int c = e % p.ro16, d = e / p.ro16;
+        double y = 0.0;
+        if (c < ro && d < ri) y = A.transY ? A.Y[c + d * A.ldy] : A.Y[d + c * A.ldy];
+        Ys[d * p.ldy + c] = A.xy_sign * y;
+      }
+      ys_staged = true;
+    }
     if (use_tile) {
       // the tile into LDS: partial outputs, X rows, W rows (zero beyond the data)
 #pragma unroll
@@ -613,8 +656,14 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
         if (c < p.ro16) Ot[lrow * p.ldot + c] = (ok && c < ro) ? v[q] : 0.0;
       }
       for (int c = 8 * tpr + cg; c < p.ro16; c += tpr) Ot[lrow * p.ldot + c] = 0.0;
-      if (hasX)
-        for (int d = cg; d < p.ri4; d += tpr) Xt[lrow * p.ldxt + d] = (ok && d < ri) ? A.X[i + d * A.ldx] : 0.0;
+      if (hasX) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int d = cg + u * tpr;
+          if (d < p.ri4) Xt[lrow * p.ldxt + d] = xr[u];
+        }
+        for (int d = cg + 16 * tpr; d < p.ri4; d += tpr) Xt[lrow * p.ldxt + d] = (ok && d < ri) ? A.X[i + d * A.ldx] : 0.0;
+      }
       if (hasW && reduce)
         for (int c = cg; c < p.rw16; c += tpr) Wt[lrow * p.ldwt + c] = (ok && c < rw) ? A.W[i + c * A.ldw] : 0.0;
       __syncthreads();
@@ -736,18 +785,17 @@ hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !A.reduce) return e;
   const int ab = A.r_out * rw;
-  hipLaunchKernelGGL(k_reduce, dim3((ab + 63) / 64), dim3(kThreads), 0, st, (const double*)A.gpart, grid, ab,
-                     A.epi.small->Cred);
+  double* Cr = A.epi.small->Cred;
+  hipLaunchKernelGGL(k_reduce, dim3((ab + 63) / 64), dim3(kThreads), 0, st, (const double*)A.gpart, grid, ab, Cr);
   e = hipGetLastError();
   if (e != hipSuccess || A.epi.kind == EPI_NONE) return e;
-  const double* Cr = A.epi.small->Cred;
   const bool chol = A.epi.kind == EPI_CHOL1 || A.epi.kind == EPI_CHOL2;
   if (chol && A.r_out <= 16)
-    hipLaunchKernelGGL(k_epilogue<16>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, Cr, A.r_out, rw);
+    hipLaunchKernelGGL(k_epilogue<16>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rw);
   else if (chol && A.r_out <= 32)
-    hipLaunchKernelGGL(k_epilogue<32>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, Cr, A.r_out, rw);
+    hipLaunchKernelGGL(k_epilogue<32>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rw);
   else
-    hipLaunchKernelGGL(k_epilogue<64>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, Cr, A.r_out, rw);
+    hipLaunchKernelGGL(k_epilogue<64>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rw);
   return hipGetLastError();
 }
 

@@ -82,3 +82,45 @@ def test_bf16_encoding_round_to_nearest_even(lri):
     assert back[0, 0] == 1.0 and back[0, 1] == 1.0
     assert abs(back[0, 3] - np.float32(-3.140625)) == 0
     assert back[0, 4] == 0.0
+
+
+def test_operator_descriptor_matches_the_oracle(lri):
+    """Host side of the matrix-ODE descriptor: the tridiagonal, its bound and F agree with the oracle's."""
+    from oracle import dlra_de
+
+    n, m = 12, 9
+    A = lri.laplacian_1d(n)
+    assert np.array_equal(A.dense(), dlra_de.dirichlet_laplacian(n))
+    P = lri.laplacian_1d(m, 2.0, periodic=True)
+    Pd = P.dense()
+    assert Pd[0, m - 1] == 2.0 and Pd[m - 1, 0] == 2.0 and np.allclose(Pd, Pd.T)
+    for T in (A, P, lri.SymTridiagonal(np.arange(5.0), [1.0, -2.0, 0.5, 3.0])):
+        assert T.gershgorin() >= np.max(np.abs(np.linalg.eigvalsh(T.dense()))) - 1e-12
+    rng = np.random.default_rng(0)
+    G, H = rng.standard_normal((n, 2)), rng.standard_normal((m, 2))
+    F = lri.LyapunovOperator(A, P, G, H)
+    Fo = dlra_de.LyapunovOperator(A.dense(), Pd, G, H)
+    Y = rng.standard_normal((n, m))
+    assert np.allclose(F(Y), Fo(Y))
+    assert lri.rk4_substeps(F.spectral_bound(), 1e-3) == dlra_de.rk4_substeps(F.spectral_bound(), 1e-3)
+
+
+def test_operator_descriptor_validation(lri):
+    with pytest.raises(ValueError):
+        lri.SymTridiagonal(np.ones(4), np.ones(4))  # Dirichlet needs n - 1 off-diagonals
+    with pytest.raises(ValueError):
+        lri.SymTridiagonal(np.ones(2), np.ones(2), periodic=True)
+    with pytest.raises(ValueError):
+        lri.LyapunovOperator(lri.laplacian_1d(4), lri.laplacian_1d(4), np.ones((4, 5)), np.ones((4, 5)))
+    with pytest.raises(ValueError):
+        lri.LyapunovOperator(lri.laplacian_1d(4), lri.laplacian_1d(4), np.ones((4, 1)), None)
+    u0 = lri.SVDLikeRepresentation(np.eye(4, 2), np.eye(2), np.eye(4, 2))
+    F = lri.LyapunovOperator(lri.laplacian_1d(4), lri.laplacian_1d(5))
+    with pytest.raises(ValueError):  # operator 4 x 5 vs state 4 x 4
+        lri.init(lri.MatrixDEProblem(F, u0, (0.0, 1.0)), lri.UnconventionalAlgorithm(), 0.1)
+
+
+def test_de_entry_points_reject_a_null_context(native):
+    assert native.lri_step_de_bug(None, 0.1, 1) == -1
+    assert native.lri_step_de_ksl(None, 0, 0.1, 1) == -1
+    assert native.lri_set_operator_lyapunov(None, None, None, 0, None, None, 0, 0, None, 0, None, 0) == -1
