@@ -25,6 +25,7 @@ Reported beside the headline value:
   sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --workload de      (the matrix-ODE BUG step, n = m = 4096)
 """
 from __future__ import annotations
 
@@ -184,6 +185,115 @@ def cpu_baseline(n, r, steps=3):
     }
 
 
+# ---------------------------------------------------------------------------- matrix-ODE workload
+
+
+def de_cpu_baseline(n, r, dt, N, seed):
+    """Oracle BUG step in factored form (sparse Laplacian, O((n + m) r^2) per right-hand side)."""
+    import scipy.sparse as sp
+    from oracle import dlra_de
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    U, S, V, a, b = de_initial(n, r, seed)
+    L = sp.csr_matrix(sp.diags([np.ones(n - 1), -2.0 * np.ones(n), np.ones(n - 1)], [-1, 0, 1]) * float(n * n))
+    op = dlra_de.LowRankLyapunov(L, L, a, b)
+    Ns = min(N, 1000)  # same sub-step size h = dt / N, over Ns sub-steps
+    t0 = time.perf_counter()
+    dlra_de.bug_de_step_lowrank(op, U, S, V, 0.0, dt * Ns / N, Ns)
+    el = time.perf_counter() - t0
+    return {
+        "value": Ns / (N * el),
+        "unit": "steps/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"one BUG step of the FP64 NumPy/SciPy oracle with the right-hand sides in factored form (sparse "
+                  f"Laplacian) at n=m={n}, r={r}, run with {Ns} of the {N} RK4 sub-steps per sub-problem (same "
+                  f"sub-step size) and scaled by {Ns}/{N}; the RK4 sub-steps are >99% of the oracle's step time; "
+                  f"OpenBLAS threads={threads}",
+    }
+
+
+def de_initial(n, r, seed):
+    rng = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    S = np.diag(10.0 ** (-np.arange(r) / 4.0))
+    a = rng.standard_normal((n, 1))
+    b = rng.standard_normal((n, 1))
+    return U, S, V, a, b
+
+
+def run_de(args, torch, lri):
+    """dX/dt = L X + X L' + a b' (L: 1-D Dirichlet Laplacian n^2 [1,-2,1]), BUG, FP64, RK4 sub-steps."""
+    n = m = args.de_n
+    r = args.rank
+    dt = 1e-4
+    U0, S0, V0, a, b = de_initial(n, r, args.seed)
+    Lop = lri.laplacian_1d(n)
+    F = lri.LyapunovOperator(Lop, Lop, a, b)
+    N = lri.rk4_substeps(F.spectral_bound(), dt)
+    ctx = lri.native.Context(n, m, r, 0)
+    ctx.set_state(U0, S0, V0)
+    ctx.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False, a, b)
+    log(f"DE workload n=m={n} r={r} dt={dt}: {N} RK4 sub-steps per sub-problem; warm-up {args.warmup_de}")
+    for _ in range(args.warmup_de):
+        ctx.step_de_bug(dt, N)
+    ctx.set_state(U0, S0, V0)
+    ctx.timing(1)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.de_steps):
+        ctx.step_de_bug(dt, N)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = args.de_steps / el
+    de_ms, de_cnt = ctx.timing_read(lri.native.KERNEL_DE)
+    sm_ms, sm_cnt = ctx.timing_read(lri.native.KERNEL_SMALL)
+    ctx.timing(0)
+    ctx.close()
+    avg_ms = de_ms / max(de_cnt, 1)
+    flops = 37.0 * N * r * (n + m)  # RK4 on the K and L columns of one launch (37 flops per row per sub-step)
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    log(f"DE BUG: {value:.2f} steps/s ({1e3 * el / args.de_steps:.2f} ms/step); RK4 launch {avg_ms:.3f} ms")
+    out = {
+        "metric": "steps/s of the BUG integrator on dX/dt = L X + X L' + a b' at n=m=4096, r=32 (FP64, RK4 sub-steps)",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.de_steps,
+        "warmup": args.warmup_de,
+        "ms_per_step": 1e3 * el / args.de_steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"unconventional (BUG) integrator on a MatrixDEProblem, n=m={n}, r={r}, dt={dt}, "
+                               f"L = n^2 tridiag(1,-2,1) Dirichlet, G = a b' (seeded N(0,1)), Y0 = U diag(10^(-i/4)) V', "
+                               f"{N} fixed RK4 sub-steps per sub-problem", "n": n, "m": m, "r": r, "dt": dt,
+                   "substeps": N, "parallelism": "single GPU"},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": achieved / 78.6, "traffic": None, "kernel": "k_rk4_columns",
+                     "flops_per_launch": flops,
+                     "note": "one workgroup per column: 2r = 64 of 256 CUs busy"},
+        "kernels": {"rk4": {"total_ms": de_ms, "launches": de_cnt, "avg_ms": avg_ms},
+                    "small": {"total_ms": sm_ms, "launches": sm_cnt, "avg_ms": sm_ms / max(sm_cnt, 1)}},
+    }
+    if not args.no_cpu:
+        log("CPU baseline (oracle, factored right-hand sides) on one step")
+        out["cpu_baseline"] = de_cpu_baseline(n, r, dt, N, args.seed)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +309,11 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the error check")
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP-event kernel timing in the timed region (for external profilers)")
+    ap.add_argument("--workload", choices=["ksl", "de"], default="ksl",
+                    help="ksl: the headline data integrator; de: the BUG matrix-ODE step at n=m=4096")
+    ap.add_argument("--de-n", type=int, default=4096)
+    ap.add_argument("--de-steps", type=int, default=20)
+    ap.add_argument("--warmup-de", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB buffer (a plain copy kernel of known traffic, to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -217,6 +332,9 @@ def main():
     lri.native.lib()
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
+    if args.workload == "de":
+        run_de(args, torch, lri)
+        return
     dev = torch.device("cuda:0")
     n = m = args.n
     r = args.rank

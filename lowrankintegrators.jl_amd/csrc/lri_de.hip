@@ -35,10 +35,13 @@ namespace lri {
 // ----------------------------------------------------------------------------
 
 template <int RPT, bool REG>
-__global__ __launch_bounds__(1024) void k_rk4_columns(RK4ColArgs P) {
+__global__ __launch_bounds__(1024) void k_rk4_columns(RK4ColArgs P0, RK4ColArgs P1) {
   __shared__ double s_first[2][1024], s_last[2][1024];
+  // workgroups [0, P0.ncols) integrate the columns of P0, the rest those of P1
+  const bool second = (int)blockIdx.x >= P0.ncols;
+  const RK4ColArgs P = second ? P1 : P0;
   const int t = threadIdx.x;
-  const int j = blockIdx.x;
+  const int j = second ? (int)blockIdx.x - P0.ncols : (int)blockIdx.x;
   const int64_t n = P.n;
   const int64_t r0 = (int64_t)t * RPT;
   const int last_t = (int)((n - 1) / RPT);
@@ -138,20 +141,28 @@ __global__ __launch_bounds__(1024) void k_rk4_columns(RK4ColArgs P) {
     if (u < nmine) P.Y[r0 + u + (int64_t)j * P.ldy] = y[u];
 }
 
-hipError_t launch_rk4_columns(const RK4ColArgs& P, hipStream_t st) {
-  if (P.ncols < 1 || P.nsub < 1) return hipSuccess;
-  if (P.n < 1 || P.n > 16384 || P.q < 0 || P.q > kMaxForce) return hipErrorInvalidValue;
+hipError_t launch_rk4_columns(const RK4ColArgs& P0, const RK4ColArgs& P1, hipStream_t st) {
+  const RK4ColArgs* ps[2] = {&P0, &P1};
+  int64_t nmax = 0;
+  for (const RK4ColArgs* P : ps) {
+    if (P->ncols < 1) continue;
+    if (P->nsub < 1 || P->n < 1 || P->n > 16384 || P->q < 0 || P->q > kMaxForce) return hipErrorInvalidValue;
+    nmax = P->n > nmax ? P->n : nmax;
+  }
+  if (P0.ncols < 0 || P1.ncols < 0) return hipErrorInvalidValue;
+  const int ncols = P0.ncols + P1.ncols;
+  if (ncols < 1) return hipSuccess;
   int rpt = 4;
-  while ((int64_t)rpt * 1024 < P.n) rpt *= 2;
-  int nt = (int)((P.n + rpt - 1) / rpt);
+  while ((int64_t)rpt * 1024 < nmax) rpt *= 2;
+  int nt = (int)((nmax + rpt - 1) / rpt);
   nt = (nt + 63) & ~63;
-  const dim3 grid((unsigned)P.ncols), blk((unsigned)nt);
+  const dim3 grid((unsigned)ncols), blk((unsigned)nt);
   if (rpt == 4)
-    hipLaunchKernelGGL((k_rk4_columns<4, true>), grid, blk, 0, st, P);
+    hipLaunchKernelGGL((k_rk4_columns<4, true>), grid, blk, 0, st, P0, P1);
   else if (rpt == 8)
-    hipLaunchKernelGGL((k_rk4_columns<8, true>), grid, blk, 0, st, P);
+    hipLaunchKernelGGL((k_rk4_columns<8, true>), grid, blk, 0, st, P0, P1);
   else
-    hipLaunchKernelGGL((k_rk4_columns<16, false>), grid, blk, 0, st, P);
+    hipLaunchKernelGGL((k_rk4_columns<16, false>), grid, blk, 0, st, P0, P1);
   return hipGetLastError();
 }
 

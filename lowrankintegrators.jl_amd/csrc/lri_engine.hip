@@ -63,8 +63,8 @@ struct DeSmall {
   double beta[kMaxAug];
   double gaT[kMaxForce * kMaxAug];  // G'U (q x k)
   double hbT[kMaxForce * kMaxAug];  // H'V (q x k)
-  double CW[kMaxAug * kMaxAug];
-  double coef[kMaxForce * kMaxAug];
+  double CW[2][kMaxAug * kMaxAug];       // op(C0) W of up to two concurrent sub-problems
+  double coef[2][kMaxForce * kMaxAug];
   double Snew[kMaxAug * kMaxAug];
   double ework[kMaxAug * kMaxAug + 2];
   double Uscr[kMaxAug * kMaxAug];
@@ -289,8 +289,13 @@ struct lri_ctx {
   // CholeskyQR2 of X (rows x r, ld rows) given the first kernel already
   // produced X and EPI_CHOL1 (R1 in small).  Writes Q to `out` (ld rows),
   // Rtot in small; optional packed output and cross reduction with W.
+  //
+  // passes = 3 is shifted CholeskyQR3: the first factor (EPI_CHOL1 with
+  // shift_rows) came from G + s I, so Q1 = X R1^{-1} is only moderately
+  // conditioned; two more CholeskyQR passes make it orthogonal to O(eps) for
+  // cond(X) up to ~1/eps.  X is overwritten by the middle pass.
   void cholqr_tail(double* X, double* tmp, int64_t rows_, int rr, double* out, bool pk_forN, const double* W,
-                   int rw, EpiArgs final_epi, int sfinal) {
+                   int rw, EpiArgs final_epi, int sfinal, int passes = 2) {
     RowArgs<double> B;
     B.rows = rows_;
     B.r_out = rr;
@@ -304,15 +309,36 @@ struct lri_ctx {
     B.reduce = 1;
     B.epi.kind = EPI_CHOL2;
     B.epi.breakdown_is_error = final_epi.breakdown_is_error;
-    B.epi.S = sfinal ? S : nullptr;
+    B.epi.S = (sfinal && passes == 2) ? S : nullptr;
     B.epi.ld_s = (int)rcap;
     B.epi.transpose = sfinal == 2;
     rows(B);
+    const double* last = tmp;
+    if (passes == 3) {
+      RowArgs<double> B3;  // X = tmp R2^{-1}; its Gram -> R3, Rtot = R3 R2 R1
+      B3.rows = rows_;
+      B3.r_out = rr;
+      B3.r_in = rr;
+      B3.X = tmp;
+      B3.ldx = rows_;
+      B3.Y = small->R2inv;
+      B3.ldy = rr;
+      B3.out = X;
+      B3.ldo = rows_;
+      B3.reduce = 1;
+      B3.epi.kind = EPI_CHOL3;
+      B3.epi.breakdown_is_error = final_epi.breakdown_is_error;
+      B3.epi.S = sfinal ? S : nullptr;
+      B3.epi.ld_s = (int)rcap;
+      B3.epi.transpose = sfinal == 2;
+      rows(B3);
+      last = X;
+    }
     RowArgs<double> C;
     C.rows = rows_;
     C.r_out = rr;
     C.r_in = rr;
-    C.X = tmp;
+    C.X = last;
     C.ldx = rows_;
     C.Y = small->R2inv;
     C.ldy = rr;
@@ -732,21 +758,25 @@ struct lri_ctx {
   // One K-type sub-problem on side `nside` (rows n, operator A, forcing G; or
   // rows m, operator B, forcing H):  Z' = T Z + Z Co + F FXo,  Z0 = X0 op(C0),
   // where Co = Wo diag(ev) Wo' is the other side's Galerkin matrix and FXo its
-  // forcing projection (q x k).  Writes Z(t + dt) to `out` with the reduction
-  // `epi` on its Gram matrix (EPI_NONE: no reduction).
-  void de_substep(const double* X0, bool nside, int k, const double* C0, int ldc, int transC, const double* Wo,
-                  const double* ev, const double* FXo, double h, int64_t nsub, double* out, double* scratch,
-                  EpiArgs epi) {
+  // forcing projection (q x k).  de_prepare writes Zbar0 = Z0 Wo to `scratch`
+  // (slot: which of the two concurrent sub-problems) and returns the RK4
+  // arguments; de_run integrates one or two prepared sub-problems in one
+  // launch; de_finish writes Z = Zbar Wo' to `out` with the reduction `epi` on
+  // its Gram matrix (EPI_NONE: no reduction).
+  RK4ColArgs de_prepare(const double* X0, bool nside, int k, const double* C0, int ldc, int transC,
+                        const double* Wo, const double* ev, const double* FXo, int slot, double h, int64_t nsub,
+                        double* scratch) {
     const int64_t rows_ = nside ? n : m;
-    timed(LRI_KERNEL_SMALL,
-          [&] { return launch_de_prep(C0, ldc, transC, Wo, FXo, k, op_q, de->CW, de->coef, stream); });
+    double* CW = de->CW[slot];
+    double* coef = de->coef[slot];
+    timed(LRI_KERNEL_SMALL, [&] { return launch_de_prep(C0, ldc, transC, Wo, FXo, k, op_q, CW, coef, stream); });
     RowArgs<double> A;  // Zbar0 = X0 op(C0) Wo
     A.rows = rows_;
     A.r_out = k;
     A.r_in = k;
     A.X = X0;
     A.ldx = rows_;
-    A.Y = de->CW;
+    A.Y = CW;
     A.ldy = k;
     A.out = scratch;
     A.ldo = rows_;
@@ -764,11 +794,17 @@ struct lri_ctx {
     P.mu = ev;
     P.g = nside ? op_G : op_H;
     P.ldg = rows_;
-    P.coef = de->coef;
+    P.coef = coef;
     P.q = op_q;
     P.h = h;
     P.nsub = nsub;
-    timed(LRI_KERNEL_DE, [&] { return launch_rk4_columns(P, stream); });
+    return P;
+  }
+  void de_run(const RK4ColArgs& P0, const RK4ColArgs& P1 = RK4ColArgs()) {
+    timed(LRI_KERNEL_DE, [&] { return launch_rk4_columns(P0, P1, stream); });
+  }
+  void de_finish(bool nside, int k, const double* Wo, double* out, const double* scratch, EpiArgs epi) {
+    const int64_t rows_ = nside ? n : m;
     RowArgs<double> B;  // Z = Zbar Wo'
     B.rows = rows_;
     B.r_out = k;
@@ -785,6 +821,12 @@ struct lri_ctx {
       B.epi = epi;
     }
     rows(B);
+  }
+  void de_substep(const double* X0, bool nside, int k, const double* C0, int ldc, int transC, const double* Wo,
+                  const double* ev, const double* FXo, double h, int64_t nsub, double* out, double* scratch,
+                  EpiArgs epi) {
+    de_run(de_prepare(X0, nside, k, C0, ldc, transC, Wo, ev, FXo, 0, h, nsub, scratch));
+    de_finish(nside, k, Wo, out, scratch, epi);
   }
   void de_sstep(int k, int l, const double* S0, int lds0, int transS0, double* S1, int lds1, double sgn, double h,
                 int64_t nsub) {
@@ -811,59 +853,77 @@ struct lri_ctx {
   }
 
   // primal_LT_step! with the DE sub-problems (projector_splitting.jl:52-80,129-152)
+  // The QRs of the matrix-ODE sub-steps use shifted CholeskyQR3: a stiff or
+  // strongly graded state (e.g. singular values down to 1e-8) makes the K and
+  // L factors too ill-conditioned for CholeskyQR2.  A numerically dependent
+  // direction (a stiff sub-step can collapse K onto a few directions) becomes
+  // a zero column instead of an error; Householder QR would put an arbitrary
+  // orthonormal direction there.
+  static EpiArgs shifted_chol(int64_t rows_) {
+    EpiArgs e;
+    e.kind = EPI_CHOL1;
+    e.shift_rows = rows_;
+    e.breakdown_is_error = 0;
+    return e;
+  }
+  static EpiArgs de_epi(int kind = EPI_NONE) {
+    EpiArgs e;
+    e.kind = kind;
+    e.breakdown_is_error = 0;
+    return e;
+  }
+
   void de_ksl_primal(double dt, int64_t nsub) {
     const int rr = (int)r;
     const double h = dt / (double)nsub;
-    EpiArgs chol, none;
-    chol.kind = EPI_CHOL1;
+    const EpiArgs none = de_epi();
     de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
     de_eig(de->Gb, rr, de->Wb, de->beta);
-    de_substep(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, chol);  // K
-    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 0);                                   // U1, R
+    de_substep(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, shifted_chol(n));  // K
+    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 0, 3);                                            // U1, R
     de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U1'AU1
     de_eig(de->Ga, rr, de->Wa, de->alpha);
     de_sstep(rr, rr, small->Rtot, rr, 0, de->Snew, rr, -1.0, h, nsub);  // S~
-    de_substep(V, false, rr, de->Snew, rr, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, chol);  // L
-    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 2);  // V1, S = R'
+    de_substep(V, false, rr, de->Snew, rr, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, shifted_chol(m));  // L
+    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 2, 3);  // V1, S = R'
   }
 
   // dual_LT_step! with the DE sub-problems (projector_splitting.jl:52-80,166-189)
   void de_ksl_dual(double dt, int64_t nsub) {
     const int rr = (int)r;
     const double h = dt / (double)nsub;
-    EpiArgs chol, none;
-    chol.kind = EPI_CHOL1;
+    const EpiArgs none = de_epi();
     de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U0'AU0
     de_eig(de->Ga, rr, de->Wa, de->alpha);
-    de_substep(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, chol);  // L
-    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 0);                                     // V1, R
+    de_substep(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, shifted_chol(m));  // L
+    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 0, 3);                                              // V1, R
     de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V1'BV1
     de_eig(de->Gb, rr, de->Wb, de->beta);
     de_sstep(rr, rr, small->Rtot, rr, 1, de->Snew, rr, -1.0, h, nsub);  // S~ from R'
-    de_substep(U, true, rr, de->Snew, rr, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, chol);  // K
-    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 1);  // U1, S = R
+    de_substep(U, true, rr, de->Snew, rr, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, shifted_chol(n));  // K
+    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 1, 3);  // U1, S = R
   }
 
   // unconventional_step! with the DE sub-problems (unconventional.jl:51-79,133-157)
   void de_bug(double dt, int64_t nsub) {
     const int rr = (int)r;
     const double h = dt / (double)nsub;
-    EpiArgs chol;
-    chol.kind = EPI_CHOL1;
     de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
     de_eig(de->Gb, rr, de->Wb, de->beta);
-    de_substep(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, chol);
-    EpiArgs eM;
-    eM.kind = EPI_BUG_M;  // M = U1' U0
-    cholqr_tail(K, K2, n, rr, U2, false, U, rr, eM, 0);
     de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U0'AU0
     de_eig(de->Ga, rr, de->Wa, de->alpha);
-    de_substep(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, chol);
-    EpiArgs eN;
-    eN.kind = EPI_BUG_N;  // N = V1' V0, Sinit = M S0 N'
+    // the K and L sub-problems are independent: one RK4 launch for both
+    const RK4ColArgs Pk = de_prepare(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, 0, h, nsub, K2);
+    const RK4ColArgs Pl = de_prepare(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, 1, h, nsub, L2);
+    de_run(Pk, Pl);
+    de_finish(true, rr, de->Wb, K, K2, shifted_chol(n));
+    EpiArgs eM = de_epi(EPI_BUG_M);  // M = U1' U0
+    cholqr_tail(K, K2, n, rr, U2, false, U, rr, eM, 0, 3);
+    de_finish(false, rr, de->Wa, L, L2, shifted_chol(m));
+    EpiArgs eN = de_epi(EPI_BUG_N);  // N = V1' V0, Sinit = M S0 N'
     eN.S = S;
     eN.ld_s = (int)rcap;
-    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0);
+    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0, 3);
     de_galerkin(U2, true, rr, de->Ga, de->gaT);  // At = U1'AU1
     de_eig(de->Ga, rr, de->Wa, de->alpha);
     de_galerkin(V2, false, rr, de->Gb, de->hbT);  // Bt = V1'BV1
@@ -882,10 +942,13 @@ struct lri_ctx {
     EpiArgs none;
     de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
     de_eig(de->Gb, rr, de->Wb, de->beta);
-    de_substep(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, h, nsub, K, K2, none);
     de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U0'AU0
     de_eig(de->Ga, rr, de->Wa, de->alpha);
-    de_substep(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, h, nsub, L, L2, none);
+    const RK4ColArgs Pk = de_prepare(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, 0, h, nsub, K2);
+    const RK4ColArgs Pl = de_prepare(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, 1, h, nsub, L2);
+    de_run(Pk, Pl);
+    de_finish(true, rr, de->Wb, K, K2, none);
+    de_finish(false, rr, de->Wa, L, L2, none);
     HIPCHECK(hipMemcpyAsync(Uhat, U, sizeof(double) * n * rr, hipMemcpyDeviceToDevice, stream));
     EpiArgs eM;
     eM.kind = EPI_RA_M;

@@ -39,6 +39,7 @@ enum EpiKind {
   EPI_RA_S,    // Shat = Sinit + C'
   EPI_STORE,   // Ctmp = C
   EPI_COPY,    // dst = C (a x b, ld a)
+  EPI_CHOL3,   // third factor of shifted CholeskyQR3: C = Gram -> R3^{-1} (into R2inv), Rtot = R3 Rtot; if S: S = Rtot
 };
 
 struct EpiArgs {
@@ -50,6 +51,9 @@ struct EpiArgs {
   double* S = nullptr;          // state S (EPI_SFINAL / EPI_BUG_* / EPI_RA_N)
   int ld_s = 0;
   double* dst = nullptr;        // EPI_COPY
+  // EPI_CHOL1: when > 0, factor G + s I with s = 11 (rows k + k (k + 1)) u trace(G)
+  // (the shift of shifted CholeskyQR3 for an n x k factor with n = shift_rows)
+  int64_t shift_rows = 0;
 };
 
 // Row kernel (see lri_rowops.hip).
@@ -167,7 +171,9 @@ struct RK4ColArgs {
   double h = 0.0;
   int64_t nsub = 0;
 };
-hipError_t launch_rk4_columns(const RK4ColArgs& P, hipStream_t st);
+// One launch for the columns of two sub-problems (P1.ncols may be 0), e.g. the
+// independent K and L steps of the unconventional integrators.
+hipError_t launch_rk4_columns(const RK4ColArgs& P0, const RK4ColArgs& P1, hipStream_t st);
 
 // Small preparation for a rotated sub-problem: CW = op(C) W (k x k), coef = FX W (q x k).
 hipError_t launch_de_prep(const double* C, int ldc, int transC, const double* W, const double* FX, int k, int q,

@@ -242,6 +242,7 @@ __host__ __device__ inline int epi_scratch_doubles(int kind, int a, int b) {
   switch (kind) {
     case EPI_CHOL1:
     case EPI_CHOL2:
+    case EPI_CHOL3:
     case EPI_BUG_N:
       return 2 * k * k;
     case EPI_RA_N:
@@ -261,10 +262,25 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
   SmallState* st = E.small;
   switch (E.kind) {
     case EPI_CHOL1:
-    case EPI_CHOL2: {
+    case EPI_CHOL2:
+    case EPI_CHOL3: {
       const int k = a;
       __shared__ unsigned char dropped[kMaxAug];
       __shared__ int s_big;
+      __shared__ double s_shift;
+      if (E.kind == EPI_CHOL1 && E.shift_rows > 0) {
+        // shifted CholeskyQR3, first factor: G + s I (Fukaya, Kannan, Nakatsukasa, Yamamoto,
+        // Yanagisawa 2020), with ||G||_2 bounded by trace(G)
+        if (t < 64) {
+          double tr = 0.0;
+          for (int i = t; i < k; i += 64) tr += C[i + i * k];
+          for (int off = 32; off > 0; off >>= 1) tr += __shfl_xor(tr, off);
+          if (t == 0) s_shift = 11.0 * ((double)E.shift_rows * k + (double)k * (k + 1)) * (0.5 * kEps) * tr;
+        }
+        __syncthreads();
+        for (int i = t; i < k; i += blockDim.x) C[i + i * k] += s_shift;
+        __syncthreads();
+      }
       double* R = scr;
       double* Rinv = scr + k * k;
       int nf = 0, nd = 0;
@@ -273,7 +289,7 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
       // F = F1 + F2, F1 = upper(E) (diagonal halved), F2 = upper(-F1'F1)
       // (diagonal halved), and R^{-1} = I - F + F^2, all exact to O(|E|^3).
       bool pert = false;
-      if (E.kind == EPI_CHOL2 && st->ndropped == 0) {
+      if (E.kind != EPI_CHOL1 && st->ndropped == 0) {
         if (t == 0) s_big = 0;
         __syncthreads();
         double mx = 0.0;
@@ -324,10 +340,12 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
           st->R1inv[e] = Rinv[e];
         }
       } else {
-        // Rtot = R2 * R1 (R1 was written by an earlier launch on this stream); C is free now
+        // Rtot = R2 * R1 (CHOL2) or R3 * Rtot (CHOL3); the factor on the right was
+        // written by an earlier launch on this stream; C is free now
+        const double* right = E.kind == EPI_CHOL2 ? st->R1 : st->Rtot;
         for (int e = t; e < k * k; e += blockDim.x) {
           st->R2inv[e] = Rinv[e];
-          C[e] = st->R1[e];
+          C[e] = right[e];
         }
         __syncthreads();
         small_gemm(R, 0, C, 0, st->Rtot, k);
@@ -789,7 +807,7 @@ hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
   hipLaunchKernelGGL(k_reduce, dim3((ab + 63) / 64), dim3(kThreads), 0, st, (const double*)A.gpart, grid, ab, Cr);
   e = hipGetLastError();
   if (e != hipSuccess || A.epi.kind == EPI_NONE) return e;
-  const bool chol = A.epi.kind == EPI_CHOL1 || A.epi.kind == EPI_CHOL2;
+  const bool chol = A.epi.kind == EPI_CHOL1 || A.epi.kind == EPI_CHOL2 || A.epi.kind == EPI_CHOL3;
   if (chol && A.r_out <= 16)
     hipLaunchKernelGGL(k_epilogue<16>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rw);
   else if (chol && A.r_out <= 32)

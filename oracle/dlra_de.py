@@ -146,6 +146,55 @@ def rank_adaptive_bug_de_step(F, U, S, V, t, dt, N, tol, rmax):
     return Uh @ Us[:, :r1], np.diag(sig[:r1]), Vh @ Vst[:r1].T, r1
 
 
+class LowRankLyapunov:
+    """F(Y) = A Y + Y B' + G H' applied to low-rank arguments without forming n x m
+    matrices, as LowRankArithmetic propagates the factors of a TwoFactor /
+    SVDLike argument through such an F (the algebra below is exact).  A and B
+    may be scipy sparse matrices; G is n x q and H is m x q."""
+
+    def __init__(self, A, B, G, H):
+        self.A, self.B = A, B
+        self.G = np.asarray(G, dtype=np.float64).reshape(A.shape[0], -1)
+        self.H = np.asarray(H, dtype=np.float64).reshape(B.shape[0], -1)
+
+    def dense(self) -> LyapunovOperator:
+        d = lambda X: X.toarray() if hasattr(X, "toarray") else np.asarray(X)
+        return LyapunovOperator(d(self.A), d(self.B), self.G, self.H)
+
+    def K_rhs(self, V):
+        """K' = F(K V') V = A K + K (V'B'V) + G (H'V)  (V'V = I)."""
+        Bt = V.T @ (self.B.T @ V)
+        c = self.G @ (self.H.T @ V)
+        return lambda K, t: self.A @ K + K @ Bt + c
+
+    def L_rhs(self, U):
+        """L' = F(U L')' U = B L + L (U'AU)' + H (G'U)  (U'U = I)."""
+        At = U.T @ (self.A @ U)
+        c = self.H @ (self.G.T @ U)
+        return lambda L, t: self.B @ L + L @ At.T + c
+
+    def S_rhs(self, U, V, sign=1.0):
+        """S' = sign U'F(U S V')V = sign ((U'AU) S + S (V'B'V) + (U'G)(H'V))."""
+        At = U.T @ (self.A @ U)
+        Bt = V.T @ (self.B.T @ V)
+        c = (U.T @ self.G) @ (self.H.T @ V)
+        return lambda S_, t: sign * (At @ S_ + S_ @ Bt + c)
+
+
+def bug_de_step_lowrank(op: LowRankLyapunov, U, S, V, t, dt, N):
+    """unconventional.jl:133-157 with the DE sub-problems evaluated in factored form
+    (the cost profile of the reference with LowRankArithmetic: O((n + m) r^2) per
+    right-hand side instead of O(n m r))."""
+    K = rk4(op.K_rhs(V), U @ S, t, dt, N)
+    U1, _ = _qr(K)
+    M = U1.T @ U
+    L = rk4(op.L_rhs(U), V @ S.T, t, dt, N)
+    V1, _ = _qr(L)
+    Nn = V1.T @ V
+    S1 = rk4(op.S_rhs(U1, V1), M @ S @ Nn.T, t, dt, N)
+    return U1, S1, V1
+
+
 def solve_de(prob: MatrixDEProblem, alg, dt, substeps: int) -> DLRSolution:
     """``solve(prob::MatrixDEProblem, alg, dt)`` (primitives.jl:68-76) with RK4 sub-steps."""
     t0, tf = prob.tspan

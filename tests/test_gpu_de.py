@@ -253,3 +253,42 @@ def test_operator_validation(lri):
             ctx.step_de_ksl(7, 0.1, 2)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("name", ["bug", "primal", "dual"])
+def test_graded_state_needs_shifted_cholqr(lri, name):
+    """Singular values 10^(-i/4), i < 32 (cond ~ 6e7, the initial data of the 4096 x 4096
+    heat-equation workload): CholeskyQR2's Gram matrix would be numerically singular;
+    the shifted CholeskyQR3 of the DE steps matches the oracle's Householder QR.  Y is
+    compared to 1e-10; the directions of the smallest singular values are determined
+    only to ~eps / sigma_min (~1e-8), so the subspaces get a 1e-6 bound."""
+    n, m, r = 200, 160, 32
+    say(f"case: graded state {name}, n={n} m={m} r={r}")
+    dev, host = operator_pair(lri, n, m, 1, False, False, seed=12, sa=float(n * n), sb=float(m * m))
+    u = random_factors(np.random.default_rng(13), n, m, r, decay=0.25)
+    dt = 1e-5
+    N = lri.rk4_substeps(dev.spectral_bound(), dt)
+    got = run_one(lri, name, dev, u, dt, N)
+    ref = oracle_one(name, host, u, dt, N)
+    assert_close(got, ref, 1e-10, 1e-6)
+
+
+def test_stiff_collapse_is_not_an_error(lri):
+    """dt * rho ~ 50: the K and L sub-steps damp every direction but the forcing's
+    steady state, so K1 is numerically rank-deficient.  The engine keeps the
+    dependent directions as zero columns (Householder QR would complete them
+    arbitrarily), so the step must not fail, and Y must still agree with the
+    oracle up to the size of what the oracle's arbitrary directions carry."""
+    n = m = 128
+    r = 8
+    say(f"case: stiff collapse BUG, n={n} r={r}")
+    dev, host = operator_pair(lri, n, m, 1, False, False, seed=14, sa=n * n, sb=m * m)
+    u = random_factors(np.random.default_rng(15), n, m, r, decay=1.0)
+    dt = 2e-4
+    N = lri.rk4_substeps(dev.spectral_bound(), dt)
+    got = run_one(lri, "bug", dev, u, dt, N)
+    assert np.all(np.isfinite(got[0])) and np.all(np.isfinite(got[1])) and np.all(np.isfinite(got[2]))
+    ref = oracle_one("bug", host, u, dt, N)
+    e = rel_err(*got, *ref)
+    say(f"  rel err vs oracle {e:.2e}")
+    assert e < 1e-4

@@ -122,3 +122,19 @@ def test_scalar_operator_closed_form(name):
     else:
         got, f = dlra_de.bug_de_step(F, u.U, u.S, u.V, 0.0, dt, N), R(z) ** N
     assert rel_err(*got, u.U, f * u.S, u.V) < 1e-13
+
+
+def test_lowrank_evaluation_matches_dense():
+    """The factored right-hand sides equal the dense F on the same arguments."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(9)
+    n, m, r = 26, 21, 4
+    A = sp.csr_matrix(dlra_de.dirichlet_laplacian(n, 5.0))
+    B = sp.csr_matrix(dlra_de.dirichlet_laplacian(m, 3.0) + np.diag(rng.uniform(-1, 0, m)))
+    G, H = rng.standard_normal((n, 2)), rng.standard_normal((m, 2))
+    op = dlra_de.LowRankLyapunov(A, B, G, H)
+    u = random_factors(rng, n, m, r)
+    a = dlra_de.bug_de_step_lowrank(op, u.U, u.S, u.V, 0.0, 0.05, 4)
+    b = dlra_de.bug_de_step(op.dense(), u.U, u.S, u.V, 0.0, 0.05, 4)
+    assert rel_err(*a, *b) < 1e-13
