@@ -19,6 +19,8 @@
 // then runs all of its RK4 stages inside one workgroup.  The S step is
 // diagonal in the product eigenbasis and its N RK4 steps are applied in closed
 // form entry by entry.
+#include <algorithm>
+
 #include "lri_common.h"
 #include "lri_kernels.h"
 
@@ -205,6 +207,26 @@ hipError_t launch_de_prep(const double* C, int ldc, int transC, const double* W,
 // the N-fold composition of this affine map, formed by repeated squaring.
 // ----------------------------------------------------------------------------
 
+// N classical RK4 steps of size h on y' = a y + b (z = h a) map y0 to
+// Ra y0 + Qb b: one step is y -> R(z) y + h Pz(z) b with the stability
+// polynomial R = 1 + z + z^2/2 + z^3/6 + z^4/24 and Pz = 1 + z/2 + z^2/6 + z^3/24,
+// and the N-fold composition is formed by repeated squaring.
+__device__ __forceinline__ void rk4_power(double z, double h, int64_t N, double& Ra, double& Qb) {
+  const double R = 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0 + z / 24.0)));
+  const double Pz = 1.0 + z * (0.5 + z * (1.0 / 6.0 + z / 24.0));
+  double Rm = R, Qm = h * Pz;  // one step (b = 1)
+  Ra = 1.0;
+  Qb = 0.0;
+  for (int64_t nn = N; nn > 0; nn >>= 1) {
+    if (nn & 1) {
+      Qb = Rm * Qb + Qm;
+      Ra = Rm * Ra;
+    }
+    Qm = Rm * Qm + Qm;
+    Rm = Rm * Rm;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_s_substep(SStepArgs P) {
   const int k = P.k, l = P.l, q = P.q, t = threadIdx.x;
   double* Sb = P.work;       // k x l
@@ -231,21 +253,9 @@ __global__ __launch_bounds__(256) void k_s_substep(SStepArgs P) {
       cb += ga * hb;
     }
     const double a = P.sgn * (P.alpha[i] + P.beta[jj]);
-    const double b = P.sgn * cb;
-    const double z = P.h * a;
-    const double R = 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0 + z / 24.0)));
-    const double Pz = 1.0 + z * (0.5 + z * (1.0 / 6.0 + z / 24.0));
-    double Rm = R, Qm = P.h * Pz * b;  // one step: y -> Rm y + Qm
-    double Ra = 1.0, Qa = 0.0;         // accumulated map
-    for (int64_t nn = P.nsub; nn > 0; nn >>= 1) {
-      if (nn & 1) {
-        Qa = Rm * Qa + Qm;
-        Ra = Rm * Ra;
-      }
-      Qm = Rm * Qm + Qm;
-      Rm = Rm * Rm;
-    }
-    Sb[e] = Ra * s + Qa;
+    double Ra, Qb;
+    rk4_power(P.h * a, P.h, P.nsub, Ra, Qb);
+    Sb[e] = Ra * s + Qb * (P.sgn * cb);
   }
   __syncthreads();
   // T = Sbar Wb'
@@ -273,20 +283,112 @@ hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st) {
 }
 
 // ----------------------------------------------------------------------------
+// Constant-coefficient operators T = d I + e (S + S'), Dirichlet or periodic,
+// have an analytic orthonormal eigenbasis (the discrete sine basis, or the real
+// Fourier basis).  In it the column problems of a K/L sub-step decouple
+// entrywise, zhat_ij' = (lambda_i + mu_j) zhat_ij + bhat_ij, and their N RK4
+// steps are applied in closed form (rk4_power): two n x n x k products replace
+// the N * 4 stencil stages of k_rk4_columns with the same result up to rounding.
+// ----------------------------------------------------------------------------
+
+// Q (n x n, column-major) and lambda: Dirichlet
+//   Q(i, k) = sqrt(2 / (n + 1)) sin(pi (i + 1)(k + 1) / (n + 1)),
+//   lambda_k = d + 2 e cos(pi (k + 1) / (n + 1));
+// periodic (n >= 3): column 0 = 1 / sqrt(n) (lambda = d + 2 e); columns 2c - 1 and
+// 2c = sqrt(2 / n) cos / sin(2 pi i c / n) (lambda = d + 2 e cos(2 pi c / n)); for
+// even n the last column is (-1)^i / sqrt(n) (lambda = d - 2 e).  Arguments are
+// reduced exactly in integers before sinpi / cospi.
+__global__ void k_eig_tridiag_const(double* __restrict__ Q, double* __restrict__ lam, int64_t n, double d, double e,
+                                    int periodic) {
+  const int64_t total = n * n;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx % n, k = idx / n;
+    double v;
+    if (!periodic) {
+      const int64_t p = ((i + 1) * (k + 1)) % (2 * (n + 1));
+      v = sqrt(2.0 / (double)(n + 1)) * sinpi((double)p / (double)(n + 1));
+      if (i == 0) lam[k] = d + 2.0 * e * cospi((double)(k + 1) / (double)(n + 1));
+    } else {
+      double lk;
+      if (k == 0) {
+        v = 1.0 / sqrt((double)n);
+        lk = d + 2.0 * e;
+      } else if ((n % 2 == 0) && k == n - 1) {
+        v = (i % 2 == 0 ? 1.0 : -1.0) / sqrt((double)n);
+        lk = d - 2.0 * e;
+      } else {
+        const int64_t c = (k + 1) / 2;
+        const int64_t p = (i * c) % n;
+        const double x = 2.0 * (double)p / (double)n;
+        v = sqrt(2.0 / (double)n) * ((k % 2 == 1) ? cospi(x) : sinpi(x));
+        lk = d + 2.0 * e * cospi(2.0 * (double)c / (double)n);
+      }
+      if (i == 0) lam[k] = lk;
+    }
+    Q[idx] = v;
+  }
+}
+
+hipError_t launch_eig_tridiag_const(double* Q, double* lam, int64_t n, double d, double e, int periodic,
+                                    hipStream_t st) {
+  if (n < 1 || (periodic && n < 3)) return hipErrorInvalidValue;
+  const int64_t total = n * n;
+  const int grid = (int)std::min<int64_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_eig_tridiag_const, dim3(grid), dim3(256), 0, st, Q, lam, n, d, e, periodic);
+  return hipGetLastError();
+}
+
+// zhat_ij <- Ra zhat_ij + Qb bhat_ij with a = lambda_i + mu_j, bhat = fbar coef
+// (fbar n x q, coef q x k): N RK4 steps of the decoupled entries
+__global__ void k_spectral_flow(double* __restrict__ Z, int64_t n, int k, const double* __restrict__ lam,
+                                const double* __restrict__ mu, const double* __restrict__ fbar,
+                                const double* __restrict__ coef, int q, double h, int64_t N) {
+  const int64_t total = n * (int64_t)k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx % n;
+    const int j = (int)(idx / n);
+    double b = 0.0;
+    for (int qq = 0; qq < q; ++qq) b += fbar[i + qq * n] * coef[qq + j * q];
+    double Ra, Qb;
+    rk4_power(h * (lam[i] + mu[j]), h, N, Ra, Qb);
+    Z[idx] = Ra * Z[idx] + Qb * b;
+  }
+}
+
+hipError_t launch_spectral_flow(double* Z, int64_t n, int k, const double* lam, const double* mu, const double* fbar,
+                                const double* coef, int q, double h, int64_t N, hipStream_t st) {
+  if (n < 1 || k < 1 || q < 0 || q > kMaxForce || N < 1) return hipErrorInvalidValue;
+  const int64_t total = n * (int64_t)k;
+  const int grid = (int)std::min<int64_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_spectral_flow, dim3(grid), dim3(256), 0, st, Z, n, k, lam, mu, fbar, coef, q, h, N);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
 // Symmetric k x k eigen-decomposition (k <= 128) as the one-sided Jacobi SVD of
 // M + sigma I with sigma = max_i sum_j |M_ij| >= spectral radius, so the
 // shifted matrix is positive semidefinite: its right singular vectors are
 // eigenvectors of M and its singular values minus sigma the eigenvalues.
 // ----------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_shift_sym(const double* __restrict__ M, int ldm, int k,
-                                                   double* __restrict__ out, double* __restrict__ shift) {
+struct ShiftArgs {
+  const double* M;
+  int ldm, k;
+  double *out, *shift, *ev;
+};
+
+// workgroup b: the shifted matrix of problem b
+__global__ __launch_bounds__(256) void k_shift_sym(ShiftArgs A0, ShiftArgs A1) {
+  const ShiftArgs A = blockIdx.x == 0 ? A0 : A1;
   __shared__ double s_max[256];
   const int t = threadIdx.x;
+  const int k = A.k;
   double mx = 0.0;
   for (int i = t; i < k; i += blockDim.x) {
     double s = 0.0;
-    for (int jj = 0; jj < k; ++jj) s += fabs(M[i + jj * ldm]);
+    for (int jj = 0; jj < k; ++jj) s += fabs(A.M[i + jj * A.ldm]);
     mx = fmax(mx, s);
   }
   s_max[t] = mx;
@@ -299,38 +401,46 @@ __global__ __launch_bounds__(256) void k_shift_sym(const double* __restrict__ M,
   for (int e = t; e < k * k; e += blockDim.x) {
     const int i = e % k, jj = e / k;
     // symmetrised: the Galerkin products are symmetric up to rounding
-    const double v = 0.5 * (M[i + jj * ldm] + M[jj + i * ldm]);
-    out[e] = v + (i == jj ? sg : 0.0);
+    const double v = 0.5 * (A.M[i + jj * A.ldm] + A.M[jj + i * A.ldm]);
+    A.out[e] = v + (i == jj ? sg : 0.0);
   }
-  if (t == 0) *shift = sg;
+  if (t == 0) *A.shift = sg;
 }
 
-__global__ void k_unshift(double* __restrict__ ev, int k, const double* __restrict__ shift) {
-  for (int i = threadIdx.x; i < k; i += blockDim.x) ev[i] -= *shift;
+__global__ void k_unshift(ShiftArgs A0, ShiftArgs A1) {
+  const ShiftArgs A = blockIdx.x == 0 ? A0 : A1;
+  for (int i = threadIdx.x; i < A.k; i += blockDim.x) A.ev[i] -= *A.shift;
 }
 
-hipError_t launch_sym_eig(const double* M, int ldm, int k, double* W, double* ev, double* work, double* Uscr,
-                          double* svd_work, int* r_scr, hipStream_t st) {
-  if (k < 1 || k > kMaxAug) return hipErrorInvalidValue;
-  double* shifted = work;
-  double* shift = work + (size_t)k * k;
-  hipLaunchKernelGGL(k_shift_sym, dim3(1), dim3(256), 0, st, M, ldm, k, shifted, shift);
+hipError_t launch_sym_eig(const SymEigArgs* P, int count, hipStream_t st) {
+  if (count < 1 || count > 2) return hipErrorInvalidValue;
+  ShiftArgs sa[2];
+  SvdArgs sv[2];
+  for (int b = 0; b < count; ++b) {
+    const SymEigArgs& p = P[b];
+    if (p.k < 1 || p.k > kMaxAug) return hipErrorInvalidValue;
+    sa[b] = ShiftArgs{p.M, p.ldm, p.k, p.work, p.work + (size_t)p.k * p.k, p.ev};
+    sv[b].S = p.work;
+    sv[b].lds = p.k;
+    sv[b].k = p.k;
+    sv[b].U = p.Uscr;
+    sv[b].V = p.W;
+    sv[b].sigma = p.ev;
+    sv[b].r_out = p.r_scr;
+    sv[b].tol = 0.0;
+    sv[b].rmax = p.k;
+    sv[b].work = p.svd_work;
+  }
+  if (count == 1) {
+    sa[1] = sa[0];
+    sv[1] = sv[0];
+  }
+  hipLaunchKernelGGL(k_shift_sym, dim3(count), dim3(256), 0, st, sa[0], sa[1]);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  SvdArgs sv;
-  sv.S = shifted;
-  sv.lds = k;
-  sv.k = k;
-  sv.U = Uscr;
-  sv.V = W;
-  sv.sigma = ev;
-  sv.r_out = r_scr;
-  sv.tol = 0.0;
-  sv.rmax = k;
-  sv.work = svd_work;
-  e = launch_jacobi_svd(sv, st);
+  e = count == 1 ? launch_jacobi_svd(sv[0], st) : launch_jacobi_svd2(sv[0], sv[1], st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_unshift, dim3(1), dim3(128), 0, st, ev, k, (const double*)shift);
+  hipLaunchKernelGGL(k_unshift, dim3(count), dim3(128), 0, st, sa[0], sa[1]);
   return hipGetLastError();
 }
 

@@ -66,10 +66,11 @@ struct DeSmall {
   double CW[2][kMaxAug * kMaxAug];       // op(C0) W of up to two concurrent sub-problems
   double coef[2][kMaxForce * kMaxAug];
   double Snew[kMaxAug * kMaxAug];
-  double ework[kMaxAug * kMaxAug + 2];
-  double Uscr[kMaxAug * kMaxAug];
+  double ework[2][kMaxAug * kMaxAug + 2];  // shifted matrix + shift, per concurrent eigenproblem
+  double Uscr[2][kMaxAug * kMaxAug];
+  double svd2[kMaxAug * kMaxAug];          // global Jacobi work of the second eigenproblem (k > 64)
   double swork[2 * kMaxAug * kMaxAug];
-  int rscr;
+  int rscr[2];
 };
 
 }  // namespace
@@ -120,6 +121,12 @@ struct lri_ctx {
   int op_aper = 0, op_bper = 0, op_q = 0;
   DeSmall* de = nullptr;
   bool pack_enabled = true;
+  // constant-coefficient operators: analytic eigenbasis Q (rows x rows), its
+  // eigenvalues, and Q' G / Q' H, for the closed-form RK4 sub-steps (one per side;
+  // side 1 may alias side 0)
+  bool spec[2] = {false, false};
+  double *specQ[2] = {nullptr, nullptr}, *specLam[2] = {nullptr, nullptr}, *specF[2] = {nullptr, nullptr};
+  bool spec_alias = false;
 
   // ------------------------------------------------------------------ helpers
   int pass_mode() const {
@@ -751,9 +758,27 @@ struct lri_ctx {
       rows(B);
     }
   }
+  SymEigArgs eig_args(int slot, const double* Cg, int k, double* W, double* ev) {
+    SymEigArgs a;
+    a.M = Cg;
+    a.ldm = k;
+    a.k = k;
+    a.W = W;
+    a.ev = ev;
+    a.work = de->ework[slot];
+    a.Uscr = de->Uscr[slot];
+    a.svd_work = slot == 0 ? svd_work : de->svd2;
+    a.r_scr = &de->rscr[slot];
+    return a;
+  }
   void de_eig(const double* Cg, int k, double* W, double* ev) {
-    timed(LRI_KERNEL_SMALL,
-          [&] { return launch_sym_eig(Cg, k, k, W, ev, de->ework, de->Uscr, svd_work, &de->rscr, stream); });
+    const SymEigArgs a = eig_args(0, Cg, k, W, ev);
+    timed(LRI_KERNEL_SMALL, [&] { return launch_sym_eig(&a, 1, stream); });
+  }
+  // the two Galerkin eigenproblems of a step, one workgroup each in one launch
+  void de_eig2(int ka, int kb) {
+    const SymEigArgs a[2] = {eig_args(0, de->Ga, ka, de->Wa, de->alpha), eig_args(1, de->Gb, kb, de->Wb, de->beta)};
+    timed(LRI_KERNEL_SMALL, [&] { return launch_sym_eig(a, 2, stream); });
   }
   // One K-type sub-problem on side `nside` (rows n, operator A, forcing G; or
   // rows m, operator B, forcing H):  Z' = T Z + Z Co + F FXo,  Z0 = X0 op(C0),
@@ -782,6 +807,7 @@ struct lri_ctx {
     A.ldo = rows_;
     rows(A);
     RK4ColArgs P;
+    P.side = nside ? 0 : 1;
     P.n = rows_;
     P.ncols = k;
     P.X = scratch;
@@ -800,9 +826,73 @@ struct lri_ctx {
     P.nsub = nsub;
     return P;
   }
-  void de_run(const RK4ColArgs& P0, const RK4ColArgs& P1 = RK4ColArgs()) {
-    timed(LRI_KERNEL_DE, [&] { return launch_rk4_columns(P0, P1, stream); });
+  // out (rn x k) = op(Qm) Bsrc for an rn x rn FP64 matrix Qm (op: transpose when
+  // trans), on the FP64 streaming passes with a single operand: pack Bsrc into
+  // the panel with rn rows, one pass, then the sum of the split partials.
+  void spectral_gemm(bool trans, const double* Qm, int64_t rn, const double* Bsrc, int k, double* out) {
+    const int rb = rb_for(PASS_F64, k);
+    double* pk = (rn == n) ? pkT_f64 : pkN_f64;
+    RowArgs<double> A;
+    A.rows = rn;
+    A.r_out = k;
+    A.base = Bsrc;
+    A.ld_base = rn;
+    A.pk_f64 = pk;
+    A.pk_cols = rb;
+    rows(A);
+    if (rn == n) pkT_mode = -1;  // the panel no longer holds a state basis
+    else pkN_mode = -1;
+    PassArgs a;
+    a.anext = Qm;
+    a.aprev = nullptr;
+    a.dtype = LRI_F64;
+    a.lda = rn;
+    a.n = rn;
+    a.m = rn;
+    a.rb = rb;
+    a.B = pk;
+    a.part = partT;
+    const int64_t need = plan_pass(PASS_F64, trans, rn, rn, rb, &a.splits, &a.chunk);
+    REQUIRE((size_t)need * sizeof(double) <= part_bytes, "internal: partial buffer too small");
+    timed(LRI_KERNEL_DE, [&] { return trans ? launch_passT(PASS_F64, a, stream) : launch_passN(PASS_F64, a, stream); });
+    RowArgs<double> S;
+    S.rows = rn;
+    S.nsplit = a.splits;
+    S.rb = rb;
+    S.r_out = k;
+    S.parts = (const double*)partT;
+    S.out = out;
+    S.ldo = rn;
+    rows(S);
   }
+
+  // Integrate one or two prepared sub-problems: sides with a constant operator
+  // in the analytic eigenbasis (Zhat = Q' Z, closed-form flow, Z = Q Zhat, with
+  // Zhat in the P / Q work buffers), the others in one k_rk4_columns launch.
+  void de_run(const RK4ColArgs& P0, const RK4ColArgs& P1 = RK4ColArgs()) {
+    RK4ColArgs rk[2];
+    int nrk = 0;
+    for (const RK4ColArgs* Pp : {&P0, &P1}) {
+      const RK4ColArgs& Pa = *Pp;
+      if (Pa.ncols < 1) continue;
+      if (!spec[Pa.side]) {
+        rk[nrk++] = Pa;
+        continue;
+      }
+      double* Zh = Pa.side == 0 ? this->P : this->Q;
+      spectral_gemm(true, specQ[Pa.side], Pa.n, Pa.X, Pa.ncols, Zh);
+      timed(LRI_KERNEL_DE, [&] {
+        return launch_spectral_flow(Zh, Pa.n, Pa.ncols, specLam[Pa.side], Pa.mu, specF[Pa.side], Pa.coef, Pa.q,
+                                    Pa.h, Pa.nsub, stream);
+      });
+      spectral_gemm(false, specQ[Pa.side], Pa.n, Zh, Pa.ncols, Pa.Y);
+    }
+    if (nrk > 0) {
+      const RK4ColArgs none;
+      timed(LRI_KERNEL_DE, [&] { return launch_rk4_columns(rk[0], nrk > 1 ? rk[1] : none, stream); });
+    }
+  }
+
   void de_finish(bool nside, int k, const double* Wo, double* out, const double* scratch, EpiArgs epi) {
     const int64_t rows_ = nside ? n : m;
     RowArgs<double> B;  // Z = Zbar Wo'
@@ -909,9 +999,8 @@ struct lri_ctx {
     const int rr = (int)r;
     const double h = dt / (double)nsub;
     de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
-    de_eig(de->Gb, rr, de->Wb, de->beta);
-    de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U0'AU0
-    de_eig(de->Ga, rr, de->Wa, de->alpha);
+    de_galerkin(U, true, rr, de->Ga, de->gaT);   // At = U0'AU0
+    de_eig2(rr, rr);
     // the K and L sub-problems are independent: one RK4 launch for both
     const RK4ColArgs Pk = de_prepare(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, 0, h, nsub, K2);
     const RK4ColArgs Pl = de_prepare(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, 1, h, nsub, L2);
@@ -924,10 +1013,9 @@ struct lri_ctx {
     eN.S = S;
     eN.ld_s = (int)rcap;
     cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0, 3);
-    de_galerkin(U2, true, rr, de->Ga, de->gaT);  // At = U1'AU1
-    de_eig(de->Ga, rr, de->Wa, de->alpha);
+    de_galerkin(U2, true, rr, de->Ga, de->gaT);   // At = U1'AU1
     de_galerkin(V2, false, rr, de->Gb, de->hbT);  // Bt = V1'BV1
-    de_eig(de->Gb, rr, de->Wb, de->beta);
+    de_eig2(rr, rr);
     de_sstep(rr, rr, small->Sinit, rr, 0, S, (int)rcap, 1.0, h, nsub);
     std::swap(U, U2);
     std::swap(V, V2);
@@ -941,9 +1029,8 @@ struct lri_ctx {
     const double h = dt / (double)nsub;
     EpiArgs none;
     de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
-    de_eig(de->Gb, rr, de->Wb, de->beta);
-    de_galerkin(U, true, rr, de->Ga, de->gaT);  // At = U0'AU0
-    de_eig(de->Ga, rr, de->Wa, de->alpha);
+    de_galerkin(U, true, rr, de->Ga, de->gaT);   // At = U0'AU0
+    de_eig2(rr, rr);
     const RK4ColArgs Pk = de_prepare(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, 0, h, nsub, K2);
     const RK4ColArgs Pl = de_prepare(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, 1, h, nsub, L2);
     de_run(Pk, Pl);
@@ -960,11 +1047,60 @@ struct lri_ctx {
     eN.ld_s = (int)rcap;
     complete_basis(L, V, m, rr, L2, L2 + m * rcap, Vhat + m * rr, eN);
     de_galerkin(Uhat, true, k2, de->Ga, de->gaT);
-    de_eig(de->Ga, k2, de->Wa, de->alpha);
     de_galerkin(Vhat, false, k2, de->Gb, de->hbT);
-    de_eig(de->Gb, k2, de->Wb, de->beta);
+    de_eig2(k2, k2);
     de_sstep(k2, k2, small->Sinit, k2, 0, small->Shat, k2, 1.0, h, nsub);
     return ra_finish(k2, tol, rmax);
+  }
+
+  // Constant-coefficient sides get their analytic eigenbasis (used by de_run);
+  // G and H are projected once: specF = Q' G (or Q' H).
+  void setup_spectral(const double* ad, const double* ae, bool aper, const double* bd, const double* be, bool bper) {
+    auto constant = [](const double* d, const double* e, int64_t len, bool per, double* d0, double* e0) {
+      *d0 = d[0];
+      *e0 = (len > 1 || per) ? e[0] : 0.0;
+      for (int64_t i = 1; i < len; ++i)
+        if (d[i] != *d0) return false;
+      const int64_t ne = per ? len : len - 1;
+      for (int64_t i = 1; i < ne; ++i)
+        if (e[i] != *e0) return false;
+      return true;
+    };
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (int side = 0; side < 2; ++side) {
+      if (side == 1 && spec_alias) specQ[1] = specLam[1] = nullptr;  // owned by side 0
+      for (double** p : {&specQ[side], &specLam[side], &specF[side]})
+        if (*p) {
+          HIPCHECK(hipFree(*p));
+          *p = nullptr;
+        }
+      spec[side] = false;
+    }
+    spec_alias = false;
+    double da, ea, db, eb;
+    const bool ca = constant(ad, ae, n, aper, &da, &ea);
+    const bool cb = constant(bd, be, m, bper, &db, &eb);
+    for (int side = 0; side < 2; ++side) {
+      const bool c = side == 0 ? ca : cb;
+      if (!c) continue;
+      const int64_t rn = side == 0 ? n : m;
+      if (side == 1 && ca && m == n && aper == bper && da == db && ea == eb) {
+        spec_alias = true;
+        specQ[1] = specQ[0];
+        specLam[1] = specLam[0];
+        HIPCHECK(hipMalloc((void**)&specF[1], sizeof(double) * rn * LRI_MAX_FORCE));
+      } else {
+        HIPCHECK(hipMalloc((void**)&specQ[side], sizeof(double) * rn * rn));
+        HIPCHECK(hipMalloc((void**)&specLam[side], sizeof(double) * rn));
+        HIPCHECK(hipMalloc((void**)&specF[side], sizeof(double) * rn * LRI_MAX_FORCE));
+        HIPCHECK(launch_eig_tridiag_const(specQ[side], specLam[side], rn, side == 0 ? da : db, side == 0 ? ea : eb,
+                                          side == 0 ? aper : bper, stream));
+      }
+      HIPCHECK(hipMemsetAsync(specF[side], 0, sizeof(double) * rn * LRI_MAX_FORCE, stream));
+      if (op_q > 0) spectral_gemm(true, specQ[side], rn, side == 0 ? op_G : op_H, op_q, specF[side]);
+      spec[side] = true;
+    }
+    HIPCHECK(hipStreamSynchronize(stream));
   }
 
   void check_de(double dt, int64_t nsub) {
@@ -1019,9 +1155,13 @@ void free_ctx(lri_ctx* c) {
                   c->L,      c->L2,     c->Q,      c->Uhat,    c->Vhat,    c->pkN_f64, c->pkT_f64, c->pkN_f32,
                   c->pkT_f32, c->pkN_hi, c->pkN_lo, c->pkT_hi,  c->pkT_lo,  c->partN,  c->partT,  c->gpart,
                   c->flags,  c->small,  c->svd_work, c->op_ad,  c->op_ae,   c->op_bd,   c->op_be,  c->op_G,
-                  c->op_H,   c->de};
+                  c->op_H,   c->de,     c->specQ[0], c->specLam[0], c->specF[0]};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  if (c->specF[1]) (void)hipFree(c->specF[1]);
+  if (!c->spec_alias)
+    for (void* p : {(void*)c->specQ[1], (void*)c->specLam[1]})
+      if (p) (void)hipFree(p);
   for (auto& t : c->pending) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -1115,6 +1255,9 @@ int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
         size_t es = mode == PASS_F64 ? 8 : 4;
         pb = std::max(pb, es * (size_t)plan_pass(mode, false, n, m, rb, &s, &ch));
         pb = std::max(pb, es * (size_t)plan_pass(mode, true, m, n, rb, &s, &ch));
+        if (mode == PASS_F64 && rb <= 64)  // the matrix-ODE eigenbasis transforms (n x n, m x m)
+          for (int64_t d : {n, m})
+            for (bool tr : {false, true}) pb = std::max(pb, es * (size_t)plan_pass(mode, tr, d, d, rb, &s, &ch));
       }
     c->part_bytes = pb;
     alloc(&c->partN, pb);
@@ -1350,6 +1493,7 @@ int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* 
     ctx->op_aper = a_periodic != 0;
     ctx->op_bper = b_periodic != 0;
     ctx->op_q = (int)q;
+    ctx->setup_spectral(a_diag, a_off, a_periodic != 0, b_diag, b_off, b_periodic != 0);
     ctx->op_set = true;
     return LRI_OK;
   });

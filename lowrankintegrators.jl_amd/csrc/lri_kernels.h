@@ -120,13 +120,15 @@ struct SvdArgs {
   double* work = nullptr;  // global ke x ke scratch, used when it does not fit in LDS
 };
 hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st);
+// two independent decompositions in one launch (one workgroup each)
+hipError_t launch_jacobi_svd2(SvdArgs P0, SvdArgs P1, hipStream_t st);
 
 // Product passes (see lri_tsmm.hip).
 enum PassMode { PASS_F64 = 0, PASS_BF16X3 = 1, PASS_F32 = 2 };
 
 struct PassArgs {
   const void* anext = nullptr;  // A[k_next], column-major, ld lda
-  const void* aprev = nullptr;  // A[k_prev]
+  const void* aprev = nullptr;  // A[k_prev]; FP64 passes: nullptr streams A[k_next] itself
   int dtype = 0;                // LRI_F64 / LRI_F32 / LRI_BF16
   int64_t lda = 0;
   int64_t n = 0, m = 0;
@@ -154,6 +156,7 @@ constexpr int kMaxForce = 4;
 
 // y_j' = (T + mu_j I) y_j + sum_k g_k coef[k + j q], T symmetric tridiagonal
 struct RK4ColArgs {
+  int side = 0;               // host bookkeeping: 0 = the n-side operator A, 1 = the m-side operator B
   int64_t n = 0;
   int ncols = 0;
   const double* X = nullptr;  // initial columns, n x ncols (ld ldx)
@@ -200,10 +203,24 @@ struct SStepArgs {
 };
 hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st);
 
+// Analytic eigenbasis of a constant symmetric tridiagonal d I + e (S + S')
+// (Dirichlet, or periodic with n >= 3): Q n x n column-major, lam n.
+hipError_t launch_eig_tridiag_const(double* Q, double* lam, int64_t n, double d, double e, int periodic,
+                                    hipStream_t st);
+// Z (n x k, ld n) <- N RK4 steps of z_ij' = (lam_i + mu_j) z_ij + (fbar coef)_ij, in closed form.
+hipError_t launch_spectral_flow(double* Z, int64_t n, int k, const double* lam, const double* mu, const double* fbar,
+                                const double* coef, int q, double h, int64_t N, hipStream_t st);
+
 // Symmetric eigen-decomposition M = W diag(ev) W' of a k x k matrix (k <= kMaxAug).
 // work: k*k + 2 doubles; svd_work: kMaxAug^2 doubles; Uscr: k*k doubles; r_scr: one int.
-hipError_t launch_sym_eig(const double* M, int ldm, int k, double* W, double* ev, double* work, double* Uscr,
-                          double* svd_work, int* r_scr, hipStream_t st);
+struct SymEigArgs {
+  const double* M = nullptr;
+  int ldm = 0, k = 0;
+  double *W = nullptr, *ev = nullptr, *work = nullptr, *Uscr = nullptr, *svd_work = nullptr;
+  int* r_scr = nullptr;
+};
+// one or two (count = 1, 2) independent decompositions
+hipError_t launch_sym_eig(const SymEigArgs* P, int count, hipStream_t st);
 
 // Element (row i, column c) of a BF16 hi/lo panel with rb columns, in the
 // order the passes load their B fragments: pass N reads 8 consecutive rows

@@ -827,8 +827,7 @@ template hipError_t launch_rows<float>(RowArgs<float>, hipStream_t);
 // Pairs are processed in round-robin (circle-method) order, npairs at a time.
 // ----------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kThreads) void k_jacobi_svd(SvdArgs P) {
-  extern __shared__ double lds[];
+__device__ void jacobi_svd_body(const SvdArgs& P, double* lds) {
   const int k = P.k;
   const int ke = (k + 1) & ~1;
   double* A = lds;                   // k rows x ke columns
@@ -934,17 +933,40 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_svd(SvdArgs P) {
   }
 }
 
-hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st) {
+// workgroup 0 decomposes P0, workgroup 1 (if launched) P1
+__global__ __launch_bounds__(kThreads) void k_jacobi_svd(SvdArgs P0, SvdArgs P1) {
+  extern __shared__ double lds[];
+  jacobi_svd_body(blockIdx.x == 0 ? P0 : P1, lds);
+}
+
+// LDS doubles of one decomposition; clears P.work when the right factor fits in LDS
+static hipError_t jacobi_plan(SvdArgs& P, size_t* lds) {
   if (P.k < 1 || P.k > kMaxAug) return hipErrorInvalidValue;
   const int ke = (P.k + 1) & ~1;
-  size_t lds = sizeof(double) * ((size_t)ke * P.k + ke);
-  if (lds + sizeof(double) * (size_t)ke * ke <= 150 * 1024) {
-    lds += sizeof(double) * (size_t)ke * ke;
+  *lds = sizeof(double) * ((size_t)ke * P.k + ke);
+  if (*lds + sizeof(double) * (size_t)ke * ke <= 150 * 1024) {
+    *lds += sizeof(double) * (size_t)ke * ke;
     P.work = nullptr;
   } else if (!P.work) {
     return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(k_jacobi_svd, dim3(1), dim3(kThreads), lds, st, P);
+  return hipSuccess;
+}
+
+hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st) {
+  size_t lds;
+  hipError_t e = jacobi_plan(P, &lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_jacobi_svd, dim3(1), dim3(kThreads), lds, st, P, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_svd2(SvdArgs P0, SvdArgs P1, hipStream_t st) {
+  size_t l0, l1;
+  hipError_t e = jacobi_plan(P0, &l0);
+  if (e == hipSuccess) e = jacobi_plan(P1, &l1);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_jacobi_svd, dim3(2), dim3(kThreads), l0 > l1 ? l0 : l1, st, P0, P1);
   return hipGetLastError();
 }
 

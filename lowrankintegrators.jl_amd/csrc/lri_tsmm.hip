@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_f64(const T* __restrict_
       const int64_t jj = j + 4 * u + lk;
       const bool jok = jj < je;
       load_rows_f64<TM>(an, jj * lda + r0, r0, n, jok, v, A1[u]);
-      load_rows_f64<TM>(ap, jj * lda + r0, r0, n, jok, v, A0[u]);
+      load_rows_f64<TM>(ap, jj * lda + r0, r0, n, jok && ap != nullptr, v, A0[u]);  // no ap: A itself
 #pragma unroll
       for (int c = 0; c < CB; ++c) Bv[u][c] = jok ? B[jj * rb + c * 16 + lr] : 0.0;
     }
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict_
     for (int t = 0; t < TJ; ++t) {
       const int64_t jj = j0 + 16 * t + lr;
       load_rows_f64<4>(an, jj * lda + r0, r0, ie, jj < m, v, A1[t]);
-      load_rows_f64<4>(ap, jj * lda + r0, r0, ie, jj < m, v, A0[t]);
+      load_rows_f64<4>(ap, jj * lda + r0, r0, ie, jj < m && ap != nullptr, v, A0[t]);  // no ap: A itself
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {

@@ -292,3 +292,23 @@ def test_stiff_collapse_is_not_an_error(lri):
     e = rel_err(*got, *ref)
     say(f"  rel err vs oracle {e:.2e}")
     assert e < 1e-4
+
+
+@pytest.mark.parametrize("name", ["bug", "primal"])
+def test_mixed_spectral_and_stencil_sides(lri, name):
+    """A constant (closed-form sub-steps in its analytic eigenbasis), B variable
+    (RK4 stencil kernel): both integrators in one step."""
+    n, m, r = 44, 39, 5
+    say(f"case: mixed sides {name}, n={n} m={m} r={r}")
+    rng = np.random.default_rng(16)
+    A = lri.laplacian_1d(n, 70.0, periodic=True)
+    B = tri(lri, m, 30.0, False, rng)
+    G, H = rng.standard_normal((n, 1)), rng.standard_normal((m, 1))
+    dev = lri.LyapunovOperator(A, B, G, H)
+    host = dlra_de.LyapunovOperator(A.dense(), B.dense(), G, H)
+    u = random_factors(np.random.default_rng(17), n, m, r)
+    dt = 0.03
+    N = lri.rk4_substeps(dev.spectral_bound(), dt)
+    got = run_one(lri, name, dev, u, dt, N)
+    ref = oracle_one(name, host, u, dt, N)
+    assert_close(got, ref)
