@@ -244,7 +244,7 @@ def run_de(args, torch, lri):
     for _ in range(args.warmup_de):
         ctx.step_de_bug(dt, N)
     ctx.set_state(U0, S0, V0)
-    ctx.timing(1)
+    ctx.timing(0 if args.no_events else 2)  # HIP events around the eigenbasis transforms only
     ctx.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -255,13 +255,19 @@ def run_de(args, torch, lri):
     el = time.perf_counter() - t0
     value = args.de_steps / el
     de_ms, de_cnt = ctx.timing_read(lri.native.KERNEL_DE)
-    sm_ms, sm_cnt = ctx.timing_read(lri.native.KERNEL_SMALL)
     ctx.timing(0)
     ctx.close()
-    avg_ms = de_ms / max(de_cnt, 1)
-    flops = 37.0 * N * r * (n + m)  # RK4 on the K and L columns of one launch (37 flops per row per sub-step)
-    achieved = flops / (avg_ms * 1e-3) / 1e12
-    log(f"DE BUG: {value:.2f} steps/s ({1e3 * el / args.de_steps:.2f} ms/step); RK4 launch {avg_ms:.3f} ms")
+    avg_ms = de_ms / de_cnt if de_cnt else None
+    # Constant coefficients: the K and L sub-steps run in the analytic eigenbasis of
+    # L (both sides share it), as one forward (Q' Z) and one backward (Q Z)
+    # transform over the 2r columns of both sub-problems, each an FP64-MFMA pass
+    # that streams the n x n basis once.
+    kcols = 2 * r
+    flops = 2.0 * n * n * kcols
+    bytes_alg = 8.0 * (n * n + 2 * n * kcols)
+    achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms else None
+    log(f"DE BUG: {value:.2f} steps/s ({1e3 * el / args.de_steps:.2f} ms/step); "
+        f"transform launch {avg_ms if avg_ms is None else round(avg_ms, 4)} ms")
     out = {
         "metric": "steps/s of the BUG integrator on dX/dt = L X + X L' + a b' at n=m=4096, r=32 (FP64, RK4 sub-steps)",
         "value": value,
@@ -279,12 +285,13 @@ def run_de(args, torch, lri):
                                f"L = n^2 tridiag(1,-2,1) Dirichlet, G = a b' (seeded N(0,1)), Y0 = U diag(10^(-i/4)) V', "
                                f"{N} fixed RK4 sub-steps per sub-problem", "n": n, "m": m, "r": r, "dt": dt,
                    "substeps": N, "parallelism": "single GPU"},
-        "roofline": {"bound": "fp64", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": achieved / 78.6, "traffic": None, "kernel": "k_rk4_columns",
-                     "flops_per_launch": flops,
-                     "note": "one workgroup per column: 2r = 64 of 256 CUs busy"},
-        "kernels": {"rk4": {"total_ms": de_ms, "launches": de_cnt, "avg_ms": avg_ms},
-                    "small": {"total_ms": sm_ms, "launches": sm_cnt, "avg_ms": sm_ms / max(sm_cnt, 1)}},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": achieved / 78.6 if achieved else None, "traffic": None,
+                     "kernel": "k_passT_f64 / k_passN_f64 (eigenbasis transforms of the K and L columns)",
+                     "flops_per_launch": flops, "bytes_per_launch": bytes_alg,
+                     "note": f"2 n^2 (2r) flops and 8 (n^2 + 4 n r) bytes per launch: {flops / bytes_alg:.1f} "
+                             "flop/B, above the FP64 machine balance (~10 flop/B), so FP64 MFMA bounds it"},
+        "kernels": {"transforms": {"total_ms": de_ms, "launches": de_cnt, "avg_ms": avg_ms}},
     }
     if not args.no_cpu:
         log("CPU baseline (oracle, factored right-hand sides) on one step")

@@ -191,13 +191,14 @@ int lri_get_stream(lri_ctx* ctx, void** stream);
 #define LRI_KERNEL_PASS_N 0 /* dA * B, reduction over the m columns */
 #define LRI_KERNEL_PASS_T 1 /* dA' * B, reduction over the n rows */
 #define LRI_KERNEL_SMALL 2  /* all FP64 n x r / r x r kernels of a step */
-#define LRI_KERNEL_DE 3     /* RK4 sub-step integration of the matrix-ODE columns */
+#define LRI_KERNEL_DE 3     /* matrix-ODE sub-steps: eigenbasis transforms of the K/L columns
+                               (FP64 passes over the n x n basis), or RK4 column integration */
 #define LRI_KERNEL_COUNT 4
 
 /* HIP-event timing on the context's stream: mode 0 disables it, mode 1 times
- * every launch of the families above, mode 2 only the product passes
- * (LRI_KERNEL_PASS_N/T, two events per pass, the least intrusive).  Enabling
- * resets the totals. */
+ * every launch of the families above, mode 2 only the streaming kernels
+ * (LRI_KERNEL_PASS_N/T and LRI_KERNEL_DE, two events per launch, the least
+ * intrusive).  Enabling resets the totals. */
 int lri_kernel_timing(lri_ctx* ctx, int mode);
 
 /* Synchronise and return the summed duration (ms) and launch count of a

@@ -172,17 +172,45 @@ hipError_t launch_rk4_columns(const RK4ColArgs& P0, const RK4ColArgs& P1, hipStr
 // CW = op(C) W (k x k) and coef = FX W (q x k), one workgroup
 // ----------------------------------------------------------------------------
 
+// CW = op(C) W (k x k) and coef = FX W (q x k); kLds stages op(C) and W in LDS
+// (k <= 64), where the products read them at an odd stride.
+template <bool kLds>
 __global__ __launch_bounds__(256) void k_de_prep(const double* __restrict__ C, int ldc, int transC,
                                                  const double* __restrict__ W, const double* __restrict__ FX, int k,
                                                  int q, double* __restrict__ CW, double* __restrict__ coef) {
-  const int t = threadIdx.x;
-  for (int e = t; e < k * k; e += blockDim.x) {
+  extern __shared__ double sm[];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int ld = k | 1;
+  if (kLds) {
+    double* Cs = sm;           // op(C), k x k, ld
+    double* Ws = sm + ld * k;  // W, k x k, ld
+    for (int e = t; e < k * k; e += nt) {
+      const int i = e % k, c = e / k;
+      Cs[i + c * ld] = transC ? C[c + i * ldc] : C[i + c * ldc];
+      Ws[i + c * ld] = W[e];
+    }
+    __syncthreads();
+    for (int e = t; e < k * k; e += nt) {
+      const int i = e % k, jj = e / k;
+      double s = 0.0;
+      for (int c = 0; c < k; ++c) s += Cs[i + c * ld] * Ws[c + jj * ld];
+      CW[e] = s;
+    }
+    for (int e = t; e < q * k; e += nt) {
+      const int qq = e % q, jj = e / q;
+      double s = 0.0;
+      for (int c = 0; c < k; ++c) s += FX[qq + c * q] * Ws[c + jj * ld];
+      coef[e] = s;
+    }
+    return;
+  }
+  for (int e = t; e < k * k; e += nt) {
     const int i = e % k, jj = e / k;
     double s = 0.0;
     for (int c = 0; c < k; ++c) s += (transC ? C[c + i * ldc] : C[i + c * ldc]) * W[c + jj * k];
     CW[e] = s;
   }
-  for (int e = t; e < q * k; e += blockDim.x) {
+  for (int e = t; e < q * k; e += nt) {
     const int qq = e % q, jj = e / q;
     double s = 0.0;
     for (int c = 0; c < k; ++c) s += FX[qq + c * q] * W[c + jj * k];
@@ -193,7 +221,11 @@ __global__ __launch_bounds__(256) void k_de_prep(const double* __restrict__ C, i
 hipError_t launch_de_prep(const double* C, int ldc, int transC, const double* W, const double* FX, int k, int q,
                           double* CW, double* coef, hipStream_t st) {
   if (k < 1 || k > kMaxAug || q < 0 || q > kMaxForce) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_de_prep, dim3(1), dim3(256), 0, st, C, ldc, transC, W, FX, k, q, CW, coef);
+  if (k <= 64)
+    hipLaunchKernelGGL(k_de_prep<true>, dim3(1), dim3(256), sizeof(double) * 2 * (size_t)(k | 1) * k, st, C, ldc,
+                       transC, W, FX, k, q, CW, coef);
+  else
+    hipLaunchKernelGGL(k_de_prep<false>, dim3(1), dim3(256), 0, st, C, ldc, transC, W, FX, k, q, CW, coef);
   return hipGetLastError();
 }
 
@@ -227,31 +259,63 @@ __device__ __forceinline__ void rk4_power(double z, double h, int64_t N, double&
   }
 }
 
+// S <- N RK4 steps of S' = sgn (Ga S + S Gb' + ga hb') in the product eigenbasis
+// (Ga = Wa diag(alpha) Wa', Gb = Wb diag(beta) Wb'): Sbar = Wa' op(S0) Wb, each
+// entry advanced in closed form, S1 = Wa Sbar Wb'.  kLds: the bases and the
+// work matrices sit in LDS (k, l <= 64), else in P.work.
+template <bool kLds>
 __global__ __launch_bounds__(256) void k_s_substep(SStepArgs P) {
-  const int k = P.k, l = P.l, q = P.q, t = threadIdx.x;
-  double* Sb = P.work;       // k x l
-  double* T = Sb + k * l;    // k x l
-  // T = S0 Wb
-  for (int e = t; e < k * l; e += blockDim.x) {
+  extern __shared__ double sm[];
+  const int k = P.k, l = P.l, q = P.q, t = threadIdx.x, nt = blockDim.x;
+  const double* Wa = P.Wa;
+  const double* Wb = P.Wb;
+  const int lwa = kLds ? (k | 1) : k;  // odd stride in LDS: Wa is also read along rows
+  double* Sb = P.work;                 // k x l
+  if (kLds) {
+    double* wa = sm;
+    double* wb = wa + lwa * k;
+    for (int e = t; e < k * k; e += nt) wa[e % k + (e / k) * lwa] = P.Wa[e];
+    for (int e = t; e < l * l; e += nt) wb[e] = P.Wb[e];
+    Wa = wa;
+    Wb = wb;
+    Sb = wb + l * l;
+  }
+  double* T = Sb + k * l;   // k x l
+  double* gh = T + k * l;   // k x q: Wa' (G' U)'
+  double* hh = gh + k * q;  // l x q: Wb' (H' V)'
+  for (int e = t; e < k * l; e += nt) {
+    const int i = e % k, jj = e / k;
+    Sb[e] = P.transS0 ? P.S0[jj + i * P.lds0] : P.S0[i + jj * P.lds0];
+  }
+  for (int e = t; e < (k + l) * q; e += nt) {
+    if (e < k * q) {
+      const int i = e % k, qq = e / k;
+      double s = 0.0;
+      for (int c = 0; c < k; ++c) s += P.Wa[c + i * k] * P.gaT[qq + c * q];
+      gh[e] = s;
+    } else {
+      const int f = e - k * q, jj = f % l, qq = f / l;
+      double s = 0.0;
+      for (int c = 0; c < l; ++c) s += P.Wb[c + jj * l] * P.hbT[qq + c * q];
+      hh[f] = s;
+    }
+  }
+  __syncthreads();
+  // T = op(S0) Wb
+  for (int e = t; e < k * l; e += nt) {
     const int i = e % k, jj = e / k;
     double s = 0.0;
-    for (int c = 0; c < l; ++c)
-      s += (P.transS0 ? P.S0[c + i * P.lds0] : P.S0[i + c * P.lds0]) * P.Wb[c + jj * l];
+    for (int c = 0; c < l; ++c) s += Sb[i + c * k] * Wb[c + jj * l];
     T[e] = s;
   }
   __syncthreads();
   // Sbar = Wa' T, then the RK4 flow per entry
-  for (int e = t; e < k * l; e += blockDim.x) {
+  for (int e = t; e < k * l; e += nt) {
     const int i = e % k, jj = e / k;
     double s = 0.0;
-    for (int c = 0; c < k; ++c) s += P.Wa[c + i * k] * T[c + jj * k];
+    for (int c = 0; c < k; ++c) s += Wa[c + i * lwa] * T[c + jj * k];
     double cb = 0.0;
-    for (int qq = 0; qq < q; ++qq) {
-      double ga = 0.0, hb = 0.0;
-      for (int c = 0; c < k; ++c) ga += P.Wa[c + i * k] * P.gaT[qq + c * q];
-      for (int c = 0; c < l; ++c) hb += P.Wb[c + jj * l] * P.hbT[qq + c * q];
-      cb += ga * hb;
-    }
+    for (int qq = 0; qq < q; ++qq) cb += gh[i + qq * k] * hh[jj + qq * l];
     const double a = P.sgn * (P.alpha[i] + P.beta[jj]);
     double Ra, Qb;
     rk4_power(P.h * a, P.h, P.nsub, Ra, Qb);
@@ -259,18 +323,18 @@ __global__ __launch_bounds__(256) void k_s_substep(SStepArgs P) {
   }
   __syncthreads();
   // T = Sbar Wb'
-  for (int e = t; e < k * l; e += blockDim.x) {
+  for (int e = t; e < k * l; e += nt) {
     const int i = e % k, jj = e / k;
     double s = 0.0;
-    for (int c = 0; c < l; ++c) s += Sb[i + c * k] * P.Wb[jj + c * l];
+    for (int c = 0; c < l; ++c) s += Sb[i + c * k] * Wb[jj + c * l];
     T[e] = s;
   }
   __syncthreads();
   // S1 = Wa T
-  for (int e = t; e < k * l; e += blockDim.x) {
+  for (int e = t; e < k * l; e += nt) {
     const int i = e % k, jj = e / k;
     double s = 0.0;
-    for (int c = 0; c < k; ++c) s += P.Wa[i + c * k] * T[c + jj * k];
+    for (int c = 0; c < k; ++c) s += Wa[i + c * lwa] * T[c + jj * k];
     P.S1[i + jj * P.lds1] = s;
   }
 }
@@ -278,7 +342,13 @@ __global__ __launch_bounds__(256) void k_s_substep(SStepArgs P) {
 hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st) {
   if (P.k < 1 || P.l < 1 || P.k > kMaxAug || P.l > kMaxAug || P.q < 0 || P.q > kMaxForce || !P.work)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_s_substep, dim3(1), dim3(256), 0, st, P);
+  if (P.k <= 64 && P.l <= 64) {
+    const size_t lds = sizeof(double) * ((size_t)(P.k | 1) * P.k + (size_t)P.l * P.l + 2 * (size_t)P.k * P.l +
+                                         (size_t)(P.k + P.l) * P.q);
+    hipLaunchKernelGGL(k_s_substep<true>, dim3(1), dim3(256), lds, st, P);
+  } else {
+    hipLaunchKernelGGL(k_s_substep<false>, dim3(1), dim3(256), 0, st, P);
+  }
   return hipGetLastError();
 }
 
@@ -385,12 +455,29 @@ __global__ __launch_bounds__(256) void k_shift_sym(ShiftArgs A0, ShiftArgs A1) {
   __shared__ double s_max[256];
   const int t = threadIdx.x;
   const int k = A.k;
-  double mx = 0.0;
-  for (int i = t; i < k; i += blockDim.x) {
-    double s = 0.0;
-    for (int jj = 0; jj < k; ++jj) s += fabs(A.M[i + jj * A.ldm]);
-    mx = fmax(mx, s);
+  // symmetrised copy (the Galerkin products are symmetric up to rounding), then
+  // sigma = max row sum of |M| over the copy
+  for (int e = t; e < k * k; e += blockDim.x) {
+    const int i = e % k, jj = e / k;
+    A.out[e] = 0.5 * (A.M[i + jj * A.ldm] + A.M[jj + i * A.ldm]);
   }
+  __syncthreads();
+  // row sums in np parts of the columns (fixed order), then over the parts
+  const int np = blockDim.x / k;  // k <= 128: at least 2
+  if (t < np * k) {
+    const int i = t % k, p = t / k;
+    double s = 0.0;
+    for (int jj = p; jj < k; jj += np) s += fabs(A.out[i + jj * k]);
+    s_max[t] = s;
+  }
+  __syncthreads();
+  double mx = 0.0;
+  if (t < k) {
+    double s = 0.0;
+    for (int p = 0; p < np; ++p) s += s_max[t + p * k];
+    mx = s;
+  }
+  __syncthreads();
   s_max[t] = mx;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -398,12 +485,7 @@ __global__ __launch_bounds__(256) void k_shift_sym(ShiftArgs A0, ShiftArgs A1) {
     __syncthreads();
   }
   const double sg = s_max[0];
-  for (int e = t; e < k * k; e += blockDim.x) {
-    const int i = e % k, jj = e / k;
-    // symmetrised: the Galerkin products are symmetric up to rounding
-    const double v = 0.5 * (A.M[i + jj * A.ldm] + A.M[jj + i * A.ldm]);
-    A.out[e] = v + (i == jj ? sg : 0.0);
-  }
+  for (int i = t; i < k; i += blockDim.x) A.out[i + i * k] += sg;
   if (t == 0) *A.shift = sg;
 }
 

@@ -69,7 +69,7 @@ struct DeSmall {
   double ework[2][kMaxAug * kMaxAug + 2];  // shifted matrix + shift, per concurrent eigenproblem
   double Uscr[2][kMaxAug * kMaxAug];
   double svd2[kMaxAug * kMaxAug];          // global Jacobi work of the second eigenproblem (k > 64)
-  double swork[2 * kMaxAug * kMaxAug];
+  double swork[2 * kMaxAug * kMaxAug + 2 * kMaxAug * kMaxForce];
   int rscr[2];
 };
 
@@ -127,6 +127,11 @@ struct lri_ctx {
   bool spec[2] = {false, false};
   double *specQ[2] = {nullptr, nullptr}, *specLam[2] = {nullptr, nullptr}, *specF[2] = {nullptr, nullptr};
   bool spec_alias = false;
+  // The Galerkin matrices U'AU, V'BV, U'G, H'V and their eigendecompositions in
+  // the DE work area belong to the current U and V: a BUG step ends with the
+  // S sub-step's ones, which are the next BUG step's K/L ones. Any other entry
+  // point clears the flag.
+  bool de_gal_fresh = false;
 
   // ------------------------------------------------------------------ helpers
   int pass_mode() const {
@@ -158,7 +163,7 @@ struct lri_ctx {
   }
   template <class F>
   void timed(int family, F&& f) {
-    if (!timing || (timing_passes_only && family != LRI_KERNEL_PASS_N && family != LRI_KERNEL_PASS_T)) {
+    if (!timing || (timing_passes_only && family == LRI_KERNEL_SMALL)) {
       HIPCHECK(f());
       return;
     }
@@ -870,6 +875,23 @@ struct lri_ctx {
   // in the analytic eigenbasis (Zhat = Q' Z, closed-form flow, Z = Q Zhat, with
   // Zhat in the P / Q work buffers), the others in one k_rk4_columns launch.
   void de_run(const RK4ColArgs& P0, const RK4ColArgs& P1 = RK4ColArgs()) {
+    if (P0.ncols > 0 && P1.ncols > 0 && spec[P0.side] && spec[P1.side] && specQ[P0.side] == specQ[P1.side] &&
+        P0.n == P1.n && P1.X == P0.X + P0.n * P0.ncols && P1.Y == P1.X && P0.Y == P0.X &&
+        P0.ncols + P1.ncols <= 64) {
+      // one eigenbasis, adjacent columns: one pair of transforms over both
+      const int k0 = P0.ncols, k1 = P1.ncols;
+      double* Zh = this->P;
+      spectral_gemm(true, specQ[P0.side], P0.n, P0.X, k0 + k1, Zh);
+      timed(LRI_KERNEL_SMALL, [&] {
+        hipError_t e = launch_spectral_flow(Zh, P0.n, k0, specLam[P0.side], P0.mu, specF[P0.side], P0.coef, P0.q,
+                                            P0.h, P0.nsub, stream);
+        if (e != hipSuccess) return e;
+        return launch_spectral_flow(Zh + P0.n * k0, P1.n, k1, specLam[P1.side], P1.mu, specF[P1.side], P1.coef,
+                                    P1.q, P1.h, P1.nsub, stream);
+      });
+      spectral_gemm(false, specQ[P0.side], P0.n, Zh, k0 + k1, P0.Y);
+      return;
+    }
     RK4ColArgs rk[2];
     int nrk = 0;
     for (const RK4ColArgs* Pp : {&P0, &P1}) {
@@ -881,7 +903,7 @@ struct lri_ctx {
       }
       double* Zh = Pa.side == 0 ? this->P : this->Q;
       spectral_gemm(true, specQ[Pa.side], Pa.n, Pa.X, Pa.ncols, Zh);
-      timed(LRI_KERNEL_DE, [&] {
+      timed(LRI_KERNEL_SMALL, [&] {
         return launch_spectral_flow(Zh, Pa.n, Pa.ncols, specLam[Pa.side], Pa.mu, specF[Pa.side], Pa.coef, Pa.q,
                                     Pa.h, Pa.nsub, stream);
       });
@@ -995,20 +1017,32 @@ struct lri_ctx {
   }
 
   // unconventional_step! with the DE sub-problems (unconventional.jl:51-79,133-157)
+  // Scratch columns of the L sub-problem of a step that runs K and L together:
+  // right after the K ones in K2 when both sides share one eigenbasis (so that
+  // de_run transforms both at once), else L2.
+  double* de_lscratch(int k) {
+    const bool fuse = n == m && spec[0] && spec[1] && specQ[0] == specQ[1] && 2 * k <= 64;
+    return fuse ? K2 + n * k : L2;
+  }
+
   void de_bug(double dt, int64_t nsub) {
     const int rr = (int)r;
     const double h = dt / (double)nsub;
-    de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
-    de_galerkin(U, true, rr, de->Ga, de->gaT);   // At = U0'AU0
-    de_eig2(rr, rr);
+    if (!de_gal_fresh) {
+      de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
+      de_galerkin(U, true, rr, de->Ga, de->gaT);   // At = U0'AU0
+      de_eig2(rr, rr);
+    }
+    de_gal_fresh = false;
     // the K and L sub-problems are independent: one RK4 launch for both
+    double* lscr = de_lscratch(rr);
     const RK4ColArgs Pk = de_prepare(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, 0, h, nsub, K2);
-    const RK4ColArgs Pl = de_prepare(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, 1, h, nsub, L2);
+    const RK4ColArgs Pl = de_prepare(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, 1, h, nsub, lscr);
     de_run(Pk, Pl);
     de_finish(true, rr, de->Wb, K, K2, shifted_chol(n));
     EpiArgs eM = de_epi(EPI_BUG_M);  // M = U1' U0
     cholqr_tail(K, K2, n, rr, U2, false, U, rr, eM, 0, 3);
-    de_finish(false, rr, de->Wa, L, L2, shifted_chol(m));
+    de_finish(false, rr, de->Wa, L, lscr, shifted_chol(m));
     EpiArgs eN = de_epi(EPI_BUG_N);  // N = V1' V0, Sinit = M S0 N'
     eN.S = S;
     eN.ld_s = (int)rcap;
@@ -1019,6 +1053,7 @@ struct lri_ctx {
     de_sstep(rr, rr, small->Sinit, rr, 0, S, (int)rcap, 1.0, h, nsub);
     std::swap(U, U2);
     std::swap(V, V2);
+    de_gal_fresh = true;  // Ga, Gb, gaT, hbT and their eigenpairs now belong to U, V
   }
 
   // rankadaptive_unconventional_step! with the DE sub-problems
@@ -1031,11 +1066,12 @@ struct lri_ctx {
     de_galerkin(V, false, rr, de->Gb, de->hbT);  // Bt = V0'BV0
     de_galerkin(U, true, rr, de->Ga, de->gaT);   // At = U0'AU0
     de_eig2(rr, rr);
+    double* lscr = de_lscratch(rr);
     const RK4ColArgs Pk = de_prepare(U, true, rr, S, (int)rcap, 0, de->Wb, de->beta, de->hbT, 0, h, nsub, K2);
-    const RK4ColArgs Pl = de_prepare(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, 1, h, nsub, L2);
+    const RK4ColArgs Pl = de_prepare(V, false, rr, S, (int)rcap, 1, de->Wa, de->alpha, de->gaT, 1, h, nsub, lscr);
     de_run(Pk, Pl);
     de_finish(true, rr, de->Wb, K, K2, none);
-    de_finish(false, rr, de->Wa, L, L2, none);
+    de_finish(false, rr, de->Wa, L, lscr, none);
     HIPCHECK(hipMemcpyAsync(Uhat, U, sizeof(double) * n * rr, hipMemcpyDeviceToDevice, stream));
     EpiArgs eM;
     eM.kind = EPI_RA_M;
@@ -1124,9 +1160,10 @@ struct lri_ctx {
 
 namespace {
 
-template <class F>
+template <bool KeepDeGalerkin = false, class F>
 int guarded(lri_ctx* ctx, F&& f) {
   if (!ctx) return LRI_EINVAL;
+  if (!KeepDeGalerkin) ctx->de_gal_fresh = false;
   try {
     HIPCHECK(hipSetDevice(ctx->device));
     int st = f();
@@ -1286,7 +1323,7 @@ int lri_destroy(lri_ctx* ctx) {
 }
 
 int lri_set_precision(lri_ctx* ctx, int precision) {
-  return guarded(ctx, [&] {
+  return guarded<true>(ctx, [&] {
     REQUIRE(precision == LRI_PREC_F64 || precision == LRI_PREC_MIXED, "precision must be LRI_PREC_F64 or LRI_PREC_MIXED");
     ctx->prec = precision;
     return LRI_OK;
@@ -1313,7 +1350,7 @@ int lri_set_state(lri_ctx* ctx, int64_t r, const double* U, int64_t ldu, const d
 
 int lri_get_state(lri_ctx* ctx, int64_t* r_out, double* U, int64_t ldu, double* S, int64_t lds, double* V,
                   int64_t ldv) {
-  return guarded(ctx, [&] {
+  return guarded<true>(ctx, [&] {
     int st = ctx->read_flags();
     const int64_t r = ctx->r;
     const size_t d = sizeof(double);
@@ -1337,7 +1374,7 @@ int lri_get_state(lri_ctx* ctx, int64_t* r_out, double* U, int64_t ldu, double* 
 }
 
 int lri_get_rank(lri_ctx* ctx, int64_t* r_out) {
-  return guarded(ctx, [&] {
+  return guarded<true>(ctx, [&] {
     REQUIRE(r_out, "r_out is NULL");
     int st = ctx->read_flags();
     *r_out = ctx->r;
@@ -1517,7 +1554,7 @@ int lri_step_de_ksl(lri_ctx* ctx, int order, double dt, int64_t substeps) {
 }
 
 int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps) {
-  return guarded(ctx, [&] {
+  return guarded<true>(ctx, [&] {
     ctx->check_de(dt, substeps);
     DeScope scope(ctx);
     ctx->de_bug(dt, substeps);
@@ -1541,11 +1578,11 @@ int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, in
 }
 
 int lri_synchronize(lri_ctx* ctx) {
-  return guarded(ctx, [&] { return ctx->read_flags(); });
+  return guarded<true>(ctx, [&] { return ctx->read_flags(); });
 }
 
 int lri_get_stream(lri_ctx* ctx, void** stream) {
-  return guarded(ctx, [&] {
+  return guarded<true>(ctx, [&] {
     REQUIRE(stream, "stream is NULL");
     *stream = (void*)ctx->stream;
     return LRI_OK;
@@ -1553,7 +1590,7 @@ int lri_get_stream(lri_ctx* ctx, void** stream) {
 }
 
 int lri_kernel_timing(lri_ctx* ctx, int mode) {
-  return guarded(ctx, [&] {
+  return guarded<true>(ctx, [&] {
     REQUIRE(mode >= 0 && mode <= 2, "timing mode must be 0, 1 or 2");
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     ctx->drain_timing();
@@ -1568,7 +1605,7 @@ int lri_kernel_timing(lri_ctx* ctx, int mode) {
 }
 
 int lri_kernel_timing_read(lri_ctx* ctx, int family, double* total_ms, int64_t* launches) {
-  return guarded(ctx, [&] {
+  return guarded<true>(ctx, [&] {
     REQUIRE(family >= 0 && family < LRI_KERNEL_COUNT, "unknown kernel family");
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     ctx->drain_timing();

@@ -199,7 +199,7 @@ struct SStepArgs {
   double sgn = 1.0;
   double h = 0.0;
   int64_t nsub = 0;
-  double* work = nullptr;       // 2 k l doubles
+  double* work = nullptr;       // 2 k l + (k + l) kMaxForce doubles
 };
 hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st);
 

@@ -49,6 +49,16 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
   return __builtin_fma(0.5 * y, r, y);
 }
 
+// 1/d for normal d: the hardware estimate refined by two Newton steps
+// y <- y + y (1 - d y).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double r = __builtin_fma(-d, y, 1.0);
+  y = __builtin_fma(y, r, y);
+  r = __builtin_fma(-d, y, 1.0);
+  return __builtin_fma(y, r, y);
+}
+
 // Exponent e (even) with max_diag * 2^-e in [0.5, 2): the Gram matrix is
 // scaled by 2^-e (exactly), so R scales by 2^(-e/2) and every pivot stays a
 // normal number; zero, NaN and Inf give 0.
@@ -827,6 +837,164 @@ template hipError_t launch_rows<float>(RowArgs<float>, hipStream_t);
 // Pairs are processed in round-robin (circle-method) order, npairs at a time.
 // ----------------------------------------------------------------------------
 
+// Singular values (column norms of the rotated A, k x ke, ld lda), the sorted
+// outputs U = A / sigma and V, and the truncation rank; nthr threads take part.
+__device__ void jacobi_finish(const SvdArgs& P, const double* A, int lda, const double* Vm, int ldv, double* sig,
+                              int nthr) {
+  const int k = P.k;
+  const int ke = (k + 1) & ~1;
+  const int t = threadIdx.x;
+  for (int j = t; j < ke; j += nthr) {
+    double s = 0.0;
+    if (j < k)
+      for (int i = 0; i < k; ++i) s += A[i + j * lda] * A[i + j * lda];
+    sig[j] = (j < k) ? sqrt(s) : -1.0;
+  }
+  __syncthreads();
+  for (int j = t; j < k; j += nthr) {
+    int pos = 0;
+    for (int l = 0; l < k; ++l) pos += (sig[l] > sig[j]) || (sig[l] == sig[j] && l < j);
+    P.sigma[pos] = sig[j];
+    const double inv = sig[j] > 0.0 ? 1.0 / sig[j] : 0.0;
+    for (int i = 0; i < k; ++i) {
+      P.U[i + pos * k] = A[i + j * lda] * inv;
+      P.V[i + pos * k] = Vm[i + j * ldv];
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    // P.sigma holds the sorted values (written above by this workgroup)
+    double tail = 0.0;
+    int rho = k;
+    for (int j = k - 1; j >= 1; --j) {
+      double nt = tail + P.sigma[j] * P.sigma[j];
+      if (sqrt(nt) <= P.tol) {
+        tail = nt;
+        rho = j;
+      } else {
+        break;
+      }
+    }
+    if (rho > P.rmax) rho = (int)P.rmax;
+    if (rho < 1) rho = 1;
+    *P.r_out = rho;
+  }
+}
+
+// x from lane (lane ^ m) of its row of 16 for the DPP patterns used below
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Sum over aligned groups of G lanes (G a power of two), left in every lane of
+// the group.  Quad swaps (quad_perm [1,0,3,2] and [2,3,0,1]) make every quad
+// uniform; the half-row mirror (lane i <- 7 - i) then pairs the two quads of
+// an 8-lane group and the row mirror (i <- 15 - i) the two halves of a row.
+template <int G>
+__device__ __forceinline__ double group_sum(double x) {
+  if (G >= 2) x += dpp_f64<0xB1>(x);
+  if (G >= 4) x += dpp_f64<0x4E>(x);
+  if (G >= 8) x += dpp_f64<0x141>(x);
+  if (G >= 16) x += dpp_f64<0x140>(x);
+  if (G >= 32) x += __shfl_xor(x, 16, 32);
+  if (G >= 64) x += __shfl_xor(x, 32, 64);
+  return x;
+}
+
+// The same Jacobi sweeps for k <= 32 in one wavefront: G lanes per column pair,
+// rows ln, ln + G, ... of the pair in each lane, the group sums through DPP,
+// no workgroup barriers (LDS operations of one wavefront complete in order).
+template <int G>
+__device__ void jacobi_wave_body(const SvdArgs& P) {
+  constexpr int R = (128 / (G * G)) > 0 ? 128 / (G * G) : 1;  // rows per lane: ke <= 128 / G
+  constexpr int LDA = 33;                                       // odd column stride against bank conflicts
+  __shared__ double A[LDA * 32];
+  __shared__ double Vm[LDA * 32];
+  __shared__ double sig[32];
+  // rows i = ln + G j of every lane stay below 32 unless G = 64 (ke = 2)
+  constexpr bool kGuard = G * R > 32;
+  const int k = P.k;
+  const int ke = (k + 1) & ~1;
+  const int t = threadIdx.x;
+  // the whole arrays: rows and columns past k (A) and ke (V) hold zeros, which
+  // the rotations keep, so the row loops below need no bounds
+  for (int e = t; e < LDA * 32; e += 64) {
+    const int i = e % LDA, j = e / LDA;
+    A[e] = (i < k && j < k) ? P.S[i + j * P.lds] : 0.0;
+    Vm[e] = (i == j && i < ke) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int npairs = ke / 2;
+  const int pair = t / G, ln = t % G;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int rot = 0;
+    for (int round = 0; round < ke - 1; ++round) {
+      int pa = 0, pb = 0;
+      const bool act = pair < npairs;
+      if (pair == 0) {
+        pa = 0;
+        pb = 1 + (round % (ke - 1));
+      } else if (act) {
+        pa = 1 + ((round + pair) % (ke - 1));
+        pb = 1 + ((round - pair + (ke - 1)) % (ke - 1));
+      }
+      double x[R], y[R], u[R], w[R];
+      double al = 0.0, be = 0.0, ga = 0.0;
+      // the pair's rows of A and of V, all loaded up front (one LDS wait)
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int i = ln + G * j;
+        const bool ok = !kGuard || i < ke;  // inactive groups read column 0 and never rotate
+        x[j] = ok ? A[i + pa * LDA] : 0.0;
+        y[j] = ok ? A[i + pb * LDA] : 0.0;
+        u[j] = ok ? Vm[i + pa * LDA] : 0.0;
+        w[j] = ok ? Vm[i + pb * LDA] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        al = __builtin_fma(x[j], x[j], al);
+        be = __builtin_fma(y[j], y[j], be);
+        ga = __builtin_fma(x[j], y[j], ga);
+      }
+      al = group_sum<G>(al);
+      be = group_sum<G>(be);
+      ga = group_sum<G>(ga);
+      // rotate unless |ga| <= 4 eps sqrt(al be); tan of the angle
+      // t = sign(d) 2ga / (|d| + sqrt(d^2 + 4ga^2)), d = be - al (|t| <= 1),
+      // through refined hardware reciprocals (column norms are normal numbers)
+      if (act && ga * ga > 16.0 * kEps * kEps * al * be) {
+        const double d = be - al;
+        const double g2 = 2.0 * ga;
+        const double q = __builtin_fma(d, d, g2 * g2);
+        const double den = fabs(d) + q * rsqrt_nr(q);
+        double tt = g2 * rcp_nr(den);
+        if (d < 0.0) tt = -tt;
+        const double c = rsqrt_nr(__builtin_fma(tt, tt, 1.0));
+        const double sn = c * tt;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const int i = ln + G * j;
+          if (!kGuard || i < ke) {
+            A[i + pa * LDA] = c * x[j] - sn * y[j];
+            A[i + pb * LDA] = sn * x[j] + c * y[j];
+            Vm[i + pa * LDA] = c * u[j] - sn * w[j];
+            Vm[i + pb * LDA] = sn * u[j] + c * w[j];
+          }
+        }
+        rot = 1;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!__any(rot)) break;
+  }
+  __syncthreads();
+  jacobi_finish(P, A, LDA, Vm, LDA, sig, 64);
+}
+
 __device__ void jacobi_svd_body(const SvdArgs& P, double* lds) {
   const int k = P.k;
   const int ke = (k + 1) & ~1;
@@ -896,47 +1064,29 @@ __device__ void jacobi_svd_body(const SvdArgs& P, double* lds) {
     __syncthreads();
     if (!any) break;
   }
-  for (int j = t; j < ke; j += kThreads) {
-    double s = 0.0;
-    if (j < k)
-      for (int i = 0; i < k; ++i) s += A[i + j * k] * A[i + j * k];
-    sig[j] = (j < k) ? sqrt(s) : -1.0;
-  }
-  __syncthreads();
-  for (int j = t; j < k; j += kThreads) {
-    int pos = 0;
-    for (int l = 0; l < k; ++l) pos += (sig[l] > sig[j]) || (sig[l] == sig[j] && l < j);
-    P.sigma[pos] = sig[j];
-    const double inv = sig[j] > 0.0 ? 1.0 / sig[j] : 0.0;
-    for (int i = 0; i < k; ++i) {
-      P.U[i + pos * k] = A[i + j * k] * inv;
-      P.V[i + pos * k] = Vm[i + j * ke];
-    }
-  }
-  __syncthreads();
-  if (t == 0) {
-    // P.sigma holds the sorted values (written above by this workgroup)
-    double tail = 0.0;
-    int rho = k;
-    for (int j = k - 1; j >= 1; --j) {
-      double nt = tail + P.sigma[j] * P.sigma[j];
-      if (sqrt(nt) <= P.tol) {
-        tail = nt;
-        rho = j;
-      } else {
-        break;
-      }
-    }
-    if (rho > P.rmax) rho = (int)P.rmax;
-    if (rho < 1) rho = 1;
-    *P.r_out = rho;
-  }
+  jacobi_finish(P, A, k, Vm, ke, sig, kThreads);
 }
 
 // workgroup 0 decomposes P0, workgroup 1 (if launched) P1
 __global__ __launch_bounds__(kThreads) void k_jacobi_svd(SvdArgs P0, SvdArgs P1) {
   extern __shared__ double lds[];
   jacobi_svd_body(blockIdx.x == 0 ? P0 : P1, lds);
+}
+
+// one wavefront per decomposition, k <= 32 (workgroup b decomposes P_b)
+__global__ __launch_bounds__(64) void k_jacobi_wave(SvdArgs P0, SvdArgs P1) {
+  const SvdArgs& P = blockIdx.x == 0 ? P0 : P1;
+  const int np = ((P.k + 1) & ~1) / 2;
+  if (np > 8)
+    jacobi_wave_body<4>(P);
+  else if (np > 4)
+    jacobi_wave_body<8>(P);
+  else if (np > 2)
+    jacobi_wave_body<16>(P);
+  else if (np > 1)
+    jacobi_wave_body<32>(P);
+  else
+    jacobi_wave_body<64>(P);
 }
 
 // LDS doubles of one decomposition; clears P.work when the right factor fits in LDS
@@ -957,7 +1107,10 @@ hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st) {
   size_t lds;
   hipError_t e = jacobi_plan(P, &lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_jacobi_svd, dim3(1), dim3(kThreads), lds, st, P, P);
+  if (P.k <= 32)
+    hipLaunchKernelGGL(k_jacobi_wave, dim3(1), dim3(64), 0, st, P, P);
+  else
+    hipLaunchKernelGGL(k_jacobi_svd, dim3(1), dim3(kThreads), lds, st, P, P);
   return hipGetLastError();
 }
 
@@ -966,7 +1119,10 @@ hipError_t launch_jacobi_svd2(SvdArgs P0, SvdArgs P1, hipStream_t st) {
   hipError_t e = jacobi_plan(P0, &l0);
   if (e == hipSuccess) e = jacobi_plan(P1, &l1);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_jacobi_svd, dim3(2), dim3(kThreads), l0 > l1 ? l0 : l1, st, P0, P1);
+  if (P0.k <= 32 && P1.k <= 32)
+    hipLaunchKernelGGL(k_jacobi_wave, dim3(2), dim3(64), 0, st, P0, P1);
+  else
+    hipLaunchKernelGGL(k_jacobi_svd, dim3(2), dim3(kThreads), l0 > l1 ? l0 : l1, st, P0, P1);
   return hipGetLastError();
 }
 

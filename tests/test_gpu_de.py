@@ -312,3 +312,44 @@ def test_mixed_spectral_and_stencil_sides(lri, name):
     got = run_one(lri, name, dev, u, dt, N)
     ref = oracle_one(name, host, u, dt, N)
     assert_close(got, ref)
+
+
+def test_bug_steps_reuse_galerkin_data(lri):
+    """One operator on both sides (n = m): the K and L sub-steps share one pair of
+    eigenbasis transforms, and a BUG step reuses the Galerkin matrices and
+    eigenpairs its predecessor's S sub-step left for the new bases.  A set_state
+    or a KSL step in between must make the next BUG step recompute them."""
+    n = m = 64
+    r = 5
+    say(f"case: BUG steps with reused Galerkin data, n=m={n} r={r}")
+    dev, host = operator_pair(lri, n, m, 1, False, False, seed=18, sa=30.0, sb=30.0)
+    u = random_factors(np.random.default_rng(19), n, m, r)
+    u2 = random_factors(np.random.default_rng(20), n, m, r)
+    dt = 0.02
+    N = lri.rk4_substeps(dev.spectral_bound(), dt)
+    nat = lri.native
+    ctx = nat.Context(n, m, r)
+    try:
+        ctx.set_state(u.U, u.S, u.V)
+        ctx.set_operator_lyapunov(dev.A.d, dev.A.e, False, dev.B.d, dev.B.e, False, dev.G, dev.H)
+        ref = (u.U, u.S, u.V)
+        for step in range(3):
+            ctx.step_de_bug(dt, N)
+            ref = dlra_de.bug_de_step(host, *ref, 0.0, dt, N)
+            say(f"  BUG step {step + 1}")
+            assert_close(ctx.get_state(), ref)
+        ctx.set_state(u2.U, u2.S, u2.V)
+        ctx.step_de_bug(dt, N)
+        ref = dlra_de.bug_de_step(host, u2.U, u2.S, u2.V, 0.0, dt, N)
+        say("  BUG step after set_state")
+        assert_close(ctx.get_state(), ref)
+        ctx.step_de_ksl(nat.PRIMAL, dt, N)
+        ref = dlra_de.ksl_primal_de_step(host, *ref, 0.0, dt, N)
+        say("  KSL primal step")
+        assert_close(ctx.get_state(), ref)
+        ctx.step_de_bug(dt, N)
+        ref = dlra_de.bug_de_step(host, *ref, 0.0, dt, N)
+        say("  BUG step after KSL")
+        assert_close(ctx.get_state(), ref)
+    finally:
+        ctx.close()
