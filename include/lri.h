@@ -141,6 +141,17 @@ int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next);
 int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax,
                     int64_t* r_new);
 
+/* Greedy step on an SVD-like state (greedy_integrator.jl:94-104,106-111)
+ * with X = A[k_next] itself (update_data!, data_integrator.jl:22-28):
+ * U1 = orth(X V0), V1 = orth(X' U0), S1 = U1' X V1.  Three passes over X. */
+int lri_step_greedy(lri_ctx* ctx, int64_t k_next);
+
+/* Greedy step on a TwoFactorRepresentation Y = U Z' (greedy_integrator.jl:84-92):
+ * the state's V holds Z (m x r, not orthonormal; S is not used).
+ * Z1 = X' U0 and U1 = Q P' for the SVD X Z1 = Q Sigma P' (the orthogonal
+ * polar factor of X Z1), X = A[k_next].  Two passes over X. */
+int lri_step_greedy_twofactor(lri_ctx* ctx, int64_t k_next);
+
 /* ---- matrix ODE (MatrixDEProblem, primitives.jl:13-17) --------------------
  *
  * The reference evaluates a user closure F(Y, t) on low-rank arguments inside

@@ -1,7 +1,7 @@
 """MI355X (gfx950) engine for the per-step DLRA hot path of LowRankIntegrators.jl.
 
 The steps of the projector-splitting (KSL), unconventional (BUG) and
-rank-adaptive BUG integrators on data problems, and on matrix ODEs with a
+rank-adaptive BUG integrators (and the Greedy integrator) on data problems, and on matrix ODEs with a
 tridiagonal Lyapunov-type right-hand side, run in ``liblri_hip.so``
 (hand-written HIP kernels behind the C ABI of ``include/lri.h``); this package
 is the host side: a ctypes binding (``native``) and a mirror of the
@@ -12,6 +12,7 @@ from .api import (  # noqa: F401
     DLRIntegrator,
     DLRSolution,
     DualLieTrotter,
+    GreedyIntegrator,
     LyapunovOperator,
     Matrix,
     MatrixDataProblem,

@@ -67,8 +67,19 @@ class TwoFactorRepresentation:
     U: np.ndarray
     Z: np.ndarray
 
+    @property
+    def rank(self) -> int:
+        return int(self.U.shape[1])
+
+    @property
+    def shape(self):
+        return (self.U.shape[0], self.Z.shape[0])
+
     def matrix(self) -> np.ndarray:
         return self.U @ self.Z.T
+
+    def copy(self) -> "TwoFactorRepresentation":
+        return TwoFactorRepresentation(self.U.copy(), self.Z.copy())
 
 
 def Matrix(u) -> np.ndarray:
@@ -274,6 +285,17 @@ class RankAdaptiveUnconventionalAlgorithm(AbstractDLRAlgorithm):
         self.rmax = int(rmax)
 
 
+class GreedyIntegrator(AbstractDLRAlgorithm):
+    """``GreedyIntegrator(; Z_alg, Z_kwargs)`` (greedy_integrator.jl:16-22).  ``Z_alg`` and
+    ``Z_kwargs`` drive the Z sub-problem of a ``MatrixHybridProblem``, which this
+    engine does not run; on a ``MatrixDataProblem`` they are unused, as in the
+    reference."""
+
+    def __init__(self, *, Z_alg=None, Z_kwargs=None):
+        self.Z_alg = Z_alg
+        self.Z_kwargs = dict(Z_kwargs or {})
+
+
 # ----------------------------------------------------------------------------
 # solution / integrator (primitives.jl:46-104)
 # ----------------------------------------------------------------------------
@@ -328,10 +350,12 @@ class DLRIntegrator:
         self._array_data = isinstance(prob, MatrixDataProblem) and not callable(prob.y)
         self._slot_of_t = 0  # function data: ring slot (of 3) holding y(t)
         self.rank = u.rank
+        self.twofactor = isinstance(u, TwoFactorRepresentation)
 
     def refresh_u(self):
         U, S, V = self.ctx.get_state()
-        self.u = SVDLikeRepresentation(U, S, V)
+        # a two-factor state keeps Z in the V slot of the device state
+        self.u = TwoFactorRepresentation(U, V) if self.twofactor else SVDLikeRepresentation(U, S, V)
         return self.u
 
 
@@ -344,16 +368,23 @@ def init(prob, alg, dt, *, precision: str = "f64", snapshot_dtype: Optional[str]
          save_everystep: bool = True, device: int = 0, substeps: Optional[int] = None) -> DLRIntegrator:
     """``init(prob, alg, dt)`` (projector_splitting.jl:107-115 and the BUG/RA counterparts)."""
     if isinstance(prob, MatrixHybridProblem):
-        raise NotImplementedError("MatrixHybridProblem belongs to the Greedy integrator, which this engine does not run")
+        raise NotImplementedError("MatrixHybridProblem: its Z sub-problem is a user closure integrated by an ODE solver "
+                                  "(greedy_integrator.jl:41-46,73-82), which cannot run on the device")
     if isinstance(prob, MatrixDEProblem):
         return _init_de(prob, alg, dt, save_everystep=save_everystep, device=device, substeps=substeps)
-    if not isinstance(alg, (ProjectorSplitting, UnconventionalAlgorithm, RankAdaptiveUnconventionalAlgorithm)):
+    if not isinstance(alg, (ProjectorSplitting, UnconventionalAlgorithm, RankAdaptiveUnconventionalAlgorithm,
+                            GreedyIntegrator)):
         raise TypeError(f"unsupported algorithm {alg!r}")
     t0, tf = prob.tspan
     if not tf > t0:
         raise AssertionError("Integration in reverse time direction is not supported")
     u0 = prob.u0
-    n, m = u0.U.shape[0], u0.V.shape[0]
+    if isinstance(u0, TwoFactorRepresentation) and not isinstance(alg, GreedyIntegrator):
+        # only greedy_integrator.jl has TwoFactorRepresentation methods (:24-30,84-92)
+        raise TypeError("a TwoFactorRepresentation state needs the GreedyIntegrator")
+    twofactor = isinstance(u0, TwoFactorRepresentation)
+    n = u0.U.shape[0]
+    m = (u0.Z if twofactor else u0.V).shape[0]
     r = u0.rank
     if isinstance(alg, RankAdaptiveUnconventionalAlgorithm):
         rcap = int(min(alg.rmax, native.MAX_RANK, n, m))
@@ -365,7 +396,10 @@ def init(prob, alg, dt, *, precision: str = "f64", snapshot_dtype: Optional[str]
         raise ValueError(f"rank {r} exceeds LRI_MAX_RANK = {native.MAX_RANK}")
     ctx = native.Context(n, m, rcap, device)
     ctx.set_precision(native.PREC_MIXED if precision == "mixed" else native.PREC_F64)
-    ctx.set_state(u0.U, u0.S, u0.V)
+    if twofactor:
+        ctx.set_state(u0.U, np.eye(r), u0.Z)  # Z in the V slot, S unused
+    else:
+        ctx.set_state(u0.U, u0.S, u0.V)
 
     # data: device tensors are bound as borrowed snapshots, host data is uploaded
     y = prob.y
@@ -465,7 +499,12 @@ def step(integrator: DLRIntegrator, alg=None, dt=None) -> None:
     else:
         cur = integrator._slot_of_t
         free = [s for s in range(3) if s != cur]
-        if isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang):
+        if isinstance(alg, GreedyIntegrator):
+            # greedy_step! reads only y(t + dt) (update_data!, data_integrator.jl:22-24)
+            ctx.upload(free[0], _encode(integrator.prob.y(t + dt), integrator.snapshot_dtype))
+            _dispatch(integrator, alg, cur, free[0], None)
+            integrator._slot_of_t = free[0]
+        elif isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang):
             ctx.upload(free[0], _encode(integrator.prob.y(t + dt / 2), integrator.snapshot_dtype))
             ctx.upload(free[1], _encode(integrator.prob.y(t + dt / 2 + dt / 2), integrator.snapshot_dtype))
             ctx.step_ksl_strang(cur, free[0], free[1])
@@ -480,7 +519,12 @@ def step(integrator: DLRIntegrator, alg=None, dt=None) -> None:
 
 def _dispatch(integrator, alg, kp, kn, _):
     ctx = integrator.ctx
-    if isinstance(alg, ProjectorSplitting):
+    if isinstance(alg, GreedyIntegrator):
+        if integrator.twofactor:
+            ctx.step_greedy_twofactor(kn)
+        else:
+            ctx.step_greedy(kn)
+    elif isinstance(alg, ProjectorSplitting):
         ctx.step_ksl(native.PRIMAL if isinstance(alg.order, PrimalLieTrotter) else native.DUAL, kp, kn)
     elif isinstance(alg, UnconventionalAlgorithm):
         ctx.step_bug(kp, kn)

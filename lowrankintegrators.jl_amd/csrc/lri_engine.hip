@@ -234,12 +234,13 @@ struct lri_ctx {
     if (!forN && (pkT_mode != mode || pkT_rb != rb)) pack(U, n, n, r, false);
   }
 
-  // The product passes over dA = A[kn] - A[kp] with the currently packed B.
+  // The product passes over dA = A[kn] - A[kp] with the currently packed B
+  // (kp < 0: over A[kn] itself).
   int passN(int64_t kp, int64_t kn, int64_t cols, void* part) {
     const int mode = pass_mode();
     PassArgs a;
     a.anext = snaps[kn];
-    a.aprev = snaps[kp];
+    a.aprev = kp >= 0 ? snaps[kp] : nullptr;
     a.dtype = sdtype;
     a.lda = sld;
     a.n = n;
@@ -263,7 +264,7 @@ struct lri_ctx {
     const int mode = pass_mode();
     PassArgs a;
     a.anext = snaps[kn];
-    a.aprev = snaps[kp];
+    a.aprev = kp >= 0 ? snaps[kp] : nullptr;
     a.dtype = sdtype;
     a.lda = sld;
     a.n = n;
@@ -538,6 +539,123 @@ struct lri_ctx {
     });
     std::swap(U, U2);
     std::swap(V, V2);
+  }
+
+  // greedy_step! on an SVD-like state (greedy_integrator.jl:94-104) with
+  // X = A[kn] (update_data!, data_integrator.jl:22-28): U1 = orth(X V0),
+  // V1 = orth(X' U0), S1 = U1' X V1.  The QRs are shifted CholeskyQR3 (the
+  // products carry the snapshot's whole spectrum); dependent directions become
+  // zero columns.
+  void greedy(int64_t kn) {
+    const int rr = (int)r;
+    const int rb = rb_for(pass_mode(), rr);
+    const EpiArgs none = de_epi();
+    ensure_packed(true);
+    ensure_packed(false);
+    const int sN = passN(-1, kn, rr, partN);  // X V0
+    const int sT = passT(-1, kn, rr, partT);  // X' U0
+    with_parts(partN, [&](auto A) {
+      A.rows = n;
+      A.nsplit = sN;
+      A.rb = rb;
+      A.r_out = rr;
+      A.out = K;
+      A.ldo = n;
+      A.reduce = 1;
+      A.epi = shifted_chol(n);
+      rows(A);
+    });
+    cholqr_tail(K, K2, n, rr, U2, false, nullptr, 0, none, 0, 3);  // U1 (packed for pass T)
+    with_parts(partT, [&](auto A) {
+      A.rows = m;
+      A.nsplit = sT;
+      A.rb = rb;
+      A.r_out = rr;
+      A.out = L;
+      A.ldo = m;
+      A.reduce = 1;
+      A.epi = shifted_chol(m);
+      rows(A);
+    });
+    cholqr_tail(L, L2, m, rr, V2, true, nullptr, 0, none, 0, 3);  // V1 (packed for pass N)
+    const int sN2 = passN(-1, kn, rr, partN);  // X V1
+    with_parts(partN, [&](auto A) {  // C = (X V1)' U1 = S1'
+      A.rows = n;
+      A.nsplit = sN2;
+      A.rb = rb;
+      A.r_out = rr;
+      A.reduce = 1;
+      A.W = U2;
+      A.ldw = n;
+      A.rw = rr;
+      A.epi.kind = EPI_COPY;
+      A.epi.transpose = 1;
+      A.epi.dst = S;
+      A.epi.ld_dst = (int)rcap;
+      rows(A);
+    });
+    std::swap(U, U2);
+    std::swap(V, V2);
+  }
+
+  // greedy_step! on a two-factor state U Z' (greedy_integrator.jl:84-92), Z in
+  // the V buffer: Z1 = X' U0, then U1 = Q P' from the SVD X Z1 = Q Sigma P',
+  // the orthogonal polar factor of X Z1, formed as X Z1 = Qc R (shifted
+  // CholeskyQR3), R = Ur Sigma P' (Jacobi), U1 = Qc (Ur P').
+  void greedy_twofactor(int64_t kn) {
+    const int rr = (int)r;
+    const int rb = rb_for(pass_mode(), rr);
+    const EpiArgs none = de_epi();
+    ensure_packed(false);
+    const int sT = passT(-1, kn, rr, partT);  // Z1 = X' U0
+    with_parts(partT, [&](auto A) {
+      A.rows = m;
+      A.nsplit = sT;
+      A.rb = rb;
+      A.r_out = rr;
+      A.out = V;
+      A.ldo = m;
+      set_pack_outputs(A, true, pass_mode(), rb);
+      rows(A);
+    });
+    const int sN = passN(-1, kn, rr, partN);  // X Z1
+    with_parts(partN, [&](auto A) {
+      A.rows = n;
+      A.nsplit = sN;
+      A.rb = rb;
+      A.r_out = rr;
+      A.out = K;
+      A.ldo = n;
+      A.reduce = 1;
+      A.epi = shifted_chol(n);
+      rows(A);
+    });
+    cholqr_tail(K, K2, n, rr, U2, false, nullptr, 0, none, 0, 3);  // Qc, R = Rtot
+    SvdArgs sv;
+    sv.S = small->Rtot;
+    sv.lds = rr;
+    sv.k = rr;
+    sv.U = small->Ush;
+    sv.V = small->Vsh;
+    sv.sigma = small->sigma;
+    sv.r_out = &small->r_new;
+    sv.tol = 0.0;
+    sv.rmax = rr;
+    sv.work = svd_work;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_jacobi_svd(sv, stream); });
+    timed(LRI_KERNEL_SMALL, [&] { return launch_small_abt(small->Ush, small->Vsh, rr, small->Ctmp, stream); });
+    RowArgs<double> B;  // U1 = Qc (Ur P')
+    B.rows = n;
+    B.r_out = rr;
+    B.r_in = rr;
+    B.X = U2;
+    B.ldx = n;
+    B.Y = small->Ctmp;
+    B.ldy = rr;
+    B.out = U;
+    B.ldo = n;
+    set_pack_outputs(B, false, pass_mode(), rb);
+    rows(B);
   }
 
   // Orthonormal completion of span(X) against an orthonormal B: Q_perp with
@@ -1459,6 +1577,22 @@ int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next) {
   return guarded(ctx, [&] {
     ctx->check_step(k_prev, k_next);
     ctx->bug(k_prev, k_next);
+    return LRI_OK;
+  });
+}
+
+int lri_step_greedy(lri_ctx* ctx, int64_t k_next) {
+  return guarded(ctx, [&] {
+    ctx->check_step(k_next, k_next);
+    ctx->greedy(k_next);
+    return LRI_OK;
+  });
+}
+
+int lri_step_greedy_twofactor(lri_ctx* ctx, int64_t k_next) {
+  return guarded(ctx, [&] {
+    ctx->check_step(k_next, k_next);
+    ctx->greedy_twofactor(k_next);
     return LRI_OK;
   });
 }

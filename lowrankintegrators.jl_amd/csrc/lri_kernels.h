@@ -38,7 +38,7 @@ enum EpiKind {
   EPI_RA_N,    // N = [I; C], Sinit = M S0 N'
   EPI_RA_S,    // Shat = Sinit + C'
   EPI_STORE,   // Ctmp = C
-  EPI_COPY,    // dst = C (a x b, ld a)
+  EPI_COPY,    // dst = C (a x b; transpose: C')
   EPI_CHOL3,   // third factor of shifted CholeskyQR3: C = Gram -> R3^{-1} (into R2inv), Rtot = R3 Rtot; if S: S = Rtot
 };
 
@@ -51,6 +51,7 @@ struct EpiArgs {
   double* S = nullptr;          // state S (EPI_SFINAL / EPI_BUG_* / EPI_RA_N)
   int ld_s = 0;
   double* dst = nullptr;        // EPI_COPY
+  int ld_dst = 0;               // EPI_COPY: leading dimension of dst (0: the rows of what is stored)
   // EPI_CHOL1: when > 0, factor G + s I with s = 11 (rows k + k (k + 1)) u trace(G)
   // (the shift of shifted CholeskyQR3 for an n x k factor with n = shift_rows)
   int64_t shift_rows = 0;
@@ -120,6 +121,8 @@ struct SvdArgs {
   double* work = nullptr;  // global ke x ke scratch, used when it does not fit in LDS
 };
 hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st);
+// C = A B' for k x k matrices (ld k), k <= kMaxAug, one workgroup.
+hipError_t launch_small_abt(const double* A, const double* B, int k, double* C, hipStream_t st);
 // two independent decompositions in one launch (one workgroup each)
 hipError_t launch_jacobi_svd2(SvdArgs P0, SvdArgs P1, hipStream_t st);
 
@@ -128,7 +131,7 @@ enum PassMode { PASS_F64 = 0, PASS_BF16X3 = 1, PASS_F32 = 2 };
 
 struct PassArgs {
   const void* anext = nullptr;  // A[k_next], column-major, ld lda
-  const void* aprev = nullptr;  // A[k_prev]; FP64 passes: nullptr streams A[k_next] itself
+  const void* aprev = nullptr;  // A[k_prev]; nullptr streams A[k_next] itself
   int dtype = 0;                // LRI_F64 / LRI_F32 / LRI_BF16
   int64_t lda = 0;
   int64_t n = 0, m = 0;

@@ -452,7 +452,14 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
       break;
     }
     case EPI_COPY: {
-      for (int e = t; e < a * b; e += blockDim.x) E.dst[e] = C[e];
+      const int ld = E.ld_dst > 0 ? E.ld_dst : (E.transpose ? b : a);
+      for (int e = t; e < a * b; e += blockDim.x) {
+        const int i = e % a, j = e / a;
+        if (E.transpose)
+          E.dst[j + i * ld] = C[e];
+        else
+          E.dst[i + j * ld] = C[e];
+      }
       break;
     }
     case EPI_RA_S: {
@@ -1065,6 +1072,22 @@ __device__ void jacobi_svd_body(const SvdArgs& P, double* lds) {
     if (!any) break;
   }
   jacobi_finish(P, A, k, Vm, ke, sig, kThreads);
+}
+
+__global__ __launch_bounds__(kThreads) void k_small_abt(const double* __restrict__ A, const double* __restrict__ B,
+                                                        int k, double* __restrict__ C) {
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    const int i = e % k, j = e / k;
+    double s = 0.0;
+    for (int l = 0; l < k; ++l) s += A[i + l * k] * B[j + l * k];
+    C[e] = s;
+  }
+}
+
+hipError_t launch_small_abt(const double* A, const double* B, int k, double* C, hipStream_t st) {
+  if (k < 1 || k > kMaxAug) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_small_abt, dim3(1), dim3(kThreads), 0, st, A, B, k, C);
+  return hipGetLastError();
 }
 
 // workgroup 0 decomposes P0, workgroup 1 (if launched) P1

@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_f32(const float* __restr
 #pragma unroll
       for (int t = 0; t < TM; ++t) {
         const int64_t ii = i0 + 32 * t + lr;
-        xa[u][t] = (jok && ii < n) ? an[jj * lda + ii] - ap[jj * lda + ii] : 0.0f;
+        xa[u][t] = (jok && ii < n) ? an[jj * lda + ii] - (ap ? ap[jj * lda + ii] : 0.0f) : 0.0f;
       }
     }
   };
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f32(const float* __restr
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int64_t ii = i + 8 * (e / 4) + 4 * lk + (e % 4);
-        xa[t][e] = (jj < m && ii < ie) ? an[jj * lda + ii] - ap[jj * lda + ii] : 0.0f;
+        xa[t][e] = (jj < m && ii < ie) ? an[jj * lda + ii] - (ap ? ap[jj * lda + ii] : 0.0f) : 0.0f;
       }
     }
 #pragma unroll
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_bf16x3(const uint16_t* _
       const int64_t jj = j0 + 8 * p + lcol;
       const int64_t r0 = i + 8 * loct;
       A1[p] = load_raw8(an, jj * lda + r0, r0, ie, jj < m, v);
-      A0[p] = load_raw8(ap, jj * lda + r0, r0, ie, jj < m, v);
+      A0[p] = load_raw8(ap, jj * lda + r0, r0, ie, jj < m && ap != nullptr, v);  // no ap: A itself
     }
   };
   if (R.b < R.e) load(R.b, xn, xp);
@@ -656,7 +656,7 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_bf16x3(const uint16_t* _
       const int64_t jj = j + col;
       const int64_t r0 = i0 + 8 * oct;
       A1[q] = load_raw8(an, jj * lda + r0, r0, n, jj < je, v);
-      A0[q] = load_raw8(ap, jj * lda + r0, r0, n, jj < je, v);
+      A0[q] = load_raw8(ap, jj * lda + r0, r0, n, jj < je && ap != nullptr, v);  // no ap: A itself
     }
 #pragma unroll
     for (int c = 0; c < CB; ++c) {

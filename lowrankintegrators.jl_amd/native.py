@@ -48,6 +48,8 @@ SIGNATURES = {
     "lri_step_ksl_strang": (_int, [_vp, _i64, _i64, _i64]),
     "lri_step_bug": (_int, [_vp, _i64, _i64]),
     "lri_step_bug_ra": (_int, [_vp, _i64, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
+    "lri_step_greedy": (_int, [_vp, _i64]),
+    "lri_step_greedy_twofactor": (_int, [_vp, _i64]),
     "lri_set_operator_lyapunov": (_int, [_vp, _dp, _dp, _int, _dp, _dp, _int, _i64, _dp, _i64, _dp, _i64]),
     "lri_step_de_ksl": (_int, [_vp, _int, ctypes.c_double, _i64]),
     "lri_step_de_bug": (_int, [_vp, ctypes.c_double, _i64]),
@@ -201,6 +203,12 @@ class Context:
         r = _i64()
         check(lib().lri_step_bug_ra(self.h, int(kp), int(kn), float(tol), int(rmax), ctypes.byref(r)), self.h)
         return int(r.value)
+
+    def step_greedy(self, kn: int):
+        check(lib().lri_step_greedy(self.h, int(kn)), self.h)
+
+    def step_greedy_twofactor(self, kn: int):
+        check(lib().lri_step_greedy_twofactor(self.h, int(kn)), self.h)
 
     # -- matrix ODE
     def set_operator_lyapunov(self, a_diag, a_off, a_periodic, b_diag, b_off, b_periodic, G=None, H=None):

@@ -9,6 +9,7 @@ Reference files followed (paths relative to the reference checkout):
 * projector splitting (KSL) ................ ``src/integrators/projector_splitting.jl:87-211``
 * unconventional (BUG) ..................... ``src/integrators/unconventional.jl:86-164``
 * rank-adaptive BUG ........................ ``src/integrators/rank_adaptive_unconventional.jl:106-233``
+* greedy (data problems) ................... ``src/integrators/greedy_integrator.jl:24-39,84-111``
 
 Each step exists in two modes:
 
@@ -44,7 +45,7 @@ from typing import Any, Callable, List, Sequence, Union
 
 import numpy as np
 
-from .lra import SVDLike, truncate_to_tolerance
+from .lra import SVDLike, TwoFactor, truncate_to_tolerance
 
 # ----------------------------------------------------------------------------
 # problem / algorithm / solution types  (primitives.jl:13-63, *_integrator.jl)
@@ -97,6 +98,11 @@ class UnconventionalAlgorithm:  # unconventional.jl:13-21
 class RankAdaptiveUnconventionalAlgorithm:  # rank_adaptive_unconventional.jl:16-23
     tol: float
     rmax: int = 2 ** 62
+
+
+@dataclass
+class GreedyIntegrator:  # greedy_integrator.jl:16-22 (Z_alg / Z_kwargs: hybrid problems only)
+    pass
 
 
 @dataclass
@@ -267,6 +273,32 @@ def rank_adaptive_bug_step(U, S, V, dA, tol, rmax, mode="efficient"):
     return U1, np.diag(sig[:r1]), V1, r1
 
 
+def greedy_step(U, S, V, X):
+    """``greedy_step!`` for an SVD-like state (greedy_integrator.jl:94-104); X = y(t+dt).
+
+    XV = X*V; XU = X'*U (old bases)      (:97-99)
+    U1 = Q of qr(XV); V1 = Q of qr(XU)   (:100-101)
+    S1 = U1' * (X*V1)                    (:102-103)
+    """
+    XV = X @ V
+    XU = X.T @ U
+    U1, _ = _qr(XV)
+    V1, _ = _qr(XU)
+    S1 = U1.T @ (X @ V1)
+    return U1, S1, V1
+
+
+def greedy_step_twofactor(U, Z, X):
+    """``greedy_step!`` for Y = U*Z' with data (greedy_integrator.jl:84-92); X = y(t+dt).
+
+    Z1 = X'*U (old U)                    (:88)
+    Q, _, P = svd(X*Z1); U1 = Q*P'       (:89-91)
+    """
+    Z1 = X.T @ U
+    Q, _, Pt = np.linalg.svd(X @ Z1, full_matrices=False)
+    return Q @ Pt, Z1
+
+
 # ----------------------------------------------------------------------------
 # the driver  (primitives.jl:68-104)
 # ----------------------------------------------------------------------------
@@ -301,6 +333,25 @@ def solve(prob: MatrixDataProblem, alg, dt=None, mode="efficient", save_everyste
     it = 0
     T = tf - t0
     while (tf - t) / T > 1e-8:
+        if isinstance(alg, GreedyIntegrator):
+            # greedy_integrator.jl:106-111: only X = y(t+dt) enters the step
+            ycur = update_data(prob.y, t, dt)
+            if isinstance(u, TwoFactor):
+                u = TwoFactor(*greedy_step_twofactor(u.U, u.Z, ycur))
+            else:
+                u = SVDLike(*greedy_step(u.U, u.S, u.V, ycur))
+            t = t + dt
+            it += 1
+            if save_everystep or (tf - t) / T <= 1e-8:
+                if it <= len(Ys) - 1:
+                    Ys[it] = u.copy()
+                    ts[it] = t
+                else:
+                    Ys.append(u.copy())
+                    ts.append(t)
+            continue
+        if isinstance(u, TwoFactor):
+            raise TypeError("a TwoFactor state needs the GreedyIntegrator (greedy_integrator.jl:24-30)")
         U, S, V = u.U, u.S, u.V
         if isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang):
             # projector_splitting.jl:205-211: primal(t, dt/2) then dual(t+dt/2, dt/2)

@@ -64,8 +64,15 @@ class TwoFactor:
     U: np.ndarray
     Z: np.ndarray
 
+    @property
+    def rank(self) -> int:
+        return int(self.U.shape[1])
+
     def to_matrix(self) -> np.ndarray:
         return self.U @ self.Z.T
+
+    def copy(self) -> "TwoFactor":
+        return TwoFactor(self.U.copy(), self.Z.copy())
 
 
 def truncate_to_tolerance(sigma: np.ndarray, tol: float) -> int:

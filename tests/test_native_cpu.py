@@ -65,6 +65,12 @@ def test_api_rejects_out_of_scope_problems(lri):
         lri.solve(lri.MatrixDataProblem(lambda t: np.zeros((4, 4)), u0, (0.0, 1.0)), lri.UnconventionalAlgorithm())
     with pytest.raises(AssertionError):
         lri.init(lri.MatrixDataProblem(lambda t: np.zeros((4, 4)), u0, (1.0, 0.0)), lri.UnconventionalAlgorithm(), 0.1)
+    # only the Greedy integrator has two-factor methods (greedy_integrator.jl:24-30)
+    tf0 = lri.TwoFactorRepresentation(np.eye(4, 2), np.ones((4, 2)))
+    with pytest.raises(TypeError):
+        lri.init(lri.MatrixDataProblem(lambda t: np.zeros((4, 4)), tf0, (0.0, 1.0)), lri.UnconventionalAlgorithm(), 0.1)
+    with pytest.raises(NotImplementedError):  # the hybrid problem's Z dynamics are a closure
+        lri.init(lri.MatrixHybridProblem(None, lambda z, u, t: z, tf0, (0.0, 1.0)), lri.GreedyIntegrator(), 0.1)
 
 
 def test_init_sol_slots(lri):

@@ -19,7 +19,7 @@ import pytest
 
 from oracle import dlra
 from oracle.data import rotating_data, random_factors, skew
-from oracle.lra import SVDLike, truncate_to_tolerance, truncated_svd
+from oracle.lra import SVDLike, TwoFactor, truncate_to_tolerance, truncated_svd
 from oracle.metrics import frob_diff, frob_norm, orth_defect, proj_dist, rel_err
 
 
@@ -114,6 +114,46 @@ def test_bug_closed_form(seed):
     Vb = orth_svd(u.V @ u.S.T + dA.T @ u.U, u.rank)
     Yb = Ub @ Ub.T @ (Y0 + dA) @ Vb @ Vb.T
     assert np.linalg.norm(U1 @ S1 @ V1.T - Yb) / np.linalg.norm(Yb) < 1e-13
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_greedy_closed_form(seed):
+    """greedy_integrator.jl:94-104: Y1 = P_U1 X P_V1 with U1 spanning X V0 and V1 spanning X' U0."""
+    u, dA = case(seed)
+    X = u.to_matrix() + dA
+    U1, S1, V1 = dlra.greedy_step(u.U, u.S, u.V, X)
+    Ub = orth_svd(X @ u.V, u.rank)
+    Vb = orth_svd(X.T @ u.U, u.rank)
+    Yb = Ub @ Ub.T @ X @ Vb @ Vb.T
+    assert np.linalg.norm(U1 @ S1 @ V1.T - Yb) / np.linalg.norm(Yb) < 1e-13
+    assert orth_defect(U1) < 1e-14 and orth_defect(V1) < 1e-14
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_greedy_twofactor_is_polar_factor(seed):
+    """greedy_integrator.jl:84-92: Z1 = X' U0 and U1 the orthogonal polar factor of
+    M = X Z1, i.e. U1'U1 = I and U1'M symmetric positive definite (which fixes U1
+    for full-rank M; Q P vs Q P' or a missing transpose breaks the symmetry)."""
+    u, dA = case(seed)
+    X = u.to_matrix() + dA
+    U1, Z1 = dlra.greedy_step_twofactor(u.U, u.U @ u.S, X)
+    assert np.allclose(Z1, X.T @ u.U, rtol=0, atol=1e-14)
+    M = X @ Z1
+    H = U1.T @ M
+    assert orth_defect(U1) < 1e-14
+    assert np.linalg.norm(H - H.T) / np.linalg.norm(H) < 1e-13
+    assert np.linalg.eigvalsh(0.5 * (H + H.T)).min() > 0
+    assert np.linalg.norm(U1 @ H - M) / np.linalg.norm(M) < 1e-13
+
+
+def test_greedy_exact_on_rank_r_data():
+    """Rank-r data with generic bases: range(X V0) = range(X), so one SVD-like step gives X."""
+    rng = np.random.default_rng(12)
+    n, m, r = 40, 30, 4
+    X = rng.standard_normal((n, r)) @ rng.standard_normal((r, m))
+    u = random_factors(rng, n, m, r)
+    U1, S1, V1 = dlra.greedy_step(u.U, u.S, u.V, X)
+    assert np.linalg.norm(U1 @ S1 @ V1.T - X) / np.linalg.norm(X) < 1e-13
 
 
 @pytest.mark.parametrize("seed", [0, 1])
@@ -247,8 +287,13 @@ def test_reference_consistency_function_vs_snapshots():
     dprob = dlra.MatrixDataProblem(data, X0)
     for alg in (dlra.UnconventionalAlgorithm(), dlra.ProjectorSplitting(dlra.PrimalLieTrotter()),
                 dlra.ProjectorSplitting(dlra.DualLieTrotter()),
-                dlra.RankAdaptiveUnconventionalAlgorithm(1e-8, rmax=20)):
+                dlra.RankAdaptiveUnconventionalAlgorithm(1e-8, rmax=20), dlra.GreedyIntegrator()):
         a = dlra.solve(prob, alg, 1e-2).Y[-1].to_matrix()
         b = dlra.solve(dprob, alg).Y[-1].to_matrix()
         # Julia's elementwise isapprox, rtol = sqrt(eps), atol = 0
         assert np.all(np.abs(a - b) <= np.sqrt(np.finfo(float).eps) * np.maximum(np.abs(a), np.abs(b)))
+    # the two-factor Greedy state (greedy_integrator.jl:24-30,84-92)
+    tf0 = TwoFactor(X0.U.copy(), X0.V @ X0.S.T)
+    a = dlra.solve(dlra.MatrixDataProblem(Y, tf0, (0.0, 1.0)), dlra.GreedyIntegrator(), 1e-2).Y[-1].to_matrix()
+    b = dlra.solve(dlra.MatrixDataProblem(data, tf0), dlra.GreedyIntegrator()).Y[-1].to_matrix()
+    assert np.all(np.abs(a - b) <= np.sqrt(np.finfo(float).eps) * np.maximum(np.abs(a), np.abs(b)))
