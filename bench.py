@@ -185,6 +185,119 @@ def cpu_baseline(n, r, steps=3):
     }
 
 
+# ---------------------------------------------------------------------------- Greedy workload
+
+
+def greedy_cpu_baseline(n, r, steps=2):
+    """Oracle Greedy step (3 products with the FP64 snapshot) on a bounded sample."""
+    from oracle import dlra
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    rng = np.random.default_rng(0)
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    S = np.diag(2.0 ** -np.arange(r))
+    ring = [np.asfortranarray(1e-3 * rng.standard_normal((n, n))) for _ in range(2)]
+    U, S, V = dlra.greedy_step(U, S, V, ring[0])  # warm-up
+    t0 = time.perf_counter()
+    for i in range(steps):
+        U, S, V = dlra.greedy_step(U, S, V, ring[(i + 1) % 2])
+    el = time.perf_counter() - t0
+    return {
+        "value": steps / el,
+        "unit": "steps/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"{steps} Greedy steps (+1 warm-up) of the FP64 NumPy oracle (greedy_integrator.jl:94-104: "
+                  f"X V, X' U, two Householder QRs, X V1, U1' (X V1)) at n=m={n}, r={r} on 2 resident FP64 "
+                  f"snapshots; OpenBLAS threads={threads}",
+    }
+
+
+def run_greedy(args, torch, lri, snaps, U0, S0, V0, n, r):
+    """Greedy integrator (SVD-like state) over the BF16 snapshots: 3 single-snapshot passes per step."""
+    N = len(snaps)
+    ctx = lri.native.Context(n, n, r, 0)
+    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, n, owners=snaps)
+    ctx.set_precision(lri.native.PREC_MIXED)
+    ctx.set_state(U0, S0, V0)
+
+    def sweep(nsteps):
+        for i in range(nsteps):
+            ctx.step_greedy(1 + i % (N - 1))
+
+    log(f"Greedy: warm-up {args.warmup} steps")
+    sweep(args.warmup)
+    ctx.set_state(U0, S0, V0)
+    ctx.timing(0 if args.no_events else 2)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sweep(args.steps)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = args.steps / el
+    fam = {}
+    for name, f in (("pass_N", lri.native.KERNEL_PASS_N), ("pass_T", lri.native.KERNEL_PASS_T)):
+        ms, cnt = ctx.timing_read(f)
+        fam[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt if cnt else None}
+    ctx.timing(0)
+    log(f"Greedy mixed: {value:.1f} steps/s ({1e3 * el / args.steps:.3f} ms/step)")
+    timed_fam = {k: v for k, v in fam.items() if v["avg_ms"]}
+    dom = max(timed_fam, key=lambda k: timed_fam[k]["avg_ms"]) if timed_fam else None
+    bytes_per_launch = 1.0 * n * n * 2  # one BF16 snapshot streamed once
+    achieved = bytes_per_launch / (fam[dom]["avg_ms"] * 1e-3) / 1e9 if dom else None
+    out = {
+        "metric": "steps/s of the Greedy integrator (SVD-like state) at n=m=16384, r=32, BF16 snapshots",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"GreedyIntegrator on a MatrixDataProblem, n=m={n}, r={r}, {N} BF16 snapshots "
+                               f"resident in HBM (X = A[k] itself), BF16x3 matrix-core products with FP32 "
+                               f"accumulation, FP64 shifted CholeskyQR3", "n": n, "m": n, "r": r, "snapshots": N,
+                   "snapshot_dtype": "bf16", "precision": "mixed", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None, "kernel": dom,
+                     "bytes_per_launch": bytes_per_launch},
+        "kernels": fam,
+    }
+    if not args.no_check:
+        ctx.set_state(U0, S0, V0)
+        sweep(N - 1)
+        Ym = ctx.get_state()
+        ctx.set_precision(lri.native.PREC_F64)
+        ctx.set_state(U0, S0, V0)
+        t0 = time.perf_counter()
+        sweep(N - 1)
+        Yd = ctx.get_state()
+        out["fp64_products_on_bf16_steps_per_s"] = (N - 1) / (time.perf_counter() - t0)
+        from oracle.metrics import rel_err
+
+        out["rel_err_vs_fp64"] = rel_err(*Ym, *Yd)
+        log(f"Greedy rel err mixed vs FP64 after {N - 1} steps: {out['rel_err_vs_fp64']:.3e}")
+    ctx.close()
+    if not args.no_cpu:
+        log("CPU baseline (oracle Greedy) on a bounded sample")
+        out["cpu_baseline"] = greedy_cpu_baseline(n, r)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 # ---------------------------------------------------------------------------- matrix-ODE workload
 
 
@@ -316,8 +429,9 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the error check")
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP-event kernel timing in the timed region (for external profilers)")
-    ap.add_argument("--workload", choices=["ksl", "de"], default="ksl",
-                    help="ksl: the headline data integrator; de: the BUG matrix-ODE step at n=m=4096")
+    ap.add_argument("--workload", choices=["ksl", "de", "greedy"], default="ksl",
+                    help="ksl: the headline data integrator; de: the BUG matrix-ODE step at n=m=4096; "
+                         "greedy: the Greedy data integrator on the headline snapshots")
     ap.add_argument("--de-n", type=int, default=4096)
     ap.add_argument("--de-steps", type=int, default=20)
     ap.add_argument("--warmup-de", type=int, default=2)
@@ -365,6 +479,10 @@ def main():
     gen_s = time.perf_counter() - tg
     log(f"generation {gen_s:.1f} s; randomized SVD of A_0")
     U0, S0, V0 = randomized_svd(torch, snaps[0], r, args.seed)
+
+    if args.workload == "greedy":
+        run_greedy(args, torch, lri, snaps, U0, S0, V0, n, r)
+        return
 
     ctx = lri.native.Context(n, m, r, 0)
     ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, n, owners=snaps)
