@@ -176,6 +176,23 @@ int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* 
                               const double* b_diag, const double* b_off, int b_periodic, int64_t q,
                               const double* G, int64_t ldg, const double* H, int64_t ldh);
 
+/* Adds the quadratic term -(D Y) o Y (o: elementwise product) to the operator
+ * set by lri_set_operator_lyapunov (which clears it):
+ *
+ *     F(Y) = A Y + Y B' + G H' - (D Y) o Y,   D = tridiag(dl, dd, du) on the n rows,
+ *
+ * dl[i] multiplying Y[i-1, :], du[i] Y[i+1, :] (n entries each; with
+ * `periodic` row 0 couples to row n-1 through dl[0] and row n-1 to row 0
+ * through du[n-1], else those two are unused).  Viscous Burgers with one
+ * parameter sample per column (test/data_agnostic_approximation.jl:16-33,
+ * Delta*rho - (grad*rho) .* rho): A = nu Laplacian, D = central gradient.
+ * The K/L/S sub-problems (projector_splitting.jl:52-80, unconventional.jl:51-79,
+ * rank_adaptive_unconventional.jl:59-86) then carry the tensors
+ * T = V x V x V and W = (D U) x U x U and are integrated by RK4 stage by stage;
+ * rank <= LRI_QUAD_MAX_RANK (the rank-adaptive step: 2 r <= LRI_QUAD_MAX_RANK). */
+#define LRI_QUAD_MAX_RANK 32
+int lri_set_operator_quadratic(lri_ctx* ctx, const double* dl, const double* dd, const double* du, int periodic);
+
 /* One KSL step of the matrix ODE over [t, t+dt]: order LRI_PRIMAL
  * (projector_splitting.jl:129-152,191-196), LRI_DUAL (:166-189,198-203) or
  * LRI_STRANG (primal over dt/2, then dual over dt/2, :205-211; each half with

@@ -15,6 +15,9 @@ from .api import (  # noqa: F401
     GreedyIntegrator,
     LyapunovOperator,
     Matrix,
+    QuadraticOperator,
+    Tridiagonal,
+    burgers_operator,
     MatrixDataProblem,
     MatrixDEProblem,
     MatrixHybridProblem,

@@ -208,6 +208,76 @@ class LyapunovOperator:
         return self.A.gershgorin() + self.B.gershgorin()
 
 
+class Tridiagonal:
+    """General tridiagonal n x n matrix acting on rows: row i is
+    ``dl[i] Y[i-1] + dd[i] Y[i] + du[i] Y[i+1]`` (n entries each); with ``periodic``
+    row 0 takes ``dl[0] Y[n-1]`` and row n-1 ``du[n-1] Y[0]``, else those are unused."""
+
+    def __init__(self, dl, dd, du, periodic: bool = False):
+        self.dl, self.dd, self.du = (np.asarray(x, dtype=np.float64).reshape(-1).copy() for x in (dl, dd, du))
+        self.periodic = bool(periodic)
+        n = self.dd.shape[0]
+        if self.dl.shape[0] != n or self.du.shape[0] != n:
+            raise ValueError("dl, dd and du need n entries each")
+        if self.periodic and n < 3:
+            raise ValueError("a periodic tridiagonal needs n >= 3")
+
+    @property
+    def n(self) -> int:
+        return self.dd.shape[0]
+
+    def dense(self) -> np.ndarray:
+        n = self.n
+        D = np.diag(self.dd)
+        D[np.arange(1, n), np.arange(n - 1)] = self.dl[1:]
+        D[np.arange(n - 1), np.arange(1, n)] = self.du[: n - 1]
+        if self.periodic:
+            D[0, n - 1] += self.dl[0]
+            D[n - 1, 0] += self.du[n - 1]
+        return D
+
+    def row_bound(self) -> float:
+        """max_i sum_j |D_ij| (infinity norm)."""
+        lo = np.abs(self.dl).copy()
+        hi = np.abs(self.du).copy()
+        if not self.periodic:
+            lo[0] = 0.0
+            hi[-1] = 0.0
+        return float(np.max(lo + np.abs(self.dd) + hi))
+
+
+class QuadraticOperator(LyapunovOperator):
+    """F(Y, t) = A Y + Y B' + G H' - (D Y) o Y (o: elementwise), D a ``Tridiagonal`` on the
+    n rows.  Viscous Burgers with one parameter sample per column
+    (test/data_agnostic_approximation.jl:16-33) is ``burgers_operator``."""
+
+    def __init__(self, A: SymTridiagonal, B: SymTridiagonal, D: Tridiagonal, G=None, H=None):
+        super().__init__(A, B, G, H)
+        if D.n != A.n:
+            raise ValueError(f"D is {D.n} x {D.n}, A is {A.n} x {A.n}")
+        self.D = D
+
+    def __call__(self, Y, t=0.0):
+        return super().__call__(Y, t) - (self.D.dense() @ Y) * Y
+
+    def spectral_bound(self, ymax: float = 0.0) -> float:
+        """Bound of the Jacobian's spectral radius near states with max |Y_ij| <= ymax:
+        rho(A) + rho(B) + 2 ||D||_inf ymax."""
+        return super().spectral_bound() + 2.0 * self.D.row_bound() * ymax
+
+
+def burgers_operator(n: int, m: int, nu: float, length: float = math.pi) -> QuadraticOperator:
+    """Delta rho - (grad rho) .* rho on a periodic grid of n cells of [0, length)
+    (test/data_agnostic_approximation.jl:4-33): Delta = nu/dx^2 tridiag(1, -2, 1),
+    grad = 1/(2 dx) tridiag(-1, 0, 1), both periodic; m columns (samples)."""
+    dx = length / n
+    A = SymTridiagonal(-2.0 * nu / dx ** 2 * np.ones(n), nu / dx ** 2 * np.ones(n), periodic=True)
+    B = SymTridiagonal(np.zeros(m), np.zeros(m - 1))
+    g = 0.5 / dx
+    D = Tridiagonal(-g * np.ones(n), np.zeros(n), g * np.ones(n), periodic=True)
+    return QuadraticOperator(A, B, D)
+
+
 def rk4_substeps(rho: float, dt: float, cap: float = 2.5) -> int:
     """Fewest RK4 steps with h * rho <= cap (RK4's real stability interval is [-2.78, 0])."""
     return max(1, int(math.ceil(abs(dt) * rho / cap)))
@@ -429,7 +499,8 @@ def _init_de(prob, alg, dt, *, save_everystep, device, substeps) -> DLRIntegrato
     F = prob.f
     if not isinstance(F, LyapunovOperator):
         raise NotImplementedError("a Python callable cannot run on the device: give the right-hand side as a "
-                                  "LyapunovOperator (A Y + Y B' + G H')")
+                                  "LyapunovOperator (A Y + Y B' + G H') or a QuadraticOperator "
+                                  "(... - (D Y) o Y)")
     if not isinstance(alg, (ProjectorSplitting, UnconventionalAlgorithm, RankAdaptiveUnconventionalAlgorithm)):
         raise TypeError(f"unsupported algorithm {alg!r}")
     t0, tf = prob.tspan
@@ -453,10 +524,23 @@ def _init_de(prob, alg, dt, *, save_everystep, device, substeps) -> DLRIntegrato
     q = F.G.shape[1]
     ctx.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic,
                               F.G if q else None, F.H if q else None)
+    quad = isinstance(F, QuadraticOperator)
+    if quad:
+        # the rank-adaptive S sub-step runs in the augmented 2 r bases
+        kmax = 2 * rcap if isinstance(alg, RankAdaptiveUnconventionalAlgorithm) else r
+        if kmax > native.QUAD_MAX_RANK:
+            raise ValueError(f"the quadratic operator needs rank <= {native.QUAD_MAX_RANK} "
+                             f"(rank-adaptive: 2 rmax <= {native.QUAD_MAX_RANK})")
+        ctx.set_operator_quadratic(F.D.dl, F.D.dd, F.D.du, F.D.periodic)
     sol = init_sol(dt, t0, tf, u0)
     sol.Y[0] = u0.copy()
     integ = DLRIntegrator(prob, alg, dt, ctx, u0.copy(), sol, snapshot_dtype=None, save_everystep=save_everystep)
     integ.substeps = int(substeps) if substeps is not None else None
+    if quad and integ.substeps is None:
+        # the Jacobian of -(D Y) o Y scales with max |Y|: bounded from the factors of Y0
+        # (|Y_ij| <= ||(U S)_i|| ||V_j||), with a factor 2 of headroom for growth
+        ymax = float(np.max(np.linalg.norm(u0.U @ u0.S, axis=1)) * np.max(np.linalg.norm(u0.V, axis=1)))
+        integ.quad_ymax = 2.0 * ymax
     return integ
 
 
@@ -465,7 +549,9 @@ def _step_de(integrator, alg, dt) -> None:
     N = integrator.substeps
     if N is None:
         half = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang)
-        N = rk4_substeps(integrator.prob.f.spectral_bound(), dt / 2 if half else dt)
+        F = integrator.prob.f
+        rho = F.spectral_bound(integrator.quad_ymax) if isinstance(F, QuadraticOperator) else F.spectral_bound()
+        N = rk4_substeps(rho, dt / 2 if half else dt)
     if isinstance(alg, ProjectorSplitting):
         order = {PrimalLieTrotter: native.PRIMAL, DualLieTrotter: native.DUAL, Strang: native.STRANG}[type(alg.order)]
         ctx.step_de_ksl(order, dt, N)

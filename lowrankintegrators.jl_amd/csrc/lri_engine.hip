@@ -132,6 +132,12 @@ struct lri_ctx {
   // S sub-step's ones, which are the next BUG step's K/L ones. Any other entry
   // point clears the flag.
   bool de_gal_fresh = false;
+  // quadratic term -(D Y) o Y (D tridiagonal on the n rows), its tensors
+  // T = V x V x V and W = (D U) x U x U, and their scratch
+  bool op_quad = false;
+  double *op_dl = nullptr, *op_dd = nullptr, *op_du = nullptr;
+  int op_dper = 0;
+  double *quad_part = nullptr, *quad_T = nullptr, *quad_W = nullptr, *quad_work = nullptr;
 
   // ------------------------------------------------------------------ helpers
   int pass_mode() const {
@@ -1103,7 +1109,214 @@ struct lri_ctx {
     return e;
   }
 
+  // ---------------------------------------------------- quadratic operator
+  // T = V x V x V (m-side basis X, no stencil) or W = (D U) x U x U (n side)
+  void quad_tensor(const double* X, bool nside, int k, double* out) {
+    TensorArgs Q;
+    Q.rows = nside ? n : m;
+    Q.k = k;
+    Q.Y = X;
+    Q.ldy = Q.rows;
+    if (nside) {
+      Q.dl = op_dl;
+      Q.dd = op_dd;
+      Q.du = op_du;
+      Q.dper = op_dper;
+    }
+    Q.part = quad_part;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_tensor3(Q, out, stream); });
+  }
+  // Galerkin matrices, forcing projections and the tensor of one basis
+  void quad_galerkin(const double* X, bool nside, int k) {
+    if (nside) {
+      de_galerkin(X, true, k, de->Ga, de->gaT);
+      quad_tensor(X, true, k, quad_W);
+    } else {
+      de_galerkin(X, false, k, de->Gb, de->hbT);
+      quad_tensor(X, false, k, quad_T);
+    }
+  }
+  // X0 = B0 op(C) (rows x k) into X, the initial value of a K or L sub-problem
+  void quad_init(const double* B0, int64_t rows_, int k, const double* C, int ldc, int transC, double* X) {
+    RowArgs<double> A;
+    A.rows = rows_;
+    A.r_out = k;
+    A.r_in = k;
+    A.X = B0;
+    A.ldx = rows_;
+    A.Y = C;
+    A.ldy = ldc;
+    A.transY = transC;
+    A.out = X;
+    A.ldo = rows_;
+    rows(A);
+  }
+  // N RK4 steps of the K (nside: M = V'BV, forcing G (H'V), tensor T, D on K) or
+  // L sub-problem (M = U'AU, forcing H (G'U), tensor W, no stencil) on X in place;
+  // scr holds 2 k columns, acc k columns (ld rows).
+  void quad_rows(bool nside, int k, double* X, double* scr, double* acc, double h, int64_t nsub) {
+    const int64_t rows_ = nside ? n : m;
+    QuadStageArgs Q;
+    Q.n = rows_;
+    Q.k = k;
+    Q.ld = rows_;
+    Q.X0 = X;
+    Q.Acc = acc;
+    Q.h = h;
+    Q.st_d = nside ? op_ad : op_bd;
+    Q.st_e = nside ? op_ae : op_be;
+    Q.periodic = nside ? op_aper : op_bper;
+    Q.M = nside ? de->Gb : de->Ga;
+    Q.g = nside ? op_G : op_H;
+    Q.ldg = rows_;
+    Q.q = op_q;
+    Q.hv = nside ? de->hbT : de->gaT;
+    Q.T = nside ? quad_T : quad_W;
+    if (nside) {
+      Q.dl = op_dl;
+      Q.dd = op_dd;
+      Q.du = op_du;
+      Q.dper = op_dper;
+    }
+    double* xa = scr;
+    double* xb = scr + rows_ * k;
+    const double* in[4] = {X, xa, xb, xa};
+    double* outp[4] = {xa, xb, xa, nullptr};
+    for (int64_t st = 0; st < nsub; ++st)
+      for (int s = 0; s < 4; ++s) {
+        Q.stage = s;
+        Q.Xs = in[s];
+        Q.Xn = outp[s];
+        timed(LRI_KERNEL_DE, [&] { return launch_quad_stage(Q, stream); });
+      }
+  }
+  // Gram of X (rows x k) and the shifted first CholeskyQR factor
+  void quad_gram(const double* X, int64_t rows_, int k, EpiArgs epi) {
+    RowArgs<double> A;
+    A.rows = rows_;
+    A.r_out = k;
+    A.base = X;
+    A.ld_base = rows_;
+    A.reduce = 1;
+    A.epi = epi;
+    rows(A);
+  }
+  void quad_sstep(int k, int l, const double* S0, int lds0, int transS0, double* S1, int lds1, double sgn, double h,
+                  int64_t nsub) {
+    QuadSArgs Q;
+    Q.k = k;
+    Q.l = l;
+    Q.q = op_q;
+    Q.S0 = S0;
+    Q.lds0 = lds0;
+    Q.transS0 = transS0;
+    Q.S1 = S1;
+    Q.lds1 = lds1;
+    Q.Ga = de->Ga;
+    Q.Gb = de->Gb;
+    Q.gaT = de->gaT;
+    Q.hbT = de->hbT;
+    Q.W = quad_W;
+    Q.T = quad_T;
+    Q.sgn = sgn;
+    Q.h = h;
+    Q.nsub = nsub;
+    Q.work = quad_work;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_quad_sstep(Q, stream); });
+  }
+
+  void quad_ksl_primal(double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    const EpiArgs none = de_epi();
+    quad_galerkin(V, false, rr);  // V0'BV0, H'V0, T(V0)
+    quad_init(U, n, rr, S, (int)rcap, 0, K);
+    quad_rows(true, rr, K, K2, P, h, nsub);
+    quad_gram(K, n, rr, shifted_chol(n));
+    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 0, 3);  // U1, R
+    quad_galerkin(U, true, rr);                                    // U1'AU1, G'U1, W(U1)
+    quad_sstep(rr, rr, small->Rtot, rr, 0, de->Snew, rr, -1.0, h, nsub);  // S~
+    quad_init(V, m, rr, de->Snew, rr, 1, L);
+    quad_rows(false, rr, L, L2, Q, h, nsub);
+    quad_gram(L, m, rr, shifted_chol(m));
+    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 2, 3);  // V1, S = R'
+  }
+
+  void quad_ksl_dual(double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    const EpiArgs none = de_epi();
+    quad_galerkin(U, true, rr);  // U0'AU0, G'U0, W(U0)
+    quad_init(V, m, rr, S, (int)rcap, 1, L);
+    quad_rows(false, rr, L, L2, Q, h, nsub);
+    quad_gram(L, m, rr, shifted_chol(m));
+    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 0, 3);  // V1, R
+    quad_galerkin(V, false, rr);                                  // V1'BV1, H'V1, T(V1)
+    quad_sstep(rr, rr, small->Rtot, rr, 1, de->Snew, rr, -1.0, h, nsub);  // S~ from R'
+    quad_init(U, n, rr, de->Snew, rr, 0, K);
+    quad_rows(true, rr, K, K2, P, h, nsub);
+    quad_gram(K, n, rr, shifted_chol(n));
+    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 1, 3);  // U1, S = R
+  }
+
+  void quad_bug(double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    if (!de_gal_fresh) {
+      quad_galerkin(V, false, rr);
+      quad_galerkin(U, true, rr);
+    }
+    de_gal_fresh = false;
+    quad_init(U, n, rr, S, (int)rcap, 0, K);
+    quad_rows(true, rr, K, K2, P, h, nsub);
+    quad_init(V, m, rr, S, (int)rcap, 1, L);
+    quad_rows(false, rr, L, L2, Q, h, nsub);
+    quad_gram(K, n, rr, shifted_chol(n));
+    cholqr_tail(K, K2, n, rr, U2, false, U, rr, de_epi(EPI_BUG_M), 0, 3);  // U1, M = U1'U0
+    quad_gram(L, m, rr, shifted_chol(m));
+    EpiArgs eN = de_epi(EPI_BUG_N);  // N = V1'V0, Sinit = M S0 N'
+    eN.S = S;
+    eN.ld_s = (int)rcap;
+    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0, 3);
+    quad_galerkin(U2, true, rr);
+    quad_galerkin(V2, false, rr);
+    quad_sstep(rr, rr, small->Sinit, rr, 0, S, (int)rcap, 1.0, h, nsub);
+    std::swap(U, U2);
+    std::swap(V, V2);
+    de_gal_fresh = true;  // the Galerkin data and tensors now belong to U, V
+  }
+
+  int64_t quad_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax) {
+    const int rr = (int)r;
+    const int k2 = 2 * rr;
+    const double h = dt / (double)nsub;
+    quad_galerkin(V, false, rr);
+    quad_galerkin(U, true, rr);
+    quad_init(U, n, rr, S, (int)rcap, 0, K);
+    quad_rows(true, rr, K, K2, P, h, nsub);
+    quad_init(V, m, rr, S, (int)rcap, 1, L);
+    quad_rows(false, rr, L, L2, Q, h, nsub);
+    HIPCHECK(hipMemcpyAsync(Uhat, U, sizeof(double) * n * rr, hipMemcpyDeviceToDevice, stream));
+    EpiArgs eM;
+    eM.kind = EPI_RA_M;
+    complete_basis(K, U, n, rr, K2, K2 + n * rcap, Uhat + n * rr, eM);
+    HIPCHECK(hipMemcpyAsync(Vhat, V, sizeof(double) * m * rr, hipMemcpyDeviceToDevice, stream));
+    EpiArgs eN;
+    eN.kind = EPI_RA_N;  // Sinit = M S0 N' (2r x 2r)
+    eN.S = S;
+    eN.ld_s = (int)rcap;
+    complete_basis(L, V, m, rr, L2, L2 + m * rcap, Vhat + m * rr, eN);
+    quad_galerkin(Uhat, true, k2);
+    quad_galerkin(Vhat, false, k2);
+    quad_sstep(k2, k2, small->Sinit, k2, 0, small->Shat, k2, 1.0, h, nsub);
+    return ra_finish(k2, tol, rmax);
+  }
+
   void de_ksl_primal(double dt, int64_t nsub) {
+    if (op_quad) {
+      quad_ksl_primal(dt, nsub);
+      return;
+    }
     const int rr = (int)r;
     const double h = dt / (double)nsub;
     const EpiArgs none = de_epi();
@@ -1120,6 +1333,10 @@ struct lri_ctx {
 
   // dual_LT_step! with the DE sub-problems (projector_splitting.jl:52-80,166-189)
   void de_ksl_dual(double dt, int64_t nsub) {
+    if (op_quad) {
+      quad_ksl_dual(dt, nsub);
+      return;
+    }
     const int rr = (int)r;
     const double h = dt / (double)nsub;
     const EpiArgs none = de_epi();
@@ -1144,6 +1361,10 @@ struct lri_ctx {
   }
 
   void de_bug(double dt, int64_t nsub) {
+    if (op_quad) {
+      quad_bug(dt, nsub);
+      return;
+    }
     const int rr = (int)r;
     const double h = dt / (double)nsub;
     if (!de_gal_fresh) {
@@ -1177,6 +1398,7 @@ struct lri_ctx {
   // rankadaptive_unconventional_step! with the DE sub-problems
   // (rank_adaptive_unconventional.jl:59-86,194-233)
   int64_t de_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax) {
+    if (op_quad) return quad_bug_ra(dt, nsub, tol, rmax);
     const int rr = (int)r;
     const int k2 = 2 * rr;
     const double h = dt / (double)nsub;
@@ -1262,6 +1484,7 @@ struct lri_ctx {
     REQUIRE(op_set, "no operator (call lri_set_operator_lyapunov)");
     REQUIRE(dt > 0.0 && std::isfinite(dt), "dt must be positive and finite");
     REQUIRE(nsub >= 1, "substeps must be >= 1");
+    REQUIRE(!op_quad || r <= kQuadMaxRank, "the quadratic operator needs rank <= LRI_QUAD_MAX_RANK");
   }
 
   int read_flags() {
@@ -1310,7 +1533,8 @@ void free_ctx(lri_ctx* c) {
                   c->L,      c->L2,     c->Q,      c->Uhat,    c->Vhat,    c->pkN_f64, c->pkT_f64, c->pkN_f32,
                   c->pkT_f32, c->pkN_hi, c->pkN_lo, c->pkT_hi,  c->pkT_lo,  c->partN,  c->partT,  c->gpart,
                   c->flags,  c->small,  c->svd_work, c->op_ad,  c->op_ae,   c->op_bd,   c->op_be,  c->op_G,
-                  c->op_H,   c->de,     c->specQ[0], c->specLam[0], c->specF[0]};
+                  c->op_H,   c->de,     c->specQ[0], c->specLam[0], c->specF[0], c->op_dl, c->op_dd, c->op_du,
+                  c->quad_part, c->quad_T, c->quad_W, c->quad_work};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->specF[1]) (void)hipFree(c->specF[1]);
@@ -1665,7 +1889,38 @@ int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* 
     ctx->op_bper = b_periodic != 0;
     ctx->op_q = (int)q;
     ctx->setup_spectral(a_diag, a_off, a_periodic != 0, b_diag, b_off, b_periodic != 0);
+    ctx->op_quad = false;
     ctx->op_set = true;
+    return LRI_OK;
+  });
+}
+
+int lri_set_operator_quadratic(lri_ctx* ctx, const double* dl, const double* dd, const double* du, int periodic) {
+  return guarded(ctx, [&] {
+    REQUIRE(ctx->op_set, "set the linear part first (lri_set_operator_lyapunov)");
+    REQUIRE(dl && dd && du, "dl, dd and du are required (n entries each)");
+    REQUIRE(!periodic || ctx->n >= 3, "a periodic operator needs n >= 3");
+    const size_t d = sizeof(double);
+    const int64_t n = ctx->n;
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    auto alloc = [&](double** p, size_t count) {
+      if (!*p) HIPCHECK(hipMalloc((void**)p, d * count));
+      HIPCHECK(hipMemset(*p, 0, d * count));
+    };
+    const size_t k3 = (size_t)kQuadMaxRank * kQuadMaxRank * kQuadMaxRank;
+    const int64_t rows = std::max(ctx->n, ctx->m);
+    alloc(&ctx->op_dl, n);
+    alloc(&ctx->op_dd, n);
+    alloc(&ctx->op_du, n);
+    alloc(&ctx->quad_part, (size_t)((rows + 63) / 64) * k3);
+    alloc(&ctx->quad_T, k3);
+    alloc(&ctx->quad_W, k3);
+    alloc(&ctx->quad_work, 2 * k3);
+    HIPCHECK(hipMemcpy(ctx->op_dl, dl, d * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(ctx->op_dd, dd, d * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(ctx->op_du, du, d * n, hipMemcpyHostToDevice));
+    ctx->op_dper = periodic != 0;
+    ctx->op_quad = true;
     return LRI_OK;
   });
 }
@@ -1701,6 +1956,8 @@ int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, in
     ctx->check_de(dt, substeps);
     REQUIRE(tol >= 0.0, "tol must be >= 0");
     REQUIRE(rmax >= 1, "rmax must be >= 1");
+    REQUIRE(!ctx->op_quad || 2 * ctx->r <= kQuadMaxRank,
+            "the rank-adaptive step with the quadratic operator needs 2 r <= LRI_QUAD_MAX_RANK");
     int64_t r1 = 0;
     {
       DeScope scope(ctx);

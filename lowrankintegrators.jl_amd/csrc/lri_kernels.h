@@ -206,6 +206,71 @@ struct SStepArgs {
 };
 hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st);
 
+// Quadratic right-hand sides F(Y) = A Y + Y B' + G H' - (D Y) o Y (see lri_quad.hip).
+constexpr int kQuadMaxRank = 32;  // columns of a sub-problem with the quadratic term
+
+// part[g][c + k (a + k b)] = sum over the 64-row chunk g of X_ra Y_rb Y_rc,
+// X = D Y (tridiagonal dl/dd/du, periodic dper) when dd != nullptr, else Y;
+// launch_tensor3 then sums the chunks in order into out (k^3).
+struct TensorArgs {
+  int64_t rows = 0;
+  int k = 0;
+  const double* Y = nullptr;
+  int64_t ldy = 0;
+  const double *dl = nullptr, *dd = nullptr, *du = nullptr;
+  int dper = 0;
+  double* part = nullptr;  // ceil(rows / 64) * k^3
+};
+hipError_t launch_tensor3(const TensorArgs& Q, double* out, hipStream_t st);
+
+// One RK4 stage (s = 0..3) of X' = A X + X M + g hv - N(X), N_ic = sum_ab (P X)_ia X_ib T_abc
+// (P = D when dd != nullptr, else I), over all n rows; X0 holds the state of the
+// RK4 step and is advanced in place by stage 3.
+struct QuadStageArgs {
+  int64_t n = 0;
+  int k = 0;
+  int64_t ld = 0;  // of X0, Xs, Xn, Acc
+  double* X0 = nullptr;
+  const double* Xs = nullptr;
+  double* Xn = nullptr;
+  double* Acc = nullptr;
+  int stage = 0;
+  double h = 0.0;
+  const double* st_d = nullptr;  // A: symmetric tridiagonal (nullptr: none)
+  const double* st_e = nullptr;
+  int periodic = 0;
+  const double* M = nullptr;     // k x k (nullptr: none)
+  const double* g = nullptr;     // n x q
+  int64_t ldg = 0;
+  int q = 0;
+  const double* hv = nullptr;    // q x k
+  const double* T = nullptr;     // k^3 (nullptr: no quadratic term)
+  const double *dl = nullptr, *dd = nullptr, *du = nullptr;
+  int dper = 0;
+};
+hipError_t launch_quad_stage(const QuadStageArgs& Q, hipStream_t st);
+
+// N RK4 steps of S' = sgn (Ga S + S Gb + (gaT' hbT) - P(S)) (k x l), one workgroup.
+struct QuadSArgs {
+  int k = 0, l = 0, q = 0;
+  const double* S0 = nullptr;
+  int lds0 = 0;
+  int transS0 = 0;
+  double* S1 = nullptr;
+  int lds1 = 0;
+  const double* Ga = nullptr;   // k x k
+  const double* Gb = nullptr;   // l x l
+  const double* gaT = nullptr;  // q x k
+  const double* hbT = nullptr;  // q x l
+  const double* W = nullptr;    // k^3, W[c + k (e + k f)] (nullptr: no quadratic term)
+  const double* T = nullptr;    // l^3
+  double sgn = 1.0;
+  double h = 0.0;
+  int64_t nsub = 0;
+  double* work = nullptr;       // k l k + l l k doubles
+};
+hipError_t launch_quad_sstep(const QuadSArgs& Q, hipStream_t st);
+
 // Analytic eigenbasis of a constant symmetric tridiagonal d I + e (S + S')
 // (Dirichlet, or periodic with n >= 3): Q n x n column-major, lam n.
 hipError_t launch_eig_tridiag_const(double* Q, double* lam, int64_t n, double d, double e, int periodic,

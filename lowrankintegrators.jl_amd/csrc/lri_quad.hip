@@ -1,0 +1,279 @@
+// Matrix-ODE sub-steps for right-hand sides with a quadratic term,
+//
+//     F(Y) = A Y + Y B' + G H' - (D Y) o Y          (o: elementwise product)
+//
+// A, B symmetric tridiagonal and D a general tridiagonal acting on the rows
+// (e.g. viscous Burgers: A = nu Laplacian, D = central gradient, both
+// periodic; each column one parameter sample).  With U, V orthonormal and
+// frozen during a sub-step, the reference's default right-hand sides
+// (projector_splitting.jl:52-80, unconventional.jl:51-79,
+// rank_adaptive_unconventional.jl:59-86) are
+//
+//     K' = A K + K (V'BV) + G (H'V) - N,  N_ic = sum_ab (D K)_ia K_ib T_abc,
+//     L' = B L + L (U'AU) + H (G'U) - Q,  Q_jc = sum_ab L_ja L_jb W_abc,
+//     S' = sgn ((U'AU) S + S (V'BV) + (U'G)(H'V)' - P),
+//          P_cd = sum_{e f a b} W_efc S_ea S_fb T_abd,
+//
+// with the third-order tensors T_abc = sum_j V_ja V_jb V_jc and
+// W_abc = sum_i (D U)_ia U_ib U_ic.  The sub-problems are no longer linear, so
+// every one of the N classical RK4 steps evaluates them: the K and L steps one
+// launch per RK4 stage over all rows (a stage needs the neighbouring rows of
+// the previous one), the S step in one workgroup.
+#include <algorithm>
+
+#include "lri_common.h"
+#include "lri_kernels.h"
+
+namespace lri {
+
+namespace {
+
+// RK4 stage weights: y_{s+1} input = y0 + c_s h k_s, accumulation weight w_s
+__device__ __forceinline__ double stage_c(int s) { return s == 2 ? 1.0 : 0.5; }
+__device__ __forceinline__ double stage_w(int s) { return (s == 1 || s == 2) ? 2.0 : 1.0; }
+
+// row i of X (n x k, ld n), with periodic wrap or zero outside
+__device__ __forceinline__ int64_t wrap_row(int64_t i, int64_t n, int periodic, bool* ok) {
+  if (i < 0) {
+    *ok = periodic != 0;
+    return i + n;
+  }
+  if (i >= n) {
+    *ok = periodic != 0;
+    return i - n;
+  }
+  *ok = true;
+  return i;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Partial third-order tensors: workgroup g sums the rows of chunk g,
+//   part[g][c + k (a + k b)] = sum_{r in chunk} X_ra Y_rb Y_rc,
+// X = D Y (the tridiagonal D applied to Y) when Q.dd != nullptr, else X = Y.
+// ----------------------------------------------------------------------------
+
+constexpr int kTensorRows = 64;
+
+__global__ __launch_bounds__(256) void k_tensor3_part(TensorArgs Q) {
+  __shared__ double xs[kTensorRows * kQuadMaxRank];
+  __shared__ double ys[kTensorRows * kQuadMaxRank];
+  const int k = Q.k, t = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * kTensorRows;
+  for (int e = t; e < kTensorRows * k; e += blockDim.x) {
+    const int lr = e % kTensorRows, c = e / kTensorRows;
+    const int64_t i = r0 + lr;
+    double y = 0.0, x = 0.0;
+    if (i < Q.rows) {
+      y = Q.Y[i + c * Q.ldy];
+      if (Q.dd) {
+        bool okl, oku;
+        const int64_t il = wrap_row(i - 1, Q.rows, Q.dper, &okl);
+        const int64_t iu = wrap_row(i + 1, Q.rows, Q.dper, &oku);
+        x = Q.dd[i] * y;
+        if (okl) x += Q.dl[i] * Q.Y[il + c * Q.ldy];
+        if (oku) x += Q.du[i] * Q.Y[iu + c * Q.ldy];
+      } else {
+        x = y;
+      }
+    }
+    xs[lr * k + c] = x;
+    ys[lr * k + c] = y;
+  }
+  __syncthreads();
+  const int k3 = k * k * k;
+  double* out = Q.part + (int64_t)blockIdx.x * k3;
+  for (int e = t; e < k3; e += blockDim.x) {
+    const int c = e % k, a = (e / k) % k, b = e / (k * k);
+    double s = 0.0;
+    for (int lr = 0; lr < kTensorRows; ++lr) s += xs[lr * k + a] * ys[lr * k + b] * ys[lr * k + c];
+    out[e] = s;
+  }
+}
+
+// out[e] = sum over chunks, in chunk order
+__global__ __launch_bounds__(256) void k_tensor3_sum(const double* __restrict__ part, int nchunks, int k3,
+                                                     double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k3) return;
+  double s = 0.0;
+  for (int g = 0; g < nchunks; ++g) s += part[(int64_t)g * k3 + e];
+  out[e] = s;
+}
+
+hipError_t launch_tensor3(const TensorArgs& Q, double* out, hipStream_t st) {
+  if (Q.k < 1 || Q.k > kQuadMaxRank || Q.rows < 1 || !Q.part) return hipErrorInvalidValue;
+  const int nchunks = (int)((Q.rows + kTensorRows - 1) / kTensorRows);
+  hipLaunchKernelGGL(k_tensor3_part, dim3(nchunks), dim3(256), 0, st, Q);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int k3 = Q.k * Q.k * Q.k;
+  hipLaunchKernelGGL(k_tensor3_sum, dim3((k3 + 255) / 256), dim3(256), 0, st, (const double*)Q.part, nchunks, k3,
+                     out);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// One RK4 stage of the K or L sub-problem over all rows.  Workgroup: R = 256/k
+// rows x k columns, thread (row lr = t % R, column c = t / R).  The stage's
+// rows (with one halo row on each side) are staged in LDS, then
+//   f_ic = (A Xs)_ic + (Xs M)_ic + (g hv)_ic - sum_ab (P Xs)_ia Xs_ib T_abc
+// (P = D when Q.dd, else the identity) and the RK4 bookkeeping of stage s:
+//   s < 3: Acc (=, +=) w_s f, Xn = X0 + c_s h f;   s = 3: X0 += h/6 (Acc + f).
+// ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_quad_stage(QuadStageArgs Q) {
+  __shared__ double xs[256 + 2 * kQuadMaxRank];  // (R + 2) x k rows of the stage input
+  __shared__ double ps[256];                     // R x k rows of P Xs
+  const int k = Q.k, t = threadIdx.x;
+  const int R = 256 / k;
+  const int64_t r0 = (int64_t)blockIdx.x * R;
+  const int64_t n = Q.n;
+  // rows r0 - 1 .. r0 + R into xs[lr * k + c]; a row outside [0, n) takes its
+  // periodic image (used only by a periodic stencil)
+  for (int e = t; e < (R + 2) * k; e += blockDim.x) {
+    const int lr = e % (R + 2), c = e / (R + 2);
+    int64_t ir = r0 + lr - 1;
+    if (ir < 0) ir += n;
+    else if (ir >= n) ir -= n;
+    xs[lr * k + c] = (ir >= 0 && ir < n) ? Q.Xs[ir + c * Q.ld] : 0.0;
+  }
+  __syncthreads();
+  const int lr = t % R, c = t / R;
+  const int64_t i = r0 + lr;
+  const bool act = c < k && lr < R && i < n;
+  if (act) {
+    const double xm = xs[lr * k + c], x0 = xs[(lr + 1) * k + c], xp = xs[(lr + 2) * k + c];
+    const bool hasl = i > 0 || Q.dper, hasu = i < n - 1 || Q.dper;
+    ps[lr * k + c] = Q.dd ? (Q.dd[i] * x0 + (hasl ? Q.dl[i] * xm : 0.0) + (hasu ? Q.du[i] * xp : 0.0)) : x0;
+  }
+  __syncthreads();
+  if (!act) return;
+  const double* xrow = xs + (lr + 1) * k;
+  const double xm = xs[lr * k + c], x0 = xrow[c], xp = xs[(lr + 2) * k + c];
+  double f = 0.0;
+  // A Xs: symmetric tridiagonal, e[i] couples rows i and i + 1 (e[n-1]: n-1 and 0 when periodic)
+  if (Q.st_d) {
+    f = Q.st_d[i] * x0;
+    if (i > 0) f += Q.st_e[i - 1] * xm;
+    else if (Q.periodic) f += Q.st_e[n - 1] * xm;
+    if (i < n - 1) f += Q.st_e[i] * xp;
+    else if (Q.periodic) f += Q.st_e[n - 1] * xp;
+  }
+  if (Q.M)
+    for (int l = 0; l < k; ++l) f += xrow[l] * Q.M[l + c * k];
+  for (int qq = 0; qq < Q.q; ++qq) f += Q.g[i + qq * Q.ldg] * Q.hv[qq + c * Q.q];
+  if (Q.T) {
+    const double* prow = ps + lr * k;
+    double nl = 0.0;
+    for (int a = 0; a < k; ++a) {
+      double inner = 0.0;
+      const double* Ta = Q.T + (int64_t)k * a;  // T[c + k (a + k b)]
+      for (int b = 0; b < k; ++b) inner += xrow[b] * Ta[c + (int64_t)k * k * b];
+      nl += prow[a] * inner;
+    }
+    f -= nl;
+  }
+  const int64_t o = i + c * Q.ld;
+  const int s = Q.stage;
+  if (s < 3) {
+    Q.Acc[o] = (s == 0 ? 0.0 : Q.Acc[o]) + stage_w(s) * f;
+    Q.Xn[o] = Q.X0[o] + stage_c(s) * Q.h * f;
+  } else {
+    Q.X0[o] = Q.X0[o] + (Q.h / 6.0) * (Q.Acc[o] + f);
+  }
+}
+
+hipError_t launch_quad_stage(const QuadStageArgs& Q, hipStream_t st) {
+  if (Q.k < 1 || Q.k > kQuadMaxRank || Q.n < 1 || Q.stage < 0 || Q.stage > 3 || Q.q < 0 || Q.q > kMaxForce)
+    return hipErrorInvalidValue;
+  const int R = 256 / Q.k;
+  const int64_t blocks = (Q.n + R - 1) / R;
+  hipLaunchKernelGGL(k_quad_stage, dim3((unsigned)blocks), dim3(256), 0, st, Q);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// S sub-step: N RK4 steps of
+//   S' = sgn (Ga S + S Gb + cb - P(S)),  P_cd = sum_{a b} Z_abc T_abd,
+//   Z_abc = sum_e S_ea Y_ebc,  Y_ebc = sum_f S_fb W_efc,
+// S k x l in LDS, Y (k x l x k) and Z (l x l x k) in global scratch; one workgroup.
+// ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_quad_sstep(QuadSArgs Q) {
+  __shared__ double y0[kQuadMaxRank * kQuadMaxRank], ys[kQuadMaxRank * kQuadMaxRank];
+  __shared__ double acc[kQuadMaxRank * kQuadMaxRank], cb[kQuadMaxRank * kQuadMaxRank];
+  const int k = Q.k, l = Q.l, t = threadIdx.x, nt = blockDim.x;
+  const int kl = k * l;
+  for (int e = t; e < kl; e += nt) {
+    const int i = e % k, j = e / k;
+    y0[e] = Q.transS0 ? Q.S0[j + i * Q.lds0] : Q.S0[i + j * Q.lds0];
+    double s = 0.0;
+    for (int qq = 0; qq < Q.q; ++qq) s += Q.gaT[qq + i * Q.q] * Q.hbT[qq + j * Q.q];
+    cb[e] = s;
+  }
+  __syncthreads();
+  double* Y = Q.work;          // k x l x k: Y[c + k (e + k b)]
+  double* Z = Y + k * l * k;   // l x l x k: Z[c + k (a + l b)]
+  const double h = Q.h;
+  for (int64_t step = 0; step < Q.nsub; ++step) {
+    for (int e = t; e < kl; e += nt) ys[e] = y0[e];
+    __syncthreads();
+    for (int s = 0; s < 4; ++s) {
+      if (Q.W) {
+        for (int e = t; e < k * l * k; e += nt) {  // Y_ebc = sum_f S_fb W_efc
+          const int c = e % k, ee = (e / k) % k, b = e / (k * k);
+          double v = 0.0;
+          for (int f = 0; f < k; ++f) v += ys[f + b * k] * Q.W[c + k * (ee + k * f)];
+          Y[e] = v;
+        }
+        __syncthreads();
+        for (int e = t; e < l * l * k; e += nt) {  // Z_abc = sum_e S_ea Y_ebc
+          const int c = e % k, a = (e / k) % l, b = e / (k * l);
+          double v = 0.0;
+          for (int ee = 0; ee < k; ++ee) v += ys[ee + a * k] * Y[c + k * (ee + k * b)];
+          Z[e] = v;
+        }
+        __syncthreads();
+      }
+      double fv[(kQuadMaxRank * kQuadMaxRank + 255) / 256];
+      int cnt = 0;
+      for (int e = t; e < kl; e += nt, ++cnt) {
+        const int i = e % k, j = e / k;
+        double v = cb[e];
+        for (int m = 0; m < k; ++m) v += Q.Ga[i + m * k] * ys[m + j * k];
+        for (int m = 0; m < l; ++m) v += ys[i + m * k] * Q.Gb[m + j * l];
+        if (Q.W) {
+          double p = 0.0;  // P_ij = sum_ab Z_abi T_abj, T[j + l (a + l b)]
+          for (int b = 0; b < l; ++b)
+            for (int a = 0; a < l; ++a) p += Z[i + k * (a + l * b)] * Q.T[j + l * (a + l * b)];
+          v -= p;
+        }
+        fv[cnt] = Q.sgn * v;
+      }
+      __syncthreads();  // every stage input read before it is replaced
+      cnt = 0;
+      for (int e = t; e < kl; e += nt, ++cnt) {
+        const double f = fv[cnt];
+        if (s < 3) {
+          acc[e] = (s == 0 ? 0.0 : acc[e]) + stage_w(s) * f;
+          ys[e] = y0[e] + stage_c(s) * h * f;
+        } else {
+          y0[e] = y0[e] + (h / 6.0) * (acc[e] + f);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = t; e < kl; e += nt) Q.S1[(e % k) + (e / k) * Q.lds1] = y0[e];
+}
+
+hipError_t launch_quad_sstep(const QuadSArgs& Q, hipStream_t st) {
+  if (Q.k < 1 || Q.l < 1 || Q.k > kQuadMaxRank || Q.l > kQuadMaxRank || Q.q < 0 || Q.q > kMaxForce || !Q.work)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_quad_sstep, dim3(1), dim3(256), 0, st, Q);
+  return hipGetLastError();
+}
+
+}  // namespace lri

@@ -22,6 +22,7 @@ KERNEL_PASS_N, KERNEL_PASS_T, KERNEL_SMALL, KERNEL_DE = 0, 1, 2, 3
 MAX_RANK = 64
 MAX_FORCE = 4
 DE_MAX_ROWS = 16384
+QUAD_MAX_RANK = 32
 
 DTYPES = {"f64": F64, "f32": F32, "bf16": BF16}
 
@@ -51,6 +52,7 @@ SIGNATURES = {
     "lri_step_greedy": (_int, [_vp, _i64]),
     "lri_step_greedy_twofactor": (_int, [_vp, _i64]),
     "lri_set_operator_lyapunov": (_int, [_vp, _dp, _dp, _int, _dp, _dp, _int, _i64, _dp, _i64, _dp, _i64]),
+    "lri_set_operator_quadratic": (_int, [_vp, _dp, _dp, _dp, _int]),
     "lri_step_de_ksl": (_int, [_vp, _int, ctypes.c_double, _i64]),
     "lri_step_de_bug": (_int, [_vp, ctypes.c_double, _i64]),
     "lri_step_de_bug_ra": (_int, [_vp, ctypes.c_double, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
@@ -241,6 +243,17 @@ class Context:
             q = Gf.shape[1]
         check(lib().lri_set_operator_lyapunov(self.h, _dptr(ad), _dptr(ao), int(bool(a_periodic)), _dptr(bd), _dptr(bo),
                                               int(bool(b_periodic)), int(q), _dptr(Gf), self.n, _dptr(Hf), self.m),
+              self.h)
+
+    def set_operator_quadratic(self, dl, dd, du, periodic):
+        """Adds -(D Y) o Y, D = tridiag(dl, dd, du) on the n rows (n entries each)."""
+        vs = []
+        for x, name in ((dl, "dl"), (dd, "dd"), (du, "du")):
+            x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+            if x.shape[0] != self.n:
+                raise ValueError(f"{name} must have {self.n} entries, got {x.shape[0]}")
+            vs.append(x)
+        check(lib().lri_set_operator_quadratic(self.h, _dptr(vs[0]), _dptr(vs[1]), _dptr(vs[2]), int(bool(periodic))),
               self.h)
 
     def step_de_ksl(self, order: int, dt: float, substeps: int):

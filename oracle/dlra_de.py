@@ -31,6 +31,8 @@ data path when F(Y, t) = dA/dt is constant in Y.
 """
 from __future__ import annotations
 
+import math
+
 from dataclasses import dataclass, field
 from typing import Any, Callable, List
 
@@ -61,6 +63,59 @@ class LyapunovOperator:
 
     def __call__(self, Y, t=0.0):
         return self.A @ Y + Y @ self.B.T + self.g @ self.h.T
+
+
+@dataclass
+class QuadraticOperator:
+    """F(Y, t) = A Y + Y B' + g h' - (D Y) o Y (o: elementwise), dense A, B, D.
+
+    With A = nu * Laplacian and D = central gradient (both periodic) and no B,
+    g, h this is the viscous Burgers right-hand side of the reference's
+    ``test/data_agnostic_approximation.jl:31-33`` (Delta*rho - (grad*rho) .* rho),
+    each column one parameter sample."""
+
+    A: np.ndarray
+    B: np.ndarray
+    g: np.ndarray  # n x q
+    h: np.ndarray  # m x q
+    D: np.ndarray
+
+    def __call__(self, Y, t=0.0):
+        return self.A @ Y + Y @ self.B.T + self.g @ self.h.T - (self.D @ Y) * Y
+
+
+def burgers_matrices(n: int, nu: float, length: float = math.pi):
+    """Delta (times nu) and the central gradient on a periodic grid of n cells,
+    entry by entry as ``test/data_agnostic_approximation.jl:6-29`` builds them."""
+    dx = length / n
+    left = lambda i: i - 1 if i > 0 else n - 1  # noqa: E731
+    right = lambda i: i + 1 if i < n - 1 else 0  # noqa: E731
+    Lap = np.zeros((n, n))
+    Grad = np.zeros((n, n))
+    for i in range(n):
+        Lap[i, left(i)] = nu / dx ** 2
+        Lap[i, i] = -2 * nu / dx ** 2
+        Lap[i, right(i)] = nu / dx ** 2
+    for i in range(n):
+        Grad[i, left(i)] = -1 / 2 / dx
+        Grad[i, right(i)] = 1 / 2 / dx
+    return Lap, Grad
+
+
+def burgers_initial(n: int, m: int, sigma=(0.5, 0.5), length: float = math.pi):
+    """rho0_mat of ``test/data_agnostic_approximation.jl:35-44``: the deterministic
+    profile plus sigma_1 xi_1 sin(2 pi x) + sigma_2 xi_2 sin(3 pi x) over an m x m
+    grid of (xi_1, xi_2) in [-1, 1]^2 (columns in Julia's column-major order of the
+    comprehension), n x m^2."""
+    dx = length / n
+    x = dx / 2 + dx * np.arange(n)
+    ub = 0.5 * (np.exp(np.cos(x)) - 1.5) * np.sin(x + 2 * math.pi * 0.37)
+    xs = np.linspace(-1.0, 1.0, m)
+    cols = []
+    for x2 in xs:  # the comprehension's first index (xi_1) runs fastest
+        for x1 in xs:
+            cols.append(ub + sigma[0] * x1 * np.sin(2 * math.pi * x) + sigma[1] * x2 * np.sin(3 * math.pi * x))
+    return np.stack(cols, axis=1)
 
 
 def dirichlet_laplacian(n: int, scale: float | None = None) -> np.ndarray:

@@ -138,3 +138,72 @@ def test_lowrank_evaluation_matches_dense():
     a = dlra_de.bug_de_step_lowrank(op, u.U, u.S, u.V, 0.0, 0.05, 4)
     b = dlra_de.bug_de_step(op.dense(), u.U, u.S, u.V, 0.0, 0.05, 4)
     assert rel_err(*a, *b) < 1e-13
+
+
+# --------------------------------------------------------------- quadratic (Burgers) operator
+
+
+def test_burgers_matrices_are_the_reference_stencils():
+    """data_agnostic_approximation.jl:6-29 entry by entry, checked against the closed
+    forms they discretise: for rho = sin(2x) (period pi), Delta rho -> -4 nu sin(2x)
+    and grad rho -> 2 cos(2x) with O(dx^2) error; rows sum to zero (periodic)."""
+    n, nu = 1000, 0.005
+    Lap, Grad = dlra_de.burgers_matrices(n, nu)
+    dx = np.pi / n
+    x = dx / 2 + dx * np.arange(n)
+    rho = np.sin(2 * x)
+    assert np.abs(Lap @ rho + 4 * nu * rho).max() < 4 * nu * 4 * dx ** 2
+    assert np.abs(Grad @ rho - 2 * np.cos(2 * x)).max() < 8 * dx ** 2
+    assert np.abs(Lap.sum(axis=1)).max() < 1e-9 and np.abs(Grad.sum(axis=1)).max() < 1e-9
+    assert np.allclose(Grad, -Grad.T) and np.allclose(Lap, Lap.T)
+    # the reference's Burgers right-hand side
+    Y = np.stack([rho, np.cos(x)], 1)
+    F = dlra_de.QuadraticOperator(Lap, np.zeros((2, 2)), np.zeros((n, 1)), np.zeros((2, 1)), Grad)
+    assert np.allclose(F(Y), Lap @ Y - (Grad @ Y) * Y, rtol=0, atol=1e-12)
+
+
+def test_burgers_initial_condition():
+    """rho0_mat (data_agnostic_approximation.jl:35-44): column (i1, i2) of the m x m
+    grid, xi_1 fastest, is ub(x) + 0.5 xi_1 sin(2 pi x) + 0.5 xi_2 sin(3 pi x)."""
+    n, m = 50, 4
+    R = dlra_de.burgers_initial(n, m)
+    assert R.shape == (n, m * m)
+    dx = np.pi / n
+    x = dx / 2 + dx * np.arange(n)
+    xi = np.linspace(-1, 1, m)
+    ub = 0.5 * (np.exp(np.cos(x)) - 1.5) * np.sin(x + 2 * np.pi * 0.37)
+    col = 1 + m * 2  # xi_1 = xi[1], xi_2 = xi[2]
+    assert np.allclose(R[:, col], ub + 0.5 * xi[1] * np.sin(2 * np.pi * x) + 0.5 * xi[2] * np.sin(3 * np.pi * x))
+
+
+def _tensor(X, Y):
+    """T_abc = sum_r X_ra Y_rb Y_rc."""
+    return np.einsum("ra,rb,rc->abc", X, Y, Y)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_quadratic_subproblem_rhs_tensor_forms(periodic):
+    """The factored K/L/S right-hand sides of the quadratic operator (the form the
+    engine evaluates) against the dense projections F(K V') V, F(U L')' U,
+    U' F(U S V') V: N_ic = sum_ab (D K)_ia K_ib T_abc with T = V x V x V,
+    Q_jc = sum_ab L_ja L_jb W_abc and P_cd = sum W_efc S_ea S_fb T_abd with
+    W = (D U) x U x U."""
+    rng = np.random.default_rng(3)
+    n, m, r, q = 30, 22, 4, 2
+    A = rng.standard_normal((n, n)); A = A + A.T
+    B = rng.standard_normal((m, m)); B = B + B.T
+    D = np.diag(rng.standard_normal(n)) + np.diag(rng.standard_normal(n - 1), 1) + np.diag(rng.standard_normal(n - 1), -1)
+    if periodic:
+        D[0, n - 1], D[n - 1, 0] = rng.standard_normal(2)
+    G, H = rng.standard_normal((n, q)), rng.standard_normal((m, q))
+    F = dlra_de.QuadraticOperator(A, B, G, H, D)
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    K, L, S = rng.standard_normal((n, r)), rng.standard_normal((m, r)), rng.standard_normal((r, r))
+    T, W = _tensor(V, V), _tensor(D @ U, U)
+    Kf = A @ K + K @ (V.T @ B @ V) + G @ (H.T @ V) - np.einsum("ia,ib,abc->ic", D @ K, K, T)
+    assert np.allclose(Kf, F(K @ V.T) @ V, rtol=0, atol=1e-11)
+    Lf = B @ L + L @ (U.T @ A @ U) + H @ (G.T @ U) - np.einsum("ja,jb,abc->jc", L, L, W)
+    assert np.allclose(Lf, F(U @ L.T).T @ U, rtol=0, atol=1e-11)
+    Sf = (U.T @ A @ U) @ S + S @ (V.T @ B @ V) + (U.T @ G) @ (H.T @ V) - np.einsum("efc,ea,fb,abd->cd", W, S, S, T)
+    assert np.allclose(Sf, U.T @ F(U @ S @ V.T) @ V, rtol=0, atol=1e-11)
