@@ -285,7 +285,7 @@ def run_greedy(args, torch, lri, snaps, U0, S0, V0, n, r):
         sweep(N - 1)
         Yd = ctx.get_state()
         out["fp64_products_on_bf16_steps_per_s"] = (N - 1) / (time.perf_counter() - t0)
-        from oracle.metrics import rel_err
+        rel_err = lri.metrics.rel_err
 
         out["rel_err_vs_fp64"] = rel_err(*Ym, *Yd)
         log(f"Greedy rel err mixed vs FP64 after {N - 1} steps: {out['rel_err_vs_fp64']:.3e}")
@@ -329,6 +329,90 @@ def de_cpu_baseline(n, r, dt, N, seed):
                   f"sub-step size) and scaled by {Ns}/{N}; the RK4 sub-steps are >99% of the oracle's step time; "
                   f"OpenBLAS threads={threads}",
     }
+
+
+def run_burgers(args, torch, lri):
+    """The Burgers problem of the reference's DE test and notebook
+    (test/data_agnostic_approximation.jl:4-58, examples/burgers_uncertainty.ipynb): BUG,
+    n = 1000 cells, m = 20^2 samples, rank 15, dt = 1e-2, 100 steps on [0, 1], FP64."""
+    n, msq, r, nu, dt = 1000, 20, 15, 0.005, 1e-2
+    m = msq * msq
+    steps = 100
+    U0, s0, V0t = np.linalg.svd(lri.burgers_samples(n, msq), full_matrices=False)
+    U0, S0, V0 = U0[:, :r].copy(), np.diag(s0[:r]), V0t[:r].T.copy()
+    F = lri.burgers_operator(n, m, nu)
+    ymax = float(np.abs(U0 @ S0 @ V0.T).max())
+    N = lri.rk4_substeps(F.spectral_bound(2 * ymax), dt)
+    ctx = lri.native.Context(n, m, r, 0)
+    ctx.set_operator_lyapunov(F.A.d, F.A.e, True, F.B.d, F.B.e, False)
+    ctx.set_operator_quadratic(F.D.dl, F.D.dd, F.D.du, True)
+    log(f"Burgers BUG n={n} m={m} r={r} dt={dt}: {N} RK4 sub-steps per sub-problem")
+    for _ in range(args.warmup_de):
+        ctx.set_state(U0, S0, V0)
+        for _ in range(5):
+            ctx.step_de_bug(dt, N)
+    ctx.set_state(U0, S0, V0)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.step_de_bug(dt, N)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    U1, S1, V1 = ctx.get_state()
+    ctx.close()
+    log(f"Burgers BUG: {steps / el:.1f} steps/s, 100-step solve {el:.3f} s")
+    out = {
+        "metric": "steps/s of BUG on the reference's Burgers problem (n=1000, m=400, r=15, dt=1e-2), FP64",
+        "value": steps / el,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup_de,
+        "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"unconventional (BUG) integrator on the viscous Burgers MatrixDEProblem of "
+                               f"test/data_agnostic_approximation.jl (periodic, nu={nu}, n={n} cells of [0, pi), "
+                               f"m={msq}^2 samples) at rank {r}, dt={dt}, t in [0, 1], {N} RK4 sub-steps per "
+                               f"sub-problem", "n": n, "m": m, "r": r, "dt": dt, "substeps": N,
+                   "parallelism": "single GPU"},
+        "solve_s": el,
+        "notebook_reference_s": {"value": 20.197, "note": "examples/burgers_uncertainty.ipynb:181: the same "
+                                 "problem solved by the reference (adaptive Tsit5 sub-steps, Julia 1.6.3, first "
+                                 "call including compilation, hardware unspecified); not the same sub-step method"},
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        import scipy.sparse as sps
+        from oracle import dlra_de
+
+        Lap, Grad = dlra_de.burgers_matrices(n, nu)
+        Fh = dlra_de.QuadraticOperator(sps.csr_matrix(Lap), sps.csr_matrix((m, m)), np.zeros((n, 1)),
+                                       np.zeros((m, 1)), sps.csr_matrix(Grad))
+        ns = 3
+        t0 = time.perf_counter()
+        U, S, V = U0, S0, V0
+        for k in range(ns):
+            U, S, V = dlra_de.bug_de_step(Fh, U, S, V, k * dt, dt, N)
+        cel = time.perf_counter() - t0
+        try:
+            from threadpoolctl import threadpool_info
+
+            threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+        except Exception:
+            threads = os.cpu_count() or 1
+        out["cpu_baseline"] = {"value": ns / cel, "unit": "steps/s", "cores": int(threads), "kind": "port",
+                               "sample": f"{ns} BUG steps of the FP64 NumPy/SciPy oracle (dense right-hand side "
+                                         f"with sparse stencils, the same {N} RK4 sub-steps); OpenBLAS "
+                                         f"threads={threads}"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
 
 
 def de_initial(n, r, seed):
@@ -429,9 +513,10 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the error check")
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP-event kernel timing in the timed region (for external profilers)")
-    ap.add_argument("--workload", choices=["ksl", "de", "greedy"], default="ksl",
+    ap.add_argument("--workload", choices=["ksl", "de", "greedy", "burgers"], default="ksl",
                     help="ksl: the headline data integrator; de: the BUG matrix-ODE step at n=m=4096; "
-                         "greedy: the Greedy data integrator on the headline snapshots")
+                         "greedy: the Greedy data integrator on the headline snapshots; burgers: BUG on the "
+                         "reference's Burgers problem")
     ap.add_argument("--de-n", type=int, default=4096)
     ap.add_argument("--de-steps", type=int, default=20)
     ap.add_argument("--warmup-de", type=int, default=2)
@@ -455,6 +540,9 @@ def main():
         raise SystemExit("bench.py needs a GPU")
     if args.workload == "de":
         run_de(args, torch, lri)
+        return
+    if args.workload == "burgers":
+        run_burgers(args, torch, lri)
         return
     dev = torch.device("cuda:0")
     n = m = args.n
@@ -558,7 +646,7 @@ def main():
         run_steps(ctx, lri, npairs, npairs)
         Yd = ctx.get_state()
         f64_on_bf16 = npairs / (time.perf_counter() - t0)
-        from oracle.metrics import rel_err
+        rel_err = lri.metrics.rel_err
 
         out["rel_err_vs_fp64"] = rel_err(*Ym, *Yd)
         out["fp64_products_on_bf16_steps_per_s"] = f64_on_bf16
@@ -594,7 +682,7 @@ def main():
                 c2.set_state(U0, S0, V0)
                 run_steps(c2, lri, R - 1, R - 1)
                 Yd = c2.get_state()
-                from oracle.metrics import rel_err
+                rel_err = lri.metrics.rel_err
 
                 out["fp32_rel_err_vs_fp64"] = rel_err(*Yf, *Yd)
             c2.close()

@@ -7,7 +7,7 @@ tridiagonal Lyapunov-type right-hand side, run in ``liblri_hip.so``
 is the host side: a ctypes binding (``native``) and a mirror of the
 reference's Julia API (``api``).
 """
-from . import native  # noqa: F401
+from . import metrics, native  # noqa: F401
 from .api import (  # noqa: F401
     DLRIntegrator,
     DLRSolution,
@@ -18,6 +18,7 @@ from .api import (  # noqa: F401
     QuadraticOperator,
     Tridiagonal,
     burgers_operator,
+    burgers_samples,
     MatrixDataProblem,
     MatrixDEProblem,
     MatrixHybridProblem,

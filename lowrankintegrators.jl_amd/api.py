@@ -278,6 +278,22 @@ def burgers_operator(n: int, m: int, nu: float, length: float = math.pi) -> Quad
     return QuadraticOperator(A, B, D)
 
 
+def burgers_samples(n: int, msq: int, sigma=(0.5, 0.5), length: float = math.pi) -> np.ndarray:
+    """The initial samples of the reference's Burgers problem
+    (test/data_agnostic_approximation.jl:35-44), n x msq^2: the profile
+    0.5 (exp(cos x) - 1.5) sin(x + 0.74 pi) on the cell centres of [0, length) plus
+    sigma_1 xi_1 sin(2 pi x) + sigma_2 xi_2 sin(3 pi x) for (xi_1, xi_2) on an
+    msq x msq grid of [-1, 1]^2, xi_1 varying fastest."""
+    dx = length / n
+    x = dx / 2 + dx * np.arange(n)
+    ub = 0.5 * (np.exp(np.cos(x)) - 1.5) * np.sin(x + 2 * math.pi * 0.37)
+    xi = np.linspace(-1.0, 1.0, msq)
+    x1 = np.tile(xi, msq)
+    x2 = np.repeat(xi, msq)
+    return (ub[:, None] + sigma[0] * np.sin(2 * math.pi * x)[:, None] * x1[None, :]
+            + sigma[1] * np.sin(3 * math.pi * x)[:, None] * x2[None, :])
+
+
 def rk4_substeps(rho: float, dt: float, cap: float = 2.5) -> int:
     """Fewest RK4 steps with h * rho <= cap (RK4's real stability interval is [-2.78, 0])."""
     return max(1, int(math.ceil(abs(dt) * rho / cap)))

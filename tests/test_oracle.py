@@ -297,3 +297,17 @@ def test_reference_consistency_function_vs_snapshots():
     a = dlra.solve(dlra.MatrixDataProblem(Y, tf0, (0.0, 1.0)), dlra.GreedyIntegrator(), 1e-2).Y[-1].to_matrix()
     b = dlra.solve(dlra.MatrixDataProblem(data, tf0), dlra.GreedyIntegrator()).Y[-1].to_matrix()
     assert np.all(np.abs(a - b) <= np.sqrt(np.finfo(float).eps) * np.maximum(np.abs(a), np.abs(b)))
+
+
+def test_package_rel_err_matches_the_oracle_metric(lri):
+    """lowrankintegrators.jl_amd.metrics.rel_err (bench.py's error check) against the
+    oracle's factored metric and the dense difference, including ~1e-13 differences."""
+    rng = np.random.default_rng(21)
+    u = random_factors(rng, 40, 30, 4)
+    U2 = u.U + 1e-13 * rng.standard_normal(u.U.shape)
+    S2 = u.S + 1e-2 * np.diag(rng.standard_normal(4))
+    for Sb in (u.S, S2):
+        a = lri.metrics.rel_err(U2, Sb, u.V, u.U, u.S, u.V)
+        b = rel_err(U2, Sb, u.V, u.U, u.S, u.V)
+        dense = np.linalg.norm(U2 @ Sb @ u.V.T - u.to_matrix()) / np.linalg.norm(u.to_matrix())
+        assert abs(a - b) <= 1e-3 * b and abs(a - dense) <= 1e-3 * dense + 1e-16

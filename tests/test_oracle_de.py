@@ -207,3 +207,9 @@ def test_quadratic_subproblem_rhs_tensor_forms(periodic):
     assert np.allclose(Lf, F(U @ L.T).T @ U, rtol=0, atol=1e-11)
     Sf = (U.T @ A @ U) @ S + S @ (V.T @ B @ V) + (U.T @ G) @ (H.T @ V) - np.einsum("efc,ea,fb,abd->cd", W, S, S, T)
     assert np.allclose(Sf, U.T @ F(U @ S @ V.T) @ V, rtol=0, atol=1e-11)
+
+
+def test_api_burgers_samples_match_the_oracle(lri):
+    """The package's Burgers initial samples (used by bench.py) equal the oracle's
+    restatement of data_agnostic_approximation.jl:35-44."""
+    assert np.allclose(lri.burgers_samples(60, 5), dlra_de.burgers_initial(60, 5), rtol=0, atol=1e-15)
