@@ -415,6 +415,116 @@ def run_burgers(args, torch, lri):
     print(json.dumps(out), flush=True)
 
 
+def batch_problems(count, seed):
+    """512-problem workload: n = m log-uniform in [256, 4096], initial rank uniform in
+    {4..64}, Y0 = U diag(10^(-i/4)) V' with random orthonormal U, V."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(256), np.log(4096), count)).astype(np.int64)
+    rs = rng.integers(4, 65, count)
+    out = []
+    for n, r in zip(ns, rs):
+        U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+        V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+        out.append((int(n), int(r), U, np.diag(10.0 ** (-np.arange(r) / 4.0)), V))
+    return out
+
+
+def batch_cpu_baseline(probs, dt, rmax, tol, lri):
+    """Oracle rank-adaptive BUG step (factored right-hand sides) on a bounded sample."""
+    import scipy.sparse as sp
+    from oracle import dlra_de
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    order = np.argsort([p[0] for p in probs])
+    pick = [order[len(order) // 8], order[len(order) // 2], order[7 * len(order) // 8]]
+    est = 0.0
+    for i in pick:
+        n, r, U, S, V = probs[i]
+        L = sp.csr_matrix(sp.diags([np.ones(n - 1), -2.0 * np.ones(n), np.ones(n - 1)], [-1, 0, 1]) * float(n * n))
+        op = dlra_de.LowRankLyapunov(L, L, np.zeros((n, 1)), np.zeros((n, 1)))
+        N = lri.rk4_substeps(8.0 * n * n, dt)
+        Ns = min(N, 200)
+        t0 = time.perf_counter()
+        dlra_de.rank_adaptive_bug_de_step_lowrank(op, U, S, V, 0.0, dt * Ns / N, Ns, tol, rmax)
+        est += (time.perf_counter() - t0) * N / Ns
+    per_step = est / len(pick)
+    return {"value": 1.0 / per_step, "unit": "problem-steps/s", "cores": int(threads), "kind": "port",
+            "sample": f"one rank-adaptive BUG step of the FP64 NumPy/SciPy oracle (factored right-hand sides) on "
+                      f"3 problems of the batch (n = {[probs[i][0] for i in pick]}, r = "
+                      f"{[probs[i][1] for i in pick]}), each with min(N, 200) of its N RK4 sub-steps scaled to "
+                      f"N, averaged; OpenBLAS threads={threads}"}
+
+
+def run_batched(args, torch, lri):
+    """512 independent rank-adaptive BUG solves of dX/dt = L X + X L' (L the Dirichlet
+    Laplacian n^2 tridiag(1, -2, 1)), n log-uniform in [256, 4096], ranks 4..64,
+    tol 1e-8, dt 1e-4, FP64."""
+    count, dt, tol, rmax = args.batch_count, 1e-4, 1e-8, 64
+    probs = batch_problems(count, args.seed)
+    log(f"batch of {count} problems: n in [{min(p[0] for p in probs)}, {max(p[0] for p in probs)}], "
+        f"ranks in [{min(p[1] for p in probs)}, {max(p[1] for p in probs)}]; creating contexts")
+    tc = time.perf_counter()
+    b = lri.native.Batch([p[0] for p in probs], [p[0] for p in probs], [rmax] * count)
+    Ns = []
+    for (n, r, U, S, V), c in zip(probs, b.members):
+        Lop = lri.laplacian_1d(n)
+        c.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False)
+        c.set_state(U, S, V)
+        Ns.append(lri.rk4_substeps(2.0 * Lop.gershgorin(), dt))
+    b.synchronize()
+    log(f"set-up {time.perf_counter() - tc:.1f} s; RK4 sub-steps per problem in [{min(Ns)}, {max(Ns)}]")
+    for _ in range(args.warmup_de):
+        b.step_de_bug_ra(dt, Ns, tol, rmax)
+    for (n, r, U, S, V), c in zip(probs, b.members):
+        c.set_state(U, S, V)
+    b.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.batch_steps):
+        ranks = b.step_de_bug_ra(dt, Ns, tol, rmax)
+    b.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    b.close()
+    value = count * args.batch_steps / el
+    hist = np.bincount(ranks, minlength=rmax + 1)
+    log(f"batched RA: {value:.1f} problem-steps/s ({1e3 * el / args.batch_steps:.1f} ms per batch step); "
+        f"final ranks in [{ranks.min()}, {ranks.max()}]")
+    out = {
+        "metric": "problem-steps/s of 512 independent rank-adaptive BUG solves of dX/dt = L X + X L', n=m "
+                  "log-uniform in [256, 4096], initial ranks 4..64, tol 1e-8, dt 1e-4 (FP64)",
+        "value": value,
+        "unit": "problem-steps/s",
+        "n_gpus": 1,
+        "steps": args.batch_steps,
+        "warmup": args.warmup_de,
+        "ms_per_step": 1e3 * el / args.batch_steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{count} rank-adaptive BUG solves (one context and stream each, every batch step "
+                               f"queues all members before waiting), L = n^2 tridiag(1,-2,1) Dirichlet, no forcing, "
+                               f"Y0 = U diag(10^(-i/4)) V', rmax={rmax}", "problems": count, "dt": dt, "tol": tol,
+                   "rmax": rmax, "parallelism": "single GPU"},
+        "rank_histogram": {str(k): int(v) for k, v in enumerate(hist) if v},
+        "initial_rank_histogram": {str(k): int(v) for k, v in enumerate(np.bincount([p[1] for p in probs])) if v},
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        log("CPU baseline (oracle) on a bounded sample")
+        out["cpu_baseline"] = batch_cpu_baseline(probs, dt, rmax, tol, lri)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def de_initial(n, r, seed):
     rng = np.random.default_rng(seed)
     U, _ = np.linalg.qr(rng.standard_normal((n, r)))
@@ -513,10 +623,12 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the error check")
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP-event kernel timing in the timed region (for external profilers)")
-    ap.add_argument("--workload", choices=["ksl", "de", "greedy", "burgers"], default="ksl",
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--workload", choices=["ksl", "de", "greedy", "burgers", "batched"], default="ksl",
                     help="ksl: the headline data integrator; de: the BUG matrix-ODE step at n=m=4096; "
                          "greedy: the Greedy data integrator on the headline snapshots; burgers: BUG on the "
-                         "reference's Burgers problem")
+                         "reference's Burgers problem; batched: 512 uneven rank-adaptive DE solves")
     ap.add_argument("--de-n", type=int, default=4096)
     ap.add_argument("--de-steps", type=int, default=20)
     ap.add_argument("--warmup-de", type=int, default=2)
@@ -543,6 +655,9 @@ def main():
         return
     if args.workload == "burgers":
         run_burgers(args, torch, lri)
+        return
+    if args.workload == "batched":
+        run_batched(args, torch, lri)
         return
     dev = torch.device("cuda:0")
     n = m = args.n

@@ -206,6 +206,33 @@ int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps);
  * truncation and *r_new as lri_step_bug_ra (synchronising). */
 int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, int64_t rmax, int64_t* r_new);
 
+/* ---- batches of independent problems -------------------------------------
+ *
+ * Many small solves of uneven n, m and rank (the reference runs them one
+ * `solve` at a time, primitives.jl:68-80).  A batch owns one context per
+ * problem, each on its own stream; a batch step queues the step of every
+ * member before it waits for any, so their kernels overlap on the device.
+ * Members are set up through their contexts (lri_batch_context: state,
+ * operator); they are owned by the batch. */
+typedef struct lri_batch lri_batch;
+int lri_batch_create(lri_batch** out, int device, int64_t count, const int64_t* n, const int64_t* m,
+                     const int64_t* rcap);
+int lri_batch_destroy(lri_batch* batch);
+/* The message of the last failure ("problem i: ..."), or of a failed create. */
+const char* lri_batch_last_error(const lri_batch* batch);
+int lri_batch_context(lri_batch* batch, int64_t i, lri_ctx** ctx);
+/* lri_step_de_bug on every member (substeps[i] RK4 sub-steps for member i). */
+int lri_batch_step_de_bug(lri_batch* batch, double dt, const int64_t* substeps);
+/* lri_step_de_bug_ra on every member; r_new[i] receives member i's new rank.
+ * All members' steps are queued up to their SVD before the first rank is read. */
+int lri_batch_step_de_bug_ra(lri_batch* batch, double dt, const int64_t* substeps, double tol, int64_t rmax,
+                             int64_t* r_new);
+/* Rank-adaptive batch steps run each member's step up to its SVD as a HIP
+ * graph captured for the member's current rank (default on = 1), so one
+ * member-step costs one graph launch on the host; 0: direct launches. */
+int lri_batch_use_graphs(lri_batch* batch, int on);
+int lri_batch_synchronize(lri_batch* batch);
+
 /* Wait for all queued work; returns a latched device-side status, if any. */
 int lri_synchronize(lri_ctx* ctx);
 

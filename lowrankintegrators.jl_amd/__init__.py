@@ -35,6 +35,7 @@ from .api import (  # noqa: F401
     rank,
     rk4_substeps,
     solve,
+    solve_batch,
     step,
     truncated_svd,
     update_sol,

@@ -669,3 +669,80 @@ def solve(prob, alg, dt=None, **kw) -> DLRSolution:
         return integ.sol
     finally:
         integ.ctx.close()
+
+
+def solve_batch(probs, alg, dt, *, substeps=None, save_everystep: bool = False, device: int = 0):
+    """``solve`` for many independent ``MatrixDEProblem``s of uneven sizes and ranks at once
+    (the reference solves them one ``solve`` call at a time, primitives.jl:68-80): one
+    context per problem in an ``lri_batch``, whose steps queue every member's work
+    before waiting for any.  ``alg``: ``UnconventionalAlgorithm`` or
+    ``RankAdaptiveUnconventionalAlgorithm``; all problems share ``dt`` and ``tspan``.
+    Returns one ``DLRSolution`` per problem (with ``save_everystep=False``: the initial
+    and the final state)."""
+    if not probs:
+        return []
+    ra = isinstance(alg, RankAdaptiveUnconventionalAlgorithm)
+    if not (ra or isinstance(alg, UnconventionalAlgorithm)):
+        raise TypeError("solve_batch runs the unconventional and rank-adaptive integrators")
+    t0, tf = probs[0].tspan
+    if any(p.tspan != (t0, tf) for p in probs):
+        raise ValueError("all problems of a batch share tspan")
+    if not tf > t0:
+        raise AssertionError("Integration in reverse time direction is not supported")
+    ns, ms, rcaps = [], [], []
+    for p in probs:
+        if not isinstance(p, MatrixDEProblem) or not isinstance(p.f, LyapunovOperator):
+            raise TypeError("solve_batch takes MatrixDEProblems with a LyapunovOperator or QuadraticOperator")
+        n, m, r = p.u0.U.shape[0], p.u0.V.shape[0], p.u0.rank
+        ns.append(n)
+        ms.append(m)
+        rcaps.append(int(min(alg.rmax, native.MAX_RANK, n, m)) if ra else r)
+        if r > rcaps[-1]:
+            raise ValueError(f"initial rank {r} exceeds the capacity {rcaps[-1]}")
+    b = native.Batch(ns, ms, rcaps, device)
+    try:
+        Ns = []
+        for p, c in zip(probs, b.members):
+            F = p.f
+            c.set_state(p.u0.U, p.u0.S, p.u0.V)
+            q = F.G.shape[1]
+            c.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic,
+                                    F.G if q else None, F.H if q else None)
+            if isinstance(F, QuadraticOperator):
+                c.set_operator_quadratic(F.D.dl, F.D.dd, F.D.du, F.D.periodic)
+                ymax = float(np.max(np.linalg.norm(p.u0.U @ p.u0.S, axis=1)) * np.max(np.linalg.norm(p.u0.V, axis=1)))
+                rho = F.spectral_bound(2.0 * ymax)
+            else:
+                rho = F.spectral_bound()
+            Ns.append(int(substeps) if substeps is not None else rk4_substeps(rho, dt))
+        sols = [init_sol(dt, t0, tf, p.u0) for p in probs]
+        for s_, p in zip(sols, probs):
+            s_.Y[0] = p.u0.copy()
+        t, it, T = t0, 0, tf - t0
+        while (tf - t) / T > 1e-8:
+            if ra:
+                b.step_de_bug_ra(dt, Ns, alg.tol, alg.rmax)
+            else:
+                b.step_de_bug(dt, Ns)
+            t = t + dt
+            it += 1
+            last = not ((tf - t) / T > 1e-8)
+            if save_everystep or last:
+                for s_, c in zip(sols, b.members):
+                    U, S, V = c.get_state()
+                    u = SVDLikeRepresentation(U, S, V)
+                    if it <= len(s_.Y) - 1:
+                        s_.Y[it] = u
+                        s_.t[it] = t
+                    else:
+                        s_.Y.append(u)
+                        s_.t.append(t)
+        b.synchronize()
+        if not save_everystep:
+            for s_ in sols:
+                keep = [k for k, y in enumerate(s_.Y) if y is not None]
+                s_.Y = [s_.Y[k] for k in keep]
+                s_.t = [s_.t[k] for k in keep]
+        return sols
+    finally:
+        b.close()

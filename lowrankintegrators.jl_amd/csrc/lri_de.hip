@@ -264,7 +264,7 @@ __device__ __forceinline__ void rk4_power(double z, double h, int64_t N, double&
 // entry advanced in closed form, S1 = Wa Sbar Wb'.  kLds: the bases and the
 // work matrices sit in LDS (k, l <= 64), else in P.work.
 template <bool kLds>
-__global__ __launch_bounds__(256) void k_s_substep(SStepArgs P) {
+__global__ __launch_bounds__(1024) void k_s_substep(SStepArgs P) {
   extern __shared__ double sm[];
   const int k = P.k, l = P.l, q = P.q, t = threadIdx.x, nt = blockDim.x;
   const double* Wa = P.Wa;
@@ -345,9 +345,9 @@ hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st) {
   if (P.k <= 64 && P.l <= 64) {
     const size_t lds = sizeof(double) * ((size_t)(P.k | 1) * P.k + (size_t)P.l * P.l + 2 * (size_t)P.k * P.l +
                                          (size_t)(P.k + P.l) * P.q);
-    hipLaunchKernelGGL(k_s_substep<true>, dim3(1), dim3(256), lds, st, P);
+    hipLaunchKernelGGL(k_s_substep<true>, dim3(1), dim3(P.k * P.l > 1024 ? 1024 : 256), lds, st, P);
   } else {
-    hipLaunchKernelGGL(k_s_substep<false>, dim3(1), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(k_s_substep<false>, dim3(1), dim3(1024), 0, st, P);
   }
   return hipGetLastError();
 }

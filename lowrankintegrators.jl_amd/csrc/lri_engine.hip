@@ -132,6 +132,16 @@ struct lri_ctx {
   // S sub-step's ones, which are the next BUG step's K/L ones. Any other entry
   // point clears the flag.
   bool de_gal_fresh = false;
+  int* r_host = nullptr;  // pinned: the rank of a rank-adaptive step
+  int ra_k2 = 0;
+  bool defer_rank_copy = false;
+  // captured rank-adaptive DE steps (up to the SVD), one per rank and step parameters
+  struct RaGraph {
+    int64_t r, nsub, rmax;
+    double dt, tol;
+    hipGraphExec_t exec;
+  };
+  std::vector<RaGraph> ra_graphs;
   // quadratic term -(D Y) o Y (D tridiagonal on the n rows), its tensors
   // T = V x V x V and W = (D U) x U x U, and their scratch
   bool op_quad = false;
@@ -813,7 +823,15 @@ struct lri_ctx {
 
   // SVD of the 2r x 2r core small->Shat, truncation, and the rotation of Uhat,
   // Vhat into the new state (rank_adaptive_unconventional.jl:222-231).
+  // ra_finish in two halves, so that a batch of contexts can queue the SVDs of
+  // all its members before waiting for any rank: ra_svd launches the SVD and
+  // copies the new rank to pinned host memory; ra_rotate waits for it and
+  // forms U, S, V of the new rank.
   int64_t ra_finish(int k2, double tol, int64_t rmax) {
+    ra_svd(k2, tol, rmax);
+    return ra_rotate();
+  }
+  void ra_svd(int k2, double tol, int64_t rmax) {
     SvdArgs sv;
     sv.S = small->Shat;
     sv.lds = k2;
@@ -826,9 +844,50 @@ struct lri_ctx {
     sv.rmax = std::min<int64_t>(rmax, rcap);
     sv.work = svd_work;
     timed(LRI_KERNEL_SMALL, [&] { return launch_jacobi_svd(sv, stream); });
-    int r1 = 0;
-    HIPCHECK(hipMemcpyAsync(&r1, &small->r_new, sizeof(int), hipMemcpyDeviceToHost, stream));
+    if (!defer_rank_copy) HIPCHECK(hipMemcpyAsync(r_host, &small->r_new, sizeof(int), hipMemcpyDeviceToHost, stream));
+    ra_k2 = k2;
+  }
+
+  // A rank-adaptive DE step up to its SVD as a captured graph (captured on first
+  // use for the current rank and step parameters, replayed after), then the
+  // copy of the new rank; ra_rotate completes the step.  The step's launches
+  // read and write only buffers fixed at creation, so a graph stays valid
+  // while the rank is unchanged.
+  void de_bug_ra_queue_graph(double dt, int64_t nsub, double tol, int64_t rmax) {
+    hipGraphExec_t exec = nullptr;
+    for (const RaGraph& g : ra_graphs)
+      if (g.r == r && g.nsub == nsub && g.rmax == rmax && g.dt == dt && g.tol == tol) exec = g.exec;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      defer_rank_copy = true;
+      try {
+        de_bug_ra(dt, nsub, tol, rmax, false);
+      } catch (...) {
+        defer_rank_copy = false;
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      defer_rank_copy = false;
+      HIPCHECK(hipStreamEndCapture(stream, &graph));
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHECK(e);
+      if (ra_graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(ra_graphs.front().exec);
+        ra_graphs.erase(ra_graphs.begin());
+      }
+      ra_graphs.push_back(RaGraph{r, nsub, rmax, dt, tol, exec});
+    }
+    HIPCHECK(hipGraphLaunch(exec, stream));
+    ra_k2 = 2 * (int)r;
+    HIPCHECK(hipMemcpyAsync(r_host, &small->r_new, sizeof(int), hipMemcpyDeviceToHost, stream));
+  }
+  int64_t ra_rotate() {
+    const int k2 = ra_k2;
     HIPCHECK(hipStreamSynchronize(stream));
+    const int r1 = *r_host;
     REQUIRE(r1 >= 1 && r1 <= rcap, "internal: rank-adaptive step returned an invalid rank");
     // U = Uhat Ush[:, :r1], V = Vhat Vsh[:, :r1], S = diag(sigma[:r1])
     for (int side = 0; side < 2; ++side) {
@@ -1286,7 +1345,7 @@ struct lri_ctx {
     de_gal_fresh = true;  // the Galerkin data and tensors now belong to U, V
   }
 
-  int64_t quad_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax) {
+  int64_t quad_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax, bool complete = true) {
     const int rr = (int)r;
     const int k2 = 2 * rr;
     const double h = dt / (double)nsub;
@@ -1309,7 +1368,8 @@ struct lri_ctx {
     quad_galerkin(Uhat, true, k2);
     quad_galerkin(Vhat, false, k2);
     quad_sstep(k2, k2, small->Sinit, k2, 0, small->Shat, k2, 1.0, h, nsub);
-    return ra_finish(k2, tol, rmax);
+    ra_svd(k2, tol, rmax);
+    return complete ? ra_rotate() : 0;
   }
 
   void de_ksl_primal(double dt, int64_t nsub) {
@@ -1397,8 +1457,9 @@ struct lri_ctx {
 
   // rankadaptive_unconventional_step! with the DE sub-problems
   // (rank_adaptive_unconventional.jl:59-86,194-233)
-  int64_t de_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax) {
-    if (op_quad) return quad_bug_ra(dt, nsub, tol, rmax);
+  // complete = false: stop after queueing the SVD (ra_rotate finishes the step)
+  int64_t de_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax, bool complete = true) {
+    if (op_quad) return quad_bug_ra(dt, nsub, tol, rmax, complete);
     const int rr = (int)r;
     const int k2 = 2 * rr;
     const double h = dt / (double)nsub;
@@ -1426,7 +1487,8 @@ struct lri_ctx {
     de_galerkin(Vhat, false, k2, de->Gb, de->hbT);
     de_eig2(k2, k2);
     de_sstep(k2, k2, small->Sinit, k2, 0, small->Shat, k2, 1.0, h, nsub);
-    return ra_finish(k2, tol, rmax);
+    ra_svd(k2, tol, rmax);
+    return complete ? ra_rotate() : 0;
   }
 
   // Constant-coefficient sides get their analytic eigenbasis (used by de_run);
@@ -1541,6 +1603,8 @@ void free_ctx(lri_ctx* c) {
   if (!c->spec_alias)
     for (void* p : {(void*)c->specQ[1], (void*)c->specLam[1]})
       if (p) (void)hipFree(p);
+  if (c->r_host) (void)hipHostFree(c->r_host);
+  for (auto& g : c->ra_graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& t : c->pending) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -1550,6 +1614,33 @@ void free_ctx(lri_ctx* c) {
   delete c;
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------- batches
+// Independent problems of uneven n, m and rank, one context (and stream) each.
+// A batch step queues every member's step before it waits for any of them, so
+// the members' kernels overlap on the device.
+struct lri_batch {
+  int device = 0;
+  std::vector<lri_ctx*> ctx;
+  std::string err;
+  bool graphs = true;  // rank-adaptive members step through captured graphs
+};
+
+namespace {
+// run f(i, ctx) on every member; the first failure is reported as "problem i: ..."
+template <class F>
+int batch_each(lri_batch* b, F&& f) {
+  if (!b) return LRI_EINVAL;
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    const int st = f((int64_t)i, b->ctx[i]);
+    if (st != LRI_OK) {
+      b->err = "problem " + std::to_string(i) + ": " + lri_last_error(b->ctx[i]);
+      return st;
+    }
+  }
+  return LRI_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1645,6 +1736,8 @@ int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
     alloc((void**)&c->flags, 64);
     alloc((void**)&c->small, sizeof(SmallState));
     alloc((void**)&c->svd_work, d * kMaxAug * kMaxAug);
+    HIPCHECK(hipHostMalloc((void**)&c->r_host, sizeof(int), hipHostMallocDefault));
+    *c->r_host = 0;
     HIPCHECK(hipDeviceSynchronize());
   } catch (const Failure& e) {
     g_create_error = e.msg;
@@ -1966,6 +2059,109 @@ int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, in
     if (r_new) *r_new = r1;
     return ctx->read_flags();
   });
+}
+
+
+int lri_batch_create(lri_batch** out, int device, int64_t count, const int64_t* n, const int64_t* m,
+                     const int64_t* rcap) {
+  g_create_error.clear();
+  if (!out || count < 1 || !n || !m || !rcap) {
+    g_create_error = "out, n, m and rcap are required and count >= 1";
+    return LRI_EINVAL;
+  }
+  *out = nullptr;
+  lri_batch* b = nullptr;
+  try {
+    b = new lri_batch();
+  } catch (const std::bad_alloc&) {
+    g_create_error = "host allocation failed";
+    return LRI_EOOM;
+  }
+  b->device = device;
+  for (int64_t i = 0; i < count; ++i) {
+    lri_ctx* c = nullptr;
+    const int st = lri_create(&c, device, n[i], m[i], rcap[i]);
+    if (st != LRI_OK) {
+      g_create_error = "problem " + std::to_string(i) + ": " + g_create_error;
+      for (lri_ctx* x : b->ctx) free_ctx(x);
+      delete b;
+      return st;
+    }
+    b->ctx.push_back(c);
+  }
+  *out = b;
+  return LRI_OK;
+}
+
+int lri_batch_destroy(lri_batch* b) {
+  if (!b) return LRI_OK;
+  for (lri_ctx* c : b->ctx) free_ctx(c);
+  delete b;
+  return LRI_OK;
+}
+
+const char* lri_batch_last_error(const lri_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int lri_batch_context(lri_batch* b, int64_t i, lri_ctx** ctx) {
+  if (!b || !ctx || i < 0 || i >= (int64_t)b->ctx.size()) return LRI_EINVAL;
+  *ctx = b->ctx[i];
+  return LRI_OK;
+}
+
+
+int lri_batch_step_de_bug(lri_batch* b, double dt, const int64_t* substeps) {
+  if (!substeps) return LRI_EINVAL;
+  return batch_each(b, [&](int64_t i, lri_ctx* c) { return lri_step_de_bug(c, dt, substeps[i]); });
+}
+
+int lri_batch_step_de_bug_ra(lri_batch* b, double dt, const int64_t* substeps, double tol, int64_t rmax,
+                             int64_t* r_new) {
+  if (!b || !substeps) return LRI_EINVAL;
+  // queue every member's step up to its SVD ...
+  int st = batch_each(b, [&](int64_t i, lri_ctx* c) {
+    return guarded(c, [&] {
+      c->check_de(dt, substeps[i]);
+      REQUIRE(tol >= 0.0, "tol must be >= 0");
+      REQUIRE(rmax >= 1, "rmax must be >= 1");
+      REQUIRE(!c->op_quad || 2 * c->r <= kQuadMaxRank,
+              "the rank-adaptive step with the quadratic operator needs 2 r <= LRI_QUAD_MAX_RANK");
+      DeScope scope(c);
+      if (b->graphs && !c->timing)
+        c->de_bug_ra_queue_graph(dt, substeps[i], tol, rmax);
+      else
+        c->de_bug_ra(dt, substeps[i], tol, rmax, false);
+      return LRI_OK;
+    });
+  });
+  if (st != LRI_OK) {
+    for (lri_ctx* c : b->ctx) (void)hipStreamSynchronize(c->stream);
+    return st;
+  }
+  // ... then wait for the ranks in order and queue each member's rotation ...
+  st = batch_each(b, [&](int64_t i, lri_ctx* c) {
+    return guarded(c, [&] {
+      DeScope scope(c);
+      const int64_t r1 = c->ra_rotate();
+      if (r_new) r_new[i] = r1;
+      return LRI_OK;
+    });
+  });
+  if (st != LRI_OK) {
+    for (lri_ctx* c : b->ctx) (void)hipStreamSynchronize(c->stream);
+    return st;
+  }
+  // ... and collect the device-side conditions once all are queued
+  return batch_each(b, [&](int64_t, lri_ctx* c) { return guarded<true>(c, [&] { return c->read_flags(); }); });
+}
+
+int lri_batch_use_graphs(lri_batch* b, int on) {
+  if (!b) return LRI_EINVAL;
+  b->graphs = on != 0;
+  return LRI_OK;
+}
+
+int lri_batch_synchronize(lri_batch* b) {
+  return batch_each(b, [&](int64_t, lri_ctx* c) { return lri_synchronize(c); });
 }
 
 int lri_synchronize(lri_ctx* ctx) {

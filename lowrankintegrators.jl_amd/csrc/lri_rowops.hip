@@ -1002,22 +1002,43 @@ __device__ void jacobi_wave_body(const SvdArgs& P) {
   jacobi_finish(P, A, LDA, Vm, LDA, sig, 64);
 }
 
+// group_sum for a group size known only at run time (uniform across the workgroup)
+__device__ __forceinline__ double group_sum_rt(double x, int g) {
+  switch (g) {
+    case 1: return x;
+    case 2: return group_sum<2>(x);
+    case 4: return group_sum<4>(x);
+    case 8: return group_sum<8>(x);
+    case 16: return group_sum<16>(x);
+    case 32: return group_sum<32>(x);
+    default: return group_sum<64>(x);
+  }
+}
+
+// The workgroup form (k <= 128, blockDim.x threads): gsz lanes per column pair,
+// a workgroup barrier per round.  A (k x ke) in LDS at an odd stride; V in
+// LDS when it fits, else in P.work.
 __device__ void jacobi_svd_body(const SvdArgs& P, double* lds) {
   const int k = P.k;
   const int ke = (k + 1) & ~1;
-  double* A = lds;                   // k rows x ke columns
-  double* sig = A + (size_t)ke * k;  // ke
-  double* Vm = P.work ? P.work : sig + ke;  // ke x ke, LDS when it fits
-  const int t = threadIdx.x;
-  for (int e = t; e < ke * k; e += kThreads) {
-    int i = e % k, j = e / k;
-    A[e] = (j < k) ? P.S[i + j * P.lds] : 0.0;
+  const int nt = blockDim.x, t = threadIdx.x;
+  const int lda = k | 1;
+  const int ldv = P.work ? ke : (ke | 1);
+  double* A = lds;                      // k rows x ke columns, ld lda
+  double* sig = A + (size_t)lda * ke;   // ke
+  double* Vm = P.work ? P.work : sig + ke;  // ke x ke, ld ldv
+  for (int e = t; e < ke * k; e += nt) {
+    const int i = e % k, j = e / k;
+    A[i + j * lda] = (j < k) ? P.S[i + j * P.lds] : 0.0;
   }
-  for (int e = t; e < ke * ke; e += kThreads) Vm[e] = ((e % ke) == (e / ke)) ? 1.0 : 0.0;
+  for (int e = t; e < ke * ke; e += nt) {
+    const int i = e % ke, j = e / ke;
+    Vm[i + j * ldv] = (i == j) ? 1.0 : 0.0;
+  }
   __syncthreads();
   const int npairs = ke / 2;
   int gsz = 1;
-  while (gsz * 2 * npairs <= kThreads && gsz < 64) gsz *= 2;
+  while (gsz * 2 * npairs <= nt && gsz < 64) gsz *= 2;
   __shared__ int s_rot;
   for (int sweep = 0; sweep < 60; ++sweep) {
     if (t == 0) s_rot = 0;
@@ -1037,31 +1058,34 @@ __device__ void jacobi_svd_body(const SvdArgs& P, double* lds) {
       double al = 0.0, be = 0.0, ga = 0.0;
       if (pa >= 0) {
         for (int i = ln; i < k; i += gsz) {
-          double x = A[i + pa * k], y = A[i + pb * k];
-          al += x * x;
-          be += y * y;
-          ga += x * y;
+          const double x = A[i + pa * lda], y = A[i + pb * lda];
+          al = __builtin_fma(x, x, al);
+          be = __builtin_fma(y, y, be);
+          ga = __builtin_fma(x, y, ga);
         }
       }
-      for (int off = gsz / 2; off > 0; off >>= 1) {
-        al += __shfl_xor(al, off, gsz);
-        be += __shfl_xor(be, off, gsz);
-        ga += __shfl_xor(ga, off, gsz);
-      }
-      if (pa >= 0 && ga != 0.0 && fabs(ga) > 4.0 * kEps * sqrt(al * be)) {
-        double zeta = (be - al) / (2.0 * ga);
-        double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        double c = 1.0 / sqrt(1.0 + tt * tt);
-        double s = c * tt;
+      al = group_sum_rt(al, gsz);
+      be = group_sum_rt(be, gsz);
+      ga = group_sum_rt(ga, gsz);
+      // rotate unless |ga| <= 4 eps sqrt(al be) (as in the wavefront form)
+      if (pa >= 0 && ga * ga > 16.0 * kEps * kEps * al * be) {
+        const double d = be - al;
+        const double g2 = 2.0 * ga;
+        const double q = __builtin_fma(d, d, g2 * g2);
+        const double den = fabs(d) + q * rsqrt_nr(q);
+        double tt = g2 * rcp_nr(den);
+        if (d < 0.0) tt = -tt;
+        const double c = rsqrt_nr(__builtin_fma(tt, tt, 1.0));
+        const double sn = c * tt;
         for (int i = ln; i < k; i += gsz) {
-          double x = A[i + pa * k], y = A[i + pb * k];
-          A[i + pa * k] = c * x - s * y;
-          A[i + pb * k] = s * x + c * y;
+          const double x = A[i + pa * lda], y = A[i + pb * lda];
+          A[i + pa * lda] = c * x - sn * y;
+          A[i + pb * lda] = sn * x + c * y;
         }
         for (int i = ln; i < ke; i += gsz) {
-          double x = Vm[i + pa * ke], y = Vm[i + pb * ke];
-          Vm[i + pa * ke] = c * x - s * y;
-          Vm[i + pb * ke] = s * x + c * y;
+          const double x = Vm[i + pa * ldv], y = Vm[i + pb * ldv];
+          Vm[i + pa * ldv] = c * x - sn * y;
+          Vm[i + pb * ldv] = sn * x + c * y;
         }
         if (ln == 0) s_rot = 1;
       }
@@ -1071,7 +1095,7 @@ __device__ void jacobi_svd_body(const SvdArgs& P, double* lds) {
     __syncthreads();
     if (!any) break;
   }
-  jacobi_finish(P, A, k, Vm, ke, sig, kThreads);
+  jacobi_finish(P, A, lda, Vm, ldv, sig, nt);
 }
 
 __global__ __launch_bounds__(kThreads) void k_small_abt(const double* __restrict__ A, const double* __restrict__ B,
@@ -1091,7 +1115,7 @@ hipError_t launch_small_abt(const double* A, const double* B, int k, double* C, 
 }
 
 // workgroup 0 decomposes P0, workgroup 1 (if launched) P1
-__global__ __launch_bounds__(kThreads) void k_jacobi_svd(SvdArgs P0, SvdArgs P1) {
+__global__ __launch_bounds__(1024) void k_jacobi_svd(SvdArgs P0, SvdArgs P1) {
   extern __shared__ double lds[];
   jacobi_svd_body(blockIdx.x == 0 ? P0 : P1, lds);
 }
@@ -1116,15 +1140,18 @@ __global__ __launch_bounds__(64) void k_jacobi_wave(SvdArgs P0, SvdArgs P1) {
 static hipError_t jacobi_plan(SvdArgs& P, size_t* lds) {
   if (P.k < 1 || P.k > kMaxAug) return hipErrorInvalidValue;
   const int ke = (P.k + 1) & ~1;
-  *lds = sizeof(double) * ((size_t)ke * P.k + ke);
-  if (*lds + sizeof(double) * (size_t)ke * ke <= 150 * 1024) {
-    *lds += sizeof(double) * (size_t)ke * ke;
+  *lds = sizeof(double) * ((size_t)(P.k | 1) * ke + ke);
+  if (*lds + sizeof(double) * (size_t)(ke | 1) * ke <= 150 * 1024) {
+    *lds += sizeof(double) * (size_t)(ke | 1) * ke;
     P.work = nullptr;
   } else if (!P.work) {
     return hipErrorInvalidValue;
   }
   return hipSuccess;
 }
+
+// threads of the workgroup form: 8 or more lanes per column pair up to 1024
+static int jacobi_threads(int k) { return k <= 64 ? 512 : 1024; }
 
 hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st) {
   size_t lds;
@@ -1133,7 +1160,7 @@ hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st) {
   if (P.k <= 32)
     hipLaunchKernelGGL(k_jacobi_wave, dim3(1), dim3(64), 0, st, P, P);
   else
-    hipLaunchKernelGGL(k_jacobi_svd, dim3(1), dim3(kThreads), lds, st, P, P);
+    hipLaunchKernelGGL(k_jacobi_svd, dim3(1), dim3(jacobi_threads(P.k)), lds, st, P, P);
   return hipGetLastError();
 }
 
@@ -1145,7 +1172,8 @@ hipError_t launch_jacobi_svd2(SvdArgs P0, SvdArgs P1, hipStream_t st) {
   if (P0.k <= 32 && P1.k <= 32)
     hipLaunchKernelGGL(k_jacobi_wave, dim3(2), dim3(64), 0, st, P0, P1);
   else
-    hipLaunchKernelGGL(k_jacobi_svd, dim3(2), dim3(kThreads), l0 > l1 ? l0 : l1, st, P0, P1);
+    hipLaunchKernelGGL(k_jacobi_svd, dim3(2), dim3(jacobi_threads(std::max(P0.k, P1.k))), l0 > l1 ? l0 : l1, st,
+                       P0, P1);
   return hipGetLastError();
 }
 

@@ -56,6 +56,16 @@ SIGNATURES = {
     "lri_step_de_ksl": (_int, [_vp, _int, ctypes.c_double, _i64]),
     "lri_step_de_bug": (_int, [_vp, ctypes.c_double, _i64]),
     "lri_step_de_bug_ra": (_int, [_vp, ctypes.c_double, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
+    "lri_batch_create": (_int, [ctypes.POINTER(_vp), _int, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                ctypes.POINTER(_i64)]),
+    "lri_batch_destroy": (_int, [_vp]),
+    "lri_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "lri_batch_context": (_int, [_vp, _i64, ctypes.POINTER(_vp)]),
+    "lri_batch_step_de_bug": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_i64)]),
+    "lri_batch_step_de_bug_ra": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_i64), ctypes.c_double, _i64,
+                                        ctypes.POINTER(_i64)]),
+    "lri_batch_use_graphs": (_int, [_vp, _int]),
+    "lri_batch_synchronize": (_int, [_vp]),
     "lri_synchronize": (_int, [_vp]),
     "lri_get_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "lri_kernel_timing": (_int, [_vp, _int]),
@@ -304,3 +314,96 @@ def selftest_mfma(which: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
     check(lib().lri_selftest_mfma(int(which), a.ctypes.data_as(_vp), b.ctypes.data_as(_vp),
                                   out.ctypes.data_as(_vp)), None)
     return out
+
+
+def _check_batch(status, b):
+    if status == OK:
+        return
+    L = lib()
+    raw = L.lri_batch_last_error(b)
+    msg = raw.decode(errors="replace") if raw else L.lri_status_string(status).decode()
+    if status == EBREAKDOWN:
+        raise BreakdownError(status, msg)
+    if status == ENONFINITE:
+        raise NonFiniteError(status, msg)
+    if status == EINVAL:
+        raise ValueError(msg)
+    raise LRIError(status, msg)
+
+
+class _Member(Context):
+    """A batch member: the context API on a context the batch owns."""
+
+    def __init__(self, h, n, m, rcap):  # noqa: super().__init__ would create a context
+        self.h = h
+        self.n, self.m, self.rcap = int(n), int(m), int(rcap)
+        self._bound = None
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Owning wrapper of an lri_batch: independent problems of uneven sizes, one
+    context each; ``members[i]`` is member i's context (state, operator)."""
+
+    def __init__(self, ns, ms, rcaps, device: int = 0):
+        L = lib()
+        ns, ms, rcaps = (np.ascontiguousarray(x, dtype=np.int64) for x in (ns, ms, rcaps))
+        if not (ns.shape == ms.shape == rcaps.shape) or ns.ndim != 1:
+            raise ValueError("ns, ms and rcaps need one entry per problem")
+        h = _vp()
+        ip = ctypes.POINTER(_i64)
+        st = L.lri_batch_create(ctypes.byref(h), int(device), ns.shape[0], ns.ctypes.data_as(ip),
+                                ms.ctypes.data_as(ip), rcaps.ctypes.data_as(ip))
+        if st != OK:
+            _check_batch(st, None)
+        self.h = h
+        self.members = []
+        for i in range(ns.shape[0]):
+            c = _vp()
+            _check_batch(L.lri_batch_context(h, i, ctypes.byref(c)), h)
+            self.members.append(_Member(c, ns[i], ms[i], rcaps[i]))
+
+    def __len__(self):
+        return len(self.members)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for c in self.members:
+                c.close()
+            lib().lri_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _sub(self, substeps):
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(substeps, dtype=np.int64), (len(self),)))
+        return s, s.ctypes.data_as(ctypes.POINTER(_i64))
+
+    def step_de_bug(self, dt: float, substeps):
+        s, p = self._sub(substeps)
+        _check_batch(lib().lri_batch_step_de_bug(self.h, float(dt), p), self.h)
+
+    def step_de_bug_ra(self, dt: float, substeps, tol: float, rmax: int) -> np.ndarray:
+        s, p = self._sub(substeps)
+        r = np.zeros(len(self), dtype=np.int64)
+        _check_batch(lib().lri_batch_step_de_bug_ra(self.h, float(dt), p, float(tol), int(rmax),
+                                                     r.ctypes.data_as(ctypes.POINTER(_i64))), self.h)
+        return r
+
+    def use_graphs(self, on: bool):
+        _check_batch(lib().lri_batch_use_graphs(self.h, int(bool(on))), self.h)
+
+    def synchronize(self):
+        _check_batch(lib().lri_batch_synchronize(self.h), self.h)

@@ -250,6 +250,22 @@ def bug_de_step_lowrank(op: LowRankLyapunov, U, S, V, t, dt, N):
     return U1, S1, V1
 
 
+def rank_adaptive_bug_de_step_lowrank(op: LowRankLyapunov, U, S, V, t, dt, N, tol, rmax):
+    """rank_adaptive_unconventional.jl:194-233 with the DE sub-problems in factored form
+    (as ``bug_de_step_lowrank``); the S sub-problem in the augmented bases is
+    S' = Uh' F(Uh S Vh') Vh = (Uh'AUh) S + S (Vh'B'Vh) + (Uh'G)(H'Vh)."""
+    K = rk4(op.K_rhs(V), U @ S, t, dt, N)
+    Uh, _ = _qr(np.hstack([K, U]))
+    M = Uh.T @ U
+    L = rk4(op.L_rhs(U), V @ S.T, t, dt, N)
+    Vh, _ = _qr(np.hstack([L, V]))
+    Nn = Vh.T @ V
+    Sh = rk4(op.S_rhs(Uh, Vh), M @ S @ Nn.T, t, dt, N)
+    Us, sig, Vst = np.linalg.svd(Sh)
+    r1 = int(min(rmax, truncate_to_tolerance(sig, tol)))
+    return Uh @ Us[:, :r1], np.diag(sig[:r1]), Vh @ Vst[:r1].T, r1
+
+
 def solve_de(prob: MatrixDEProblem, alg, dt, substeps: int) -> DLRSolution:
     """``solve(prob::MatrixDEProblem, alg, dt)`` (primitives.jl:68-76) with RK4 sub-steps."""
     t0, tf = prob.tspan

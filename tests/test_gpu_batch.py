@@ -1,0 +1,123 @@
+"""Batches of independent matrix-ODE solves of uneven sizes and ranks (needs an MI355X).
+
+A batch step queues every member's step before it waits for any of them; each
+member runs exactly the kernels a context of its own runs, in the same order,
+so the results must agree bit for bit with one context per problem, and with
+the CPU oracle to the FP64 bound (relative Frobenius error <= 1e-10).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import dlra_de
+from oracle.data import random_factors
+from oracle.metrics import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+SIZES = [(48, 48, 3), (130, 97, 6), (64, 64, 8), (256, 256, 4), (77, 120, 5), (300, 300, 16)]
+
+
+def problems(lri, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (n, m, r) in enumerate(SIZES):
+        if k % 2 == 0:  # constant coefficients (analytic eigenbasis)
+            A = lri.laplacian_1d(n, 40.0, periodic=(k % 4 == 0))
+            B = lri.laplacian_1d(m, 40.0)
+        else:  # variable coefficients (RK4 column kernel)
+            A = lri.SymTridiagonal(-rng.uniform(40, 80, n), rng.uniform(10, 20, n - 1))
+            B = lri.SymTridiagonal(-rng.uniform(40, 80, m), rng.uniform(10, 20, m - 1))
+        F = lri.LyapunovOperator(A, B, rng.standard_normal((n, 1)), rng.standard_normal((m, 1)))
+        u = random_factors(rng, n, m, r, decay=0.5)
+        out.append((F, u))
+    return out
+
+
+def run_alone(lri, F, u, steps, dt, N, tol, rmax):
+    n, m = u.U.shape[0], u.V.shape[0]
+    with lri.native.Context(n, m, min(rmax, n, m)) as c:
+        c.set_state(u.U, u.S, u.V)
+        c.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic, F.G, F.H)
+        ranks = [c.step_de_bug_ra(dt, N, tol, rmax) for _ in range(steps)]
+        return c.get_state(), ranks
+
+
+def test_batch_rank_adaptive_matches_single_contexts(lri):
+    probs = problems(lri)
+    dt, tol, rmax, steps = 2e-3, 1e-6, 24, 4
+    Ns = [lri.rk4_substeps(F.spectral_bound(), dt) for F, _ in probs]
+    say(f"case: batch of {len(probs)} RA problems, sizes {SIZES}, {steps} steps")
+    with lri.native.Batch([u.U.shape[0] for _, u in probs], [u.V.shape[0] for _, u in probs],
+                          [min(rmax, u.U.shape[0], u.V.shape[0]) for _, u in probs]) as b:
+        for (F, u), c in zip(probs, b.members):
+            c.set_state(u.U, u.S, u.V)
+            c.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic, F.G, F.H)
+        ranks = [b.step_de_bug_ra(dt, Ns, tol, rmax) for _ in range(steps)]
+        got = [c.get_state() for c in b.members]
+    for i, (F, u) in enumerate(probs):
+        ref, rk = run_alone(lri, F, u, steps, dt, Ns[i], tol, rmax)
+        assert [int(r[i]) for r in ranks] == rk
+        for a, b_ in zip(got[i], ref):
+            assert a.shape == b_.shape and np.array_equal(a, b_), f"problem {i}"
+    # one member against the oracle
+    F, u = probs[1]
+    host = dlra_de.LyapunovOperator(F.A.dense(), F.B.dense(), F.G, F.H)
+    U, S, V = u.U, u.S, u.V
+    for _ in range(steps):
+        U, S, V, _ = dlra_de.rank_adaptive_bug_de_step(host, U, S, V, 0.0, dt, Ns[1], tol, rmax)
+    e = rel_err(*got[1], U, S, V)
+    say(f"  member 1 vs oracle after {steps} steps: rank {S.shape[0]}, rel err {e:.2e}")
+    assert got[1][1].shape == S.shape and e <= 1e-10
+
+
+def test_solve_batch_bug(lri):
+    probs = problems(lri, seed=1)
+    dt, tf = 1e-3, 3e-3
+    P = [lri.MatrixDEProblem(F, lri.SVDLikeRepresentation(u.U, u.S, u.V), (0.0, tf)) for F, u in probs]
+    sols = lri.solve_batch(P, lri.UnconventionalAlgorithm(), dt, substeps=6)
+    assert len(sols) == len(P)
+    for (F, u), s in zip(probs, sols):
+        assert len(s.Y) == 2 and abs(s.t[-1] - tf) < 1e-15
+        host = dlra_de.LyapunovOperator(F.A.dense(), F.B.dense(), F.G, F.H)
+        U, S, V = u.U, u.S, u.V
+        for k in range(3):
+            U, S, V = dlra_de.bug_de_step(host, U, S, V, k * dt, dt, 6)
+        y = s.Y[-1]
+        assert rel_err(y.U, y.S, y.V, U, S, V) <= 1e-10
+
+
+def test_batch_graph_replays_match_direct_launches(lri):
+    """With rmax = r and tol = 0 the rank never changes, so every member replays the
+    graph captured at its first step; each replay (with host reads of the state
+    in between) must equal the same batch stepped with direct launches."""
+    probs = problems(lri, seed=2)
+    dt, steps = 2e-3, 6
+    Ns = [lri.rk4_substeps(F.spectral_bound(), dt) for F, _ in probs]
+    rs = [u.S.shape[0] for _, u in probs]
+    say(f"case: graph replays, {len(probs)} members, fixed ranks {rs}, {steps} steps")
+    states = {}
+    for graphs in (True, False):
+        with lri.native.Batch([u.U.shape[0] for _, u in probs], [u.V.shape[0] for _, u in probs], rs) as b:
+            b.use_graphs(graphs)
+            for (F, u), c in zip(probs, b.members):
+                c.set_state(u.U, u.S, u.V)
+                c.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic, F.G, F.H)
+            seq = []
+            for _ in range(steps):
+                # rmax = the member's own capacity (= its rank): pass the largest; each context clips it
+                ranks = b.step_de_bug_ra(dt, Ns, 0.0, max(rs))
+                assert list(ranks) == rs
+                seq.append([c.get_state() for c in b.members])
+            states[graphs] = seq
+    for k in range(steps):
+        for i in range(len(probs)):
+            for a, b_ in zip(states[True][k][i], states[False][k][i]):
+                assert np.array_equal(a, b_), f"step {k}, member {i}"

@@ -138,6 +138,10 @@ def test_lowrank_evaluation_matches_dense():
     a = dlra_de.bug_de_step_lowrank(op, u.U, u.S, u.V, 0.0, 0.05, 4)
     b = dlra_de.bug_de_step(op.dense(), u.U, u.S, u.V, 0.0, 0.05, 4)
     assert rel_err(*a, *b) < 1e-13
+    for tol, rmax in ((1e-3, 8), (1e-12, 5)):
+        a = dlra_de.rank_adaptive_bug_de_step_lowrank(op, u.U, u.S, u.V, 0.0, 0.05, 4, tol, rmax)
+        b = dlra_de.rank_adaptive_bug_de_step(op.dense(), u.U, u.S, u.V, 0.0, 0.05, 4, tol, rmax)
+        assert a[3] == b[3] and rel_err(*a[:3], *b[:3]) < 1e-13
 
 
 # --------------------------------------------------------------- quadratic (Burgers) operator
