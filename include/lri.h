@@ -141,6 +141,18 @@ int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next);
 int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax,
                     int64_t* r_new);
 
+/* Reverse (adjoint) of the BUG data step (unconventional.jl:121-164): with the
+ * context's state (U0, S0, V0) at the start of the step from A[k_prev] to
+ * A[k_next], overwrite the cotangents (Ub, Sb, Vb) of the step's result
+ * (U1, S1, V1) (host, column-major n x r, r x r, m x r) with those of
+ * (U0, S0, V0).  The forward step is recomputed (up to U1, V1, their R
+ * factors, M and N); the state is left unchanged.  Four passes over the
+ * increment.  The reference has no adjoint (Julia AD would differentiate
+ * unconventional_step!); a ChainRulesCore rrule calls this per step (a
+ * checkpointed driver recomputes the states). */
+int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
+                         int64_t ldsb, double* Vb, int64_t ldvb);
+
 /* Greedy step on an SVD-like state (greedy_integrator.jl:94-104,106-111)
  * with X = A[k_next] itself (update_data!, data_integrator.jl:22-28):
  * U1 = orth(X V0), V1 = orth(X' U0), S1 = U1' X V1.  Three passes over X. */

@@ -36,6 +36,7 @@ from .api import (  # noqa: F401
     rk4_substeps,
     solve,
     solve_batch,
+    solve_gradient,
     step,
     truncated_svd,
     update_sol,

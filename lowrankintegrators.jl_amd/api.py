@@ -746,3 +746,48 @@ def solve_batch(probs, alg, dt, *, substeps=None, save_everystep: bool = False, 
         return sols
     finally:
         b.close()
+
+
+def solve_gradient(prob, alg, loss_grad, *, checkpoint: int = 20, precision: str = "f64",
+                   snapshot_dtype: Optional[str] = None, device: int = 0):
+    """Gradient of a loss of the final state of ``solve(prob, UnconventionalAlgorithm())`` on
+    array-form data with respect to the initial factors (U0, S0, V0), by the engine's
+    hand-written reverse step (``lri_step_bug_adjoint``) with checkpointed
+    recomputation: states are kept every ``checkpoint`` steps on the way forward and
+    recomputed segment by segment on the way back.
+
+    ``loss_grad(U, S, V) -> (value, Ub, Sb, Vb)`` gives the loss of the final factors
+    and its cotangents.  Returns ``(value, (Ub0, Sb0, Vb0))``.  (The reference has no
+    adjoint; in Julia this is the body of a ChainRulesCore ``rrule`` for ``solve``.)"""
+    if not isinstance(alg, UnconventionalAlgorithm):
+        raise TypeError("the adjoint is built for the unconventional (BUG) integrator")
+    if not isinstance(prob, MatrixDataProblem) or callable(prob.y):
+        raise TypeError("solve_gradient takes a MatrixDataProblem with array-form data")
+    if checkpoint < 1:
+        raise ValueError("checkpoint must be >= 1")
+    integ = init(prob, alg, 1, precision=precision, snapshot_dtype=snapshot_dtype, save_everystep=False,
+                 device=device)
+    ctx = integ.ctx
+    try:
+        t0, tf = prob.tspan
+        T = tf - t0  # steps k -> k + 1 over snapshot indices t0 - 1 .. tf - 1
+        k0 = t0 - 1
+        cps = {0: ctx.get_state()}
+        for s_ in range(T):
+            ctx.step_bug(k0 + s_, k0 + s_ + 1)
+            if (s_ + 1) % checkpoint == 0 and s_ + 1 < T:
+                cps[s_ + 1] = ctx.get_state()
+        value, Ub, Sb, Vb = loss_grad(*ctx.get_state())
+        for c0 in sorted(cps, reverse=True):
+            c1 = min(c0 + checkpoint, T)
+            states = [cps[c0]]
+            ctx.set_state(*cps[c0])
+            for s_ in range(c0, c1 - 1):
+                ctx.step_bug(k0 + s_, k0 + s_ + 1)
+                states.append(ctx.get_state())
+            for s_ in range(c1 - 1, c0 - 1, -1):
+                ctx.set_state(*states[s_ - c0])
+                Ub, Sb, Vb = ctx.step_bug_adjoint(k0 + s_, k0 + s_ + 1, Ub, Sb, Vb)
+        return value, (Ub, Sb, Vb)
+    finally:
+        ctx.close()

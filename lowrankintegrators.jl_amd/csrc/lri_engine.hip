@@ -135,6 +135,7 @@ struct lri_ctx {
   int* r_host = nullptr;  // pinned: the rank of a rank-adaptive step
   int ra_k2 = 0;
   bool defer_rank_copy = false;
+  BugAdjSmall* adj = nullptr;  // r x r work of the BUG adjoint (allocated on first use)
   // captured rank-adaptive DE steps (up to the SVD), one per rank and step parameters
   struct RaGraph {
     int64_t r, nsub, rmax;
@@ -672,6 +673,212 @@ struct lri_ctx {
     B.ldo = n;
     set_pack_outputs(B, false, pass_mode(), rb);
     rows(B);
+  }
+
+  // ------------------------------------------------------------- adjoint
+  // Reverse of unconventional_step! on data (unconventional.jl:133-157): with the
+  // state (U0, S0, V0) of the step and the cotangents (Ub, Sb, Vb) of its result
+  // (U1, S1, V1) in P[:, :r], adj->Sb1 and Q[:, :r], leaves the cotangents of
+  // (U0, S0, V0) in P[:, :r], adj->Sb0 and Q[:, :r].  The forward step is
+  // recomputed up to U1, V1, the R factors, M and N (S1 is not needed); then
+  //   Mb = Sb1 N S0', Nb = Sb1' M S0, Sb0 = M' Sb1 N,
+  //   Ub1 += dA V1 Sb1' + U0 Mb',  Vb1 += dA' U1 Sb1 + V0 Nb',
+  //   Kbar = (Ub1 + U1 copyltu(-Ub1'U1)) RK^{-T},  Lbar likewise,
+  //   Ub0 = U1 Mb + Kbar S0' + dA Lbar,  Vb0 = V1 Nb + Lbar S0 + dA' Kbar,
+  //   Sb0 += U0' Kbar + Lbar' V0.
+  // Four passes over dA (two forward, two reverse).
+  void bug_adjoint(int64_t kp, int64_t kn) {
+    const int rr = (int)r;
+    const int mode = pass_mode();
+    const int rb = rb_for(mode, rr);
+    const size_t rsz = sizeof(double) * rr * rr;
+    // forward: K1 = U0 S0 + dA V0 -> U1, RK ; L1 = V0 S0' + dA' U0 -> V1, RL ; M, N
+    ensure_packed(true);
+    ensure_packed(false);
+    const int sN = passN(kp, kn, rr, partN);
+    const int sT = passT(kp, kn, rr, partT);
+    with_parts(partN, [&](auto A) {
+      A.rows = n;
+      A.nsplit = sN;
+      A.rb = rb;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = U;
+      A.ldx = n;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.out = K;
+      A.ldo = n;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs eM;
+    eM.kind = EPI_BUG_M;  // M = U1' U0
+    cholqr_tail(K, K2, n, rr, U2, false, U, rr, eM, 0);
+    HIPCHECK(hipMemcpyAsync(adj->RK, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    with_parts(partT, [&](auto A) {
+      A.rows = m;
+      A.nsplit = sT;
+      A.rb = rb;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = V;
+      A.ldx = m;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.transY = 1;
+      A.out = L;
+      A.ldo = m;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs eNs;
+    eNs.kind = EPI_COPY;  // N = V1' V0
+    eNs.dst = small->N;
+    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eNs, 0);
+    HIPCHECK(hipMemcpyAsync(adj->RL, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    timed(LRI_KERNEL_SMALL, [&] { return launch_bug_adj_pre(adj, small->M, small->N, S, (int)rcap, rr, stream); });
+    double* Ub = P;           // in: Ub1, out: Ub0
+    double* Ubf = P + n * rr;  // Ub1 + dA V1 Sb1' + U0 Mb'
+    double* Vb = Q;
+    double* Vbf = Q + m * rr;
+    // pass operands: V1 Sb1' (N side), U1 Sb1 (T side)
+    for (int side = 0; side < 2; ++side) {
+      RowArgs<double> A;
+      A.rows = side == 0 ? m : n;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = side == 0 ? V2 : U2;
+      A.ldx = A.rows;
+      A.Y = adj->Sb1;
+      A.ldy = rr;
+      A.transY = side == 0;
+      set_pack_outputs(A, side == 0, mode, rb);
+      rows(A);
+    }
+    const int sN1 = passN(kp, kn, rr, partN);  // dA V1 Sb1'
+    const int sT1 = passT(kp, kn, rr, partT);  // dA' U1 Sb1
+    with_parts(partN, [&](auto A) {  // Ubf = parts + Ub + U0 Mb' ; C = Ubf' U1 -> CqK
+      A.rows = n;
+      A.nsplit = sN1;
+      A.rb = rb;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.base = Ub;
+      A.ld_base = n;
+      A.X = U;
+      A.ldx = n;
+      A.Y = adj->Mb;
+      A.ldy = rr;
+      A.transY = 1;
+      A.out = Ubf;
+      A.ldo = n;
+      A.reduce = 1;
+      A.W = U2;
+      A.ldw = n;
+      A.rw = rr;
+      A.epi.kind = EPI_COPYLTU;
+      A.epi.dst = adj->CqK;
+      rows(A);
+    });
+    with_parts(partT, [&](auto A) {  // Vbf = parts + Vb + V0 Nb' ; C = Vbf' V1 -> CqL
+      A.rows = m;
+      A.nsplit = sT1;
+      A.rb = rb;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.base = Vb;
+      A.ld_base = m;
+      A.X = V;
+      A.ldx = m;
+      A.Y = adj->Nb;
+      A.ldy = rr;
+      A.transY = 1;
+      A.out = Vbf;
+      A.ldo = m;
+      A.reduce = 1;
+      A.W = V2;
+      A.ldw = m;
+      A.rw = rr;
+      A.epi.kind = EPI_COPYLTU;
+      A.epi.dst = adj->CqL;
+      rows(A);
+    });
+    // Kbar = (Ubf + U1 CqK) RK^{-T} (packed for pass T; C = Kbar' U0 -> CK), Lbar likewise
+    for (int side = 0; side < 2; ++side) {
+      const int64_t rows_ = side == 0 ? n : m;
+      double* Z = side == 0 ? K2 : L2;
+      double* bar = side == 0 ? K : L;
+      RowArgs<double> A;
+      A.rows = rows_;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.base = side == 0 ? Ubf : Vbf;
+      A.ld_base = rows_;
+      A.X = side == 0 ? U2 : V2;
+      A.ldx = rows_;
+      A.Y = side == 0 ? adj->CqK : adj->CqL;
+      A.ldy = rr;
+      A.out = Z;
+      A.ldo = rows_;
+      rows(A);
+      RowArgs<double> B;
+      B.rows = rows_;
+      B.r_out = rr;
+      B.r_in = rr;
+      B.X = Z;
+      B.ldx = rows_;
+      B.Y = side == 0 ? adj->RKinvT : adj->RLinvT;
+      B.ldy = rr;
+      B.out = bar;
+      B.ldo = rows_;
+      set_pack_outputs(B, side == 1, mode, rb);  // Kbar for pass T, Lbar for pass N
+      B.reduce = 1;
+      B.W = side == 0 ? U : V;
+      B.ldw = rows_;
+      B.rw = rr;
+      B.epi.kind = EPI_COPY;
+      B.epi.dst = side == 0 ? adj->CK : adj->CL;
+      rows(B);
+    }
+    const int sN2 = passN(kp, kn, rr, partN);  // dA Lbar
+    const int sT2 = passT(kp, kn, rr, partT);  // dA' Kbar
+    for (int side = 0; side < 2; ++side) {
+      const int64_t rows_ = side == 0 ? n : m;
+      double* T0 = (side == 0 ? K2 : L2) + rows_ * rr;
+      with_parts(side == 0 ? partN : partT, [&](auto A) {  // T0 = parts + (U1 Mb | V1 Nb)
+        A.rows = rows_;
+        A.nsplit = side == 0 ? sN2 : sT2;
+        A.rb = rb;
+        A.r_out = rr;
+        A.r_in = rr;
+        A.X = side == 0 ? U2 : V2;
+        A.ldx = rows_;
+        A.Y = side == 0 ? adj->Mb : adj->Nb;
+        A.ldy = rr;
+        A.out = T0;
+        A.ldo = rows_;
+        rows(A);
+      });
+      RowArgs<double> B;  // (Ub0 | Vb0) = T0 + (Kbar S0' | Lbar S0)
+      B.rows = rows_;
+      B.r_out = rr;
+      B.r_in = rr;
+      B.base = T0;
+      B.ld_base = rows_;
+      B.X = side == 0 ? K : L;
+      B.ldx = rows_;
+      B.Y = S;
+      B.ldy = (int)rcap;
+      B.transY = side == 0;
+      B.out = side == 0 ? Ub : Vb;
+      B.ldo = rows_;
+      rows(B);
+    }
+    timed(LRI_KERNEL_SMALL, [&] { return launch_bug_adj_post(adj, rr, stream); });
+    pkN_mode = pkT_mode = -1;  // the panels hold adjoint operands now
   }
 
   // Orthonormal completion of span(X) against an orthonormal B: Q_perp with
@@ -1604,6 +1811,7 @@ void free_ctx(lri_ctx* c) {
     for (void* p : {(void*)c->specQ[1], (void*)c->specLam[1]})
       if (p) (void)hipFree(p);
   if (c->r_host) (void)hipHostFree(c->r_host);
+  if (c->adj) (void)hipFree(c->adj);
   for (auto& g : c->ra_graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& t : c->pending) {
     (void)hipEventDestroy(t.a);
@@ -1895,6 +2103,29 @@ int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next) {
     ctx->check_step(k_prev, k_next);
     ctx->bug(k_prev, k_next);
     return LRI_OK;
+  });
+}
+
+int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
+                         int64_t ldsb, double* Vb, int64_t ldvb) {
+  return guarded(ctx, [&] {
+    ctx->check_step(k_prev, k_next);
+    REQUIRE(Ub && Sb && Vb, "Ub, Sb and Vb are required");
+    const int64_t r = ctx->r, n = ctx->n, m = ctx->m;
+    REQUIRE(ldub >= n && ldvb >= m && ldsb >= r, "leading dimensions");
+    if (!ctx->adj) {
+      HIPCHECK(hipMalloc((void**)&ctx->adj, sizeof(BugAdjSmall)));
+      HIPCHECK(hipMemset(ctx->adj, 0, sizeof(BugAdjSmall)));
+    }
+    const size_t d = sizeof(double);
+    HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyHostToDevice, ctx->stream));
+    ctx->bug_adjoint(k_prev, k_next);
+    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->adj->Sb0, d * r, d * r, r, hipMemcpyDeviceToHost, ctx->stream));
+    return ctx->read_flags();
   });
 }
 

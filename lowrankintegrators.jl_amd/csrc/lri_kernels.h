@@ -40,6 +40,7 @@ enum EpiKind {
   EPI_STORE,   // Ctmp = C
   EPI_COPY,    // dst = C (a x b; transpose: C')
   EPI_CHOL3,   // third factor of shifted CholeskyQR3: C = Gram -> R3^{-1} (into R2inv), Rtot = R3 Rtot; if S: S = Rtot
+  EPI_COPYLTU, // dst = copyltu(-C) = tril(-C) + tril(-C, -1)' (a x a, ld a): the thin-QR adjoint's coefficient
 };
 
 struct EpiArgs {
@@ -121,6 +122,22 @@ struct SvdArgs {
   double* work = nullptr;  // global ke x ke scratch, used when it does not fit in LDS
 };
 hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st);
+// Reverse (adjoint) of the data-driven BUG step, r x r parts (one workgroup each).
+struct BugAdjSmall {
+  double RK[kMaxRank * kMaxRank], RL[kMaxRank * kMaxRank];          // R of the QRs of K1 and L1
+  double RKinvT[kMaxRank * kMaxRank], RLinvT[kMaxRank * kMaxRank];  // their inverse transposes
+  double Sb1[kMaxRank * kMaxRank];                                  // cotangent of S1
+  double Mb[kMaxRank * kMaxRank], Nb[kMaxRank * kMaxRank];          // of M = U1'U0, N = V1'V0
+  double Sb0[kMaxRank * kMaxRank];                                  // M' Sb1 N, then the full cotangent of S0
+  double CK[kMaxRank * kMaxRank], CL[kMaxRank * kMaxRank];          // Kbar' U0, Lbar' V0
+  double CqK[kMaxRank * kMaxRank], CqL[kMaxRank * kMaxRank];        // copyltu(-Ubar1' U1), copyltu(-Vbar1' V1)
+};
+// Mb = Sb1 N S0', Nb = Sb1' M S0, Sb0 = M' Sb1 N, RK^{-T}, RL^{-T} (r x r, ld r; S0 ld lds)
+hipError_t launch_bug_adj_pre(BugAdjSmall* A, const double* M, const double* N, const double* S0, int lds, int r,
+                              hipStream_t st);
+// Sb0 += CK' + CL
+hipError_t launch_bug_adj_post(BugAdjSmall* A, int r, hipStream_t st);
+
 // C = A B' for k x k matrices (ld k), k <= kMaxAug, one workgroup.
 hipError_t launch_small_abt(const double* A, const double* B, int k, double* C, hipStream_t st);
 // two independent decompositions in one launch (one workgroup each)

@@ -447,6 +447,13 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
       }
       break;
     }
+    case EPI_COPYLTU: {
+      for (int e = t; e < a * a; e += blockDim.x) {
+        const int i = e % a, j = e / a;
+        E.dst[e] = (i >= j) ? -C[i + j * a] : -C[j + i * a];
+      }
+      break;
+    }
     case EPI_STORE: {
       for (int e = t; e < a * b; e += blockDim.x) st->Ctmp[e] = C[e];
       break;
@@ -1111,6 +1118,75 @@ __global__ __launch_bounds__(kThreads) void k_small_abt(const double* __restrict
 hipError_t launch_small_abt(const double* A, const double* B, int k, double* C, hipStream_t st) {
   if (k < 1 || k > kMaxAug) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_small_abt, dim3(1), dim3(kThreads), 0, st, A, B, k, C);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// r x r parts of the BUG adjoint
+// ---------------------------------------------------------------------------
+
+// X = R^{-T} for upper triangular R (r x r, ld r): thread j solves R x = e_j by
+// back substitution (column j of R^{-1}), stored as row j of R^{-T}.
+__device__ void inv_transpose_upper(const double* R, int r, double* X) {
+  for (int j = threadIdx.x; j < r; j += blockDim.x) {
+    double x[kMaxRank];
+    for (int i = r - 1; i >= 0; --i) {
+      double v = (i == j) ? 1.0 : 0.0;
+      for (int l = i + 1; l < r; ++l) v -= R[i + l * r] * x[l];
+      x[i] = (i > j) ? 0.0 : v / R[i + i * r];
+    }
+    for (int i = 0; i < r; ++i) X[j + i * r] = x[i];  // (R^{-1})_{ij} -> (R^{-T})_{ji}
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_bug_adj_pre(BugAdjSmall* A, const double* M, const double* N,
+                                                          const double* S0, int lds, int r) {
+  __shared__ double T1[kMaxRank * kMaxRank], T2[kMaxRank * kMaxRank];
+  const int t = threadIdx.x, rr = r * r;
+  for (int e = t; e < rr; e += blockDim.x) {  // T1 = Sb1 N, T2 = M S0
+    const int i = e % r, j = e / r;
+    double a = 0.0, b = 0.0;
+    for (int l = 0; l < r; ++l) {
+      a += A->Sb1[i + l * r] * N[l + j * r];
+      b += M[i + l * r] * S0[l + j * lds];
+    }
+    T1[e] = a;
+    T2[e] = b;
+  }
+  __syncthreads();
+  for (int e = t; e < rr; e += blockDim.x) {
+    const int i = e % r, j = e / r;
+    double mb = 0.0, nb = 0.0, sb = 0.0;
+    for (int l = 0; l < r; ++l) {
+      mb += T1[i + l * r] * S0[j + l * lds];    // (Sb1 N) S0'
+      nb += A->Sb1[l + i * r] * T2[l + j * r];  // Sb1' (M S0)
+      sb += M[l + i * r] * T1[l + j * r];       // M' (Sb1 N)
+    }
+    A->Mb[e] = mb;
+    A->Nb[e] = nb;
+    A->Sb0[e] = sb;
+  }
+  inv_transpose_upper(A->RK, r, A->RKinvT);
+  inv_transpose_upper(A->RL, r, A->RLinvT);
+}
+
+__global__ __launch_bounds__(kThreads) void k_bug_adj_post(BugAdjSmall* A, int r) {
+  for (int e = threadIdx.x; e < r * r; e += blockDim.x) {
+    const int i = e % r, j = e / r;
+    A->Sb0[e] += A->CK[j + i * r] + A->CL[e];
+  }
+}
+
+hipError_t launch_bug_adj_pre(BugAdjSmall* A, const double* M, const double* N, const double* S0, int lds, int r,
+                              hipStream_t st) {
+  if (r < 1 || r > kMaxRank) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_bug_adj_pre, dim3(1), dim3(kThreads), 0, st, A, M, N, S0, lds, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_bug_adj_post(BugAdjSmall* A, int r, hipStream_t st) {
+  if (r < 1 || r > kMaxRank) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_bug_adj_post, dim3(1), dim3(kThreads), 0, st, A, r);
   return hipGetLastError();
 }
 

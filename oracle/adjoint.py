@@ -1,0 +1,91 @@
+"""Reverse-mode (adjoint) restatement of the data-driven BUG step.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+The reference has no adjoint: its integrators are differentiated, if at all,
+by Julia AD through ``unconventional_step!`` (``src/integrators/unconventional.jl:133-157``).
+This module is the CPU reference gradient for the engine's hand-written
+reverse step: the vector-Jacobian product of
+
+    K1 = U0 S0 + dA V0,   U1 = Q(K1)           (unconventional.jl:137-142)
+    L1 = V0 S0' + dA' U0, V1 = Q(L1)           (:145-150)
+    M = U1' U0, N = V1' V0
+    S1 = M S0 N' + U1' dA V1                   (:154-156)
+
+with the thin-QR rule for Q = Q(K), K = Q R, R unused:
+    Kbar = (Qbar + Q copyltu(-Qbar' Q)) R^{-T},  copyltu(X) = tril(X) + tril(X, -1)'.
+
+Pinning (``tests/test_oracle_adjoint.py``): central finite differences of the
+forward step (``dlra.bug_step``) along random directions, one step and
+several steps; the QR rule separately against finite differences of
+``numpy.linalg.qr``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .dlra import _qr
+
+
+def copyltu(X):
+    return np.tril(X) + np.tril(X, -1).T
+
+
+def qr_vjp(Q, R, Qbar):
+    """Cotangent of K for Q = Q(K) (thin QR, R not used downstream)."""
+    M = -Qbar.T @ Q
+    return np.linalg.solve(R, (Qbar + Q @ copyltu(M)).T).T  # (.) R^{-T}
+
+
+def bug_step_vjp(U, S, V, dA, Ub1, Sb1, Vb1):
+    """Cotangents (Ub0, Sb0, Vb0, dAb) of the BUG data step's inputs, given those
+    of its outputs (U1, S1, V1); recomputes the forward step (same QR as dlra)."""
+    K = U @ S + dA @ V
+    U1, RK = _qr(K)
+    L = V @ S.T + dA.T @ U
+    V1, RL = _qr(L)
+    M = U1.T @ U
+    N = V1.T @ V
+    # S1 = M S N' + U1' dA V1
+    Mb = Sb1 @ N @ S.T
+    Nb = Sb1.T @ M @ S
+    Sb0 = M.T @ Sb1 @ N
+    Ub1 = Ub1 + dA @ V1 @ Sb1.T + U @ Mb.T
+    Vb1 = Vb1 + dA.T @ U1 @ Sb1 + V @ Nb.T
+    dAb = U1 @ Sb1 @ V1.T
+    Ub0 = U1 @ Mb
+    Vb0 = V1 @ Nb
+    # QR backward
+    Kb = qr_vjp(U1, RK, Ub1)
+    Lb = qr_vjp(V1, RL, Vb1)
+    # K = U S + dA V ; L = V S' + dA' U
+    Ub0 = Ub0 + Kb @ S.T + dA @ Lb
+    Sb0 = Sb0 + U.T @ Kb + Lb.T @ V
+    Vb0 = Vb0 + dA.T @ Kb + Lb @ S
+    dAb = dAb + Kb @ V.T + U @ Lb.T
+    return Ub0, Sb0, Vb0, dAb
+
+
+def bug_solve_gradient(U0, S0, V0, snaps, loss_grad, checkpoint=20):
+    """Gradient of loss(U_T, S_T, V_T) after the BUG steps over snaps[0..T] with
+    respect to (U0, S0, V0): checkpoints every `checkpoint` steps on the way
+    forward, recomputation inside each segment on the way back.
+    loss_grad(U, S, V) -> (value, Ub, Sb, Vb)."""
+    from .dlra import bug_step
+
+    T = len(snaps) - 1
+    cps = {0: (U0, S0, V0)}
+    U, S, V = U0, S0, V0
+    for k in range(T):
+        U, S, V = bug_step(U, S, V, snaps[k + 1] - snaps[k])
+        if (k + 1) % checkpoint == 0 and k + 1 < T:
+            cps[k + 1] = (U, S, V)
+    value, Ub, Sb, Vb = loss_grad(U, S, V)
+    for c0 in sorted(cps, reverse=True):
+        c1 = min(c0 + checkpoint, T)
+        states = [cps[c0]]
+        for k in range(c0, c1 - 1):
+            states.append(bug_step(*states[-1], snaps[k + 1] - snaps[k]))
+        for k in range(c1 - 1, c0 - 1, -1):
+            Ub, Sb, Vb, _ = bug_step_vjp(*states[k - c0], snaps[k + 1] - snaps[k], Ub, Sb, Vb)
+    return value, Ub, Sb, Vb
