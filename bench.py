@@ -185,6 +185,79 @@ def cpu_baseline(n, r, steps=3):
     }
 
 
+# ---------------------------------------------------------------------------- adjoint workload
+
+
+def run_adjoint(args, torch, lri, snaps, U0, S0, V0, n, r):
+    """Reverse BUG steps (lri_step_bug_adjoint) over the BF16 snapshots, mixed precision:
+    each recomputes the forward step up to U1, V1 (2 passes) and makes 4 reverse passes."""
+    N = len(snaps)
+    rng = np.random.default_rng(args.seed + 7)
+    Ub = rng.standard_normal((n, r))
+    Sb = rng.standard_normal((r, r))
+    Vb = rng.standard_normal((n, r))
+    ctx = lri.native.Context(n, n, r, 0)
+    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, n, owners=snaps)
+    ctx.set_precision(lri.native.PREC_MIXED)
+    ctx.set_state(U0, S0, V0)
+    steps = min(args.steps, 100)
+    for k in range(3):
+        ctx.step_bug_adjoint(k, k + 1, Ub, Sb, Vb)
+    ctx.timing(0 if args.no_events else 2)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        kk = k % (N - 1)
+        ctx.step_bug_adjoint(kk, kk + 1, Ub, Sb, Vb)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    fam = {}
+    for name, f in (("pass_N", lri.native.KERNEL_PASS_N), ("pass_T", lri.native.KERNEL_PASS_T)):
+        ms, cnt = ctx.timing_read(f)
+        fam[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt if cnt else None}
+    ctx.timing(0)
+    # forward BUG steps on the same context for the ratio
+    ctx.set_state(U0, S0, V0)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for k in range(steps):
+        kk = k % (N - 1)
+        ctx.step_bug(kk, kk + 1)
+    ctx.synchronize()
+    fwd = steps / (time.perf_counter() - t1)
+    ctx.close()
+    value = steps / el
+    log(f"adjoint: {value:.1f} reverse steps/s ({1e3 * el / steps:.3f} ms); forward BUG {fwd:.1f} steps/s")
+    dom = max((k for k in fam if fam[k]["avg_ms"]), key=lambda k: fam[k]["avg_ms"], default=None)
+    bytes_per_launch = 2.0 * n * n * 2
+    achieved = bytes_per_launch / (fam[dom]["avg_ms"] * 1e-3) / 1e9 if dom else None
+    out = {
+        "metric": "reverse (adjoint) BUG steps/s at n=m=16384, r=32, BF16 snapshots, mixed precision",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 3,
+        "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"lri_step_bug_adjoint over {N} BF16 snapshots at n=m={n}, r={r} (host cotangents "
+                               f"in and out each step: 2 (n + m) r + r^2 doubles over PCIe)", "n": n, "r": r},
+        "forward_bug_steps_per_s": fwd,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None, "kernel": dom,
+                     "bytes_per_launch": bytes_per_launch},
+        "kernels": fam,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 # ---------------------------------------------------------------------------- Greedy workload
 
 
@@ -625,10 +698,11 @@ def main():
                     help="no HIP-event kernel timing in the timed region (for external profilers)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--workload", choices=["ksl", "de", "greedy", "burgers", "batched"], default="ksl",
+    ap.add_argument("--workload", choices=["ksl", "de", "greedy", "burgers", "batched", "adjoint"], default="ksl",
                     help="ksl: the headline data integrator; de: the BUG matrix-ODE step at n=m=4096; "
                          "greedy: the Greedy data integrator on the headline snapshots; burgers: BUG on the "
-                         "reference's Burgers problem; batched: 512 uneven rank-adaptive DE solves")
+                         "reference's Burgers problem; batched: 512 uneven rank-adaptive DE solves; adjoint: "
+                         "reverse BUG steps on the headline snapshots")
     ap.add_argument("--de-n", type=int, default=4096)
     ap.add_argument("--de-steps", type=int, default=20)
     ap.add_argument("--warmup-de", type=int, default=2)
@@ -685,6 +759,9 @@ def main():
 
     if args.workload == "greedy":
         run_greedy(args, torch, lri, snaps, U0, S0, V0, n, r)
+        return
+    if args.workload == "adjoint":
+        run_adjoint(args, torch, lri, snaps, U0, S0, V0, n, r)
         return
 
     ctx = lri.native.Context(n, m, r, 0)
