@@ -201,11 +201,38 @@ hipError_t launch_quad_stage(const QuadStageArgs& Q, hipStream_t st) {
 // S k x l in LDS, Y (k x l x k) and Z (l x l x k) in global scratch; one workgroup.
 // ----------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_quad_sstep(QuadSArgs Q) {
-  __shared__ double y0[kQuadMaxRank * kQuadMaxRank], ys[kQuadMaxRank * kQuadMaxRank];
-  __shared__ double acc[kQuadMaxRank * kQuadMaxRank], cb[kQuadMaxRank * kQuadMaxRank];
+// kLds: the state, its RK4 work, the tensors W, T and the contractions Y, Z all
+// in LDS (quad_s_lds_doubles(k, l) <= kQuadSLds), else Y, Z in Q.work and W, T read
+// from global memory.
+constexpr int kQuadSLds = 17408;  // 136 KB
+
+__host__ __device__ inline int quad_s_lds_doubles(int k, int l) {
+  return 4 * k * l + k * l * k + l * l * k + k * k * k + l * l * l;
+}
+
+template <bool kLds>
+__global__ __launch_bounds__(1024) void k_quad_sstep(QuadSArgs Q) {
+  __shared__ double st_small[kLds ? 1 : 4 * kQuadMaxRank * kQuadMaxRank];
+  extern __shared__ double dyn[];
   const int k = Q.k, l = Q.l, t = threadIdx.x, nt = blockDim.x;
   const int kl = k * l;
+  double* base = kLds ? dyn : st_small;
+  double* y0 = base;
+  double* ys = y0 + kl;
+  double* acc = ys + kl;
+  double* cb = acc + kl;
+  double* Y = kLds ? cb + kl : Q.work;           // k x l x k: Y[c + k (e + k b)]
+  double* Z = Y + k * l * k;                     // l x l x k: Z[c + k (a + l b)]
+  const double* W = Q.W;
+  const double* T = Q.T;
+  if (kLds && Q.W) {
+    double* w = Z + l * l * k;
+    double* tt = w + k * k * k;
+    for (int e = t; e < k * k * k; e += nt) w[e] = Q.W[e];
+    for (int e = t; e < l * l * l; e += nt) tt[e] = Q.T[e];
+    W = w;
+    T = tt;
+  }
   for (int e = t; e < kl; e += nt) {
     const int i = e % k, j = e / k;
     y0[e] = Q.transS0 ? Q.S0[j + i * Q.lds0] : Q.S0[i + j * Q.lds0];
@@ -214,8 +241,6 @@ __global__ __launch_bounds__(256) void k_quad_sstep(QuadSArgs Q) {
     cb[e] = s;
   }
   __syncthreads();
-  double* Y = Q.work;          // k x l x k: Y[c + k (e + k b)]
-  double* Z = Y + k * l * k;   // l x l x k: Z[c + k (a + l b)]
   const double h = Q.h;
   for (int64_t step = 0; step < Q.nsub; ++step) {
     for (int e = t; e < kl; e += nt) ys[e] = y0[e];
@@ -225,7 +250,7 @@ __global__ __launch_bounds__(256) void k_quad_sstep(QuadSArgs Q) {
         for (int e = t; e < k * l * k; e += nt) {  // Y_ebc = sum_f S_fb W_efc
           const int c = e % k, ee = (e / k) % k, b = e / (k * k);
           double v = 0.0;
-          for (int f = 0; f < k; ++f) v += ys[f + b * k] * Q.W[c + k * (ee + k * f)];
+          for (int f = 0; f < k; ++f) v += ys[f + b * k] * W[c + k * (ee + k * f)];
           Y[e] = v;
         }
         __syncthreads();
@@ -237,7 +262,7 @@ __global__ __launch_bounds__(256) void k_quad_sstep(QuadSArgs Q) {
         }
         __syncthreads();
       }
-      double fv[(kQuadMaxRank * kQuadMaxRank + 255) / 256];
+      double fv[(kQuadMaxRank * kQuadMaxRank + 1023) / 1024];
       int cnt = 0;
       for (int e = t; e < kl; e += nt, ++cnt) {
         const int i = e % k, j = e / k;
@@ -247,7 +272,7 @@ __global__ __launch_bounds__(256) void k_quad_sstep(QuadSArgs Q) {
         if (Q.W) {
           double p = 0.0;  // P_ij = sum_ab Z_abi T_abj, T[j + l (a + l b)]
           for (int b = 0; b < l; ++b)
-            for (int a = 0; a < l; ++a) p += Z[i + k * (a + l * b)] * Q.T[j + l * (a + l * b)];
+            for (int a = 0; a < l; ++a) p += Z[i + k * (a + l * b)] * T[j + l * (a + l * b)];
           v -= p;
         }
         fv[cnt] = Q.sgn * v;
@@ -272,7 +297,11 @@ __global__ __launch_bounds__(256) void k_quad_sstep(QuadSArgs Q) {
 hipError_t launch_quad_sstep(const QuadSArgs& Q, hipStream_t st) {
   if (Q.k < 1 || Q.l < 1 || Q.k > kQuadMaxRank || Q.l > kQuadMaxRank || Q.q < 0 || Q.q > kMaxForce || !Q.work)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_quad_sstep, dim3(1), dim3(256), 0, st, Q);
+  const int need = quad_s_lds_doubles(Q.k, Q.l);
+  if (need <= kQuadSLds)
+    hipLaunchKernelGGL(k_quad_sstep<true>, dim3(1), dim3(1024), sizeof(double) * need, st, Q);
+  else
+    hipLaunchKernelGGL(k_quad_sstep<false>, dim3(1), dim3(1024), 0, st, Q);
   return hipGetLastError();
 }
 
