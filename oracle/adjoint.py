@@ -17,8 +17,10 @@ with the thin-QR rule for Q = Q(K), K = Q R, R unused:
 
 Pinning (``tests/test_oracle_adjoint.py``): central finite differences of the
 forward step (``dlra.bug_step``) along random directions, one step and
-several steps; the QR rule separately against finite differences of
-``numpy.linalg.qr``.
+several steps; the QR rules separately against finite differences of
+``numpy.linalg.qr``; and a forward-mode (tangent) restatement of the step,
+``bug_step_jvp``, derived independently, against the reverse one through
+<ybar, J xdot> = <J' ybar, xdot>.
 """
 from __future__ import annotations
 
@@ -64,6 +66,36 @@ def bug_step_vjp(U, S, V, dA, Ub1, Sb1, Vb1):
     Vb0 = Vb0 + dA.T @ Kb + Lb @ S
     dAb = dAb + Kb @ V.T + U @ Lb.T
     return Ub0, Sb0, Vb0, dAb
+
+
+def qr_jvp(Q, R, Kd):
+    """Tangents (Qd, Rd) of the thin QR K = Q R along Kd (forward mode):
+    C = Q' Kd R^{-1}, X = Rd R^{-1} upper triangular with Q'Qd = C - X antisymmetric."""
+    Ri = np.linalg.inv(R)
+    C = Q.T @ Kd @ Ri
+    X = np.triu(C) + np.triu(C.T, 1)
+    Qd = Q @ (C - X) + (Kd @ Ri - Q @ (Q.T @ Kd @ Ri))
+    return Qd, X @ R
+
+
+def bug_step_jvp(U, S, V, dA, Ud, Sd, Vd, dAd):
+    """Forward-mode tangents (U1d, S1d, V1d) of the BUG data step along (Ud, Sd, Vd, dAd):
+    an independent derivation of the same derivative as bug_step_vjp (the two agree
+    through <ybar, J xdot> = <J' ybar, xdot>)."""
+    K = U @ S + dA @ V
+    Kd = Ud @ S + U @ Sd + dAd @ V + dA @ Vd
+    U1, RK = _qr(K)
+    U1d, _ = qr_jvp(U1, RK, Kd)
+    L = V @ S.T + dA.T @ U
+    Ld = Vd @ S.T + V @ Sd.T + dAd.T @ U + dA.T @ Ud
+    V1, RL = _qr(L)
+    V1d, _ = qr_jvp(V1, RL, Ld)
+    M, N = U1.T @ U, V1.T @ V
+    Md = U1d.T @ U + U1.T @ Ud
+    Nd = V1d.T @ V + V1.T @ Vd
+    S1d = (Md @ S @ N.T + M @ Sd @ N.T + M @ S @ Nd.T
+           + U1d.T @ dA @ V1 + U1.T @ dAd @ V1 + U1.T @ dA @ V1d)
+    return U1d, S1d, V1d
 
 
 def bug_solve_gradient(U0, S0, V0, snaps, loss_grad, checkpoint=20):

@@ -74,3 +74,32 @@ def test_checkpointed_gradient_against_finite_differences():
     g1 = adjoint.bug_solve_gradient(u.U, u.S, u.V, snaps, linear_loss(C), checkpoint=1)
     for a, b in zip(g1[1:], (Ub, Sb, Vb)):
         assert np.allclose(a, b, rtol=0, atol=1e-12 * np.abs(b).max())
+
+
+def test_qr_jvp_against_finite_differences():
+    rng = np.random.default_rng(3)
+    K = rng.standard_normal((25, 4))
+    Kd = rng.standard_normal(K.shape)
+    Q, R = np.linalg.qr(K)
+    Qd, Rd = adjoint.qr_jvp(Q, R, Kd)
+    e = 1e-6
+    Qp, Rp = np.linalg.qr(K + e * Kd)
+    Qm, Rm = np.linalg.qr(K - e * Kd)
+    assert np.abs((Qp - Qm) / (2 * e) - Qd).max() < 1e-7
+    assert np.abs((Rp - Rm) / (2 * e) - Rd).max() < 1e-7
+
+
+def test_forward_mode_agrees_with_reverse_mode():
+    """<ybar, J xdot> (forward mode) = <J' ybar, xdot> (reverse mode), for random
+    tangents and cotangents: the CPU forward-mode reference for the reverse step."""
+    rng = np.random.default_rng(4)
+    n, m, r = 45, 33, 5
+    u = random_factors(rng, n, m, r, decay=0.4)
+    dA = 1e-1 * rng.standard_normal((n, m))
+    xd = _dirs(rng, (n, r), (r, r), (m, r), (n, m))
+    yb = _dirs(rng, (n, r), (r, r), (m, r))
+    jv = adjoint.bug_step_jvp(u.U, u.S, u.V, dA, *xd)
+    vj = adjoint.bug_step_vjp(u.U, u.S, u.V, dA, *yb)
+    lhs = sum(np.sum(a * b) for a, b in zip(yb, jv))
+    rhs = sum(np.sum(a * b) for a, b in zip(vj, xd))
+    assert abs(lhs - rhs) < 1e-11 * max(1.0, abs(lhs))
