@@ -153,6 +153,12 @@ int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, in
 int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
                          int64_t ldsb, double* Vb, int64_t ldvb);
 
+/* The same for the primal KSL data step (projector_splitting.jl:129-152,
+ * 191-196): both QR factors carry cotangents there (S1 = R2', S~ = R1 - U1'dA V0).
+ * Four passes over the increment. */
+int lri_step_ksl_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
+                         int64_t ldsb, double* Vb, int64_t ldvb);
+
 /* Greedy step on an SVD-like state (greedy_integrator.jl:94-104,106-111)
  * with X = A[k_next] itself (update_data!, data_integrator.jl:22-28):
  * U1 = orth(X V0), V1 = orth(X' U0), S1 = U1' X V1.  Three passes over X. */

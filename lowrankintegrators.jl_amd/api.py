@@ -750,17 +750,21 @@ def solve_batch(probs, alg, dt, *, substeps=None, save_everystep: bool = False, 
 
 def solve_gradient(prob, alg, loss_grad, *, checkpoint: int = 20, precision: str = "f64",
                    snapshot_dtype: Optional[str] = None, device: int = 0):
-    """Gradient of a loss of the final state of ``solve(prob, UnconventionalAlgorithm())`` on
-    array-form data with respect to the initial factors (U0, S0, V0), by the engine's
-    hand-written reverse step (``lri_step_bug_adjoint``) with checkpointed
+    """Gradient of a loss of the final state of ``solve(prob, alg)`` on array-form data
+    (``alg``: ``UnconventionalAlgorithm()`` or ``ProjectorSplitting(PrimalLieTrotter())``)
+    with respect to the initial factors (U0, S0, V0), by the engine's hand-written
+    reverse steps (``lri_step_bug_adjoint`` / ``lri_step_ksl_adjoint``) with checkpointed
     recomputation: states are kept every ``checkpoint`` steps on the way forward and
     recomputed segment by segment on the way back.
 
     ``loss_grad(U, S, V) -> (value, Ub, Sb, Vb)`` gives the loss of the final factors
     and its cotangents.  Returns ``(value, (Ub0, Sb0, Vb0))``.  (The reference has no
     adjoint; in Julia this is the body of a ChainRulesCore ``rrule`` for ``solve``.)"""
-    if not isinstance(alg, UnconventionalAlgorithm):
-        raise TypeError("the adjoint is built for the unconventional (BUG) integrator")
+    ksl = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, PrimalLieTrotter)
+    if not (ksl or isinstance(alg, UnconventionalAlgorithm)):
+        raise TypeError("the adjoint is built for the unconventional (BUG) and primal KSL integrators")
+    fwd = (lambda c, a, b: c.step_ksl(native.PRIMAL, a, b)) if ksl else (lambda c, a, b: c.step_bug(a, b))
+    rev = (lambda c, *x: c.step_ksl_adjoint(*x)) if ksl else (lambda c, *x: c.step_bug_adjoint(*x))
     if not isinstance(prob, MatrixDataProblem) or callable(prob.y):
         raise TypeError("solve_gradient takes a MatrixDataProblem with array-form data")
     if checkpoint < 1:
@@ -774,7 +778,7 @@ def solve_gradient(prob, alg, loss_grad, *, checkpoint: int = 20, precision: str
         k0 = t0 - 1
         cps = {0: ctx.get_state()}
         for s_ in range(T):
-            ctx.step_bug(k0 + s_, k0 + s_ + 1)
+            fwd(ctx, k0 + s_, k0 + s_ + 1)
             if (s_ + 1) % checkpoint == 0 and s_ + 1 < T:
                 cps[s_ + 1] = ctx.get_state()
         value, Ub, Sb, Vb = loss_grad(*ctx.get_state())
@@ -783,11 +787,11 @@ def solve_gradient(prob, alg, loss_grad, *, checkpoint: int = 20, precision: str
             states = [cps[c0]]
             ctx.set_state(*cps[c0])
             for s_ in range(c0, c1 - 1):
-                ctx.step_bug(k0 + s_, k0 + s_ + 1)
+                fwd(ctx, k0 + s_, k0 + s_ + 1)
                 states.append(ctx.get_state())
             for s_ in range(c1 - 1, c0 - 1, -1):
                 ctx.set_state(*states[s_ - c0])
-                Ub, Sb, Vb = ctx.step_bug_adjoint(k0 + s_, k0 + s_ + 1, Ub, Sb, Vb)
+                Ub, Sb, Vb = rev(ctx, k0 + s_, k0 + s_ + 1, Ub, Sb, Vb)
         return value, (Ub, Sb, Vb)
     finally:
         ctx.close()

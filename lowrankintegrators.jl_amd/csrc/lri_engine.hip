@@ -881,6 +881,229 @@ struct lri_ctx {
     pkN_mode = pkT_mode = -1;  // the panels hold adjoint operands now
   }
 
+  // Reverse of primal_LT_step! on data (projector_splitting.jl:129-152), same
+  // conventions as bug_adjoint.  Forward recompute: P = dA V0, K = U0 S0 + P ->
+  // (U1, R1), S~ = R1 - U1'P, L = V0 S~' + dA'U1 -> (V1, R2); S1 = R2'.  Reverse
+  // (both QR factors carry cotangents: Kbar = (Qbar + Q copyltu(R Rbar' - Qbar'Q)) R^{-T}):
+  //   Lbar from (Vb1, R2bar = Sb1');  S~bar = Lbar' V0;  Vb0 = Lbar S~ + dA'(Kbar - U1 S~bar);
+  //   Ub1 += dA Lbar - P S~bar';  Kbar from (Ub1, R1bar = S~bar);  Ub0 = Kbar S0';  Sb0 = U0'Kbar.
+  void ksl_adjoint(int64_t kp, int64_t kn) {
+    const int rr = (int)r;
+    const int mode = pass_mode();
+    const int rb = rb_for(mode, rr);
+    const size_t rsz = sizeof(double) * rr * rr;
+    ensure_packed(true);
+    const int sN = passN(kp, kn, rr, partN);
+    with_parts(partN, [&](auto A) {  // K = U S + dA V ; P = dA V ; Gram -> R1
+      A.rows = n;
+      A.nsplit = sN;
+      A.rb = rb;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.sum_out = P;
+      A.ld_sum = n;
+      A.X = U;
+      A.ldx = n;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.out = K;
+      A.ldo = n;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs e;
+    e.kind = EPI_STILDE;  // S~ = R - U1' P
+    cholqr_tail(K, K2, n, rr, U2, false, P, rr, e, 0);  // U1 (packed for pass T)
+    HIPCHECK(hipMemcpyAsync(adj->RK, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(adj->St, small->St, rsz, hipMemcpyDeviceToDevice, stream));
+    const int sT = passT(kp, kn, rr, partT);
+    with_parts(partT, [&](auto A) {  // L = V S~' + dA' U1 ; Gram -> R1
+      A.rows = m;
+      A.nsplit = sT;
+      A.rb = rb;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = V;
+      A.ldx = m;
+      A.Y = small->St;
+      A.ldy = rr;
+      A.transY = 1;
+      A.out = L;
+      A.ldo = m;
+      A.reduce = 1;
+      A.epi.kind = EPI_CHOL1;
+      rows(A);
+    });
+    EpiArgs none;
+    cholqr_tail(L, L2, m, rr, V2, true, nullptr, 0, none, 0);  // V1
+    HIPCHECK(hipMemcpyAsync(adj->RL, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    timed(LRI_KERNEL_SMALL, [&] { return launch_ksl_adj_small(adj, rr, 0, stream); });
+    double* Ub = P + n * rr;   // P[:, :r] holds dA V0; the cotangents sit in the next r columns
+    double* Vb = Q;
+    double* Vbf = Q + m * rr;
+    (void)Vbf;
+    // CqL = copyltu(R2 Sb1 - Vb1' V1)
+    {
+      RowArgs<double> A;
+      A.rows = m;
+      A.r_out = rr;
+      A.base = Vb;
+      A.ld_base = m;
+      A.reduce = 1;
+      A.W = V2;
+      A.ldw = m;
+      A.rw = rr;
+      A.epi.kind = EPI_COPYLTU;
+      A.epi.S = adj->Mb;
+      A.epi.ld_s = rr;
+      A.epi.dst = adj->CqL;
+      rows(A);
+    }
+    // Lbar = (Vb1 + V1 CqL) R2^{-T}, packed for pass N; C = Lbar' V0 = S~bar -> CL
+    {
+      RowArgs<double> A;
+      A.rows = m;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.base = Vb;
+      A.ld_base = m;
+      A.X = V2;
+      A.ldx = m;
+      A.Y = adj->CqL;
+      A.ldy = rr;
+      A.out = L2;
+      A.ldo = m;
+      rows(A);
+      RowArgs<double> B;
+      B.rows = m;
+      B.r_out = rr;
+      B.r_in = rr;
+      B.X = L2;
+      B.ldx = m;
+      B.Y = adj->RLinvT;
+      B.ldy = rr;
+      B.out = L;
+      B.ldo = m;
+      set_pack_outputs(B, true, mode, rb);
+      B.reduce = 1;
+      B.W = V;
+      B.ldw = m;
+      B.rw = rr;
+      B.epi.kind = EPI_COPY;
+      B.epi.dst = adj->CL;
+      rows(B);
+    }
+    timed(LRI_KERNEL_SMALL, [&] { return launch_ksl_adj_small(adj, rr, 1, stream); });  // Nb = R1 S~bar'
+    const int sN1 = passN(kp, kn, rr, partN);  // dA Lbar
+    with_parts(partN, [&](auto A) {  // Ubf = parts + Ub1 - P S~bar' ; CqK = copyltu(R1 S~bar' - Ubf' U1)
+      A.rows = n;
+      A.nsplit = sN1;
+      A.rb = rb;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.base = Ub;
+      A.ld_base = n;
+      A.X = P;
+      A.ldx = n;
+      A.Y = adj->CL;
+      A.ldy = rr;
+      A.transY = 1;
+      A.xy_sign = -1.0;
+      A.out = K2;
+      A.ldo = n;
+      A.reduce = 1;
+      A.W = U2;
+      A.ldw = n;
+      A.rw = rr;
+      A.epi.kind = EPI_COPYLTU;
+      A.epi.S = adj->Nb;
+      A.epi.ld_s = rr;
+      A.epi.dst = adj->CqK;
+      rows(A);
+    });
+    // Kbar = (Ubf + U1 CqK) R1^{-T} -> K ; C = Kbar' U0 -> CK
+    {
+      RowArgs<double> A;
+      A.rows = n;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.base = K2;
+      A.ld_base = n;
+      A.X = U2;
+      A.ldx = n;
+      A.Y = adj->CqK;
+      A.ldy = rr;
+      A.out = K2 + n * rr;
+      A.ldo = n;
+      rows(A);
+      RowArgs<double> B;
+      B.rows = n;
+      B.r_out = rr;
+      B.r_in = rr;
+      B.X = K2 + n * rr;
+      B.ldx = n;
+      B.Y = adj->RKinvT;
+      B.ldy = rr;
+      B.out = K;
+      B.ldo = n;
+      B.reduce = 1;
+      B.W = U;
+      B.ldw = n;
+      B.rw = rr;
+      B.epi.kind = EPI_COPY;
+      B.epi.dst = adj->CK;
+      rows(B);
+    }
+    // pass T operand Kbar - U1 S~bar
+    {
+      RowArgs<double> A;
+      A.rows = n;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.base = K;
+      A.ld_base = n;
+      A.X = U2;
+      A.ldx = n;
+      A.Y = adj->CL;
+      A.ldy = rr;
+      A.xy_sign = -1.0;
+      set_pack_outputs(A, false, mode, rb);
+      rows(A);
+    }
+    const int sT1 = passT(kp, kn, rr, partT);  // dA'(Kbar - U1 S~bar)
+    with_parts(partT, [&](auto A) {  // Vb0 = parts + Lbar S~
+      A.rows = m;
+      A.nsplit = sT1;
+      A.rb = rb;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = L;
+      A.ldx = m;
+      A.Y = adj->St;
+      A.ldy = rr;
+      A.out = Vb;
+      A.ldo = m;
+      rows(A);
+    });
+    {
+      RowArgs<double> A;  // Ub0 = Kbar S0'
+      A.rows = n;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = K;
+      A.ldx = n;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.transY = 1;
+      A.out = Ub;
+      A.ldo = n;
+      rows(A);
+    }
+    timed(LRI_KERNEL_SMALL, [&] { return launch_ksl_adj_small(adj, rr, 2, stream); });  // Sb0 = CK'
+    pkN_mode = pkT_mode = -1;
+  }
+
   // Orthonormal completion of span(X) against an orthonormal B: Q_perp with
   // [B | Q_perp] orthonormal (dependent directions become zero columns).
   // X = first kernel's output (rows x rr), block Gram-Schmidt twice, then CholeskyQR2.
@@ -2106,8 +2329,11 @@ int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next) {
   });
 }
 
-int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
-                         int64_t ldsb, double* Vb, int64_t ldvb) {
+namespace {
+// the host side of the reverse steps: cotangents in, step, cotangents out; the
+// KSL step keeps dA V0 in P[:, :r], so its cotangent of U sits in P[:, r:2r]
+int adjoint_entry(lri_ctx* ctx, bool ksl, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
+                  int64_t ldsb, double* Vb, int64_t ldvb) {
   return guarded(ctx, [&] {
     ctx->check_step(k_prev, k_next);
     REQUIRE(Ub && Sb && Vb, "Ub, Sb and Vb are required");
@@ -2118,15 +2344,30 @@ int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* U
       HIPCHECK(hipMemset(ctx->adj, 0, sizeof(BugAdjSmall)));
     }
     const size_t d = sizeof(double);
-    HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r, hipMemcpyHostToDevice, ctx->stream));
+    double* ubuf = ksl ? ctx->P + n * r : ctx->P;
+    HIPCHECK(hipMemcpy2DAsync(ubuf, d * n, Ub, d * ldub, d * n, r, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyHostToDevice, ctx->stream));
-    ctx->bug_adjoint(k_prev, k_next);
-    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDeviceToHost, ctx->stream));
+    if (ksl)
+      ctx->ksl_adjoint(k_prev, k_next);
+    else
+      ctx->bug_adjoint(k_prev, k_next);
+    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ubuf, d * n, d * n, r, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->adj->Sb0, d * r, d * r, r, hipMemcpyDeviceToHost, ctx->stream));
     return ctx->read_flags();
   });
+}
+}  // namespace
+
+int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
+                         int64_t ldsb, double* Vb, int64_t ldvb) {
+  return adjoint_entry(ctx, false, k_prev, k_next, Ub, ldub, Sb, ldsb, Vb, ldvb);
+}
+
+int lri_step_ksl_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
+                         int64_t ldsb, double* Vb, int64_t ldvb) {
+  return adjoint_entry(ctx, true, k_prev, k_next, Ub, ldub, Sb, ldsb, Vb, ldvb);
 }
 
 int lri_step_greedy(lri_ctx* ctx, int64_t k_next) {

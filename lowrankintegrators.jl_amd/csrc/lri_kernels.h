@@ -40,7 +40,8 @@ enum EpiKind {
   EPI_STORE,   // Ctmp = C
   EPI_COPY,    // dst = C (a x b; transpose: C')
   EPI_CHOL3,   // third factor of shifted CholeskyQR3: C = Gram -> R3^{-1} (into R2inv), Rtot = R3 Rtot; if S: S = Rtot
-  EPI_COPYLTU, // dst = copyltu(-C) = tril(-C) + tril(-C, -1)' (a x a, ld a): the thin-QR adjoint's coefficient
+  EPI_COPYLTU, // dst = copyltu(X - C), X = S (ld ld_s) or 0, copyltu(Y) = tril(Y) + tril(Y, -1)' (a x a, ld a):
+               // the thin-QR adjoint's coefficient
 };
 
 struct EpiArgs {
@@ -131,12 +132,16 @@ struct BugAdjSmall {
   double Sb0[kMaxRank * kMaxRank];                                  // M' Sb1 N, then the full cotangent of S0
   double CK[kMaxRank * kMaxRank], CL[kMaxRank * kMaxRank];          // Kbar' U0, Lbar' V0
   double CqK[kMaxRank * kMaxRank], CqL[kMaxRank * kMaxRank];        // copyltu(-Ubar1' U1), copyltu(-Vbar1' V1)
+  double St[kMaxRank * kMaxRank];                                   // KSL: S~ of the forward step
 };
 // Mb = Sb1 N S0', Nb = Sb1' M S0, Sb0 = M' Sb1 N, RK^{-T}, RL^{-T} (r x r, ld r; S0 ld lds)
 hipError_t launch_bug_adj_pre(BugAdjSmall* A, const double* M, const double* N, const double* S0, int lds, int r,
                               hipStream_t st);
 // Sb0 += CK' + CL
 hipError_t launch_bug_adj_post(BugAdjSmall* A, int r, hipStream_t st);
+// Reverse of the primal KSL step, r x r parts: stage 0: RK^{-T}, RL^{-T}, Mb = RL Sb1 (= R2 R2bar');
+// stage 1: Nb = RK CL' (= R1 R1bar', R1bar = St bar = CL); stage 2: Sb0 = CK'.
+hipError_t launch_ksl_adj_small(BugAdjSmall* A, int r, int stage, hipStream_t st);
 
 // C = A B' for k x k matrices (ld k), k <= kMaxAug, one workgroup.
 hipError_t launch_small_abt(const double* A, const double* B, int k, double* C, hipStream_t st);

@@ -450,7 +450,8 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
     case EPI_COPYLTU: {
       for (int e = t; e < a * a; e += blockDim.x) {
         const int i = e % a, j = e / a;
-        E.dst[e] = (i >= j) ? -C[i + j * a] : -C[j + i * a];
+        const int p = i >= j ? i : j, q = i >= j ? j : i;  // the lower-triangle entry (p >= q)
+        E.dst[e] = (E.S ? E.S[p + q * E.ld_s] : 0.0) - C[p + q * a];
       }
       break;
     }
@@ -1175,6 +1176,35 @@ __global__ __launch_bounds__(kThreads) void k_bug_adj_post(BugAdjSmall* A, int r
     const int i = e % r, j = e / r;
     A->Sb0[e] += A->CK[j + i * r] + A->CL[e];
   }
+}
+
+__global__ __launch_bounds__(kThreads) void k_ksl_adj_small(BugAdjSmall* A, int r, int stage) {
+  const int rr = r * r;
+  if (stage == 0) {
+    inv_transpose_upper(A->RK, r, A->RKinvT);
+    inv_transpose_upper(A->RL, r, A->RLinvT);
+    for (int e = threadIdx.x; e < rr; e += blockDim.x) {  // Mb = RL Sb1
+      const int i = e % r, j = e / r;
+      double v = 0.0;
+      for (int l = 0; l < r; ++l) v += A->RL[i + l * r] * A->Sb1[l + j * r];
+      A->Mb[e] = v;
+    }
+  } else if (stage == 1) {
+    for (int e = threadIdx.x; e < rr; e += blockDim.x) {  // Nb = RK CL'
+      const int i = e % r, j = e / r;
+      double v = 0.0;
+      for (int l = 0; l < r; ++l) v += A->RK[i + l * r] * A->CL[j + l * r];
+      A->Nb[e] = v;
+    }
+  } else {
+    for (int e = threadIdx.x; e < rr; e += blockDim.x) A->Sb0[e] = A->CK[(e / r) + (e % r) * r];
+  }
+}
+
+hipError_t launch_ksl_adj_small(BugAdjSmall* A, int r, int stage, hipStream_t st) {
+  if (r < 1 || r > kMaxRank || stage < 0 || stage > 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ksl_adj_small, dim3(1), dim3(kThreads), 0, st, A, r, stage);
+  return hipGetLastError();
 }
 
 hipError_t launch_bug_adj_pre(BugAdjSmall* A, const double* M, const double* N, const double* S0, int lds, int r,

@@ -50,6 +50,7 @@ SIGNATURES = {
     "lri_step_bug": (_int, [_vp, _i64, _i64]),
     "lri_step_bug_ra": (_int, [_vp, _i64, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
     "lri_step_bug_adjoint": (_int, [_vp, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
+    "lri_step_ksl_adjoint": (_int, [_vp, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
     "lri_step_greedy": (_int, [_vp, _i64]),
     "lri_step_greedy_twofactor": (_int, [_vp, _i64]),
     "lri_set_operator_lyapunov": (_int, [_vp, _dp, _dp, _int, _dp, _dp, _int, _i64, _dp, _i64, _dp, _i64]),
@@ -219,11 +220,18 @@ class Context:
 
     def step_bug_adjoint(self, kp: int, kn: int, Ub, Sb, Vb):
         """Cotangents of the BUG step's inputs from those of its result (host arrays)."""
+        return self._adjoint(lib().lri_step_bug_adjoint, kp, kn, Ub, Sb, Vb)
+
+    def step_ksl_adjoint(self, kp: int, kn: int, Ub, Sb, Vb):
+        """Cotangents of the primal KSL step's inputs from those of its result (host arrays)."""
+        return self._adjoint(lib().lri_step_ksl_adjoint, kp, kn, Ub, Sb, Vb)
+
+    def _adjoint(self, fn, kp, kn, Ub, Sb, Vb):
         Ub = np.asfortranarray(Ub, dtype=np.float64).copy(order="F")
         Sb = np.asfortranarray(Sb, dtype=np.float64).copy(order="F")
         Vb = np.asfortranarray(Vb, dtype=np.float64).copy(order="F")
-        check(lib().lri_step_bug_adjoint(self.h, int(kp), int(kn), _dptr(Ub), Ub.shape[0], _dptr(Sb), Sb.shape[0],
-                                         _dptr(Vb), Vb.shape[0]), self.h)
+        check(fn(self.h, int(kp), int(kn), _dptr(Ub), Ub.shape[0], _dptr(Sb), Sb.shape[0], _dptr(Vb), Vb.shape[0]),
+              self.h)
         return Ub, Sb, Vb
 
     def step_greedy(self, kn: int):

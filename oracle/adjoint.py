@@ -39,6 +39,54 @@ def qr_vjp(Q, R, Qbar):
     return np.linalg.solve(R, (Qbar + Q @ copyltu(M)).T).T  # (.) R^{-T}
 
 
+def qr_vjp_full(Q, R, Qbar, Rbar):
+    """Cotangent of K for (Q, R) = QR(K) when both factors are used downstream:
+    Kbar = (Qbar + Q copyltu(R Rbar' - Qbar' Q)) R^{-T}."""
+    M = R @ Rbar.T - Qbar.T @ Q
+    return np.linalg.solve(R, (Qbar + Q @ copyltu(M)).T).T
+
+
+def ksl_primal_step_vjp(U, S, V, dA, Ub1, Sb1, Vb1):
+    """Cotangents (Ub0, Sb0, Vb0, dAb) of the primal KSL data step
+    (projector_splitting.jl:129-152):
+        P = dA V, K = U S + P -> (U1, R1);  St = R1 - U1' P;
+        L = V St' + dA' U1 -> (V1, R2);     S1 = R2'."""
+    P = dA @ V
+    U1, R1 = _qr(U @ S + P)
+    St = R1 - U1.T @ P
+    V1, R2 = _qr(V @ St.T + dA.T @ U1)
+    Lb = qr_vjp_full(V1, R2, Vb1, Sb1.T)
+    Vb0 = Lb @ St
+    Stb = Lb.T @ V
+    Ub1 = Ub1 + dA @ Lb - P @ Stb.T
+    dAb = U1 @ Lb.T
+    Pb = -U1 @ Stb
+    Kb = qr_vjp_full(U1, R1, Ub1, Stb)
+    Ub0 = Kb @ S.T
+    Sb0 = U.T @ Kb
+    Pb = Pb + Kb
+    Vb0 = Vb0 + dA.T @ Pb
+    dAb = dAb + Pb @ V.T
+    return Ub0, Sb0, Vb0, dAb
+
+
+def ksl_primal_step_jvp(U, S, V, dA, Ud, Sd, Vd, dAd):
+    """Forward-mode tangents (U1d, S1d, V1d) of the primal KSL data step."""
+    P = dA @ V
+    Pd = dAd @ V + dA @ Vd
+    K = U @ S + P
+    Kd = Ud @ S + U @ Sd + Pd
+    U1, R1 = _qr(K)
+    U1d, R1d = qr_jvp(U1, R1, Kd)
+    St = R1 - U1.T @ P
+    Std = R1d - U1d.T @ P - U1.T @ Pd
+    L = V @ St.T + dA.T @ U1
+    Ld = Vd @ St.T + V @ Std.T + dAd.T @ U1 + dA.T @ U1d
+    V1, R2 = _qr(L)
+    V1d, R2d = qr_jvp(V1, R2, Ld)
+    return U1d, R2d.T, V1d
+
+
 def bug_step_vjp(U, S, V, dA, Ub1, Sb1, Vb1):
     """Cotangents (Ub0, Sb0, Vb0, dAb) of the BUG data step's inputs, given those
     of its outputs (U1, S1, V1); recomputes the forward step (same QR as dlra)."""
@@ -98,18 +146,21 @@ def bug_step_jvp(U, S, V, dA, Ud, Sd, Vd, dAd):
     return U1d, S1d, V1d
 
 
-def bug_solve_gradient(U0, S0, V0, snaps, loss_grad, checkpoint=20):
-    """Gradient of loss(U_T, S_T, V_T) after the BUG steps over snaps[0..T] with
-    respect to (U0, S0, V0): checkpoints every `checkpoint` steps on the way
-    forward, recomputation inside each segment on the way back.
+def bug_solve_gradient(U0, S0, V0, snaps, loss_grad, checkpoint=20, ksl=False):
+    """Gradient of loss(U_T, S_T, V_T) after the BUG (ksl: primal KSL) steps over
+    snaps[0..T] with respect to (U0, S0, V0): checkpoints every `checkpoint` steps
+    on the way forward, recomputation inside each segment on the way back.
     loss_grad(U, S, V) -> (value, Ub, Sb, Vb)."""
-    from .dlra import bug_step
+    from .dlra import bug_step, ksl_primal_step
+
+    step = ksl_primal_step if ksl else bug_step
+    vjp = ksl_primal_step_vjp if ksl else bug_step_vjp
 
     T = len(snaps) - 1
     cps = {0: (U0, S0, V0)}
     U, S, V = U0, S0, V0
     for k in range(T):
-        U, S, V = bug_step(U, S, V, snaps[k + 1] - snaps[k])
+        U, S, V = step(U, S, V, snaps[k + 1] - snaps[k])
         if (k + 1) % checkpoint == 0 and k + 1 < T:
             cps[k + 1] = (U, S, V)
     value, Ub, Sb, Vb = loss_grad(U, S, V)
@@ -117,7 +168,7 @@ def bug_solve_gradient(U0, S0, V0, snaps, loss_grad, checkpoint=20):
         c1 = min(c0 + checkpoint, T)
         states = [cps[c0]]
         for k in range(c0, c1 - 1):
-            states.append(bug_step(*states[-1], snaps[k + 1] - snaps[k]))
+            states.append(step(*states[-1], snaps[k + 1] - snaps[k]))
         for k in range(c1 - 1, c0 - 1, -1):
-            Ub, Sb, Vb, _ = bug_step_vjp(*states[k - c0], snaps[k + 1] - snaps[k], Ub, Sb, Vb)
+            Ub, Sb, Vb, _ = vjp(*states[k - c0], snaps[k + 1] - snaps[k], Ub, Sb, Vb)
     return value, Ub, Sb, Vb

@@ -121,3 +121,58 @@ def test_gradient_against_cpu_forward_mode(lri):
         rev = sum(np.sum(a * d) for a, d in zip(g, D))
         say(f"  direction {trial}: forward mode {fwd:.10e}, engine reverse mode {rev:.10e}")
         assert abs(fwd - rev) <= 1e-10 * max(1.0, abs(fwd))
+
+
+@pytest.mark.parametrize("shape", [(64, 48, 4), (300, 200, 16)])
+def test_ksl_one_step_gradient_vs_oracle(lri, shape):
+    n, m, r = shape
+    say(f"case: KSL adjoint one step n={n} m={m} r={r}")
+    rng = np.random.default_rng(4)
+    u = random_factors(rng, n, m, r, decay=0.3)
+    A0 = rng.standard_normal((n, m))
+    A1 = A0 + 5e-2 * rng.standard_normal((n, m))
+    lg = linear_loss(rng.standard_normal((n, m)))
+    with lri.native.Context(n, m, r) as ctx:
+        ctx.reserve(2, lri.native.F64)
+        ctx.upload(0, np.asfortranarray(A0))
+        ctx.upload(1, np.asfortranarray(A1))
+        ctx.set_state(u.U, u.S, u.V)
+        ctx.step_ksl(lri.native.PRIMAL, 0, 1)
+        _, Ub1, Sb1, Vb1 = lg(*ctx.get_state())
+        ctx.set_state(u.U, u.S, u.V)
+        g = ctx.step_ksl_adjoint(0, 1, Ub1, Sb1, Vb1)
+    o1 = dlra.ksl_primal_step(u.U, u.S, u.V, A1 - A0)
+    _, oUb, oSb, oVb = lg(*o1)
+    go = adjoint.ksl_primal_step_vjp(u.U, u.S, u.V, A1 - A0, oUb, oSb, oVb)
+    errs = [rel(a, b) for a, b in zip(g, go[:3])]
+    say(f"  gradient rel err U {errs[0]:.2e} S {errs[1]:.2e} V {errs[2]:.2e}")
+    assert max(errs) <= 1e-9
+
+
+def test_ksl_checkpointed_solve_gradient(lri):
+    n, m, r, T = 150, 110, 5, 10
+    rng = np.random.default_rng(5)
+    u = random_factors(rng, n, m, r, decay=0.4)
+    snaps = [u.to_matrix()]
+    for _ in range(T):
+        snaps.append(snaps[-1] + 2e-2 * rng.standard_normal((n, m)))
+    lg = linear_loss(rng.standard_normal((n, m)))
+    y0 = lri.SVDLikeRepresentation(u.U, u.S, u.V)
+    alg = lri.ProjectorSplitting(lri.PrimalLieTrotter())
+    value, g = lri.solve_gradient(lri.MatrixDataProblem(snaps, y0), alg, lg, checkpoint=4)
+    ov, *og = adjoint.bug_solve_gradient(u.U, u.S, u.V, snaps, lg, checkpoint=4, ksl=True)
+    errs = [rel(a, b) for a, b in zip(g, og)]
+    say(f"  KSL loss {value:.6e} (oracle {ov:.6e}); gradient rel err {max(errs):.2e}")
+    assert abs(value - ov) <= 1e-10 * abs(ov) and max(errs) <= 1e-9
+    D = [rng.standard_normal(x.shape) for x in (u.U, u.S, u.V)]
+    e = 1e-6
+
+    def run(sign):
+        y = lri.SVDLikeRepresentation(u.U + sign * e * D[0], u.S + sign * e * D[1], u.V + sign * e * D[2])
+        Y = lri.solve(lri.MatrixDataProblem(snaps, y), alg).Y[-1]
+        return lg(Y.U, Y.S, Y.V)[0]
+
+    fd = (run(1.0) - run(-1.0)) / (2 * e)
+    an = sum(np.sum(a * d) for a, d in zip(g, D))
+    say(f"  directional derivative {an:.8e}, finite difference {fd:.8e}")
+    assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd))

@@ -103,3 +103,26 @@ def test_forward_mode_agrees_with_reverse_mode():
     lhs = sum(np.sum(a * b) for a, b in zip(yb, jv))
     rhs = sum(np.sum(a * b) for a, b in zip(vj, xd))
     assert abs(lhs - rhs) < 1e-11 * max(1.0, abs(lhs))
+
+
+def test_ksl_primal_vjp_against_finite_differences_and_forward_mode():
+    rng = np.random.default_rng(5)
+    n, m, r = 38, 29, 4
+    u = random_factors(rng, n, m, r, decay=0.5)
+    dA = 1e-1 * rng.standard_normal((n, m))
+    yb = _dirs(rng, (n, r), (r, r), (m, r))
+    g = adjoint.ksl_primal_step_vjp(u.U, u.S, u.V, dA, *yb)
+
+    def f(U, S, V, A):
+        U1, S1, V1 = dlra.ksl_primal_step(U, S, V, A)
+        return np.sum(yb[0] * U1) + np.sum(yb[1] * S1) + np.sum(yb[2] * V1)
+
+    D = _dirs(rng, (n, r), (r, r), (m, r), (n, m))
+    e = 1e-6
+    fd = (f(u.U + e * D[0], u.S + e * D[1], u.V + e * D[2], dA + e * D[3])
+          - f(u.U - e * D[0], u.S - e * D[1], u.V - e * D[2], dA - e * D[3])) / (2 * e)
+    an = sum(np.sum(a * d) for a, d in zip(g, D))
+    assert abs(fd - an) < 1e-6 * max(1.0, abs(fd))
+    jv = adjoint.ksl_primal_step_jvp(u.U, u.S, u.V, dA, *D)
+    lhs = sum(np.sum(a * b) for a, b in zip(yb, jv))
+    assert abs(lhs - an) < 1e-11 * max(1.0, abs(lhs))
