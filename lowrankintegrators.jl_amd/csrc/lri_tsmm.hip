@@ -735,6 +735,229 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_bf16x3(const uint16_t* _
       }
 }
 
+// ----------------------------------------------------------------------------
+// Single-operand forms (A itself, no previous snapshot): A is exact in BF16, so
+// its hi part is the raw value and its lo part vanishes (2 products per k-step
+// instead of 3), and the load slots the difference would use fetch the next
+// tile instead, keeping as many bytes in flight per wave as the two-operand
+// passes.  Pass N takes 32 columns per iteration, pass T 128 rows.
+// ----------------------------------------------------------------------------
+
+template <int TM, int CB>
+__global__ __launch_bounds__(kPassThreads) void k_passN_bf16_single(const uint16_t* __restrict__ an, int64_t lda,
+                                                                    int64_t n, int64_t m,
+                                                                    const uint16_t* __restrict__ Bhi,
+                                                                    const uint16_t* __restrict__ Blo, int rb,
+                                                                    int64_t jchunk, float* __restrict__ part,
+                                                                    int vec) {
+  using L = PassNLds<TM, CB>;
+  constexpr int LDC = L::LDC;
+  constexpr int NV = TM * CB * 16;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[L::BYTES];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 31, h = lane >> 5;
+  const int64_t i0 = (int64_t)blockIdx.x * (32 * TM);
+  const int s = blockIdx.y;
+  const int64_t jb = (int64_t)s * jchunk;
+  const int64_t je = min(m, jb + jchunk);
+  const WaveRange R = wave_range(jb, je, wave, 32);
+  const bool v = vec != 0;
+  uint16_t* th = reinterpret_cast<uint16_t*>(smem) + (wave * 2 + 0) * 16 * LDC;
+  f32x16_t acc[TM][CB];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[t][c] = f32x16_t{};
+  // tile g (0: columns j..j+15, 1: j+16..j+31) with its B fragments
+  u32x4_t xa[2][TM], nxa[2][TM], bh[2][CB], bl[2][CB], nbh[2][CB], nbl[2][CB];
+  auto load = [&](int64_t j, u32x4_t (&X)[2][TM], u32x4_t (&H)[2][CB], u32x4_t (&Lo)[2][CB]) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+#pragma unroll
+      for (int q = 0; q < TM; ++q) {
+        const int b = q * 64 + lane;
+        const int col = b / (4 * TM), oct = b % (4 * TM);
+        const int64_t jj = j + 16 * g + col;
+        const int64_t r0 = i0 + 8 * oct;
+        X[g][q] = load_raw8(an, jj * lda + r0, r0, n, jj < je, v);
+      }
+      // the second tile's fragments exist only inside the wave's range (the panel ends there)
+      const bool bok = j + 16 * g < R.e;
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        const int64_t boff = (((((j >> 4) + g) * 2 + h) * rb) + c * 32 + lc) * 8;
+        H[g][c] = bok ? *reinterpret_cast<const u32x4_t*>(Bhi + boff) : u32x4_t{0u, 0u, 0u, 0u};
+        Lo[g][c] = bok ? *reinterpret_cast<const u32x4_t*>(Blo + boff) : u32x4_t{0u, 0u, 0u, 0u};
+      }
+    }
+  };
+  if (R.b < R.e) load(R.b, xa, bh, bl);
+  for (int64_t j = R.b; j < R.e; j += 32) {
+    const bool more = j + 32 < R.e;
+    if (more) load(j + 32, nxa, nbh, nbl);
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+#pragma unroll
+      for (int q = 0; q < TM; ++q) {
+        const int b = q * 64 + lane;
+        const int col = b / (4 * TM), oct = b % (4 * TM);
+        uint2* ph = reinterpret_cast<uint2*>(th + col * LDC + 8 * oct);
+        ph[0] = uint2{xa[g][q][0], xa[g][q][1]};
+        ph[1] = uint2{xa[g][q][2], xa[g][q][3]};
+      }
+      wave_lds_fence();
+#pragma unroll
+      for (int t = 0; t < TM; ++t) {
+        const int row = 32 * t + lc;
+        u32x4_t ah;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c0 = 8 * h + 2 * q;
+          ah[q] = uint32_t(th[c0 * LDC + row]) | (uint32_t(th[(c0 + 1) * LDC + row]) << 16);
+        }
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          acc[t][c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bh[g][c]), acc[t][c]);
+          acc[t][c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bl[g][c]), acc[t][c]);
+        }
+      }
+      wave_lds_fence();
+    }
+    if (more) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+#pragma unroll
+        for (int q = 0; q < TM; ++q) xa[g][q] = nxa[g][q];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          bh[g][c] = nbh[g][c];
+          bl[g][c] = nbl[g][c];
+        }
+      }
+    }
+  }
+  float flat[NV];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int w = 0; w < 16; ++w) flat[(t * CB + c) * 16 + w] = acc[t][c][w];
+  wg_sum<float, NV>(flat, reinterpret_cast<float*>(smem), wave, lane);
+  if (wave != 0) return;
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int w = 0; w < 16; ++w) {
+        const int64_t ii = i0 + 32 * t + 8 * (w / 4) + 4 * h + (w % 4);
+        if (ii < n) part[((int64_t)s * rb + c * 32 + lc) * n + ii] = flat[(t * CB + c) * 16 + w];
+      }
+}
+
+template <int CB>
+__global__ __launch_bounds__(kPassThreads) void k_passT_bf16_single(const uint16_t* __restrict__ an, int64_t lda,
+                                                                    int64_t n, int64_t m,
+                                                                    const uint16_t* __restrict__ Bhi,
+                                                                    const uint16_t* __restrict__ Blo, int rb,
+                                                                    int64_t ichunk, float* __restrict__ part,
+                                                                    int vec) {
+  constexpr int NV = CB * 16;
+  constexpr int STAGE = 4 * 2 * 32 * kTPitch * 2;  // bytes
+  constexpr int RED = 3 * NV * 64 * 4;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[STAGE > RED ? STAGE : RED];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lc = lane & 31, h = lane >> 5;
+  const int64_t j0 = (int64_t)blockIdx.x * 32;
+  const int s = blockIdx.y;
+  const int64_t ib = (int64_t)s * ichunk;
+  const int64_t ie = min(n, ib + ichunk);
+  const WaveRange R = wave_range(ib, ie, wave, 128);
+  const bool v = vec != 0;
+  uint16_t* th = reinterpret_cast<uint16_t*>(smem) + (wave * 2 + 0) * 32 * kTPitch;
+  f32x16_t acc[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) acc[c] = f32x16_t{};
+  const int lcol = lane >> 3, loct = lane & 7;
+  // tile g: rows i + 64 g .. i + 64 g + 63, with the B fragments of its 64-row block
+  // (loaded only inside the wave's range: the panel ends there)
+  u32x4_t xa[2][4], nxa[2][4], bh[2][4][CB], bl[2][4][CB], nbh[2][4][CB], nbl[2][4][CB];
+  auto load = [&](int64_t i, u32x4_t (&X)[2][4], u32x4_t (&H)[2][4][CB], u32x4_t (&Lo)[2][4][CB]) {
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int64_t jj = j0 + 8 * p + lcol;
+        const int64_t r0 = i + 64 * g + 8 * loct;
+        X[g][p] = load_raw8(an, jj * lda + r0, r0, ie, jj < m, v);
+      }
+      const bool bok = i + 64 * g < R.e;
+      const int64_t blk = (i >> 6) + g;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          const int64_t off = ((((blk * 4 + q) * 2 + h) * rb) + c * 32 + lc) * 8;
+          H[g][q][c] = bok ? *reinterpret_cast<const u32x4_t*>(Bhi + off) : u32x4_t{0u, 0u, 0u, 0u};
+          Lo[g][q][c] = bok ? *reinterpret_cast<const u32x4_t*>(Blo + off) : u32x4_t{0u, 0u, 0u, 0u};
+        }
+    }
+  };
+  if (R.b < R.e) load(R.b, xa, bh, bl);
+  for (int64_t i = R.b; i < R.e; i += 128) {
+    const bool more = i + 128 < R.e;
+    if (more) load(i + 128, nxa, nbh, nbl);
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int col = 8 * p + lcol;
+        *reinterpret_cast<u32x4_t*>(th + col * kTPitch + 8 * loct) = xa[g][p];
+      }
+      wave_lds_fence();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const u32x4_t ah = *reinterpret_cast<const u32x4_t*>(th + lc * kTPitch + 32 * h + 8 * q);
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          acc[c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bh[g][q][c]), acc[c]);
+          acc[c] = mfma_bf16(as_bf16x8(ah), as_bf16x8(bl[g][q][c]), acc[c]);
+        }
+      }
+      wave_lds_fence();
+    }
+    if (more) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) xa[g][p] = nxa[g][p];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int c = 0; c < CB; ++c) {
+            bh[g][q][c] = nbh[g][q][c];
+            bl[g][q][c] = nbl[g][q][c];
+          }
+      }
+    }
+  }
+  float flat[NV];
+#pragma unroll
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int w = 0; w < 16; ++w) flat[c * 16 + w] = acc[c][w];
+  wg_sum<float, NV>(flat, reinterpret_cast<float*>(smem), wave, lane);
+  if (wave != 0) return;
+#pragma unroll
+  for (int c = 0; c < CB; ++c)
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      const int64_t jj = j0 + 8 * (w / 4) + 4 * h + (w % 4);
+      if (jj < m) part[((int64_t)s * rb + c * 32 + lc) * m + jj] = flat[c * 16 + w];
+    }
+}
+
 // ============================================================================
 // planning and launch
 // ============================================================================
@@ -820,6 +1043,15 @@ static hipError_t run_pass(int mode, bool transposed, const PassArgs& a, hipStre
     const uint16_t* an = (const uint16_t*)a.anext;
     const uint16_t* ap = (const uint16_t*)a.aprev;
     const int v = vec_ok(a) ? 1 : 0;
+    if (!ap) {
+      if (transposed)
+        hipLaunchKernelGGL((k_passT_bf16_single<CB>), grid, blk, 0, st, an, a.lda, a.n, a.m, (const uint16_t*)a.B,
+                           (const uint16_t*)a.B2, a.rb, a.chunk, (float*)a.part, v);
+      else
+        hipLaunchKernelGGL((k_passN_bf16_single<TM, CB>), grid, blk, 0, st, an, a.lda, a.n, a.m,
+                           (const uint16_t*)a.B, (const uint16_t*)a.B2, a.rb, a.chunk, (float*)a.part, v);
+      return hipGetLastError();
+    }
     if (transposed)
       hipLaunchKernelGGL((k_passT_bf16x3<CB>), grid, blk, 0, st, an, ap, a.lda, a.n, a.m,
                          (const uint16_t*)a.B, (const uint16_t*)a.B2, a.ldb, a.rb, a.chunk, (float*)a.part, v);
