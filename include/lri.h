@@ -224,6 +224,23 @@ int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps);
  * truncation and *r_new as lri_step_bug_ra (synchronising). */
 int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, int64_t rmax, int64_t* r_new);
 
+/* Reverse (adjoint) of lri_step_de_bug (unconventional.jl:133-164 with the
+ * sub-problem right-hand sides of :51-79): with the context's state
+ * (U0, S0, V0) at the start of the step, overwrite the cotangents (Ub, Sb, Vb)
+ * of the step's result (host, column-major n x r, r x r, m x r) with those of
+ * (U0, S0, V0), and add to scale_bar[0], scale_bar[1] (if not NULL) the
+ * derivatives along scales a, b of the two linear parts,
+ * F = a A Y + b Y B' + G H' - (D Y) o Y at a = b = 1 (A = nu Laplacian:
+ * dL/dnu = scale_bar[0] / nu).  The forward step is recomputed (keeping the RK4
+ * sub-step starts of the K and L sub-problems); the state is left unchanged.
+ * Needs the quadratic operator (lri_set_operator_quadratic; D = 0 for a linear
+ * right-hand side).  The derivative is that of the discrete step: the same RK4
+ * sub-steps and the QR with a positive diagonal of R.  The reference has no
+ * adjoint (Julia AD would differentiate unconventional_step!); a ChainRulesCore
+ * rrule calls this per step. */
+int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* Ub, int64_t ldub, double* Sb,
+                            int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar);
+
 /* ---- batches of independent problems -------------------------------------
  *
  * Many small solves of uneven n, m and rank (the reference runs them one

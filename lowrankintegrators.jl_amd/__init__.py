@@ -16,9 +16,13 @@ from .api import (  # noqa: F401
     LyapunovOperator,
     Matrix,
     QuadraticOperator,
+    SineBurgers,
     Tridiagonal,
     burgers_operator,
     burgers_samples,
+    de_substeps,
+    mean_column_loss,
+    periodic_burgers_operator,
     MatrixDataProblem,
     MatrixDEProblem,
     MatrixHybridProblem,

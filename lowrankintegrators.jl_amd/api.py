@@ -748,18 +748,29 @@ def solve_batch(probs, alg, dt, *, substeps=None, save_everystep: bool = False, 
         b.close()
 
 
-def solve_gradient(prob, alg, loss_grad, *, checkpoint: int = 20, precision: str = "f64",
-                   snapshot_dtype: Optional[str] = None, device: int = 0):
-    """Gradient of a loss of the final state of ``solve(prob, alg)`` on array-form data
-    (``alg``: ``UnconventionalAlgorithm()`` or ``ProjectorSplitting(PrimalLieTrotter())``)
-    with respect to the initial factors (U0, S0, V0), by the engine's hand-written
-    reverse steps (``lri_step_bug_adjoint`` / ``lri_step_ksl_adjoint``) with checkpointed
-    recomputation: states are kept every ``checkpoint`` steps on the way forward and
-    recomputed segment by segment on the way back.
+def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, precision: str = "f64",
+                   snapshot_dtype: Optional[str] = None, substeps: Optional[int] = None, device: int = 0):
+    """Gradient of a loss of the final state of ``solve(prob, alg[, dt])`` with respect to
+    the initial factors (U0, S0, V0), by the engine's hand-written reverse steps with
+    checkpointed recomputation: states are kept every ``checkpoint`` steps on the way
+    forward and recomputed segment by segment on the way back.
+
+    - array-form data (``MatrixDataProblem``), ``alg`` ``UnconventionalAlgorithm()`` or
+      ``ProjectorSplitting(PrimalLieTrotter())``: ``lri_step_bug_adjoint`` /
+      ``lri_step_ksl_adjoint``; returns ``(value, (Ub0, Sb0, Vb0))``;
+    - a matrix ODE (``MatrixDEProblem`` with a ``QuadraticOperator``; D = 0 for a linear
+      one) and ``UnconventionalAlgorithm()``: ``lri_step_de_bug_adjoint``, with ``dt`` and
+      the RK4 ``substeps`` as ``solve``; returns ``(value, (Ub0, Sb0, Vb0), (abar, bbar))``,
+      abar and bbar the derivatives along scales of the operator's A and B
+      (A = nu Laplacian: dL/dnu = abar / nu).
 
     ``loss_grad(U, S, V) -> (value, Ub, Sb, Vb)`` gives the loss of the final factors
-    and its cotangents.  Returns ``(value, (Ub0, Sb0, Vb0))``.  (The reference has no
-    adjoint; in Julia this is the body of a ChainRulesCore ``rrule`` for ``solve``.)"""
+    and its cotangents.  (The reference has no adjoint; in Julia this is the body of a
+    ChainRulesCore ``rrule`` for ``solve``.)"""
+    if checkpoint < 1:
+        raise ValueError("checkpoint must be >= 1")
+    if isinstance(prob, MatrixDEProblem):
+        return _solve_gradient_de(prob, alg, loss_grad, dt, checkpoint, substeps, device)
     ksl = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, PrimalLieTrotter)
     if not (ksl or isinstance(alg, UnconventionalAlgorithm)):
         raise TypeError("the adjoint is built for the unconventional (BUG) and primal KSL integrators")
@@ -767,8 +778,6 @@ def solve_gradient(prob, alg, loss_grad, *, checkpoint: int = 20, precision: str
     rev = (lambda c, *x: c.step_ksl_adjoint(*x)) if ksl else (lambda c, *x: c.step_bug_adjoint(*x))
     if not isinstance(prob, MatrixDataProblem) or callable(prob.y):
         raise TypeError("solve_gradient takes a MatrixDataProblem with array-form data")
-    if checkpoint < 1:
-        raise ValueError("checkpoint must be >= 1")
     integ = init(prob, alg, 1, precision=precision, snapshot_dtype=snapshot_dtype, save_everystep=False,
                  device=device)
     ctx = integ.ctx
@@ -795,3 +804,138 @@ def solve_gradient(prob, alg, loss_grad, *, checkpoint: int = 20, precision: str
         return value, (Ub, Sb, Vb)
     finally:
         ctx.close()
+
+
+def _de_step_count(tspan, dt) -> int:
+    """The number of steps ``solve`` takes (t accumulated, primitives.jl:71-74)."""
+    t0, tf = tspan
+    T = tf - t0
+    t, k = t0, 0
+    while (tf - t) / T > 1e-8:
+        t = t + dt
+        k += 1
+    return k
+
+
+def de_substeps(prob, dt, substeps: Optional[int] = None) -> int:
+    """The RK4 sub-steps ``solve`` uses for a matrix ODE with step ``dt``."""
+    if substeps is not None:
+        return int(substeps)
+    F, u0 = prob.f, prob.u0
+    if isinstance(F, QuadraticOperator):
+        ymax = float(np.max(np.linalg.norm(u0.U @ u0.S, axis=1)) * np.max(np.linalg.norm(u0.V, axis=1)))
+        return rk4_substeps(F.spectral_bound(2.0 * ymax), dt)
+    return rk4_substeps(F.spectral_bound(), dt)
+
+
+def _solve_gradient_de(prob, alg, loss_grad, dt, checkpoint, substeps, device):
+    if not isinstance(alg, UnconventionalAlgorithm):
+        raise TypeError("the matrix-ODE adjoint is built for the unconventional (BUG) integrator")
+    if not isinstance(prob.f, QuadraticOperator):
+        raise TypeError("the matrix-ODE adjoint takes a QuadraticOperator (D = 0 for a linear right-hand side)")
+    if dt is None:
+        raise AssertionError("If the data is not provided as array, integration stepsize needs to be specified")
+    N = de_substeps(prob, dt, substeps)
+    steps = _de_step_count(prob.tspan, dt)
+    integ = init(prob, alg, dt, save_everystep=False, device=device, substeps=N)
+    ctx = integ.ctx
+    try:
+        cps = {0: ctx.get_state()}
+        for s_ in range(steps):
+            ctx.step_de_bug(dt, N)
+            if (s_ + 1) % checkpoint == 0 and s_ + 1 < steps:
+                cps[s_ + 1] = ctx.get_state()
+        value, Ub, Sb, Vb = loss_grad(*ctx.get_state())
+        abar = bbar = 0.0
+        for c0 in sorted(cps, reverse=True):
+            c1 = min(c0 + checkpoint, steps)
+            states = [cps[c0]]
+            ctx.set_state(*cps[c0])
+            for _ in range(c0, c1 - 1):
+                ctx.step_de_bug(dt, N)
+                states.append(ctx.get_state())
+            for s_ in range(c1 - 1, c0 - 1, -1):
+                ctx.set_state(*states[s_ - c0])
+                Ub, Sb, Vb, (a, b) = ctx.step_de_bug_adjoint(dt, N, Ub, Sb, Vb)
+                abar += a
+                bbar += b
+        return value, (Ub, Sb, Vb), (abar, bbar)
+    finally:
+        ctx.close()
+
+
+# ----------------------------------------------------------------------------
+# the parametric Burgers problem of the differentiation workload
+# ----------------------------------------------------------------------------
+
+
+def _qr_pos(A):
+    Q, R = np.linalg.qr(A, mode="reduced")
+    s = np.sign(np.diag(R))
+    s[s == 0] = 1.0
+    return Q * s, R * s[:, None]
+
+
+def periodic_burgers_operator(n: int, m: int, nu: float) -> QuadraticOperator:
+    """nu u_xx - u u_x on n points x_i = i / n of the periodic [0, 1), central differences:
+    A = nu / dx^2 tridiag(1, -2, 1), D = 1 / (2 dx) tridiag(-1, 0, 1), both periodic;
+    m columns (samples), no coupling between them (B = 0)."""
+    dx = 1.0 / n
+    A = SymTridiagonal(-2.0 * nu / dx ** 2 * np.ones(n), nu / dx ** 2 * np.ones(n), periodic=True)
+    B = SymTridiagonal(np.zeros(m), np.zeros(m - 1))
+    g = 0.5 / dx
+    D = Tridiagonal(-g * np.ones(n), np.zeros(n), g * np.ones(n), periodic=True)
+    return QuadraticOperator(A, B, D)
+
+
+class SineBurgers:
+    """Parametric viscous Burgers with sine initial data (one sample per column):
+    u0(x; xi) = sum_{k=1..K} xi_k sin(2 pi k x) / k on n points of [0, 1), xi K x m,
+    viscosity nu, rank r >= K.  The rank-K initial matrix Y0 = Phi xi is represented
+    as U0 S0 V0' with U0 the normalised sines k = 1..r (orthonormal on the grid, and
+    independent of xi), V0 = Q([xi' | omega]) (the QR with a positive diagonal;
+    omega a fixed m x (r - K) Gaussian of ``seed``) and S0 = U0' Phi xi V0.
+
+    ``problem(tspan)`` is the ``MatrixDEProblem``; ``pullback(Ub0, Sb0, Vb0)`` maps
+    cotangents of (U0, S0, V0) to that of xi (U0 does not depend on xi)."""
+
+    def __init__(self, n: int, xi: np.ndarray, nu: float, r: int, seed: int = 0):
+        xi = np.asarray(xi, dtype=np.float64)
+        K, m = xi.shape
+        if not (K <= r <= m):
+            raise ValueError(f"need K = {K} <= r = {r} <= m = {m}")
+        self.n, self.m, self.r, self.nu, self.xi = n, m, r, float(nu), xi
+        x = np.arange(n) / n
+        self.U0 = np.stack([np.sin(2 * np.pi * k * x) for k in range(1, r + 1)], axis=1) * np.sqrt(2.0 / n)
+        Phi = np.stack([np.sin(2 * np.pi * k * x) / k for k in range(1, K + 1)], axis=1)
+        self.C = self.U0.T @ Phi  # r x K
+        self.omega = np.random.default_rng(seed).standard_normal((m, r - K))
+        self.V0, self.R = _qr_pos(np.hstack([xi.T, self.omega]))
+        self.S0 = self.C @ (xi @ self.V0)
+        self.op = periodic_burgers_operator(n, m, nu)
+
+    def initial(self) -> SVDLikeRepresentation:
+        return SVDLikeRepresentation(self.U0.copy(), self.S0.copy(), self.V0.copy())
+
+    def problem(self, tspan=(0.0, 0.02)) -> "MatrixDEProblem":
+        return MatrixDEProblem(self.op, self.initial(), tspan)
+
+    def pullback(self, Ub0, Sb0, Vb0) -> np.ndarray:
+        K = self.xi.shape[0]
+        xib = self.C.T @ Sb0 @ self.V0.T                  # S0 = C xi V0
+        Vb = Vb0 + self.xi.T @ (self.C.T @ Sb0)
+        Q, R = self.V0, self.R                            # V0 = Q([xi' | omega]), R not used
+        M = -Vb.T @ Q
+        Mlt = np.tril(M) + np.tril(M, -1).T
+        Ab = np.linalg.solve(R, (Vb + Q @ Mlt).T).T
+        return xib + Ab[:, :K].T
+
+
+def mean_column_loss(U, S, V):
+    """L = || mean over columns of U S V' ||^2 and its cotangents (Ub, Sb, Vb)."""
+    m = V.shape[0]
+    w = V.sum(axis=0) / m
+    y = S @ w
+    z = U @ y
+    Uz = U.T @ z
+    return float(z @ z), 2.0 * np.outer(z, y), 2.0 * np.outer(Uz, w), np.outer(np.ones(m), 2.0 * S.T @ Uz) / m

@@ -73,6 +73,23 @@ struct DeSmall {
   int rscr[2];
 };
 
+// Galerkin data, tensors and cotangents of the matrix-ODE BUG adjoint
+// (device-resident; k <= kQuadMaxRank, ld k; index 0: the step's start bases
+// U0, V0, index 1: the new bases U1, V1)
+constexpr int kQ2 = kQuadMaxRank * kQuadMaxRank;
+constexpr int kQ3 = kQ2 * kQuadMaxRank;
+constexpr int kQPart = kQ3 + kQ2 + kMaxForce * kQuadMaxRank + 1;
+struct QuadAdj {
+  double Ga0[kQ2], Gb0[kQ2], gaT0[kMaxForce * kQuadMaxRank], hbT0[kMaxForce * kQuadMaxRank];
+  double Gab1[kQ2], Gbb1[kQ2], gaTb1[kMaxForce * kQuadMaxRank], hbTb1[kMaxForce * kQuadMaxRank];
+  double Sib[kQ2];  // cotangent of the S sub-step's start M S0 N'
+  double scal[2];   // derivatives along the scales of A and B
+  double W0[kQ3], T0[kQ3], Wb1[kQ3], Tb1[kQ3];
+  // summed reverse-stage partials: K side [T0bar | Gb0bar | hbT0bar | <kbar, A X>],
+  // L side [W0bar | Ga0bar | gaT0bar | <lbar, B X>]
+  double sumK[kQPart], sumL[kQPart];
+};
+
 }  // namespace
 
 struct lri_ctx {
@@ -149,6 +166,11 @@ struct lri_ctx {
   double *op_dl = nullptr, *op_dd = nullptr, *op_du = nullptr;
   int op_dper = 0;
   double *quad_part = nullptr, *quad_T = nullptr, *quad_W = nullptr, *quad_work = nullptr;
+  // matrix-ODE adjoint (allocated on first use): r x r data and the row work
+  // (sub-step starts of K and L, stage inputs, cotangents, stage partials)
+  QuadAdj* qadj = nullptr;
+  double* qadj_rows = nullptr;
+  size_t qadj_rows_len = 0;
 
   // ------------------------------------------------------------------ helpers
   int pass_mode() const {
@@ -1640,44 +1662,62 @@ struct lri_ctx {
     A.ldo = rows_;
     rows(A);
   }
-  // N RK4 steps of the K (nside: M = V'BV, forcing G (H'V), tensor T, D on K) or
-  // L sub-problem (M = U'AU, forcing H (G'U), tensor W, no stencil) on X in place;
-  // scr holds 2 k columns, acc k columns (ld rows).
-  void quad_rows(bool nside, int k, double* X, double* scr, double* acc, double h, int64_t nsub) {
+  // The stage arguments of the K (nside: operator A, D on K, forcing G) or L
+  // sub-problem (operator B, no stencil, forcing H) with the Galerkin matrix M,
+  // forcing projection hv and tensor T of the other side.
+  QuadStageArgs quad_stage_args(bool nside, int k, const double* M, const double* hv, const double* T, double h) {
     const int64_t rows_ = nside ? n : m;
     QuadStageArgs Q;
     Q.n = rows_;
     Q.k = k;
     Q.ld = rows_;
-    Q.X0 = X;
-    Q.Acc = acc;
     Q.h = h;
     Q.st_d = nside ? op_ad : op_bd;
     Q.st_e = nside ? op_ae : op_be;
     Q.periodic = nside ? op_aper : op_bper;
-    Q.M = nside ? de->Gb : de->Ga;
+    Q.M = M;
     Q.g = nside ? op_G : op_H;
     Q.ldg = rows_;
     Q.q = op_q;
-    Q.hv = nside ? de->hbT : de->gaT;
-    Q.T = nside ? quad_T : quad_W;
+    Q.hv = hv;
+    Q.T = T;
     if (nside) {
       Q.dl = op_dl;
       Q.dd = op_dd;
       Q.du = op_du;
       Q.dper = op_dper;
     }
+    return Q;
+  }
+  // N RK4 steps of the K (nside: M = V'BV, forcing G (H'V), tensor T, D on K) or
+  // L sub-problem (M = U'AU, forcing H (G'U), tensor W, no stencil) on X in place;
+  // scr holds 2 k columns, acc k columns (ld rows).  With `store`, the start of
+  // sub-step st is kept at store + st rows k.
+  void quad_rows_ex(bool nside, int k, const double* M, const double* hv, const double* T, double* X, double* scr,
+                    double* acc, double h, int64_t nsub, double* store) {
+    const int64_t rows_ = nside ? n : m;
+    QuadStageArgs Q = quad_stage_args(nside, k, M, hv, T, h);
+    Q.X0 = X;
+    Q.Acc = acc;
     double* xa = scr;
     double* xb = scr + rows_ * k;
     const double* in[4] = {X, xa, xb, xa};
     double* outp[4] = {xa, xb, xa, nullptr};
-    for (int64_t st = 0; st < nsub; ++st)
+    for (int64_t st = 0; st < nsub; ++st) {
+      if (store)
+        HIPCHECK(hipMemcpyAsync(store + st * rows_ * k, X, sizeof(double) * rows_ * k, hipMemcpyDeviceToDevice,
+                                stream));
       for (int s = 0; s < 4; ++s) {
         Q.stage = s;
         Q.Xs = in[s];
         Q.Xn = outp[s];
         timed(LRI_KERNEL_DE, [&] { return launch_quad_stage(Q, stream); });
       }
+    }
+  }
+  void quad_rows(bool nside, int k, double* X, double* scr, double* acc, double h, int64_t nsub) {
+    quad_rows_ex(nside, k, nside ? de->Gb : de->Ga, nside ? de->hbT : de->gaT, nside ? quad_T : quad_W, X, scr, acc,
+                 h, nsub, nullptr);
   }
   // Gram of X (rows x k) and the shifted first CholeskyQR factor
   void quad_gram(const double* X, int64_t rows_, int k, EpiArgs epi) {
@@ -1773,6 +1813,312 @@ struct lri_ctx {
     std::swap(U, U2);
     std::swap(V, V2);
     de_gal_fresh = true;  // the Galerkin data and tensors now belong to U, V
+  }
+
+  // ------------------------------------------------- matrix-ODE adjoint
+  // Reverse of quad_bug (unconventional_step! with the DE sub-problems,
+  // unconventional.jl:133-157, :51-79) for the cotangents (Ub, Sb, Vb) of the
+  // result (U1, S1, V1) in P[:, :r], adj->Sb1 and Q[:, :r]; leaves those of
+  // (U0, S0, V0) in P[:, :r], adj->Sb0 and Q[:, :r], and in qadj->scal the
+  // derivatives along scales a, b of the linear parts (F = a A Y + b Y B' + ...).
+  // The forward step is recomputed keeping the RK4 sub-step starts of K and L;
+  // then, with M = U1'U0, N = V1'V0 and the S sub-step from M S0 N':
+  //   S reverse -> Sibar and the cotangents of U1'AU1, V1'BV1, G'U1, H'V1, W(U1), T(V1);
+  //   Mb = Sibar N S0', Nb = Sibar' M S0, Sb0 = M' Sibar N;
+  //   Ub1 += (basis cotangent of U1's Galerkin data) + U0 Mb', Vb1 likewise;
+  //   Kbar1 = (Ub1 + U1 copyltu(-Ub1'U1)) RK^{-T}, Lbar1 likewise;
+  //   K reverse -> Kbar0 and the cotangents of V0'BV0, H'V0, T(V0); L reverse likewise;
+  //   Ub0 = (basis cotangent of U0's Galerkin data) + U1 Mb + Kbar0 S0',
+  //   Vb0 = (... of V0's) + V1 Nb + Lbar0 S0,  Sb0 += U0'Kbar0 + Lbar0'V0.
+  size_t qadj_rows_need(int64_t nsub) const {
+    const int rr = (int)r;
+    const int64_t mx = std::max(n, m);
+    const int plen = quad_adj_part_len(rr, op_q);
+    return (size_t)(nsub * (n + m) * rr + 8 * mx * rr) +
+           (size_t)plen * (quad_adj_grid(n, rr) + quad_adj_grid(m, rr)) + (size_t)quad_sadj_work(rr, rr, nsub);
+  }
+  void qadj_reserve(int64_t nsub) {
+    if (!qadj) {
+      HIPCHECK(hipMalloc((void**)&qadj, sizeof(QuadAdj)));
+      HIPCHECK(hipMemset(qadj, 0, sizeof(QuadAdj)));
+    }
+    const size_t need = qadj_rows_need(nsub);
+    if (need > qadj_rows_len) {
+      HIPCHECK(hipStreamSynchronize(stream));
+      if (qadj_rows) HIPCHECK(hipFree(qadj_rows));
+      qadj_rows = nullptr;
+      qadj_rows_len = 0;
+      HIPCHECK(hipMalloc((void**)&qadj_rows, sizeof(double) * need));
+      qadj_rows_len = need;
+    }
+  }
+  // Reverse of the N RK4 steps of quad_rows_ex: store holds the sub-step starts,
+  // ybar the cotangent of the result on entry and of the start on exit; the
+  // summed parameter cotangents go to sum.  work: 7 rows x k.
+  void quad_rows_reverse(bool nside, int k, const double* M, const double* hv, const double* T, const double* store,
+                         double* ybar, double* work, double* part, double* sum, double h, int64_t nsub) {
+    const int64_t rows_ = nside ? n : m;
+    const int64_t sz = rows_ * k;
+    double* x2 = work;
+    double* x3 = x2 + sz;
+    double* x4 = x3 + sz;
+    double* xa = x4 + sz;
+    double* xb = xa + sz;
+    double* ya = xb + sz;
+    double* accs = ya + sz;
+    const int plen = quad_adj_part_len(k, op_q);
+    const int grid = quad_adj_grid(rows_, k);
+    HIPCHECK(hipMemsetAsync(part, 0, sizeof(double) * plen * grid, stream));
+    QuadStageArgs F = quad_stage_args(nside, k, M, hv, T, h);
+    F.Acc = accs;
+    QuadAdjStageArgs R;
+    R.n = rows_;
+    R.k = k;
+    R.ld = rows_;
+    R.h = h;
+    R.st_d = F.st_d;
+    R.st_e = F.st_e;
+    R.periodic = F.periodic;
+    R.M = M;
+    R.g = F.g;
+    R.ldg = F.ldg;
+    R.q = F.q;
+    R.T = T;
+    R.dl = F.dl;
+    R.dd = F.dd;
+    R.du = F.du;
+    R.dper = F.dper;
+    R.part = part;
+    double* yb = ybar;
+    double* yn = ya;
+    for (int64_t st = nsub - 1; st >= 0; --st) {
+      double* y = const_cast<double*>(store) + st * sz;  // read-only for stages 0..2
+      F.X0 = y;
+      const double* fin[3] = {y, x2, x3};
+      double* fout[3] = {x2, x3, x4};
+      for (int s = 0; s < 3; ++s) {  // the stage inputs X2, X3, X4 of this sub-step
+        F.stage = s;
+        F.Xs = fin[s];
+        F.Xn = fout[s];
+        timed(LRI_KERNEL_DE, [&] { return launch_quad_stage(F, stream); });
+      }
+      const double* in[4] = {y, x2, x3, x4};
+      double* xbo = xa;
+      const double* xbp = nullptr;
+      for (int s = 3; s >= 0; --s) {
+        R.stage = s;
+        R.X = in[s];
+        R.Ybar = yb;
+        R.Xbp = xbp;
+        R.Xbo = xbo;
+        R.Acc = yn;
+        timed(LRI_KERNEL_DE, [&] { return launch_quad_adj_stage(R, stream); });
+        xbp = xbo;
+        xbo = xbo == xa ? xb : xa;
+      }
+      std::swap(yb, yn);
+    }
+    if (yb != ybar) HIPCHECK(hipMemcpyAsync(ybar, yb, sizeof(double) * sz, hipMemcpyDeviceToDevice, stream));
+    timed(LRI_KERNEL_SMALL, [&] { return launch_part_sum(part, grid, plen, sum, stream); });
+  }
+  BasisVjpArgs basis_vjp_args(bool nside, int k, const double* X, const double* Tb, const double* Gb,
+                              const double* fb) {
+    BasisVjpArgs B;
+    B.rows = nside ? n : m;
+    B.k = k;
+    B.X = X;
+    B.ldx = B.rows;
+    if (nside) {
+      B.dl = op_dl;
+      B.dd = op_dd;
+      B.du = op_du;
+      B.dper = op_dper;
+    }
+    B.Tb = Tb;
+    B.st_d = nside ? op_ad : op_bd;
+    B.st_e = nside ? op_ae : op_be;
+    B.periodic = nside ? op_aper : op_bper;
+    B.Gb = Gb;
+    B.F = nside ? op_G : op_H;
+    B.ldf = B.rows;
+    B.q = op_q;
+    B.fb = fb;
+    return B;
+  }
+  void quad_bug_adjoint(double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    const size_t d = sizeof(double);
+    const int64_t mx = std::max(n, m);
+    const int plen = quad_adj_part_len(rr, op_q);
+    qadj_reserve(nsub);
+    double* stK = qadj_rows;
+    double* stL = stK + nsub * n * rr;
+    double* work = stL + nsub * m * rr;  // 8 mx rr
+    double* partK = work + 8 * mx * rr;
+    double* partL = partK + (size_t)plen * quad_adj_grid(n, rr);
+    double* swork = partL + (size_t)plen * quad_adj_grid(m, rr);
+    double* kbar = work + 7 * mx * rr;  // the K (then L) cotangent, beside the 7-column reverse work
+    QuadAdj* qa = qadj;
+    // forward: Galerkin data of U0, V0 (kept), K and L with their sub-step starts
+    quad_galerkin(V, false, rr);
+    quad_galerkin(U, true, rr);
+    de_gal_fresh = false;
+    const size_t rsz = d * rr * rr, fsz = d * op_q * rr, tsz = d * rr * rr * rr;
+    HIPCHECK(hipMemcpyAsync(qa->Ga0, de->Ga, rsz, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(qa->Gb0, de->Gb, rsz, hipMemcpyDeviceToDevice, stream));
+    if (op_q) {
+      HIPCHECK(hipMemcpyAsync(qa->gaT0, de->gaT, fsz, hipMemcpyDeviceToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(qa->hbT0, de->hbT, fsz, hipMemcpyDeviceToDevice, stream));
+    }
+    HIPCHECK(hipMemcpyAsync(qa->W0, quad_W, tsz, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(qa->T0, quad_T, tsz, hipMemcpyDeviceToDevice, stream));
+    quad_init(U, n, rr, S, (int)rcap, 0, K);
+    quad_rows_ex(true, rr, qa->Gb0, qa->hbT0, qa->T0, K, work, work + 2 * mx * rr, h, nsub, stK);
+    quad_init(V, m, rr, S, (int)rcap, 1, L);
+    quad_rows_ex(false, rr, qa->Ga0, qa->gaT0, qa->W0, L, work, work + 2 * mx * rr, h, nsub, stL);
+    quad_gram(K, n, rr, shifted_chol(n));
+    cholqr_tail(K, K2, n, rr, U2, false, U, rr, de_epi(EPI_BUG_M), 0, 3);  // U1, M = U1'U0
+    HIPCHECK(hipMemcpyAsync(adj->RK, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    quad_gram(L, m, rr, shifted_chol(m));
+    EpiArgs eN = de_epi(EPI_BUG_N);  // N = V1'V0, Sinit = M S0 N'
+    eN.S = S;
+    eN.ld_s = (int)rcap;
+    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0, 3);
+    HIPCHECK(hipMemcpyAsync(adj->RL, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    quad_galerkin(U2, true, rr);  // U1'AU1, G'U1, W(U1)
+    quad_galerkin(V2, false, rr);  // V1'BV1, H'V1, T(V1)
+    // S sub-step reverse
+    QuadSAdjArgs SA;
+    SA.k = SA.l = rr;
+    SA.q = op_q;
+    SA.S0 = small->Sinit;
+    SA.lds0 = rr;
+    SA.Sb1 = adj->Sb1;
+    SA.ldsb1 = rr;
+    SA.Sb0 = qa->Sib;
+    SA.Ga = de->Ga;
+    SA.Gb = de->Gb;
+    SA.gaT = de->gaT;
+    SA.hbT = de->hbT;
+    SA.W = quad_W;
+    SA.T = quad_T;
+    SA.sgn = 1.0;
+    SA.h = h;
+    SA.nsub = nsub;
+    SA.Gab = qa->Gab1;
+    SA.Gbb = qa->Gbb1;
+    SA.gaTb = qa->gaTb1;
+    SA.hbTb = qa->hbTb1;
+    SA.Wb = qa->Wb1;
+    SA.Tb = qa->Tb1;
+    SA.work = swork;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_quad_sadj(SA, stream); });
+    HIPCHECK(hipMemcpyAsync(adj->Sb1, qa->Sib, rsz, hipMemcpyDeviceToDevice, stream));
+    // Mb, Nb, Sb0 = M' Sibar N, RK^{-T}, RL^{-T}
+    timed(LRI_KERNEL_SMALL, [&] { return launch_bug_adj_pre(adj, small->M, small->N, S, (int)rcap, rr, stream); });
+    double* Ubf = P + n * rr;
+    double* Vbf = Q + m * rr;
+    for (int side = 0; side < 2; ++side) {  // the full cotangents of U1 and V1
+      const bool ns = side == 0;
+      BasisVjpArgs B = basis_vjp_args(ns, rr, ns ? U2 : V2, ns ? qa->Wb1 : qa->Tb1, ns ? qa->Gab1 : qa->Gbb1,
+                                      ns ? qa->gaTb1 : qa->hbTb1);
+      B.X2 = ns ? U : V;
+      B.ldx2 = B.rows;
+      B.Y2 = ns ? adj->Mb : adj->Nb;
+      B.ldy2 = rr;
+      B.transY2 = 1;
+      B.base = ns ? P : Q;
+      B.ldb = B.rows;
+      B.out = ns ? Ubf : Vbf;
+      B.ldo = B.rows;
+      timed(LRI_KERNEL_DE, [&] { return launch_basis_vjp(B, stream); });
+    }
+    // reverse of each sub-problem: QR backward, then the RK4 steps
+    for (int side = 0; side < 2; ++side) {
+      const bool ns = side == 0;
+      const int64_t rows_ = ns ? n : m;
+      RowArgs<double> A;  // copyltu(-Xbf' X1)
+      A.rows = rows_;
+      A.r_out = rr;
+      A.base = ns ? Ubf : Vbf;
+      A.ld_base = rows_;
+      A.reduce = 1;
+      A.W = ns ? U2 : V2;
+      A.ldw = rows_;
+      A.rw = rr;
+      A.epi.kind = EPI_COPYLTU;
+      A.epi.dst = ns ? adj->CqK : adj->CqL;
+      rows(A);
+      double* Z = ns ? K2 : L2;
+      RowArgs<double> B;  // Z = Xbf + X1 Cq
+      B.rows = rows_;
+      B.r_out = rr;
+      B.r_in = rr;
+      B.base = ns ? Ubf : Vbf;
+      B.ld_base = rows_;
+      B.X = ns ? U2 : V2;
+      B.ldx = rows_;
+      B.Y = ns ? adj->CqK : adj->CqL;
+      B.ldy = rr;
+      B.out = Z;
+      B.ldo = rows_;
+      rows(B);
+      RowArgs<double> C;  // Kbar1 = Z RK^{-T}
+      C.rows = rows_;
+      C.r_out = rr;
+      C.r_in = rr;
+      C.X = Z;
+      C.ldx = rows_;
+      C.Y = ns ? adj->RKinvT : adj->RLinvT;
+      C.ldy = rr;
+      C.out = kbar;
+      C.ldo = rows_;
+      rows(C);
+      if (ns)
+        quad_rows_reverse(true, rr, qa->Gb0, qa->hbT0, qa->T0, stK, kbar, work, partK, qa->sumK, h, nsub);
+      else
+        quad_rows_reverse(false, rr, qa->Ga0, qa->gaT0, qa->W0, stL, kbar, work, partL, qa->sumL, h, nsub);
+      // Kbar0 -> K (Lbar0 -> L), and its reduction against U0 (V0): CK = Kbar0' U0, CL = Lbar0' V0
+      RowArgs<double> D;
+      D.rows = rows_;
+      D.r_out = rr;
+      D.base = kbar;
+      D.ld_base = rows_;
+      D.out = ns ? K : L;
+      D.ldo = rows_;
+      D.reduce = 1;
+      D.W = ns ? U : V;
+      D.ldw = rows_;
+      D.rw = rr;
+      D.epi.kind = EPI_COPY;
+      D.epi.dst = ns ? adj->CK : adj->CL;
+      rows(D);
+    }
+    // Ub0, Vb0
+    const int k3 = rr * rr * rr;
+    for (int side = 0; side < 2; ++side) {
+      const bool ns = side == 0;
+      const double* sum = ns ? qa->sumL : qa->sumK;  // U0's data enter the L sub-problem, V0's the K one
+      BasisVjpArgs B = basis_vjp_args(ns, rr, ns ? U : V, sum, sum + k3, sum + k3 + rr * rr);
+      B.X2 = ns ? U2 : V2;
+      B.ldx2 = B.rows;
+      B.Y2 = ns ? adj->Mb : adj->Nb;
+      B.ldy2 = rr;
+      B.X3 = ns ? K : L;
+      B.ldx3 = B.rows;
+      B.Y3 = S;
+      B.ldy3 = (int)rcap;
+      B.transY3 = ns;  // Kbar0 S0', Lbar0 S0
+      B.out = ns ? P : Q;
+      B.ldo = B.rows;
+      timed(LRI_KERNEL_DE, [&] { return launch_basis_vjp(B, stream); });
+    }
+    timed(LRI_KERNEL_SMALL, [&] { return launch_bug_adj_post(adj, rr, stream); });  // Sb0 += CK' + CL
+    timed(LRI_KERNEL_SMALL, [&] {
+      return launch_quad_adj_scales(qa->sumK, qa->sumL, plen, qa->sumL + k3, qa->Ga0, qa->Gab1, de->Ga,
+                                    qa->sumK + k3, qa->Gb0, qa->Gbb1, de->Gb, rr, qa->scal, stream);
+    });
   }
 
   int64_t quad_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax, bool complete = true) {
@@ -2035,6 +2381,8 @@ void free_ctx(lri_ctx* c) {
       if (p) (void)hipFree(p);
   if (c->r_host) (void)hipHostFree(c->r_host);
   if (c->adj) (void)hipFree(c->adj);
+  if (c->qadj) (void)hipFree(c->qadj);
+  if (c->qadj_rows) (void)hipFree(c->qadj_rows);
   for (auto& g : c->ra_graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& t : c->pending) {
     (void)hipEventDestroy(t.a);
@@ -2513,6 +2861,40 @@ int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps) {
     DeScope scope(ctx);
     ctx->de_bug(dt, substeps);
     return LRI_OK;
+  });
+}
+
+int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* Ub, int64_t ldub, double* Sb,
+                            int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar) {
+  return guarded(ctx, [&] {
+    ctx->check_de(dt, substeps);
+    REQUIRE(ctx->op_quad,
+            "the matrix-ODE adjoint needs the quadratic operator (lri_set_operator_quadratic; D = 0 for a "
+            "linear right-hand side)");
+    REQUIRE(Ub && Sb && Vb, "Ub, Sb and Vb are required");
+    const int64_t r = ctx->r, n = ctx->n, m = ctx->m;
+    REQUIRE(ldub >= n && ldvb >= m && ldsb >= r, "leading dimensions");
+    DeScope scope(ctx);
+    if (!ctx->adj) {
+      HIPCHECK(hipMalloc((void**)&ctx->adj, sizeof(BugAdjSmall)));
+      HIPCHECK(hipMemset(ctx->adj, 0, sizeof(BugAdjSmall)));
+    }
+    const size_t d = sizeof(double);
+    HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyHostToDevice, ctx->stream));
+    ctx->quad_bug_adjoint(dt, substeps);
+    double sc[2] = {0.0, 0.0};
+    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->adj->Sb0, d * r, d * r, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(sc, ctx->qadj->scal, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
+    const int f = ctx->read_flags();
+    if (scale_bar) {
+      scale_bar[0] += sc[0];
+      scale_bar[1] += sc[1];
+    }
+    return f;
   });
 }
 

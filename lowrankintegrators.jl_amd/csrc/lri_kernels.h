@@ -293,6 +293,109 @@ struct QuadSArgs {
 };
 hipError_t launch_quad_sstep(const QuadSArgs& Q, hipStream_t st);
 
+// Reverse (adjoint) of the quadratic sub-steps (see lri_quad_adj.hip).
+// One reverse RK4 stage s of X' = f(X) (f as in QuadStageArgs) over all rows:
+// with Ybar the cotangent of the RK4 step's result and Xbp that of stage s + 1's
+// input, kbar = (h w_s / 6) Ybar + c_s h Xbp (w = 1, 2, 2, 1; c = 1/2, 1/2, 1),
+//   Xbo = J_f(X)' kbar,   Acc (=: s = 3 from Ybar, else +=) Xbo,
+// and each workgroup adds its rows' parameter cotangents into its slice of
+// part: [k^3: Tbar][k^2: Mbar][q k: hvbar][1: <kbar, A X>].
+struct QuadAdjStageArgs {
+  int64_t n = 0;
+  int k = 0;
+  int64_t ld = 0;
+  const double* X = nullptr;
+  const double* Ybar = nullptr;
+  const double* Xbp = nullptr;  // nullptr for s = 3
+  double* Xbo = nullptr;
+  double* Acc = nullptr;
+  int stage = 0;
+  double h = 0.0;
+  const double* st_d = nullptr;
+  const double* st_e = nullptr;
+  int periodic = 0;
+  const double* M = nullptr;
+  const double* g = nullptr;
+  int64_t ldg = 0;
+  int q = 0;
+  const double* T = nullptr;
+  const double *dl = nullptr, *dd = nullptr, *du = nullptr;
+  int dper = 0;
+  double* part = nullptr;
+  int grid = 0;  // workgroups (filled by the launcher)
+};
+__host__ __device__ inline int quad_adj_part_len(int k, int q) { return k * k * k + k * k + q * k + 1; }
+// workgroups of the reverse stage for `rows` rows (each owns one slice of part)
+int quad_adj_grid(int64_t rows, int k);
+hipError_t launch_quad_adj_stage(QuadAdjStageArgs Q, hipStream_t st);
+// out[e] = sum over g < count of part[g * len + e] (in order)
+hipError_t launch_part_sum(const double* part, int count, int len, double* out, hipStream_t st);
+
+// Reverse of the S sub-step (N RK4 steps of S' = sgn (Ga S + S Gb + gaT' hbT - P(S))),
+// one workgroup: recomputes the trajectory from S0, then returns the cotangent
+// Sb0 of S0 (k x l, ld k) from that of the result (Sb1) and the cotangents of
+// Ga, Gb, gaT, hbT, W, T (written, same layouts).
+struct QuadSAdjArgs {
+  int k = 0, l = 0, q = 0;
+  const double* S0 = nullptr;
+  int lds0 = 0;
+  const double* Sb1 = nullptr;
+  int ldsb1 = 0;
+  double* Sb0 = nullptr;
+  const double *Ga = nullptr, *Gb = nullptr, *gaT = nullptr, *hbT = nullptr, *W = nullptr, *T = nullptr;
+  double sgn = 1.0;
+  double h = 0.0;
+  int64_t nsub = 0;
+  double *Gab = nullptr, *Gbb = nullptr, *gaTb = nullptr, *hbTb = nullptr, *Wb = nullptr, *Tb = nullptr;
+  double* work = nullptr;  // quad_sadj_work(k, l, nsub) doubles
+};
+inline int64_t quad_sadj_work(int k, int l, int64_t nsub) {
+  const int64_t r = k > l ? k : l;
+  return 4 * nsub * k * l + 7 * r * r * r;  // the stage inputs, then seven r^3 intermediates
+}
+hipError_t launch_quad_sadj(const QuadSAdjArgs& Q, hipStream_t st);
+
+// Cotangent of a basis X (rows x k) from those of its Galerkin data:
+//   out = base + [tensor] + A X (Gb + Gb') + F fb + X2 op(Y2) + X3 op(Y3),
+// tensor: the cotangent through W_abc = sum_i (D X)_ia X_ib X_ic (D = I when
+// dd == nullptr) given Tb (k^3, [c + k (a + k b)]).  Every term is optional.
+struct BasisVjpArgs {
+  int64_t rows = 0;
+  int k = 0;
+  const double* X = nullptr;
+  int64_t ldx = 0;
+  const double *dl = nullptr, *dd = nullptr, *du = nullptr;
+  int dper = 0;
+  const double* Tb = nullptr;
+  const double* st_d = nullptr;
+  const double* st_e = nullptr;
+  int periodic = 0;
+  const double* Gb = nullptr;
+  const double* F = nullptr;
+  int64_t ldf = 0;
+  int q = 0;
+  const double* fb = nullptr;
+  const double* X2 = nullptr;
+  int64_t ldx2 = 0;
+  const double* Y2 = nullptr;
+  int ldy2 = 0, transY2 = 0;
+  const double* X3 = nullptr;
+  int64_t ldx3 = 0;
+  const double* Y3 = nullptr;
+  int ldy3 = 0, transY3 = 0;
+  const double* base = nullptr;
+  int64_t ldb = 0;
+  double* out = nullptr;
+  int64_t ldo = 0;
+};
+hipError_t launch_basis_vjp(const BasisVjpArgs& Q, hipStream_t st);
+// out[0] = sumK[len-1] + <Gab0, Ga0> + <Gab1, Ga1>, out[1] = sumL[len-1] + <Gbb0, Gb0> + <Gbb1, Gb1>
+// (the derivatives along the scales of the two linear parts; k x k, ld k)
+hipError_t launch_quad_adj_scales(const double* sumK, const double* sumL, int len, const double* Gab0,
+                                  const double* Ga0, const double* Gab1, const double* Ga1, const double* Gbb0,
+                                  const double* Gb0, const double* Gbb1, const double* Gb1, int k, double* out,
+                                  hipStream_t st);
+
 // Analytic eigenbasis of a constant symmetric tridiagonal d I + e (S + S')
 // (Dirichlet, or periodic with n >= 3): Q n x n column-major, lam n.
 hipError_t launch_eig_tridiag_const(double* Q, double* lam, int64_t n, double d, double e, int periodic,

@@ -1,0 +1,628 @@
+// Reverse (adjoint) of the matrix-ODE sub-steps with the quadratic right-hand
+// side of lri_quad.hip,
+//
+//     K' = f(K) = A K + K M + g hv - N(K),  N_ic = sum_ab (D K)_ia K_ib T_abc,
+//     S' = sgn (Ga S + S Gb + gaT' hbT - P(S)),  P_ij = sum_efab S_ea S_fb W_efi T_abj,
+//
+// integrated by N classical RK4 steps.  For a stage input X and the cotangent
+// kbar of the stage's value f(X):
+//
+//     J_f(X)' kbar = A kbar + kbar M' - (D' P + Q),
+//         P_ia = sum_bc kbar_ic X_ib T_abc,  Q_ib = sum_ac kbar_ic (D X)_ia T_abc,
+//     Tbar_abc -= sum_i kbar_ic (D X)_ia X_ib,  Mbar += X' kbar,  hvbar += g' kbar,
+//
+// and the RK4 step y+ = y + h/6 (k1 + 2 k2 + 2 k3 + k4), X1 = y, X2 = y + h/2 k1,
+// X3 = y + h/2 k2, X4 = y + h k3, runs backwards stage by stage:
+//     kbar_s = (h w_s / 6) ybar+ + c_s h Xbar_{s+1},  Xbar_s = J_f(X_s)' kbar_s,
+//     ybar = ybar+ + sum_s Xbar_s.
+// The K and L sub-problems need neighbouring rows (A and D are tridiagonal), so
+// a stage is one launch over all rows; the r x r S sub-problem runs in one
+// workgroup, and the basis cotangents from the Galerkin data (X'AX, F'X and the
+// tensor sum_i (D X)_ia X_ib X_ic) are one row kernel.
+#include <algorithm>
+
+#include "lri_common.h"
+#include "lri_kernels.h"
+
+namespace lri {
+
+namespace {
+
+__device__ __forceinline__ double stage_c(int s) { return s == 2 ? 1.0 : 0.5; }
+__device__ __forceinline__ double stage_w(int s) { return (s == 1 || s == 2) ? 2.0 : 1.0; }
+
+// Rows r0 - 1 .. r0 + R of a rows x k matrix (ld) into buf[lr * k + c], a row
+// outside [0, n) taking its periodic image (only read by periodic stencils) and
+// rows past the image range reading as zero.
+__device__ __forceinline__ void stage_rows(double* buf, const double* src, int64_t ld, int64_t r0, int R, int k,
+                                           int64_t n) {
+  for (int e = threadIdx.x; e < (R + 2) * k; e += blockDim.x) {
+    const int lr = e % (R + 2), c = e / (R + 2);
+    int64_t ir = r0 + lr - 1;
+    if (ir < 0) ir += n;
+    else if (ir >= n) ir -= n;
+    buf[lr * k + c] = (ir >= 0 && ir < n) ? src[ir + c * ld] : 0.0;
+  }
+}
+
+// (D x)_i for staged rows (x[lr * k + c] holds row r0 + lr - 1); D = I when dd == nullptr
+__device__ __forceinline__ double apply_d(const double* x, int lr, int c, int k, int64_t i, int64_t n,
+                                          const double* dl, const double* dd, const double* du, int dper) {
+  const double x0 = x[(lr + 1) * k + c];
+  if (!dd) return x0;
+  const bool hasl = i > 0 || dper, hasu = i < n - 1 || dper;
+  return dd[i] * x0 + (hasl ? dl[i] * x[lr * k + c] : 0.0) + (hasu ? du[i] * x[(lr + 2) * k + c] : 0.0);
+}
+
+// (D' y)_i: column i of D is dd_i (row i), du_{i-1} (row i - 1), dl_{i+1} (row i + 1)
+__device__ __forceinline__ double apply_dt(const double* y, int lr, int c, int k, int64_t i, int64_t n,
+                                           const double* dl, const double* dd, const double* du, int dper) {
+  const double y0 = y[(lr + 1) * k + c];
+  if (!dd) return y0;
+  const bool hasl = i > 0 || dper, hasu = i < n - 1 || dper;
+  const int64_t im = i > 0 ? i - 1 : n - 1, ip = i < n - 1 ? i + 1 : 0;
+  return dd[i] * y0 + (hasl ? du[im] * y[lr * k + c] : 0.0) + (hasu ? dl[ip] * y[(lr + 2) * k + c] : 0.0);
+}
+
+// (A y)_i for the symmetric tridiagonal A (st_e[i] couples rows i and i + 1,
+// st_e[n - 1] the last and first row when periodic)
+__device__ __forceinline__ double apply_a(const double* y, int lr, int c, int k, int64_t i, int64_t n,
+                                          const double* st_d, const double* st_e, int periodic) {
+  double v = st_d[i] * y[(lr + 1) * k + c];
+  if (i > 0) v += st_e[i - 1] * y[lr * k + c];
+  else if (periodic) v += st_e[n - 1] * y[lr * k + c];
+  if (i < n - 1) v += st_e[i] * y[(lr + 2) * k + c];
+  else if (periodic) v += st_e[n - 1] * y[(lr + 2) * k + c];
+  return v;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// One reverse RK4 stage over all rows.  Workgroup: tiles of R = 256 / k rows,
+// thread (row lr = t % R, column c = t / R); a workgroup walks the tiles
+// blockIdx.x, blockIdx.x + grid, ... and adds their parameter cotangents into
+// its own slice of Q.part (no atomics: the sum over workgroups is taken later,
+// in order).
+// ----------------------------------------------------------------------------
+
+constexpr int kAdjMaxGrid = 512;
+
+int quad_adj_grid(int64_t rows, int k) {
+  const int R = 256 / k;
+  const int64_t tiles = (rows + R - 1) / R;
+  return (int)std::min<int64_t>(tiles, kAdjMaxGrid);
+}
+
+__global__ __launch_bounds__(256) void k_quad_adj_stage(QuadAdjStageArgs Q) {
+  __shared__ double xs[256 + 2 * kQuadMaxRank];  // stage input rows r0 - 1 .. r0 + R
+  __shared__ double ks[256 + 2 * kQuadMaxRank];  // kbar, same rows
+  __shared__ double ps[256 + 2 * kQuadMaxRank];  // P, same rows
+  __shared__ double dxs[256];                    // (D X), rows r0 .. r0 + R - 1
+  __shared__ double red[256];
+  const int k = Q.k, t = threadIdx.x;
+  const int R = 256 / k;
+  const int64_t n = Q.n;
+  const int s = Q.stage;
+  const double wy = Q.h * stage_w(s) / 6.0;
+  const double wx = s < 3 ? stage_c(s) * Q.h : 0.0;
+  const int k3 = k * k * k;
+  const int plen = quad_adj_part_len(k, Q.q);
+  double* part = Q.part + (int64_t)blockIdx.x * plen;
+  const int64_t ntiles = (n + R - 1) / R;
+  double scale_acc = 0.0;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t r0 = tile * R;
+    __syncthreads();  // the previous tile's LDS reads are done
+    for (int e = t; e < (R + 2) * k; e += blockDim.x) {
+      const int lr = e % (R + 2), c = e / (R + 2);
+      int64_t ir = r0 + lr - 1;
+      if (ir < 0) ir += n;
+      else if (ir >= n) ir -= n;
+      double x = 0.0, kb = 0.0;
+      if (ir >= 0 && ir < n) {
+        const int64_t o = ir + c * Q.ld;
+        x = Q.X[o];
+        kb = wy * Q.Ybar[o] + (s < 3 ? wx * Q.Xbp[o] : 0.0);
+      }
+      xs[lr * k + c] = x;
+      ks[lr * k + c] = kb;
+    }
+    __syncthreads();
+    if (Q.T) {
+      for (int e = t; e < (R + 2) * k; e += blockDim.x) {  // P_ia = sum_b X_ib sum_c kbar_ic T_abc
+        const int lr = e / k, a = e % k;
+        const double* xr = xs + lr * k;
+        const double* kr = ks + lr * k;
+        double p = 0.0;
+        for (int b = 0; b < k; ++b) {
+          const double* Tab = Q.T + (int64_t)k * (a + k * b);
+          double in = 0.0;
+          for (int c = 0; c < k; ++c) in += kr[c] * Tab[c];
+          p += xr[b] * in;
+        }
+        ps[e] = p;
+      }
+      for (int e = t; e < R * k; e += blockDim.x) {
+        const int lr = e / k, a = e % k;
+        const int64_t i = r0 + lr;
+        dxs[e] = i < n ? apply_d(xs, lr, a, k, i, n, Q.dl, Q.dd, Q.du, Q.dper) : 0.0;
+      }
+    }
+    __syncthreads();
+    {
+      const int lr = t % R, c = t / R;
+      const int64_t i = r0 + lr;
+      if (c < k && i < n) {
+        const double* kr = ks + (lr + 1) * k;
+        double xb = 0.0;
+        if (Q.st_d) {
+          xb = apply_a(ks, lr, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
+          scale_acc += kr[c] * apply_a(xs, lr, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
+        }
+        if (Q.M)
+          for (int l = 0; l < k; ++l) xb += kr[l] * Q.M[c + l * k];
+        if (Q.T) {
+          const double dp = apply_dt(ps, lr, c, k, i, n, Q.dl, Q.dd, Q.du, Q.dper);
+          const double* dr = dxs + lr * k;
+          double qv = 0.0;  // Q_ic = sum_a (D X)_ia sum_x kbar_ix T_acx
+          for (int a = 0; a < k; ++a) {
+            const double* Tac = Q.T + (int64_t)k * (a + k * c);
+            double in = 0.0;
+            for (int x = 0; x < k; ++x) in += kr[x] * Tac[x];
+            qv += dr[a] * in;
+          }
+          xb -= dp + qv;
+        }
+        const int64_t o = i + c * Q.ld;
+        Q.Xbo[o] = xb;
+        Q.Acc[o] = (s == 3 ? Q.Ybar[o] : Q.Acc[o]) + xb;
+      }
+    }
+    // parameter cotangents of this tile's rows
+    const int nr = (int)std::min<int64_t>(R, n - r0);
+    if (Q.T) {
+      for (int e = t; e < k3; e += blockDim.x) {  // Tbar_abc -= sum_i kbar_ic (D X)_ia X_ib
+        const int c = e % k, a = (e / k) % k, b = e / (k * k);
+        double v = 0.0;
+        for (int lr = 0; lr < nr; ++lr) v += ks[(lr + 1) * k + c] * dxs[lr * k + a] * xs[(lr + 1) * k + b];
+        part[e] -= v;
+      }
+    }
+    if (Q.M) {
+      for (int e = t; e < k * k; e += blockDim.x) {  // Mbar_lc = sum_i X_il kbar_ic
+        const int l = e % k, c = e / k;
+        double v = 0.0;
+        for (int lr = 0; lr < nr; ++lr) v += xs[(lr + 1) * k + l] * ks[(lr + 1) * k + c];
+        part[k3 + e] += v;
+      }
+    }
+    for (int e = t; e < Q.q * k; e += blockDim.x) {  // hvbar_qc = sum_i g_iq kbar_ic
+      const int qq = e % Q.q, c = e / Q.q;
+      double v = 0.0;
+      for (int lr = 0; lr < nr; ++lr) v += Q.g[(r0 + lr) + qq * Q.ldg] * ks[(lr + 1) * k + c];
+      part[k3 + k * k + e] += v;
+    }
+  }
+  red[t] = scale_acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) part[plen - 1] += red[0];
+}
+
+hipError_t launch_quad_adj_stage(QuadAdjStageArgs Q, hipStream_t st) {
+  if (Q.k < 1 || Q.k > kQuadMaxRank || Q.n < 1 || Q.stage < 0 || Q.stage > 3 || Q.q < 0 || Q.q > kMaxForce ||
+      !Q.part || !Q.X || !Q.Ybar || !Q.Xbo || !Q.Acc || (Q.stage < 3 && !Q.Xbp) || (Q.q > 0 && !Q.g) ||
+      (Q.st_d && !Q.st_e))
+    return hipErrorInvalidValue;
+  Q.grid = quad_adj_grid(Q.n, Q.k);
+  hipLaunchKernelGGL(k_quad_adj_stage, dim3(Q.grid), dim3(256), 0, st, Q);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_part_sum(const double* __restrict__ part, int count, int len,
+                                                  double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  double s = 0.0;
+  for (int g = 0; g < count; ++g) s += part[(int64_t)g * len + e];
+  out[e] = s;
+}
+
+hipError_t launch_part_sum(const double* part, int count, int len, double* out, hipStream_t st) {
+  if (count < 1 || len < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_part_sum, dim3((len + 255) / 256), dim3(256), 0, st, part, count, len, out);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// Reverse of the S sub-step, one workgroup of 1024 threads.  The k x l
+// matrices live in LDS; the third-order intermediates in Q.work (global):
+//   Y_ebc = sum_f X_fb W_efc,  Z_abc = sum_e X_ea Y_ebc      (forward P)
+//   C_efj = sum_i pb_ij W_efi,  X1_ebj = sum_a X_ea T_abj,
+//   D1_ebj = sum_f C_efj X_fb,  E1_faj = sum_e C_efj X_ea,  Zt_efj = sum_b X_fb X1_ebj,
+// giving  J_P' pb = sum_bj D1_ibj T_abj + sum_a'j E1_ia'j T_a'aj,
+//         Wbar_efi -= sum_j pb_ij Zt_efj,  Tbar_abj -= sum_i Z_abi pb_ij.
+// ----------------------------------------------------------------------------
+
+namespace {
+
+struct SAdjWork {
+  double *Y, *Z, *C, *X1, *D1, *E1, *Zt;
+};
+
+// out = f(X) (k x l, ld k), out != X
+__device__ void sadj_f(const QuadSAdjArgs& Q, const double* X, double* out, const double* cb, const SAdjWork& w) {
+  const int k = Q.k, l = Q.l, t = threadIdx.x, nt = blockDim.x;
+  if (Q.W) {
+    for (int e = t; e < k * l * k; e += nt) {
+      const int c = e % k, ee = (e / k) % k, b = e / (k * k);
+      double v = 0.0;
+      for (int f = 0; f < k; ++f) v += X[f + b * k] * Q.W[c + k * (ee + k * f)];
+      w.Y[e] = v;
+    }
+    __syncthreads();
+    for (int e = t; e < l * l * k; e += nt) {
+      const int c = e % k, a = (e / k) % l, b = e / (k * l);
+      double v = 0.0;
+      for (int ee = 0; ee < k; ++ee) v += X[ee + a * k] * w.Y[c + k * (ee + k * b)];
+      w.Z[e] = v;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < k * l; e += nt) {
+    const int i = e % k, j = e / k;
+    double v = cb[e];
+    for (int m = 0; m < k; ++m) v += Q.Ga[i + m * k] * X[m + j * k];
+    for (int m = 0; m < l; ++m) v += X[i + m * k] * Q.Gb[m + j * l];
+    if (Q.W) {
+      double p = 0.0;
+      for (int b = 0; b < l; ++b)
+        for (int a = 0; a < l; ++a) p += w.Z[i + k * (a + l * b)] * Q.T[j + l * (a + l * b)];
+      v -= p;
+    }
+    out[e] = Q.sgn * v;
+  }
+  __syncthreads();
+}
+
+// xb = J_f(X)' kb; accumulates Gab, Gbb, cbar (LDS) and Wb, Tb (global)
+__device__ void sadj_rev(const QuadSAdjArgs& Q, const double* X, const double* kb, double* pb, double* xb,
+                         double* Gab, double* Gbb, double* cbar, const SAdjWork& w) {
+  const int k = Q.k, l = Q.l, t = threadIdx.x, nt = blockDim.x;
+  for (int e = t; e < k * l; e += nt) pb[e] = Q.sgn * kb[e];
+  __syncthreads();
+  if (Q.W) {
+    for (int e = t; e < k * l * k; e += nt) {  // Y_ebc
+      const int c = e % k, ee = (e / k) % k, b = e / (k * k);
+      double v = 0.0;
+      for (int f = 0; f < k; ++f) v += X[f + b * k] * Q.W[c + k * (ee + k * f)];
+      w.Y[e] = v;
+    }
+    for (int e = t; e < k * k * l; e += nt) {  // C_efj at e + k (f + k j)
+      const int ee = e % k, f = (e / k) % k, j = e / (k * k);
+      const double* Wef = Q.W + k * (ee + k * f);
+      double v = 0.0;
+      for (int i = 0; i < k; ++i) v += pb[i + j * k] * Wef[i];
+      w.C[e] = v;
+    }
+    for (int e = t; e < k * l * l; e += nt) {  // X1_ebj at e + k (b + l j)
+      const int ee = e % k, b = (e / k) % l, j = e / (k * l);
+      double v = 0.0;
+      for (int a = 0; a < l; ++a) v += X[ee + a * k] * Q.T[j + l * (a + l * b)];
+      w.X1[e] = v;
+    }
+    __syncthreads();
+    for (int e = t; e < l * l * k; e += nt) {  // Z_abc at c + k (a + l b)
+      const int c = e % k, a = (e / k) % l, b = e / (k * l);
+      double v = 0.0;
+      for (int ee = 0; ee < k; ++ee) v += X[ee + a * k] * w.Y[c + k * (ee + k * b)];
+      w.Z[e] = v;
+    }
+    for (int e = t; e < k * l * l; e += nt) {  // D1_ebj at e + k (b + l j)
+      const int ee = e % k, b = (e / k) % l, j = e / (k * l);
+      double v = 0.0;
+      for (int f = 0; f < k; ++f) v += w.C[ee + k * (f + k * j)] * X[f + b * k];
+      w.D1[e] = v;
+    }
+    for (int e = t; e < k * l * l; e += nt) {  // E1_faj at f + k (a + l j)
+      const int f = e % k, a = (e / k) % l, j = e / (k * l);
+      double v = 0.0;
+      for (int ee = 0; ee < k; ++ee) v += w.C[ee + k * (f + k * j)] * X[ee + a * k];
+      w.E1[e] = v;
+    }
+    for (int e = t; e < k * k * l; e += nt) {  // Zt_efj at e + k (f + k j)
+      const int ee = e % k, f = (e / k) % k, j = e / (k * k);
+      double v = 0.0;
+      for (int b = 0; b < l; ++b) v += X[f + b * k] * w.X1[ee + k * (b + l * j)];
+      w.Zt[e] = v;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < k * l; e += nt) {
+    const int i = e % k, a = e / k;
+    double v = 0.0;
+    for (int m = 0; m < k; ++m) v += Q.Ga[m + i * k] * pb[m + a * k];
+    for (int m = 0; m < l; ++m) v += pb[i + m * k] * Q.Gb[a + m * l];
+    if (Q.W) {
+      double p = 0.0;
+      for (int j = 0; j < l; ++j)
+        for (int b = 0; b < l; ++b)
+          p += w.D1[i + k * (b + l * j)] * Q.T[j + l * (a + l * b)] + w.E1[i + k * (b + l * j)] * Q.T[j + l * (b + l * a)];
+      v -= p;
+    }
+    xb[e] = v;
+  }
+  if (Q.W) {
+    for (int e = t; e < k * k * k; e += nt) {  // Wb at [c + k (e + k f)]
+      const int c = e % k, ee = (e / k) % k, f = e / (k * k);
+      double v = 0.0;
+      for (int j = 0; j < l; ++j) v += pb[c + j * k] * w.Zt[ee + k * (f + k * j)];
+      Q.Wb[e] -= v;
+    }
+    for (int e = t; e < l * l * l; e += nt) {  // Tb at [j + l (a + l b)]
+      const int j = e % l, a = (e / l) % l, b = e / (l * l);
+      double v = 0.0;
+      for (int i = 0; i < k; ++i) v += w.Z[i + k * (a + l * b)] * pb[i + j * k];
+      Q.Tb[e] -= v;
+    }
+  }
+  for (int e = t; e < k * k; e += nt) {  // Gab_im += sum_a pb_ia X_ma
+    const int i = e % k, m = e / k;
+    double v = 0.0;
+    for (int a = 0; a < l; ++a) v += pb[i + a * k] * X[m + a * k];
+    Gab[e] += v;
+  }
+  for (int e = t; e < l * l; e += nt) {  // Gbb_mj += sum_i X_im pb_ij
+    const int m = e % l, j = e / l;
+    double v = 0.0;
+    for (int i = 0; i < k; ++i) v += X[i + m * k] * pb[i + j * k];
+    Gbb[e] += v;
+  }
+  for (int e = t; e < k * l; e += nt) cbar[e] += pb[e];
+  __syncthreads();
+}
+
+}  // namespace
+
+__host__ __device__ inline int quad_sadj_lds_doubles(int k, int l) { return 10 * k * l + k * k + l * l; }
+
+__global__ __launch_bounds__(1024) void k_quad_sadj(QuadSAdjArgs Q) {
+  extern __shared__ double lds[];
+  const int k = Q.k, l = Q.l, kl = k * l, t = threadIdx.x, nt = blockDim.x;
+  double* y = lds;
+  double* fv = y + kl;
+  double* x = fv + kl;
+  double* kb = x + kl;
+  double* pb = kb + kl;
+  double* xb = pb + kl;
+  double* yb = xb + kl;
+  double* acc = yb + kl;
+  double* cb = acc + kl;
+  double* cbar = cb + kl;
+  double* Gab = cbar + kl;
+  double* Gbb = Gab + k * k;
+  const int64_t r = k > l ? k : l;
+  const int64_t r3 = r * r * r;
+  double* store = Q.work;  // [sub-step][stage][k l]: the stage inputs
+  SAdjWork w;
+  w.Y = store + Q.nsub * 4 * kl;
+  w.Z = w.Y + r3;
+  w.C = w.Z + r3;
+  w.X1 = w.C + r3;
+  w.D1 = w.X1 + r3;
+  w.E1 = w.D1 + r3;
+  w.Zt = w.E1 + r3;
+  const double h = Q.h;
+  for (int e = t; e < kl; e += nt) {
+    const int i = e % k, j = e / k;
+    y[e] = Q.S0[i + j * Q.lds0];
+    yb[e] = Q.Sb1[i + j * Q.ldsb1];
+    double s = 0.0;
+    for (int qq = 0; qq < Q.q; ++qq) s += Q.gaT[qq + i * Q.q] * Q.hbT[qq + j * Q.q];
+    cb[e] = s;
+    cbar[e] = 0.0;
+  }
+  for (int e = t; e < k * k; e += nt) Gab[e] = 0.0;
+  for (int e = t; e < l * l; e += nt) Gbb[e] = 0.0;
+  if (Q.W) {
+    for (int e = t; e < k * k * k; e += nt) Q.Wb[e] = 0.0;
+    for (int e = t; e < l * l * l; e += nt) Q.Tb[e] = 0.0;
+  }
+  __syncthreads();
+  // forward trajectory, keeping every stage input
+  for (int64_t st = 0; st < Q.nsub; ++st) {
+    double* X4 = store + st * 4 * kl;
+    for (int e = t; e < kl; e += nt) {
+      X4[e] = y[e];
+      x[e] = y[e];
+    }
+    __syncthreads();
+    for (int s = 0; s < 4; ++s) {
+      sadj_f(Q, x, fv, cb, w);
+      for (int e = t; e < kl; e += nt) {
+        if (s < 3) {
+          acc[e] = (s == 0 ? 0.0 : acc[e]) + stage_w(s) * fv[e];
+          x[e] = y[e] + stage_c(s) * h * fv[e];
+          X4[(s + 1) * kl + e] = x[e];
+        } else {
+          y[e] = y[e] + (h / 6.0) * (acc[e] + fv[e]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // reverse
+  for (int64_t st = Q.nsub - 1; st >= 0; --st) {
+    const double* X4 = store + st * 4 * kl;
+    for (int s = 3; s >= 0; --s) {
+      for (int e = t; e < kl; e += nt) {
+        x[e] = X4[s * kl + e];
+        kb[e] = (h * stage_w(s) / 6.0) * yb[e] + (s < 3 ? stage_c(s) * h * xb[e] : 0.0);
+      }
+      __syncthreads();
+      sadj_rev(Q, x, kb, pb, xb, Gab, Gbb, cbar, w);
+      for (int e = t; e < kl; e += nt) acc[e] = (s == 3 ? yb[e] : acc[e]) + xb[e];
+      __syncthreads();
+    }
+    for (int e = t; e < kl; e += nt) yb[e] = acc[e];
+    __syncthreads();
+  }
+  for (int e = t; e < kl; e += nt) Q.Sb0[e] = yb[e];
+  for (int e = t; e < k * k; e += nt) Q.Gab[e] = Gab[e];
+  for (int e = t; e < l * l; e += nt) Q.Gbb[e] = Gbb[e];
+  for (int e = t; e < Q.q * k; e += nt) {  // gaT_qi: cb_ij = sum_q gaT_qi hbT_qj
+    const int qq = e % Q.q, i = e / Q.q;
+    double v = 0.0;
+    for (int j = 0; j < l; ++j) v += Q.hbT[qq + j * Q.q] * cbar[i + j * k];
+    Q.gaTb[e] = v;
+  }
+  for (int e = t; e < Q.q * l; e += nt) {
+    const int qq = e % Q.q, j = e / Q.q;
+    double v = 0.0;
+    for (int i = 0; i < k; ++i) v += Q.gaT[qq + i * Q.q] * cbar[i + j * k];
+    Q.hbTb[e] = v;
+  }
+}
+
+hipError_t launch_quad_sadj(const QuadSAdjArgs& Q, hipStream_t st) {
+  if (Q.k < 1 || Q.l < 1 || Q.k > kQuadMaxRank || Q.l > kQuadMaxRank || Q.q < 0 || Q.q > kMaxForce || !Q.work ||
+      Q.nsub < 1 || !Q.S0 || !Q.Sb1 || !Q.Sb0 || !Q.Ga || !Q.Gb || !Q.Gab || !Q.Gbb ||
+      (Q.q > 0 && (!Q.gaT || !Q.hbT || !Q.gaTb || !Q.hbTb)) || (Q.W && (!Q.T || !Q.Wb || !Q.Tb)))
+    return hipErrorInvalidValue;
+  const size_t lds = sizeof(double) * quad_sadj_lds_doubles(Q.k, Q.l);
+  hipLaunchKernelGGL(k_quad_sadj, dim3(1), dim3(1024), lds, st, Q);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// Basis cotangents from the Galerkin data's: tiles of R = 256 / k rows with one
+// halo row each side (A, D and D' are tridiagonal).
+// ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_basis_vjp(BasisVjpArgs Q) {
+  __shared__ double xs[256 + 2 * kQuadMaxRank];
+  __shared__ double xsig[256 + 2 * kQuadMaxRank];  // X (Gb + Gb')
+  __shared__ double dxb[256 + 2 * kQuadMaxRank];   // dxbar_ia = sum_bc Tb_abc X_ib X_ic
+  __shared__ double dxs[256];                      // (D X)
+  __shared__ double sig[kQuadMaxRank * kQuadMaxRank];
+  const int k = Q.k, t = threadIdx.x;
+  const int R = 256 / k;
+  const int64_t n = Q.rows;
+  for (int e = t; e < k * k; e += blockDim.x) {
+    const int l = e % k, c = e / k;
+    sig[e] = Q.Gb ? Q.Gb[l + c * k] + Q.Gb[c + l * k] : 0.0;
+  }
+  const int64_t ntiles = (n + R - 1) / R;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t r0 = tile * R;
+    __syncthreads();
+    stage_rows(xs, Q.X, Q.ldx, r0, R, k, n);
+    __syncthreads();
+    for (int e = t; e < (R + 2) * k; e += blockDim.x) {
+      const int lr = e / k, a = e % k;
+      const double* xr = xs + lr * k;
+      double v = 0.0, d = 0.0;
+      if (Q.Gb)
+        for (int l = 0; l < k; ++l) v += xr[l] * sig[l + a * k];
+      if (Q.Tb)
+        for (int b = 0; b < k; ++b) {
+          const double* Tab = Q.Tb + (int64_t)k * (a + k * b);
+          double in = 0.0;
+          for (int c = 0; c < k; ++c) in += Tab[c] * xr[c];
+          d += xr[b] * in;
+        }
+      xsig[e] = v;
+      dxb[e] = d;
+    }
+    if (Q.Tb)
+      for (int e = t; e < R * k; e += blockDim.x) {
+        const int lr = e / k, a = e % k;
+        const int64_t i = r0 + lr;
+        dxs[e] = i < n ? apply_d(xs, lr, a, k, i, n, Q.dl, Q.dd, Q.du, Q.dper) : 0.0;
+      }
+    __syncthreads();
+    const int lr = t % R, c = t / R;
+    const int64_t i = r0 + lr;
+    if (c < k && i < n) {
+      double v = Q.base ? Q.base[i + c * Q.ldb] : 0.0;
+      if (Q.Tb) {
+        const double* dr = dxs + lr * k;
+        const double* xr = xs + (lr + 1) * k;
+        double tv = 0.0;
+        for (int a = 0; a < k; ++a) {
+          const double* Tac = Q.Tb + (int64_t)k * (a + k * c);  // Tb_{a c x}
+          double in1 = 0.0, in2 = 0.0;
+          for (int x = 0; x < k; ++x) {
+            in1 += Tac[x] * xr[x];
+            in2 += Q.Tb[c + (int64_t)k * (a + k * x)] * xr[x];  // Tb_{a x c}
+          }
+          tv += dr[a] * (in1 + in2);
+        }
+        v += tv + apply_dt(dxb, lr, c, k, i, n, Q.dl, Q.dd, Q.du, Q.dper);
+      }
+      if (Q.Gb && Q.st_d) v += apply_a(xsig, lr, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
+      for (int qq = 0; qq < Q.q; ++qq) v += Q.F[i + qq * Q.ldf] * Q.fb[qq + c * Q.q];
+      if (Q.X2)
+        for (int l = 0; l < k; ++l)
+          v += Q.X2[i + l * Q.ldx2] * (Q.transY2 ? Q.Y2[c + l * Q.ldy2] : Q.Y2[l + c * Q.ldy2]);
+      if (Q.X3)
+        for (int l = 0; l < k; ++l)
+          v += Q.X3[i + l * Q.ldx3] * (Q.transY3 ? Q.Y3[c + l * Q.ldy3] : Q.Y3[l + c * Q.ldy3]);
+      Q.out[i + c * Q.ldo] = v;
+    }
+  }
+}
+
+hipError_t launch_basis_vjp(const BasisVjpArgs& Q, hipStream_t st) {
+  if (Q.k < 1 || Q.k > kQuadMaxRank || Q.rows < 1 || !Q.X || !Q.out || Q.q < 0 || Q.q > kMaxForce ||
+      (Q.q > 0 && (!Q.F || !Q.fb)) || (Q.Gb && Q.st_d && !Q.st_e) || (Q.X2 && !Q.Y2) || (Q.X3 && !Q.Y3))
+    return hipErrorInvalidValue;
+  const int R = 256 / Q.k;
+  const int64_t tiles = (Q.rows + R - 1) / R;
+  const int grid = (int)std::min<int64_t>(tiles, 1024);
+  hipLaunchKernelGGL(k_basis_vjp, dim3(grid), dim3(256), 0, st, Q);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_quad_adj_scales(const double* sumK, const double* sumL, int len,
+                                                         const double* Gab0, const double* Ga0, const double* Gab1,
+                                                         const double* Ga1, const double* Gbb0, const double* Gb0,
+                                                         const double* Gbb1, const double* Gb1, int k, double* out) {
+  __shared__ double ra[256], rb[256];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int e = t; e < k * k; e += blockDim.x) {
+    a += Gab0[e] * Ga0[e] + Gab1[e] * Ga1[e];
+    b += Gbb0[e] * Gb0[e] + Gbb1[e] * Gb1[e];
+  }
+  ra[t] = a;
+  rb[t] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      ra[t] += ra[t + w];
+      rb[t] += rb[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = sumK[len - 1] + ra[0];
+    out[1] = sumL[len - 1] + rb[0];
+  }
+}
+
+hipError_t launch_quad_adj_scales(const double* sumK, const double* sumL, int len, const double* Gab0,
+                                  const double* Ga0, const double* Gab1, const double* Ga1, const double* Gbb0,
+                                  const double* Gb0, const double* Gbb1, const double* Gb1, int k, double* out,
+                                  hipStream_t st) {
+  if (k < 1 || k > kQuadMaxRank || len < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_quad_adj_scales, dim3(1), dim3(256), 0, st, sumK, sumL, len, Gab0, Ga0, Gab1, Ga1, Gbb0, Gb0,
+                     Gbb1, Gb1, k, out);
+  return hipGetLastError();
+}
+
+}  // namespace lri

@@ -1,0 +1,170 @@
+"""Pins of the CPU reverse- and forward-mode references of the matrix-ODE BUG step
+(oracle/adjoint_de.py): finite differences, forward/reverse duality, the RK4
+stability polynomial, and the setup of the differentiation workload."""
+import numpy as np
+import pytest
+
+from oracle import adjoint_de as ad
+
+
+def random_op(rng, n, m, q=1, periodic=False, scale=1.0):
+    def sym_tri(k, d0, e0):
+        T = np.diag(-rng.uniform(*d0, k)) + np.diag(e := rng.uniform(*e0, k - 1), 1) + np.diag(e, -1)
+        return T
+    A = scale * sym_tri(n, (20, 40), (5, 15))
+    B = scale * sym_tri(m, (10, 20), (2, 6))
+    if periodic:
+        A[0, -1] = A[-1, 0] = 7.0 * scale
+    D = np.diag(rng.uniform(-1, 1, n)) + np.diag(rng.uniform(-3, 3, n - 1), 1) + np.diag(rng.uniform(-3, 3, n - 1), -1)
+    if periodic:
+        D[0, -1], D[-1, 0] = 2.0, -1.5
+    return ad.ScaledQuadratic(A, B, rng.standard_normal((n, q)), rng.standard_normal((m, q)), D)
+
+
+def random_state(rng, n, m, r):
+    U = np.linalg.qr(rng.standard_normal((n, r)))[0]
+    V = np.linalg.qr(rng.standard_normal((m, r)))[0]
+    S = np.diag(np.linspace(1.0, 0.2, r)) + 0.05 * rng.standard_normal((r, r))
+    return U, S, V
+
+
+def pair(x, y):
+    return sum(float(np.sum(a * b)) for a, b in zip(x, y))
+
+
+def test_rk4_reverse_is_the_transposed_stability_polynomial():
+    rng = np.random.default_rng(0)
+    Lam = rng.standard_normal((5, 5))
+    h, N = 0.1, 4
+    P = sum(np.linalg.matrix_power(h * Lam, j) / f for j, f in enumerate([1, 1, 2, 6, 24]))
+    y0 = rng.standard_normal((5, 2))
+    ys = ad.rk4_states(lambda y: Lam @ y, y0, h, N)
+    assert np.allclose(ys[-1], np.linalg.matrix_power(P, N) @ y0, rtol=1e-13, atol=1e-13)
+    yb = rng.standard_normal((5, 2))
+    got = ad.rk4_vjp(lambda y: Lam @ y, lambda x, kb: Lam.T @ kb, ys, h, yb)
+    assert np.allclose(got, np.linalg.matrix_power(P.T, N) @ yb, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_step_vjp_matches_finite_differences(periodic):
+    rng = np.random.default_rng(1 + periodic)
+    n, m, r, dt, N = 14, 11, 3, 0.01, 3
+    F = random_op(rng, n, m, periodic=periodic)
+    U, S, V = random_state(rng, n, m, r)
+    C = [rng.standard_normal(x.shape) for x in (U, S, V)]  # loss <C, (U1, S1, V1)>
+    Ub, Sb, Vb, abar, bbar = ad.bug_de_step_vjp(F, U, S, V, dt, N, *C)
+
+    def loss(U_, S_, V_, a=1.0, b=1.0):
+        return pair(C, ad.bug_de_step(F.scaled(a, b), U_, S_, V_, dt, N))
+
+    dirs = [rng.standard_normal(x.shape) for x in (U, S, V)]
+    eps = 1e-6
+    fd = (loss(*(x + eps * d for x, d in zip((U, S, V), dirs))) - loss(*(x - eps * d for x, d in zip((U, S, V),
+                                                                                                     dirs)))) / (2 * eps)
+    an = pair((Ub, Sb, Vb), dirs)
+    assert abs(fd - an) <= 1e-7 * max(1.0, abs(an))
+    fa = (loss(U, S, V, 1 + eps) - loss(U, S, V, 1 - eps)) / (2 * eps)
+    fb = (loss(U, S, V, 1, 1 + eps) - loss(U, S, V, 1, 1 - eps)) / (2 * eps)
+    assert abs(fa - abar) <= 1e-7 * max(1.0, abs(abar))
+    assert abs(fb - bbar) <= 1e-7 * max(1.0, abs(bbar))
+
+
+def test_forward_and_reverse_mode_are_dual():
+    rng = np.random.default_rng(3)
+    n, m, r, dt, N = 16, 13, 4, 0.01, 2
+    F = random_op(rng, n, m, q=2, periodic=True)
+    U, S, V = random_state(rng, n, m, r)
+    xd = [rng.standard_normal(x.shape) for x in (U, S, V)]
+    ad_, bd_ = 0.7, -0.3
+    yb = [rng.standard_normal(x.shape) for x in (U, S, V)]
+    J = ad.bug_de_step_jvp(F, U, S, V, dt, N, *xd, ad_, bd_)
+    Ub, Sb, Vb, abar, bbar = ad.bug_de_step_vjp(F, U, S, V, dt, N, *yb)
+    lhs = pair(yb, J)
+    rhs = pair((Ub, Sb, Vb), xd) + abar * ad_ + bbar * bd_
+    assert abs(lhs - rhs) <= 1e-11 * max(1.0, abs(lhs))
+
+
+def test_checkpointed_solve_gradient_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    n, m, r, dt, N, steps = 12, 10, 3, 0.01, 2, 5
+    F = random_op(rng, n, m, periodic=True)
+    U, S, V = random_state(rng, n, m, r)
+    lg = ad.mean_column_loss
+    val, Ub, Sb, Vb, abar, bbar = ad.bug_de_solve_gradient(F, U, S, V, dt, N, steps, lg, checkpoint=2)
+    _, Ub2, Sb2, Vb2, abar2, _ = ad.bug_de_solve_gradient(F, U, S, V, dt, N, steps, lg, checkpoint=7)
+    for a, b in ((Ub, Ub2), (Sb, Sb2), (Vb, Vb2)):
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+    assert abs(abar - abar2) <= 1e-12 * abs(abar)
+
+    def loss(x, a=1.0):
+        return lg(*ad.bug_de_solve(F.scaled(a, 1.0), *x, dt, N, steps))[0]
+
+    dirs = [rng.standard_normal(x.shape) for x in (U, S, V)]
+    eps = 1e-6
+    fd = (loss([x + eps * d for x, d in zip((U, S, V), dirs)]) - loss([x - eps * d for x, d in zip((U, S, V), dirs)])) / (2 * eps)
+    assert abs(fd - pair((Ub, Sb, Vb), dirs)) <= 1e-7 * max(1.0, abs(fd))
+    fa = (loss((U, S, V), 1 + eps) - loss((U, S, V), 1 - eps)) / (2 * eps)
+    assert abs(fa - abar) <= 1e-7 * max(1.0, abs(fa))
+    assert val == pytest.approx(loss((U, S, V)), rel=1e-14)
+
+
+def test_mean_column_loss_gradient():
+    rng = np.random.default_rng(5)
+    U, S, V = random_state(rng, 9, 7, 3)
+    U = U + 0.1 * rng.standard_normal(U.shape)  # not orthonormal: the general rule
+    val, Ub, Sb, Vb = ad.mean_column_loss(U, S, V)
+    Y = U @ S @ V.T
+    assert val == pytest.approx(float(np.sum(Y.mean(axis=1) ** 2)), rel=1e-14)
+    dirs = [rng.standard_normal(x.shape) for x in (U, S, V)]
+    eps = 1e-6
+    f = lambda t: ad.mean_column_loss(*(x + t * d for x, d in zip((U, S, V), dirs)))[0]  # noqa: E731
+    assert (f(eps) - f(-eps)) / (2 * eps) == pytest.approx(pair((Ub, Sb, Vb), dirs), rel=1e-8)
+
+
+def test_periodic_burgers_and_sine_state():
+    n, m, r = 64, 20, 12
+    Lap, Grad = ad.periodic_burgers(n, 0.01)
+    x = np.arange(n) / n
+    s = np.sin(2 * np.pi * 3 * x)
+    # spectral exactness of the stencils on a sine: -nu (2 - 2 cos(k dx)) / dx^2 and sin(k dx) / dx
+    kdx = 2 * np.pi * 3 / n
+    assert np.allclose(Lap @ s, -0.01 * (2 - 2 * np.cos(kdx)) * n * n * s, atol=1e-10)
+    assert np.allclose(Grad @ s, np.sin(kdx) * n * np.cos(2 * np.pi * 3 * x), atol=1e-10)
+    rng = np.random.default_rng(6)
+    xi = rng.standard_normal((8, m))
+    U0, S0, V0, Y0 = ad.sine_state(n, xi, r, rng.standard_normal((m, r - 8)))
+    assert np.allclose(U0.T @ U0, np.eye(r), atol=1e-13) and np.allclose(V0.T @ V0, np.eye(r), atol=1e-13)
+    assert np.allclose(U0 @ S0 @ V0.T, Y0, atol=1e-12)
+    assert np.allclose(S0[8:], 0.0, atol=1e-12) and np.allclose(S0[:, 8:], 0.0, atol=1e-12)
+
+
+def test_package_setup_of_the_differentiation_workload_matches_the_oracle(lri):
+    """The package's SineBurgers / periodic_burgers_operator / mean_column_loss (host
+    set-up of the differentiation workload) against the restatements above, and the
+    xi pullback against central differences of the set-up."""
+    n, m, r, K = 64, 24, 12, 8
+    rng = np.random.default_rng(7)
+    xi = rng.standard_normal((K, m))
+    sb = lri.SineBurgers(n, xi, 0.01, r, seed=3)
+    U0, S0, V0, _ = ad.sine_state(n, xi, r, np.random.default_rng(3).standard_normal((m, r - K)))
+    assert np.allclose(sb.U0, U0, atol=1e-14) and np.allclose(sb.V0, V0, atol=1e-13)
+    assert np.allclose(sb.S0, S0, atol=1e-12)
+    Lap, Grad = ad.periodic_burgers(n, 0.01)
+    op = sb.op
+    assert np.allclose(op.A.dense(), Lap, atol=1e-9) and np.allclose(op.D.dense(), Grad, atol=1e-12)
+    assert np.allclose(op.B.dense(), 0.0)
+    U, S, V = random_state(rng, n, m, r)
+    for a, b in zip(lri.mean_column_loss(U, S, V), ad.mean_column_loss(U, S, V)):
+        assert np.allclose(a, b, rtol=1e-14, atol=1e-15)
+    # xi pullback: d/dt <C, (S0, V0)(xi + t d)>
+    Cs, Cv = rng.standard_normal((r, r)), rng.standard_normal((m, r))
+    d = rng.standard_normal(xi.shape)
+    eps = 1e-6
+
+    def f(t):
+        z = lri.SineBurgers(n, xi + t * d, 0.01, r, seed=3)
+        return float(np.sum(Cs * z.S0) + np.sum(Cv * z.V0))
+
+    fd = (f(eps) - f(-eps)) / (2 * eps)
+    an = float(np.sum(sb.pullback(np.zeros((n, r)), Cs, Cv) * d))
+    assert abs(fd - an) <= 1e-7 * max(1.0, abs(an))
