@@ -26,6 +26,7 @@ Reported beside the headline value:
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload de      (the matrix-ODE BUG step, n = m = 4096)
+       python bench.py --workload gradient  (the (nu, xi) gradient through a Burgers BUG solve)
 """
 from __future__ import annotations
 
@@ -681,6 +682,138 @@ def run_de(args, torch, lri):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------- differentiation workload
+
+
+def run_burgers_gradient(args, torch, lri):
+    """Gradient of L = ||mean_cols(Y(T))||^2 after a BUG solve of parametric viscous
+    Burgers (periodic, n = 2048 points, central differences, m = 1024 samples,
+    u0 = sum_{k=1..8} xi_k sin(2 pi k x) / k, nu = 1e-2, r = 16, dt = 1e-4, 200 steps)
+    with respect to nu and the 8 x m xi, by lri_step_de_bug_adjoint with a checkpoint
+    every 20 steps; FP64."""
+    n, m, r, K, nu, dt, steps, cp = 2048, 1024, 16, 8, 1e-2, 1e-4, 200, 20
+    xi = np.random.default_rng(args.seed).standard_normal((K, m))
+    sb = lri.SineBurgers(n, xi, nu, r, seed=args.seed)
+    prob = sb.problem((0.0, steps * dt))
+    N = lri.de_substeps(prob, dt)
+    alg = lri.UnconventionalAlgorithm()
+    log(f"Burgers gradient n={n} m={m} r={r} dt={dt}, {steps} steps, {N} RK4 sub-steps, checkpoint {cp}")
+    lri.solve_gradient(sb.problem((0.0, 3 * dt)), alg, lri.mean_column_loss, dt, substeps=N)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    val, (Ub, Sb, Vb), (abar, _) = lri.solve_gradient(prob, alg, lri.mean_column_loss, dt, checkpoint=cp,
+                                                      substeps=N)
+    el = time.perf_counter() - t0
+    g_xi = sb.pullback(Ub, Sb, Vb)
+    g_nu = abar / nu
+    t1 = time.perf_counter()
+    lri.solve(prob, alg, dt, substeps=N, save_everystep=False)
+    fwd_s = time.perf_counter() - t1
+    log(f"gradient {el:.3f} s (forward solve alone {fwd_s:.3f} s); L = {val:.6e}, dL/dnu = {g_nu:.6e}")
+
+    # kernel families over reverse steps from the initial state (HIP events on the engine's stream)
+    F = sb.op
+    ctx = lri.native.Context(n, m, r, 0)
+    ctx.set_state(sb.U0, sb.S0, sb.V0)
+    ctx.set_operator_lyapunov(F.A.d, F.A.e, True, F.B.d, F.B.e, False)
+    ctx.set_operator_quadratic(F.D.dl, F.D.dd, F.D.du, True)
+    rng = np.random.default_rng(args.seed + 3)
+    cot = [rng.standard_normal(x.shape) for x in (sb.U0, sb.S0, sb.V0)]
+    ctx.step_de_bug_adjoint(dt, N, *cot)
+    nrev = 20
+    ctx.timing(0 if args.no_events else 1)
+    ctx.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(nrev):
+        ctx.step_de_bug_adjoint(dt, N, *cot)
+    ctx.synchronize()
+    rev_s = (time.perf_counter() - t2) / nrev
+    fam = {}
+    for name, f in (("de_rows", lri.native.KERNEL_DE), ("small", lri.native.KERNEL_SMALL)):
+        ms, cnt = ctx.timing_read(f)
+        fam[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt if cnt else None}
+    ctx.timing(0)
+    ctx.close()
+    # algorithmic bytes of the row kernels of one reverse step: forward stages read X_s, X0, Acc and
+    # write X_n, Acc (5 rows x r doubles), reverse stages read X, Ybar, Xbar', Acc and write Xbar, Acc (6),
+    # each basis cotangent reads X, base, X2, X3 and writes out (5)
+    rows = n + m
+    per_step = 8 * r * rows * (N * 4 * 5 + N * (3 * 5 + 4 * 6) + 2 * 5)
+    de_ms = fam["de_rows"]["total_ms"]
+    achieved = per_step * nrev / (de_ms * 1e-3) / 1e9 if de_ms else None
+    out = {
+        "metric": "gradient of ||mean_cols(Y(T))||^2 w.r.t. (nu, xi) through a 200-step BUG Burgers solve "
+                  "(n=2048, m=1024, r=16): forward + checkpointed reverse steps/s",
+        "value": 2 * steps / el,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 3,
+        "ms_per_step": 1e3 * el / (2 * steps),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"BUG solve of parametric viscous Burgers on a periodic grid (n={n} points, central "
+                               f"differences, m={m} samples, u0 = sum_(k<=8) xi_k sin(2 pi k x)/k, nu={nu}), r={r}, "
+                               f"dt={dt}, {steps} steps with {N} RK4 sub-steps per sub-problem; gradient by the "
+                               f"hand-written adjoint with a checkpoint every {cp} steps (forward pass, then "
+                               f"recomputation and reverse step per step)", "n": n, "m": m, "r": r, "dt": dt,
+                   "steps": steps, "substeps": N, "checkpoint": cp, "parallelism": "single GPU"},
+        "gradient_s": el,
+        "forward_solve_s": fwd_s,
+        "reverse_step_ms": 1e3 * rev_s,
+        "loss": val,
+        "dL_dnu": g_nu,
+        "dL_dxi_norm": float(np.linalg.norm(g_xi)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": "row kernels of the reverse step (RK4 stages, reverse stages, basis cotangents)",
+                     "bytes_per_launch": None, "bytes_per_reverse_step": per_step},
+        "kernels": fam,
+    }
+    if not args.no_check:
+        d_xi = np.random.default_rng(args.seed + 5).standard_normal(xi.shape)
+        d_xi /= np.linalg.norm(d_xi)
+
+        def L(x):
+            s = lri.SineBurgers(n, x, nu, r, seed=args.seed)
+            y = lri.solve(s.problem((0.0, steps * dt)), alg, dt, substeps=N, save_everystep=False).Y[-1]
+            return lri.mean_column_loss(y.U, y.S, y.V)[0]
+
+        h = 1e-4
+        fd = (L(xi + h * d_xi) - L(xi - h * d_xi)) / (2 * h)
+        an = float(np.sum(g_xi * d_xi))
+        out["check"] = {"directional_derivative_adjoint": an, "central_difference": fd,
+                        "rel_diff": abs(an - fd) / abs(fd), "h": h}
+        log(f"d/dxi along a unit direction: adjoint {an:.10e}, central difference {fd:.10e}")
+    if not args.no_cpu:
+        import scipy.sparse as sps
+        from oracle import adjoint_de
+
+        Lap, Grad = adjoint_de.periodic_burgers(n, nu)
+        host = adjoint_de.ScaledQuadratic(sps.csr_matrix(Lap), sps.csr_matrix((m, m)), np.zeros((n, 0)),
+                                          np.zeros((m, 0)), sps.csr_matrix(Grad))
+        t3 = time.perf_counter()
+        adjoint_de.bug_de_step_vjp(host, sb.U0, sb.S0, sb.V0, dt, N, *cot)
+        cel = time.perf_counter() - t3
+        try:
+            from threadpoolctl import threadpool_info
+
+            threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+        except Exception:
+            threads = os.cpu_count() or 1
+        out["cpu_baseline"] = {"value": 1.0 / cel, "unit": "reverse steps/s", "cores": int(threads), "kind": "port",
+                               "sample": f"1 reverse step (with its forward recomputation) of the FP64 NumPy/SciPy "
+                                         f"oracle's dense adjoint at the same size and {N} RK4 sub-steps; engine: "
+                                         f"{1e3 * rev_s:.2f} ms per reverse step; OpenBLAS threads={threads}"}
+        log(f"CPU oracle: {cel:.2f} s per reverse step")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -698,11 +831,13 @@ def main():
                     help="no HIP-event kernel timing in the timed region (for external profilers)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--workload", choices=["ksl", "de", "greedy", "burgers", "batched", "adjoint"], default="ksl",
+    ap.add_argument("--workload", choices=["ksl", "de", "greedy", "burgers", "batched", "adjoint", "gradient"],
+                    default="ksl",
                     help="ksl: the headline data integrator; de: the BUG matrix-ODE step at n=m=4096; "
                          "greedy: the Greedy data integrator on the headline snapshots; burgers: BUG on the "
                          "reference's Burgers problem; batched: 512 uneven rank-adaptive DE solves; adjoint: "
-                         "reverse BUG steps on the headline snapshots")
+                         "reverse BUG steps on the headline snapshots; gradient: (nu, xi) gradient through a "
+                         "parametric Burgers BUG solve")
     ap.add_argument("--de-n", type=int, default=4096)
     ap.add_argument("--de-steps", type=int, default=20)
     ap.add_argument("--warmup-de", type=int, default=2)
@@ -732,6 +867,9 @@ def main():
         return
     if args.workload == "batched":
         run_batched(args, torch, lri)
+        return
+    if args.workload == "gradient":
+        run_burgers_gradient(args, torch, lri)
         return
     dev = torch.device("cuda:0")
     n = m = args.n
