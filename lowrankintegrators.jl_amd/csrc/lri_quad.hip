@@ -115,81 +115,106 @@ hipError_t launch_tensor3(const TensorArgs& Q, double* out, hipStream_t st) {
 }
 
 // ----------------------------------------------------------------------------
-// One RK4 stage of the K or L sub-problem over all rows.  Workgroup: R = 256/k
-// rows x k columns, thread (row lr = t % R, column c = t / R).  The stage's
-// rows (with one halo row on each side) are staged in LDS, then
-//   f_ic = (A Xs)_ic + (Xs M)_ic + (g hv)_ic - sum_ab (P Xs)_ia Xs_ib T_abc
-// (P = D when Q.dd, else the identity) and the RK4 bookkeeping of stage s:
+// One RK4 stage of the K or L sub-problem over all rows.  Workgroup: a tile of
+// 16 rows (4 waves).  The tile's rows (with one halo row on each side) are
+// staged in LDS, then
+//   f_ic = (A Xs)_ic + (Xs M)_ic + (g hv)_ic - N_ic,  N_ic = sum_ab (P Xs)_ia Xs_ib T_abc
+// (P = D when Q.dd, else the identity), and the RK4 bookkeeping of stage s:
 //   s < 3: Acc (=, +=) w_s f, Xn = X0 + c_s h f;   s = 3: X0 += h/6 (Acc + f).
+// The quadratic term is a product on the FP64 matrix cores,
+//   N (16 x k) = Z (16 x k^2) T (k^2 x k),  Z_i,(a + k b) = (P Xs)_ia Xs_ib,
+// T read once per tile in the k^2 order of its storage T[c + k (a + k b)]:
+// wave w takes column block w % nct (nct = ceil(k / 16)) and the w / nct-th of
+// the 4 / nct slices of the k^2 reduction; the slices are summed through LDS.
 // ----------------------------------------------------------------------------
 
+constexpr int kStageRows = 16;
+
 __global__ __launch_bounds__(256) void k_quad_stage(QuadStageArgs Q) {
-  __shared__ double xs[256 + 2 * kQuadMaxRank];  // (R + 2) x k rows of the stage input
-  __shared__ double ps[256];                     // R x k rows of P Xs
+  __shared__ double xs[(kStageRows + 2) * kQuadMaxRank];  // rows r0 - 1 .. r0 + 16: xs[lr * k + c]
+  __shared__ double ps[kStageRows * kQuadMaxRank];        // (P Xs) rows r0 .. r0 + 15
+  __shared__ double nl[4][kStageRows * kQuadMaxRank];     // slices of N, [row * 32 + c]
   const int k = Q.k, t = threadIdx.x;
-  const int R = 256 / k;
-  const int64_t r0 = (int64_t)blockIdx.x * R;
+  const int lane = t & 63, wave = t >> 6, lr = lane & 15, lk = lane >> 4;
+  const int64_t r0 = (int64_t)blockIdx.x * kStageRows;
   const int64_t n = Q.n;
-  // rows r0 - 1 .. r0 + R into xs[lr * k + c]; a row outside [0, n) takes its
-  // periodic image (used only by a periodic stencil)
-  for (int e = t; e < (R + 2) * k; e += blockDim.x) {
-    const int lr = e % (R + 2), c = e / (R + 2);
-    int64_t ir = r0 + lr - 1;
+  // a row outside [0, n) takes its periodic image (used only by a periodic stencil)
+  for (int e = t; e < (kStageRows + 2) * k; e += blockDim.x) {
+    const int lrow = e % (kStageRows + 2), c = e / (kStageRows + 2);
+    int64_t ir = r0 + lrow - 1;
     if (ir < 0) ir += n;
     else if (ir >= n) ir -= n;
-    xs[lr * k + c] = (ir >= 0 && ir < n) ? Q.Xs[ir + c * Q.ld] : 0.0;
+    xs[lrow * k + c] = (ir >= 0 && ir < n) ? Q.Xs[ir + c * Q.ld] : 0.0;
   }
   __syncthreads();
-  const int lr = t % R, c = t / R;
-  const int64_t i = r0 + lr;
-  const bool act = c < k && lr < R && i < n;
-  if (act) {
-    const double xm = xs[lr * k + c], x0 = xs[(lr + 1) * k + c], xp = xs[(lr + 2) * k + c];
-    const bool hasl = i > 0 || Q.dper, hasu = i < n - 1 || Q.dper;
-    ps[lr * k + c] = Q.dd ? (Q.dd[i] * x0 + (hasl ? Q.dl[i] * xm : 0.0) + (hasu ? Q.du[i] * xp : 0.0)) : x0;
-  }
-  __syncthreads();
-  if (!act) return;
-  const double* xrow = xs + (lr + 1) * k;
-  const double xm = xs[lr * k + c], x0 = xrow[c], xp = xs[(lr + 2) * k + c];
-  double f = 0.0;
-  // A Xs: symmetric tridiagonal, e[i] couples rows i and i + 1 (e[n-1]: n-1 and 0 when periodic)
-  if (Q.st_d) {
-    f = Q.st_d[i] * x0;
-    if (i > 0) f += Q.st_e[i - 1] * xm;
-    else if (Q.periodic) f += Q.st_e[n - 1] * xm;
-    if (i < n - 1) f += Q.st_e[i] * xp;
-    else if (Q.periodic) f += Q.st_e[n - 1] * xp;
-  }
-  if (Q.M)
-    for (int l = 0; l < k; ++l) f += xrow[l] * Q.M[l + c * k];
-  for (int qq = 0; qq < Q.q; ++qq) f += Q.g[i + qq * Q.ldg] * Q.hv[qq + c * Q.q];
-  if (Q.T) {
-    const double* prow = ps + lr * k;
-    double nl = 0.0;
-    for (int a = 0; a < k; ++a) {
-      double inner = 0.0;
-      const double* Ta = Q.T + (int64_t)k * a;  // T[c + k (a + k b)]
-      for (int b = 0; b < k; ++b) inner += xrow[b] * Ta[c + (int64_t)k * k * b];
-      nl += prow[a] * inner;
+  for (int e = t; e < kStageRows * k; e += blockDim.x) {
+    const int lrow = e % kStageRows, c = e / kStageRows;
+    const int64_t i = r0 + lrow;
+    double p = 0.0;
+    if (i < n) {
+      const double xm = xs[lrow * k + c], x0 = xs[(lrow + 1) * k + c], xp = xs[(lrow + 2) * k + c];
+      const bool hasl = i > 0 || Q.dper, hasu = i < n - 1 || Q.dper;
+      p = Q.dd ? (Q.dd[i] * x0 + (hasl ? Q.dl[i] * xm : 0.0) + (hasu ? Q.du[i] * xp : 0.0)) : x0;
     }
-    f -= nl;
+    ps[lrow * k + c] = p;
   }
-  const int64_t o = i + c * Q.ld;
-  const int s = Q.stage;
-  if (s < 3) {
-    Q.Acc[o] = (s == 0 ? 0.0 : Q.Acc[o]) + stage_w(s) * f;
-    Q.Xn[o] = Q.X0[o] + stage_c(s) * Q.h * f;
-  } else {
-    Q.X0[o] = Q.X0[o] + (Q.h / 6.0) * (Q.Acc[o] + f);
+  __syncthreads();
+  const int nct = (k + 15) / 16, nch = 4 / nct;
+  if (Q.T) {
+    const int ct = wave % nct, ch = wave / nct;
+    const int kk2 = k * k;
+    const int clen = (kk2 + 4 * nch - 1) / (4 * nch) * 4;
+    const int kbeg = ch * clen, kend = min(kbeg + clen, kk2);
+    const int c = ct * 16 + lr;
+    f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+    // the tensor operand of the next k-step is loaded before this one's product
+    double bn = (kbeg + lk < kend && c < k) ? Q.T[c + (int64_t)k * (kbeg + lk)] : 0.0;
+    for (int kk0 = kbeg; kk0 < kend; kk0 += 4) {
+      const int kk = kk0 + lk;
+      const double b = bn;
+      bn = (kk + 4 < kend && c < k) ? Q.T[c + (int64_t)k * (kk + 4)] : 0.0;
+      const double a = kk < kend ? ps[lr * k + kk % k] * xs[(lr + 1) * k + kk / k] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) nl[ch][(4 * u + lk) * kQuadMaxRank + c] = acc[u];
+  }
+  __syncthreads();
+  for (int e = t; e < kStageRows * k; e += blockDim.x) {
+    const int lrow = e % kStageRows, c = e / kStageRows;
+    const int64_t i = r0 + lrow;
+    if (i >= n) continue;
+    const double* xrow = xs + (lrow + 1) * k;
+    const double xm = xs[lrow * k + c], x0 = xrow[c], xp = xs[(lrow + 2) * k + c];
+    double f = 0.0;
+    // A Xs: symmetric tridiagonal, e[i] couples rows i and i + 1 (e[n-1]: n-1 and 0 when periodic)
+    if (Q.st_d) {
+      f = Q.st_d[i] * x0;
+      if (i > 0) f += Q.st_e[i - 1] * xm;
+      else if (Q.periodic) f += Q.st_e[n - 1] * xm;
+      if (i < n - 1) f += Q.st_e[i] * xp;
+      else if (Q.periodic) f += Q.st_e[n - 1] * xp;
+    }
+    if (Q.M)
+      for (int l = 0; l < k; ++l) f += xrow[l] * Q.M[l + c * k];
+    for (int qq = 0; qq < Q.q; ++qq) f += Q.g[i + qq * Q.ldg] * Q.hv[qq + c * Q.q];
+    if (Q.T)
+      for (int ch = 0; ch < nch; ++ch) f -= nl[ch][lrow * kQuadMaxRank + c];
+    const int64_t o = i + c * Q.ld;
+    const int s = Q.stage;
+    if (s < 3) {
+      Q.Acc[o] = (s == 0 ? 0.0 : Q.Acc[o]) + stage_w(s) * f;
+      Q.Xn[o] = Q.X0[o] + stage_c(s) * Q.h * f;
+    } else {
+      Q.X0[o] = Q.X0[o] + (Q.h / 6.0) * (Q.Acc[o] + f);
+    }
   }
 }
 
 hipError_t launch_quad_stage(const QuadStageArgs& Q, hipStream_t st) {
   if (Q.k < 1 || Q.k > kQuadMaxRank || Q.n < 1 || Q.stage < 0 || Q.stage > 3 || Q.q < 0 || Q.q > kMaxForce)
     return hipErrorInvalidValue;
-  const int R = 256 / Q.k;
-  const int64_t blocks = (Q.n + R - 1) / R;
+  const int64_t blocks = (Q.n + kStageRows - 1) / kStageRows;
   hipLaunchKernelGGL(k_quad_stage, dim3((unsigned)blocks), dim3(256), 0, st, Q);
   return hipGetLastError();
 }
