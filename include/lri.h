@@ -159,6 +159,21 @@ int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* U
 int lri_step_ksl_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
                          int64_t ldsb, double* Vb, int64_t ldvb);
 
+/* Reverse of the rank-adaptive BUG data step (rank_adaptive_unconventional.jl:182-233)
+ * with the new rank r_new held fixed (the rank the forward step returned; the
+ * recomputed step must reproduce it, else LRI_EINVAL): on entry the first
+ * r_new columns of Ub (n rows) and Vb (m rows) and the r_new x r_new Sb hold the
+ * cotangents of the result (U1, S1 = diag(sigma), V1) -- only Sb's diagonal
+ * enters -- on exit the first r columns and the r x r Sb hold those of
+ * (U0, S0, V0); the arrays hold max(r, r_new) columns.  The step is recomputed
+ * up to the SVD of the 2r x 2r core (three passes), then the SVD rule (distinct
+ * singular values), the augmented bases' QR rule and four more passes.  The
+ * derivative is the one along orthonormal bases (the step assumes U0'U0 = I).
+ * The reference has no adjoint (Julia AD would differentiate
+ * rankadaptive_unconventional_step!). */
+int lri_step_bug_ra_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax, int64_t r_new,
+                            double* Ub, int64_t ldub, double* Sb, int64_t ldsb, double* Vb, int64_t ldvb);
+
 /* Greedy step on an SVD-like state (greedy_integrator.jl:94-104,106-111)
  * with X = A[k_next] itself (update_data!, data_integrator.jl:22-28):
  * U1 = orth(X V0), V1 = orth(X' U0), S1 = U1' X V1.  Three passes over X. */

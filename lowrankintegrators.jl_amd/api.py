@@ -755,9 +755,10 @@ def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, preci
     checkpointed recomputation: states are kept every ``checkpoint`` steps on the way
     forward and recomputed segment by segment on the way back.
 
-    - array-form data (``MatrixDataProblem``), ``alg`` ``UnconventionalAlgorithm()`` or
-      ``ProjectorSplitting(PrimalLieTrotter())``: ``lri_step_bug_adjoint`` /
-      ``lri_step_ksl_adjoint``; returns ``(value, (Ub0, Sb0, Vb0))``;
+    - array-form data (``MatrixDataProblem``), ``alg`` ``UnconventionalAlgorithm()``,
+      ``RankAdaptiveUnconventionalAlgorithm(tol; rmax)`` (the ranks of the forward solve held
+      fixed) or ``ProjectorSplitting(PrimalLieTrotter())``: ``lri_step_bug_adjoint`` /
+      ``lri_step_bug_ra_adjoint`` / ``lri_step_ksl_adjoint``; returns ``(value, (Ub0, Sb0, Vb0))``;
     - a matrix ODE (``MatrixDEProblem`` with a ``QuadraticOperator``; D = 0 for a linear
       one) and ``UnconventionalAlgorithm()``: ``lri_step_de_bug_adjoint``, with ``dt`` and
       the RK4 ``substeps`` as ``solve``; returns ``(value, (Ub0, Sb0, Vb0), (abar, bbar))``,
@@ -772,10 +773,19 @@ def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, preci
     if isinstance(prob, MatrixDEProblem):
         return _solve_gradient_de(prob, alg, loss_grad, dt, checkpoint, substeps, device)
     ksl = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, PrimalLieTrotter)
-    if not (ksl or isinstance(alg, UnconventionalAlgorithm)):
-        raise TypeError("the adjoint is built for the unconventional (BUG) and primal KSL integrators")
-    fwd = (lambda c, a, b: c.step_ksl(native.PRIMAL, a, b)) if ksl else (lambda c, a, b: c.step_bug(a, b))
-    rev = (lambda c, *x: c.step_ksl_adjoint(*x)) if ksl else (lambda c, *x: c.step_bug_adjoint(*x))
+    ra = isinstance(alg, RankAdaptiveUnconventionalAlgorithm)
+    if not (ksl or ra or isinstance(alg, UnconventionalAlgorithm)):
+        raise TypeError("the adjoint is built for the unconventional (BUG), rank-adaptive BUG and primal KSL "
+                        "integrators")
+    if ksl:
+        fwd = lambda c, a, b: c.step_ksl(native.PRIMAL, a, b)  # noqa: E731
+        rev = lambda c, a, b, *x: c.step_ksl_adjoint(a, b, *x)  # noqa: E731
+    elif ra:
+        fwd = lambda c, a, b: c.step_bug_ra(a, b, alg.tol, alg.rmax)  # noqa: E731
+        rev = lambda c, a, b, *x: c.step_bug_ra_adjoint(a, b, alg.tol, alg.rmax, *x)  # noqa: E731
+    else:
+        fwd = lambda c, a, b: c.step_bug(a, b)  # noqa: E731
+        rev = lambda c, a, b, *x: c.step_bug_adjoint(a, b, *x)  # noqa: E731
     if not isinstance(prob, MatrixDataProblem) or callable(prob.y):
         raise TypeError("solve_gradient takes a MatrixDataProblem with array-form data")
     integ = init(prob, alg, 1, precision=precision, snapshot_dtype=snapshot_dtype, save_everystep=False,

@@ -153,6 +153,7 @@ struct lri_ctx {
   int ra_k2 = 0;
   bool defer_rank_copy = false;
   BugAdjSmall* adj = nullptr;  // r x r work of the BUG adjoint (allocated on first use)
+  RaAdjSmall* radj = nullptr;  // 2r x 2r work of the rank-adaptive adjoint (allocated on first use)
   // captured rank-adaptive DE steps (up to the SVD), one per rank and step parameters
   struct RaGraph {
     int64_t r, nsub, rmax;
@@ -1273,6 +1274,287 @@ struct lri_ctx {
     return ra_finish(k2, tol, rmax);
   }
 
+  // Reverse of rankadaptive_unconventional_step! on data
+  // (rank_adaptive_unconventional.jl:194-233), the new rank r1 held fixed: with
+  // the state (U0, S0, V0) of the step and the cotangents (Ub1, Sb1, Vb1) of its
+  // result (n x r1, r1 x r1, m x r1) in P, radj->Sb1 and Q, leaves those of
+  // (U0, S0, V0) in P[:, :r], radj->Sb0 and Q[:, :r].  The forward step is
+  // recomputed up to the SVD Shat = Us Sig Vs' (three passes) with
+  //   Uhat = [U0 | Q_U], Q_U R_U = Z_U = (I - U0 U0') K1,  K1 = U0 S0 + dA V0
+  // (the reference's Q([K1 U0]) spans the same space, and U1 S1 V1' does not
+  // depend on the basis); then
+  //   Uhbar = Ub1 Us[:, :r1]',  Shbar = the SVD rule on (Uhat'Ub1, diag Sb1, Vhat'Vb1),
+  //   Uhbar += U0 (S0 N' Shbar') + dA (Vhat Shbar'),  Vhbar likewise (two passes),
+  //   Zbar = (Qbar + Q_U copyltu(-Qbar'Q_U)) R_U^{-T} for Qbar = Uhbar[:, r:],
+  //   Kbar1 = Zbar - U0 (U0'Zbar),
+  //   Ub0 = Uhbar[:, :r] + Uhat (Shbar N S0') - Zbar (K1'U0) - K1 (Zbar'U0) + Kbar1 S0' + dA Lbar1,
+  //   Vb0 likewise, Sb0 = M' Shbar N (U0'Kbar1 and Lbar1'V0 vanish for orthonormal bases).
+  // Seven passes over dA.  The rule needs distinct singular values of Shat and
+  // full-rank completions.
+  void bug_ra_adjoint(int64_t kp, int64_t kn, double tol, int64_t rmax, int r1) {
+    const int rr = (int)r;
+    const int k2 = 2 * rr;
+    const int mode = pass_mode();
+    const size_t rsz = sizeof(double) * rr * rr;
+    RaAdjSmall* A = radj;
+    // forward, as bug_ra up to the SVD (the state is left unchanged)
+    ensure_packed(true);
+    ensure_packed(false);
+    const int sN = passN(kp, kn, rr, partN);
+    const int sT = passT(kp, kn, rr, partT);
+    for (int side = 0; side < 2; ++side) {
+      const bool ns = side == 0;
+      const int64_t rows_ = ns ? n : m;
+      with_parts(ns ? partN : partT, [&](auto X) {  // K1 = U S + dA V ; L1 = V S' + dA' U
+        X.rows = rows_;
+        X.nsplit = ns ? sN : sT;
+        X.rb = rb_for(mode, rr);
+        X.r_out = rr;
+        X.r_in = rr;
+        X.X = ns ? U : V;
+        X.ldx = rows_;
+        X.Y = S;
+        X.ldy = (int)rcap;
+        X.transY = ns ? 0 : 1;
+        X.out = ns ? K : L;
+        X.ldo = rows_;
+        rows(X);
+      });
+      HIPCHECK(hipMemcpyAsync(ns ? Uhat : Vhat, ns ? U : V, sizeof(double) * rows_ * rr, hipMemcpyDeviceToDevice,
+                              stream));
+      EpiArgs e;
+      e.kind = ns ? EPI_RA_M : EPI_RA_N;
+      if (!ns) {
+        e.S = S;
+        e.ld_s = (int)rcap;
+      }
+      double* t1 = ns ? K2 : L2;
+      complete_basis(ns ? K : L, ns ? U : V, rows_, rr, t1, t1 + rows_ * rcap, (ns ? Uhat : Vhat) + rows_ * rr, e);
+      HIPCHECK(hipMemcpyAsync(ns ? A->RU : A->RV, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    }
+    pack(Vhat, m, m, k2, true);
+    const int sN2 = passN(kp, kn, k2, partN);
+    with_parts(partN, [&](auto X) {  // Shat = M S0 N' + Uhat' dA Vhat
+      X.rows = n;
+      X.nsplit = sN2;
+      X.rb = rb_for(mode, k2);
+      X.r_out = k2;
+      X.reduce = 1;
+      X.W = Uhat;
+      X.ldw = n;
+      X.rw = k2;
+      X.epi.kind = EPI_RA_S;
+      rows(X);
+    });
+    ra_svd(k2, tol, rmax);
+    HIPCHECK(hipStreamSynchronize(stream));
+    REQUIRE(*r_host == r1, "the cotangents' rank differs from the step's new rank (recomputed: " +
+                               std::to_string(*r_host) + ")");
+    // Uhbar = Ub1 Us[:, :r1]' (K2 / L2) and Uhat' Ub1 (radj->Ubs / Vbs)
+    for (int side = 0; side < 2; ++side) {
+      const bool ns = side == 0;
+      const int64_t rows_ = ns ? n : m;
+      RowArgs<double> X;
+      X.rows = rows_;
+      X.r_out = k2;
+      X.r_in = r1;
+      X.X = ns ? P : Q;
+      X.ldx = rows_;
+      X.Y = ns ? small->Ush : small->Vsh;
+      X.ldy = k2;
+      X.transY = 1;
+      X.out = ns ? K2 : L2;
+      X.ldo = rows_;
+      rows(X);
+      RowArgs<double> Y;
+      Y.rows = rows_;
+      Y.r_out = r1;
+      Y.base = ns ? P : Q;
+      Y.ld_base = rows_;
+      Y.reduce = 1;
+      Y.W = ns ? Uhat : Vhat;
+      Y.ldw = rows_;
+      Y.rw = k2;
+      Y.epi.kind = EPI_COPY;
+      Y.epi.transpose = 1;
+      Y.epi.dst = ns ? A->Ubs : A->Vbs;
+      rows(Y);
+    }
+    for (int stage = 0; stage < 5; ++stage)
+      timed(LRI_KERNEL_SMALL, [&] { return launch_ra_adj_small(A, small, S, (int)rcap, rr, r1, stage, stream); });
+    // pass operands Vhat Shbar' (N) and Uhat Shbar (T); Uhbar += parts + U0 G1 -> P, Vhbar likewise -> Q
+    for (int side = 0; side < 2; ++side) {
+      const bool ns = side == 0;
+      RowArgs<double> X;
+      X.rows = ns ? m : n;
+      X.r_out = k2;
+      X.r_in = k2;
+      X.X = ns ? Vhat : Uhat;
+      X.ldx = X.rows;
+      X.Y = A->Shb;
+      X.ldy = k2;
+      X.transY = ns ? 1 : 0;
+      set_pack_outputs(X, ns, mode, rb_for(mode, k2));
+      rows(X);
+    }
+    const int sN3 = passN(kp, kn, k2, partN);
+    const int sT3 = passT(kp, kn, k2, partT);
+    for (int side = 0; side < 2; ++side) {
+      const bool ns = side == 0;
+      const int64_t rows_ = ns ? n : m;
+      with_parts(ns ? partN : partT, [&](auto X) {
+        X.rows = rows_;
+        X.nsplit = ns ? sN3 : sT3;
+        X.rb = rb_for(mode, k2);
+        X.r_out = k2;
+        X.r_in = rr;
+        X.base = ns ? K2 : L2;
+        X.ld_base = rows_;
+        X.X = ns ? U : V;
+        X.ldx = rows_;
+        X.Y = ns ? A->G1 : A->G2;
+        X.ldy = rr;
+        X.out = ns ? P : Q;
+        X.ldo = rows_;
+        rows(X);
+      });
+    }
+    // the completions' reverse, and Ub0 / Vb0 up to the last pass
+    for (int side = 0; side < 2; ++side) {
+      const bool ns = side == 0;
+      const int64_t rows_ = ns ? n : m;
+      double* acc = ns ? P : Q;                 // [:, :r] Ub0 so far, [:, r:2r] Qbar
+      const double* Qbar = acc + rows_ * rr;
+      const double* Qc = (ns ? Uhat : Vhat) + rows_ * rr;
+      const double* B0 = ns ? U : V;
+      const double* X1 = ns ? K : L;
+      double* Zb = ns ? K2 : L2;
+      double* tmp = Zb + rows_ * rcap;
+      double* Xb1 = ns ? U2 : V2;               // Kbar1 / Lbar1
+      {
+        RowArgs<double> X;  // copyltu(-Qbar' Q)
+        X.rows = rows_;
+        X.r_out = rr;
+        X.base = Qbar;
+        X.ld_base = rows_;
+        X.reduce = 1;
+        X.W = Qc;
+        X.ldw = rows_;
+        X.rw = rr;
+        X.epi.kind = EPI_COPYLTU;
+        X.epi.dst = ns ? A->CqU : A->CqV;
+        rows(X);
+      }
+      {
+        RowArgs<double> X;  // tmp = Qbar + Q Cq
+        X.rows = rows_;
+        X.r_out = rr;
+        X.r_in = rr;
+        X.base = Qbar;
+        X.ld_base = rows_;
+        X.X = Qc;
+        X.ldx = rows_;
+        X.Y = ns ? A->CqU : A->CqV;
+        X.ldy = rr;
+        X.out = tmp;
+        X.ldo = rows_;
+        rows(X);
+      }
+      {
+        RowArgs<double> X;  // Zbar = tmp R^{-T}; Zbar' B0
+        X.rows = rows_;
+        X.r_out = rr;
+        X.r_in = rr;
+        X.X = tmp;
+        X.ldx = rows_;
+        X.Y = ns ? A->RUinvT : A->RVinvT;
+        X.ldy = rr;
+        X.out = Zb;
+        X.ldo = rows_;
+        X.reduce = 1;
+        X.W = B0;
+        X.ldw = rows_;
+        X.rw = rr;
+        X.epi.kind = EPI_COPY;
+        X.epi.dst = ns ? A->ZU : A->ZV;
+        rows(X);
+      }
+      {
+        RowArgs<double> X;  // K1' U0
+        X.rows = rows_;
+        X.r_out = rr;
+        X.base = X1;
+        X.ld_base = rows_;
+        X.reduce = 1;
+        X.W = B0;
+        X.ldw = rows_;
+        X.rw = rr;
+        X.epi.kind = EPI_COPY;
+        X.epi.dst = ns ? A->KU : A->LV;
+        rows(X);
+      }
+      {
+        RowArgs<double> X;  // Kbar1 = Zbar - U0 (Zbar'U0)', packed for the last pass
+        X.rows = rows_;
+        X.r_out = rr;
+        X.r_in = rr;
+        X.base = Zb;
+        X.ld_base = rows_;
+        X.X = B0;
+        X.ldx = rows_;
+        X.Y = ns ? A->ZU : A->ZV;
+        X.ldy = rr;
+        X.transY = 1;
+        X.xy_sign = -1.0;
+        X.out = Xb1;
+        X.ldo = rows_;
+        set_pack_outputs(X, !ns, mode, rb_for(mode, rr));  // Kbar1 for pass T, Lbar1 for pass N
+        rows(X);
+      }
+      // acc[:, :r] += Xhat G3 - Zbar (K1'U0) - K1 (Zbar'U0) + Kbar1 S0' (V side: ... + Lbar1 S0)
+      const double* Xs[4] = {ns ? Uhat : Vhat, Zb, X1, Xb1};
+      const double* Ys[4] = {ns ? A->G3 : A->G4, ns ? A->KU : A->LV, ns ? A->ZU : A->ZV, S};
+      const int ri[4] = {k2, rr, rr, rr};
+      const int ldys[4] = {k2, rr, rr, (int)rcap};
+      const int trs[4] = {0, 0, 0, ns ? 1 : 0};
+      const double sg[4] = {1.0, -1.0, -1.0, 1.0};
+      for (int u = 0; u < 4; ++u) {
+        RowArgs<double> X;
+        X.rows = rows_;
+        X.r_out = rr;
+        X.r_in = ri[u];
+        X.base = acc;
+        X.ld_base = rows_;
+        X.X = Xs[u];
+        X.ldx = rows_;
+        X.Y = Ys[u];
+        X.ldy = ldys[u];
+        X.transY = trs[u];
+        X.xy_sign = sg[u];
+        X.out = acc;
+        X.ldo = rows_;
+        rows(X);
+      }
+    }
+    const int sN4 = passN(kp, kn, rr, partN);  // dA Lbar1
+    const int sT4 = passT(kp, kn, rr, partT);  // dA' Kbar1
+    for (int side = 0; side < 2; ++side) {
+      const bool ns = side == 0;
+      with_parts(ns ? partN : partT, [&](auto X) {
+        X.rows = ns ? n : m;
+        X.nsplit = ns ? sN4 : sT4;
+        X.rb = rb_for(mode, rr);
+        X.r_out = rr;
+        X.base = ns ? P : Q;
+        X.ld_base = X.rows;
+        X.out = ns ? P : Q;
+        X.ldo = X.rows;
+        rows(X);
+      });
+    }
+    pkN_mode = pkT_mode = -1;  // the panels hold adjoint operands now
+  }
+
   // SVD of the 2r x 2r core small->Shat, truncation, and the rotation of Uhat,
   // Vhat into the new state (rank_adaptive_unconventional.jl:222-231).
   // ra_finish in two halves, so that a batch of contexts can queue the SVDs of
@@ -2381,6 +2663,7 @@ void free_ctx(lri_ctx* c) {
       if (p) (void)hipFree(p);
   if (c->r_host) (void)hipHostFree(c->r_host);
   if (c->adj) (void)hipFree(c->adj);
+  if (c->radj) (void)hipFree(c->radj);
   if (c->qadj) (void)hipFree(c->qadj);
   if (c->qadj_rows) (void)hipFree(c->qadj_rows);
   for (auto& g : c->ra_graphs) (void)hipGraphExecDestroy(g.exec);
@@ -2716,6 +2999,34 @@ int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* U
 int lri_step_ksl_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
                          int64_t ldsb, double* Vb, int64_t ldvb) {
   return adjoint_entry(ctx, true, k_prev, k_next, Ub, ldub, Sb, ldsb, Vb, ldvb);
+}
+
+int lri_step_bug_ra_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax, int64_t r_new,
+                            double* Ub, int64_t ldub, double* Sb, int64_t ldsb, double* Vb, int64_t ldvb) {
+  return guarded(ctx, [&] {
+    ctx->check_step(k_prev, k_next);
+    REQUIRE(tol >= 0.0, "tol must be >= 0");
+    REQUIRE(rmax >= 1, "rmax must be >= 1");
+    REQUIRE(Ub && Sb && Vb, "Ub, Sb and Vb are required");
+    const int64_t r = ctx->r, n = ctx->n, m = ctx->m;
+    REQUIRE(r_new >= 1 && r_new <= std::min<int64_t>(2 * r, ctx->rcap) && r_new <= rmax, "r_new out of range");
+    const int64_t rw = std::max(r, r_new);
+    REQUIRE(ldub >= n && ldvb >= m && ldsb >= rw, "leading dimensions");
+    if (!ctx->radj) {
+      HIPCHECK(hipMalloc((void**)&ctx->radj, sizeof(RaAdjSmall)));
+      HIPCHECK(hipMemset(ctx->radj, 0, sizeof(RaAdjSmall)));
+    }
+    const size_t d = sizeof(double);
+    HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r_new, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r_new, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->radj->Sb1, d * r_new, Sb, d * ldsb, d * r_new, r_new, hipMemcpyHostToDevice,
+                              ctx->stream));
+    ctx->bug_ra_adjoint(k_prev, k_next, tol, rmax, (int)r_new);
+    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->radj->Sb0, d * r, d * r, r, hipMemcpyDeviceToHost, ctx->stream));
+    return ctx->read_flags();
+  });
 }
 
 int lri_step_greedy(lri_ctx* ctx, int64_t k_next) {

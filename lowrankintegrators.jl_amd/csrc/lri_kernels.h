@@ -143,6 +143,32 @@ hipError_t launch_bug_adj_post(BugAdjSmall* A, int r, hipStream_t st);
 // stage 1: Nb = RK CL' (= R1 R1bar', R1bar = St bar = CL); stage 2: Sb0 = CK'.
 hipError_t launch_ksl_adj_small(BugAdjSmall* A, int r, int stage, hipStream_t st);
 
+// Reverse of the rank-adaptive BUG data step, small parts (k2 = 2r, r1 the new rank).
+struct RaAdjSmall {
+  double Ubs[kMaxAug * kMaxRank];    // Uhat' Ubar1 (k2 x r1, ld k2)
+  double Vbs[kMaxAug * kMaxRank];    // Vhat' Vbar1
+  double Sb1[kMaxRank * kMaxRank];   // cotangent of S1 (r1 x r1, ld r1; its diagonal enters)
+  double Jm[kMaxAug * kMaxRank];     // Ush' Ubs, Vsh' Vbs (k2 x r1, ld k2)
+  double Km[kMaxAug * kMaxRank];
+  double core[kMaxAug * kMaxAug];    // the SVD rule's core, then core Vsh'
+  double Shb[kMaxAug * kMaxAug];     // cotangent of Shat (k2 x k2, ld k2)
+  double G1[kMaxRank * kMaxAug];     // S0 N' Shb' (r x k2, ld r)
+  double G2[kMaxRank * kMaxAug];     // S0' M' Shb (r x k2, ld r)
+  double G3[kMaxAug * kMaxRank];     // Shb N S0' (k2 x r, ld k2)
+  double G4[kMaxAug * kMaxRank];     // Shb' M S0 (k2 x r, ld k2)
+  double Sb0[kMaxRank * kMaxRank];   // M' Shb N, the cotangent of S0 (r x r, ld r)
+  double RU[kMaxRank * kMaxRank], RV[kMaxRank * kMaxRank];          // R of the completions Q_U, Q_V
+  double RUinvT[kMaxRank * kMaxRank], RVinvT[kMaxRank * kMaxRank];
+  double CqU[kMaxRank * kMaxRank], CqV[kMaxRank * kMaxRank];        // copyltu(-Qbar' Q)
+  double ZU[kMaxRank * kMaxRank], ZV[kMaxRank * kMaxRank];          // Zbar' U0, Zbar' V0
+  double KU[kMaxRank * kMaxRank], LV[kMaxRank * kMaxRank];          // K1' U0, L1' V0
+};
+// stage 0: Jm, Km, RU^{-T}, RV^{-T};  stage 1: core;  stage 2: core <- core Vsh';  stage 3: Shb = Ush core,
+// stage 4: G1..G4, Sb0 (SVD rule: Shb = Ush [(F o (J - J')) Sig + diag(sb) + Sig (F o (K - K'))] Vsh',
+// F_ij = 1 / (sig_j^2 - sig_i^2) off the diagonal and 0 where i, j >= r1)
+hipError_t launch_ra_adj_small(RaAdjSmall* A, const SmallState* st, const double* S0, int lds, int r, int r1,
+                               int stage, hipStream_t s);
+
 // C = A B' for k x k matrices (ld k), k <= kMaxAug, one workgroup.
 hipError_t launch_small_abt(const double* A, const double* B, int k, double* C, hipStream_t st);
 // two independent decompositions in one launch (one workgroup each)

@@ -1207,6 +1207,110 @@ hipError_t launch_ksl_adj_small(BugAdjSmall* A, int r, int stage, hipStream_t st
   return hipGetLastError();
 }
 
+// Reverse of the rank-adaptive step, small parts; one workgroup per stage (see lri_kernels.h).
+__global__ __launch_bounds__(kThreads) void k_ra_adj_small(RaAdjSmall* A, const SmallState* st, const double* S0,
+                                                           int lds, int r, int r1, int stage) {
+  const int t = threadIdx.x, nt = blockDim.x, k2 = 2 * r;
+  const double* Us = st->Ush;
+  const double* Vs = st->Vsh;
+  const double* sig = st->sigma;
+  const double* M = st->M;  // k2 x r, ld k2
+  const double* N = st->N;
+  if (stage == 0) {
+    for (int e = t; e < k2 * r1; e += nt) {  // Jm = Ush' Ubs, Km = Vsh' Vbs
+      const int i = e % k2, j = e / k2;
+      double a = 0.0, b = 0.0;
+      for (int l = 0; l < k2; ++l) {
+        a += Us[l + i * k2] * A->Ubs[l + j * k2];
+        b += Vs[l + i * k2] * A->Vbs[l + j * k2];
+      }
+      A->Jm[e] = a;
+      A->Km[e] = b;
+    }
+    inv_transpose_upper(A->RU, r, A->RUinvT);
+    inv_transpose_upper(A->RV, r, A->RVinvT);
+  } else if (stage == 1) {
+    for (int e = t; e < k2 * k2; e += nt) {
+      const int i = e % k2, j = e / k2;
+      double v = 0.0;
+      if (i != j && (i < r1 || j < r1)) {
+        const double F = 1.0 / (sig[j] * sig[j] - sig[i] * sig[i]);
+        const double jij = j < r1 ? A->Jm[i + j * k2] : 0.0, jji = i < r1 ? A->Jm[j + i * k2] : 0.0;
+        const double kij = j < r1 ? A->Km[i + j * k2] : 0.0, kji = i < r1 ? A->Km[j + i * k2] : 0.0;
+        v = F * (jij - jji) * sig[j] + sig[i] * F * (kij - kji);
+      }
+      if (i == j && i < r1) v = A->Sb1[i + i * r1];
+      A->core[e] = v;
+    }
+  } else if (stage == 2) {
+    for (int e = t; e < k2 * k2; e += nt) {  // Shb <- core Vsh' (temporarily)
+      const int i = e % k2, j = e / k2;
+      double v = 0.0;
+      for (int l = 0; l < k2; ++l) v += A->core[i + l * k2] * Vs[j + l * k2];
+      A->Shb[e] = v;
+    }
+  } else if (stage == 3) {
+    for (int e = t; e < k2 * k2; e += nt) {  // core <- Ush (core Vsh')
+      const int i = e % k2, j = e / k2;
+      double v = 0.0;
+      for (int l = 0; l < k2; ++l) v += Us[i + l * k2] * A->Shb[l + j * k2];
+      A->core[e] = v;
+    }
+  } else {
+    // Shb = the cotangent of Shat (in core); Jm <- S0 N' (r x k2), Km <- S0' M' (r x k2), ld r
+    for (int e = t; e < k2 * k2; e += nt) A->Shb[e] = A->core[e];
+    for (int e = t; e < r * k2; e += nt) {
+      const int i = e % r, l = e / r;
+      double sn = 0.0, sm = 0.0;
+      for (int q = 0; q < r; ++q) {
+        sn += S0[i + q * lds] * N[l + q * k2];
+        sm += S0[q + i * lds] * M[l + q * k2];
+      }
+      A->Jm[e] = sn;
+      A->Km[e] = sm;
+    }
+    __syncthreads();
+    const double* Sb = A->Shb;
+    for (int e = t; e < r * k2; e += nt) {  // G1 = (S0 N') Shb', G2 = (S0' M') Shb  (r x k2)
+      const int i = e % r, j = e / r;
+      double g1 = 0.0, g2 = 0.0;
+      for (int l = 0; l < k2; ++l) {
+        g1 += A->Jm[i + l * r] * Sb[j + l * k2];
+        g2 += A->Km[i + l * r] * Sb[l + j * k2];
+      }
+      A->G1[e] = g1;
+      A->G2[e] = g2;
+    }
+    for (int e = t; e < k2 * r; e += nt) {  // G3 = Shb (N S0'), G4 = Shb' (M S0), core <- Shb N  (k2 x r)
+      const int i = e % k2, j = e / k2;
+      double g3 = 0.0, g4 = 0.0, sn = 0.0;
+      for (int l = 0; l < k2; ++l) {
+        g3 += Sb[i + l * k2] * A->Jm[j + l * r];
+        g4 += Sb[l + i * k2] * A->Km[j + l * r];
+        sn += Sb[i + l * k2] * N[l + j * k2];
+      }
+      A->G3[e] = g3;
+      A->G4[e] = g4;
+      A->core[e] = sn;
+    }
+    __syncthreads();
+    for (int e = t; e < r * r; e += nt) {  // Sb0 = M' (Shb N)
+      const int i = e % r, j = e / r;
+      double v = 0.0;
+      for (int l = 0; l < k2; ++l) v += M[l + i * k2] * A->core[l + j * k2];
+      A->Sb0[e] = v;
+    }
+  }
+}
+
+hipError_t launch_ra_adj_small(RaAdjSmall* A, const SmallState* st, const double* S0, int lds, int r, int r1,
+                               int stage, hipStream_t s) {
+  if (r < 1 || r > kMaxRank || r1 < 1 || r1 > 2 * r || r1 > kMaxRank || stage < 0 || stage > 4)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_ra_adj_small, dim3(1), dim3(kThreads), 0, s, A, st, S0, lds, r, r1, stage);
+  return hipGetLastError();
+}
+
 hipError_t launch_bug_adj_pre(BugAdjSmall* A, const double* M, const double* N, const double* S0, int lds, int r,
                               hipStream_t st) {
   if (r < 1 || r > kMaxRank) return hipErrorInvalidValue;

@@ -51,6 +51,8 @@ SIGNATURES = {
     "lri_step_bug_ra": (_int, [_vp, _i64, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
     "lri_step_bug_adjoint": (_int, [_vp, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
     "lri_step_ksl_adjoint": (_int, [_vp, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
+    "lri_step_bug_ra_adjoint": (_int, [_vp, _i64, _i64, ctypes.c_double, _i64, _i64, _dp, _i64, _dp, _i64, _dp,
+                                       _i64]),
     "lri_step_greedy": (_int, [_vp, _i64]),
     "lri_step_greedy_twofactor": (_int, [_vp, _i64]),
     "lri_set_operator_lyapunov": (_int, [_vp, _dp, _dp, _int, _dp, _dp, _int, _i64, _dp, _i64, _dp, _i64]),
@@ -226,6 +228,23 @@ class Context:
     def step_ksl_adjoint(self, kp: int, kn: int, Ub, Sb, Vb):
         """Cotangents of the primal KSL step's inputs from those of its result (host arrays)."""
         return self._adjoint(lib().lri_step_ksl_adjoint, kp, kn, Ub, Sb, Vb)
+
+    def step_bug_ra_adjoint(self, kp: int, kn: int, tol: float, rmax: int, Ub, Sb, Vb):
+        """Cotangents of the rank-adaptive BUG step's inputs (rank r) from those of its
+        result (rank r1 = Ub.shape[1], the rank the forward step returned)."""
+        n, r1 = Ub.shape
+        m = Vb.shape[0]
+        r = self.rank()
+        w = max(r, r1)
+        U = np.zeros((n, w), order="F")
+        V = np.zeros((m, w), order="F")
+        S = np.zeros((w, w), order="F")
+        U[:, :r1] = Ub
+        V[:, :r1] = Vb
+        S[:r1, :r1] = Sb
+        check(lib().lri_step_bug_ra_adjoint(self.h, int(kp), int(kn), float(tol), int(rmax), int(r1), _dptr(U), n,
+                                            _dptr(S), w, _dptr(V), m), self.h)
+        return U[:, :r].copy(), S[:r, :r].copy(), V[:, :r].copy()
 
     def _adjoint(self, fn, kp, kn, Ub, Sb, Vb):
         Ub = np.asfortranarray(Ub, dtype=np.float64).copy(order="F")

@@ -172,3 +172,118 @@ def bug_solve_gradient(U0, S0, V0, snaps, loss_grad, checkpoint=20, ksl=False):
         for k in range(c1 - 1, c0 - 1, -1):
             Ub, Sb, Vb, _ = vjp(*states[k - c0], snaps[k + 1] - snaps[k], Ub, Sb, Vb)
     return value, Ub, Sb, Vb
+
+
+# ----------------------------------------------------------------------------
+# rank-adaptive BUG step: SVD backward
+# ----------------------------------------------------------------------------
+
+
+def svd_vjp(Us, sig, Vs, Ub, sb, Vb):
+    """Cotangent of a square A = Us diag(sig) Vs' (full SVD, singular values distinct)
+    given cotangents of its leading factors Us[:, :k], sig[:k], Vs[:, :k] (Ub 2r x k,
+    sb k, Vb 2r x k; the discarded factors carry none):
+        Abar = Us [ (F o (Us'Ubf - Ubf'Us)) Sig + diag(sbf) + Sig (F o (Vs'Vbf - Vbf'Vs)) ] Vs',
+    F_ij = 1 / (sig_j^2 - sig_i^2) (i != j; 0 where both i, j >= k), Ubf, Vbf, sbf the
+    cotangents padded with zeros."""
+    q = Us.shape[0]
+    k = Ub.shape[1]
+    Ubf = np.zeros((q, q))
+    Vbf = np.zeros((q, q))
+    Ubf[:, :k] = Ub
+    Vbf[:, :k] = Vb
+    sbf = np.zeros(q)
+    sbf[:k] = sb
+    s2 = sig ** 2
+    D = s2[None, :] - s2[:, None]
+    F = np.zeros((q, q))
+    keep = np.zeros((q, q), dtype=bool)
+    keep[:k, :] = True
+    keep[:, :k] = True
+    np.fill_diagonal(keep, False)
+    F[keep] = 1.0 / D[keep]
+    J = Us.T @ Ubf
+    K = Vs.T @ Vbf
+    Sg = np.diag(sig)
+    core = (F * (J - J.T)) @ Sg + np.diag(sbf) + Sg @ (F * (K - K.T))
+    return Us @ core @ Vs.T
+
+
+def _complete(B, X):
+    """Q with [B | Q] orthonormal spanning [B | X] (B orthonormal): the QR of
+    (I - B B') X, and its R."""
+    Z = X - B @ (B.T @ X)
+    Q, R = np.linalg.qr(Z, mode="reduced")
+    s = np.sign(np.diag(R))
+    s[s == 0] = 1.0
+    return Q * s, R * s[:, None], Z
+
+
+def ra_bug_step_aug(U, S, V, dA, tol, rmax):
+    """The rank-adaptive BUG data step in the form its reverse differentiates:
+    Uhat = [U0 | Q((I - U0 U0') K1)], Vhat likewise, then as the reference.  The
+    result U1 S1 V1' equals the reference's (same spans); the factors may differ in
+    the signs of singular-vector pairs, so cotangents for ``ra_bug_step_vjp`` are
+    formed from these factors.  Returns (U1, S1, V1, r1)."""
+    K1 = U @ S + dA @ V
+    L1 = V @ S.T + dA.T @ U
+    Uh = np.hstack([U, _complete(U, K1)[0]])
+    Vh = np.hstack([V, _complete(V, L1)[0]])
+    Us, sig, Vst = np.linalg.svd(Uh.T @ (U @ S @ V.T + dA) @ Vh)
+    from .lra import truncate_to_tolerance
+
+    r1 = int(min(rmax, truncate_to_tolerance(sig, tol)))
+    return Uh @ Us[:, :r1], np.diag(sig[:r1]), Vh @ Vst[:r1].T, r1
+
+
+def ra_bug_step_vjp(U, S, V, dA, tol, rmax, Ub1, Sb1, Vb1):
+    """Cotangents (Ub0, Sb0, Vb0, dAb) of the rank-adaptive BUG data step
+    (rank_adaptive_unconventional.jl:194-233) given those of its result
+    (U1 n x r1, S1 = diag(sigma[:r1]), V1 m x r1), the rank r1 held fixed.  The
+    augmented bases are taken as Uhat = [U0 | Q((I - U0 U0') K1)], which spans the same
+    space as the reference's Q([K1 U0]) (the result U1 S1 V1' does not depend on the
+    choice); only the diagonal of Sb1 enters (S1 is diagonal).  Returns also r1."""
+    K1 = U @ S + dA @ V
+    L1 = V @ S.T + dA.T @ U
+    Qu, Ru, _ = _complete(U, K1)
+    Qv, Rv, _ = _complete(V, L1)
+    Uh = np.hstack([U, Qu])
+    Vh = np.hstack([V, Qv])
+    Y = U @ S @ V.T + dA
+    Sh = Uh.T @ Y @ Vh
+    Us, sig, Vst = np.linalg.svd(Sh)
+    Vs = Vst.T
+    from .lra import truncate_to_tolerance
+
+    r1 = int(min(rmax, truncate_to_tolerance(sig, tol)))
+    assert Ub1.shape[1] == r1, (Ub1.shape, r1)
+    # U1 = Uh Us[:, :r1], V1 = Vh Vs[:, :r1], S1 = diag(sig[:r1])
+    Uhb = Ub1 @ Us[:, :r1].T
+    Vhb = Vb1 @ Vs[:, :r1].T
+    Shb = svd_vjp(Us, sig, Vs, Uh.T @ Ub1, np.diag(Sb1).copy(), Vh.T @ Vb1)
+    # Sh = Uh' Y Vh, Y = U S V' + dA
+    Uhb = Uhb + Y @ Vh @ Shb.T
+    Vhb = Vhb + Y.T @ Uh @ Shb
+    Yb = Uh @ Shb @ Vh.T
+    r = U.shape[1]
+    Ub0 = Uhb[:, :r] + Yb @ V @ S.T
+    Vb0 = Vhb[:, :r] + Yb.T @ U @ S
+    Sb0 = U.T @ Yb @ V
+    dAb = Yb
+    # Q = Q((I - U U') K1): Zbar by the thin-QR rule, then Z = K1 - U (U'K1)
+    for side in (0, 1):
+        B, X, Qc, Rc, Qb = (U, K1, Qu, Ru, Uhb[:, r:]) if side == 0 else (V, L1, Qv, Rv, Vhb[:, r:])
+        Zb = qr_vjp(Qc, Rc, Qb)
+        Xb = Zb - B @ (B.T @ Zb)
+        Bb = -Zb @ (X.T @ B) - X @ (Zb.T @ B)
+        if side == 0:
+            Ub0 = Ub0 + Bb + Xb @ S.T  # K1 = U S + dA V
+            Sb0 = Sb0 + U.T @ Xb
+            Vb0 = Vb0 + dA.T @ Xb
+            dAb = dAb + Xb @ V.T
+        else:
+            Vb0 = Vb0 + Bb + Xb @ S  # L1 = V S' + dA' U
+            Sb0 = Sb0 + Xb.T @ V
+            Ub0 = Ub0 + dA @ Xb
+            dAb = dAb + U @ Xb.T
+    return Ub0, Sb0, Vb0, dAb, r1

@@ -176,3 +176,99 @@ def test_ksl_checkpointed_solve_gradient(lri):
     an = sum(np.sum(a * d) for a, d in zip(g, D))
     say(f"  directional derivative {an:.8e}, finite difference {fd:.8e}")
     assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd))
+
+
+# ---------------------------------------------------------------------------- rank-adaptive step
+
+
+def sigma_loss(C, c):
+    """<C, U S V'> + <c, diag S> (S diagonal after a rank-adaptive step): basis-free."""
+    def lg(U, S, V):
+        k = S.shape[0]
+        w = np.zeros(k)
+        w[:min(k, len(c))] = c[:k]
+        return (float(np.sum(C * (U @ S @ V.T))) + float(w @ np.diag(S)), C @ V @ S.T, U.T @ C @ V + np.diag(w),
+                C.T @ U @ S)
+    return lg
+
+
+def ra_case(seed, n, m, r):
+    rng = np.random.default_rng(seed)
+    u = random_factors(rng, n, m, r, decay=0.5)
+    A0 = u.to_matrix()
+    dA = 0.3 * rng.standard_normal((n, 2 * r)) @ rng.standard_normal((2 * r, m)) / np.sqrt(m)
+    return rng, u, A0, A0 + dA
+
+
+@pytest.mark.parametrize("shape", [(64, 48, 4, 6), (200, 136, 8, 12), (37, 53, 3, 2), (300, 260, 16, 24)])
+def test_ra_one_step_gradient_vs_oracle(lri, shape):
+    """rankadaptive_unconventional_step! reversed (SVD rule, augmented-basis QR rule) against
+    oracle.adjoint.ra_bug_step_vjp; rmax caps the new rank (grown, or shrunk below r)."""
+    n, m, r, rmax = shape
+    tol = 1e-12
+    say(f"case: rank-adaptive BUG adjoint one step n={n} m={m} r={r} rmax={rmax}")
+    rng, u, A0, A1 = ra_case(3, n, m, r)
+    C = rng.standard_normal((n, m))
+    lg = sigma_loss(C, rng.standard_normal(2 * r))
+    with lri.native.Context(n, m, max(r, min(rmax, n, m, 2 * r))) as ctx:
+        ctx.reserve(2, lri.native.F64)
+        ctx.upload(0, np.asfortranarray(A0))
+        ctx.upload(1, np.asfortranarray(A1))
+        ctx.set_state(u.U, u.S, u.V)
+        r1 = ctx.step_bug_ra(0, 1, tol, rmax)
+        _, Ub1, Sb1, Vb1 = lg(*ctx.get_state())
+        ctx.set_state(u.U, u.S, u.V)
+        g = ctx.step_bug_ra_adjoint(0, 1, tol, rmax, Ub1, Sb1, Vb1)
+        U, S, V = ctx.get_state()
+        assert np.array_equal(U, u.U) and np.array_equal(S, u.S)
+    Ua, Sa, Va, ra_ = adjoint.ra_bug_step_aug(u.U, u.S, u.V, A1 - A0, tol, rmax)
+    assert ra_ == r1
+    _, oUb, oSb, oVb = lg(Ua, Sa, Va)
+    go = adjoint.ra_bug_step_vjp(u.U, u.S, u.V, A1 - A0, tol, rmax, oUb, oSb, oVb)
+    errs = [rel(a, b) for a, b in zip(g, go[:3])]
+    say(f"  r1={r1}: gradient rel err U {errs[0]:.2e} S {errs[1]:.2e} V {errs[2]:.2e}")
+    assert max(errs) <= 1e-9
+
+
+def test_ra_checkpointed_solve_gradient(lri):
+    """solve_gradient with RankAdaptiveUnconventionalAlgorithm over 8 steps (checkpoints every
+    3), the ranks of the forward solve held, against the oracle's chained reverse steps and a
+    central difference of the engine's solve along a direction tangent to the bases."""
+    n, m, r, T, rmax, tol = 120, 90, 4, 8, 10, 1e-3
+    rng = np.random.default_rng(4)
+    u = random_factors(rng, n, m, r, decay=0.5)
+    snaps = [u.to_matrix()]
+    for _ in range(T):
+        snaps.append(snaps[-1] + 0.05 * rng.standard_normal((n, 3)) @ rng.standard_normal((3, m)) / np.sqrt(m)
+                     + 1e-3 * rng.standard_normal((n, m)))
+    C = rng.standard_normal((n, m))
+    lg = linear_loss(C)
+    alg = lri.RankAdaptiveUnconventionalAlgorithm(tol, rmax=rmax)
+    prob = lri.MatrixDataProblem(snaps, lri.SVDLikeRepresentation(u.U, u.S, u.V))
+    value, g = lri.solve_gradient(prob, alg, lg, checkpoint=3)
+    states = [(u.U, u.S, u.V)]
+    for k in range(T):
+        states.append(adjoint.ra_bug_step_aug(*states[-1], snaps[k + 1] - snaps[k], tol, rmax)[:3])
+    ov, oUb, oSb, oVb = lg(*states[-1])
+    for k in range(T - 1, -1, -1):
+        oUb, oSb, oVb, _, _ = adjoint.ra_bug_step_vjp(*states[k], snaps[k + 1] - snaps[k], tol, rmax, oUb, oSb, oVb)
+    ranks = [s[1].shape[0] for s in states]
+    errs = [rel(a, b) for a, b in zip(g, (oUb, oSb, oVb))]
+    say(f"case: rank-adaptive checkpointed gradient, ranks {ranks}: rel err U {errs[0]:.2e} S {errs[1]:.2e} "
+        f"V {errs[2]:.2e}")
+    assert abs(value - ov) <= 1e-10 * abs(ov)
+    assert max(errs) <= 1e-8
+    G, W = rng.standard_normal((n, r)), rng.standard_normal((r, r))
+    dU = G - u.U @ (u.U.T @ G) + u.U @ (W - W.T)
+    dS = rng.standard_normal((r, r))
+
+    def L(t):
+        y0 = lri.SVDLikeRepresentation(u.U + t * dU, u.S + t * dS, u.V)
+        y = lri.solve(lri.MatrixDataProblem(snaps, y0), alg, save_everystep=False).Y[-1]
+        return lg(y.U, y.S, y.V)[0]
+
+    eps = 1e-6
+    fd = (L(eps) - L(-eps)) / (2 * eps)
+    an = float(np.sum(g[0] * dU) + np.sum(g[1] * dS))
+    say(f"  directional derivative: adjoint {an:.10e}, central difference {fd:.10e}")
+    assert abs(an - fd) <= 1e-6 * max(1.0, abs(fd))

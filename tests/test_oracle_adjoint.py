@@ -126,3 +126,71 @@ def test_ksl_primal_vjp_against_finite_differences_and_forward_mode():
     jv = adjoint.ksl_primal_step_jvp(u.U, u.S, u.V, dA, *D)
     lhs = sum(np.sum(a * b) for a, b in zip(yb, jv))
     assert abs(lhs - an) < 1e-11 * max(1.0, abs(lhs))
+
+
+def test_svd_vjp_matches_finite_differences():
+    rng = np.random.default_rng(31)
+    q, k = 8, 3
+    A = rng.standard_normal((q, q))
+    Us, sig, Vst = np.linalg.svd(A)
+    C1, C2 = rng.standard_normal((q, q)), rng.standard_normal((q, q))
+    c3 = rng.standard_normal(k)
+
+    def loss(A_):  # basis-free in the kept factors: <C1, Uk Sk Vk'> + <c3, sig_k> + <C2, Uk Uk'>
+        u, s, vt = np.linalg.svd(A_)
+        return (float(np.sum(C1 * ((u[:, :k] * s[:k]) @ vt[:k]))) + float(c3 @ s[:k])
+                + float(np.sum(C2 * (u[:, :k] @ u[:, :k].T))))
+
+    Uk, Vk = Us[:, :k], Vst[:k].T
+    Ub = C1 @ Vk * sig[:k] + (C2 + C2.T) @ Uk
+    Vb = C1.T @ Uk * sig[:k]
+    sb = np.einsum("ij,ij->j", Uk, C1 @ Vk) + c3
+    Ab = adjoint.svd_vjp(Us, sig, Vst.T, Ub, sb, Vb)
+    D = rng.standard_normal((q, q))
+    eps = 1e-6
+    fd = (loss(A + eps * D) - loss(A - eps * D)) / (2 * eps)
+    assert abs(fd - float(np.sum(Ab * D))) <= 1e-7 * max(1.0, abs(fd))
+
+
+def test_ra_bug_step_vjp_matches_finite_differences():
+    """Basis-free loss <C, U1 S1 V1'> + <c, sigma> of the rank-adaptive step (the
+    reference's ordering Q([K1 U0]) in the forward) against the reverse in the
+    [U0 | Q((I - U0 U0') K1)] form, the truncation rank held by a clear gap."""
+    rng = np.random.default_rng(32)
+    n, m, r = 30, 24, 4
+    U = np.linalg.qr(rng.standard_normal((n, r)))[0]
+    V = np.linalg.qr(rng.standard_normal((m, r)))[0]
+    S = np.diag([3.0, 2.0, 1.2, 0.7]) + 0.05 * rng.standard_normal((r, r))
+    dA = 0.5 * rng.standard_normal((n, 4)) @ rng.standard_normal((4, m)) / np.sqrt(m) + 1e-3 * rng.standard_normal((n, m))
+    tol, rmax = 1e-8, 6  # sigma = 3.59 3.31 2.82 2.55 1.65 1.57 | 1.02 0.54: r1 = rmax = 6
+    U1, S1, V1, r1 = dlra.rank_adaptive_bug_step(U, S, V, dA, tol, rmax)
+    assert r1 == 6
+    Ua, Sa, Va, ra_ = adjoint.ra_bug_step_aug(U, S, V, dA, tol, rmax)  # the same result, its own factors
+    assert ra_ == r1 and np.allclose(Ua @ Sa @ Va.T, U1 @ S1 @ V1.T, atol=1e-12)
+    U1, S1, V1 = Ua, Sa, Va
+    C = rng.standard_normal((n, m))
+    cs = rng.standard_normal(r1)
+    Ub1 = C @ V1 @ S1.T
+    Vb1 = C.T @ U1 @ S1
+    Sb1 = U1.T @ C @ V1 + np.diag(cs)
+    Ub0, Sb0, Vb0, dAb, r1b = adjoint.ra_bug_step_vjp(U, S, V, dA, tol, rmax, Ub1, Sb1, Vb1)
+    assert r1b == r1
+
+    def loss(U_, S_, V_, dA_):
+        a, b, c, rr = dlra.rank_adaptive_bug_step(U_, S_, V_, dA_, tol, rmax)
+        assert rr == r1
+        return float(np.sum(C * (a @ b @ c.T))) + float(cs @ np.diag(b))
+
+    def tangent(X):  # directions along orthonormal bases: (I - XX') G + X (W - W')
+        G, W = rng.standard_normal(X.shape), rng.standard_normal((X.shape[1], X.shape[1]))
+        return G - X @ (X.T @ G) + X @ (W - W.T)
+
+    # the reverse assumes orthonormal U0, V0 (as the engine does); the reference's map and
+    # this one agree on orthonormal bases, so they are compared along tangent directions
+    dirs = [tangent(U), rng.standard_normal(S.shape), tangent(V), rng.standard_normal(dA.shape)]
+    eps = 1e-6
+    plus = loss(*(x + eps * d for x, d in zip((U, S, V, dA), dirs)))
+    minus = loss(*(x - eps * d for x, d in zip((U, S, V, dA), dirs)))
+    fd = (plus - minus) / (2 * eps)
+    an = sum(float(np.sum(g * d)) for g, d in zip((Ub0, Sb0, Vb0, dAb), dirs))
+    assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd))
