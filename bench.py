@@ -814,6 +814,35 @@ def run_burgers_gradient(args, torch, lri):
     print(json.dumps(out), flush=True)
 
 
+def run_stream_reference(torch):
+    """Streaming bandwidth of torch's own kernels on the same device, as context for the
+    passes' roofline fraction: a 2 GiB copy, a 2 GiB sum, and a column sum of a
+    16384 x 65536 BF16 matrix (read-only, the passes' access pattern)."""
+    dev = torch.device("cuda:0")
+    n = 1 << 30
+    x = torch.ones(n, dtype=torch.bfloat16, device=dev)
+    y = torch.empty_like(x)
+    x2 = x.view(16384, 65536)
+
+    def timeit(f, reps=10):
+        for _ in range(3):
+            f()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            f()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps * 1e-3
+
+    out = {"copy_TBps": 4 * n / timeit(lambda: y.copy_(x)) / 1e12,
+           "sum_TBps": 2 * n / timeit(lambda: x.sum(dtype=torch.float32)) / 1e12,
+           "column_sum_TBps": 2 * n / timeit(lambda: x2.sum(dim=0, dtype=torch.float32)) / 1e12,
+           "note": "torch kernels on 2 GiB of BF16; copy counts read + write bytes"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -841,6 +870,8 @@ def main():
     ap.add_argument("--de-n", type=int, default=4096)
     ap.add_argument("--de-steps", type=int, default=20)
     ap.add_argument("--warmup-de", type=int, default=2)
+    ap.add_argument("--stream-reference", action="store_true",
+                    help="only measure torch's streaming kernels (copy, sum, column sum) on 2 GiB")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy a 1 GiB buffer (a plain copy kernel of known traffic, to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -859,6 +890,9 @@ def main():
     lri.native.lib()
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
+    if args.stream_reference:
+        run_stream_reference(torch)
+        return
     if args.workload == "de":
         run_de(args, torch, lri)
         return
