@@ -79,44 +79,59 @@ __device__ __forceinline__ double apply_a(const double* y, int lr, int c, int k,
 }  // namespace
 
 // ----------------------------------------------------------------------------
-// One reverse RK4 stage over all rows.  Workgroup: tiles of R = 256 / k rows,
-// thread (row lr = t % R, column c = t / R); a workgroup walks the tiles
-// blockIdx.x, blockIdx.x + grid, ... and adds their parameter cotangents into
-// its own slice of Q.part (no atomics: the sum over workgroups is taken later,
-// in order).
+// One reverse RK4 stage over all rows.  Workgroup: tiles of 16 rows (4 waves);
+// a workgroup walks the tiles blockIdx.x, blockIdx.x + grid, ... and adds their
+// parameter cotangents into its own slice of Q.part (no atomics: the sum over
+// workgroups is taken later, in order).  The three tensor contractions of a
+// tile are products on the FP64 matrix cores, T read in its storage order
+// T[c + k (a + k b)]:
+//   P (18 x k, the tile and its halo rows) = Z1 T1,  Z1_i,(c + k b) = X_ib kbar_ic,
+//   Q (16 x k) = Z2 T2,  Z2_i,(c + k a) = (D X)_ia kbar_ic,
+//   Tbar (k^2 x k) -= Z3' kbar,  Z3_i,(a + k b) = (D X)_ia X_ib,
+// wave w taking column block w % nct (nct = ceil(k / 16)) and the w / nct-th
+// slice of the k^2 reduction of P and Q (summed through LDS), and every fourth
+// 16 x 16 block of Tbar.
 // ----------------------------------------------------------------------------
 
 constexpr int kAdjMaxGrid = 512;
+constexpr int kAdjRows = 16;
 
 int quad_adj_grid(int64_t rows, int k) {
-  const int R = 256 / k;
-  const int64_t tiles = (rows + R - 1) / R;
+  (void)k;
+  const int64_t tiles = (rows + kAdjRows - 1) / kAdjRows;
   return (int)std::min<int64_t>(tiles, kAdjMaxGrid);
 }
 
 __global__ __launch_bounds__(256) void k_quad_adj_stage(QuadAdjStageArgs Q) {
-  __shared__ double xs[256 + 2 * kQuadMaxRank];  // stage input rows r0 - 1 .. r0 + R
-  __shared__ double ks[256 + 2 * kQuadMaxRank];  // kbar, same rows
-  __shared__ double ps[256 + 2 * kQuadMaxRank];  // P, same rows
-  __shared__ double dxs[256];                    // (D X), rows r0 .. r0 + R - 1
+  constexpr int R = kAdjRows;
+  __shared__ double xs[(R + 2) * kQuadMaxRank];  // stage input rows r0 - 1 .. r0 + 16: [lr * k + c]
+  __shared__ double ks[(R + 2) * kQuadMaxRank];  // kbar, same rows
+  __shared__ double dxs[R * kQuadMaxRank];       // (D X), rows r0 .. r0 + 15 (zero past n)
+  __shared__ double pP[4][(R + 2) * kQuadMaxRank];  // slices of P, [row * 32 + a]
+  __shared__ double pQ[4][R * kQuadMaxRank];        // slices of Q
   __shared__ double red[256];
   const int k = Q.k, t = threadIdx.x;
-  const int R = 256 / k;
+  const int lane = t & 63, wave = t >> 6, lr = lane & 15, lk = lane >> 4;
   const int64_t n = Q.n;
   const int s = Q.stage;
   const double wy = Q.h * stage_w(s) / 6.0;
   const double wx = s < 3 ? stage_c(s) * Q.h : 0.0;
-  const int k3 = k * k * k;
+  const int k2 = k * k, k3 = k2 * k;
   const int plen = quad_adj_part_len(k, Q.q);
   double* part = Q.part + (int64_t)blockIdx.x * plen;
   const int64_t ntiles = (n + R - 1) / R;
+  const int nct = (k + 15) / 16, nch = 4 / nct;
+  const int ct = wave % nct, ch = wave / nct;
+  const int clen = (k2 + 4 * nch - 1) / (4 * nch) * 4;
+  const int kbeg = ch * clen, kend = min(kbeg + clen, k2);
+  const int col = ct * 16 + lr;  // this lane's output column in P and Q
   double scale_acc = 0.0;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t r0 = tile * R;
     __syncthreads();  // the previous tile's LDS reads are done
     for (int e = t; e < (R + 2) * k; e += blockDim.x) {
-      const int lr = e % (R + 2), c = e / (R + 2);
-      int64_t ir = r0 + lr - 1;
+      const int lrow = e % (R + 2), c = e / (R + 2);
+      int64_t ir = r0 + lrow - 1;
       if (ir < 0) ir += n;
       else if (ir >= n) ir -= n;
       double x = 0.0, kb = 0.0;
@@ -125,82 +140,117 @@ __global__ __launch_bounds__(256) void k_quad_adj_stage(QuadAdjStageArgs Q) {
         x = Q.X[o];
         kb = wy * Q.Ybar[o] + (s < 3 ? wx * Q.Xbp[o] : 0.0);
       }
-      xs[lr * k + c] = x;
-      ks[lr * k + c] = kb;
+      xs[lrow * k + c] = x;
+      ks[lrow * k + c] = kb;
     }
     __syncthreads();
     if (Q.T) {
-      for (int e = t; e < (R + 2) * k; e += blockDim.x) {  // P_ia = sum_b X_ib sum_c kbar_ic T_abc
-        const int lr = e / k, a = e % k;
-        const double* xr = xs + lr * k;
-        const double* kr = ks + lr * k;
-        double p = 0.0;
-        for (int b = 0; b < k; ++b) {
-          const double* Tab = Q.T + (int64_t)k * (a + k * b);
-          double in = 0.0;
-          for (int c = 0; c < k; ++c) in += kr[c] * Tab[c];
-          p += xr[b] * in;
-        }
-        ps[e] = p;
-      }
       for (int e = t; e < R * k; e += blockDim.x) {
-        const int lr = e / k, a = e % k;
-        const int64_t i = r0 + lr;
-        dxs[e] = i < n ? apply_d(xs, lr, a, k, i, n, Q.dl, Q.dd, Q.du, Q.dper) : 0.0;
+        const int lrow = e / k, a = e % k;
+        const int64_t i = r0 + lrow;
+        dxs[e] = i < n ? apply_d(xs, lrow, a, k, i, n, Q.dl, Q.dd, Q.du, Q.dper) : 0.0;
+      }
+      // P on the staged rows 0..15 (acc1) and 16, 17 (acc2), same tensor operand
+      f64x4_t acc1 = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+      for (int kk0 = kbeg; kk0 < kend; kk0 += 4) {
+        const int kk = kk0 + lk;  // (c, b) = (kk % k, kk / k)
+        double a1 = 0.0, a2 = 0.0, b = 0.0;
+        if (kk < kend) {
+          const int c = kk % k, bb = kk / k;
+          a1 = xs[lr * k + bb] * ks[lr * k + c];
+          if (lr < 2) a2 = xs[(R + lr) * k + bb] * ks[(R + lr) * k + c];
+          if (col < k) b = Q.T[c + (int64_t)k * (col + (int64_t)k * bb)];
+        }
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b, acc1, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b, acc2, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        pP[ch][(4 * u + lk) * kQuadMaxRank + col] = acc1[u];
+        if (4 * u + lk < 2) pP[ch][(R + 4 * u + lk) * kQuadMaxRank + col] = acc2[u];
       }
     }
-    __syncthreads();
-    {
-      const int lr = t % R, c = t / R;
-      const int64_t i = r0 + lr;
-      if (c < k && i < n) {
-        const double* kr = ks + (lr + 1) * k;
-        double xb = 0.0;
-        if (Q.st_d) {
-          xb = apply_a(ks, lr, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
-          scale_acc += kr[c] * apply_a(xs, lr, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
-        }
-        if (Q.M)
-          for (int l = 0; l < k; ++l) xb += kr[l] * Q.M[c + l * k];
-        if (Q.T) {
-          const double dp = apply_dt(ps, lr, c, k, i, n, Q.dl, Q.dd, Q.du, Q.dper);
-          const double* dr = dxs + lr * k;
-          double qv = 0.0;  // Q_ic = sum_a (D X)_ia sum_x kbar_ix T_acx
-          for (int a = 0; a < k; ++a) {
-            const double* Tac = Q.T + (int64_t)k * (a + k * c);
-            double in = 0.0;
-            for (int x = 0; x < k; ++x) in += kr[x] * Tac[x];
-            qv += dr[a] * in;
-          }
-          xb -= dp + qv;
-        }
-        const int64_t o = i + c * Q.ld;
-        Q.Xbo[o] = xb;
-        Q.Acc[o] = (s == 3 ? Q.Ybar[o] : Q.Acc[o]) + xb;
-      }
-    }
-    // parameter cotangents of this tile's rows
-    const int nr = (int)std::min<int64_t>(R, n - r0);
+    __syncthreads();  // dxs ready
     if (Q.T) {
-      for (int e = t; e < k3; e += blockDim.x) {  // Tbar_abc -= sum_i kbar_ic (D X)_ia X_ib
-        const int c = e % k, a = (e / k) % k, b = e / (k * k);
-        double v = 0.0;
-        for (int lr = 0; lr < nr; ++lr) v += ks[(lr + 1) * k + c] * dxs[lr * k + a] * xs[(lr + 1) * k + b];
-        part[e] -= v;
+      f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+      for (int kk0 = kbeg; kk0 < kend; kk0 += 4) {
+        const int kk = kk0 + lk;  // (c, a) = (kk % k, kk / k)
+        double a = 0.0, b = 0.0;
+        if (kk < kend) {
+          a = dxs[lr * k + kk / k] * ks[(lr + 1) * k + kk % k];
+          if (col < k) b = Q.T[kk + (int64_t)k2 * col];
+        }
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pQ[ch][(4 * u + lk) * kQuadMaxRank + col] = acc[u];
+      // Tbar -= Z3' kbar over the tile's rows: 16 x 16 blocks (a + k b rows, c columns)
+      const int nab = (k2 + 15) / 16;
+      for (int blk = wave; blk < nab * nct; blk += 4) {
+        const int ab0 = (blk % nab) * 16, c0 = (blk / nab) * 16;
+        const int ab = ab0 + lr, c = c0 + lr;
+        f64x4_t tacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i0 = 0; i0 < R; i0 += 4) {
+          const int i = i0 + lk;
+          const double a = ab < k2 ? dxs[i * k + ab % k] * xs[(i + 1) * k + ab / k] : 0.0;
+          const double b = c < k ? ks[(i + 1) * k + c] : 0.0;
+          tacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, tacc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int abu = ab0 + 4 * u + lk;
+          if (abu < k2 && c < k) part[c + (int64_t)k * abu] -= tacc[u];
+        }
+      }
+    }
+    __syncthreads();  // P and Q slices ready
+    const int nr = (int)std::min<int64_t>(R, n - r0);
+    for (int e = t; e < R * k; e += blockDim.x) {
+      const int lrow = e % R, c = e / R;
+      const int64_t i = r0 + lrow;
+      if (i >= n) continue;
+      const double* kr = ks + (lrow + 1) * k;
+      double xb = 0.0;
+      if (Q.st_d) {
+        xb = apply_a(ks, lrow, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
+        scale_acc += kr[c] * apply_a(xs, lrow, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
+      }
+      if (Q.M)
+        for (int l = 0; l < k; ++l) xb += kr[l] * Q.M[c + l * k];
+      if (Q.T) {
+        // (D' P)_ic from the summed slices of P on rows i - 1, i, i + 1 (staged lrow, lrow + 1, lrow + 2)
+        double pm = 0.0, p0 = 0.0, pp = 0.0, qv = 0.0;
+        for (int h = 0; h < nch; ++h) {
+          pm += pP[h][lrow * kQuadMaxRank + c];
+          p0 += pP[h][(lrow + 1) * kQuadMaxRank + c];
+          pp += pP[h][(lrow + 2) * kQuadMaxRank + c];
+          qv += pQ[h][lrow * kQuadMaxRank + c];
+        }
+        double dp = p0;
+        if (Q.dd) {
+          const bool hasl = i > 0 || Q.dper, hasu = i < n - 1 || Q.dper;
+          const int64_t im = i > 0 ? i - 1 : n - 1, ip = i < n - 1 ? i + 1 : 0;
+          dp = Q.dd[i] * p0 + (hasl ? Q.du[im] * pm : 0.0) + (hasu ? Q.dl[ip] * pp : 0.0);
+        }
+        xb -= dp + qv;
+      }
+      const int64_t o = i + c * Q.ld;
+      Q.Xbo[o] = xb;
+      Q.Acc[o] = (s == 3 ? Q.Ybar[o] : Q.Acc[o]) + xb;
     }
     if (Q.M) {
       for (int e = t; e < k * k; e += blockDim.x) {  // Mbar_lc = sum_i X_il kbar_ic
         const int l = e % k, c = e / k;
         double v = 0.0;
-        for (int lr = 0; lr < nr; ++lr) v += xs[(lr + 1) * k + l] * ks[(lr + 1) * k + c];
+        for (int lrow = 0; lrow < nr; ++lrow) v += xs[(lrow + 1) * k + l] * ks[(lrow + 1) * k + c];
         part[k3 + e] += v;
       }
     }
     for (int e = t; e < Q.q * k; e += blockDim.x) {  // hvbar_qc = sum_i g_iq kbar_ic
       const int qq = e % Q.q, c = e / Q.q;
       double v = 0.0;
-      for (int lr = 0; lr < nr; ++lr) v += Q.g[(r0 + lr) + qq * Q.ldg] * ks[(lr + 1) * k + c];
+      for (int lrow = 0; lrow < nr; ++lrow) v += Q.g[(r0 + lrow) + qq * Q.ldg] * ks[(lrow + 1) * k + c];
       part[k3 + k * k + e] += v;
     }
   }
