@@ -39,6 +39,38 @@ __device__ __forceinline__ void split_bf16(float x, uint32_t& hi, uint32_t& lo) 
   lo = f32_to_bf16_rne(r);
 }
 
+// ---------------------------------------------------------------- small products on the matrix cores
+// C(m, n) = sum_kk A(b, m, kk) B(b, kk, n) for b < nbatch, in 16 x 16 blocks
+// distributed over the workgroup's waves (v_mfma_f64_16x16x4_f64: lane l holds
+// A(m0 + l % 16, k0 + l / 16), B(k0 + l / 16, n0 + l % 16) and the results
+// (m0 + 4 u + l / 16, n0 + l % 16), u < 4); store(b, m, n, value) is called once
+// per element.  The operand functors are only called in range.  Every wave of
+// the workgroup must call it (the loop bounds are wave-uniform).
+template <class FA, class FB, class FS>
+__device__ __forceinline__ void wave_gemm(int M, int N, int K, int nbatch, FA A, FB B, FS store) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+  const int per = tm * tn;
+  for (int tile = wave; tile < per * nbatch; tile += nw) {
+    const int bt = tile / per, rem = tile - bt * per;
+    const int m0 = (rem % tm) << 4, n0 = (rem / tm) << 4;
+    const int mm = m0 + lr, nn = n0 + lr;
+    f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int kk = k0 + lk;
+      const double a = (mm < M && kk < K) ? A(bt, mm, kk) : 0.0;
+      const double b = (kk < K && nn < N) ? B(bt, kk, nn) : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int mo = m0 + 4 * u + lk;
+      if (mo < M && nn < N) store(bt, mo, nn, acc[u]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- packed operand layouts
 // FP64 products take the B operand row-major (k x rb, rb a multiple of 16);
 // MIXED products take column-major bf16 hi/lo (or f32) panels, c-major with a

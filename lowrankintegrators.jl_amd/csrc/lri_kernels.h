@@ -377,7 +377,7 @@ struct QuadSAdjArgs {
 };
 inline int64_t quad_sadj_work(int k, int l, int64_t nsub) {
   const int64_t r = k > l ? k : l;
-  return 4 * nsub * k * l + 7 * r * r * r;  // the stage inputs, then seven r^3 intermediates
+  return 4 * nsub * k * l + 3 * r * r * r;  // the stage inputs, then three r^3 intermediates
 }
 hipError_t launch_quad_sadj(const QuadSAdjArgs& Q, hipStream_t st);
 

@@ -229,15 +229,15 @@ hipError_t launch_quad_stage(const QuadStageArgs& Q, hipStream_t st) {
 // kLds: the state, its RK4 work, the tensors W, T and the contractions Y, Z all
 // in LDS (quad_s_lds_doubles(k, l) <= kQuadSLds), else Y, Z in Q.work and W, T read
 // from global memory.
-constexpr int kQuadSLds = 17408;  // 136 KB
+constexpr int kQuadSLds = 20400;  // 159 KB of the 160 KB LDS
 
 __host__ __device__ inline int quad_s_lds_doubles(int k, int l) {
-  return 4 * k * l + k * l * k + l * l * k + k * k * k + l * l * l;
+  return 5 * k * l + k * l * k + l * l * k + k * k * k + l * l * l;
 }
 
 template <bool kLds>
 __global__ __launch_bounds__(1024) void k_quad_sstep(QuadSArgs Q) {
-  __shared__ double st_small[kLds ? 1 : 4 * kQuadMaxRank * kQuadMaxRank];
+  __shared__ double st_small[kLds ? 1 : 5 * kQuadMaxRank * kQuadMaxRank];
   extern __shared__ double dyn[];
   const int k = Q.k, l = Q.l, t = threadIdx.x, nt = blockDim.x;
   const int kl = k * l;
@@ -246,7 +246,8 @@ __global__ __launch_bounds__(1024) void k_quad_sstep(QuadSArgs Q) {
   double* ys = y0 + kl;
   double* acc = ys + kl;
   double* cb = acc + kl;
-  double* Y = kLds ? cb + kl : Q.work;           // k x l x k: Y[c + k (e + k b)]
+  double* pv = cb + kl;                          // the quadratic term P of a stage
+  double* Y = kLds ? pv + kl : Q.work;           // k x l x k: Y[c + k (e + k b)]
   double* Z = Y + k * l * k;                     // l x l x k: Z[c + k (a + l b)]
   const double* W = Q.W;
   const double* T = Q.T;
@@ -272,19 +273,20 @@ __global__ __launch_bounds__(1024) void k_quad_sstep(QuadSArgs Q) {
     __syncthreads();
     for (int s = 0; s < 4; ++s) {
       if (Q.W) {
-        for (int e = t; e < k * l * k; e += nt) {  // Y_ebc = sum_f S_fb W_efc
-          const int c = e % k, ee = (e / k) % k, b = e / (k * k);
-          double v = 0.0;
-          for (int f = 0; f < k; ++f) v += ys[f + b * k] * W[c + k * (ee + k * f)];
-          Y[e] = v;
-        }
+        // on the matrix cores: Y_(c + k e), b = sum_f W_(c + k e), f S_fb;  Z_(c, a), b = sum_e Y_(c, e), b S_ea;
+        // P_ij = sum_(a + l b) Z_i, (a + l b) T_(a + l b), j  (into pv)
+        const int kk2 = k * k;
+        wave_gemm(kk2, l, k, 1, [&](int, int m, int f) { return W[m + kk2 * f]; },
+                  [&](int, int f, int b) { return ys[f + b * k]; },
+                  [&](int, int m, int b, double v) { Y[m + kk2 * b] = v; });
         __syncthreads();
-        for (int e = t; e < l * l * k; e += nt) {  // Z_abc = sum_e S_ea Y_ebc
-          const int c = e % k, a = (e / k) % l, b = e / (k * l);
-          double v = 0.0;
-          for (int ee = 0; ee < k; ++ee) v += ys[ee + a * k] * Y[c + k * (ee + k * b)];
-          Z[e] = v;
-        }
+        wave_gemm(k, l, k, l, [&](int b, int c, int e) { return Y[c + k * e + kk2 * b]; },
+                  [&](int, int e, int a) { return ys[e + a * k]; },
+                  [&](int b, int c, int a, double v) { Z[c + k * a + k * l * b] = v; });
+        __syncthreads();
+        wave_gemm(k, l, l * l, 1, [&](int, int i, int ab) { return Z[i + k * ab]; },
+                  [&](int, int ab, int j) { return T[j + l * ab]; },
+                  [&](int, int i, int j, double v) { pv[i + j * k] = v; });
         __syncthreads();
       }
       double fv[(kQuadMaxRank * kQuadMaxRank + 1023) / 1024];
@@ -294,12 +296,7 @@ __global__ __launch_bounds__(1024) void k_quad_sstep(QuadSArgs Q) {
         double v = cb[e];
         for (int m = 0; m < k; ++m) v += Q.Ga[i + m * k] * ys[m + j * k];
         for (int m = 0; m < l; ++m) v += ys[i + m * k] * Q.Gb[m + j * l];
-        if (Q.W) {
-          double p = 0.0;  // P_ij = sum_ab Z_abi T_abj, T[j + l (a + l b)]
-          for (int b = 0; b < l; ++b)
-            for (int a = 0; a < l; ++a) p += Z[i + k * (a + l * b)] * T[j + l * (a + l * b)];
-          v -= p;
-        }
+        if (Q.W) v -= pv[e];
         fv[cnt] = Q.sgn * v;
       }
       __syncthreads();  // every stage input read before it is replaced

@@ -289,38 +289,46 @@ hipError_t launch_part_sum(const double* part, int count, int len, double* out, 
 }
 
 // ----------------------------------------------------------------------------
-// Reverse of the S sub-step, one workgroup of 1024 threads.  The k x l
-// matrices live in LDS; the third-order intermediates in Q.work (global):
-//   Y_ebc = sum_f X_fb W_efc,  Z_abc = sum_e X_ea Y_ebc      (forward P)
-//   C_efj = sum_i pb_ij W_efi,  X1_ebj = sum_a X_ea T_abj,
-//   D1_ebj = sum_f C_efj X_fb,  E1_faj = sum_e C_efj X_ea,  Zt_efj = sum_b X_fb X1_ebj,
-// giving  J_P' pb = sum_bj D1_ibj T_abj + sum_a'j E1_ia'j T_a'aj,
-//         Wbar_efi -= sum_j pb_ij Zt_efj,  Tbar_abj -= sum_i Z_abi pb_ij.
+// Reverse of the S sub-step, one workgroup of 1024 threads (16 waves).  The
+// k x l matrices live in LDS; the third-order intermediates in three buffers
+// B0, B1, B2 (LDS when they and T fit, else Q.work), reused as they die:
+//   forward f:  Y_ebc = sum_f X_fb W_efc -> B0,  Z_abc = sum_e X_ea Y_ebc -> B1,
+//               P = Z' T (k x l);
+//   reverse:    Y -> B0, Z -> B1;  Tbar_abj -= sum_i Z_abi pb_ij, X1_ebj = sum_a X_ea T_abj -> B0;
+//               Zt_efj = sum_b X_fb X1_ebj -> B1;  Wbar_efi -= sum_j pb_ij Zt_efj,
+//               C_efj = sum_i pb_ij W_efi -> B0;  D1_ebj = sum_f C_efj X_fb -> B1,
+//               E1_faj = sum_e C_efj X_ea -> B2;
+//               J_P' pb = sum_bj D1_ibj T_abj + sum_a'j E1_ia'j T_a'aj,
+// every contraction a product on the FP64 matrix cores (wave_gemm).
 // ----------------------------------------------------------------------------
 
 namespace {
 
-struct SAdjWork {
-  double *Y, *Z, *C, *X1, *D1, *E1, *Zt;
+struct SAdjBufs {
+  double *B0, *B1, *B2;
+  const double* T;  // the tensor T (LDS copy or Q.T)
 };
 
-// out = f(X) (k x l, ld k), out != X
-__device__ void sadj_f(const QuadSAdjArgs& Q, const double* X, double* out, const double* cb, const SAdjWork& w) {
+// out = f(X) (k x l, ld k); tmp: k x l scratch
+__device__ void sadj_f(const QuadSAdjArgs& Q, const double* X, double* out, const double* cb, double* tmp,
+                       const SAdjBufs& w) {
   const int k = Q.k, l = Q.l, t = threadIdx.x, nt = blockDim.x;
-  if (Q.W) {
-    for (int e = t; e < k * l * k; e += nt) {
-      const int c = e % k, ee = (e / k) % k, b = e / (k * k);
-      double v = 0.0;
-      for (int f = 0; f < k; ++f) v += X[f + b * k] * Q.W[c + k * (ee + k * f)];
-      w.Y[e] = v;
-    }
+  const double* W = Q.W;
+  const double* T = w.T;
+  double* B0 = w.B0;
+  double* B1 = w.B1;
+  if (W) {
+    wave_gemm(k * k, l, k, 1, [&](int, int m, int kk) { return W[m + k * k * kk]; },
+              [&](int, int kk, int nn) { return X[kk + nn * k]; },
+              [&](int, int m, int nn, double v) { B0[m + k * k * nn] = v; });
     __syncthreads();
-    for (int e = t; e < l * l * k; e += nt) {
-      const int c = e % k, a = (e / k) % l, b = e / (k * l);
-      double v = 0.0;
-      for (int ee = 0; ee < k; ++ee) v += X[ee + a * k] * w.Y[c + k * (ee + k * b)];
-      w.Z[e] = v;
-    }
+    wave_gemm(k, l, k, l, [&](int b, int c, int e) { return B0[c + k * e + k * k * b]; },
+              [&](int, int e, int a) { return X[e + a * k]; },
+              [&](int b, int c, int a, double v) { B1[c + k * a + k * l * b] = v; });
+    __syncthreads();
+    wave_gemm(k, l, l * l, 1, [&](int, int i, int ab) { return B1[i + k * ab]; },
+              [&](int, int ab, int j) { return T[j + l * ab]; },
+              [&](int, int i, int j, double v) { tmp[i + j * k] = v; });
     __syncthreads();
   }
   for (int e = t; e < k * l; e += nt) {
@@ -328,12 +336,7 @@ __device__ void sadj_f(const QuadSAdjArgs& Q, const double* X, double* out, cons
     double v = cb[e];
     for (int m = 0; m < k; ++m) v += Q.Ga[i + m * k] * X[m + j * k];
     for (int m = 0; m < l; ++m) v += X[i + m * k] * Q.Gb[m + j * l];
-    if (Q.W) {
-      double p = 0.0;
-      for (int b = 0; b < l; ++b)
-        for (int a = 0; a < l; ++a) p += w.Z[i + k * (a + l * b)] * Q.T[j + l * (a + l * b)];
-      v -= p;
-    }
+    if (W) v -= tmp[e];
     out[e] = Q.sgn * v;
   }
   __syncthreads();
@@ -341,55 +344,58 @@ __device__ void sadj_f(const QuadSAdjArgs& Q, const double* X, double* out, cons
 
 // xb = J_f(X)' kb; accumulates Gab, Gbb, cbar (LDS) and Wb, Tb (global)
 __device__ void sadj_rev(const QuadSAdjArgs& Q, const double* X, const double* kb, double* pb, double* xb,
-                         double* Gab, double* Gbb, double* cbar, const SAdjWork& w) {
+                         double* tmp, double* Gab, double* Gbb, double* cbar, const SAdjBufs& w) {
   const int k = Q.k, l = Q.l, t = threadIdx.x, nt = blockDim.x;
+  const int kk2 = k * k, ll2 = l * l;
+  const double* W = Q.W;
+  const double* T = w.T;
+  double* B0 = w.B0;
+  double* B1 = w.B1;
+  double* B2 = w.B2;
   for (int e = t; e < k * l; e += nt) pb[e] = Q.sgn * kb[e];
   __syncthreads();
-  if (Q.W) {
-    for (int e = t; e < k * l * k; e += nt) {  // Y_ebc
-      const int c = e % k, ee = (e / k) % k, b = e / (k * k);
-      double v = 0.0;
-      for (int f = 0; f < k; ++f) v += X[f + b * k] * Q.W[c + k * (ee + k * f)];
-      w.Y[e] = v;
-    }
-    for (int e = t; e < k * k * l; e += nt) {  // C_efj at e + k (f + k j)
-      const int ee = e % k, f = (e / k) % k, j = e / (k * k);
-      const double* Wef = Q.W + k * (ee + k * f);
-      double v = 0.0;
-      for (int i = 0; i < k; ++i) v += pb[i + j * k] * Wef[i];
-      w.C[e] = v;
-    }
-    for (int e = t; e < k * l * l; e += nt) {  // X1_ebj at e + k (b + l j)
-      const int ee = e % k, b = (e / k) % l, j = e / (k * l);
-      double v = 0.0;
-      for (int a = 0; a < l; ++a) v += X[ee + a * k] * Q.T[j + l * (a + l * b)];
-      w.X1[e] = v;
-    }
+  if (W) {
+    wave_gemm(kk2, l, k, 1, [&](int, int m, int kk) { return W[m + kk2 * kk]; },  // Y -> B0
+              [&](int, int kk, int nn) { return X[kk + nn * k]; },
+              [&](int, int m, int nn, double v) { B0[m + kk2 * nn] = v; });
     __syncthreads();
-    for (int e = t; e < l * l * k; e += nt) {  // Z_abc at c + k (a + l b)
-      const int c = e % k, a = (e / k) % l, b = e / (k * l);
-      double v = 0.0;
-      for (int ee = 0; ee < k; ++ee) v += X[ee + a * k] * w.Y[c + k * (ee + k * b)];
-      w.Z[e] = v;
-    }
-    for (int e = t; e < k * l * l; e += nt) {  // D1_ebj at e + k (b + l j)
-      const int ee = e % k, b = (e / k) % l, j = e / (k * l);
-      double v = 0.0;
-      for (int f = 0; f < k; ++f) v += w.C[ee + k * (f + k * j)] * X[f + b * k];
-      w.D1[e] = v;
-    }
-    for (int e = t; e < k * l * l; e += nt) {  // E1_faj at f + k (a + l j)
-      const int f = e % k, a = (e / k) % l, j = e / (k * l);
-      double v = 0.0;
-      for (int ee = 0; ee < k; ++ee) v += w.C[ee + k * (f + k * j)] * X[ee + a * k];
-      w.E1[e] = v;
-    }
-    for (int e = t; e < k * k * l; e += nt) {  // Zt_efj at e + k (f + k j)
-      const int ee = e % k, f = (e / k) % k, j = e / (k * k);
-      double v = 0.0;
-      for (int b = 0; b < l; ++b) v += X[f + b * k] * w.X1[ee + k * (b + l * j)];
-      w.Zt[e] = v;
-    }
+    wave_gemm(k, l, k, l, [&](int b, int c, int e) { return B0[c + k * e + kk2 * b]; },  // Z -> B1
+              [&](int, int e, int a) { return X[e + a * k]; },
+              [&](int b, int c, int a, double v) { B1[c + k * a + k * l * b] = v; });
+    __syncthreads();
+    wave_gemm(l, ll2, k, 1, [&](int, int j, int i) { return pb[i + j * k]; },  // Tbar -= pb' Z
+              [&](int, int i, int ab) { return B1[i + k * ab]; },
+              [&](int, int j, int ab, double v) { Q.Tb[j + l * ab] -= v; });
+    wave_gemm(k, l, l, l, [&](int b, int e, int a) { return X[e + a * k]; },  // X1 -> B0
+              [&](int b, int a, int j) { return T[j + l * a + ll2 * b]; },
+              [&](int b, int e, int j, double v) { B0[e + k * b + k * l * j] = v; });
+    __syncthreads();
+    wave_gemm(k, k, l, l, [&](int j, int e, int b) { return B0[e + k * b + k * l * j]; },  // Zt -> B1
+              [&](int, int b, int f) { return X[f + b * k]; },
+              [&](int j, int e, int f, double v) { B1[e + k * f + kk2 * j] = v; });
+    __syncthreads();
+    wave_gemm(k, kk2, l, 1, [&](int, int c, int j) { return pb[c + j * k]; },  // Wbar -= pb Zt
+              [&](int, int j, int ef) { return B1[ef + kk2 * j]; },
+              [&](int, int c, int ef, double v) { Q.Wb[c + k * ef] -= v; });
+    wave_gemm(kk2, l, k, 1, [&](int, int ef, int i) { return W[i + k * ef]; },  // C -> B0
+              [&](int, int i, int j) { return pb[i + j * k]; },
+              [&](int, int ef, int j, double v) { B0[ef + kk2 * j] = v; });
+    __syncthreads();
+    wave_gemm(k, l, k, l, [&](int j, int e, int f) { return B0[e + k * f + kk2 * j]; },  // D1 -> B1
+              [&](int, int f, int b) { return X[f + b * k]; },
+              [&](int j, int e, int b, double v) { B1[e + k * b + k * l * j] = v; });
+    wave_gemm(k, l, k, l, [&](int j, int f, int e) { return B0[e + k * f + kk2 * j]; },  // E1 -> B2
+              [&](int, int e, int a) { return X[e + a * k]; },
+              [&](int j, int f, int a, double v) { B2[f + k * a + k * l * j] = v; });
+    __syncthreads();
+    wave_gemm(k, l, 2 * ll2, 1,  // J_P' pb -> tmp
+              [&](int, int i, int q) { return q < ll2 ? B1[i + k * q] : B2[i + k * (q - ll2)]; },
+              [&](int, int q, int a) {
+                if (q < ll2) return T[q / l + l * a + ll2 * (q % l)];  // T_{a b j}, q = b + l j
+                const int q2 = q - ll2;
+                return T[q2 / l + l * (q2 % l) + ll2 * a];  // T_{b a j}
+              },
+              [&](int, int i, int a, double v) { tmp[i + a * k] = v; });
     __syncthreads();
   }
   for (int e = t; e < k * l; e += nt) {
@@ -397,28 +403,8 @@ __device__ void sadj_rev(const QuadSAdjArgs& Q, const double* X, const double* k
     double v = 0.0;
     for (int m = 0; m < k; ++m) v += Q.Ga[m + i * k] * pb[m + a * k];
     for (int m = 0; m < l; ++m) v += pb[i + m * k] * Q.Gb[a + m * l];
-    if (Q.W) {
-      double p = 0.0;
-      for (int j = 0; j < l; ++j)
-        for (int b = 0; b < l; ++b)
-          p += w.D1[i + k * (b + l * j)] * Q.T[j + l * (a + l * b)] + w.E1[i + k * (b + l * j)] * Q.T[j + l * (b + l * a)];
-      v -= p;
-    }
+    if (W) v -= tmp[e];
     xb[e] = v;
-  }
-  if (Q.W) {
-    for (int e = t; e < k * k * k; e += nt) {  // Wb at [c + k (e + k f)]
-      const int c = e % k, ee = (e / k) % k, f = e / (k * k);
-      double v = 0.0;
-      for (int j = 0; j < l; ++j) v += pb[c + j * k] * w.Zt[ee + k * (f + k * j)];
-      Q.Wb[e] -= v;
-    }
-    for (int e = t; e < l * l * l; e += nt) {  // Tb at [j + l (a + l b)]
-      const int j = e % l, a = (e / l) % l, b = e / (l * l);
-      double v = 0.0;
-      for (int i = 0; i < k; ++i) v += w.Z[i + k * (a + l * b)] * pb[i + j * k];
-      Q.Tb[e] -= v;
-    }
   }
   for (int e = t; e < k * k; e += nt) {  // Gab_im += sum_a pb_ia X_ma
     const int i = e % k, m = e / k;
@@ -438,8 +424,13 @@ __device__ void sadj_rev(const QuadSAdjArgs& Q, const double* X, const double* k
 
 }  // namespace
 
-__host__ __device__ inline int quad_sadj_lds_doubles(int k, int l) { return 10 * k * l + k * k + l * l; }
+// LDS doubles: 11 k l + k^2 + l^2 for the matrices, then (when they fit) the
+// three intermediate buffers and T
+__host__ __device__ inline int quad_sadj_small_doubles(int k, int l) { return 11 * k * l + k * k + l * l; }
+__host__ __device__ inline int quad_sadj_r3(int k, int l) { return k * l * (k > l ? k : l); }
+constexpr int kSAdjLdsMax = 20480;  // 160 KB
 
+template <bool kLds>
 __global__ __launch_bounds__(1024) void k_quad_sadj(QuadSAdjArgs Q) {
   extern __shared__ double lds[];
   const int k = Q.k, l = Q.l, kl = k * l, t = threadIdx.x, nt = blockDim.x;
@@ -453,19 +444,26 @@ __global__ __launch_bounds__(1024) void k_quad_sadj(QuadSAdjArgs Q) {
   double* acc = yb + kl;
   double* cb = acc + kl;
   double* cbar = cb + kl;
-  double* Gab = cbar + kl;
+  double* tmp = cbar + kl;
+  double* Gab = tmp + kl;
   double* Gbb = Gab + k * k;
-  const int64_t r = k > l ? k : l;
-  const int64_t r3 = r * r * r;
+  const int64_t r3 = quad_sadj_r3(k, l);
   double* store = Q.work;  // [sub-step][stage][k l]: the stage inputs
-  SAdjWork w;
-  w.Y = store + Q.nsub * 4 * kl;
-  w.Z = w.Y + r3;
-  w.C = w.Z + r3;
-  w.X1 = w.C + r3;
-  w.D1 = w.X1 + r3;
-  w.E1 = w.D1 + r3;
-  w.Zt = w.E1 + r3;
+  SAdjBufs w;
+  if (kLds) {
+    w.B0 = Gbb + l * l;
+    w.B1 = w.B0 + r3;
+    w.B2 = w.B1 + r3;
+    double* tl = w.B2 + r3;
+    if (Q.T)
+      for (int e = t; e < l * l * l; e += nt) tl[e] = Q.T[e];
+    w.T = tl;
+  } else {
+    w.B0 = store + Q.nsub * 4 * kl;
+    w.B1 = w.B0 + r3;
+    w.B2 = w.B1 + r3;
+    w.T = Q.T;
+  }
   const double h = Q.h;
   for (int e = t; e < kl; e += nt) {
     const int i = e % k, j = e / k;
@@ -492,7 +490,7 @@ __global__ __launch_bounds__(1024) void k_quad_sadj(QuadSAdjArgs Q) {
     }
     __syncthreads();
     for (int s = 0; s < 4; ++s) {
-      sadj_f(Q, x, fv, cb, w);
+      sadj_f(Q, x, fv, cb, tmp, w);
       for (int e = t; e < kl; e += nt) {
         if (s < 3) {
           acc[e] = (s == 0 ? 0.0 : acc[e]) + stage_w(s) * fv[e];
@@ -514,7 +512,7 @@ __global__ __launch_bounds__(1024) void k_quad_sadj(QuadSAdjArgs Q) {
         kb[e] = (h * stage_w(s) / 6.0) * yb[e] + (s < 3 ? stage_c(s) * h * xb[e] : 0.0);
       }
       __syncthreads();
-      sadj_rev(Q, x, kb, pb, xb, Gab, Gbb, cbar, w);
+      sadj_rev(Q, x, kb, pb, xb, tmp, Gab, Gbb, cbar, w);
       for (int e = t; e < kl; e += nt) acc[e] = (s == 3 ? yb[e] : acc[e]) + xb[e];
       __syncthreads();
     }
@@ -543,8 +541,12 @@ hipError_t launch_quad_sadj(const QuadSAdjArgs& Q, hipStream_t st) {
       Q.nsub < 1 || !Q.S0 || !Q.Sb1 || !Q.Sb0 || !Q.Ga || !Q.Gb || !Q.Gab || !Q.Gbb ||
       (Q.q > 0 && (!Q.gaT || !Q.hbT || !Q.gaTb || !Q.hbTb)) || (Q.W && (!Q.T || !Q.Wb || !Q.Tb)))
     return hipErrorInvalidValue;
-  const size_t lds = sizeof(double) * quad_sadj_lds_doubles(Q.k, Q.l);
-  hipLaunchKernelGGL(k_quad_sadj, dim3(1), dim3(1024), lds, st, Q);
+  const int small = quad_sadj_small_doubles(Q.k, Q.l);
+  const int full = small + 3 * quad_sadj_r3(Q.k, Q.l) + Q.l * Q.l * Q.l;
+  if (full <= kSAdjLdsMax)
+    hipLaunchKernelGGL(k_quad_sadj<true>, dim3(1), dim3(1024), sizeof(double) * full, st, Q);
+  else
+    hipLaunchKernelGGL(k_quad_sadj<false>, dim3(1), dim3(1024), sizeof(double) * small, st, Q);
   return hipGetLastError();
 }
 
