@@ -840,6 +840,11 @@ def run_stream_reference(torch):
            "sum_TBps": 2 * n / timeit(lambda: x.sum(dtype=torch.float32)) / 1e12,
            "column_sum_TBps": 2 * n / timeit(lambda: x2.sum(dim=0, dtype=torch.float32)) / 1e12,
            "note": "torch kernels on 2 GiB of BF16; copy counts read + write bytes"}
+    import __graft_entry__
+
+    lri = __graft_entry__.load_package()
+    for name, pat in (("bare_read_contiguous_TBps", 0), ("bare_read_8col_128B_TBps", 1), ("bare_read_1col_1KB_TBps", 2)):
+        out[name] = lri.native.stream_probe(pat)
     print(json.dumps(out), flush=True)
 
 

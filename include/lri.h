@@ -320,6 +320,13 @@ int lri_kernel_timing_read(lri_ctx* ctx, int family, double* total_ms, int64_t* 
  * 1024 floats).  Results are ordered [lane][register]. */
 int lri_selftest_mfma(int which, const void* a, const void* b, void* out);
 
+/* Bandwidth context for the passes (no counterpart in the reference): what
+ * bare non-temporal 16-byte loads stream, in TB/s, over two column-major
+ * 16384 x 16384 BF16 matrices (1 GiB) on the current device, no arithmetic:
+ * pattern 0 contiguous, 1 = a wave instruction covering 8 columns x 128 B (the
+ * pass over dA' x U), 2 = 1 column x 1 KB.  Allocates and frees 1 GiB. */
+int lri_stream_probe(int pattern, double* tbps);
+
 /* One product pass with the current state and precision, reduced to FP64 and
  * copied to host (synchronising): which = 0 gives dA * V (n x r, ld ldo >= n),
  * which = 1 gives dA' * U (m x r, ld ldo >= m), dA = A[k_next] - A[k_prev].

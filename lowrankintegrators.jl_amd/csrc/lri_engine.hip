@@ -3405,4 +3405,14 @@ int lri_selftest_mfma(int which, const void* a, const void* b, void* out) {
   return LRI_OK;
 }
 
+int lri_stream_probe(int pattern, double* tbps) {
+  if (pattern < 0 || pattern > 2 || !tbps) return LRI_EINVAL;
+  hipError_t e = stream_probe(pattern, tbps);
+  if (e != hipSuccess) {
+    g_create_error = std::string("stream probe: ") + hipGetErrorString(e);
+    return LRI_EHIP;
+  }
+  return LRI_OK;
+}
+
 }  // extern "C"

@@ -201,6 +201,9 @@ hipError_t launch_passN(int mode, const PassArgs& a, hipStream_t st);
 hipError_t launch_passT(int mode, const PassArgs& a, hipStream_t st);
 
 hipError_t selftest_mfma(int which, const void* a, const void* b, void* out);
+// Bare streaming-read bandwidth (TB/s) over two 16384^2 BF16 matrices; pattern 0 contiguous,
+// 1: 8 columns x 128 B per wave instruction, 2: 1 column x 1 KB.
+hipError_t stream_probe(int pattern, double* tbps);
 
 // Matrix-ODE sub-steps (see lri_de.hip).  The forcing has rank q <= kMaxForce.
 constexpr int kMaxForce = 4;

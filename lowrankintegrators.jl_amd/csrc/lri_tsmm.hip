@@ -1109,6 +1109,88 @@ __global__ void k_selftest(int which, const void* a, const void* b, void* out) {
   }
 }
 
+// ============================================================================
+// Streaming-read probe: what bare non-temporal 16-byte loads reach over two
+// column-major 16384 x 16384 BF16 matrices (no arithmetic), as context for the
+// passes' fraction of the HBM peak.  pattern 0: contiguous; 1: a wave
+// instruction covers 8 columns x 128 B (pass T's pattern); 2: 1 column x 1 KB.
+// 512 workgroups of 256 threads, each over 32 columns (pattern 0: a grid-stride
+// sweep of the same bytes).
+// ============================================================================
+
+constexpr int64_t kProbeN = 16384;
+
+template <int CPW>
+__global__ __launch_bounds__(256) void k_stream_probe(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b,
+                                                      uint32_t* __restrict__ sink) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t acc = 0;
+  if (CPW == 0) {
+    const int64_t n16 = kProbeN * kProbeN / 8;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) {
+      const u32x4_t x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(a) + i);
+      const u32x4_t y = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(b) + i);
+      acc ^= x[0] ^ x[1] ^ x[2] ^ x[3] ^ y[0] ^ y[1] ^ y[2] ^ y[3];
+    }
+  } else {
+    constexpr int lpc = 64 / (CPW > 0 ? CPW : 1);  // lanes per column
+    constexpr int rpi = lpc * 8;                     // rows per column per instruction
+    const int64_t c0 = (int64_t)blockIdx.x * 32;
+    for (int cg = wave * CPW; cg < 32; cg += 4 * CPW) {
+      const int64_t col = c0 + cg + lane / lpc;
+      const int64_t rbase = (lane % lpc) * 8;
+      for (int64_t r = 0; r < kProbeN; r += 4 * rpi) {
+        u32x4_t x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t off = col * kProbeN + r + (int64_t)u * rpi + rbase;
+          x[u] = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(a + off));
+          y[u] = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(b + off));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc ^= x[u][0] ^ x[u][1] ^ x[u][2] ^ x[u][3] ^ y[u][0] ^ y[u][1] ^ y[u][2] ^ y[u][3];
+      }
+    }
+  }
+  if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads
+}
+
+hipError_t stream_probe(int pattern, double* tbps) {
+  if (pattern < 0 || pattern > 2 || !tbps) return hipErrorInvalidValue;
+  const size_t bytes = (size_t)kProbeN * kProbeN * 2;
+  uint16_t *a = nullptr, *b = nullptr;
+  uint32_t* sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&a, bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&b, bytes);
+  if (e == hipSuccess) e = hipMalloc((void**)&sink, 64);
+  if (e == hipSuccess) e = hipMemset(a, 1, bytes);
+  if (e == hipSuccess) e = hipMemset(b, 2, bytes);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float ms = 0.f;
+  const dim3 grid((unsigned)(kProbeN / 32)), blk(256);
+  auto launch = [&] {
+    if (pattern == 0) hipLaunchKernelGGL(k_stream_probe<0>, grid, blk, 0, 0, a, b, sink);
+    else if (pattern == 1) hipLaunchKernelGGL(k_stream_probe<8>, grid, blk, 0, 0, a, b, sink);
+    else hipLaunchKernelGGL(k_stream_probe<1>, grid, blk, 0, 0, a, b, sink);
+    return hipGetLastError();
+  };
+  for (int it = 0; it < 3 && e == hipSuccess; ++it) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  for (int it = 0; it < 10 && e == hipSuccess; ++it) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) *tbps = 2.0 * bytes / (ms / 10 * 1e-3) / 1e12;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (sink) (void)hipFree(sink);
+  if (b) (void)hipFree(b);
+  if (a) (void)hipFree(a);
+  return e;
+}
+
 hipError_t selftest_mfma(int which, const void* a, const void* b, void* out) {
   size_t in_bytes = which == 0 ? 64 * 8 : (which == 1 ? 64 * 16 : 64 * 4);
   size_t out_bytes = which == 0 ? 256 * 8 : 1024 * 4;

@@ -76,6 +76,7 @@ SIGNATURES = {
     "lri_kernel_timing": (_int, [_vp, _int]),
     "lri_kernel_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)]),
     "lri_selftest_mfma": (_int, [_int, _vp, _vp, _vp]),
+    "lri_stream_probe": (_int, [_int, _dp]),
     "lri_debug_product": (_int, [_vp, _int, _i64, _i64, _dp, _i64]),
 }
 
@@ -456,3 +457,10 @@ class Batch:
 
     def synchronize(self):
         _check_batch(lib().lri_batch_synchronize(self.h), self.h)
+
+
+def stream_probe(pattern: int) -> float:
+    """Bare streaming-read bandwidth (TB/s) of the device (see lri_stream_probe)."""
+    v = np.zeros(1)
+    check(lib().lri_stream_probe(int(pattern), _dptr(v)))
+    return float(v[0])
