@@ -2835,10 +2835,10 @@ int lri_set_state(lri_ctx* ctx, int64_t r, const double* U, int64_t ldu, const d
     REQUIRE(ldu >= ctx->n && ldv >= ctx->m && lds >= r, "leading dimension too small");
     const size_t d = sizeof(double);
     HIPCHECK(hipStreamSynchronize(ctx->stream));
-    HIPCHECK(hipMemcpy2D(ctx->U, d * ctx->n, U, d * ldu, d * ctx->n, r, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy2D(ctx->V, d * ctx->m, V, d * ldv, d * ctx->m, r, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy2D(ctx->U, d * ctx->n, U, d * ldu, d * ctx->n, r, hipMemcpyDefault));
+    HIPCHECK(hipMemcpy2D(ctx->V, d * ctx->m, V, d * ldv, d * ctx->m, r, hipMemcpyDefault));
     HIPCHECK(hipMemset(ctx->S, 0, d * ctx->rcap * ctx->rcap));
-    HIPCHECK(hipMemcpy2D(ctx->S, d * ctx->rcap, S, d * lds, d * r, r, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy2D(ctx->S, d * ctx->rcap, S, d * lds, d * r, r, hipMemcpyDefault));
     ctx->r = r;
     ctx->pkN_mode = ctx->pkT_mode = -1;
     return LRI_OK;
@@ -2855,15 +2855,15 @@ int lri_get_state(lri_ctx* ctx, int64_t* r_out, double* U, int64_t ldu, double* 
     if (r >= 1) {
       if (U) {
         REQUIRE(ldu >= ctx->n, "ldu too small");
-        HIPCHECK(hipMemcpy2D(U, d * ldu, ctx->U, d * ctx->n, d * ctx->n, r, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy2D(U, d * ldu, ctx->U, d * ctx->n, d * ctx->n, r, hipMemcpyDefault));
       }
       if (V) {
         REQUIRE(ldv >= ctx->m, "ldv too small");
-        HIPCHECK(hipMemcpy2D(V, d * ldv, ctx->V, d * ctx->m, d * ctx->m, r, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy2D(V, d * ldv, ctx->V, d * ctx->m, d * ctx->m, r, hipMemcpyDefault));
       }
       if (S) {
         REQUIRE(lds >= r, "lds too small");
-        HIPCHECK(hipMemcpy2D(S, d * lds, ctx->S, d * ctx->rcap, d * r, r, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy2D(S, d * lds, ctx->S, d * ctx->rcap, d * r, r, hipMemcpyDefault));
       }
     }
     return st;
@@ -2976,16 +2976,16 @@ int adjoint_entry(lri_ctx* ctx, bool ksl, int64_t k_prev, int64_t k_next, double
     }
     const size_t d = sizeof(double);
     double* ubuf = ksl ? ctx->P + n * r : ctx->P;
-    HIPCHECK(hipMemcpy2DAsync(ubuf, d * n, Ub, d * ldub, d * n, r, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ubuf, d * n, Ub, d * ldub, d * n, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyDefault, ctx->stream));
     if (ksl)
       ctx->ksl_adjoint(k_prev, k_next);
     else
       ctx->bug_adjoint(k_prev, k_next);
-    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ubuf, d * n, d * n, r, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->adj->Sb0, d * r, d * r, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ubuf, d * n, d * n, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->adj->Sb0, d * r, d * r, r, hipMemcpyDefault, ctx->stream));
     return ctx->read_flags();
   });
 }
@@ -3017,14 +3017,14 @@ int lri_step_bug_ra_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double
       HIPCHECK(hipMemset(ctx->radj, 0, sizeof(RaAdjSmall)));
     }
     const size_t d = sizeof(double);
-    HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r_new, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r_new, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(ctx->radj->Sb1, d * r_new, Sb, d * ldsb, d * r_new, r_new, hipMemcpyHostToDevice,
+    HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r_new, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r_new, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->radj->Sb1, d * r_new, Sb, d * ldsb, d * r_new, r_new, hipMemcpyDefault,
                               ctx->stream));
     ctx->bug_ra_adjoint(k_prev, k_next, tol, rmax, (int)r_new);
-    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->radj->Sb0, d * r, d * r, r, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->radj->Sb0, d * r, d * r, r, hipMemcpyDefault, ctx->stream));
     return ctx->read_flags();
   });
 }
@@ -3191,15 +3191,15 @@ int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* U
       HIPCHECK(hipMemset(ctx->adj, 0, sizeof(BugAdjSmall)));
     }
     const size_t d = sizeof(double);
-    HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyDefault, ctx->stream));
     ctx->quad_bug_adjoint(dt, substeps);
     double sc[2] = {0.0, 0.0};
-    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->adj->Sb0, d * r, d * r, r, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpyAsync(sc, ctx->qadj->scal, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->adj->Sb0, d * r, d * r, r, hipMemcpyDefault, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(sc, ctx->qadj->scal, sizeof(sc), hipMemcpyDefault, ctx->stream));
     const int f = ctx->read_flags();
     if (scale_bar) {
       scale_bar[0] += sc[0];

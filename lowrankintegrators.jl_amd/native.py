@@ -230,6 +230,16 @@ class Context:
         """Cotangents of the primal KSL step's inputs from those of its result (host arrays)."""
         return self._adjoint(lib().lri_step_ksl_adjoint, kp, kn, Ub, Sb, Vb)
 
+    def step_adjoint_ptr(self, kind: str, kp: int, kn: int, ub: int, ldub: int, sb: int, ldsb: int, vb: int,
+                         ldvb: int):
+        """The BUG (kind "bug") or primal KSL ("ksl") reverse step on cotangents at raw
+        addresses -- device memory (e.g. torch tensors' data_ptr()) or host -- column-major
+        with the given leading dimensions, overwritten in place."""
+        fn = {"bug": lib().lri_step_bug_adjoint, "ksl": lib().lri_step_ksl_adjoint}[kind]
+        P = ctypes.POINTER(ctypes.c_double)
+        check(fn(self.h, int(kp), int(kn), ctypes.cast(ub, P), int(ldub), ctypes.cast(sb, P), int(ldsb),
+                 ctypes.cast(vb, P), int(ldvb)), self.h)
+
     def step_bug_ra_adjoint(self, kp: int, kn: int, tol: float, rmax: int, Ub, Sb, Vb):
         """Cotangents of the rank-adaptive BUG step's inputs (rank r) from those of its
         result (rank r1 = Ub.shape[1], the rank the forward step returned)."""

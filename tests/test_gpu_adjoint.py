@@ -272,3 +272,46 @@ def test_ra_checkpointed_solve_gradient(lri):
     an = float(np.sum(g[0] * dU) + np.sum(g[1] * dS))
     say(f"  directional derivative: adjoint {an:.10e}, central difference {fd:.10e}")
     assert abs(an - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_adjoint_with_device_cotangents(lri):
+    """The reverse step reads and writes cotangents in device memory as well (hipMemcpyDefault):
+    the same result as with host arrays, bit for bit.  The device buffers come from the HIP
+    runtime the library itself runs on."""
+    import ctypes
+
+    lri.native.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime already loaded by the library
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    H2D, D2H = 1, 2
+
+    n, m, r = 96, 80, 5
+    rng = np.random.default_rng(9)
+    u = random_factors(rng, n, m, r, decay=0.3)
+    A0 = rng.standard_normal((n, m))
+    A1 = A0 + 5e-2 * rng.standard_normal((n, m))
+    cot = [np.asfortranarray(rng.standard_normal(s)) for s in ((n, r), (r, r), (m, r))]
+    ptrs = []
+    try:
+        for x in cot:
+            p = ctypes.c_void_p()
+            assert hip.hipMalloc(ctypes.byref(p), x.nbytes) == 0
+            ptrs.append(p)
+            assert hip.hipMemcpy(p, x.ctypes.data, x.nbytes, H2D) == 0
+        with lri.native.Context(n, m, r) as ctx:
+            ctx.reserve(2, lri.native.F64)
+            ctx.upload(0, np.asfortranarray(A0))
+            ctx.upload(1, np.asfortranarray(A1))
+            ctx.set_state(u.U, u.S, u.V)
+            host = ctx.step_bug_adjoint(0, 1, *cot)
+            ctx.step_adjoint_ptr("bug", 0, 1, ptrs[0].value, n, ptrs[1].value, r, ptrs[2].value, m)
+            ctx.synchronize()
+        for p, x, h in zip(ptrs, cot, host):
+            got = np.empty_like(x)
+            assert hip.hipMemcpy(got.ctypes.data, p, x.nbytes, D2H) == 0
+            assert np.array_equal(got, h)
+    finally:
+        for p in ptrs:
+            hip.hipFree(p)
