@@ -18,7 +18,9 @@
  *    lri_get_rank, lri_step_bug_ra).
  *  - Matrices crossing the ABI are column-major (Julia's layout) with an
  *    explicit leading dimension.  U is n x r, S is r x r, V is m x r, and
- *    Y = U * S * V'.  The state is held in FP64 on the device.
+ *    Y = U * S * V'.  The state is held in FP64 on the device.  The arrays of
+ *    lri_set_state / lri_get_state and of the adjoint entry points may be host
+ *    or device memory (the copies infer the direction).
  *  - Snapshots are n x m column-major arrays A_k.  Steps take snapshot
  *    indices k_prev, k_next and use the increment dA = A[k_next] - A[k_prev],
  *    formed on the fly (it is never materialised).
