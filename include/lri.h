@@ -258,6 +258,12 @@ int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, in
 int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* Ub, int64_t ldub, double* Sb,
                             int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar);
 
+/* The same for the primal KSL matrix-ODE step (lri_step_de_ksl with LRI_PRIMAL;
+ * projector_splitting.jl:52-80,129-152): the S sub-step runs backwards in time
+ * from R1, and both QR factors carry cotangents (S1 = R2'). */
+int lri_step_de_ksl_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* Ub, int64_t ldub, double* Sb,
+                            int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar);
+
 /* ---- batches of independent problems -------------------------------------
  *
  * Many small solves of uneven n, m and rank (the reference runs them one

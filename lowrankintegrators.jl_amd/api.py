@@ -760,7 +760,8 @@ def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, preci
       fixed) or ``ProjectorSplitting(PrimalLieTrotter())``: ``lri_step_bug_adjoint`` /
       ``lri_step_bug_ra_adjoint`` / ``lri_step_ksl_adjoint``; returns ``(value, (Ub0, Sb0, Vb0))``;
     - a matrix ODE (``MatrixDEProblem`` with a ``QuadraticOperator``; D = 0 for a linear
-      one) and ``UnconventionalAlgorithm()``: ``lri_step_de_bug_adjoint``, with ``dt`` and
+      one) and ``UnconventionalAlgorithm()`` or ``ProjectorSplitting(PrimalLieTrotter())``:
+      ``lri_step_de_bug_adjoint`` / ``lri_step_de_ksl_adjoint``, with ``dt`` and
       the RK4 ``substeps`` as ``solve``; returns ``(value, (Ub0, Sb0, Vb0), (abar, bbar))``,
       abar and bbar the derivatives along scales of the operator's A and B
       (A = nu Laplacian: dL/dnu = abar / nu).
@@ -839,8 +840,12 @@ def de_substeps(prob, dt, substeps: Optional[int] = None) -> int:
 
 
 def _solve_gradient_de(prob, alg, loss_grad, dt, checkpoint, substeps, device):
-    if not isinstance(alg, UnconventionalAlgorithm):
-        raise TypeError("the matrix-ODE adjoint is built for the unconventional (BUG) integrator")
+    ksl = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, PrimalLieTrotter)
+    if not (ksl or isinstance(alg, UnconventionalAlgorithm)):
+        raise TypeError("the matrix-ODE adjoint is built for the unconventional (BUG) and primal KSL integrators")
+    fwd = (lambda c: c.step_de_ksl(native.PRIMAL, dt, N)) if ksl else (lambda c: c.step_de_bug(dt, N))  # noqa: E731
+    rev = (lambda c, *x: c.step_de_ksl_adjoint(dt, N, *x)) if ksl else (  # noqa: E731
+        lambda c, *x: c.step_de_bug_adjoint(dt, N, *x))
     if not isinstance(prob.f, QuadraticOperator):
         raise TypeError("the matrix-ODE adjoint takes a QuadraticOperator (D = 0 for a linear right-hand side)")
     if dt is None:
@@ -852,7 +857,7 @@ def _solve_gradient_de(prob, alg, loss_grad, dt, checkpoint, substeps, device):
     try:
         cps = {0: ctx.get_state()}
         for s_ in range(steps):
-            ctx.step_de_bug(dt, N)
+            fwd(ctx)
             if (s_ + 1) % checkpoint == 0 and s_ + 1 < steps:
                 cps[s_ + 1] = ctx.get_state()
         value, Ub, Sb, Vb = loss_grad(*ctx.get_state())
@@ -862,11 +867,11 @@ def _solve_gradient_de(prob, alg, loss_grad, dt, checkpoint, substeps, device):
             states = [cps[c0]]
             ctx.set_state(*cps[c0])
             for _ in range(c0, c1 - 1):
-                ctx.step_de_bug(dt, N)
+                fwd(ctx)
                 states.append(ctx.get_state())
             for s_ in range(c1 - 1, c0 - 1, -1):
                 ctx.set_state(*states[s_ - c0])
-                Ub, Sb, Vb, (a, b) = ctx.step_de_bug_adjoint(dt, N, Ub, Sb, Vb)
+                Ub, Sb, Vb, (a, b) = rev(ctx, Ub, Sb, Vb)
                 abar += a
                 bbar += b
         return value, (Ub, Sb, Vb), (abar, bbar)

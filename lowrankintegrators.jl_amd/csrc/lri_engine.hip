@@ -2403,6 +2403,205 @@ struct lri_ctx {
     });
   }
 
+  // Reverse of quad_ksl_primal (primal_LT_step! with the DE sub-problems,
+  // projector_splitting.jl:52-80,129-152): K -> (U1, R1); S~ from R1 backwards in
+  // time (sgn -1) with U1's and V0's Galerkin data; L from V0 S~' with U1's ->
+  // (V1, R2); S1 = R2'.  Both QR factors carry cotangents:
+  //   Lbar1 = (Vb1 + V1 copyltu(R2 Sb1 - Vb1'V1)) R2^{-T}  (R2bar = Sb1'),
+  //   L reverse -> Lbar0, cotangents of U1's data;  Vb0 = Lbar0 S~ + ...,  S~bar = Lbar0' V0,
+  //   S reverse -> R1bar and cotangents of U1's and V0's data,
+  //   Kbar1 = (Ub1 + ... + U1 copyltu(R1 R1bar' - Ub1'U1)) R1^{-T},  K reverse -> Kbar0,
+  //   Ub0 = Kbar0 S0',  Sb0 = U0'Kbar0,  Vb0 += (basis cotangent of V0's data).
+  // Cotangents as in quad_bug_adjoint (P, adj->Sb1 / adj->Sb0, Q, qadj->scal).
+  void quad_ksl_adjoint(double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    const size_t d = sizeof(double);
+    const int64_t mx = std::max(n, m);
+    const int plen = quad_adj_part_len(rr, op_q);
+    qadj_reserve(nsub);
+    double* stK = qadj_rows;
+    double* stL = stK + nsub * n * rr;
+    double* work = stL + nsub * m * rr;
+    double* partK = work + 8 * mx * rr;
+    double* partL = partK + (size_t)plen * quad_adj_grid(n, rr);
+    double* swork = partL + (size_t)plen * quad_adj_grid(m, rr);
+    double* kbar = work + 7 * mx * rr;
+    QuadAdj* qa = qadj;
+    const size_t rsz = d * rr * rr, fsz = d * op_q * rr, tsz = d * rr * rr * rr;
+    // forward: V0's Galerkin data (kept in qa->Gb0, hbT0, T0), K with its sub-step starts
+    quad_galerkin(V, false, rr);
+    de_gal_fresh = false;
+    HIPCHECK(hipMemcpyAsync(qa->Gb0, de->Gb, rsz, hipMemcpyDeviceToDevice, stream));
+    if (op_q) HIPCHECK(hipMemcpyAsync(qa->hbT0, de->hbT, fsz, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(qa->T0, quad_T, tsz, hipMemcpyDeviceToDevice, stream));
+    quad_init(U, n, rr, S, (int)rcap, 0, K);
+    quad_rows_ex(true, rr, qa->Gb0, qa->hbT0, qa->T0, K, work, work + 2 * mx * rr, h, nsub, stK);
+    quad_gram(K, n, rr, shifted_chol(n));
+    cholqr_tail(K, K2, n, rr, U2, false, nullptr, 0, de_epi(), 0, 3);  // U1, R1
+    HIPCHECK(hipMemcpyAsync(adj->RK, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    // U1's Galerkin data (kept in qa->Ga0, gaT0, W0); S~ from R1; L with its sub-step starts
+    quad_galerkin(U2, true, rr);
+    HIPCHECK(hipMemcpyAsync(qa->Ga0, de->Ga, rsz, hipMemcpyDeviceToDevice, stream));
+    if (op_q) HIPCHECK(hipMemcpyAsync(qa->gaT0, de->gaT, fsz, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(qa->W0, quad_W, tsz, hipMemcpyDeviceToDevice, stream));
+    quad_sstep(rr, rr, adj->RK, rr, 0, de->Snew, rr, -1.0, h, nsub);  // S~
+    quad_init(V, m, rr, de->Snew, rr, 1, L);
+    quad_rows_ex(false, rr, qa->Ga0, qa->gaT0, qa->W0, L, work, work + 2 * mx * rr, h, nsub, stL);
+    quad_gram(L, m, rr, shifted_chol(m));
+    cholqr_tail(L, L2, m, rr, V2, true, nullptr, 0, de_epi(), 0, 3);  // V1, R2
+    HIPCHECK(hipMemcpyAsync(adj->RL, small->Rtot, rsz, hipMemcpyDeviceToDevice, stream));
+    // Lbar1: R2 R2bar' = RL Sb1 (adj->Mb), then the QR rule
+    timed(LRI_KERNEL_SMALL, [&] { return launch_ksl_adj_small(adj, rr, 0, stream); });
+    auto qr_rule = [&](bool ns, const double* Xb, const double* Q1, const double* RRbar, double* Cq,
+                       const double* RinvT, double* Z) {
+      const int64_t rows_ = ns ? n : m;
+      RowArgs<double> A;  // Cq = copyltu(R Rbar' - Xb' Q1)
+      A.rows = rows_;
+      A.r_out = rr;
+      A.base = Xb;
+      A.ld_base = rows_;
+      A.reduce = 1;
+      A.W = Q1;
+      A.ldw = rows_;
+      A.rw = rr;
+      A.epi.kind = EPI_COPYLTU;
+      A.epi.S = const_cast<double*>(RRbar);
+      A.epi.ld_s = rr;
+      A.epi.dst = Cq;
+      rows(A);
+      RowArgs<double> B;  // Z = Xb + Q1 Cq
+      B.rows = rows_;
+      B.r_out = rr;
+      B.r_in = rr;
+      B.base = Xb;
+      B.ld_base = rows_;
+      B.X = Q1;
+      B.ldx = rows_;
+      B.Y = Cq;
+      B.ldy = rr;
+      B.out = Z;
+      B.ldo = rows_;
+      rows(B);
+      RowArgs<double> C;  // kbar = Z R^{-T}
+      C.rows = rows_;
+      C.r_out = rr;
+      C.r_in = rr;
+      C.X = Z;
+      C.ldx = rows_;
+      C.Y = RinvT;
+      C.ldy = rr;
+      C.out = kbar;
+      C.ldo = rows_;
+      rows(C);
+    };
+    qr_rule(false, Q, V2, adj->Mb, adj->CqL, adj->RLinvT, L2);
+    quad_rows_reverse(false, rr, qa->Ga0, qa->gaT0, qa->W0, stL, kbar, work, partL, qa->sumL, h, nsub);  // Lbar0
+    {
+      RowArgs<double> A;  // Lbar0 -> L, S~bar = Lbar0' V0 -> qa->Sib
+      A.rows = m;
+      A.r_out = rr;
+      A.base = kbar;
+      A.ld_base = m;
+      A.out = L;
+      A.ldo = m;
+      A.reduce = 1;
+      A.W = V;
+      A.ldw = m;
+      A.rw = rr;
+      A.epi.kind = EPI_COPY;
+      A.epi.dst = qa->Sib;
+      rows(A);
+    }
+    // S~ reverse (sgn -1 from R1): R1bar -> adj->CL
+    QuadSAdjArgs SA;
+    SA.k = SA.l = rr;
+    SA.q = op_q;
+    SA.S0 = adj->RK;
+    SA.lds0 = rr;
+    SA.Sb1 = qa->Sib;
+    SA.ldsb1 = rr;
+    SA.Sb0 = adj->CL;
+    SA.Ga = qa->Ga0;
+    SA.Gb = qa->Gb0;
+    SA.gaT = qa->gaT0;
+    SA.hbT = qa->hbT0;
+    SA.W = qa->W0;
+    SA.T = qa->T0;
+    SA.sgn = -1.0;
+    SA.h = h;
+    SA.nsub = nsub;
+    SA.Gab = qa->Gab1;
+    SA.Gbb = qa->Gbb1;
+    SA.gaTb = qa->gaTb1;
+    SA.hbTb = qa->hbTb1;
+    SA.Wb = qa->Wb1;
+    SA.Tb = qa->Tb1;
+    SA.work = swork;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_quad_sadj(SA, stream); });
+    timed(LRI_KERNEL_SMALL, [&] { return launch_ksl_adj_small(adj, rr, 1, stream); });  // Nb = RK CL' = R1 R1bar'
+    // the full cotangent of U1: Ub1 + its Galerkin data's (from the L and S reverses)
+    const int k3 = rr * rr * rr;
+    double* Ubf = P + n * rr;
+    {
+      BasisVjpArgs B = basis_vjp_args(true, rr, U2, qa->sumL, qa->sumL + k3, qa->sumL + k3 + rr * rr);
+      B.Tb2 = qa->Wb1;
+      B.Gb2 = qa->Gab1;
+      B.fb2 = op_q ? qa->gaTb1 : nullptr;
+      B.base = P;
+      B.ldb = n;
+      B.out = Ubf;
+      B.ldo = n;
+      timed(LRI_KERNEL_DE, [&] { return launch_basis_vjp(B, stream); });
+    }
+    qr_rule(true, Ubf, U2, adj->Nb, adj->CqK, adj->RKinvT, K2);
+    quad_rows_reverse(true, rr, qa->Gb0, qa->hbT0, qa->T0, stK, kbar, work, partK, qa->sumK, h, nsub);  // Kbar0
+    {
+      RowArgs<double> A;  // Ub0 = Kbar0 S0'; Sb0 = U0' Kbar0 (C = Kbar0' U0, stored transposed)
+      A.rows = n;
+      A.r_out = rr;
+      A.r_in = rr;
+      A.X = kbar;
+      A.ldx = n;
+      A.Y = S;
+      A.ldy = (int)rcap;
+      A.transY = 1;
+      A.out = P;
+      A.ldo = n;
+      rows(A);
+      RowArgs<double> B;
+      B.rows = n;
+      B.r_out = rr;
+      B.base = kbar;
+      B.ld_base = n;
+      B.reduce = 1;
+      B.W = U;
+      B.ldw = n;
+      B.rw = rr;
+      B.epi.kind = EPI_COPY;
+      B.epi.transpose = 1;
+      B.epi.dst = adj->Sb0;
+      rows(B);
+    }
+    {
+      BasisVjpArgs B = basis_vjp_args(false, rr, V, qa->sumK, qa->sumK + k3, qa->sumK + k3 + rr * rr);
+      B.Tb2 = qa->Tb1;
+      B.Gb2 = qa->Gbb1;
+      B.fb2 = op_q ? qa->hbTb1 : nullptr;
+      B.X3 = L;  // + Lbar0 S~
+      B.ldx3 = m;
+      B.Y3 = de->Snew;
+      B.ldy3 = rr;
+      B.out = Q;
+      B.ldo = m;
+      timed(LRI_KERNEL_DE, [&] { return launch_basis_vjp(B, stream); });
+    }
+    timed(LRI_KERNEL_SMALL, [&] {
+      return launch_quad_adj_scales(qa->sumK, qa->sumL, plen, qa->sumL + k3, qa->Ga0, qa->Gab1, qa->Ga0,
+                                    qa->sumK + k3, qa->Gb0, qa->Gbb1, qa->Gb0, rr, qa->scal, stream);
+    });
+  }
+
   int64_t quad_bug_ra(double dt, int64_t nsub, double tol, int64_t rmax, bool complete = true) {
     const int rr = (int)r;
     const int k2 = 2 * rr;
@@ -3175,8 +3374,8 @@ int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps) {
   });
 }
 
-int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* Ub, int64_t ldub, double* Sb,
-                            int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar) {
+static int de_adjoint_entry(lri_ctx* ctx, bool ksl, double dt, int64_t substeps, double* Ub, int64_t ldub,
+                            double* Sb, int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar) {
   return guarded(ctx, [&] {
     ctx->check_de(dt, substeps);
     REQUIRE(ctx->op_quad,
@@ -3194,7 +3393,10 @@ int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* U
     HIPCHECK(hipMemcpy2DAsync(ctx->P, d * n, Ub, d * ldub, d * n, r, hipMemcpyDefault, ctx->stream));
     HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyDefault, ctx->stream));
     HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyDefault, ctx->stream));
-    ctx->quad_bug_adjoint(dt, substeps);
+    if (ksl)
+      ctx->quad_ksl_adjoint(dt, substeps);
+    else
+      ctx->quad_bug_adjoint(dt, substeps);
     double sc[2] = {0.0, 0.0};
     HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ctx->P, d * n, d * n, r, hipMemcpyDefault, ctx->stream));
     HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDefault, ctx->stream));
@@ -3207,6 +3409,16 @@ int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* U
     }
     return f;
   });
+}
+
+int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* Ub, int64_t ldub, double* Sb,
+                            int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar) {
+  return de_adjoint_entry(ctx, false, dt, substeps, Ub, ldub, Sb, ldsb, Vb, ldvb, scale_bar);
+}
+
+int lri_step_de_ksl_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* Ub, int64_t ldub, double* Sb,
+                            int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar) {
+  return de_adjoint_entry(ctx, true, dt, substeps, Ub, ldub, Sb, ldsb, Vb, ldvb, scale_bar);
 }
 
 int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, int64_t rmax, int64_t* r_new) {

@@ -412,6 +412,10 @@ struct BasisVjpArgs {
   int64_t ldx3 = 0;
   const double* Y3 = nullptr;
   int ldy3 = 0, transY3 = 0;
+  // optional second cotangents of the same Galerkin data (added to Tb, Gb, fb)
+  const double* Tb2 = nullptr;
+  const double* Gb2 = nullptr;
+  const double* fb2 = nullptr;
   const double* base = nullptr;
   int64_t ldb = 0;
   double* out = nullptr;

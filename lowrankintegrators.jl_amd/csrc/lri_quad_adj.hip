@@ -566,8 +566,10 @@ __global__ __launch_bounds__(256) void k_basis_vjp(BasisVjpArgs Q) {
   const int64_t n = Q.rows;
   for (int e = t; e < k * k; e += blockDim.x) {
     const int l = e % k, c = e / k;
-    sig[e] = Q.Gb ? Q.Gb[l + c * k] + Q.Gb[c + l * k] : 0.0;
+    sig[e] = (Q.Gb ? Q.Gb[l + c * k] + Q.Gb[c + l * k] : 0.0) + (Q.Gb2 ? Q.Gb2[l + c * k] + Q.Gb2[c + l * k] : 0.0);
   }
+  // the tensor cotangent as the sum of its sources
+  auto tb = [&](int64_t idx) { return Q.Tb[idx] + (Q.Tb2 ? Q.Tb2[idx] : 0.0); };
   const int64_t ntiles = (n + R - 1) / R;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t r0 = tile * R;
@@ -578,13 +580,13 @@ __global__ __launch_bounds__(256) void k_basis_vjp(BasisVjpArgs Q) {
       const int lr = e / k, a = e % k;
       const double* xr = xs + lr * k;
       double v = 0.0, d = 0.0;
-      if (Q.Gb)
+      if (Q.Gb || Q.Gb2)
         for (int l = 0; l < k; ++l) v += xr[l] * sig[l + a * k];
       if (Q.Tb)
         for (int b = 0; b < k; ++b) {
-          const double* Tab = Q.Tb + (int64_t)k * (a + k * b);
+          const int64_t ab = (int64_t)k * (a + k * b);
           double in = 0.0;
-          for (int c = 0; c < k; ++c) in += Tab[c] * xr[c];
+          for (int c = 0; c < k; ++c) in += tb(ab + c) * xr[c];
           d += xr[b] * in;
         }
       xsig[e] = v;
@@ -606,18 +608,19 @@ __global__ __launch_bounds__(256) void k_basis_vjp(BasisVjpArgs Q) {
         const double* xr = xs + (lr + 1) * k;
         double tv = 0.0;
         for (int a = 0; a < k; ++a) {
-          const double* Tac = Q.Tb + (int64_t)k * (a + k * c);  // Tb_{a c x}
+          const int64_t ac = (int64_t)k * (a + k * c);  // Tb_{a c x} at ac + x
           double in1 = 0.0, in2 = 0.0;
           for (int x = 0; x < k; ++x) {
-            in1 += Tac[x] * xr[x];
-            in2 += Q.Tb[c + (int64_t)k * (a + k * x)] * xr[x];  // Tb_{a x c}
+            in1 += tb(ac + x) * xr[x];
+            in2 += tb(c + (int64_t)k * (a + k * x)) * xr[x];  // Tb_{a x c}
           }
           tv += dr[a] * (in1 + in2);
         }
         v += tv + apply_dt(dxb, lr, c, k, i, n, Q.dl, Q.dd, Q.du, Q.dper);
       }
-      if (Q.Gb && Q.st_d) v += apply_a(xsig, lr, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
-      for (int qq = 0; qq < Q.q; ++qq) v += Q.F[i + qq * Q.ldf] * Q.fb[qq + c * Q.q];
+      if ((Q.Gb || Q.Gb2) && Q.st_d) v += apply_a(xsig, lr, c, k, i, n, Q.st_d, Q.st_e, Q.periodic);
+      for (int qq = 0; qq < Q.q; ++qq)
+        v += Q.F[i + qq * Q.ldf] * (Q.fb[qq + c * Q.q] + (Q.fb2 ? Q.fb2[qq + c * Q.q] : 0.0));
       if (Q.X2)
         for (int l = 0; l < k; ++l)
           v += Q.X2[i + l * Q.ldx2] * (Q.transY2 ? Q.Y2[c + l * Q.ldy2] : Q.Y2[l + c * Q.ldy2]);
@@ -631,7 +634,8 @@ __global__ __launch_bounds__(256) void k_basis_vjp(BasisVjpArgs Q) {
 
 hipError_t launch_basis_vjp(const BasisVjpArgs& Q, hipStream_t st) {
   if (Q.k < 1 || Q.k > kQuadMaxRank || Q.rows < 1 || !Q.X || !Q.out || Q.q < 0 || Q.q > kMaxForce ||
-      (Q.q > 0 && (!Q.F || !Q.fb)) || (Q.Gb && Q.st_d && !Q.st_e) || (Q.X2 && !Q.Y2) || (Q.X3 && !Q.Y3))
+      (Q.q > 0 && (!Q.F || !Q.fb)) || (Q.Gb && Q.st_d && !Q.st_e) || (Q.X2 && !Q.Y2) || (Q.X3 && !Q.Y3) ||
+      (Q.Tb2 && !Q.Tb))
     return hipErrorInvalidValue;
   const int R = 256 / Q.k;
   const int64_t tiles = (Q.rows + R - 1) / R;

@@ -61,6 +61,7 @@ SIGNATURES = {
     "lri_step_de_bug": (_int, [_vp, ctypes.c_double, _i64]),
     "lri_step_de_bug_ra": (_int, [_vp, ctypes.c_double, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
     "lri_step_de_bug_adjoint": (_int, [_vp, ctypes.c_double, _i64, _dp, _i64, _dp, _i64, _dp, _i64, _dp]),
+    "lri_step_de_ksl_adjoint": (_int, [_vp, ctypes.c_double, _i64, _dp, _i64, _dp, _i64, _dp, _i64, _dp]),
     "lri_batch_create": (_int, [ctypes.POINTER(_vp), _int, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i64)]),
     "lri_batch_destroy": (_int, [_vp]),
@@ -330,12 +331,19 @@ class Context:
     def step_de_bug_adjoint(self, dt: float, substeps: int, Ub, Sb, Vb):
         """Cotangents of the matrix-ODE BUG step's inputs from those of its result (host
         arrays), and the derivatives (a, b) along the scales of the operator's linear parts."""
+        return self._de_adjoint(lib().lri_step_de_bug_adjoint, dt, substeps, Ub, Sb, Vb)
+
+    def step_de_ksl_adjoint(self, dt: float, substeps: int, Ub, Sb, Vb):
+        """The same for the primal KSL matrix-ODE step."""
+        return self._de_adjoint(lib().lri_step_de_ksl_adjoint, dt, substeps, Ub, Sb, Vb)
+
+    def _de_adjoint(self, fn, dt, substeps, Ub, Sb, Vb):
         Ub = np.asfortranarray(Ub, dtype=np.float64).copy(order="F")
         Sb = np.asfortranarray(Sb, dtype=np.float64).copy(order="F")
         Vb = np.asfortranarray(Vb, dtype=np.float64).copy(order="F")
         sc = np.zeros(2)
-        check(lib().lri_step_de_bug_adjoint(self.h, float(dt), int(substeps), _dptr(Ub), Ub.shape[0], _dptr(Sb),
-                                            Sb.shape[0], _dptr(Vb), Vb.shape[0], _dptr(sc)), self.h)
+        check(fn(self.h, float(dt), int(substeps), _dptr(Ub), Ub.shape[0], _dptr(Sb), Sb.shape[0], _dptr(Vb),
+                 Vb.shape[0], _dptr(sc)), self.h)
         return Ub, Sb, Vb, (float(sc[0]), float(sc[1]))
 
     def synchronize(self):

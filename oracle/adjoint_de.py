@@ -1,4 +1,4 @@
-"""Reverse- and forward-mode restatement of the matrix-ODE BUG step.
+"""Reverse- and forward-mode restatement of the matrix-ODE BUG and primal KSL steps.
 
 TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
@@ -24,6 +24,10 @@ Everything is dense in (n, m): with Y (n x m) the sub-problem's matrix,
     J_F(Y)[d]    = A d + d B' - (D d) o Y - (D Y) o d,
     J_F(Y)'[Yb]  = A' Yb + Yb B - D'(Yb o Y) - Yb o (D Y),
 an independent route from the engine's r x r x r tensor form.
+
+The primal KSL step (``projector_splitting.jl:129-152`` with the sub-problems of
+``:52-80``) is restated the same way (``ksl_primal_de_step*``): K -> (U1, R1), the S
+sub-step backwards in time from R1, L from V0 S~' -> (V1, R2), S1 = R2'.
 
 Pinning (``tests/test_oracle_adjoint_de.py``): central finite differences of
 the forward step in every input (U0, S0, V0, a, b) along random directions,
@@ -257,20 +261,123 @@ def bug_de_step_jvp(F: ScaledQuadratic, U0, S0, V0, dt, N, Ud, Sd, Vd, ad=0.0, b
     return U1d, S1d, V1d
 
 
-def bug_de_solve(F, U, S, V, dt, N, steps):
+def ksl_primal_de_step(F: ScaledQuadratic, U0, S0, V0, dt, N):
+    """primal_LT_step! with the DE sub-problems (projector_splitting.jl:52-80,129-152),
+    positive-diagonal QR:  K -> (U1, R1);  S~' = -U1'F(U1 S V0')V0 from R1;
+    L -> (V1, R2) from V0 S~';  S1 = R2'."""
+    h = dt / N
+    fK = lambda K: F(K @ V0.T) @ V0  # noqa: E731
+    U1, R1 = qr_pos(rk4_states(fK, U0 @ S0, h, N)[-1])
+    fS = lambda S: -(U1.T @ F(U1 @ S @ V0.T) @ V0)  # noqa: E731
+    St = rk4_states(fS, R1, h, N)[-1]
+    fL = lambda L: F(U1 @ L.T).T @ U1  # noqa: E731
+    V1, R2 = qr_pos(rk4_states(fL, V0 @ St.T, h, N)[-1])
+    return U1, R2.T.copy(), V1
+
+
+def ksl_primal_de_step_vjp(F: ScaledQuadratic, U0, S0, V0, dt, N, Ub1, Sb1, Vb1):
+    """Cotangents (Ub0, Sb0, Vb0, abar, bbar) of the primal KSL matrix-ODE step; both QR
+    factors carry cotangents: Kbar = (Qbar + Q copyltu(R Rbar' - Qbar'Q)) R^{-T}."""
+    from .adjoint import qr_vjp_full
+
+    h = dt / N
+    fK = lambda K: F(K @ V0.T) @ V0  # noqa: E731
+    Ks = rk4_states(fK, U0 @ S0, h, N)
+    U1, R1 = qr_pos(Ks[-1])
+    fS = lambda S: -(U1.T @ F(U1 @ S @ V0.T) @ V0)  # noqa: E731
+    Ss = rk4_states(fS, R1, h, N)
+    St = Ss[-1]
+    fL = lambda L: F(U1 @ L.T).T @ U1  # noqa: E731
+    Ls = rk4_states(fL, V0 @ St.T, h, N)
+    V1, R2 = qr_pos(Ls[-1])
+    acc = {"U1": Ub1.copy(), "V0": np.zeros_like(V0), "a": 0.0, "b": 0.0}
+
+    def vjp_L(L, lb):
+        Y = U1 @ L.T
+        Gb, da, db = F.vjp(Y, U1 @ lb.T)
+        acc["U1"] += F(Y) @ lb + Gb @ L
+        acc["a"] += da
+        acc["b"] += db
+        return Gb.T @ U1
+
+    def vjp_S(S, sb):  # f = -U1' F(U1 S V0') V0
+        Y = U1 @ S @ V0.T
+        Gb, da, db = F.vjp(Y, -(U1 @ sb @ V0.T))
+        FY = F(Y)
+        acc["U1"] += -(FY @ V0 @ sb.T) + Gb @ V0 @ S.T
+        acc["V0"] += -(FY.T @ U1 @ sb) + Gb.T @ U1 @ S
+        acc["a"] += da
+        acc["b"] += db
+        return U1.T @ Gb @ V0
+
+    def vjp_K(K, kb):
+        Y = K @ V0.T
+        Gb, da, db = F.vjp(Y, kb @ V0.T)
+        acc["V0"] += F(Y).T @ kb + Gb.T @ K
+        acc["a"] += da
+        acc["b"] += db
+        return Gb @ V0
+
+    Lb = qr_vjp_full(V1, R2, Vb1, Sb1.T)           # S1 = R2'
+    L0b = rk4_vjp(fL, vjp_L, Ls, h, Lb)
+    Vb0 = L0b @ St                                  # L(0) = V0 St'
+    Stb = L0b.T @ V0
+    R1b = rk4_vjp(fS, vjp_S, Ss, h, Stb)
+    Kb = qr_vjp_full(U1, R1, acc["U1"], R1b)
+    K0b = rk4_vjp(fK, vjp_K, Ks, h, Kb)
+    return K0b @ S0.T, U0.T @ K0b, Vb0 + acc["V0"], acc["a"], acc["b"]
+
+
+def ksl_primal_de_step_jvp(F: ScaledQuadratic, U0, S0, V0, dt, N, Ud, Sd, Vd, ad=0.0, bd=0.0):
+    """Forward-mode tangents (U1d, S1d, V1d) of the primal KSL matrix-ODE step."""
+    h = dt / N
+    fK = lambda K: F(K @ V0.T) @ V0  # noqa: E731
+
+    def jK(K, Kd):
+        Y = K @ V0.T
+        return F.jvp(Y, Kd @ V0.T + K @ Vd.T, ad, bd) @ V0 + F(Y) @ Vd
+
+    K, Kd = rk4_jvp(fK, jK, U0 @ S0, Ud @ S0 + U0 @ Sd, h, N)
+    U1, R1 = qr_pos(K)
+    U1d, R1d = qr_jvp(U1, R1, Kd)
+    fS = lambda S: -(U1.T @ F(U1 @ S @ V0.T) @ V0)  # noqa: E731
+
+    def jS(S, Sd_):
+        Y = U1 @ S @ V0.T
+        Yd = U1d @ S @ V0.T + U1 @ Sd_ @ V0.T + U1 @ S @ Vd.T
+        FY = F(Y)
+        return -(U1d.T @ FY @ V0 + U1.T @ F.jvp(Y, Yd, ad, bd) @ V0 + U1.T @ FY @ Vd)
+
+    St, Std = rk4_jvp(fS, jS, R1, R1d, h, N)
+    fL = lambda L: F(U1 @ L.T).T @ U1  # noqa: E731
+
+    def jL(L, Ld):
+        Y = U1 @ L.T
+        return F.jvp(Y, U1d @ L.T + U1 @ Ld.T, ad, bd).T @ U1 + F(Y).T @ U1d
+
+    L, Ld = rk4_jvp(fL, jL, V0 @ St.T, Vd @ St.T + V0 @ Std.T, h, N)
+    V1, R2 = qr_pos(L)
+    V1d, R2d = qr_jvp(V1, R2, Ld)
+    return U1d, R2d.T.copy(), V1d
+
+
+def bug_de_solve(F, U, S, V, dt, N, steps, ksl=False):
+    step = ksl_primal_de_step if ksl else bug_de_step
     for _ in range(steps):
-        U, S, V = bug_de_step(F, U, S, V, dt, N)
+        U, S, V = step(F, U, S, V, dt, N)
     return U, S, V
 
 
-def bug_de_solve_gradient(F: ScaledQuadratic, U0, S0, V0, dt, N, steps, loss_grad, checkpoint=20):
-    """Gradient of loss(U_T, S_T, V_T) after `steps` BUG steps with respect to
-    (U0, S0, V0) and the scales (a, b), with checkpoints every `checkpoint` steps
-    and recomputation inside each segment.  Returns (value, Ub0, Sb0, Vb0, abar, bbar)."""
+def bug_de_solve_gradient(F: ScaledQuadratic, U0, S0, V0, dt, N, steps, loss_grad, checkpoint=20, ksl=False):
+    """Gradient of loss(U_T, S_T, V_T) after `steps` BUG (ksl: primal KSL) steps with
+    respect to (U0, S0, V0) and the scales (a, b), with checkpoints every `checkpoint`
+    steps and recomputation inside each segment.  Returns (value, Ub0, Sb0, Vb0, abar, bbar)."""
+    bug_de_step_ = ksl_primal_de_step if ksl else bug_de_step
+    bug_de_step_vjp_ = ksl_primal_de_step_vjp if ksl else bug_de_step_vjp
     cps = {0: (U0, S0, V0)}
     U, S, V = U0, S0, V0
     for k in range(steps):
-        U, S, V = bug_de_step(F, U, S, V, dt, N)
+        U, S, V = bug_de_step_(F, U, S, V, dt, N)
         if (k + 1) % checkpoint == 0 and k + 1 < steps:
             cps[k + 1] = (U, S, V)
     value, Ub, Sb, Vb = loss_grad(U, S, V)
@@ -279,9 +386,9 @@ def bug_de_solve_gradient(F: ScaledQuadratic, U0, S0, V0, dt, N, steps, loss_gra
         c1 = min(c0 + checkpoint, steps)
         states = [cps[c0]]
         for _ in range(c0, c1 - 1):
-            states.append(bug_de_step(F, *states[-1], dt, N))
+            states.append(bug_de_step_(F, *states[-1], dt, N))
         for k in range(c1 - 1, c0 - 1, -1):
-            Ub, Sb, Vb, da, db = bug_de_step_vjp(F, *states[k - c0], dt, N, Ub, Sb, Vb)
+            Ub, Sb, Vb, da, db = bug_de_step_vjp_(F, *states[k - c0], dt, N, Ub, Sb, Vb)
             abar += da
             bbar += db
     return value, Ub, Sb, Vb, abar, bbar

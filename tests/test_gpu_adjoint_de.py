@@ -208,3 +208,46 @@ def test_adjoint_argument_errors(lri):
         with pytest.raises(ValueError):
             ctx.step_de_bug_adjoint(0.01, 0, U, S, V)  # substeps < 1
         ctx.step_de_bug_adjoint(0.01, 2, U, S, V)
+
+
+@pytest.mark.parametrize("periodic,q,shape,dt", [(True, 1, (50, 37, 4), 0.01), (False, 2, (300, 200, 16), 0.05)])
+def test_ksl_one_step_adjoint_vs_oracle(lri, periodic, q, shape, dt):
+    """lri_step_de_ksl_adjoint against oracle.adjoint_de.ksl_primal_de_step_vjp (V0's cotangent on
+    the tangent space: the engine's sub-problems take V0'V0 = I)."""
+    n, m, r = shape
+    say(f"case: matrix-ODE primal KSL adjoint, one step, n={n} m={m} r={r} q={q} periodic={periodic}")
+    dev, host = random_operator(lri, n, m, q, periodic, seed=21)
+    rng = np.random.default_rng(22)
+    U, S, V = random_state(rng, n, m, r)
+    cot = [rng.standard_normal(x.shape) for x in (U, S, V)]
+    N = lri.rk4_substeps(dev.spectral_bound(2 * np.abs(U @ S @ V.T).max()), dt)
+    with context(lri, dev, U, S, V) as ctx:
+        Ub, Sb, Vb, (a, b) = ctx.step_de_ksl_adjoint(dt, N, *cot)
+    ref = ad.ksl_primal_de_step_vjp(host, U, S, V, dt, N, *cot)
+    errs = [rel(Ub, ref[0]), rel(Sb, ref[1]), rel(tangential(V, Vb), tangential(V, ref[2])),
+            abs(a - ref[3]) / abs(ref[3]), abs(b - ref[4]) / max(abs(ref[4]), 1e-300)]
+    say(f"  N={N}: rel err Ub {errs[0]:.1e} Sb {errs[1]:.1e} Vb {errs[2]:.1e} abar {errs[3]:.1e} bbar {errs[4]:.1e}")
+    assert max(errs) <= 1e-9
+
+
+def test_ksl_checkpointed_burgers_gradient_vs_oracle(lri):
+    n, m, r, K, nu = 128, 32, 12, 8, 0.01
+    dt, steps = 1e-4, 6
+    xi = np.random.default_rng(23).standard_normal((K, m))
+    sb = lri.SineBurgers(n, xi, nu, r, seed=4)
+    prob = sb.problem((0.0, steps * dt))
+    N = lri.de_substeps(prob, dt)
+    say(f"case: Burgers sine gradient, primal KSL, n={n} m={m} r={r}, {steps} steps, checkpoint 4")
+    alg = lri.ProjectorSplitting(lri.PrimalLieTrotter())
+    val, (Ub, Sb, Vb), (abar, _) = lri.solve_gradient(prob, alg, lri.mean_column_loss, dt, checkpoint=4)
+    Lap, Grad = ad.periodic_burgers(n, nu)
+    host = ad.ScaledQuadratic(sp.csr_matrix(Lap), sp.csr_matrix((m, m)), np.zeros((n, 0)), np.zeros((m, 0)),
+                              sp.csr_matrix(Grad))
+    u0 = sb.initial()
+    rv, rUb, rSb, rVb, rabar, _ = ad.bug_de_solve_gradient(host, u0.U, u0.S, u0.V, dt, N, steps, ad.mean_column_loss,
+                                                           checkpoint=4, ksl=True)
+    e_xi = rel(sb.pullback(Ub, Sb, Vb), sb.pullback(rUb, rSb, rVb))
+    e_nu = abs(abar - rabar) / abs(rabar)
+    say(f"  loss {val:.6e} (oracle {rv:.6e}); rel err dL/dxi {e_xi:.1e}, dL/dnu {e_nu:.1e}")
+    assert abs(val - rv) <= 1e-10 * abs(rv)
+    assert e_xi <= 1e-8 and e_nu <= 1e-8

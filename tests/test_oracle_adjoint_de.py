@@ -168,3 +168,28 @@ def test_package_setup_of_the_differentiation_workload_matches_the_oracle(lri):
     fd = (f(eps) - f(-eps)) / (2 * eps)
     an = float(np.sum(sb.pullback(np.zeros((n, r)), Cs, Cv) * d))
     assert abs(fd - an) <= 1e-7 * max(1.0, abs(an))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_ksl_step_vjp_matches_finite_differences_and_forward_mode(periodic):
+    rng = np.random.default_rng(11 + periodic)
+    n, m, r, dt, N = 14, 11, 3, 0.01, 3
+    F = random_op(rng, n, m, q=1, periodic=periodic)
+    U, S, V = random_state(rng, n, m, r)
+    C = [rng.standard_normal(x.shape) for x in (U, S, V)]
+    Ub, Sb, Vb, abar, bbar = ad.ksl_primal_de_step_vjp(F, U, S, V, dt, N, *C)
+
+    def loss(U_, S_, V_, a=1.0, b=1.0):
+        return pair(C, ad.ksl_primal_de_step(F.scaled(a, b), U_, S_, V_, dt, N))
+
+    dirs = [rng.standard_normal(x.shape) for x in (U, S, V)]
+    eps = 1e-6
+    fd = (loss(*(x + eps * d for x, d in zip((U, S, V), dirs))) - loss(*(x - eps * d for x, d in zip((U, S, V),
+                                                                                                     dirs)))) / (2 * eps)
+    an = pair((Ub, Sb, Vb), dirs)
+    assert abs(fd - an) <= 1e-7 * max(1.0, abs(an))
+    fa = (loss(U, S, V, 1 + eps) - loss(U, S, V, 1 - eps)) / (2 * eps)
+    fb = (loss(U, S, V, 1, 1 + eps) - loss(U, S, V, 1, 1 - eps)) / (2 * eps)
+    assert abs(fa - abar) <= 1e-7 * max(1.0, abs(abar)) and abs(fb - bbar) <= 1e-7 * max(1.0, abs(bbar))
+    J = ad.ksl_primal_de_step_jvp(F, U, S, V, dt, N, *dirs, 0.4, -0.2)
+    assert abs(pair(C, J) - (an + 0.4 * abar - 0.2 * bbar)) <= 1e-11 * max(1.0, abs(an))
