@@ -604,6 +604,112 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_bf16x3(const uint16_t* _
     }
 }
 
+// pass T, wide form (rb = 32): a workgroup covers 128 columns of dA, wave w the
+// columns 32w .. 32w+31, all four over the same rows.  Each 64-row block of the
+// B panel (hi and lo, 8 KB, contiguous in the panel) is then loaded once per
+// workgroup into LDS (double-buffered, prefetched into registers one block
+// ahead) and read by the four waves, instead of once per wave from L2.  The A
+// tiles are loaded, split and staged per wave as in k_passT_bf16x3; kSingle
+// streams A[k_next] itself (two products per k-step, see k_passT_bf16_single).
+// No cross-wave reduction: every wave owns its 32 output rows.
+template <bool kSingle>
+__global__ __launch_bounds__(kPassThreads) void k_passT_bf16_wide(const uint16_t* __restrict__ an,
+                                                                  const uint16_t* __restrict__ ap, int64_t lda,
+                                                                  int64_t n, int64_t m,
+                                                                  const uint16_t* __restrict__ Bhi,
+                                                                  const uint16_t* __restrict__ Blo, int rb,
+                                                                  int64_t ichunk, float* __restrict__ part,
+                                                                  int vec) {
+  __shared__ __attribute__((aligned(16))) uint16_t sA[4][2][32 * kTPitch];
+  __shared__ u32x4_t sB[2][2][256];  // [buffer][hi, lo][64 rows x 32 columns in fragment order]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int lc = lane & 31, h = lane >> 5;
+  const int64_t j0 = (int64_t)blockIdx.x * 128 + 32 * wave;
+  const int s = blockIdx.y;
+  const int64_t ib = (int64_t)s * ichunk;
+  const int64_t ie = min(n, ib + ichunk);
+  const bool v = vec != 0;
+  uint16_t* th = sA[wave][0];
+  uint16_t* tl = sA[wave][1];
+  f32x16_t acc = f32x16_t{};
+  const int lcol = lane >> 3, loct = lane & 7;
+  u32x4_t xn[4], xp[4], nxn[4], nxp[4];
+  auto load = [&](int64_t i, u32x4_t (&A1)[4], u32x4_t (&A0)[4]) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int64_t jj = j0 + 8 * p + lcol;
+      const int64_t r0 = i + 8 * loct;
+      A1[p] = load_raw8(an, jj * lda + r0, r0, ie, jj < m, v);
+      if (!kSingle) A0[p] = load_raw8(ap, jj * lda + r0, r0, ie, jj < m, v);
+    }
+  };
+  // the B block of rows i .. i + 63: 256 hi and 256 lo 16-byte pieces, one of each per thread
+  auto loadB = [&](int64_t i, u32x4_t& gh, u32x4_t& gl) {
+    const int64_t base = (i >> 6) * 64 * rb;
+    gh = reinterpret_cast<const u32x4_t*>(Bhi + base)[t];
+    gl = reinterpret_cast<const u32x4_t*>(Blo + base)[t];
+  };
+  u32x4_t gh, gl;
+  int buf = 0;
+  if (ib < ie) {
+    load(ib, xn, xp);
+    loadB(ib, gh, gl);
+    sB[0][0][t] = gh;
+    sB[0][1][t] = gl;
+  }
+  __syncthreads();
+  for (int64_t i = ib; i < ie; i += 64) {
+    const bool more = i + 64 < ie;
+    if (more) {
+      load(i + 64, nxn, nxp);
+      loadB(i + 64, gh, gl);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int col = 8 * p + lcol;
+      if (kSingle) {
+        *reinterpret_cast<u32x4_t*>(th + col * kTPitch + 8 * loct) = xn[p];
+      } else {
+        u32x4_t hv, lv;
+        diff_split8(xn[p], xp[p], hv, lv);
+        *reinterpret_cast<u32x4_t*>(th + col * kTPitch + 8 * loct) = hv;
+        *reinterpret_cast<u32x4_t*>(tl + col * kTPitch + 8 * loct) = lv;
+      }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const u32x4_t ah = *reinterpret_cast<const u32x4_t*>(th + lc * kTPitch + 32 * h + 8 * q);
+      const u32x4_t bh = sB[buf][0][(q * 2 + h) * 32 + lc];
+      const u32x4_t bl = sB[buf][1][(q * 2 + h) * 32 + lc];
+      acc = mfma_bf16(as_bf16x8(ah), as_bf16x8(bh), acc);
+      acc = mfma_bf16(as_bf16x8(ah), as_bf16x8(bl), acc);
+      if (!kSingle) {
+        const u32x4_t al = *reinterpret_cast<const u32x4_t*>(tl + lc * kTPitch + 32 * h + 8 * q);
+        acc = mfma_bf16(as_bf16x8(al), as_bf16x8(bh), acc);
+      }
+    }
+    if (more) {
+      sB[buf ^ 1][0][t] = gh;
+      sB[buf ^ 1][1][t] = gl;
+    }
+    __syncthreads();  // sB[buf] read by all waves, sB[buf ^ 1] written; the A staging is free again
+    if (more) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        xn[p] = nxn[p];
+        if (!kSingle) xp[p] = nxp[p];
+      }
+    }
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int w = 0; w < 16; ++w) {
+    const int64_t jj = j0 + 8 * (w / 4) + 4 * h + (w % 4);
+    if (jj < m) part[((int64_t)s * rb + lc) * m + jj] = acc[w];
+  }
+}
+
 // pass N: the operand needs 8 consecutive columns per lane but memory runs
 // along rows, so each wave stages a (32*TM rows x 16 columns) hi/lo tile in
 // its own LDS region: coalesced 16-byte loads down the columns, stored
@@ -972,7 +1078,7 @@ static int tile_units(int mode, bool transposed, int rb) {
   }
   int cb = rb / 32;
   if (mode == PASS_F32) return cb <= 1 ? 2 : 1;
-  if (transposed) return 1;
+  if (transposed) return cb <= 1 ? 4 : 1;  // rb = 32: the wide pass T (128 columns per workgroup)
   return cb <= 1 ? 4 : (cb == 2 ? 2 : 1);
 }
 
@@ -1043,6 +1149,17 @@ static hipError_t run_pass(int mode, bool transposed, const PassArgs& a, hipStre
     const uint16_t* an = (const uint16_t*)a.anext;
     const uint16_t* ap = (const uint16_t*)a.aprev;
     const int v = vec_ok(a) ? 1 : 0;
+    if constexpr (TM == 4 && CB == 1) {
+      if (transposed) {  // rb = 32: the wide pass T
+        if (!ap)
+          hipLaunchKernelGGL((k_passT_bf16_wide<true>), grid, blk, 0, st, an, an, a.lda, a.n, a.m,
+                             (const uint16_t*)a.B, (const uint16_t*)a.B2, a.rb, a.chunk, (float*)a.part, v);
+        else
+          hipLaunchKernelGGL((k_passT_bf16_wide<false>), grid, blk, 0, st, an, ap, a.lda, a.n, a.m,
+                             (const uint16_t*)a.B, (const uint16_t*)a.B2, a.rb, a.chunk, (float*)a.part, v);
+        return hipGetLastError();
+      }
+    }
     if (!ap) {
       if (transposed)
         hipLaunchKernelGGL((k_passT_bf16_single<CB>), grid, blk, 0, st, an, a.lda, a.n, a.m, (const uint16_t*)a.B,
