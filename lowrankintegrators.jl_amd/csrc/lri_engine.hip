@@ -230,6 +230,8 @@ struct lri_ctx {
   void rows(RowArgs<PT> A) {
     A.epi.small = small;
     A.epi.flags = flags;
+    A.pre.small = small;
+    A.pre.flags = flags;
     A.gpart = gpart;
     timed(LRI_KERNEL_SMALL, [&] { return launch_rows<PT>(A, stream); });
   }
@@ -347,8 +349,21 @@ struct lri_ctx {
   // shift_rows) came from G + s I, so Q1 = X R1^{-1} is only moderately
   // conditioned; two more CholeskyQR passes make it orthogonal to O(eps) for
   // cond(X) up to ~1/eps.  X is overwritten by the middle pass.
+  //
+  // With passes = 2 and an EPI_STILDE epilogue, the second Gram matrix and the
+  // cross product Q1'W are reduced together and S~ is formed in the second
+  // Cholesky epilogue (U1'W = R2^{-T} Q1'W), so the last row kernel needs no
+  // reduction.
+  //
+  // first: the caller launched the first kernel with defer_epi and these
+  // EPI_CHOL1 arguments (only where fold_chol(rr, passes)); then each Cholesky
+  // epilogue runs in the prologue of the row kernel that applies its R^{-1},
+  // which saves one launch per factor.
+  static bool fold_chol(int rr, int passes = 2) { return passes == 2 && rr <= 32; }
   void cholqr_tail(double* X, double* tmp, int64_t rows_, int rr, double* out, bool pk_forN, const double* W,
-                   int rw, EpiArgs final_epi, int sfinal, int passes = 2) {
+                   int rw, EpiArgs final_epi, int sfinal, int passes = 2, const EpiArgs* first = nullptr) {
+    const bool fuse_st = passes == 2 && W && final_epi.kind == EPI_STILDE && rw == rr && rr <= 32;
+    REQUIRE(!first || fold_chol(rr, passes), "internal: deferred Cholesky epilogue cannot be folded");
     RowArgs<double> B;
     B.rows = rows_;
     B.r_out = rr;
@@ -365,6 +380,20 @@ struct lri_ctx {
     B.epi.S = (sfinal && passes == 2) ? S : nullptr;
     B.epi.ld_s = (int)rcap;
     B.epi.transpose = sfinal == 2;
+    if (fuse_st) {
+      B.W = W;
+      B.ldw = rows_;
+      B.rw = rw;
+      B.gram_w = 1;
+      B.epi.stilde = final_epi.transpose ? 2 : 1;
+    }
+    if (first) {
+      B.pre = *first;
+      B.pre_C = small->Cred;
+      B.pre_a = rr;
+      B.pre_b = rr;
+      B.defer_epi = 1;
+    }
     rows(B);
     const double* last = tmp;
     if (passes == 3) {
@@ -397,8 +426,14 @@ struct lri_ctx {
     C.ldy = rr;
     C.out = out;
     C.ldo = rows_;
+    if (first) {
+      C.pre = B.epi;
+      C.pre_C = small->Cred;
+      C.pre_a = rr;
+      C.pre_b = fuse_st ? 2 * rr : rr;
+    }
     set_pack_outputs(C, pk_forN, pass_mode(), rb_for(pass_mode(), rr));
-    if (final_epi.kind != EPI_NONE || W) {
+    if (!fuse_st && (final_epi.kind != EPI_NONE || W)) {
       C.reduce = 1;
       C.W = W;
       C.ldw = rows_;
@@ -420,6 +455,9 @@ struct lri_ctx {
   // primal_LT_step! (projector_splitting.jl:129-152)
   void ksl_primal(int64_t kp, int64_t kn) {
     const int rr = (int)r;
+    const bool fold = fold_chol(rr);
+    EpiArgs c1;
+    c1.kind = EPI_CHOL1;
     ensure_packed(true);
     const int sN = passN(kp, kn, rr, partN);
     with_parts(partN, [&](auto A) {  // K = U S + dA V ; P = dA V ; Gram -> R1
@@ -437,12 +475,13 @@ struct lri_ctx {
       A.out = K;
       A.ldo = n;
       A.reduce = 1;
-      A.epi.kind = EPI_CHOL1;
+      A.epi = c1;
+      A.defer_epi = fold;
       rows(A);
     });
     EpiArgs e;
     e.kind = EPI_STILDE;  // S~ = R - U1' P
-    cholqr_tail(K, K2, n, rr, U, false, P, rr, e, 0);
+    cholqr_tail(K, K2, n, rr, U, false, P, rr, e, 0, 2, fold ? &c1 : nullptr);
     const int sT = passT(kp, kn, rr, partT);
     with_parts(partT, [&](auto A) {  // L = V S~' + dA' U1 ; Gram -> R1
       A.rows = m;
@@ -458,16 +497,20 @@ struct lri_ctx {
       A.out = L;
       A.ldo = m;
       A.reduce = 1;
-      A.epi.kind = EPI_CHOL1;
+      A.epi = c1;
+      A.defer_epi = fold;
       rows(A);
     });
     EpiArgs none;
-    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 2);  // V1, S = R'
+    cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 2, 2, fold ? &c1 : nullptr);  // V1, S = R'
   }
 
   // dual_LT_step! (projector_splitting.jl:166-189)
   void ksl_dual(int64_t kp, int64_t kn) {
     const int rr = (int)r;
+    const bool fold = fold_chol(rr);
+    EpiArgs c1;
+    c1.kind = EPI_CHOL1;
     ensure_packed(false);
     const int sT = passT(kp, kn, rr, partT);
     with_parts(partT, [&](auto A) {  // L = V S' + dA' U ; Q = dA' U
@@ -486,13 +529,14 @@ struct lri_ctx {
       A.out = L;
       A.ldo = m;
       A.reduce = 1;
-      A.epi.kind = EPI_CHOL1;
+      A.epi = c1;
+      A.defer_epi = fold;
       rows(A);
     });
     EpiArgs e;
     e.kind = EPI_STILDE;  // S~ = (R - V1' Q)' = R' - U' dA V1
     e.transpose = 1;
-    cholqr_tail(L, L2, m, rr, V, true, Q, rr, e, 0);
+    cholqr_tail(L, L2, m, rr, V, true, Q, rr, e, 0, 2, fold ? &c1 : nullptr);
     const int sN = passN(kp, kn, rr, partN);
     with_parts(partN, [&](auto A) {  // K = U S~ + dA V1
       A.rows = n;
@@ -507,11 +551,12 @@ struct lri_ctx {
       A.out = K;
       A.ldo = n;
       A.reduce = 1;
-      A.epi.kind = EPI_CHOL1;
+      A.epi = c1;
+      A.defer_epi = fold;
       rows(A);
     });
     EpiArgs none;
-    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 1);  // U1, S = R
+    cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 1, 2, fold ? &c1 : nullptr);  // U1, S = R
   }
 
   // unconventional_step! (unconventional.jl:133-157)

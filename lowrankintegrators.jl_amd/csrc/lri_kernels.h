@@ -57,6 +57,9 @@ struct EpiArgs {
   // EPI_CHOL1: when > 0, factor G + s I with s = 11 (rows k + k (k + 1)) u trace(G)
   // (the shift of shifted CholeskyQR3 for an n x k factor with n = shift_rows)
   int64_t shift_rows = 0;
+  // EPI_CHOL2 of a reduction out' [out | W] (b = 2a): also St = Rtot - R2^{-T} (out' W),
+  // i.e. S~ = R - U1' W with U1 = out R2^{-1} (stored transposed when stilde == 2)
+  int stilde = 0;
 };
 
 // Row kernel (see lri_rowops.hip).
@@ -95,13 +98,21 @@ struct RowArgs {
   uint16_t* pk_lo = nullptr;
   int pk_layout = 0;          // 0: pass-N order (16-row blocks), 1: pass-T order (64-row blocks)
   int pk_cols = 0;
-  // reduction C = out' * W  (W == nullptr: out' * out)
+  // reduction C = out' * W  (W == nullptr: out' * out; gram_w: out' * [out | W])
   int reduce = 0;
   const double* W = nullptr;
   int64_t ldw = 0;
   int rw = 0;
-  double* gpart = nullptr;    // [grid][r_out * rw]
+  int gram_w = 0;
+  double* gpart = nullptr;    // [grid][r_out * (columns of C)]
   EpiArgs epi;
+  int defer_epi = 0;          // reduce, but leave epi to the next row kernel's prologue
+  // prologue: the deferred EPI_CHOL1/EPI_CHOL2 epilogue of the previous row
+  // kernel on its reduced C (pre_a x pre_b), run by every workgroup (workgroup 0
+  // stores its results); its R^{-1} (pre_a = r_in = r_out <= 32) then serves as Y
+  EpiArgs pre;
+  const double* pre_C = nullptr;
+  int pre_a = 0, pre_b = 0;
   int max_blocks = 256;
   // filled by the launcher
   int tr = 0;  // rows per tile

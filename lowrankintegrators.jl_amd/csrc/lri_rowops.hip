@@ -14,6 +14,8 @@
 // U*S / V*S' products (projector_splitting.jl:133,145,170,182), and the r x r
 // S-step algebra (projector_splitting.jl:141-142; unconventional.jl:154-156;
 // rank_adaptive_unconventional.jl:219-231).
+#include <type_traits>
+
 #include "lri_common.h"
 #include "lri_kernels.h"
 
@@ -235,15 +237,15 @@ __device__ int chol_inv(const double* G, double* R, double* Rinv, int k, unsigne
   return chol_inv_window<64>(G, R, Rinv, k, dropped, nonfinite);
 }
 
-// C = op(A) * op(B), k x k, column-major.
+// C = op(A) * op(B), k x k, column-major, on the FP64 matrix cores (16 x 16
+// blocks over the workgroup's waves).  C must not alias A or B; every thread
+// of the workgroup calls it.
 __device__ void small_gemm(const double* __restrict__ A, int tA, const double* __restrict__ B, int tB,
                            double* __restrict__ C, int k) {
-  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
-    int i = e % k, j = e / k;
-    double s = 0.0;
-    for (int l = 0; l < k; ++l) s += (tA ? A[l + i * k] : A[i + l * k]) * (tB ? B[j + l * k] : B[l + j * k]);
-    C[e] = s;
-  }
+  wave_gemm(
+      k, k, k, 1, [&](int, int i, int l) { return tA ? A[l + i * k] : A[i + l * k]; },
+      [&](int, int l, int j) { return tB ? B[j + l * k] : B[l + j * k]; },
+      [&](int, int i, int j, double v) { C[i + j * k] = v; });
 }
 
 // LDS doubles the epilogue needs beyond C (a x b).
@@ -253,6 +255,7 @@ __host__ __device__ inline int epi_scratch_doubles(int kind, int a, int b) {
     case EPI_CHOL1:
     case EPI_CHOL2:
     case EPI_CHOL3:
+      return 3 * a * a;  // R, R^{-1} and Rtot (C may carry a cross term beyond its first a columns)
     case EPI_BUG_N:
       return 2 * k * k;
     case EPI_RA_N:
@@ -266,8 +269,12 @@ __host__ __device__ inline int epi_scratch_doubles(int kind, int a, int b) {
 // epilogues run by the last workgroup on the reduced C (LDS, a x b)
 // ----------------------------------------------------------------------------
 
+// writer: this workgroup stores the results to the state (the epilogue of a
+// reduction runs in one workgroup; folded into the prologue of the next row
+// kernel, every workgroup computes the factors for itself and workgroup 0
+// stores them).  The Cholesky kinds use C's first a columns as the Gram matrix.
 template <int CK>
-__device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* scr) {
+__device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* scr, bool writer = true) {
   const int t = threadIdx.x;
   SmallState* st = E.small;
   switch (E.kind) {
@@ -294,12 +301,27 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
       double* R = scr;
       double* Rinv = scr + k * k;
       int nf = 0, nd = 0;
+      // CHOL2/3: the factor on the right of Rtot (R1, or the running Rtot) and
+      // the first pass's dropped count were written by an earlier launch on
+      // this stream; their loads are issued now, ahead of the factorisation
+      // (for CK <= 32; the CK = 64 variant has no registers to spare)
+      constexpr int NQ = CK <= 32 ? (CK * CK) / kThreads : 0;
+      const double* right = E.kind == EPI_CHOL2 ? st->R1 : st->Rtot;
+      double rv[NQ > 0 ? NQ : 1];
+      if (E.kind != EPI_CHOL1) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int e = t + kThreads * q;
+          rv[q] = e < k * k ? right[e] : 0.0;
+        }
+      }
+      const int nd_first = E.kind != EPI_CHOL1 ? st->ndropped : 0;
       // CholeskyQR2's second Gram matrix is I + E with |E| ~ eps cond^2: when
       // max|E| <= 1e-6 (and the first pass dropped nothing), R = I + F with
       // F = F1 + F2, F1 = upper(E) (diagonal halved), F2 = upper(-F1'F1)
       // (diagonal halved), and R^{-1} = I - F + F^2, all exact to O(|E|^3).
       bool pert = false;
-      if (E.kind != EPI_CHOL1 && st->ndropped == 0) {
+      if (E.kind != EPI_CHOL1 && nd_first == 0) {
         if (t == 0) s_big = 0;
         __syncthreads();
         double mx = 0.0;
@@ -314,16 +336,14 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
         pert = (s_big == 0);
       }
       if (pert) {
-        double* F1 = Rinv;  // scratch: F1 and F1' (in C), then F1'F1 into R
+        double* F1 = Rinv;  // scratch: F1, then F1'F1 into R
         for (int e = t; e < k * k; e += blockDim.x) {
           const int i = e % k, jj = e / k;
           const double v = C[e] - (i == jj ? 1.0 : 0.0);
           F1[e] = i < jj ? v : (i == jj ? 0.5 * v : 0.0);
         }
         __syncthreads();
-        for (int e = t; e < k * k; e += blockDim.x) C[e] = F1[(e / k) + (e % k) * k];
-        __syncthreads();
-        small_gemm(C, 0, F1, 0, R, k);  // R <- F1' F1
+        small_gemm(F1, 1, F1, 0, R, k);  // R <- F1' F1
         __syncthreads();
         for (int e = t; e < k * k; e += blockDim.x) {  // C <- F = F1 + upper(-F1'F1) (diag halved)
           const int i = e % k, jj = e / k;
@@ -345,29 +365,51 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
         nd = chol_inv<CK>(C, R, Rinv, k, dropped, &nf);
       }
       if (E.kind == EPI_CHOL1) {
-        for (int e = t; e < k * k; e += blockDim.x) {
-          st->R1[e] = R[e];
-          st->R1inv[e] = Rinv[e];
-        }
+        if (writer)
+          for (int e = t; e < k * k; e += blockDim.x) {
+            st->R1[e] = R[e];
+            st->R1inv[e] = Rinv[e];
+          }
       } else {
-        // Rtot = R2 * R1 (CHOL2) or R3 * Rtot (CHOL3); the factor on the right was
-        // written by an earlier launch on this stream; C is free now
-        const double* right = E.kind == EPI_CHOL2 ? st->R1 : st->Rtot;
-        for (int e = t; e < k * k; e += blockDim.x) {
-          st->R2inv[e] = Rinv[e];
-          C[e] = right[e];
+        // Rtot = R2 * R1 (CHOL2) or R3 * Rtot (CHOL3) into the third scratch
+        // matrix; the factor on the right was written by an earlier launch on
+        // this stream; C is free now
+        double* Rt = scr + 2 * k * k;
+        if (writer)
+          for (int e = t; e < k * k; e += blockDim.x) st->R2inv[e] = Rinv[e];
+        if (NQ > 0) {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const int e = t + kThreads * q;
+            if (e < k * k) C[e] = rv[q];
+          }
+        } else {
+          for (int e = t; e < k * k; e += blockDim.x) C[e] = right[e];
         }
         __syncthreads();
-        small_gemm(R, 0, C, 0, st->Rtot, k);
-        if (E.S) {
-          __syncthreads();
+        small_gemm(R, 0, C, 0, Rt, k);
+        __syncthreads();
+        if (writer)
           for (int e = t; e < k * k; e += blockDim.x) {
-            int i = e % k, jj = e / k;
-            E.S[i + jj * E.ld_s] = E.transpose ? st->Rtot[jj + i * k] : st->Rtot[e];
+            const int i = e % k, jj = e / k;
+            st->Rtot[e] = Rt[e];
+            if (E.S) E.S[i + jj * E.ld_s] = E.transpose ? Rt[jj + i * k] : Rt[e];
           }
+        if (E.stilde) {
+          // C = [Q1'Q1 | Q1'W]: U1 = Q1 R2^{-1}, so U1'W = R2^{-T} (Q1'W) and
+          // S~ = Rtot - U1'W; the first k columns of C are free again
+          small_gemm(Rinv, 1, C + k * k, 0, C, k);
+          __syncthreads();
+          if (writer)
+            for (int e = t; e < k * k; e += blockDim.x) {
+              const int i = e % k, jj = e / k;
+              const double v = Rt[e] - C[e];
+              if (E.stilde == 2) st->St[jj + i * k] = v;
+              else st->St[e] = v;
+            }
         }
       }
-      if (t == 0) {
+      if (t == 0 && writer) {
         if (nf) atomicOr(E.flags, FLAG_NONFINITE);
         if (nd && E.breakdown_is_error) atomicOr(E.flags, FLAG_BREAKDOWN);
         if (E.kind == EPI_CHOL1) st->ndropped = nd;
@@ -545,6 +587,7 @@ __global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* 
 struct RowsPlan {
   int tpr, rpb;                // threads per row, rows per tile
   int ri4, ro16, rw16;         // padded widths
+  int rg16;                    // the out' * out block columns of a gram_w reduction (else 0)
   int ldy, ldxt, ldot, ldwt;   // LDS leading dimensions (doubles; odd against bank conflicts)
   int nxy, nred;               // 16x16 blocks of X*Y per tile, of the reduction
   int ys_off, xt_off, ot_off, wt_off;
@@ -554,11 +597,13 @@ struct RowsPlan {
 // threads per row: enough for r_out in groups of 8, and more (fewer rows per
 // tile) while the LDS tile would exceed 96 KB (e.g. a narrow output reduced
 // against a wide W)
-__host__ __device__ inline RowsPlan rows_plan(int ro, int ri, int rw, bool hasX, bool hasW, bool reduce) {
+__host__ __device__ inline RowsPlan rows_plan(int ro, int ri, int rw, bool hasX, bool hasW, bool reduce,
+                                              bool gramw = false) {
   RowsPlan p;
   p.ri4 = (ri + 3) & ~3;
   p.ro16 = (ro + 15) & ~15;
   p.rw16 = hasW ? ((rw + 15) & ~15) : p.ro16;
+  p.rg16 = (hasW && gramw) ? p.ro16 : 0;
   p.ldy = p.ro16 + 1;
   p.ldxt = p.ri4 + 1;
   p.ldot = p.ro16 + 1;
@@ -582,11 +627,12 @@ __host__ __device__ inline RowsPlan rows_plan(int ro, int ri, int rw, bool hasX,
     tpr *= 2;
   }
   p.nxy = hasX ? (p.rpb / 16) * (p.ro16 / 16) : 0;
-  p.nred = reduce ? (p.ro16 / 16) * (p.rw16 / 16) : 0;
+  p.nred = reduce ? (p.ro16 / 16) * ((p.rg16 + p.rw16) / 16) : 0;
   return p;
 }
 
-template <typename PT, int NRB>
+// CK > 0: with the prologue of RowArgs::pre (Cholesky variant CK)
+template <typename PT, int NRB, int CK = 0>
 __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
   extern __shared__ double lds[];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -595,7 +641,8 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
   const bool hasW = A.W != nullptr;
   const bool reduce = A.reduce != 0;
   const int rw = hasW ? A.rw : ro;
-  const RowsPlan p = rows_plan(ro, ri, rw, hasX, hasW, reduce);
+  const RowsPlan p = rows_plan(ro, ri, rw, hasX, hasW, reduce, A.gram_w != 0);
+  const int rg = p.rg16 ? ro : 0;  // columns of C before the W part
   const int rpb = p.rpb, tpr = p.tpr;
   const int lrow = t % rpb, cg = t / rpb;
   double* Ys = lds + p.ys_off;
@@ -611,6 +658,23 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
   for (int b = 0; b < NRB; ++b) racc[b] = f64x4_t{0.0, 0.0, 0.0, 0.0};
 
   bool ys_staged = !hasX;
+  if constexpr (CK > 0) {
+    // the previous reduction's Cholesky epilogue, in the LDS beyond the tile
+    const int pa = A.pre_a, pb = A.pre_b;
+    double* Cp = lds + p.lds_doubles;
+    double* scr = Cp + pa * pb;
+    for (int e = t; e < pa * pb; e += kThreads) Cp[e] = A.pre_C[e];
+    __syncthreads();
+    run_epilogue<CK>(A.pre, Cp, pa, pb, scr, blockIdx.x == 0);
+    __syncthreads();
+    // Y = its R^{-1} (pa x pa), staged as below; the tile barrier publishes it
+    const double* Ri = scr + pa * pa;
+    for (int e = t; e < p.ri4 * p.ro16; e += kThreads) {
+      const int c = e % p.ro16, d = e / p.ro16;
+      Ys[d * p.ldy + c] = (c < ro && d < ri) ? A.xy_sign * Ri[d + c * pa] : 0.0;
+    }
+    ys_staged = true;
+  }
   const int64_t ntiles = (A.rows + rpb - 1) / rpb;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t row0 = tile * rpb;
@@ -764,16 +828,19 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
       }
     }
     if (reduce) {
-      // C += Ot' * Wt over the tile's rows (rows past the data are zero)
+      // C += Ot' * Wt over the tile's rows (rows past the data are zero); with
+      // gram_w the first rg16 block columns are Ot' * Ot
       const int nbi = p.ro16 / 16;
 #pragma unroll
       for (int b = 0; b < NRB; ++b) {
         const int blk = wave + 4 * b;
         if (blk < p.nred) {
           const int c0 = (blk % nbi) * 16, j0 = (blk / nbi) * 16;
+          const double* Bt = j0 < p.rg16 ? Ot + j0 : Wt + (j0 - p.rg16);
+          const int ldb = j0 < p.rg16 ? p.ldot : ldwt;
           for (int k0 = 0; k0 < rpb; k0 += 4) {
             const double a = Ot[(k0 + lk) * p.ldot + c0 + lr];
-            const double bb = Wt[(k0 + lk) * ldwt + j0 + lr];
+            const double bb = Bt[(k0 + lk) * ldb + lr];
             racc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, racc[b], 0, 0, 0);
           }
         }
@@ -782,17 +849,20 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
     if (use_tile) __syncthreads();  // the next tile overwrites the LDS tile
   }
   if (!reduce) return;
-  double* gp = A.gpart + (int64_t)blockIdx.x * ro * rw;
+  double* gp = A.gpart + (int64_t)blockIdx.x * ro * (rg + rw);
   const int nbi = p.ro16 / 16;
 #pragma unroll
   for (int b = 0; b < NRB; ++b) {
     const int blk = wave + 4 * b;
     if (blk < p.nred) {
       const int c0 = (blk % nbi) * 16, j0 = (blk / nbi) * 16;
+      const bool og = j0 < p.rg16;
+      const int jl = (og ? j0 : j0 - p.rg16) + lr;  // column within its part
+      const int cj = og ? jl : rg + jl;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int ci = c0 + 4 * u + lk, cj = j0 + lr;
-        if (ci < ro && cj < rw) gp[ci + cj * ro] = racc[b][u];
+        const int ci = c0 + 4 * u + lk;
+        if (ci < ro && jl < (og ? ro : rw)) gp[ci + cj * ro] = racc[b][u];
       }
     }
   }
@@ -805,17 +875,44 @@ hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
   if (A.r_out < 1 || A.r_out > kMaxAug || rw < 1 || rw > kMaxAug || A.r_in < 0 || A.r_in > kMaxAug)
     return hipErrorInvalidValue;
   if (A.X && A.r_in < 1) return hipErrorInvalidValue;
-  const RowsPlan p = rows_plan(A.r_out, A.r_in, rw, A.X != nullptr, hasW, A.reduce != 0);
+  if (A.gram_w && (!hasW || !A.reduce)) return hipErrorInvalidValue;
+  const RowsPlan p = rows_plan(A.r_out, A.r_in, rw, A.X != nullptr, hasW, A.reduce != 0, A.gram_w != 0);
   if (p.tpr > 16) return hipErrorInvalidValue;
   A.tr = p.rpb;
-  const size_t lds = sizeof(double) * p.lds_doubles;
-  const size_t epi_lds = sizeof(double) * ((size_t)A.r_out * rw + epi_scratch_doubles(A.epi.kind, A.r_out, rw));
+  const int rc = (A.gram_w ? A.r_out : 0) + rw;  // columns of C
+  if (A.epi.stilde && (A.epi.kind != EPI_CHOL2 || !A.gram_w || rw != A.r_out)) return hipErrorInvalidValue;
+  const bool pre = A.pre.kind != EPI_NONE;
+  size_t pre_doubles = 0;
+  if (pre) {
+    if ((A.pre.kind != EPI_CHOL1 && A.pre.kind != EPI_CHOL2) || !A.pre_C || A.pre_a < 1 || A.pre_a > 32 ||
+        A.pre_a != A.r_in || A.pre_a != A.r_out || !A.X || A.transY ||
+        (A.pre_b != A.pre_a && A.pre_b != 2 * A.pre_a) || (A.pre.stilde && A.pre_b != 2 * A.pre_a))
+      return hipErrorInvalidValue;
+    pre_doubles = (size_t)A.pre_a * A.pre_b + epi_scratch_doubles(A.pre.kind, A.pre_a, A.pre_b);
+  }
+  const size_t lds = sizeof(double) * (p.lds_doubles + pre_doubles);
+  const size_t epi_lds = sizeof(double) * ((size_t)A.r_out * rc + epi_scratch_doubles(A.epi.kind, A.r_out, rc));
   if (lds > 150 * 1024 || epi_lds > 150 * 1024) return hipErrorInvalidValue;  // room for static shared variables
   const int64_t ntiles = (A.rows + p.rpb - 1) / p.rpb;
   int grid = (int)(ntiles < A.max_blocks ? ntiles : A.max_blocks);
   if (grid < 1) grid = 1;
   const int nrb = (p.nred + 3) / 4;
-  if (nrb <= 1)
+  if (pre) {
+    if constexpr (std::is_same<PT, double>::value) {
+      if (nrb > 2) return hipErrorInvalidValue;
+      const int ck = A.pre_a <= 16 ? 16 : 32;
+      if (ck == 16 && nrb <= 1)
+        hipLaunchKernelGGL((k_rows<PT, 1, 16>), dim3(grid), dim3(kThreads), lds, st, A);
+      else if (ck == 16)
+        hipLaunchKernelGGL((k_rows<PT, 2, 16>), dim3(grid), dim3(kThreads), lds, st, A);
+      else if (nrb <= 1)
+        hipLaunchKernelGGL((k_rows<PT, 1, 32>), dim3(grid), dim3(kThreads), lds, st, A);
+      else
+        hipLaunchKernelGGL((k_rows<PT, 2, 32>), dim3(grid), dim3(kThreads), lds, st, A);
+    } else {
+      return hipErrorInvalidValue;
+    }
+  } else if (nrb <= 1)
     hipLaunchKernelGGL((k_rows<PT, 1>), dim3(grid), dim3(kThreads), lds, st, A);
   else if (nrb <= 2)
     hipLaunchKernelGGL((k_rows<PT, 2>), dim3(grid), dim3(kThreads), lds, st, A);
@@ -827,18 +924,18 @@ hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
     hipLaunchKernelGGL((k_rows<PT, 16>), dim3(grid), dim3(kThreads), lds, st, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !A.reduce) return e;
-  const int ab = A.r_out * rw;
+  const int ab = A.r_out * rc;
   double* Cr = A.epi.small->Cred;
   hipLaunchKernelGGL(k_reduce, dim3((ab + 63) / 64), dim3(kThreads), 0, st, (const double*)A.gpart, grid, ab, Cr);
   e = hipGetLastError();
-  if (e != hipSuccess || A.epi.kind == EPI_NONE) return e;
+  if (e != hipSuccess || A.epi.kind == EPI_NONE || A.defer_epi) return e;
   const bool chol = A.epi.kind == EPI_CHOL1 || A.epi.kind == EPI_CHOL2 || A.epi.kind == EPI_CHOL3;
   if (chol && A.r_out <= 16)
-    hipLaunchKernelGGL(k_epilogue<16>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rw);
+    hipLaunchKernelGGL(k_epilogue<16>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rc);
   else if (chol && A.r_out <= 32)
-    hipLaunchKernelGGL(k_epilogue<32>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rw);
+    hipLaunchKernelGGL(k_epilogue<32>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rc);
   else
-    hipLaunchKernelGGL(k_epilogue<64>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rw);
+    hipLaunchKernelGGL(k_epilogue<64>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rc);
   return hipGetLastError();
 }
 

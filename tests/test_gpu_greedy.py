@@ -100,11 +100,14 @@ def test_greedy_several_steps_fp64(lri):
     assert np.linalg.norm(U @ Z.T - Uo @ Zo.T) / np.linalg.norm(Uo @ Zo.T) <= FP64_TOL
 
 
-@pytest.mark.parametrize("dtype", ["bf16", "f32"])
-def test_greedy_mixed_precision(lri, dtype):
-    n, r = 1024, 32
-    say(f"case: Greedy mixed precision {dtype} n={n} r={r} 10 steps")
-    d = rotating_data(n, n, r, noise=1e-3, seed=4)
+@pytest.mark.parametrize("dtype,n,m", [("bf16", 1024, 1024), ("f32", 1024, 1024), ("bf16", 1000, 700),
+                                       ("bf16", 900, 520)])
+def test_greedy_mixed_precision(lri, dtype, n, m):
+    """Ragged n (1000, 900: row blocks that end inside a 64- or 128-row step of
+    pass T, and rows that are not a multiple of 8) next to the aligned case."""
+    r = 32
+    say(f"case: Greedy mixed precision {dtype} n={n} m={m} r={r} 10 steps")
+    d = rotating_data(n, m, r, noise=1e-3, seed=4)
     u0 = d.initial()
     snaps = [d(k / 199.0) for k in range(1, 11)]
     snaps = [bf16_round(a) for a in snaps] if dtype == "bf16" else [a.astype(np.float32).astype(np.float64)
