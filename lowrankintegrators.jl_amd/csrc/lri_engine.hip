@@ -562,6 +562,9 @@ struct lri_ctx {
   // unconventional_step! (unconventional.jl:133-157)
   void bug(int64_t kp, int64_t kn) {
     const int rr = (int)r;
+    const bool fold = fold_chol(rr);
+    EpiArgs c1;
+    c1.kind = EPI_CHOL1;
     ensure_packed(true);
     ensure_packed(false);
     const int sN = passN(kp, kn, rr, partN);  // dA V0
@@ -579,12 +582,13 @@ struct lri_ctx {
       A.out = K;
       A.ldo = n;
       A.reduce = 1;
-      A.epi.kind = EPI_CHOL1;
+      A.epi = c1;
+      A.defer_epi = fold;
       rows(A);
     });
     EpiArgs eM;
     eM.kind = EPI_BUG_M;  // M = U1' U0
-    cholqr_tail(K, K2, n, rr, U2, false, U, rr, eM, 0);
+    cholqr_tail(K, K2, n, rr, U2, false, U, rr, eM, 0, 2, fold ? &c1 : nullptr);
     with_parts(partT, [&](auto A) {
       A.rows = m;
       A.nsplit = sT;
@@ -599,14 +603,15 @@ struct lri_ctx {
       A.out = L;
       A.ldo = m;
       A.reduce = 1;
-      A.epi.kind = EPI_CHOL1;
+      A.epi = c1;
+      A.defer_epi = fold;
       rows(A);
     });
     EpiArgs eN;
     eN.kind = EPI_BUG_N;  // N = V1' V0, Sinit = M S0 N'
     eN.S = S;
     eN.ld_s = (int)rcap;
-    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0);
+    cholqr_tail(L, L2, m, rr, V2, true, V, rr, eN, 0, 2, fold ? &c1 : nullptr);
     const int sN2 = passN(kp, kn, rr, partN);  // dA V1
     with_parts(partN, [&](auto A) {  // S1 = Sinit + U1' dA V1
       A.rows = n;

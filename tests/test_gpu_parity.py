@@ -140,11 +140,32 @@ def test_products(lri, dtype, precision, tol, shape):
     assert eP <= tol and eQ <= tol
 
 
-def test_zero_increment_steps(lri):
-    u, A0, _ = one_step_case(8, 96, 80, 6)
+@pytest.mark.parametrize("r", [6, 24])
+def test_zero_increment_steps(lri, r):
+    u, A0, _ = one_step_case(8, 96, 80, r)
     for kind in ("primal", "dual", "bug"):
         U, S, V = run_native_step(lri, kind, u, [A0, A0.copy()])
         assert rel_err(U, S, V, u.U, u.S, u.V) <= 1e-13, kind
+
+
+@pytest.mark.parametrize("r", [6, 24, 40])
+def test_rank_deficient_state_reports_breakdown(lri, r):
+    """A state whose S has two zero singular values and a zero increment: K =
+    U*S has two zero columns, so the first Cholesky factor drops two pivots and
+    the KSL step reports EBREAKDOWN (r = 6 and 24 run the factorisations inside
+    the row kernels, r = 40 as separate epilogue launches)."""
+    say(f"case: rank-deficient state r={r}")
+    rng = np.random.default_rng(11)
+    u = random_factors(rng, 120, 100, r, decay=0.05)
+    sig = np.linalg.svd(u.S, compute_uv=False)
+    sig[2] = sig[r - 3] = 0.0
+    Us, _ = np.linalg.qr(rng.standard_normal((r, r)))
+    Vs, _ = np.linalg.qr(rng.standard_normal((r, r)))
+    u = type(u)(u.U, Us @ np.diag(sig) @ Vs.T, u.V)
+    A0 = rng.standard_normal((120, 100))
+    for kind in ("primal", "dual"):
+        with pytest.raises(lri.native.BreakdownError):
+            run_native_step(lri, kind, u, [A0, A0.copy()])
 
 
 # ------------------------------------------------------------------ FP64 parity, one step
