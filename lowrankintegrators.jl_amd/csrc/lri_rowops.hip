@@ -89,6 +89,7 @@ template <int K>
 __device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv,
                                  int k, unsigned char* dropped, int* nonfinite) {
   __shared__ int s_nd, s_nf;
+  __shared__ double s_row[64];  // R(p, .) of the current pivot, broadcast to the wave
   if (threadIdx.x < 64) {
     const int j = threadIdx.x;
     double md = 0.0;
@@ -133,13 +134,20 @@ __device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restric
         if (j == p) dropped[p] = keep ? 0 : 1;
         // rows t > p of column j: G -= R(p,t) R(p,j), B -= R(p,t) (L^{-1})(p,j)
         // (entries of g below the diagonal are never read, so they need no mask;
-        // lanes t >= k hold R(p,t) = 0)
+        // lanes t >= k hold R(p,t) = 0).  The multipliers go through LDS: the
+        // reads of all t are independent and land in vector registers, where
+        // readlane broadcasts would serialise on the scalar registers.
+        s_row[j] = rpj;
+        wave_lds_sync();
+        double rt[K];
+#pragma unroll
+        for (int t = p + 1; t < K; ++t) rt[t] = s_row[t];
 #pragma unroll
         for (int t = p + 1; t < K; ++t) {
-          const double rpt = readlane_f64(rpj, t);
-          g[t] = __builtin_fma(-rpt, rpj, g[t]);
-          w[t] = __builtin_fma(-rpt, xpj, w[t]);
+          g[t] = __builtin_fma(-rt[t], rpj, g[t]);
+          w[t] = __builtin_fma(-rt[t], xpj, w[t]);
         }
+        wave_lds_sync();
       }
     }
     if (j == 0) {
