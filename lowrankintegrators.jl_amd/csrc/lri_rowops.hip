@@ -1231,49 +1231,110 @@ hipError_t launch_small_abt(const double* A, const double* B, int k, double* C, 
 // r x r parts of the BUG adjoint
 // ---------------------------------------------------------------------------
 
-// X = R^{-T} for upper triangular R (r x r, ld r): thread j solves R x = e_j by
-// back substitution (column j of R^{-1}), stored as row j of R^{-T}.
-__device__ void inv_transpose_upper(const double* R, int r, double* X) {
-  for (int j = threadIdx.x; j < r; j += blockDim.x) {
-    double x[kMaxRank];
+// X1 = R1^{-T} and X2 = R2^{-T} for upper triangular R1, R2 (r x r, ld r, in
+// global memory): both are staged in LDS (ws: inv_ws_doubles(r)), thread j of
+// wave 0 (R1) or wave 1 (R2) solves R x = e_j by back substitution into column
+// j of an LDS block (ld r + 1 against bank conflicts), and the columns of
+// R^{-1} are stored as the rows of X.  Called by all threads (>= 128).
+__host__ __device__ inline int inv_ws_doubles(int r) { return 2 * r * r + 2 * r * (r + 1); }
+
+__device__ void inv_transpose_upper2(const double* R1, double* X1, const double* R2, double* X2, int r,
+                                     double* ws) {
+  const int t = threadIdx.x, rr = r * r, lx = r + 1;
+  double* R1s = ws;
+  double* R2s = ws + rr;
+  double* X1s = ws + 2 * rr;
+  double* X2s = X1s + r * lx;
+  for (int e = t; e < rr; e += blockDim.x) {
+    R1s[e] = R1[e];
+    R2s[e] = R2[e];
+  }
+  __syncthreads();
+  const int which = t >> 6, j = t & 63;
+  if (which < 2 && j < r) {
+    const double* Rs = which ? R2s : R1s;
+    double* x = (which ? X2s : X1s) + j * lx;
     for (int i = r - 1; i >= 0; --i) {
       double v = (i == j) ? 1.0 : 0.0;
-      for (int l = i + 1; l < r; ++l) v -= R[i + l * r] * x[l];
-      x[i] = (i > j) ? 0.0 : v / R[i + i * r];
+      for (int l = i + 1; l <= j; ++l) v -= Rs[i + l * r] * x[l];  // x[l] = 0 for l > j
+      x[i] = (i > j) ? 0.0 : v / Rs[i + i * r];
     }
-    for (int i = 0; i < r; ++i) X[j + i * r] = x[i];  // (R^{-1})_{ij} -> (R^{-T})_{ji}
   }
+  __syncthreads();
+  for (int e = t; e < rr; e += blockDim.x) {
+    const int i = e % r, jj = e / r;  // (R^{-1})_{i jj} -> (R^{-T})_{jj i}
+    X1[jj + i * r] = X1s[i + jj * lx];
+    X2[jj + i * r] = X2s[i + jj * lx];
+  }
+}
+
+// LDS of k_bug_adj_pre: r <= 32 stages Sb1, N, M, S0, T1, T2 (6 r^2) and runs
+// the products on the matrix cores; larger r keeps T1, T2 only (2 r^2).  Then
+// the inverses' workspace reuses it.
+__host__ __device__ inline int bug_adj_pre_ws(int r) {
+  const int prod = r <= 32 ? 6 * r * r : 2 * r * r;
+  return prod > inv_ws_doubles(r) ? prod : inv_ws_doubles(r);
 }
 
 __global__ __launch_bounds__(kThreads) void k_bug_adj_pre(BugAdjSmall* A, const double* M, const double* N,
                                                           const double* S0, int lds, int r) {
-  __shared__ double T1[kMaxRank * kMaxRank], T2[kMaxRank * kMaxRank];
+  extern __shared__ double ws[];
   const int t = threadIdx.x, rr = r * r;
-  for (int e = t; e < rr; e += blockDim.x) {  // T1 = Sb1 N, T2 = M S0
-    const int i = e % r, j = e / r;
-    double a = 0.0, b = 0.0;
-    for (int l = 0; l < r; ++l) {
-      a += A->Sb1[i + l * r] * N[l + j * r];
-      b += M[i + l * r] * S0[l + j * lds];
+  if (r <= 32) {
+    double* sSb = ws;
+    double* sN = ws + rr;
+    double* sM = ws + 2 * rr;
+    double* sS = ws + 3 * rr;
+    double* T1 = ws + 4 * rr;
+    double* T2 = ws + 5 * rr;
+    for (int e = t; e < rr; e += blockDim.x) {
+      sSb[e] = A->Sb1[e];
+      sN[e] = N[e];
+      sM[e] = M[e];
+      sS[e] = S0[e % r + (e / r) * lds];
     }
-    T1[e] = a;
-    T2[e] = b;
+    __syncthreads();
+    // T1 = Sb1 N, T2 = M S0
+    wave_gemm(
+        r, r, r, 2, [&](int b, int i, int l) { return b == 0 ? sSb[i + l * r] : sM[i + l * r]; },
+        [&](int b, int l, int jj) { return b == 0 ? sN[l + jj * r] : sS[l + jj * r]; },
+        [&](int b, int i, int jj, double v) { (b == 0 ? T1 : T2)[i + jj * r] = v; });
+    __syncthreads();
+    // Mb = T1 S0', Nb = Sb1' T2, Sb0 = M' T1
+    wave_gemm(
+        r, r, r, 3,
+        [&](int b, int i, int l) { return b == 0 ? T1[i + l * r] : (b == 1 ? sSb[l + i * r] : sM[l + i * r]); },
+        [&](int b, int l, int jj) { return b == 0 ? sS[jj + l * r] : (b == 1 ? T2[l + jj * r] : T1[l + jj * r]); },
+        [&](int b, int i, int jj, double v) { (b == 0 ? A->Mb : (b == 1 ? A->Nb : A->Sb0))[i + jj * r] = v; });
+  } else {
+    double* T1 = ws;
+    double* T2 = ws + rr;
+    for (int e = t; e < rr; e += blockDim.x) {  // T1 = Sb1 N, T2 = M S0
+      const int i = e % r, j = e / r;
+      double a = 0.0, b = 0.0;
+      for (int l = 0; l < r; ++l) {
+        a += A->Sb1[i + l * r] * N[l + j * r];
+        b += M[i + l * r] * S0[l + j * lds];
+      }
+      T1[e] = a;
+      T2[e] = b;
+    }
+    __syncthreads();
+    for (int e = t; e < rr; e += blockDim.x) {
+      const int i = e % r, j = e / r;
+      double mb = 0.0, nb = 0.0, sb = 0.0;
+      for (int l = 0; l < r; ++l) {
+        mb += T1[i + l * r] * S0[j + l * lds];    // (Sb1 N) S0'
+        nb += A->Sb1[l + i * r] * T2[l + j * r];  // Sb1' (M S0)
+        sb += M[l + i * r] * T1[l + j * r];       // M' (Sb1 N)
+      }
+      A->Mb[e] = mb;
+      A->Nb[e] = nb;
+      A->Sb0[e] = sb;
+    }
   }
   __syncthreads();
-  for (int e = t; e < rr; e += blockDim.x) {
-    const int i = e % r, j = e / r;
-    double mb = 0.0, nb = 0.0, sb = 0.0;
-    for (int l = 0; l < r; ++l) {
-      mb += T1[i + l * r] * S0[j + l * lds];    // (Sb1 N) S0'
-      nb += A->Sb1[l + i * r] * T2[l + j * r];  // Sb1' (M S0)
-      sb += M[l + i * r] * T1[l + j * r];       // M' (Sb1 N)
-    }
-    A->Mb[e] = mb;
-    A->Nb[e] = nb;
-    A->Sb0[e] = sb;
-  }
-  inv_transpose_upper(A->RK, r, A->RKinvT);
-  inv_transpose_upper(A->RL, r, A->RLinvT);
+  inv_transpose_upper2(A->RK, A->RKinvT, A->RL, A->RLinvT, r, ws);
 }
 
 __global__ __launch_bounds__(kThreads) void k_bug_adj_post(BugAdjSmall* A, int r) {
@@ -1284,10 +1345,10 @@ __global__ __launch_bounds__(kThreads) void k_bug_adj_post(BugAdjSmall* A, int r
 }
 
 __global__ __launch_bounds__(kThreads) void k_ksl_adj_small(BugAdjSmall* A, int r, int stage) {
+  extern __shared__ double ws[];
   const int rr = r * r;
   if (stage == 0) {
-    inv_transpose_upper(A->RK, r, A->RKinvT);
-    inv_transpose_upper(A->RL, r, A->RLinvT);
+    inv_transpose_upper2(A->RK, A->RKinvT, A->RL, A->RLinvT, r, ws);
     for (int e = threadIdx.x; e < rr; e += blockDim.x) {  // Mb = RL Sb1
       const int i = e % r, j = e / r;
       double v = 0.0;
@@ -1308,13 +1369,15 @@ __global__ __launch_bounds__(kThreads) void k_ksl_adj_small(BugAdjSmall* A, int 
 
 hipError_t launch_ksl_adj_small(BugAdjSmall* A, int r, int stage, hipStream_t st) {
   if (r < 1 || r > kMaxRank || stage < 0 || stage > 2) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_ksl_adj_small, dim3(1), dim3(kThreads), 0, st, A, r, stage);
+  const size_t ws = stage == 0 ? sizeof(double) * inv_ws_doubles(r) : 0;
+  hipLaunchKernelGGL(k_ksl_adj_small, dim3(1), dim3(kThreads), ws, st, A, r, stage);
   return hipGetLastError();
 }
 
 // Reverse of the rank-adaptive step, small parts; one workgroup per stage (see lri_kernels.h).
 __global__ __launch_bounds__(kThreads) void k_ra_adj_small(RaAdjSmall* A, const SmallState* st, const double* S0,
                                                            int lds, int r, int r1, int stage) {
+  extern __shared__ double ws[];  // stage 0: the inverses' workspace
   const int t = threadIdx.x, nt = blockDim.x, k2 = 2 * r;
   const double* Us = st->Ush;
   const double* Vs = st->Vsh;
@@ -1332,8 +1395,8 @@ __global__ __launch_bounds__(kThreads) void k_ra_adj_small(RaAdjSmall* A, const 
       A->Jm[e] = a;
       A->Km[e] = b;
     }
-    inv_transpose_upper(A->RU, r, A->RUinvT);
-    inv_transpose_upper(A->RV, r, A->RVinvT);
+    __syncthreads();
+    inv_transpose_upper2(A->RU, A->RUinvT, A->RV, A->RVinvT, r, ws);
   } else if (stage == 1) {
     for (int e = t; e < k2 * k2; e += nt) {
       const int i = e % k2, j = e / k2;
@@ -1412,14 +1475,16 @@ hipError_t launch_ra_adj_small(RaAdjSmall* A, const SmallState* st, const double
                                int stage, hipStream_t s) {
   if (r < 1 || r > kMaxRank || r1 < 1 || r1 > 2 * r || r1 > kMaxRank || stage < 0 || stage > 4)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_ra_adj_small, dim3(1), dim3(kThreads), 0, s, A, st, S0, lds, r, r1, stage);
+  const size_t ws = stage == 0 ? sizeof(double) * inv_ws_doubles(r) : 0;
+  hipLaunchKernelGGL(k_ra_adj_small, dim3(1), dim3(kThreads), ws, s, A, st, S0, lds, r, r1, stage);
   return hipGetLastError();
 }
 
 hipError_t launch_bug_adj_pre(BugAdjSmall* A, const double* M, const double* N, const double* S0, int lds, int r,
                               hipStream_t st) {
   if (r < 1 || r > kMaxRank) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_bug_adj_pre, dim3(1), dim3(kThreads), 0, st, A, M, N, S0, lds, r);
+  const size_t ws = sizeof(double) * (size_t)bug_adj_pre_ws(r);
+  hipLaunchKernelGGL(k_bug_adj_pre, dim3(1), dim3(kThreads), ws, st, A, M, N, S0, lds, r);
   return hipGetLastError();
 }
 
