@@ -582,7 +582,8 @@ __global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* 
 // the row kernel
 //   out[i, c] = (parts ? sum_s part[s][c][i] : 0) + base_sign * op(base)[i, c]
 //             + xy_sign * sum_d X[i, d] * op(Y)(d, c)
-// and optionally the reduction C = out' * W (W == nullptr: out' * out).
+// and optionally the reduction C = out' * W (W == nullptr: out' * out;
+// gram_w: out' * [out | W]).
 // A tile has rpb rows.  For loads and stores thread t handles row t % rpb and
 // the columns 8*(t / rpb) + 0..7, so every wave reads and writes 64
 // consecutive rows of one column.  The tile then goes through LDS (Xt: the X
@@ -590,6 +591,9 @@ __global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* 
 // matrix cores (16x16x4): X*Y adds into Ot in place, and the reduction
 // accumulates 16x16 blocks of C in registers across tiles (block b belongs to
 // wave b % 4), written once per workgroup to gpart[blockIdx.x].
+// With a prologue (CK > 0), every workgroup first runs the Cholesky epilogue
+// of the previous kernel's reduction on its own LDS copy of C, and Y is that
+// epilogue's R^{-1}.
 // ----------------------------------------------------------------------------
 
 struct RowsPlan {
