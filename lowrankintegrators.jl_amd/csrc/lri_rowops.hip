@@ -71,30 +71,34 @@ __device__ __forceinline__ int chol_scale_exp(double md) {
 }
 
 // Upper Cholesky G = R'R (k <= K <= 32) and R^{-1}, computed by wave 0 alone
-// with every column in registers.  Lane j holds column j of the trailing
-// matrix, g[t] = G(t, j), and column j of B = L^{-1} (L = R', B starts as I),
-// w[t]; the pivot loop is unrolled over K, so every register index is static
-// and step p touches only rows t > p.  One right-looking sweep yields R and
-// R^{-T} together, with no back substitution.  The multipliers R(p, t) are
-// broadcast with readlane from lane t (a constant lane per unrolled step), so a
-// step needs no LDS round trip; the pivot of step p+1 is taken as soon as
-// lane p+1 has applied step p to its own diagonal.  G is read by rows (it is
-// symmetric), which keeps the LDS reads free of bank conflicts.  A pivot <=
-// tau = 10 eps max_i G(i,i) marks column p as numerically dependent:
-// dropped[p] = 1, R(p,p) = 1, row p of R is zero and row and column p of
-// R^{-1} are zero (so Q gets a zero column there).  R and Rinv are written to
-// LDS (column-major, ld k).  Returns the number of dropped columns; sets
-// *nonfinite on NaN/Inf.  Called by all threads.
+// with every column in registers.  Lane j < 32 holds column j of the trailing
+// matrix, a[t] = G(t, j), and lane 32 + j column j of B = L^{-1} (L = R', B
+// starts as I), a[t] = B(t, j); the pivot loop is unrolled over K, so every
+// register index is static and step p touches only rows t > p.  One
+// right-looking sweep yields R and R^{-T} together, with no back substitution,
+// and one FMA per row updates both halves: a[t] -= R(p, t) m, with m = R(p, j)
+// in the first half and m = B(p, j) in the second.  The multipliers R(p, t)
+// go through LDS: one store per lane, then all the reads of the step issue
+// back to back into vector registers (readlane broadcasts serialise on the
+// scalar registers, of which these kernels are short).  The pivot of step p+1
+// is taken as soon as lane p+1 has applied step p to its own diagonal.  G is
+// read by rows (it is symmetric), which keeps the LDS reads free of bank
+// conflicts.  A pivot <= tau = 10 eps max_i G(i,i) marks column p as
+// numerically dependent: dropped[p] = 1, R(p,p) = 1, row p of R is zero and
+// row and column p of R^{-1} are zero (so Q gets a zero column there).  R and
+// Rinv are written to LDS (column-major, ld k).  Returns the number of dropped
+// columns; sets *nonfinite on NaN/Inf.  Called by all threads.
 template <int K>
 __device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv,
                                  int k, unsigned char* dropped, int* nonfinite) {
+  static_assert(K <= 32, "one wave: K columns of G and K of L^{-1}");
   __shared__ int s_nd, s_nf;
-  __shared__ double s_row[64];  // R(p, .) of the current pivot, broadcast to the wave
+  __shared__ double s_row[64];  // the multipliers of the current pivot, broadcast to the wave
   if (threadIdx.x < 64) {
-    const int j = threadIdx.x;
+    const int l = threadIdx.x, h = l >> 5, j = l & 31;
     double md = 0.0;
     int nf = 0;
-    if (j < k) {
+    if (h == 0 && j < k) {
       const double dj = G[j + j * k];
       nf = !isfinite(dj);
       md = dj;
@@ -105,15 +109,17 @@ __device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restric
     }
     const int e2 = chol_scale_exp(md), eh = e2 / 2;
     const double tau = 10.0 * kEps * ldexp(md, -e2);
-    double g[K], w[K];
+    double a[K];
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-      g[t] = (t < k && j < k && t <= j) ? ldexp(G[j + t * k], -e2) : 0.0;
-      w[t] = (t == j && j < k) ? 1.0 : 0.0;
+      if (h == 0)
+        a[t] = (t < k && j < k && t <= j) ? ldexp(G[j + t * k], -e2) : 0.0;
+      else
+        a[t] = (t == j && j < k) ? 1.0 : 0.0;
     }
     int nd = 0;
     bool mydrop = false;
-    double dnext = readlane_f64(g[0], 0);
+    double dnext = readlane_f64(a[0], 0);
 #pragma unroll
     for (int p = 0; p < K; ++p) {
       if (p < k) {
@@ -124,33 +130,30 @@ __device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restric
         const double inv = keep ? rsqrt_nr(d) : 1.0;
         nd += keep ? 0 : 1;
         mydrop = mydrop || (j == p && !keep);
-        const double rpj = (keep && j > p && j < k) ? g[p] * inv : 0.0;  // R(p, j)
-        const double xpj = (keep && j < k) ? w[p] * inv : 0.0;           // (L^{-1})(p, j) = R^{-1}(j, p)
-        if (p + 1 < K) dnext = readlane_f64(__builtin_fma(-rpj, rpj, g[p + 1]), p + 1);
+        // first half: R(p, j); second half: (L^{-1})(p, j) = R^{-1}(j, p)
+        const double m = (keep && j < k && (h == 1 || j > p)) ? a[p] * inv : 0.0;
+        if (p + 1 < K) dnext = readlane_f64(__builtin_fma(-m, m, a[p + 1]), p + 1);
         if (j < k) {
-          R[p + j * k] = (j == p) ? (keep ? ldexp(d * inv, eh) : 1.0) : (j > p ? ldexp(rpj, eh) : 0.0);
-          Rinv[j + p * k] = mydrop ? 0.0 : ldexp(xpj, -eh);
+          if (h == 0)
+            R[p + j * k] = (j == p) ? (keep ? ldexp(d * inv, eh) : 1.0) : (j > p ? ldexp(m, eh) : 0.0);
+          else
+            Rinv[j + p * k] = mydrop ? 0.0 : ldexp(m, -eh);
         }
-        if (j == p) dropped[p] = keep ? 0 : 1;
-        // rows t > p of column j: G -= R(p,t) R(p,j), B -= R(p,t) (L^{-1})(p,j)
-        // (entries of g below the diagonal are never read, so they need no mask;
-        // lanes t >= k hold R(p,t) = 0).  The multipliers go through LDS: the
-        // reads of all t are independent and land in vector registers, where
-        // readlane broadcasts would serialise on the scalar registers.
-        s_row[j] = rpj;
+        if (h == 0 && j == p) dropped[p] = keep ? 0 : 1;
+        // rows t > p: G(t, j) -= R(p,t) R(p,j), B(t, j) -= R(p,t) (L^{-1})(p,j)
+        // (entries of the first half below the diagonal are never read, so they
+        // need no mask; lanes t >= k hold R(p,t) = 0)
+        s_row[l] = m;
         wave_lds_sync();
         double rt[K];
 #pragma unroll
         for (int t = p + 1; t < K; ++t) rt[t] = s_row[t];
 #pragma unroll
-        for (int t = p + 1; t < K; ++t) {
-          g[t] = __builtin_fma(-rt[t], rpj, g[t]);
-          w[t] = __builtin_fma(-rt[t], xpj, w[t]);
-        }
+        for (int t = p + 1; t < K; ++t) a[t] = __builtin_fma(-rt[t], m, a[t]);
         wave_lds_sync();
       }
     }
-    if (j == 0) {
+    if (l == 0) {
       s_nd = nd;
       s_nf = nf;
     }
