@@ -56,13 +56,34 @@ __device__ __forceinline__ void wave_gemm(int M, int N, int K, int nbatch, FA A,
     const int bt = tile / per, rem = tile - bt * per;
     const int m0 = (rem % tm) << 4, n0 = (rem / tm) << 4;
     const int mm = m0 + lr, nn = n0 + lr;
-    f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += 4) {
+    const int mc = mm < M ? mm : M - 1, nc = nn < N ? nn : N - 1;  // in-range rows for the loads
+    // two accumulators (even and odd k-steps): a dependent FP64 matrix-core
+    // product waits ~190 cycles for the previous one, so this halves the chain
+    f64x4_t acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+    int k0 = 0;
+    // four k-steps per round: the eight operand loads are unconditional (at
+    // clamped indices, zeroed by selects) and issue together
+    for (; k0 + 16 <= K; k0 += 16) {
+      double a[4], b[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        a[s] = A(bt, mc, k0 + 4 * s + lk);
+        b[s] = B(bt, k0 + 4 * s + lk, nc);
+      }
+#pragma unroll
+      for (int s = 0; s < 4; s += 2) {
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(mm < M ? a[s] : 0.0, nn < N ? b[s] : 0.0, acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(mm < M ? a[s + 1] : 0.0, nn < N ? b[s + 1] : 0.0, acc2, 0, 0,
+                                                    0);
+      }
+    }
+    for (; k0 < K; k0 += 4) {
       const int kk = k0 + lk;
       const double a = (mm < M && kk < K) ? A(bt, mm, kk) : 0.0;
       const double b = (kk < K && nn < N) ? B(bt, kk, nn) : 0.0;
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
+    acc += acc2;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int mo = m0 + 4 * u + lk;

@@ -253,9 +253,11 @@ __device__ int chol_inv(const double* G, double* R, double* Rinv, int k, unsigne
 // of the workgroup calls it.
 __device__ void small_gemm(const double* __restrict__ A, int tA, const double* __restrict__ B, int tB,
                            double* __restrict__ C, int k) {
+  // (the transposes select the index, not between two loads: a select of
+  // loads compiles to branches)
   wave_gemm(
-      k, k, k, 1, [&](int, int i, int l) { return tA ? A[l + i * k] : A[i + l * k]; },
-      [&](int, int l, int j) { return tB ? B[j + l * k] : B[l + j * k]; },
+      k, k, k, 1, [&](int, int i, int l) { return A[tA ? l + i * k : i + l * k]; },
+      [&](int, int l, int j) { return B[tB ? j + l * k : l + j * k]; },
       [&](int, int i, int j, double v) { C[i + j * k] = v; });
 }
 
