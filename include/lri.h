@@ -138,8 +138,11 @@ int lri_step_ksl_strang(lri_ctx* ctx, int64_t k_prev, int64_t k_mid, int64_t k_n
 int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next);
 
 /* Rank-adaptive BUG step (rank_adaptive_unconventional.jl:171-233).  The new
- * rank min(rmax, truncate_to_tolerance(sigma, tol)) -- further clipped to the
- * context capacity -- is written to *r_new (synchronising). */
+ * rank min(rmax, truncate_to_tolerance(sigma, tol)) is written to *r_new
+ * (synchronising).  When that rank exceeds the context capacity rcap the step
+ * returns LRI_ENOTSUP and leaves the state at the step's start (the reference's
+ * default rmax is 2^62, rank_adaptive_unconventional.jl:18-23: a rank is never
+ * clipped to the capacity silently). */
 int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax,
                     int64_t* r_new);
 
@@ -238,7 +241,8 @@ int lri_step_de_ksl(lri_ctx* ctx, int order, double dt, int64_t substeps);
 int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps);
 
 /* One rank-adaptive BUG step of the matrix ODE (rank_adaptive_unconventional.jl:171-233);
- * truncation and *r_new as lri_step_bug_ra (synchronising). */
+ * truncation, *r_new and LRI_ENOTSUP above the capacity as lri_step_bug_ra
+ * (synchronising). */
 int lri_step_de_bug_ra(lri_ctx* ctx, double dt, int64_t substeps, double tol, int64_t rmax, int64_t* r_new);
 
 /* Reverse (adjoint) of lri_step_de_bug (unconventional.jl:133-164 with the
@@ -282,13 +286,20 @@ int lri_batch_context(lri_batch* batch, int64_t i, lri_ctx** ctx);
 /* lri_step_de_bug on every member (substeps[i] RK4 sub-steps for member i). */
 int lri_batch_step_de_bug(lri_batch* batch, double dt, const int64_t* substeps);
 /* lri_step_de_bug_ra on every member; r_new[i] receives member i's new rank.
- * All members' steps are queued up to their SVD before the first rank is read. */
+ * All members' steps are queued up to their SVD before the first rank is read.
+ * Member i truncates at min(rmax, rcap[i]): its capacity is its own rmax. */
 int lri_batch_step_de_bug_ra(lri_batch* batch, double dt, const int64_t* substeps, double tol, int64_t rmax,
                              int64_t* r_new);
 /* Rank-adaptive batch steps run each member's step up to its SVD as a HIP
  * graph captured for the member's current rank (default on = 1), so one
  * member-step costs one graph launch on the host; 0: direct launches. */
 int lri_batch_use_graphs(lri_batch* batch, int on);
+/* The status of member i from the last batch call (LRI_OK or the LRI_E* that
+ * member reported).  A batch step runs every member even when one fails: a
+ * member whose step failed keeps its state at the step's start (a rank-adaptive
+ * member that needs a rank above its capacity reports LRI_ENOTSUP), every other
+ * member has advanced. */
+int lri_batch_member_status(const lri_batch* batch, int64_t i, int* status);
 int lri_batch_synchronize(lri_batch* batch);
 
 /* Wait for all queued work; returns a latched device-side status, if any. */

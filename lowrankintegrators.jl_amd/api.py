@@ -435,6 +435,9 @@ class DLRIntegrator:
         self.save_everystep = save_everystep
         self._array_data = isinstance(prob, MatrixDataProblem) and not callable(prob.y)
         self._slot_of_t = 0  # function data: ring slot (of 3) holding y(t)
+        # array data: the snapshot index of yprev; alg_cache seeds it with y[1] whatever t0 is
+        # (projector_splitting.jl:91, unconventional.jl:91, rank_adaptive_unconventional.jl:113)
+        self._kprev = 0
         self.rank = u.rank
         self.twofactor = isinstance(u, TwoFactorRepresentation)
 
@@ -594,10 +597,13 @@ def step(integrator: DLRIntegrator, alg=None, dt=None) -> None:
             raise TypeError("array-form data has no half-step snapshot: Strang needs function-form data")
         if not (_is_int(t) and _is_int(dt)):
             raise TypeError("array-form data needs integer t and dt")
-        kp, kn = t - 1, t + dt - 1
-        if kn >= len(integrator.prob.y):
-            raise IndexError(f"snapshot index {kn + 1} beyond the data (length {len(integrator.prob.y)})")
+        # update_data! reads y[t + dt]; dy = y[t + dt] - yprev, then yprev = y[t + dt]
+        # (projector_splitting.jl:117-121, data_integrator.jl:26-28)
+        kp, kn = integrator._kprev, t + dt - 1
+        if not 0 <= kn < len(integrator.prob.y):
+            raise IndexError(f"snapshot index {kn + 1} outside the data (length {len(integrator.prob.y)})")
         _dispatch(integrator, alg, kp, kn, None)
+        integrator._kprev = kn
     else:
         cur = integrator._slot_of_t
         free = [s for s in range(3) if s != cur]
@@ -794,11 +800,15 @@ def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, preci
     ctx = integ.ctx
     try:
         t0, tf = prob.tspan
-        T = tf - t0  # steps k -> k + 1 over snapshot indices t0 - 1 .. tf - 1
+        T = tf - t0  # steps over snapshot indices t0 .. tf - 1 (0-based)
         k0 = t0 - 1
+
+        def kp(s_):  # yprev starts at y[1] whatever t0 is (as in step)
+            return 0 if s_ == 0 else k0 + s_
+
         cps = {0: ctx.get_state()}
         for s_ in range(T):
-            fwd(ctx, k0 + s_, k0 + s_ + 1)
+            fwd(ctx, kp(s_), k0 + s_ + 1)
             if (s_ + 1) % checkpoint == 0 and s_ + 1 < T:
                 cps[s_ + 1] = ctx.get_state()
         value, Ub, Sb, Vb = loss_grad(*ctx.get_state())
@@ -807,11 +817,11 @@ def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, preci
             states = [cps[c0]]
             ctx.set_state(*cps[c0])
             for s_ in range(c0, c1 - 1):
-                fwd(ctx, k0 + s_, k0 + s_ + 1)
+                fwd(ctx, kp(s_), k0 + s_ + 1)
                 states.append(ctx.get_state())
             for s_ in range(c1 - 1, c0 - 1, -1):
                 ctx.set_state(*states[s_ - c0])
-                Ub, Sb, Vb = rev(ctx, k0 + s_, k0 + s_ + 1, Ub, Sb, Vb)
+                Ub, Sb, Vb = rev(ctx, kp(s_), k0 + s_ + 1, Ub, Sb, Vb)
         return value, (Ub, Sb, Vb)
     finally:
         ctx.close()

@@ -1625,7 +1625,10 @@ struct lri_ctx {
     sv.sigma = small->sigma;
     sv.r_out = &small->r_new;
     sv.tol = tol;
-    sv.rmax = std::min<int64_t>(rmax, rcap);
+    // clipped to the caller's rmax and to min(n, m) (singular values past it
+    // are those of dependent directions), not to the capacity: ra_rotate
+    // reports a rank above the capacity instead of truncating to it
+    sv.rmax = std::min<int64_t>(rmax, std::min(n, m));
     sv.work = svd_work;
     timed(LRI_KERNEL_SMALL, [&] { return launch_jacobi_svd(sv, stream); });
     if (!defer_rank_copy) HIPCHECK(hipMemcpyAsync(r_host, &small->r_new, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -1668,11 +1671,26 @@ struct lri_ctx {
     ra_k2 = 2 * (int)r;
     HIPCHECK(hipMemcpyAsync(r_host, &small->r_new, sizeof(int), hipMemcpyDeviceToHost, stream));
   }
+  // Captured graphs bake in the operator's buffers and its control flow
+  // (analytic eigenbasis or not, the quadratic term): a new operator drops them.
+  void drop_graphs() {
+    HIPCHECK(hipStreamSynchronize(stream));
+    for (auto& g : ra_graphs) (void)hipGraphExecDestroy(g.exec);
+    ra_graphs.clear();
+  }
   int64_t ra_rotate() {
     const int k2 = ra_k2;
     HIPCHECK(hipStreamSynchronize(stream));
     const int r1 = *r_host;
-    REQUIRE(r1 >= 1 && r1 <= rcap, "internal: rank-adaptive step returned an invalid rank");
+    if (r1 > rcap) {
+      // the state is left at the step's start; the panels hold the augmented bases
+      pkN_mode = pkT_mode = -1;
+      throw Failure{LRI_ENOTSUP, "the rank-adaptive step needs rank " + std::to_string(r1) +
+                                     " (truncation at tol within rmax), above the context capacity " +
+                                     std::to_string(rcap) + " (create the context with a larger rcap, at most " +
+                                     std::to_string(LRI_MAX_RANK) + ")"};
+    }
+    REQUIRE(r1 >= 1, "internal: rank-adaptive step returned an invalid rank");
     // U = Uhat Ush[:, :r1], V = Vhat Vsh[:, :r1], S = diag(sigma[:r1])
     for (int side = 0; side < 2; ++side) {
       RowArgs<double> A;
@@ -2936,21 +2954,28 @@ struct lri_batch {
   std::vector<lri_ctx*> ctx;
   std::string err;
   bool graphs = true;  // rank-adaptive members step through captured graphs
+  std::vector<int> status;  // each member's status from the last batch call
 };
 
 namespace {
-// run f(i, ctx) on every member; the first failure is reported as "problem i: ..."
+// run f(i, ctx) on every member whose status is still OK (all members when
+// `fresh`), keep each member's status, and report the first failure as
+// "problem i: ..." (a failing member does not stop the others)
 template <class F>
-int batch_each(lri_batch* b, F&& f) {
+int batch_each(lri_batch* b, F&& f, bool fresh = true) {
   if (!b) return LRI_EINVAL;
+  if (fresh) b->status.assign(b->ctx.size(), LRI_OK);
+  int first = LRI_OK;
   for (size_t i = 0; i < b->ctx.size(); ++i) {
+    if (b->status[i] != LRI_OK) continue;
     const int st = f((int64_t)i, b->ctx[i]);
-    if (st != LRI_OK) {
+    b->status[i] = st;
+    if (st != LRI_OK && first == LRI_OK) {
       b->err = "problem " + std::to_string(i) + ": " + lri_last_error(b->ctx[i]);
-      return st;
+      first = st;
     }
   }
-  return LRI_OK;
+  return first;
 }
 }  // namespace
 
@@ -3333,7 +3358,7 @@ int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* 
     REQUIRE(ctx->n <= LRI_DE_MAX_ROWS && ctx->m <= LRI_DE_MAX_ROWS, "matrix ODE needs n, m <= LRI_DE_MAX_ROWS");
     const size_t d = sizeof(double);
     const int64_t n = ctx->n, m = ctx->m;
-    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graphs();
     auto alloc = [&](double** p, size_t count) {
       if (!*p) HIPCHECK(hipMalloc((void**)p, d * count));
       HIPCHECK(hipMemset(*p, 0, d * count));
@@ -3375,7 +3400,7 @@ int lri_set_operator_quadratic(lri_ctx* ctx, const double* dl, const double* dd,
     REQUIRE(!periodic || ctx->n >= 3, "a periodic operator needs n >= 3");
     const size_t d = sizeof(double);
     const int64_t n = ctx->n;
-    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->drop_graphs();
     auto alloc = [&](double** p, size_t count) {
       if (!*p) HIPCHECK(hipMalloc((void**)p, d * count));
       HIPCHECK(hipMemset(*p, 0, d * count));
@@ -3553,32 +3578,46 @@ int lri_batch_step_de_bug_ra(lri_batch* b, double dt, const int64_t* substeps, d
       REQUIRE(!c->op_quad || 2 * c->r <= kQuadMaxRank,
               "the rank-adaptive step with the quadratic operator needs 2 r <= LRI_QUAD_MAX_RANK");
       DeScope scope(c);
+      // a member's capacity is its own rmax (the reference gives each problem its own
+      // algorithm, rank_adaptive_unconventional.jl:18-23)
+      const int64_t rm = std::min<int64_t>(rmax, c->rcap);
       if (b->graphs && !c->timing)
-        c->de_bug_ra_queue_graph(dt, substeps[i], tol, rmax);
+        c->de_bug_ra_queue_graph(dt, substeps[i], tol, rm);
       else
-        c->de_bug_ra(dt, substeps[i], tol, rmax, false);
+        c->de_bug_ra(dt, substeps[i], tol, rm, false);
       return LRI_OK;
     });
   });
-  if (st != LRI_OK) {
-    for (lri_ctx* c : b->ctx) (void)hipStreamSynchronize(c->stream);
-    return st;
-  }
-  // ... then wait for the ranks in order and queue each member's rotation ...
-  st = batch_each(b, [&](int64_t i, lri_ctx* c) {
-    return guarded(c, [&] {
-      DeScope scope(c);
-      const int64_t r1 = c->ra_rotate();
-      if (r_new) r_new[i] = r1;
-      return LRI_OK;
-    });
-  });
-  if (st != LRI_OK) {
-    for (lri_ctx* c : b->ctx) (void)hipStreamSynchronize(c->stream);
-    return st;
-  }
-  // ... and collect the device-side conditions once all are queued
-  return batch_each(b, [&](int64_t, lri_ctx* c) { return guarded<true>(c, [&] { return c->read_flags(); }); });
+  // ... then wait for the ranks in order and queue each member's rotation
+  // (members whose first half failed are skipped and keep their state) ...
+  const int st2 = batch_each(
+      b,
+      [&](int64_t i, lri_ctx* c) {
+        return guarded(c, [&] {
+          DeScope scope(c);
+          const int64_t r1 = c->ra_rotate();
+          if (r_new) r_new[i] = r1;
+          return LRI_OK;
+        });
+      },
+      false);
+  // ... and collect (and clear) the device-side conditions of every member once
+  // all are queued, including those of members that failed above
+  std::vector<int> before = b->status;
+  std::fill(b->status.begin(), b->status.end(), LRI_OK);
+  const int st3 = batch_each(b, [&](int64_t, lri_ctx* c) { return guarded<true>(c, [&] { return c->read_flags(); }); },
+                             false);
+  for (size_t i = 0; i < before.size(); ++i)
+    if (before[i] != LRI_OK) b->status[i] = before[i];
+  if (st != LRI_OK) return st;
+  if (st2 != LRI_OK) return st2;
+  return st3;
+}
+
+int lri_batch_member_status(const lri_batch* b, int64_t i, int* status) {
+  if (!b || !status || i < 0 || i >= (int64_t)b->ctx.size()) return LRI_EINVAL;
+  *status = i < (int64_t)b->status.size() ? b->status[i] : LRI_OK;
+  return LRI_OK;
 }
 
 int lri_batch_use_graphs(lri_batch* b, int on) {

@@ -71,6 +71,7 @@ SIGNATURES = {
     "lri_batch_step_de_bug_ra": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_i64), ctypes.c_double, _i64,
                                         ctypes.POINTER(_i64)]),
     "lri_batch_use_graphs": (_int, [_vp, _int]),
+    "lri_batch_member_status": (_int, [_vp, _i64, ctypes.POINTER(_int)]),
     "lri_batch_synchronize": (_int, [_vp]),
     "lri_synchronize": (_int, [_vp]),
     "lri_get_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
@@ -96,6 +97,10 @@ class BreakdownError(LRIError):
 
 class NonFiniteError(LRIError):
     pass
+
+
+class NotSupportedError(LRIError):
+    """LRI_ENOTSUP: e.g. a rank-adaptive step that needs a rank above the context capacity."""
 
 
 def lib():
@@ -125,6 +130,8 @@ def check(status, ctx=None):
         raise BreakdownError(status, msg)
     if status == ENONFINITE:
         raise NonFiniteError(status, msg)
+    if status == ENOTSUP:
+        raise NotSupportedError(status, msg)
     if status == EINVAL:
         raise ValueError(msg)
     raise LRIError(status, msg)
@@ -394,6 +401,8 @@ def _check_batch(status, b):
         raise BreakdownError(status, msg)
     if status == ENONFINITE:
         raise NonFiniteError(status, msg)
+    if status == ENOTSUP:
+        raise NotSupportedError(status, msg)
     if status == EINVAL:
         raise ValueError(msg)
     raise LRIError(status, msg)
@@ -469,6 +478,15 @@ class Batch:
         _check_batch(lib().lri_batch_step_de_bug_ra(self.h, float(dt), p, float(tol), int(rmax),
                                                      r.ctypes.data_as(ctypes.POINTER(_i64))), self.h)
         return r
+
+    def member_status(self) -> np.ndarray:
+        """Each member's status from the last batch call (0 = OK, else its LRI_E* code)."""
+        out = np.zeros(len(self), dtype=np.int64)
+        st = _int()
+        for i in range(len(self)):
+            _check_batch(lib().lri_batch_member_status(self.h, i, ctypes.byref(st)), self.h)
+            out[i] = st.value
+        return out
 
     def use_graphs(self, on: bool):
         _check_batch(lib().lri_batch_use_graphs(self.h, int(bool(on))), self.h)

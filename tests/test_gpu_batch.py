@@ -121,3 +121,64 @@ def test_batch_graph_replays_match_direct_launches(lri):
         for i in range(len(probs)):
             for a, b_ in zip(states[True][k][i], states[False][k][i]):
                 assert np.array_equal(a, b_), f"step {k}, member {i}"
+
+
+def cfg3_problems(count, seed):
+    """The batched workload's distribution: n = m log-uniform in [256, 4096], initial ranks
+    uniform in {4..64} with the last four forced to 48, 56, 64, 64, Y0 = U diag(10^(-i/4)) V'."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(256), np.log(4096), count)).astype(np.int64)
+    rs = rng.integers(4, 65, count)
+    rs[-4:] = [48, 56, 64, 64]
+    ns[-4] = 300  # one wide member small enough for the oracle
+    out = []
+    for n, r in zip(ns, rs):
+        U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+        V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+        out.append((int(n), int(r), U, np.diag(10.0 ** (-np.arange(r) / 4.0)), V))
+    return out
+
+
+def test_batch_cfg3_distribution(lri):
+    """32 members drawn like the batched workload (dX/dt = L X + X L', L the Dirichlet
+    Laplacian n^2 tridiag(1, -2, 1), tol 1e-8, dt 1e-4, rmax 64), two rank-adaptive batch
+    steps: every member bit for bit equal to a context of its own, four members (among
+    them an r = 48 one, whose first step runs the 96 x 96 SVD) against the oracle."""
+    import scipy.sparse as sp
+
+    count, dt, tol, rmax, steps = 32, 1e-4, 1e-8, 64, 2
+    probs = cfg3_problems(count, seed=3)
+    Ns = [lri.rk4_substeps(2.0 * lri.laplacian_1d(n).gershgorin(), dt) for n, *_ in probs]
+    say(f"case: cfg3-shaped batch, {count} members, n in [{min(p[0] for p in probs)}, {max(p[0] for p in probs)}], "
+        f"ranks {[p[1] for p in probs]}")
+    with lri.native.Batch([p[0] for p in probs], [p[0] for p in probs], [rmax] * count) as b:
+        for (n, r, U, S, V), c in zip(probs, b.members):
+            Lop = lri.laplacian_1d(n)
+            c.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False)
+            c.set_state(U, S, V)
+        ranks = [b.step_de_bug_ra(dt, Ns, tol, rmax) for _ in range(steps)]
+        assert not b.member_status().any()
+        got = [c.get_state() for c in b.members]
+    say(f"  ranks after {steps} steps: {list(ranks[-1])}")
+    for i, (n, r, U, S, V) in enumerate(probs):
+        with lri.native.Context(n, n, rmax) as c:
+            Lop = lri.laplacian_1d(n)
+            c.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False)
+            c.set_state(U, S, V)
+            rk = [c.step_de_bug_ra(dt, Ns[i], tol, rmax) for _ in range(steps)]
+            ref = c.get_state()
+        assert [int(x[i]) for x in ranks] == rk, f"member {i}"
+        for a, b_ in zip(got[i], ref):
+            assert a.shape == b_.shape and np.array_equal(a, b_), f"member {i}"
+    order = np.argsort([p[0] for p in probs[:-4]])
+    check = [int(order[0]), int(order[1]), int(order[2]), count - 4]
+    for i in check:
+        n, r, U, S, V = probs[i]
+        L = sp.csr_matrix(sp.diags([np.ones(n - 1), -2.0 * np.ones(n), np.ones(n - 1)], [-1, 0, 1]) * float(n * n))
+        op = dlra_de.LowRankLyapunov(L, L, np.zeros((n, 1)), np.zeros((n, 1)))
+        Uo, So, Vo = U, S, V
+        for _ in range(steps):
+            Uo, So, Vo, _ = dlra_de.rank_adaptive_bug_de_step_lowrank(op, Uo, So, Vo, 0.0, dt, Ns[i], tol, rmax)
+        e = rel_err(*got[i], Uo, So, Vo)
+        say(f"  member {i} (n={n}, r0={r}) vs oracle: rank {So.shape[0]}, rel err {e:.2e}")
+        assert got[i][1].shape == So.shape and e <= 1e-9

@@ -246,6 +246,60 @@ def test_one_step_rank_adaptive_fp64(lri, shape, tol):
     assert np.allclose(np.diag(S), np.diag(So), rtol=1e-10, atol=1e-14)
 
 
+@pytest.mark.parametrize("shape,tol,rmax", [((300, 260, 48), 1e-14, 64), ((260, 300, 48), 3e-2, 64),
+                                           ((400, 330, 64), 1e-14, 64)])
+def test_one_step_rank_adaptive_wide_fp64(lri, shape, tol, rmax):
+    """Ranks 48 and 64: the augmented core is 96 x 96 or 128 x 128, so the SVD runs the
+    256-thread Jacobi kernel (2 r > 64, V in LDS or in global scratch); tol = 1e-14 keeps
+    the rank at rmax, tol = 3e-2 truncates below it."""
+    n, m, r = shape
+    say(f"case: one RA step n={n} m={m} r={r} tol={tol} rmax={rmax}")
+    u, A0, A1 = one_step_case(13, n, m, r, inc=1e-2)
+    Uo, So, Vo, r1 = dlra.rank_adaptive_bug_step(u.U, u.S, u.V, A1 - A0, tol, rmax)
+    U, S, V = run_native_step(lri, "ra", u, [A0, A1], tol=tol, rmax=rmax)
+    say(f"  new rank {S.shape[0]} (oracle {r1})")
+    assert S.shape[0] == r1
+    assert rel_err(U, S, V, Uo, So, Vo) <= 1e-9
+    assert np.allclose(np.diag(S), np.diag(So), rtol=1e-10, atol=1e-14)
+    assert orth_defect(U) < 1e-12 and orth_defect(V) < 1e-12
+
+
+def test_rank_adaptive_capacity_overflow(lri):
+    """rmax above the context capacity and a tolerance that keeps every direction: the
+    step needs rank 2 r = 24 > rcap = 16, so it reports LRI_ENOTSUP (no silent clip) and
+    leaves the state at the step's start; the same context then steps with rmax = rcap."""
+    n, m, r, rcap = 128, 112, 12, 16
+    say(f"case: RA capacity overflow n={n} m={m} r={r} rcap={rcap}")
+    u, A0, A1 = one_step_case(14, n, m, r, inc=1e-1)
+    with lri.native.Context(n, m, rcap) as ctx:
+        ctx.set_state(u.U, u.S, u.V)
+        ctx.reserve(2, lri.native.F64)
+        ctx.upload(0, lri.api._encode(A0, "f64"))
+        ctx.upload(1, lri.api._encode(A1, "f64"))
+        with pytest.raises(lri.native.NotSupportedError):
+            ctx.step_bug_ra(0, 1, 1e-14, 64)
+        U, S, V = ctx.get_state()
+        assert S.shape[0] == r and np.array_equal(U, u.U) and np.array_equal(V, u.V)
+        assert rel_err(U, S, V, u.U, u.S, u.V) == 0.0
+        assert ctx.step_bug_ra(0, 1, 1e-14, rcap) == rcap
+        U, S, V = ctx.get_state()
+    Uo, So, Vo, r1 = dlra.rank_adaptive_bug_step(u.U, u.S, u.V, A1 - A0, 1e-14, rcap)
+    assert r1 == rcap and rel_err(U, S, V, Uo, So, Vo) <= 1e-9
+
+
+def test_solve_capacity_overflow_raises(lri):
+    """Through the API: the reference's default rmax (2^62) and a tolerance of 0 on data
+    whose increments add rank every step: the rank doubles to 64 (the capacity), then the
+    next step needs more and raises instead of clipping."""
+    n = 200
+    rng = np.random.default_rng(15)
+    u = random_factors(rng, n, n, 16, decay=0.5)
+    snaps = [rng.standard_normal((n, n)) for _ in range(4)]
+    prob = lri.MatrixDataProblem(snaps, lri.SVDLikeRepresentation(u.U, u.S, u.V))
+    with pytest.raises(lri.native.NotSupportedError):
+        lri.solve(prob, lri.RankAdaptiveUnconventionalAlgorithm(0.0))
+
+
 def test_rank_adaptive_zero_increment_keeps_rank(lri):
     u, A0, _ = one_step_case(4, 128, 96, 6)
     U, S, V = run_native_step(lri, "ra", u, [A0, A0.copy()], tol=1e-10, rmax=64)
@@ -299,6 +353,27 @@ def test_array_data_matches_function_data(lri):
         b = lri.solve(lri.MatrixDataProblem(snaps, y0), alg).Y[-1]
         A, B = a.matrix(), b.matrix()
         assert np.all(np.abs(A - B) <= np.sqrt(np.finfo(float).eps) * np.maximum(np.abs(A), np.abs(B)))
+
+
+def test_array_data_late_start(lri):
+    """tspan = (3, 9) on array data: the cache's yprev is y[1] whatever t0 is
+    (projector_splitting.jl:91, unconventional.jl:91), so the first increment is
+    y[4] - y[1], the later ones y[k+1] - y[k]; the engine's solve against the oracle's."""
+    d, u0 = cfg0_data()
+    y0 = lri.SVDLikeRepresentation(u0.U, u0.S, u0.V)
+    snaps = [d(0.01 * k) for k in range(10)]
+    from oracle import lra as olra
+
+    for name in ("ksl_primal", "bug", "ra_bug"):
+        alg = GOLDEN_ALGS[name](lri)
+        got = lri.solve(lri.MatrixDataProblem(snaps, y0, (3, 9)), alg)
+        oalg = {"ksl_primal": dlra.ProjectorSplitting(dlra.PrimalLieTrotter()),
+                "bug": dlra.UnconventionalAlgorithm(),
+                "ra_bug": dlra.RankAdaptiveUnconventionalAlgorithm(2e-2, rmax=16)}[name]
+        ref = dlra.solve(dlra.MatrixDataProblem(snaps, olra.SVDLike(u0.U, u0.S, u0.V), (3, 9)), oalg)
+        assert len(got.Y) == len(ref.Y) == 7 and list(got.t) == list(ref.t)
+        a, b = got.Y[-1], ref.Y[-1]
+        assert rel_err(a.U, a.S, a.V, b.U, b.S, b.V) <= FP64_TOL, name
 
 
 # ------------------------------------------------------------------ mixed precision
