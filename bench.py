@@ -563,6 +563,7 @@ def run_batched(args, torch, lri):
     for (n, r, U, S, V), c in zip(probs, b.members):
         Lop = lri.laplacian_1d(n)
         c.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False)
+        c.set_resident_eigenbasis(True)
         c.set_state(U, S, V)
         Ns.append(lri.rk4_substeps(2.0 * Lop.gershgorin(), dt))
     b.synchronize()
@@ -579,11 +580,14 @@ def run_batched(args, torch, lri):
     b.synchronize()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    merged_steps, launches = b.launch_stats()
     b.close()
     value = count * args.batch_steps / el
     hist = np.bincount(ranks, minlength=rmax + 1)
-    log(f"batched RA: {value:.1f} problem-steps/s ({1e3 * el / args.batch_steps:.1f} ms per batch step); "
-        f"final ranks in [{ranks.min()}, {ranks.max()}]")
+    log(f"batched RA: {value:.1f} problem-steps/s ({1e3 * el / args.batch_steps:.1f} ms per batch step, "
+        f"{launches} merged launches in the last); final ranks in [{ranks.min()}, {ranks.max()}]")
+    if merged_steps < args.batch_steps:
+        raise SystemExit("the batch did not run as merged launches")
     out = {
         "metric": "problem-steps/s of 512 independent rank-adaptive BUG solves of dX/dt = L X + X L', n=m "
                   "log-uniform in [256, 4096], initial ranks 4..64, tol 1e-8, dt 1e-4 (FP64)",
@@ -598,10 +602,11 @@ def run_batched(args, torch, lri):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{count} rank-adaptive BUG solves (one context and stream each, every batch step "
-                               f"queues all members before waiting), L = n^2 tridiag(1,-2,1) Dirichlet, no forcing, "
-                               f"Y0 = U diag(10^(-i/4)) V', rmax={rmax}", "problems": count, "dt": dt, "tol": tol,
-                   "rmax": rmax, "parallelism": "single GPU"},
+        "config": {"workload": f"{count} rank-adaptive BUG solves (one context each, resident eigenbasis; each "
+                               f"stage of a batch step is one merged launch over all members), L = n^2 "
+                               f"tridiag(1,-2,1) Dirichlet, no forcing, Y0 = U diag(10^(-i/4)) V', rmax={rmax}",
+                   "problems": count, "dt": dt, "tol": tol, "rmax": rmax, "parallelism": "single GPU",
+                   "launches_per_batch_step": launches},
         "rank_histogram": {str(k): int(v) for k, v in enumerate(hist) if v},
         "initial_rank_histogram": {str(k): int(v) for k, v in enumerate(np.bincount([p[1] for p in probs])) if v},
         "roofline": None,
@@ -633,37 +638,46 @@ def run_de(args, torch, lri):
     Lop = lri.laplacian_1d(n)
     F = lri.LyapunovOperator(Lop, Lop, a, b)
     N = lri.rk4_substeps(F.spectral_bound(), dt)
-    ctx = lri.native.Context(n, m, r, 0)
-    ctx.set_state(U0, S0, V0)
-    ctx.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False, a, b)
+    def de_run(resident, steps, events):
+        ctx = lri.native.Context(n, m, r, 0)
+        ctx.set_state(U0, S0, V0)
+        ctx.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False, a, b)
+        if resident:
+            ctx.set_resident_eigenbasis(True)
+        for _ in range(args.warmup_de):
+            ctx.step_de_bug(dt, N)
+        ctx.set_state(U0, S0, V0)
+        ctx.timing(2 if events else 0)  # HIP events around the DE-family kernels only
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.step_de_bug(dt, N)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        el_ = time.perf_counter() - t0
+        de = ctx.timing_read(lri.native.KERNEL_DE)
+        ctx.timing(0)
+        ctx.close()
+        return el_, de
+
     log(f"DE workload n=m={n} r={r} dt={dt}: {N} RK4 sub-steps per sub-problem; warm-up {args.warmup_de}")
-    for _ in range(args.warmup_de):
-        ctx.step_de_bug(dt, N)
-    ctx.set_state(U0, S0, V0)
-    ctx.timing(0 if args.no_events else 2)  # HIP events around the eigenbasis transforms only
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.de_steps):
-        ctx.step_de_bug(dt, N)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    # the state resident in the Laplacian's eigenbasis (lri_set_resident_eigenbasis): no n x n
+    # transforms per step; the K and L flows are closed-form per entry
+    el, (de_ms, de_cnt) = de_run(True, args.de_steps, not args.no_events)
     value = args.de_steps / el
-    de_ms, de_cnt = ctx.timing_read(lri.native.KERNEL_DE)
-    ctx.timing(0)
-    ctx.close()
+    el_orig, (tr_ms, tr_cnt) = de_run(False, max(4, args.de_steps // 4), True)
     avg_ms = de_ms / de_cnt if de_cnt else None
-    # Constant coefficients: the K and L sub-steps run in the analytic eigenbasis of
-    # L (both sides share it), as one forward (Q' Z) and one backward (Q Z)
-    # transform over the 2r columns of both sub-problems, each an FP64-MFMA pass
-    # that streams the n x n basis once.
+    tr_avg = tr_ms / tr_cnt if tr_cnt else None
+    # resident: the closed-form flow reads and writes the K and L columns (2 n 2r doubles each way)
     kcols = 2 * r
+    bytes_alg = 8.0 * 2 * n * kcols
+    achieved = bytes_alg / (avg_ms * 1e-3) / 1e9 if avg_ms else None
+    # original basis: one forward and one backward FP64-MFMA transform over the 2r columns
     flops = 2.0 * n * n * kcols
-    bytes_alg = 8.0 * (n * n + 2 * n * kcols)
-    achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms else None
-    log(f"DE BUG: {value:.2f} steps/s ({1e3 * el / args.de_steps:.2f} ms/step); "
-        f"transform launch {avg_ms if avg_ms is None else round(avg_ms, 4)} ms")
+    log(f"DE BUG: {value:.2f} steps/s ({1e3 * el / args.de_steps:.2f} ms/step) resident in the eigenbasis; "
+        f"original basis {max(4, args.de_steps // 4) / el_orig:.2f} steps/s (transform launch "
+        f"{tr_avg if tr_avg is None else round(tr_avg, 4)} ms)")
     out = {
         "metric": "steps/s of the BUG integrator on dX/dt = L X + X L' + a b' at n=m=4096, r=32 (FP64, RK4 sub-steps)",
         "value": value,
@@ -679,15 +693,19 @@ def run_de(args, torch, lri):
         "data": "synthetic",
         "config": {"workload": f"unconventional (BUG) integrator on a MatrixDEProblem, n=m={n}, r={r}, dt={dt}, "
                                f"L = n^2 tridiag(1,-2,1) Dirichlet, G = a b' (seeded N(0,1)), Y0 = U diag(10^(-i/4)) V', "
-                               f"{N} fixed RK4 sub-steps per sub-problem", "n": n, "m": m, "r": r, "dt": dt,
+                               f"{N} fixed RK4 sub-steps per sub-problem, state resident in the eigenbasis of L",
+                   "n": n, "m": m, "r": r, "dt": dt,
                    "substeps": N, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": achieved / 78.6 if achieved else None, "traffic": None,
-                     "kernel": "k_passT_f64 / k_passN_f64 (eigenbasis transforms of the K and L columns)",
-                     "flops_per_launch": flops, "bytes_per_launch": bytes_alg,
-                     "note": f"2 n^2 (2r) flops and 8 (n^2 + 4 n r) bytes per launch: {flops / bytes_alg:.1f} "
-                             "flop/B, above the FP64 machine balance (~10 flop/B), so FP64 MFMA bounds it"},
-        "kernels": {"transforms": {"total_ms": de_ms, "launches": de_cnt, "avg_ms": avg_ms}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": "k_spectral_flow (closed-form K and L flows in the resident eigenbasis)",
+                     "bytes_per_launch": bytes_alg,
+                     "note": "16 n (2r) bytes per launch (the K and L columns read and written once); the launch "
+                             "is latency-bound at this size"},
+        "kernels": {"flow": {"total_ms": de_ms, "launches": de_cnt, "avg_ms": avg_ms},
+                    "original_basis_transforms": {"total_ms": tr_ms, "launches": tr_cnt, "avg_ms": tr_avg,
+                                                  "tflops": flops / (tr_avg * 1e-3) / 1e12 if tr_avg else None}},
+        "original_basis_steps_per_s": max(4, args.de_steps // 4) / el_orig,
     }
     if not args.no_cpu:
         log("CPU baseline (oracle, factored right-hand sides) on one step")
@@ -876,8 +894,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP64/FP32 forms")
     ap.add_argument("--no-check", action="store_true", help="skip the error check")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default pass plans (no autotuning)")
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP-event kernel timing in the timed region (for external profilers)")
+    ap.add_argument("--event-every", type=int, default=4,
+                    help="bracket the passes of one timed step in this many with HIP events")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--workload", choices=["ksl", "de", "greedy", "burgers", "batched", "adjoint", "gradient"],
@@ -960,20 +981,51 @@ def main():
     ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, n, owners=snaps)
     ctx.set_precision(lri.native.PREC_MIXED)
     ctx.set_state(U0, S0, V0)
+    tune = None
+    if not args.no_tune:
+        # the pass autotuner: every candidate reduction chunk of both passes, the fastest kept
+        tune = ctx.autotune_passes(0, 1, reps=10)
+        log("autotuned passes: " + ", ".join(f"{k} chunk {v['chunk']} ({v['splits']} splits) {v['us']:.1f} us "
+                                             f"(default {v['default_us']:.1f} us)" for k, v in tune.items()))
 
+    check = None
+    if not args.no_check:
+        # untimed, before the timed region: one sweep in each arithmetic on the same BF16 snapshots
+        log("error check: one sweep mixed vs FP64 products on the same BF16 snapshots")
+        ctx.set_state(U0, S0, V0)
+        run_steps(ctx, lri, npairs, npairs)
+        Ym = ctx.get_state()
+        ctx.set_precision(lri.native.PREC_F64)
+        ctx.set_state(U0, S0, V0)
+        t0 = time.perf_counter()
+        run_steps(ctx, lri, npairs, npairs)
+        Yd = ctx.get_state()
+        check = (lri.metrics.rel_err(*Ym, *Yd), npairs / (time.perf_counter() - t0))
+        ctx.set_precision(lri.native.PREC_MIXED)
+        log(f"rel err mixed vs FP64 after {npairs} steps: {check[0]:.3e}")
+
+    ctx.set_state(U0, S0, V0)
     log(f"warm-up {args.warmup} steps")
     run_steps(ctx, lri, args.warmup, npairs)
     ctx.set_state(U0, S0, V0)
-    ctx.timing(0 if args.no_events else 2)  # HIP events around the two product passes of every step
-    log(f"timed {args.steps} mixed-precision steps")
+    # HIP events around the two product passes of one step in `every` (each pass of a sampled
+    # step is bracketed by two events on the engine's stream)
+    every = max(1, args.event_every)
+    ctx.timing_sample(every)
+    ctx.timing(0 if args.no_events else 2)
+    log(f"timed {args.steps} mixed-precision steps (pass events on one step in {every})")
     el = timed_steps(torch, ctx, lri, args.steps, npairs)
     value = args.steps / el
     fam = {}
     for name, f in (("pass_N", lri.native.KERNEL_PASS_N), ("pass_T", lri.native.KERNEL_PASS_T),
                     ("small", lri.native.KERNEL_SMALL)):
         ms, cnt = ctx.timing_read(f)
-        fam[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / max(cnt, 1)}
+        smp = ctx.timing_samples(f)
+        fam[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / max(cnt, 1),
+                     "min_ms": float(smp.min()) if smp.size else None,
+                     "median_ms": float(np.median(smp)) if smp.size else None}
     ctx.timing(False)
+    ctx.timing_sample(1)
     # untimed-event check of the event overhead
     ctx.set_state(U0, S0, V0)
     el_noev = timed_steps(torch, ctx, lri, min(args.steps, 2 * npairs), npairs)
@@ -1016,25 +1068,12 @@ def main():
         "kernels": fam,
         "value_without_kernel_events": value_noev,
         "generation_s": gen_s,
+        "autotune": tune,
     }
 
-    if not args.no_check:
-        log("error check: one sweep mixed vs FP64 products on the same BF16 snapshots")
-        ctx.set_precision(lri.native.PREC_MIXED)
-        ctx.set_state(U0, S0, V0)
-        run_steps(ctx, lri, npairs, npairs)
-        Ym = ctx.get_state()
-        ctx.set_precision(lri.native.PREC_F64)
-        ctx.set_state(U0, S0, V0)
-        t0 = time.perf_counter()
-        run_steps(ctx, lri, npairs, npairs)
-        Yd = ctx.get_state()
-        f64_on_bf16 = npairs / (time.perf_counter() - t0)
-        rel_err = lri.metrics.rel_err
-
-        out["rel_err_vs_fp64"] = rel_err(*Ym, *Yd)
-        out["fp64_products_on_bf16_steps_per_s"] = f64_on_bf16
-        log(f"rel err mixed vs FP64 after {npairs} steps: {out['rel_err_vs_fp64']:.3e}")
+    if check is not None:
+        out["rel_err_vs_fp64"] = check[0]
+        out["fp64_products_on_bf16_steps_per_s"] = check[1]
     ctx.close()
 
     if not args.no_forms:

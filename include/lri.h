@@ -190,6 +190,16 @@ int lri_step_greedy(lri_ctx* ctx, int64_t k_next);
  * polar factor of X Z1), X = A[k_next].  Two passes over X. */
 int lri_step_greedy_twofactor(lri_ctx* ctx, int64_t k_next);
 
+/* Greedy step on a MatrixHybridProblem (primitives.jl:36-41,
+ * greedy_integrator.jl:41-46,73-82): the two-factor state Y = U Z' with Z in the
+ * state's V slot.  The Z integrator advances Z over [t, t + dt] with U held by
+ * `substeps` classical RK4 steps of Z' = f(Z, U, t) = F(U Z')' U -- F the
+ * operator set by lri_set_operator_lyapunov / lri_set_operator_quadratic, the
+ * form of the reference's hybrid test (test/data_informed_approximation.jl:75) --
+ * then X = A[k_next] and U1 = Q P' for the SVD X Z1 = Q Sigma P'.  The
+ * reference's Z integrator is an adaptive Tsit5 ODE integrator. */
+int lri_step_hybrid_greedy(lri_ctx* ctx, int64_t k_next, double dt, int64_t substeps);
+
 /* ---- matrix ODE (MatrixDEProblem, primitives.jl:13-17) --------------------
  *
  * The reference evaluates a user closure F(Y, t) on low-rank arguments inside
@@ -230,6 +240,21 @@ int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* 
  * rank <= LRI_QUAD_MAX_RANK (the rank-adaptive step: 2 r <= LRI_QUAD_MAX_RANK). */
 #define LRI_QUAD_MAX_RANK 32
 int lri_set_operator_quadratic(lri_ctx* ctx, const double* dl, const double* dd, const double* du, int periodic);
+
+/* Keep the state in the analytic eigenbasis of a constant-coefficient operator
+ * (on = 1) or leave it (on = 0).  Needs lri_set_operator_lyapunov with constant
+ * coefficients on both sides (d I + e (shift + shift'), Dirichlet or periodic)
+ * and no quadratic term, else LRI_EINVAL.  The device state then holds
+ * Qa' U and Qb' V (Qa, Qb the orthogonal eigenbases of A and B) and the matrix-ODE
+ * steps use the operator's diagonal form, so a step makes no n x n transforms:
+ * the DLRA steps commute with an orthogonal change of basis on either side (the
+ * QRs, the Galerkin products and the SVD are basis-covariant), so the results
+ * are those of the untransformed steps up to rounding.  lri_set_state /
+ * lri_get_state / lri_trajectory_get transform at the boundary; a data-path step,
+ * an adjoint or a new operator leaves the eigenbasis first.  No counterpart in
+ * the reference (it evaluates F on the factors in the original basis,
+ * projector_splitting.jl:52-80, unconventional.jl:51-79). */
+int lri_set_resident_eigenbasis(lri_ctx* ctx, int on);
 
 /* One KSL step of the matrix ODE over [t, t+dt]: order LRI_PRIMAL
  * (projector_splitting.jl:129-152,191-196), LRI_DUAL (:166-189,198-203) or
@@ -272,10 +297,22 @@ int lri_step_de_ksl_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* U
  *
  * Many small solves of uneven n, m and rank (the reference runs them one
  * `solve` at a time, primitives.jl:68-80).  A batch owns one context per
- * problem, each on its own stream; a batch step queues the step of every
- * member before it waits for any, so their kernels overlap on the device.
- * Members are set up through their contexts (lri_batch_context: state,
- * operator); they are owned by the batch. */
+ * problem.  Members are set up through their contexts (lri_batch_context:
+ * state, operator); they are owned by the batch.
+ *
+ * Merged launches (the default): the members share one stream, and a
+ * rank-adaptive batch step whose members are all in their resident eigenbasis
+ * (lri_set_resident_eigenbasis) runs every stage of the step as one launch
+ * over all members: each member's launches are recorded once per rank and step
+ * parameters, their argument blocks kept in a device table, and position p of
+ * all members' sequences launched together, each workgroup finding its member
+ * in the table.  Each member's arithmetic is that of stepping its context
+ * alone, so the results are the same bit for bit.  The merged list runs as a
+ * captured HIP graph while it stays the same (lri_batch_use_graphs).  Other
+ * batch steps run the members one after another on the shared stream.
+ * Without merged launches (lri_batch_use_merged_launches(batch, 0)) each
+ * member has its own stream and a batch step queues every member's step before
+ * it waits for any. */
 typedef struct lri_batch lri_batch;
 int lri_batch_create(lri_batch** out, int device, int64_t count, const int64_t* n, const int64_t* m,
                      const int64_t* rcap);
@@ -287,13 +324,23 @@ int lri_batch_context(lri_batch* batch, int64_t i, lri_ctx** ctx);
 int lri_batch_step_de_bug(lri_batch* batch, double dt, const int64_t* substeps);
 /* lri_step_de_bug_ra on every member; r_new[i] receives member i's new rank.
  * All members' steps are queued up to their SVD before the first rank is read.
- * Member i truncates at min(rmax, rcap[i]): its capacity is its own rmax. */
+ * Member i truncates at min(rmax, rcap[i]): its capacity is its own rmax.  With
+ * merged launches the new ranks and condition flags of all members are read in
+ * one transfer, a member whose step raised a condition keeps its state, and
+ * the rotations to the new ranks are left queued (lri_batch_synchronize waits). */
 int lri_batch_step_de_bug_ra(lri_batch* batch, double dt, const int64_t* substeps, double tol, int64_t rmax,
                              int64_t* r_new);
-/* Rank-adaptive batch steps run each member's step up to its SVD as a HIP
- * graph captured for the member's current rank (default on = 1), so one
- * member-step costs one graph launch on the host; 0: direct launches. */
+/* HIP graphs for rank-adaptive batch steps (default on = 1): with merged
+ * launches, the merged list is captured once it repeats and replayed while it
+ * stays the same; without, each member's step up to its SVD is a graph
+ * captured for the member's current rank.  0: direct launches. */
 int lri_batch_use_graphs(lri_batch* batch, int on);
+/* Merged launches on (1, the default) or off (0); see above.  Switching waits
+ * for every member's queued work. */
+int lri_batch_use_merged_launches(lri_batch* batch, int on);
+/* The number of batch steps that ran as merged launches, and the kernel
+ * launches of the last of them (both phases and the gather of the new ranks). */
+int lri_batch_launch_stats(const lri_batch* batch, int64_t* merged_steps, int64_t* launches);
 /* The status of member i from the last batch call (LRI_OK or the LRI_E* that
  * member reported).  A batch step runs every member even when one fails: a
  * member whose step failed keeps its state at the step's start (a rank-adaptive
@@ -301,6 +348,19 @@ int lri_batch_use_graphs(lri_batch* batch, int on);
  * member has advanced. */
 int lri_batch_member_status(const lri_batch* batch, int64_t i, int* status);
 int lri_batch_synchronize(lri_batch* batch);
+
+/* ---- trajectory (DLRSolution.Y, update_sol!, primitives.jl:46-49,82-90) ----
+ *
+ * The reference deep-copies u into the solution after every step.  Here the
+ * copies stay in device memory until they are read: lri_trajectory_save queues
+ * a device-to-device copy of the current state into a slot (stream-ordered, no
+ * synchronisation), lri_trajectory_get synchronises and copies one slot out
+ * (host or device arrays, as lri_get_state).  Each slot holds
+ * (n + m) * rcap + rcap^2 doubles. */
+int lri_trajectory_reserve(lri_ctx* ctx, int64_t slots);
+int lri_trajectory_save(lri_ctx* ctx, int64_t slot);
+int lri_trajectory_get(lri_ctx* ctx, int64_t slot, int64_t* r_out, double* U, int64_t ldu, double* S, int64_t lds,
+                       double* V, int64_t ldv);
 
 /* Wait for all queued work; returns a latched device-side status, if any. */
 int lri_synchronize(lri_ctx* ctx);
@@ -328,6 +388,29 @@ int lri_kernel_timing(lri_ctx* ctx, int mode);
 /* Synchronise and return the summed duration (ms) and launch count of a
  * kernel family since timing was enabled. */
 int lri_kernel_timing_read(lri_ctx* ctx, int family, double* total_ms, int64_t* launches);
+
+/* Time only one launch in `every` of each family (default 1: all), so that the
+ * events disturb a timed loop less; kept across lri_kernel_timing calls. */
+int lri_kernel_timing_sample(lri_ctx* ctx, int every);
+
+/* Synchronise and copy up to cap per-launch durations (ms, in launch order) of
+ * a family since timing was enabled; *count receives how many there are. */
+int lri_kernel_timing_samples(lri_ctx* ctx, int family, float* ms, int64_t cap, int64_t* count);
+
+/* ---- autotuning ----------------------------------------------------------
+ *
+ * The streaming passes split their reduction range into chunks, one partial
+ * result per chunk; the default chunk aims at about 512 workgroups.  This
+ * times every candidate chunk (those for about 128 .. 3072 workgroups) of
+ * pass N and pass T over dA = A[kn] - A[kp] (kp < 0: A[kn]) at the context's
+ * current rank and precision, `reps` launches each after an untimed one, and
+ * keeps the fastest for this context's passes of that shape (the partial
+ * buffers grow if a candidate needs it).  report (8 doubles): for pass N then
+ * pass T, the chosen chunk, its split count, its mean time (us) and the mean
+ * time of the default chunk (us).  A chosen chunk changes the order in which
+ * the partials are summed, so results move within the precision's rounding.
+ * No counterpart in the reference (its products are BLAS calls). */
+int lri_autotune_passes(lri_ctx* ctx, int64_t kp, int64_t kn, int64_t reps, double* report);
 
 /* ---- diagnostics --------------------------------------------------------- */
 

@@ -44,6 +44,7 @@ from .api import (  # noqa: F401
     step,
     truncated_svd,
     update_sol,
+    fetch_sol,
 )
 
 step_ = step  # Julia's step!

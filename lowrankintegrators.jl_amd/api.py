@@ -162,6 +162,11 @@ class SymTridiagonal:
             A[0, n - 1] += self.e[n - 1]
         return A
 
+    def is_constant(self) -> bool:
+        """d I + e (shift + shift') (the off-diagonal's wrap entry too when periodic)?"""
+        ne = self.n if self.periodic else self.n - 1
+        return bool(np.all(self.d == self.d[0]) and (ne < 1 or np.all(self.e[:ne] == self.e[0])))
+
     def gershgorin(self) -> float:
         """Upper bound of the spectral radius."""
         n = self.n
@@ -308,9 +313,18 @@ class MatrixDEProblem(AbstractDLRProblem):
 
 
 class MatrixHybridProblem(AbstractDLRProblem):
-    """The reference's hybrid problem (primitives.jl:36-41, Greedy integrator only): not run here."""
+    """``MatrixHybridProblem{y,f,u0,tspan}`` (primitives.jl:36-41): data y(t) (a function,
+    or a sequence of snapshots with tspan = (1, len(y))) and the Z dynamics
+    dZ/dt = f(Z, U, t) of a two-factor state U Z' (Greedy integrator only).  On this
+    engine ``f`` is an operator descriptor F (``LyapunovOperator`` /
+    ``QuadraticOperator``) with f(Z, U, t) = F(U Z')' U, the form of the reference's
+    hybrid test (test/data_informed_approximation.jl:75, FZ = burgers(U Z')' U)."""
 
-    def __init__(self, y, f, u0, tspan):
+    def __init__(self, y, f, u0, tspan=None):
+        if tspan is None:
+            if callable(y):
+                raise ValueError("a function-form MatrixHybridProblem needs tspan")
+            tspan = (1, len(y))
         self.y, self.f, self.u0, self.tspan = y, f, u0, tspan
 
 
@@ -454,13 +468,15 @@ def _device_ptr(x):
 
 
 def init(prob, alg, dt, *, precision: str = "f64", snapshot_dtype: Optional[str] = None,
-         save_everystep: bool = True, device: int = 0, substeps: Optional[int] = None) -> DLRIntegrator:
+         save_everystep: bool = True, device: int = 0, substeps: Optional[int] = None,
+         eigenbasis: bool = True) -> DLRIntegrator:
     """``init(prob, alg, dt)`` (projector_splitting.jl:107-115 and the BUG/RA counterparts)."""
     if isinstance(prob, MatrixHybridProblem):
-        raise NotImplementedError("MatrixHybridProblem: its Z sub-problem is a user closure integrated by an ODE solver "
-                                  "(greedy_integrator.jl:41-46,73-82), which cannot run on the device")
+        return _init_hybrid(prob, alg, dt, precision=precision, snapshot_dtype=snapshot_dtype,
+                            save_everystep=save_everystep, device=device, substeps=substeps)
     if isinstance(prob, MatrixDEProblem):
-        return _init_de(prob, alg, dt, save_everystep=save_everystep, device=device, substeps=substeps)
+        return _init_de(prob, alg, dt, save_everystep=save_everystep, device=device, substeps=substeps,
+                        eigenbasis=eigenbasis)
     if not isinstance(alg, (ProjectorSplitting, UnconventionalAlgorithm, RankAdaptiveUnconventionalAlgorithm,
                             GreedyIntegrator)):
         raise TypeError(f"unsupported algorithm {alg!r}")
@@ -513,7 +529,80 @@ def init(prob, alg, dt, *, precision: str = "f64", snapshot_dtype: Optional[str]
     return DLRIntegrator(prob, alg, dt, ctx, u0.copy(), sol, snapshot_dtype=sdt, save_everystep=save_everystep)
 
 
-def _init_de(prob, alg, dt, *, save_everystep, device, substeps) -> DLRIntegrator:
+def _init_hybrid(prob, alg, dt, *, precision, snapshot_dtype, save_everystep, device, substeps) -> DLRIntegrator:
+    """``init(prob::MatrixHybridProblem, ::GreedyIntegrator, dt)`` (greedy_integrator.jl:41-46,61-71):
+    the operator descriptor of the Z dynamics and the data ring go to the device."""
+    F = prob.f
+    if not isinstance(F, LyapunovOperator):
+        raise NotImplementedError("a Python callable cannot run on the device: give the Z dynamics as a "
+                                  "LyapunovOperator or QuadraticOperator F, f(Z, U, t) = F(U Z')' U")
+    if not isinstance(alg, GreedyIntegrator):
+        raise TypeError("a MatrixHybridProblem is solved by the GreedyIntegrator (greedy_integrator.jl:41-82)")
+    u0 = prob.u0
+    if not isinstance(u0, TwoFactorRepresentation):
+        raise TypeError("a MatrixHybridProblem has a two-factor state U Z' (greedy_integrator.jl:73-82)")
+    t0, tf = prob.tspan
+    if not tf > t0:
+        raise AssertionError("Integration in reverse time direction is not supported")
+    n, m, r = u0.U.shape[0], u0.Z.shape[0], u0.rank
+    if (F.A.n, F.B.n) != (n, m):
+        raise ValueError(f"operator is {F.A.n} x {F.B.n}, state is {n} x {m}")
+    if r > native.MAX_RANK:
+        raise ValueError(f"rank {r} exceeds LRI_MAX_RANK = {native.MAX_RANK}")
+    quad = isinstance(F, QuadraticOperator)
+    if quad and r > native.QUAD_MAX_RANK:
+        raise ValueError(f"the quadratic operator needs rank <= {native.QUAD_MAX_RANK}")
+    ctx = native.Context(n, m, r, device)
+    ctx.set_precision(native.PREC_MIXED if precision == "mixed" else native.PREC_F64)
+    ctx.set_state(u0.U, np.eye(r), u0.Z)  # Z in the V slot
+    q = F.G.shape[1]
+    ctx.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic,
+                              F.G if q else None, F.H if q else None)
+    if quad:
+        ctx.set_operator_quadratic(F.D.dl, F.D.dd, F.D.du, F.D.periodic)
+    y = prob.y
+    sdt = snapshot_dtype or "f64"
+    if callable(y):
+        ctx.reserve(3, native.DTYPES[sdt])
+        ctx.upload(0, _encode(y(t0), sdt))
+    else:
+        ctx.reserve(len(y), native.DTYPES[sdt])
+        for k, a in enumerate(y):
+            ctx.upload(k, _encode(a, sdt))
+    sol = init_sol(dt, t0, tf, u0)
+    sol.Y[0] = u0.copy()
+    integ = DLRIntegrator(prob, alg, dt, ctx, u0.copy(), sol, snapshot_dtype=sdt, save_everystep=save_everystep)
+    integ._array_data = not callable(y)
+    integ.substeps = int(substeps) if substeps is not None else None
+    if quad and integ.substeps is None:
+        # |Y_ij| <= ||U_i|| ||Z_j|| bounds the Jacobian of the quadratic term (factor 2 of headroom)
+        integ.quad_ymax = 2.0 * float(np.max(np.linalg.norm(u0.U, axis=1)) * np.max(np.linalg.norm(u0.Z, axis=1)))
+    return integ
+
+
+def _step_hybrid(integrator, dt) -> None:
+    """greedy_step!(::TwoFactorRepresentation, ..., ::Type{<:MatrixHybridProblem}) (greedy_integrator.jl:73-82)."""
+    ctx, t = integrator.ctx, integrator.t
+    N = integrator.substeps
+    if N is None:
+        F = integrator.prob.f
+        rho = F.spectral_bound(integrator.quad_ymax) if isinstance(F, QuadraticOperator) else F.spectral_bound()
+        N = rk4_substeps(rho, dt)
+    y = integrator.prob.y
+    if integrator._array_data:
+        if not (_is_int(t) and _is_int(dt)):
+            raise TypeError("array-form data needs integer t and dt")
+        kn = t + dt - 1
+        if not 0 <= kn < len(y):
+            raise IndexError(f"snapshot index {kn + 1} outside the data (length {len(y)})")
+    else:
+        kn = [s_ for s_ in range(3) if s_ != integrator._slot_of_t][0]
+        ctx.upload(kn, _encode(y(t + dt), integrator.snapshot_dtype))
+        integrator._slot_of_t = kn
+    ctx.step_hybrid_greedy(kn, dt, N)
+
+
+def _init_de(prob, alg, dt, *, save_everystep, device, substeps, eigenbasis=True) -> DLRIntegrator:
     """``init`` for a matrix ODE: the operator descriptor is copied to the device."""
     F = prob.f
     if not isinstance(F, LyapunovOperator):
@@ -544,6 +633,9 @@ def _init_de(prob, alg, dt, *, save_everystep, device, substeps) -> DLRIntegrato
     ctx.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic,
                               F.G if q else None, F.H if q else None)
     quad = isinstance(F, QuadraticOperator)
+    if eigenbasis and not quad and F.A.is_constant() and F.B.is_constant():
+        # constant coefficients: the state stays in the operator's eigenbasis (no n x n transforms per step)
+        ctx.set_resident_eigenbasis(True)
     if quad:
         # the rank-adaptive S sub-step runs in the augmented 2 r bases
         kmax = 2 * rcap if isinstance(alg, RankAdaptiveUnconventionalAlgorithm) else r
@@ -589,7 +681,9 @@ def step(integrator: DLRIntegrator, alg=None, dt=None) -> None:
     dt = integrator.dt if dt is None else dt
     ctx = integrator.ctx
     t = integrator.t
-    if isinstance(integrator.prob, MatrixDEProblem):
+    if isinstance(integrator.prob, MatrixHybridProblem):
+        _step_hybrid(integrator, dt)
+    elif isinstance(integrator.prob, MatrixDEProblem):
         _step_de(integrator, alg, dt)
     elif integrator._array_data:
         if isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang):
@@ -644,11 +738,24 @@ def _dispatch(integrator, alg, kp, kn, _):
 
 
 def update_sol(integrator: DLRIntegrator, force: bool = False) -> None:
-    """``update_sol!`` (primitives.jl:82-90): store a host copy of u and t."""
+    """``update_sol!`` (primitives.jl:82-90): store a copy of u and t.  With a device
+    trajectory (``solve`` reserves one) the copy stays on the device
+    (``lri_trajectory_save``, no synchronisation) until ``fetch_sol``."""
     if not (integrator.save_everystep or force):
         return
-    u = integrator.refresh_u().copy()
     k = integrator.iter
+    traj = getattr(integrator, "_traj", None)
+    if traj is not None and k < integrator._traj_slots:
+        integrator.ctx.trajectory_save(k)
+        traj[k] = k
+        if k <= len(integrator.sol.Y) - 1:
+            integrator.sol.Y[k] = None
+            integrator.sol.t[k] = integrator.t
+        else:
+            integrator.sol.Y.append(None)
+            integrator.sol.t.append(integrator.t)
+        return
+    u = integrator.refresh_u().copy()
     if k <= len(integrator.sol.Y) - 1:
         integrator.sol.Y[k] = u
         integrator.sol.t[k] = integrator.t
@@ -657,8 +764,21 @@ def update_sol(integrator: DLRIntegrator, force: bool = False) -> None:
         integrator.sol.t.append(integrator.t)
 
 
+def fetch_sol(integrator: DLRIntegrator) -> DLRSolution:
+    """Copy the states that ``update_sol`` left in the device trajectory into ``sol.Y``."""
+    traj = getattr(integrator, "_traj", None) or {}
+    for k, slot in sorted(traj.items()):
+        U, S, V = integrator.ctx.trajectory_get(slot)
+        integrator.sol.Y[k] = TwoFactorRepresentation(U, V) if integrator.twofactor else SVDLikeRepresentation(U, S, V)
+    traj.clear()
+    return integrator.sol
+
+
 def solve(prob, alg, dt=None, **kw) -> DLRSolution:
-    """``solve(prob, alg, dt)`` / ``solve(prob::MatrixDataProblem, alg)`` (primitives.jl:68-80)."""
+    """``solve(prob, alg, dt)`` / ``solve(prob::MatrixDataProblem, alg)`` (primitives.jl:68-80).
+
+    The per-step copies of ``update_sol!`` go to a device trajectory (one slot per
+    saved step, reserved here) and are copied out once, after the last step."""
     if dt is None:
         if not isinstance(prob, MatrixDataProblem) or callable(prob.y):
             raise AssertionError("If the data is not provided as array, integration stepsize needs to be specified")
@@ -667,24 +787,31 @@ def solve(prob, alg, dt=None, **kw) -> DLRSolution:
     try:
         tf, t0 = prob.tspan[1], prob.tspan[0]
         T = tf - t0
+        if integ.save_everystep:
+            integ._traj_slots = len(integ.sol.Y) + 2  # the accumulated t can add a step past init_sol's slots
+            integ.ctx.trajectory_reserve(integ._traj_slots)
+            integ._traj = {}
         while (tf - integ.t) / T > 1e-8:
             step(integ, alg, dt)
             last = not ((tf - integ.t) / T > 1e-8)
             update_sol(integ, force=last)
         integ.ctx.synchronize()
-        return integ.sol
+        return fetch_sol(integ)
     finally:
         integ.ctx.close()
 
 
-def solve_batch(probs, alg, dt, *, substeps=None, save_everystep: bool = False, device: int = 0):
+def solve_batch(probs, alg, dt, *, substeps=None, save_everystep: bool = False, device: int = 0,
+                eigenbasis: bool = True):
     """``solve`` for many independent ``MatrixDEProblem``s of uneven sizes and ranks at once
     (the reference solves them one ``solve`` call at a time, primitives.jl:68-80): one
     context per problem in an ``lri_batch``, whose steps queue every member's work
     before waiting for any.  ``alg``: ``UnconventionalAlgorithm`` or
     ``RankAdaptiveUnconventionalAlgorithm``; all problems share ``dt`` and ``tspan``.
     Returns one ``DLRSolution`` per problem (with ``save_everystep=False``: the initial
-    and the final state)."""
+    and the final state).  With ``eigenbasis`` (default), members with constant-coefficient
+    linear operators keep their state in the operator's eigenbasis; when every member does,
+    a rank-adaptive batch step runs each of its stages as one launch over all members."""
     if not probs:
         return []
     ra = isinstance(alg, RankAdaptiveUnconventionalAlgorithm)
@@ -714,6 +841,8 @@ def solve_batch(probs, alg, dt, *, substeps=None, save_everystep: bool = False, 
             q = F.G.shape[1]
             c.set_operator_lyapunov(F.A.d, F.A.e, F.A.periodic, F.B.d, F.B.e, F.B.periodic,
                                     F.G if q else None, F.H if q else None)
+            if eigenbasis and not isinstance(F, QuadraticOperator) and F.A.is_constant() and F.B.is_constant():
+                c.set_resident_eigenbasis(True)
             if isinstance(F, QuadraticOperator):
                 c.set_operator_quadratic(F.D.dl, F.D.dd, F.D.du, F.D.periodic)
                 ymax = float(np.max(np.linalg.norm(p.u0.U @ p.u0.S, axis=1)) * np.max(np.linalg.norm(p.u0.V, axis=1)))

@@ -144,6 +144,7 @@ __global__ __launch_bounds__(1024) void k_rk4_columns(RK4ColArgs P0, RK4ColArgs 
 }
 
 hipError_t launch_rk4_columns(const RK4ColArgs& P0, const RK4ColArgs& P1, hipStream_t st) {
+  if (current_recorder()) return hipErrorNotSupported;  // no batched form
   const RK4ColArgs* ps[2] = {&P0, &P1};
   int64_t nmax = 0;
   for (const RK4ColArgs* P : ps) {
@@ -175,9 +176,9 @@ hipError_t launch_rk4_columns(const RK4ColArgs& P0, const RK4ColArgs& P1, hipStr
 // CW = op(C) W (k x k) and coef = FX W (q x k); kLds stages op(C) and W in LDS
 // (k <= 64), where the products read them at an odd stride.
 template <bool kLds>
-__global__ __launch_bounds__(256) void k_de_prep(const double* __restrict__ C, int ldc, int transC,
-                                                 const double* __restrict__ W, const double* __restrict__ FX, int k,
-                                                 int q, double* __restrict__ CW, double* __restrict__ coef) {
+__device__ __forceinline__ void de_prep_body(const double* __restrict__ C, int ldc, int transC,
+                                             const double* __restrict__ W, const double* __restrict__ FX, int k,
+                                             int q, double* __restrict__ CW, double* __restrict__ coef) {
   extern __shared__ double sm[];
   const int t = threadIdx.x, nt = blockDim.x;
   const int ld = k | 1;
@@ -218,14 +219,36 @@ __global__ __launch_bounds__(256) void k_de_prep(const double* __restrict__ C, i
   }
 }
 
+struct DePrepLaunch {
+  const double* C;
+  int ldc, transC;
+  const double *W, *FX;
+  int k, q;
+  double *CW, *coef;
+};
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_de_prep(DePrepLaunch L) {
+  de_prep_body<kLds>(L.C, L.ldc, L.transC, L.W, L.FX, L.k, L.q, L.CW, L.coef);
+}
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_de_prep_batch(const unsigned char* __restrict__ tab,
+                                                       const int* __restrict__ start, int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  const DePrepLaunch& L = *reinterpret_cast<const DePrepLaunch*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
+  de_prep_body<kLds>(L.C, L.ldc, L.transC, L.W, L.FX, L.k, L.q, L.CW, L.coef);
+}
+
 hipError_t launch_de_prep(const double* C, int ldc, int transC, const double* W, const double* FX, int k, int q,
                           double* CW, double* coef, hipStream_t st) {
   if (k < 1 || k > kMaxAug || q < 0 || q > kMaxForce) return hipErrorInvalidValue;
-  if (k <= 64)
-    hipLaunchKernelGGL(k_de_prep<true>, dim3(1), dim3(256), sizeof(double) * 2 * (size_t)(k | 1) * k, st, C, ldc,
-                       transC, W, FX, k, q, CW, coef);
+  const DePrepLaunch L{C, ldc, transC, W, FX, k, q, CW, coef};
+  const bool lds = k <= 64;
+  const size_t bytes = lds ? sizeof(double) * 2 * (size_t)(k | 1) * k : 0;
+  if (record_launch(KID_DE_PREP, lds ? 1 : 0, 1, 256, bytes, L)) return hipSuccess;
+  if (lds)
+    hipLaunchKernelGGL(k_de_prep<true>, dim3(1), dim3(256), bytes, st, L);
   else
-    hipLaunchKernelGGL(k_de_prep<false>, dim3(1), dim3(256), 0, st, C, ldc, transC, W, FX, k, q, CW, coef);
+    hipLaunchKernelGGL(k_de_prep<false>, dim3(1), dim3(256), 0, st, L);
   return hipGetLastError();
 }
 
@@ -264,7 +287,7 @@ __device__ __forceinline__ void rk4_power(double z, double h, int64_t N, double&
 // entry advanced in closed form, S1 = Wa Sbar Wb'.  kLds: the bases and the
 // work matrices sit in LDS (k, l <= 64), else in P.work.
 template <bool kLds>
-__global__ __launch_bounds__(1024) void k_s_substep(SStepArgs P) {
+__device__ __forceinline__ void s_substep_body(const SStepArgs& P) {
   extern __shared__ double sm[];
   const int k = P.k, l = P.l, q = P.q, t = threadIdx.x, nt = blockDim.x;
   const double* Wa = P.Wa;
@@ -339,14 +362,26 @@ __global__ __launch_bounds__(1024) void k_s_substep(SStepArgs P) {
   }
 }
 
+template <bool kLds>
+__global__ __launch_bounds__(1024) void k_s_substep(SStepArgs P) {
+  s_substep_body<kLds>(P);
+}
+template <bool kLds>
+__global__ __launch_bounds__(1024) void k_s_substep_batch(const unsigned char* __restrict__ tab,
+                                                          const int* __restrict__ start, int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  s_substep_body<kLds>(*reinterpret_cast<const SStepArgs*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes));
+}
+
 hipError_t launch_s_substep(const SStepArgs& P, hipStream_t st) {
   if (P.k < 1 || P.l < 1 || P.k > kMaxAug || P.l > kMaxAug || P.q < 0 || P.q > kMaxForce || !P.work)
     return hipErrorInvalidValue;
   if (P.k <= 64 && P.l <= 64) {
     const size_t lds = sizeof(double) * ((size_t)(P.k | 1) * P.k + (size_t)P.l * P.l + 2 * (size_t)P.k * P.l +
                                          (size_t)(P.k + P.l) * P.q);
-    hipLaunchKernelGGL(k_s_substep<true>, dim3(1), dim3(P.k * P.l > 1024 ? 1024 : 256), lds, st, P);
-  } else {
+    const unsigned th = P.k * P.l > 1024 ? 1024 : 256;
+    if (!record_launch(KID_SSTEP, 1, 1, th, lds, P)) hipLaunchKernelGGL(k_s_substep<true>, dim3(1), dim3(th), lds, st, P);
+  } else if (!record_launch(KID_SSTEP, 0, 1, 1024, 0, P)) {
     hipLaunchKernelGGL(k_s_substep<false>, dim3(1), dim3(1024), 0, st, P);
   }
   return hipGetLastError();
@@ -411,12 +446,27 @@ hipError_t launch_eig_tridiag_const(double* Q, double* lam, int64_t n, double d,
 
 // zhat_ij <- Ra zhat_ij + Qb bhat_ij with a = lambda_i + mu_j, bhat = fbar coef
 // (fbar n x q, coef q x k): N RK4 steps of the decoupled entries
-__global__ void k_spectral_flow(double* __restrict__ Z, int64_t n, int k, const double* __restrict__ lam,
-                                const double* __restrict__ mu, const double* __restrict__ fbar,
-                                const double* __restrict__ coef, int q, double h, int64_t N) {
+struct FlowLaunch {
+  double* Z;
+  int64_t n;
+  int k;
+  const double *lam, *mu, *fbar, *coef;
+  int q;
+  double h;
+  int64_t N;
+};
+__device__ __forceinline__ void spectral_flow_body(const FlowLaunch& L, int bid, int nblk) {
+  double* __restrict__ Z = L.Z;
+  const int64_t n = L.n;
+  const int k = L.k, q = L.q;
+  const double* __restrict__ lam = L.lam;
+  const double* __restrict__ mu = L.mu;
+  const double* __restrict__ fbar = L.fbar;
+  const double* __restrict__ coef = L.coef;
+  const double h = L.h;
+  const int64_t N = L.N;
   const int64_t total = n * (int64_t)k;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t idx = bid * (int64_t)blockDim.x + threadIdx.x; idx < total; idx += (int64_t)nblk * blockDim.x) {
     const int64_t i = idx % n;
     const int j = (int)(idx / n);
     double b = 0.0;
@@ -427,12 +477,22 @@ __global__ void k_spectral_flow(double* __restrict__ Z, int64_t n, int k, const 
   }
 }
 
+__global__ void k_spectral_flow(FlowLaunch L) { spectral_flow_body(L, blockIdx.x, gridDim.x); }
+__global__ void k_spectral_flow_batch(const unsigned char* __restrict__ tab, const int* __restrict__ start,
+                                      int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  spectral_flow_body(*reinterpret_cast<const FlowLaunch*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes), blockIdx.x - start[i],
+                     start[i + 1] - start[i]);
+}
+
 hipError_t launch_spectral_flow(double* Z, int64_t n, int k, const double* lam, const double* mu, const double* fbar,
                                 const double* coef, int q, double h, int64_t N, hipStream_t st) {
   if (n < 1 || k < 1 || q < 0 || q > kMaxForce || N < 1) return hipErrorInvalidValue;
   const int64_t total = n * (int64_t)k;
   const int grid = (int)std::min<int64_t>((total + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_spectral_flow, dim3(grid), dim3(256), 0, st, Z, n, k, lam, mu, fbar, coef, q, h, N);
+  const FlowLaunch L{Z, n, k, lam, mu, fbar, coef, q, h, N};
+  if (record_launch(KID_FLOW, 0, (unsigned)grid, 256, 0, L)) return hipSuccess;
+  hipLaunchKernelGGL(k_spectral_flow, dim3(grid), dim3(256), 0, st, L);
   return hipGetLastError();
 }
 
@@ -449,9 +509,8 @@ struct ShiftArgs {
   double *out, *shift, *ev;
 };
 
-// workgroup b: the shifted matrix of problem b
-__global__ __launch_bounds__(256) void k_shift_sym(ShiftArgs A0, ShiftArgs A1) {
-  const ShiftArgs A = blockIdx.x == 0 ? A0 : A1;
+// the shifted matrix of one problem (one workgroup)
+__device__ __forceinline__ void shift_sym_body(const ShiftArgs& A) {
   __shared__ double s_max[256];
   const int t = threadIdx.x;
   const int k = A.k;
@@ -489,9 +548,28 @@ __global__ __launch_bounds__(256) void k_shift_sym(ShiftArgs A0, ShiftArgs A1) {
   if (t == 0) *A.shift = sg;
 }
 
-__global__ void k_unshift(ShiftArgs A0, ShiftArgs A1) {
-  const ShiftArgs A = blockIdx.x == 0 ? A0 : A1;
+__device__ __forceinline__ void unshift_body(const ShiftArgs& A) {
   for (int i = threadIdx.x; i < A.k; i += blockDim.x) A.ev[i] -= *A.shift;
+}
+
+// workgroup b: problem b; the batched forms take one pair per record
+struct ShiftPair {
+  ShiftArgs A0, A1;
+};
+__global__ __launch_bounds__(256) void k_shift_sym(ShiftArgs A0, ShiftArgs A1) {
+  shift_sym_body(blockIdx.x == 0 ? A0 : A1);
+}
+__global__ void k_unshift(ShiftArgs A0, ShiftArgs A1) { unshift_body(blockIdx.x == 0 ? A0 : A1); }
+__global__ __launch_bounds__(256) void k_shift_sym_batch(const unsigned char* __restrict__ tab,
+                                                         const int* __restrict__ start, int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  const ShiftPair& S = *reinterpret_cast<const ShiftPair*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
+  shift_sym_body(blockIdx.x - start[i] == 0 ? S.A0 : S.A1);
+}
+__global__ void k_unshift_batch(const unsigned char* __restrict__ tab, const int* __restrict__ start, int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  const ShiftPair& S = *reinterpret_cast<const ShiftPair*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
+  unshift_body(blockIdx.x - start[i] == 0 ? S.A0 : S.A1);
 }
 
 hipError_t launch_sym_eig(const SymEigArgs* P, int count, hipStream_t st) {
@@ -517,12 +595,36 @@ hipError_t launch_sym_eig(const SymEigArgs* P, int count, hipStream_t st) {
     sa[1] = sa[0];
     sv[1] = sv[0];
   }
-  hipLaunchKernelGGL(k_shift_sym, dim3(count), dim3(256), 0, st, sa[0], sa[1]);
+  const ShiftPair sp{sa[0], sa[1]};
+  if (!record_launch(KID_SHIFT, 0, (unsigned)count, 256, 0, sp))
+    hipLaunchKernelGGL(k_shift_sym, dim3(count), dim3(256), 0, st, sa[0], sa[1]);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = count == 1 ? launch_jacobi_svd(sv[0], st) : launch_jacobi_svd2(sv[0], sv[1], st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_unshift, dim3(count), dim3(128), 0, st, sa[0], sa[1]);
+  if (!record_launch(KID_UNSHIFT, 0, (unsigned)count, 128, 0, sp))
+    hipLaunchKernelGGL(k_unshift, dim3(count), dim3(128), 0, st, sa[0], sa[1]);
+  return hipGetLastError();
+}
+
+hipError_t launch_de_batched(int kid, int variant, const void* tab, const int* start, int count, unsigned total,
+                             unsigned block, size_t lds, hipStream_t st) {
+  const unsigned char* t = static_cast<const unsigned char*>(tab);
+  const dim3 g(total), b(block);
+  switch (kid) {
+    case KID_DE_PREP:
+      if (variant) hipLaunchKernelGGL(k_de_prep_batch<true>, g, b, lds, st, t, start, count);
+      else hipLaunchKernelGGL(k_de_prep_batch<false>, g, b, lds, st, t, start, count);
+      break;
+    case KID_SSTEP:
+      if (variant) hipLaunchKernelGGL(k_s_substep_batch<true>, g, b, lds, st, t, start, count);
+      else hipLaunchKernelGGL(k_s_substep_batch<false>, g, b, lds, st, t, start, count);
+      break;
+    case KID_FLOW: hipLaunchKernelGGL(k_spectral_flow_batch, g, b, lds, st, t, start, count); break;
+    case KID_SHIFT: hipLaunchKernelGGL(k_shift_sym_batch, g, b, lds, st, t, start, count); break;
+    case KID_UNSHIFT: hipLaunchKernelGGL(k_unshift_batch, g, b, lds, st, t, start, count); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 // sequence of stream-ordered launches with no host synchronisation (except the
 // rank read-back of the rank-adaptive step).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -41,11 +42,63 @@ struct Failure {
     if (!(cond)) throw Failure{LRI_EINVAL, std::string(msg)}; \
   } while (0)
 
-__global__ void k_set_diag(double* S, int lds, int rcap, const double* sigma, int r) {
-  for (int e = threadIdx.x; e < rcap * rcap; e += blockDim.x) {
-    int i = e % rcap, j = e / rcap;
-    S[i + j * lds] = (i == j && i < r) ? sigma[i] : 0.0;
+// A = I (k x k, ld k)
+__global__ void k_eye(double* A, int k) {
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) A[e] = (e % k == e / k) ? 1.0 : 0.0;
+}
+
+// S (rcap x rcap, ld lds) = diag(sigma[0..r-1], 0, ...); one workgroup
+struct SetDiagArgs {
+  double* S;
+  const double* sigma;
+  int lds, rcap, r;
+};
+__device__ __forceinline__ void set_diag_body(const SetDiagArgs& a) {
+  for (int e = threadIdx.x; e < a.rcap * a.rcap; e += blockDim.x) {
+    int i = e % a.rcap, j = e / a.rcap;
+    a.S[i + j * a.lds] = (i == j && i < a.r) ? a.sigma[i] : 0.0;
   }
+}
+__global__ void k_set_diag(SetDiagArgs a) { set_diag_body(a); }
+
+// dst[0..n) = src[0..n) (FP64 words), the workgroups striding over the range
+struct CopyArgs {
+  double* dst;
+  const double* src;
+  int64_t n;
+};
+constexpr int kCopyThreads = 256;
+__device__ __forceinline__ void copy_body(const CopyArgs& a, int bid, int nblk) {
+  for (int64_t i = (int64_t)bid * kCopyThreads + threadIdx.x; i < a.n; i += (int64_t)nblk * kCopyThreads)
+    a.dst[i] = a.src[i];
+}
+
+// the batched forms (argument blocks and workgroup offsets as in lri_kernels.h)
+__global__ __launch_bounds__(256) void k_set_diag_batch(const unsigned char* __restrict__ tab,
+                                                        const int* __restrict__ start, int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  set_diag_body(*reinterpret_cast<const SetDiagArgs*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes));
+}
+__global__ __launch_bounds__(kCopyThreads) void k_copy_batch(const unsigned char* __restrict__ tab,
+                                                             const int* __restrict__ start, int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  copy_body(*reinterpret_cast<const CopyArgs*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes),
+            blockIdx.x - start[i], start[i + 1] - start[i]);
+}
+
+// Per-member status words of a batch: out[i] = the new rank of member i's
+// rank-adaptive step, out[count + i] = its condition flags, which are cleared.
+struct MemberWords {
+  const int* r_new;
+  int* flags;
+};
+__global__ __launch_bounds__(256) void k_gather_status(const MemberWords* __restrict__ w, int* __restrict__ out,
+                                                       int count) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= count) return;
+  out[i] = *w[i].r_new;
+  out[count + i] = *w[i].flags;
+  *w[i].flags = 0;
 }
 
 struct Timed {
@@ -130,6 +183,9 @@ struct lri_ctx {
   std::vector<hipEvent_t> pool;
   double tot_ms[LRI_KERNEL_COUNT] = {};
   int64_t launches[LRI_KERNEL_COUNT] = {};
+  std::vector<float> samples[LRI_KERNEL_COUNT];  // per-launch durations (ms)
+  int timing_every = 1;                           // time one launch in `timing_every` per family
+  int64_t seen[LRI_KERNEL_COUNT] = {};
   std::string err;
   // matrix-ODE operator F(Y) = A Y + Y B' + G H' (allocated on first use)
   bool op_set = false;
@@ -144,6 +200,16 @@ struct lri_ctx {
   bool spec[2] = {false, false};
   double *specQ[2] = {nullptr, nullptr}, *specLam[2] = {nullptr, nullptr}, *specF[2] = {nullptr, nullptr};
   bool spec_alias = false;
+  // Resident eigenbasis (both sides constant-coefficient, linear operator): the
+  // device state holds U^ = Qa' U and V^ = Qb' V, and the operator is its
+  // diagonal form (A -> diag(lam_a), B -> diag(lam_b), G -> Qa' G, H -> Qb' H),
+  // so a step needs no n x n transforms; the state is transformed at
+  // lri_set_state / lri_get_state.  res_saved keeps the operator's own arrays.
+  bool resident = false;
+  double* res_zero = nullptr;  // max(n, m) zeros: the off-diagonals of the diagonal forms
+  double* res_saved[6] = {};   // op_ad, op_ae, op_bd, op_be, op_G, op_H
+  int res_saved_per[2] = {0, 0};
+  std::vector<char> traj_resident;  // slot saved while resident
   // The Galerkin matrices U'AU, V'BV, U'G, H'V and their eigendecompositions in
   // the DE work area belong to the current U and V: a BUG step ends with the
   // S sub-step's ones, which are the next BUG step's K/L ones. Any other entry
@@ -161,6 +227,18 @@ struct lri_ctx {
     hipGraphExec_t exec;
   };
   std::vector<RaGraph> ra_graphs;
+  // bumped whenever recorded or captured launches of this context go stale (a
+  // new operator or basis): the merged launches of a batch key on it
+  uint64_t epoch = 0;
+  // the context's own stream; a batch with merged launches points `stream` at
+  // its shared stream instead
+  hipStream_t own_stream = nullptr;
+  // device-resident trajectory (the deep copies of update_sol!, primitives.jl:82-90):
+  // slot k holds U (n x rcap), S (rcap x rcap) and V (m x rcap) of one saved state
+  double* traj = nullptr;
+  int64_t traj_slots = 0;
+  std::vector<int64_t> traj_rank;  // rank of each slot (0: never saved)
+  int64_t traj_stride() const { return n * rcap + rcap * rcap + m * rcap; }
   // quadratic term -(D Y) o Y (D tridiagonal on the n rows), its tensors
   // T = V x V x V and W = (D U) x U x U, and their scratch
   bool op_quad = false;
@@ -203,7 +281,7 @@ struct lri_ctx {
   }
   template <class F>
   void timed(int family, F&& f) {
-    if (!timing || (timing_passes_only && family == LRI_KERNEL_SMALL)) {
+    if (!timing || (timing_passes_only && family == LRI_KERNEL_SMALL) || (seen[family]++ % timing_every) != 0) {
       HIPCHECK(f());
       return;
     }
@@ -220,6 +298,7 @@ struct lri_ctx {
       HIPCHECK(hipEventElapsedTime(&ms, t.a, t.b));
       tot_ms[t.family] += ms;
       launches[t.family] += 1;
+      samples[t.family].push_back(ms);
       pool.push_back(t.a);
       pool.push_back(t.b);
     }
@@ -256,6 +335,7 @@ struct lri_ctx {
       A.pk_f64 = forN ? pkN_f64 : pkT_f64;
     } else if (mode == PASS_F32) {
       A.pk_f32 = forN ? pkN_f32 : pkT_f32;
+      A.pk_layout = forN ? 0 : 1;
     } else {
       A.pk_hi = forN ? pkN_hi : pkT_hi;
       A.pk_lo = forN ? pkN_lo : pkT_lo;
@@ -276,9 +356,26 @@ struct lri_ctx {
     if (!forN && (pkT_mode != mode || pkT_rb != rb)) pack(U, n, n, r, false);
   }
 
-  // The product passes over dA = A[kn] - A[kp] with the currently packed B
-  // (kp < 0: over A[kn] itself).
-  int passN(int64_t kp, int64_t kn, int64_t cols, void* part) {
+  // Reduction chunks chosen by lri_autotune_passes for this context, per pass
+  // shape; the other shapes take plan_pass's default.
+  struct TunedPass {
+    int mode;
+    bool transposed;
+    int64_t out_rows, red_len;
+    int rb;
+    int64_t chunk;
+  };
+  std::vector<TunedPass> tuned;
+  int64_t plan(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int* splits, int64_t* chunk) {
+    for (const TunedPass& t : tuned)
+      if (t.mode == mode && t.transposed == transposed && t.out_rows == out_rows && t.red_len == red_len && t.rb == rb)
+        return plan_pass_chunk(out_rows, red_len, rb, t.chunk, splits, chunk);
+    return plan_pass(mode, transposed, out_rows, red_len, rb, splits, chunk);
+  }
+
+  // The arguments of a pass over dA = A[kn] - A[kp] (kp < 0: over A[kn]
+  // itself) with the currently packed B of width `cols` (splits not planned).
+  PassArgs pass_args(bool transposed, int64_t kp, int64_t kn, int64_t cols, void* part) {
     const int mode = pass_mode();
     PassArgs a;
     a.anext = snaps[kn];
@@ -288,43 +385,119 @@ struct lri_ctx {
     a.n = n;
     a.m = m;
     a.rb = rb_for(mode, cols);
-    REQUIRE(pkN_mode == mode && pkN_rb == a.rb, "internal: stale packed V panel");
-    if (mode == PASS_F64) a.B = pkN_f64;
-    else if (mode == PASS_F32) a.B = pkN_f32;
-    else {
-      a.B = pkN_hi;
-      a.B2 = pkN_lo;
-      a.ldb = ldm_pk;
+    if (!transposed) {
+      REQUIRE(pkN_mode == mode && pkN_rb == a.rb, "internal: stale packed V panel");
+      if (mode == PASS_F64) a.B = pkN_f64;
+      else if (mode == PASS_F32) a.B = pkN_f32;
+      else {
+        a.B = pkN_hi;
+        a.B2 = pkN_lo;
+        a.ldb = ldm_pk;
+      }
+    } else {
+      REQUIRE(pkT_mode == mode && pkT_rb == a.rb, "internal: stale packed U panel");
+      if (mode == PASS_F64) a.B = pkT_f64;
+      else if (mode == PASS_F32) a.B = pkT_f32;
+      else {
+        a.B = pkT_hi;
+        a.B2 = pkT_lo;
+        a.ldb = ldn_pk;
+      }
     }
     a.part = part;
-    int64_t need = plan_pass(mode, false, n, m, a.rb, &a.splits, &a.chunk);
+    return a;
+  }
+  // The product passes with the currently packed B; they return the split count.
+  int passN(int64_t kp, int64_t kn, int64_t cols, void* part) {
+    const int mode = pass_mode();
+    PassArgs a = pass_args(false, kp, kn, cols, part);
+    int64_t need = plan(mode, false, n, m, a.rb, &a.splits, &a.chunk);
     REQUIRE((size_t)need * (mode == PASS_F64 ? 8 : 4) <= part_bytes, "internal: partial buffer too small");
     timed(LRI_KERNEL_PASS_N, [&] { return launch_passN(mode, a, stream); });
     return a.splits;
   }
   int passT(int64_t kp, int64_t kn, int64_t cols, void* part) {
     const int mode = pass_mode();
-    PassArgs a;
-    a.anext = snaps[kn];
-    a.aprev = kp >= 0 ? snaps[kp] : nullptr;
-    a.dtype = sdtype;
-    a.lda = sld;
-    a.n = n;
-    a.m = m;
-    a.rb = rb_for(mode, cols);
-    REQUIRE(pkT_mode == mode && pkT_rb == a.rb, "internal: stale packed U panel");
-    if (mode == PASS_F64) a.B = pkT_f64;
-    else if (mode == PASS_F32) a.B = pkT_f32;
-    else {
-      a.B = pkT_hi;
-      a.B2 = pkT_lo;
-      a.ldb = ldn_pk;
-    }
-    a.part = part;
-    int64_t need = plan_pass(mode, true, m, n, a.rb, &a.splits, &a.chunk);
+    PassArgs a = pass_args(true, kp, kn, cols, part);
+    int64_t need = plan(mode, true, m, n, a.rb, &a.splits, &a.chunk);
     REQUIRE((size_t)need * (mode == PASS_F64 ? 8 : 4) <= part_bytes, "internal: partial buffer too small");
     timed(LRI_KERNEL_PASS_T, [&] { return launch_passT(mode, a, stream); });
     return a.splits;
+  }
+
+  // The pass autotuner: for pass N and pass T over dA = A[kn] - A[kp] at the
+  // current rank and precision, time every candidate reduction chunk
+  // (pass_chunk_candidates: about 128 .. 3072 workgroups) over `reps` launches
+  // after one untimed launch, and keep the fastest for this context.
+  // report[4 d ..] for d = 0 (pass N), 1 (pass T): chunk, splits, mean
+  // microseconds of the chosen chunk, mean microseconds of the default chunk.
+  void autotune_passes(int64_t kp, int64_t kn, int reps, double* report) {
+    const int mode = pass_mode();
+    const int rb = rb_for(mode, r);
+    const size_t es = mode == PASS_F64 ? 8 : 4;
+    ensure_packed(true);
+    ensure_packed(false);
+    for (int dir = 0; dir < 2; ++dir) {
+      const bool tr = dir == 1;
+      const int64_t out_rows = tr ? m : n, red_len = tr ? n : m;
+      int64_t cand[16];
+      const int nc = pass_chunk_candidates(mode, tr, out_rows, red_len, rb, cand, 16);
+      size_t need = 0;
+      for (int k = 0; k < nc; ++k) {
+        int sp;
+        int64_t ch;
+        need = std::max(need, es * (size_t)plan_pass_chunk(out_rows, red_len, rb, cand[k], &sp, &ch));
+      }
+      if (need > part_bytes) {  // room for the largest candidate's partials
+        HIPCHECK(hipStreamSynchronize(stream));
+        HIPCHECK(hipFree(partN));
+        HIPCHECK(hipFree(partT));
+        partN = partT = nullptr;
+        HIPCHECK(hipMalloc(&partN, need));
+        HIPCHECK(hipMalloc(&partT, need));
+        part_bytes = need;
+      }
+      tuned.erase(std::remove_if(tuned.begin(), tuned.end(),
+                                 [&](const TunedPass& t) {
+                                   return t.mode == mode && t.transposed == tr && t.out_rows == out_rows &&
+                                          t.red_len == red_len && t.rb == rb;
+                                 }),
+                  tuned.end());
+      int s0;
+      int64_t ch0;
+      plan_pass(mode, tr, out_rows, red_len, rb, &s0, &ch0);
+      PassArgs a = pass_args(tr, kp, kn, r, tr ? partT : partN);
+      hipEvent_t e0 = ev(), e1 = ev();
+      double best = 0.0, dflt = 0.0;
+      int64_t bestc = ch0;
+      for (int k = 0; k < nc; ++k) {
+        plan_pass_chunk(out_rows, red_len, rb, cand[k], &a.splits, &a.chunk);
+        auto run = [&] { HIPCHECK(tr ? launch_passT(mode, a, stream) : launch_passN(mode, a, stream)); };
+        run();
+        HIPCHECK(hipEventRecord(e0, stream));
+        for (int rep = 0; rep < reps; ++rep) run();
+        HIPCHECK(hipEventRecord(e1, stream));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / reps;
+        if (a.chunk == ch0) dflt = us;
+        if (k == 0 || us < best) {
+          best = us;
+          bestc = a.chunk;
+        }
+      }
+      pool.push_back(e0);
+      pool.push_back(e1);
+      tuned.push_back(TunedPass{mode, tr, out_rows, red_len, rb, bestc});
+      int sb;
+      int64_t cb;
+      plan_pass_chunk(out_rows, red_len, rb, bestc, &sb, &cb);
+      report[4 * dir + 0] = (double)cb;
+      report[4 * dir + 1] = (double)sb;
+      report[4 * dir + 2] = best;
+      report[4 * dir + 3] = dflt;
+    }
   }
 
   // rows kernel reading split partials of the current pass mode
@@ -443,6 +616,47 @@ struct lri_ctx {
     rows(C);
   }
 
+  // After a KSL CholeskyQR2 of X (rows_ x rr) into Q: when a pivot was dropped
+  // (a rank-deficient K or L), the QR is recomputed from X by reorthogonalised
+  // Gram-Schmidt with an orthonormal completion, as Householder QR (qr! in
+  // projector_splitting.jl:137,149,174,186) returns it, and the products of its
+  // R are rewritten: M = R - Q' W (mt: transposed) and Z = R (zt: R'); the
+  // packed copy of Q for the next pass is updated.  One one-workgroup launch
+  // that returns at once when nothing was dropped.
+  void complete_dropped(const double* X, double* Q, int64_t rows_, int rr, bool pk_forN, double* M, int ldm, int mt,
+                        const double* W, int64_t ldw, double* Z, int ldz, int zt) {
+    CompleteArgs C;
+    C.X = X;
+    C.Q = Q;
+    C.rows = rows_;
+    C.k = rr;
+    C.Rinv1 = small->R1inv;
+    C.Rinv2 = small->R2inv;
+    if (pack_enabled) {
+      const int mode = pass_mode();
+      C.pk_cols = rb_for(mode, rr);
+      if (mode == PASS_F64) {
+        C.pk_f64 = pk_forN ? pkN_f64 : pkT_f64;
+      } else if (mode == PASS_F32) {
+        C.pk_f32 = pk_forN ? pkN_f32 : pkT_f32;
+        C.pk_layout = pk_forN ? 0 : 1;
+      } else {
+        C.pk_hi = pk_forN ? pkN_hi : pkT_hi;
+        C.pk_lo = pk_forN ? pkN_lo : pkT_lo;
+        C.pk_layout = pk_forN ? 0 : 1;
+      }
+    }
+    C.M = M;
+    C.ldm = ldm;
+    C.mt = mt;
+    C.W = W;
+    C.ldw = ldw;
+    C.Z = Z;
+    C.ldz = ldz;
+    C.zt = zt;
+    timed(LRI_KERNEL_SMALL, [&] { return launch_complete_columns(C, stream); });
+  }
+
   void check_step(int64_t kp, int64_t kn) {
     REQUIRE(r >= 1, "state not set (call lri_set_state)");
     REQUIRE(!snaps.empty(), "no snapshots (call lri_snapshots_reserve/upload or lri_snapshots_bind)");
@@ -458,6 +672,7 @@ struct lri_ctx {
     const bool fold = fold_chol(rr);
     EpiArgs c1;
     c1.kind = EPI_CHOL1;
+    c1.breakdown_is_error = 0;  // dropped columns are completed
     ensure_packed(true);
     const int sN = passN(kp, kn, rr, partN);
     with_parts(partN, [&](auto A) {  // K = U S + dA V ; P = dA V ; Gram -> R1
@@ -481,7 +696,9 @@ struct lri_ctx {
     });
     EpiArgs e;
     e.kind = EPI_STILDE;  // S~ = R - U1' P
+    e.breakdown_is_error = 0;
     cholqr_tail(K, K2, n, rr, U, false, P, rr, e, 0, 2, fold ? &c1 : nullptr);
+    complete_dropped(K, U, n, rr, false, small->St, rr, 0, P, n, nullptr, 0, 0);  // S~ = R - U1' P
     const int sT = passT(kp, kn, rr, partT);
     with_parts(partT, [&](auto A) {  // L = V S~' + dA' U1 ; Gram -> R1
       A.rows = m;
@@ -502,7 +719,9 @@ struct lri_ctx {
       rows(A);
     });
     EpiArgs none;
+    none.breakdown_is_error = 0;
     cholqr_tail(L, L2, m, rr, V, true, nullptr, 0, none, 2, 2, fold ? &c1 : nullptr);  // V1, S = R'
+    complete_dropped(L, V, m, rr, true, nullptr, 0, 0, nullptr, 0, S, (int)rcap, 1);  // S = R'
   }
 
   // dual_LT_step! (projector_splitting.jl:166-189)
@@ -511,6 +730,7 @@ struct lri_ctx {
     const bool fold = fold_chol(rr);
     EpiArgs c1;
     c1.kind = EPI_CHOL1;
+    c1.breakdown_is_error = 0;  // dropped columns are completed
     ensure_packed(false);
     const int sT = passT(kp, kn, rr, partT);
     with_parts(partT, [&](auto A) {  // L = V S' + dA' U ; Q = dA' U
@@ -536,7 +756,9 @@ struct lri_ctx {
     EpiArgs e;
     e.kind = EPI_STILDE;  // S~ = (R - V1' Q)' = R' - U' dA V1
     e.transpose = 1;
+    e.breakdown_is_error = 0;
     cholqr_tail(L, L2, m, rr, V, true, Q, rr, e, 0, 2, fold ? &c1 : nullptr);
+    complete_dropped(L, V, m, rr, true, small->St, rr, 1, Q, m, nullptr, 0, 0);  // S~ = (R - V1' Q)'
     const int sN = passN(kp, kn, rr, partN);
     with_parts(partN, [&](auto A) {  // K = U S~ + dA V1
       A.rows = n;
@@ -556,7 +778,9 @@ struct lri_ctx {
       rows(A);
     });
     EpiArgs none;
+    none.breakdown_is_error = 0;
     cholqr_tail(K, K2, n, rr, U, false, nullptr, 0, none, 1, 2, fold ? &c1 : nullptr);  // U1, S = R
+    complete_dropped(K, U, n, rr, false, nullptr, 0, 0, nullptr, 0, S, (int)rcap, 0);  // S = R
   }
 
   // unconventional_step! (unconventional.jl:133-157)
@@ -695,7 +919,6 @@ struct lri_ctx {
   void greedy_twofactor(int64_t kn) {
     const int rr = (int)r;
     const int rb = rb_for(pass_mode(), rr);
-    const EpiArgs none = de_epi();
     ensure_packed(false);
     const int sT = passT(-1, kn, rr, partT);  // Z1 = X' U0
     with_parts(partT, [&](auto A) {
@@ -708,6 +931,44 @@ struct lri_ctx {
       set_pack_outputs(A, true, pass_mode(), rb);
       rows(A);
     });
+    greedy_polar(kn);
+  }
+
+  // greedy_step! on a MatrixHybridProblem (greedy_integrator.jl:73-82), the
+  // two-factor state U Z' with Z in the V slot: the Z integrator advances Z over
+  // dt with U held, Z' = f(Z, U, t) = F(U Z')' U -- for the operator descriptor
+  // the L sub-problem of the matrix-ODE steps, Z' = B Z + Z (U'AU) + H (G'U)
+  // (- Q(Z) with the tensor W = (D U) x U x U for the quadratic term), N RK4
+  // steps -- then X = A[kn] and U1 = the polar factor of X Z1 (greedy_polar).
+  void hybrid_greedy(int64_t kn, double dt, int64_t nsub) {
+    const int rr = (int)r;
+    const double h = dt / (double)nsub;
+    hipLaunchKernelGGL(k_eye, dim3(1), dim3(256), 0, stream, de->Snew, rr);
+    HIPCHECK(hipGetLastError());
+    pack_enabled = false;
+    if (op_quad) {
+      quad_galerkin(U, true, rr);                      // U'AU, G'U, W(U)
+      quad_init(V, m, rr, de->Snew, rr, 1, L);         // L = Z
+      quad_rows(false, rr, L, L2, Q, h, nsub);
+      HIPCHECK(hipMemcpyAsync(V, L, sizeof(double) * m * rr, hipMemcpyDeviceToDevice, stream));
+    } else {
+      de_galerkin(U, true, rr, de->Ga, de->gaT);       // U'AU, G'U
+      de_eig(de->Ga, rr, de->Wa, de->alpha);
+      de_substep(V, false, rr, de->Snew, rr, 1, de->Wa, de->alpha, de->gaT, h, nsub, V, L2, de_epi());
+    }
+    pack_enabled = true;
+    pkN_mode = pkT_mode = -1;
+    pack(V, m, m, rr, true);
+    greedy_polar(kn);
+  }
+
+  // U1 = Q P' for the SVD X Z1 = Q Sigma P' (greedy_integrator.jl:79-81,89-91), Z1
+  // in V and packed for pass N: X Z1 = Qc R (shifted CholeskyQR3), R = Ur Sigma P'
+  // (Jacobi), U1 = Qc (Ur P').
+  void greedy_polar(int64_t kn) {
+    const int rr = (int)r;
+    const int rb = rb_for(pass_mode(), rr);
+    const EpiArgs none = de_epi();
     const int sN = passN(-1, kn, rr, partN);  // X Z1
     with_parts(partN, [&](auto A) {
       A.rows = n;
@@ -1674,14 +1935,20 @@ struct lri_ctx {
   // Captured graphs bake in the operator's buffers and its control flow
   // (analytic eigenbasis or not, the quadratic term): a new operator drops them.
   void drop_graphs() {
+    ++epoch;
     HIPCHECK(hipStreamSynchronize(stream));
     for (auto& g : ra_graphs) (void)hipGraphExecDestroy(g.exec);
     ra_graphs.clear();
   }
   int64_t ra_rotate() {
-    const int k2 = ra_k2;
     HIPCHECK(hipStreamSynchronize(stream));
     const int r1 = *r_host;
+    ra_check_rank(r1);
+    ra_apply(ra_k2, r1);
+    r = r1;
+    return r1;
+  }
+  void ra_check_rank(int r1) {
     if (r1 > rcap) {
       // the state is left at the step's start; the panels hold the augmented bases
       pkN_mode = pkT_mode = -1;
@@ -1691,7 +1958,10 @@ struct lri_ctx {
                                      std::to_string(LRI_MAX_RANK) + ")"};
     }
     REQUIRE(r1 >= 1, "internal: rank-adaptive step returned an invalid rank");
-    // U = Uhat Ush[:, :r1], V = Vhat Vsh[:, :r1], S = diag(sigma[:r1])
+  }
+  // U = Uhat Ush[:, :r1], V = Vhat Vsh[:, :r1], S = diag(sigma[:r1]) (the
+  // launches only: the caller sets r)
+  void ra_apply(int k2, int r1) {
     for (int side = 0; side < 2; ++side) {
       RowArgs<double> A;
       A.rows = side == 0 ? n : m;
@@ -1706,10 +1976,19 @@ struct lri_ctx {
       set_pack_outputs(A, side == 1, pass_mode(), rb_for(pass_mode(), r1));
       rows(A);
     }
-    hipLaunchKernelGGL(k_set_diag, dim3(1), dim3(256), 0, stream, S, (int)rcap, (int)rcap, small->sigma, r1);
-    HIPCHECK(hipGetLastError());
-    r = r1;
-    return r1;
+    const SetDiagArgs d{S, small->sigma, (int)rcap, (int)rcap, r1};
+    if (!record_launch(KID_SET_DIAG, 0, 1, 256, 0, d)) {
+      hipLaunchKernelGGL(k_set_diag, dim3(1), dim3(256), 0, stream, d);
+      HIPCHECK(hipGetLastError());
+    }
+  }
+  // dst[0..count) = src[0..count) on the stream (recorded while a recorder is installed)
+  void dcopy(double* dst, const double* src, int64_t count) {
+    if (count <= 0) return;
+    const CopyArgs a{dst, src, count};
+    const unsigned grid = (unsigned)std::min<int64_t>((count + 4 * kCopyThreads - 1) / (4 * kCopyThreads), 256);
+    if (!record_launch(KID_COPY, 0, grid, kCopyThreads, 0, a))
+      HIPCHECK(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToDevice, stream));
   }
 
   // ------------------------------------------------------------ matrix ODE
@@ -1860,6 +2139,19 @@ struct lri_ctx {
   // in the analytic eigenbasis (Zhat = Q' Z, closed-form flow, Z = Q Zhat, with
   // Zhat in the P / Q work buffers), the others in one k_rk4_columns launch.
   void de_run(const RK4ColArgs& P0, const RK4ColArgs& P1 = RK4ColArgs()) {
+    if (resident) {
+      // the state lives in the eigenbasis: the closed-form flow, no transforms
+      for (const RK4ColArgs* Pp : {&P0, &P1}) {
+        const RK4ColArgs& Pa = *Pp;
+        if (Pa.ncols < 1) continue;
+        REQUIRE(Pa.Y == Pa.X, "internal: resident flow in place");
+        timed(LRI_KERNEL_DE, [&] {
+          return launch_spectral_flow(Pa.Y, Pa.n, Pa.ncols, specLam[Pa.side], Pa.mu, specF[Pa.side], Pa.coef, Pa.q,
+                                      Pa.h, Pa.nsub, stream);
+        });
+      }
+      return;
+    }
     if (P0.ncols > 0 && P1.ncols > 0 && spec[P0.side] && spec[P1.side] && specQ[P0.side] == specQ[P1.side] &&
         P0.n == P1.n && P1.X == P0.X + P0.n * P0.ncols && P1.Y == P1.X && P0.Y == P0.X &&
         P0.ncols + P1.ncols <= 64) {
@@ -2798,11 +3090,11 @@ struct lri_ctx {
     de_run(Pk, Pl);
     de_finish(true, rr, de->Wb, K, K2, none);
     de_finish(false, rr, de->Wa, L, lscr, none);
-    HIPCHECK(hipMemcpyAsync(Uhat, U, sizeof(double) * n * rr, hipMemcpyDeviceToDevice, stream));
+    dcopy(Uhat, U, n * rr);
     EpiArgs eM;
     eM.kind = EPI_RA_M;
     complete_basis(K, U, n, rr, K2, K2 + n * rcap, Uhat + n * rr, eM);
-    HIPCHECK(hipMemcpyAsync(Vhat, V, sizeof(double) * m * rr, hipMemcpyDeviceToDevice, stream));
+    dcopy(Vhat, V, m * rr);
     EpiArgs eN;
     eN.kind = EPI_RA_N;  // Sinit = M S0 N' (2r x 2r)
     eN.S = S;
@@ -2866,6 +3158,59 @@ struct lri_ctx {
     HIPCHECK(hipStreamSynchronize(stream));
   }
 
+  // X (rows x k, ld rows) <- op(Q) X for a side's analytic eigenbasis, through tmp
+  void basis_transform(int side, bool trans, double* X, int k, double* tmp) {
+    const int64_t rn = side == 0 ? n : m;
+    spectral_gemm(trans, specQ[side], rn, X, k, tmp);
+    HIPCHECK(hipMemcpyAsync(X, tmp, sizeof(double) * rn * k, hipMemcpyDeviceToDevice, stream));
+  }
+  void enter_resident() {
+    if (resident) return;
+    drop_graphs();
+    REQUIRE(op_set && spec[0] && spec[1] && !op_quad,
+            "the resident eigenbasis needs a linear operator with constant coefficients on both sides");
+    if (!res_zero) {
+      HIPCHECK(hipMalloc((void**)&res_zero, sizeof(double) * std::max(n, m)));
+      HIPCHECK(hipMemset(res_zero, 0, sizeof(double) * std::max(n, m)));
+    }
+    double** ops[6] = {&op_ad, &op_ae, &op_bd, &op_be, &op_G, &op_H};
+    for (int i = 0; i < 6; ++i) res_saved[i] = *ops[i];
+    res_saved_per[0] = op_aper;
+    res_saved_per[1] = op_bper;
+    op_ad = specLam[0];
+    op_ae = res_zero;
+    op_bd = specLam[1];
+    op_be = res_zero;
+    op_G = specF[0];
+    op_H = specF[1];
+    op_aper = op_bper = 0;
+    if (r >= 1) {
+      basis_transform(0, true, U, (int)r, K2);
+      basis_transform(1, true, V, (int)r, L2);
+    }
+    resident = true;
+    pkN_mode = pkT_mode = -1;
+  }
+  void leave_resident() {
+    if (!resident) return;
+    drop_graphs();
+    double** ops[6] = {&op_ad, &op_ae, &op_bd, &op_be, &op_G, &op_H};
+    for (int i = 0; i < 6; ++i) *ops[i] = res_saved[i];
+    op_aper = res_saved_per[0];
+    op_bper = res_saved_per[1];
+    resident = false;
+    if (r >= 1) {
+      basis_transform(0, false, U, (int)r, K2);
+      basis_transform(1, false, V, (int)r, L2);
+    }
+    pkN_mode = pkT_mode = -1;
+  }
+  // U = Qa U^, V = Qb V^ of the resident state (r columns) into K2 and L2
+  void resident_out(const double* Uh, const double* Vh, int rr) {
+    spectral_gemm(false, specQ[0], n, Uh, rr, K2);
+    spectral_gemm(false, specQ[1], m, Vh, rr, L2);
+  }
+
   void check_de(double dt, int64_t nsub) {
     REQUIRE(r >= 1, "state not set (call lri_set_state)");
     REQUIRE(op_set, "no operator (call lri_set_operator_lyapunov)");
@@ -2915,6 +3260,12 @@ void free_ctx(lri_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->resident) {  // the operator's own arrays (the resident ones alias specLam / specF)
+    double** ops[6] = {&c->op_ad, &c->op_ae, &c->op_bd, &c->op_be, &c->op_G, &c->op_H};
+    for (int i = 0; i < 6; ++i) *ops[i] = c->res_saved[i];
+    c->resident = false;
+  }
+  if (c->res_zero) (void)hipFree(c->res_zero);
   for (void* p : c->owned) (void)hipFree(p);
   void* bufs[] = {c->U,      c->V,      c->S,      c->U2,      c->V2,      c->K,      c->K2,     c->P,
                   c->L,      c->L2,     c->Q,      c->Uhat,    c->Vhat,    c->pkN_f64, c->pkT_f64, c->pkN_f32,
@@ -2933,28 +3284,295 @@ void free_ctx(lri_ctx* c) {
   if (c->radj) (void)hipFree(c->radj);
   if (c->qadj) (void)hipFree(c->qadj);
   if (c->qadj_rows) (void)hipFree(c->qadj_rows);
+  if (c->traj) (void)hipFree(c->traj);
   for (auto& g : c->ra_graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& t : c->pending) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------- batches
-// Independent problems of uneven n, m and rank, one context (and stream) each.
-// A batch step queues every member's step before it waits for any of them, so
-// the members' kernels overlap on the device.
+namespace {
+
+// Installs a recorder on the calling thread for its lifetime.
+struct RecorderScope {
+  Recorder* prev;
+  explicit RecorderScope(Recorder* r) : prev(current_recorder()) { current_recorder() = r; }
+  ~RecorderScope() { current_recorder() = prev; }
+};
+
+// One phase of a merged batch step.  Each member's launches are recorded once
+// per key (the step's parameters, the member's rank and its epoch) and their
+// argument blocks kept at fixed slots of a device pool: member i owns slots
+// i * cap .. (i + 1) * cap - 1, launch p of its sequence slot i * cap + p.  The
+// merged list runs position p of every member's sequence as one launch per
+// (kernel, variant, threads), with the largest LDS size of its members; the
+// launches of one position touch disjoint members' buffers, and positions run
+// in order on one stream, so each member sees its own launches in its own order.
+struct MergedPhase {
+  using Key = std::array<double, 8>;
+  struct Launch {
+    int kid, variant;
+    unsigned block, total;
+    size_t lds;
+    int count;
+    size_t meta_off;  // its workgroup offsets and slots in `meta` (2 count + 1 ints)
+  };
+  int count = 0;
+  int cap = 0;
+  std::vector<std::vector<LaunchRec>> recs;
+  std::vector<Key> keys;
+  std::vector<char> have, dirty, part, plan_part;
+  unsigned char* pool = nullptr;       // device, count * cap argument blocks
+  unsigned char* pool_host = nullptr;  // pinned staging of the same layout
+  std::vector<Launch> launches;
+  std::vector<int> meta_vec;
+  int* meta = nullptr;       // device
+  int* meta_host = nullptr;  // pinned
+  size_t meta_cap = 0;       // ints
+  bool plan_stale = true;
+  uint64_t plan_gen = 0, seen_gen = ~0ull, exec_gen = ~0ull;
+  hipGraphExec_t exec = nullptr;
+
+  void init(int n) {
+    count = n;
+    recs.assign(n, {});
+    keys.assign(n, Key{});
+    have.assign(n, 0);
+    dirty.assign(n, 0);
+    part.assign(n, 0);
+  }
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (pool) (void)hipFree(pool);
+    if (pool_host) (void)hipHostFree(pool_host);
+    if (meta) (void)hipFree(meta);
+    if (meta_host) (void)hipHostFree(meta_host);
+    exec = nullptr;
+    pool = pool_host = nullptr;
+    meta = meta_host = nullptr;
+    cap = 0;
+    meta_cap = 0;
+  }
+  // Member i's launches for `key`: recorded by f() on a miss, else the cached ones.
+  template <class F>
+  void record(int i, const Key& key, F&& f) {
+    if (have[i] && keys[i] == key) return;
+    have[i] = 0;
+    Recorder R;
+    {
+      RecorderScope scope(&R);
+      f();
+    }
+    recs[i] = std::move(R.recs);
+    keys[i] = key;
+    have[i] = 1;
+    dirty[i] = 1;
+    plan_stale = true;
+  }
+  // The merged list of the participating members; plan_gen changes only when
+  // the list itself does (new argument blocks at the same slots leave it).
+  void build_plan() {
+    std::vector<Launch> old_launches;
+    std::vector<int> old_meta;
+    old_launches.swap(launches);
+    old_meta.swap(meta_vec);
+    size_t maxlen = 0;
+    for (int i = 0; i < count; ++i)
+      if (part[i]) maxlen = std::max(maxlen, recs[i].size());
+    struct Group {
+      int kid, variant;
+      unsigned block;
+      size_t lds;
+      std::vector<int> members;
+    };
+    std::vector<Group> groups;
+    for (size_t p = 0; p < maxlen; ++p) {
+      groups.clear();
+      for (int i = 0; i < count; ++i) {
+        if (!part[i] || p >= recs[i].size() || recs[i][p].grid == 0) continue;
+        const LaunchRec& L = recs[i][p];
+        Group* g = nullptr;
+        for (Group& x : groups)
+          if (x.kid == L.kid && x.variant == L.variant && x.block == L.block) g = &x;
+        if (!g) {
+          groups.push_back(Group{L.kid, L.variant, L.block, 0, {}});
+          g = &groups.back();
+        }
+        g->lds = std::max<size_t>(g->lds, L.lds);
+        g->members.push_back(i);
+      }
+      for (const Group& g : groups) {
+        Launch ln{g.kid, g.variant, g.block, 0, g.lds, (int)g.members.size(), meta_vec.size()};
+        unsigned acc = 0;
+        for (int i : g.members) {
+          meta_vec.push_back((int)acc);
+          acc += recs[i][p].grid;
+        }
+        meta_vec.push_back((int)acc);
+        for (int i : g.members) meta_vec.push_back(i * cap + (int)p);
+        ln.total = acc;
+        launches.push_back(ln);
+      }
+    }
+    plan_part = part;
+    plan_stale = false;
+    bool same = old_meta == meta_vec && old_launches.size() == launches.size();
+    for (size_t k = 0; same && k < launches.size(); ++k) {
+      const Launch &a = launches[k], &b = old_launches[k];
+      same = a.kid == b.kid && a.variant == b.variant && a.block == b.block && a.total == b.total && a.lds == b.lds &&
+             a.count == b.count && a.meta_off == b.meta_off;
+    }
+    if (!same) ++plan_gen;
+    return;
+  }
+  // Stage the changed members' argument blocks and, after a change, the merged
+  // list.  The pinned staging is rewritten only after the stream has passed the
+  // copies that read it (a batch step waits on its stream between phases).
+  void upload(hipStream_t st) {
+    size_t need = 1;
+    for (int i = 0; i < count; ++i)
+      if (part[i]) need = std::max(need, recs[i].size());
+    if ((int)need > cap) {
+      HIPCHECK(hipStreamSynchronize(st));
+      if (pool) HIPCHECK(hipFree(pool));
+      if (pool_host) HIPCHECK(hipHostFree(pool_host));
+      pool = pool_host = nullptr;
+      cap = (int)((need + need / 4 + 15) / 16 * 16);
+      const size_t bytes = (size_t)count * cap * kRecArgBytes;
+      HIPCHECK(hipMalloc((void**)&pool, bytes));
+      HIPCHECK(hipHostMalloc((void**)&pool_host, bytes, hipHostMallocDefault));
+      for (int i = 0; i < count; ++i) dirty[i] = have[i];
+      plan_stale = true;
+      ++plan_gen;  // the launches' pool pointer changed
+    }
+    int lo = count, hi = -1, nd = 0;
+    for (int i = 0; i < count; ++i) {
+      if (!dirty[i]) continue;
+      for (size_t p = 0; p < recs[i].size(); ++p)
+        std::memcpy(pool_host + ((size_t)i * cap + p) * kRecArgBytes, recs[i][p].args, kRecArgBytes);
+      lo = std::min(lo, i);
+      hi = i;
+      ++nd;
+    }
+    if (nd > 0 && 4 * nd >= hi - lo + 1) {  // one copy over the changed span
+      const size_t off = (size_t)lo * cap * kRecArgBytes, bytes = (size_t)(hi - lo + 1) * cap * kRecArgBytes;
+      HIPCHECK(hipMemcpyAsync(pool + off, pool_host + off, bytes, hipMemcpyHostToDevice, st));
+    } else {
+      for (int i = lo; i <= hi; ++i)
+        if (dirty[i]) {
+          const size_t off = (size_t)i * cap * kRecArgBytes;
+          HIPCHECK(hipMemcpyAsync(pool + off, pool_host + off, recs[i].size() * kRecArgBytes, hipMemcpyHostToDevice,
+                                  st));
+        }
+    }
+    std::fill(dirty.begin(), dirty.end(), 0);
+    if (plan_stale || part != plan_part) {
+      const uint64_t gen = plan_gen;
+      build_plan();
+      if (gen == plan_gen && meta) return;  // the same list: the device copy stands
+      if (meta_vec.size() > meta_cap) {
+        HIPCHECK(hipStreamSynchronize(st));
+        if (meta) HIPCHECK(hipFree(meta));
+        if (meta_host) HIPCHECK(hipHostFree(meta_host));
+        meta = meta_host = nullptr;
+        meta_cap = meta_vec.size() + meta_vec.size() / 4 + 64;
+        HIPCHECK(hipMalloc((void**)&meta, sizeof(int) * meta_cap));
+        HIPCHECK(hipHostMalloc((void**)&meta_host, sizeof(int) * meta_cap, hipHostMallocDefault));
+        ++plan_gen;  // the launches' pointers changed
+      }
+      if (!meta_vec.empty()) {
+        std::memcpy(meta_host, meta_vec.data(), sizeof(int) * meta_vec.size());
+        HIPCHECK(hipMemcpyAsync(meta, meta_host, sizeof(int) * meta_vec.size(), hipMemcpyHostToDevice, st));
+      }
+    }
+  }
+  void dispatch(const Launch& ln, hipStream_t st) const {
+    const int* start = meta + ln.meta_off;
+    const dim3 g(ln.total), b(ln.block);
+    switch (ln.kid) {
+      case KID_SET_DIAG:
+        hipLaunchKernelGGL(k_set_diag_batch, g, b, ln.lds, st, pool, start, ln.count);
+        HIPCHECK(hipGetLastError());
+        break;
+      case KID_COPY:
+        hipLaunchKernelGGL(k_copy_batch, g, b, ln.lds, st, pool, start, ln.count);
+        HIPCHECK(hipGetLastError());
+        break;
+      case KID_ROWS_D:
+      case KID_ROWS_F:
+      case KID_REDUCE:
+      case KID_EPILOGUE:
+      case KID_JACOBI_WAVE:
+      case KID_JACOBI_WG:
+        HIPCHECK(launch_rowops_batched(ln.kid, ln.variant, pool, start, ln.count, ln.total, ln.block, ln.lds, st));
+        break;
+      default:
+        HIPCHECK(launch_de_batched(ln.kid, ln.variant, pool, start, ln.count, ln.total, ln.block, ln.lds, st));
+    }
+  }
+  // Run the merged list: directly the first time a list is seen, then as a
+  // captured graph while it stays the same (the graph reads the pool and the
+  // offsets at replay, so changed argument blocks need no new capture).
+  void run(hipStream_t st, bool graphs) {
+    if (graphs && exec && exec_gen == plan_gen) {
+      HIPCHECK(hipGraphLaunch(exec, st));
+      return;
+    }
+    if (graphs && seen_gen == plan_gen && !launches.empty()) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+      exec_gen = ~0ull;
+      hipGraph_t graph = nullptr;
+      HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        for (const Launch& ln : launches) dispatch(ln, st);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      HIPCHECK(hipStreamEndCapture(st, &graph));
+      hipGraphExec_t x = nullptr;
+      const hipError_t e = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHECK(e);
+      exec = x;
+      exec_gen = plan_gen;
+      HIPCHECK(hipGraphLaunch(exec, st));
+      return;
+    }
+    seen_gen = plan_gen;
+    for (const Launch& ln : launches) dispatch(ln, st);
+  }
+};
+
+}  // namespace
+
+// Independent problems of uneven n, m and rank, one context each.  With merged
+// launches (the default) the members share one stream and a rank-adaptive
+// batch step runs as one launch per stage over all members (MergedPhase);
+// without, each member keeps its own stream and a batch step queues every
+// member's step before it waits for any of them.
 struct lri_batch {
   int device = 0;
   std::vector<lri_ctx*> ctx;
   std::string err;
-  bool graphs = true;  // rank-adaptive members step through captured graphs
+  bool graphs = true;  // captured graphs (merged lists, or each member's step)
   std::vector<int> status;  // each member's status from the last batch call
+  bool merged = false;
+  hipStream_t stream = nullptr;  // shared by the members while merged
+  MergedPhase ph1, ph2;         // up to the SVD; the rotation to the new rank
+  MemberWords* words = nullptr;  // device: each member's rank and flag words
+  int* word_out = nullptr;       // device: 2 count gathered words
+  int* word_host = nullptr;      // pinned
+  int64_t merged_steps = 0;      // batch steps run as merged launches
 };
 
 namespace {
@@ -2977,6 +3595,48 @@ int batch_each(lri_batch* b, F&& f, bool fresh = true) {
   }
   return first;
 }
+
+// Point every member at the batch's shared stream (on) or back at its own
+// (off), each stream drained first.
+void batch_set_merged(lri_batch* b, bool on) {
+  if (on == b->merged) return;
+  HIPCHECK(hipSetDevice(b->device));
+  for (lri_ctx* c : b->ctx) HIPCHECK(hipStreamSynchronize(c->stream));
+  if (on) {
+    const int count = (int)b->ctx.size();
+    if (!b->stream) HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    if (!b->words) {
+      std::vector<MemberWords> w(count);
+      for (int i = 0; i < count; ++i) w[i] = MemberWords{&b->ctx[i]->small->r_new, b->ctx[i]->flags};
+      HIPCHECK(hipMalloc((void**)&b->words, sizeof(MemberWords) * count));
+      HIPCHECK(hipMemcpy(b->words, w.data(), sizeof(MemberWords) * count, hipMemcpyHostToDevice));
+      HIPCHECK(hipMalloc((void**)&b->word_out, sizeof(int) * 2 * count));
+      HIPCHECK(hipHostMalloc((void**)&b->word_host, sizeof(int) * 2 * count, hipHostMallocDefault));
+      b->ph1.init(count);
+      b->ph2.init(count);
+    }
+    for (lri_ctx* c : b->ctx) c->stream = b->stream;
+  } else {
+    for (lri_ctx* c : b->ctx) c->stream = c->own_stream;
+  }
+  b->merged = on;
+}
+
+void batch_free(lri_batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (lri_ctx* c : b->ctx) {
+    c->stream = c->own_stream;
+    free_ctx(c);
+  }
+  b->ph1.release();
+  b->ph2.release();
+  if (b->words) (void)hipFree(b->words);
+  if (b->word_out) (void)hipFree(b->word_out);
+  if (b->word_host) (void)hipHostFree(b->word_host);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
 }  // namespace
 
 extern "C" {
@@ -3021,6 +3681,7 @@ int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
     c->rcap = rcap;
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->own_stream = c->stream;
     auto alloc = [&](void** p, size_t bytes) {
       HIPCHECK(hipMalloc(p, bytes));
       HIPCHECK(hipMemset(*p, 0, bytes));
@@ -3045,8 +3706,8 @@ int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
     const int64_t rbmax = 128;
     alloc((void**)&c->pkN_f64, d * m * rbmax);
     alloc((void**)&c->pkT_f64, d * n * rbmax);
-    alloc((void**)&c->pkN_f32, 4 * m * rbmax);
-    alloc((void**)&c->pkT_f32, 4 * n * rbmax);
+    alloc((void**)&c->pkN_f32, 4 * c->ldm_pk * rbmax);
+    alloc((void**)&c->pkT_f32, 4 * c->ldn_pk * rbmax);  // pass-T order: whole 32-row blocks
     alloc((void**)&c->pkN_hi, 2 * c->ldm_pk * rbmax);
     alloc((void**)&c->pkN_lo, 2 * c->ldm_pk * rbmax);
     alloc((void**)&c->pkT_hi, 2 * c->ldn_pk * rbmax);
@@ -3115,6 +3776,10 @@ int lri_set_state(lri_ctx* ctx, int64_t r, const double* U, int64_t ldu, const d
     HIPCHECK(hipMemcpy2D(ctx->S, d * ctx->rcap, S, d * lds, d * r, r, hipMemcpyDefault));
     ctx->r = r;
     ctx->pkN_mode = ctx->pkT_mode = -1;
+    if (ctx->resident) {  // into the eigenbasis
+      ctx->basis_transform(0, true, ctx->U, (int)r, ctx->K2);
+      ctx->basis_transform(1, true, ctx->V, (int)r, ctx->L2);
+    }
     return LRI_OK;
   });
 }
@@ -3122,18 +3787,25 @@ int lri_set_state(lri_ctx* ctx, int64_t r, const double* U, int64_t ldu, const d
 int lri_get_state(lri_ctx* ctx, int64_t* r_out, double* U, int64_t ldu, double* S, int64_t lds, double* V,
                   int64_t ldv) {
   return guarded<true>(ctx, [&] {
-    int st = ctx->read_flags();
     const int64_t r = ctx->r;
+    const double* Us = ctx->U;
+    const double* Vs = ctx->V;
+    if (ctx->resident && r >= 1 && (U || V)) {  // out of the eigenbasis
+      ctx->resident_out(ctx->U, ctx->V, (int)r);
+      Us = ctx->K2;
+      Vs = ctx->L2;
+    }
+    int st = ctx->read_flags();
     const size_t d = sizeof(double);
     if (r_out) *r_out = r;
     if (r >= 1) {
       if (U) {
         REQUIRE(ldu >= ctx->n, "ldu too small");
-        HIPCHECK(hipMemcpy2D(U, d * ldu, ctx->U, d * ctx->n, d * ctx->n, r, hipMemcpyDefault));
+        HIPCHECK(hipMemcpy2D(U, d * ldu, Us, d * ctx->n, d * ctx->n, r, hipMemcpyDefault));
       }
       if (V) {
         REQUIRE(ldv >= ctx->m, "ldv too small");
-        HIPCHECK(hipMemcpy2D(V, d * ldv, ctx->V, d * ctx->m, d * ctx->m, r, hipMemcpyDefault));
+        HIPCHECK(hipMemcpy2D(V, d * ldv, Vs, d * ctx->m, d * ctx->m, r, hipMemcpyDefault));
       }
       if (S) {
         REQUIRE(lds >= r, "lds too small");
@@ -3207,6 +3879,7 @@ int lri_snapshots_bind(lri_ctx* ctx, const void* const* dev_ptrs, int64_t count,
 
 int lri_step_ksl(lri_ctx* ctx, int order, int64_t k_prev, int64_t k_next) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     REQUIRE(order == LRI_PRIMAL || order == LRI_DUAL,
             "lri_step_ksl takes LRI_PRIMAL or LRI_DUAL (Strang: lri_step_ksl_strang)");
     ctx->check_step(k_prev, k_next);
@@ -3218,6 +3891,7 @@ int lri_step_ksl(lri_ctx* ctx, int order, int64_t k_prev, int64_t k_next) {
 
 int lri_step_ksl_strang(lri_ctx* ctx, int64_t k_prev, int64_t k_mid, int64_t k_next) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_step(k_prev, k_mid);
     ctx->check_step(k_mid, k_next);
     ctx->ksl_primal(k_prev, k_mid);
@@ -3228,6 +3902,7 @@ int lri_step_ksl_strang(lri_ctx* ctx, int64_t k_prev, int64_t k_mid, int64_t k_n
 
 int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_step(k_prev, k_next);
     ctx->bug(k_prev, k_next);
     return LRI_OK;
@@ -3240,6 +3915,7 @@ namespace {
 int adjoint_entry(lri_ctx* ctx, bool ksl, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
                   int64_t ldsb, double* Vb, int64_t ldvb) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_step(k_prev, k_next);
     REQUIRE(Ub && Sb && Vb, "Ub, Sb and Vb are required");
     const int64_t r = ctx->r, n = ctx->n, m = ctx->m;
@@ -3278,6 +3954,7 @@ int lri_step_ksl_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* U
 int lri_step_bug_ra_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax, int64_t r_new,
                             double* Ub, int64_t ldub, double* Sb, int64_t ldsb, double* Vb, int64_t ldvb) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_step(k_prev, k_next);
     REQUIRE(tol >= 0.0, "tol must be >= 0");
     REQUIRE(rmax >= 1, "rmax must be >= 1");
@@ -3305,6 +3982,7 @@ int lri_step_bug_ra_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double
 
 int lri_step_greedy(lri_ctx* ctx, int64_t k_next) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_step(k_next, k_next);
     ctx->greedy(k_next);
     return LRI_OK;
@@ -3313,6 +3991,7 @@ int lri_step_greedy(lri_ctx* ctx, int64_t k_next) {
 
 int lri_step_greedy_twofactor(lri_ctx* ctx, int64_t k_next) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_step(k_next, k_next);
     ctx->greedy_twofactor(k_next);
     return LRI_OK;
@@ -3321,6 +4000,7 @@ int lri_step_greedy_twofactor(lri_ctx* ctx, int64_t k_next) {
 
 int lri_step_bug_ra(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax, int64_t* r_new) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_step(k_prev, k_next);
     REQUIRE(tol >= 0.0, "tol must be >= 0");
     REQUIRE(rmax >= 1, "rmax must be >= 1");
@@ -3349,6 +4029,7 @@ int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* 
                               const double* b_diag, const double* b_off, int b_periodic, int64_t q,
                               const double* G, int64_t ldg, const double* H, int64_t ldh) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     REQUIRE(a_diag && b_diag, "a_diag and b_diag are required");
     REQUIRE((a_off || ctx->n == 1) && (b_off || ctx->m == 1), "a_off and b_off are required");
     REQUIRE(q >= 0 && q <= LRI_MAX_FORCE, "forcing rank q must be in [0, LRI_MAX_FORCE]");
@@ -3395,6 +4076,7 @@ int lri_set_operator_lyapunov(lri_ctx* ctx, const double* a_diag, const double* 
 
 int lri_set_operator_quadratic(lri_ctx* ctx, const double* dl, const double* dd, const double* du, int periodic) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     REQUIRE(ctx->op_set, "set the linear part first (lri_set_operator_lyapunov)");
     REQUIRE(dl && dd && du, "dl, dd and du are required (n entries each)");
     REQUIRE(!periodic || ctx->n >= 3, "a periodic operator needs n >= 3");
@@ -3419,6 +4101,16 @@ int lri_set_operator_quadratic(lri_ctx* ctx, const double* dl, const double* dd,
     HIPCHECK(hipMemcpy(ctx->op_du, du, d * n, hipMemcpyHostToDevice));
     ctx->op_dper = periodic != 0;
     ctx->op_quad = true;
+    return LRI_OK;
+  });
+}
+
+int lri_step_hybrid_greedy(lri_ctx* ctx, int64_t k_next, double dt, int64_t substeps) {
+  return guarded(ctx, [&] {
+    ctx->leave_resident();
+    ctx->check_step(k_next, k_next);
+    ctx->check_de(dt, substeps);
+    ctx->hybrid_greedy(k_next, dt, substeps);
     return LRI_OK;
   });
 }
@@ -3452,6 +4144,7 @@ int lri_step_de_bug(lri_ctx* ctx, double dt, int64_t substeps) {
 static int de_adjoint_entry(lri_ctx* ctx, bool ksl, double dt, int64_t substeps, double* Ub, int64_t ldub,
                             double* Sb, int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_de(dt, substeps);
     REQUIRE(ctx->op_quad,
             "the matrix-ODE adjoint needs the quadratic operator (lri_set_operator_quadratic; D = 0 for a "
@@ -3535,11 +4228,21 @@ int lri_batch_create(lri_batch** out, int device, int64_t count, const int64_t* 
     const int st = lri_create(&c, device, n[i], m[i], rcap[i]);
     if (st != LRI_OK) {
       g_create_error = "problem " + std::to_string(i) + ": " + g_create_error;
-      for (lri_ctx* x : b->ctx) free_ctx(x);
-      delete b;
+      batch_free(b);
       return st;
     }
     b->ctx.push_back(c);
+  }
+  try {
+    batch_set_merged(b, true);
+  } catch (const Failure& e) {
+    g_create_error = e.msg;
+    batch_free(b);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_create_error = "host allocation failed";
+    batch_free(b);
+    return LRI_EOOM;
   }
   *out = b;
   return LRI_OK;
@@ -3547,8 +4250,7 @@ int lri_batch_create(lri_batch** out, int device, int64_t count, const int64_t* 
 
 int lri_batch_destroy(lri_batch* b) {
   if (!b) return LRI_OK;
-  for (lri_ctx* c : b->ctx) free_ctx(c);
-  delete b;
+  batch_free(b);
   return LRI_OK;
 }
 
@@ -3566,9 +4268,129 @@ int lri_batch_step_de_bug(lri_batch* b, double dt, const int64_t* substeps) {
   return batch_each(b, [&](int64_t i, lri_ctx* c) { return lri_step_de_bug(c, dt, substeps[i]); });
 }
 
+namespace {
+
+// Every member can run as merged launches: its step records completely (the
+// resident eigenbasis, no quadratic term, no per-launch timing).
+bool batch_mergeable(const lri_batch* b) {
+  if (!b->merged) return false;
+  for (const lri_ctx* c : b->ctx)
+    if (!c->resident || c->op_quad || c->timing) return false;
+  return true;
+}
+
+// lri_batch_step_de_bug_ra as merged launches: each member's step up to its SVD
+// (phase 1, recorded per member and reused while the member's rank, the step
+// parameters and its epoch stay), one gather of every member's new rank and
+// condition flags and one wait, then each member's rotation to its new rank
+// (phase 2).  A member whose phase 1 raised a condition, or whose new rank is
+// above its capacity, keeps its state and reports the status.  Phase 2 is left
+// queued on the shared stream.
+int merged_step_de_bug_ra(lri_batch* b, double dt, const int64_t* substeps, double tol, int64_t rmax,
+                          int64_t* r_new) {
+  const int count = (int)b->ctx.size();
+  b->status.assign(count, LRI_OK);
+  int first = LRI_OK;
+  auto fail = [&](int i, int st) {
+    b->status[i] = st;
+    if (first == LRI_OK) {
+      first = st;
+      b->err = "problem " + std::to_string(i) + ": " + lri_last_error(b->ctx[i]);
+    }
+  };
+  auto device_failure = [&](const Failure& e) {
+    b->err = e.msg;
+    for (int i = 0; i < count; ++i)
+      if (b->status[i] == LRI_OK) b->status[i] = e.status;
+    return e.status;
+  };
+  try {
+    HIPCHECK(hipSetDevice(b->device));
+  } catch (const Failure& e) {
+    return device_failure(e);
+  }
+  for (int i = 0; i < count; ++i) {
+    lri_ctx* c = b->ctx[i];
+    b->ph1.part[i] = 0;
+    const int st = guarded(c, [&] {
+      c->check_de(dt, substeps[i]);
+      REQUIRE(tol >= 0.0, "tol must be >= 0");
+      REQUIRE(rmax >= 1, "rmax must be >= 1");
+      const int64_t rm = std::min<int64_t>(rmax, c->rcap);
+      const MergedPhase::Key key{(double)c->r, (double)substeps[i], dt, tol, (double)rm, (double)c->epoch, 0.0, 0.0};
+      b->ph1.record(i, key, [&] {
+        DeScope scope(c);
+        c->defer_rank_copy = true;
+        try {
+          c->de_bug_ra(dt, substeps[i], tol, rm, false);
+        } catch (...) {
+          c->defer_rank_copy = false;
+          throw;
+        }
+        c->defer_rank_copy = false;
+      });
+      c->ra_k2 = 2 * (int)c->r;
+      return LRI_OK;
+    });
+    if (st != LRI_OK) fail(i, st);
+    else b->ph1.part[i] = 1;
+  }
+  try {
+    b->ph1.upload(b->stream);
+    b->ph1.run(b->stream, b->graphs);
+    hipLaunchKernelGGL(k_gather_status, dim3((count + 255) / 256), dim3(256), 0, b->stream, b->words, b->word_out,
+                       count);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(b->word_host, b->word_out, sizeof(int) * 2 * count, hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+  } catch (const Failure& e) {
+    return device_failure(e);
+  }
+  for (int i = 0; i < count; ++i) {
+    b->ph2.part[i] = 0;
+    if (!b->ph1.part[i]) continue;
+    lri_ctx* c = b->ctx[i];
+    const int r1 = b->word_host[i], f = b->word_host[count + i];
+    const int st = guarded(c, [&] {
+      c->pkN_mode = c->pkT_mode = -1;
+      if (f & FLAG_NONFINITE) {
+        c->err = "non-finite values reached a Gram matrix; the state is left at the step's start";
+        return LRI_ENONFINITE;
+      }
+      if (f & FLAG_BREAKDOWN) {
+        c->err = "numerically singular Gram matrix; the state is left at the step's start";
+        return LRI_EBREAKDOWN;
+      }
+      c->ra_check_rank(r1);
+      const int k2 = 2 * (int)c->r;
+      const MergedPhase::Key key{(double)k2, (double)r1, (double)c->epoch, 0.0, 0.0, 0.0, 0.0, 0.0};
+      b->ph2.record(i, key, [&] {
+        DeScope scope(c);
+        c->ra_apply(k2, r1);
+      });
+      c->r = r1;
+      if (r_new) r_new[i] = r1;
+      return LRI_OK;
+    });
+    if (st != LRI_OK) fail(i, st);
+    else b->ph2.part[i] = 1;
+  }
+  try {
+    b->ph2.upload(b->stream);
+    b->ph2.run(b->stream, b->graphs);
+  } catch (const Failure& e) {
+    return device_failure(e);
+  }
+  ++b->merged_steps;
+  return first;
+}
+
+}  // namespace
+
 int lri_batch_step_de_bug_ra(lri_batch* b, double dt, const int64_t* substeps, double tol, int64_t rmax,
                              int64_t* r_new) {
   if (!b || !substeps) return LRI_EINVAL;
+  if (batch_mergeable(b)) return merged_step_de_bug_ra(b, dt, substeps, tol, rmax, r_new);
   // queue every member's step up to its SVD ...
   int st = batch_each(b, [&](int64_t i, lri_ctx* c) {
     return guarded(c, [&] {
@@ -3626,8 +4448,96 @@ int lri_batch_use_graphs(lri_batch* b, int on) {
   return LRI_OK;
 }
 
+int lri_batch_use_merged_launches(lri_batch* b, int on) {
+  if (!b) return LRI_EINVAL;
+  try {
+    batch_set_merged(b, on != 0);
+  } catch (const Failure& e) {
+    b->err = e.msg;
+    return e.status;
+  }
+  return LRI_OK;
+}
+
+int lri_batch_launch_stats(const lri_batch* b, int64_t* merged_steps, int64_t* launches) {
+  if (!b) return LRI_EINVAL;
+  if (merged_steps) *merged_steps = b->merged_steps;
+  if (launches) *launches = b->merged_steps ? (int64_t)(b->ph1.launches.size() + 1 + b->ph2.launches.size()) : 0;
+  return LRI_OK;
+}
+
 int lri_batch_synchronize(lri_batch* b) {
   return batch_each(b, [&](int64_t, lri_ctx* c) { return lri_synchronize(c); });
+}
+
+int lri_set_resident_eigenbasis(lri_ctx* ctx, int on) {
+  return guarded(ctx, [&] {
+    if (on) ctx->enter_resident();
+    else ctx->leave_resident();
+    return LRI_OK;
+  });
+}
+
+int lri_trajectory_reserve(lri_ctx* ctx, int64_t slots) {
+  return guarded<true>(ctx, [&] {
+    REQUIRE(slots >= 0, "slots must be >= 0");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->traj) {
+      HIPCHECK(hipFree(ctx->traj));
+      ctx->traj = nullptr;
+    }
+    ctx->traj_slots = 0;
+    ctx->traj_rank.assign((size_t)slots, 0);
+    ctx->traj_resident.assign((size_t)slots, 0);
+    if (slots > 0) {
+      HIPCHECK(hipMalloc((void**)&ctx->traj, sizeof(double) * (size_t)slots * ctx->traj_stride()));
+      ctx->traj_slots = slots;
+    }
+    return LRI_OK;
+  });
+}
+
+int lri_trajectory_save(lri_ctx* ctx, int64_t slot) {
+  return guarded<true>(ctx, [&] {
+    REQUIRE(slot >= 0 && slot < ctx->traj_slots, "trajectory slot out of range (lri_trajectory_reserve)");
+    REQUIRE(ctx->r >= 1, "state not set (call lri_set_state)");
+    const size_t d = sizeof(double);
+    const int64_t n = ctx->n, m = ctx->m, rc = ctx->rcap, r = ctx->r;
+    double* base = ctx->traj + slot * ctx->traj_stride();
+    HIPCHECK(hipMemcpyAsync(base, ctx->U, d * n * r, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(base + n * rc, ctx->S, d * rc * rc, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(base + n * rc + rc * rc, ctx->V, d * m * r, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->traj_rank[(size_t)slot] = r;
+    ctx->traj_resident[(size_t)slot] = ctx->resident ? 1 : 0;
+    return LRI_OK;
+  });
+}
+
+int lri_trajectory_get(lri_ctx* ctx, int64_t slot, int64_t* r_out, double* U, int64_t ldu, double* S, int64_t lds,
+                       double* V, int64_t ldv) {
+  return guarded<true>(ctx, [&] {
+    REQUIRE(slot >= 0 && slot < ctx->traj_slots, "trajectory slot out of range (lri_trajectory_reserve)");
+    const int64_t r = ctx->traj_rank[(size_t)slot];
+    REQUIRE(r >= 1, "trajectory slot was never saved");
+    const int64_t n = ctx->n, m = ctx->m, rc = ctx->rcap;
+    REQUIRE((!U || ldu >= n) && (!S || lds >= r) && (!V || ldv >= m), "leading dimensions too small");
+    const size_t d = sizeof(double);
+    const double* base = ctx->traj + slot * ctx->traj_stride();
+    const double* Us = base;
+    const double* Vs = base + n * rc + rc * rc;
+    if (ctx->traj_resident[(size_t)slot] && (U || V)) {  // saved in the eigenbasis
+      REQUIRE(ctx->resident, "the slot was saved in the resident eigenbasis, which has been left since");
+      ctx->resident_out(Us, Vs, (int)r);
+      Us = ctx->K2;
+      Vs = ctx->L2;
+    }
+    int st = ctx->read_flags();
+    if (U) HIPCHECK(hipMemcpy2D(U, d * ldu, Us, d * n, d * n, r, hipMemcpyDefault));
+    if (S) HIPCHECK(hipMemcpy2D(S, d * lds, base + n * rc, d * rc, d * r, r, hipMemcpyDefault));
+    if (V) HIPCHECK(hipMemcpy2D(V, d * ldv, Vs, d * m, d * m, r, hipMemcpyDefault));
+    if (r_out) *r_out = r;
+    return st;
+  });
 }
 
 int lri_synchronize(lri_ctx* ctx) {
@@ -3652,7 +4562,45 @@ int lri_kernel_timing(lri_ctx* ctx, int mode) {
     for (int f = 0; f < LRI_KERNEL_COUNT; ++f) {
       ctx->tot_ms[f] = 0.0;
       ctx->launches[f] = 0;
+      ctx->samples[f].clear();
+      ctx->seen[f] = 0;
     }
+    return LRI_OK;
+  });
+}
+
+int lri_autotune_passes(lri_ctx* ctx, int64_t kp, int64_t kn, int64_t reps, double* report) {
+  return guarded<true>(ctx, [&] {
+    ctx->leave_resident();
+    REQUIRE(report, "report is required (8 doubles)");
+    REQUIRE(ctx->r >= 1, "state not set (call lri_set_state)");
+    REQUIRE(ctx->sdtype >= 0, "no snapshots (call lri_snapshots_reserve)");
+    const int64_t ns = (int64_t)ctx->snaps.size();
+    REQUIRE(kn >= 0 && kn < ns && ctx->snaps[kn], "snapshot kn is not set");
+    REQUIRE(kp < ns && (kp < 0 || ctx->snaps[kp]), "snapshot kp is not set");
+    REQUIRE(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
+    ctx->autotune_passes(kp, kn, (int)reps, report);
+    return LRI_OK;
+  });
+}
+
+int lri_kernel_timing_sample(lri_ctx* ctx, int every) {
+  return guarded<true>(ctx, [&] {
+    REQUIRE(every >= 1, "every must be >= 1");
+    ctx->timing_every = every;
+    return LRI_OK;
+  });
+}
+
+int lri_kernel_timing_samples(lri_ctx* ctx, int family, float* ms, int64_t cap, int64_t* count) {
+  return guarded<true>(ctx, [&] {
+    REQUIRE(family >= 0 && family < LRI_KERNEL_COUNT, "unknown kernel family");
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->drain_timing();
+    const auto& v = ctx->samples[family];
+    if (count) *count = (int64_t)v.size();
+    if (ms)
+      for (int64_t i = 0; i < cap && i < (int64_t)v.size(); ++i) ms[i] = v[i];
     return LRI_OK;
   });
 }
@@ -3670,6 +4618,7 @@ int lri_kernel_timing_read(lri_ctx* ctx, int family, double* total_ms, int64_t* 
 
 int lri_debug_product(lri_ctx* ctx, int which, int64_t k_prev, int64_t k_next, double* out, int64_t ldo) {
   return guarded(ctx, [&] {
+    ctx->leave_resident();
     ctx->check_step(k_prev, k_next);
     REQUIRE(which == 0 || which == 1, "which must be 0 (dA V) or 1 (dA' U)");
     REQUIRE(out != nullptr, "out is NULL");

@@ -1,9 +1,97 @@
 // Kernel argument blocks and launchers shared by the engine and the kernels.
 #pragma once
 
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
 #include "lri_common.h"
 
 namespace lri {
+
+// ----------------------------------------------------------------------------
+// Launch recording, for running the steps of many contexts as merged launches.
+// While a Recorder is installed on the calling thread, each launcher below
+// appends its launch (kernel, variant, workgroups, threads, LDS and argument
+// block) to it instead of launching.  A batch records every member's step with
+// the member's own buffers, then runs position p of all the sequences as one
+// launch per (kernel, variant, threads) of the kernel's batched form: each
+// workgroup finds its member in a table of workgroup offsets and runs the same
+// body with the member's arguments and its workgroup index within the member.
+// The arithmetic is the members' own, so the results equal those of stepping
+// each context by itself, bit for bit.
+// ----------------------------------------------------------------------------
+enum KernelId : int {
+  KID_ROWS_D = 0,   // k_rows<double, NRB, CK>; variant = NRB * 100 + CK
+  KID_ROWS_F,       // k_rows<float, NRB, 0>; variant = NRB
+  KID_REDUCE,       // k_reduce
+  KID_EPILOGUE,     // k_epilogue<CK>; variant = CK
+  KID_JACOBI_WAVE,  // k_jacobi_wave (two decompositions per record)
+  KID_JACOBI_WG,    // k_jacobi_svd
+  KID_SHIFT,        // k_shift_sym
+  KID_UNSHIFT,      // k_unshift
+  KID_DE_PREP,      // k_de_prep<kLds>; variant = kLds
+  KID_FLOW,         // k_spectral_flow
+  KID_SSTEP,        // k_s_substep<kLds>; variant = kLds
+  KID_SET_DIAG,     // k_set_diag
+  KID_COPY,         // a device-to-device copy
+  KID_COUNT
+};
+
+constexpr int kRecArgBytes = 448;
+
+struct LaunchRec {
+  int kid = 0, variant = 0;
+  unsigned grid = 0, block = 0, lds = 0, nbytes = 0;
+  alignas(16) unsigned char args[kRecArgBytes];
+};
+
+struct Recorder {
+  std::vector<LaunchRec> recs;
+};
+
+inline Recorder*& current_recorder() {
+  static thread_local Recorder* rec = nullptr;
+  return rec;
+}
+
+// Append a launch to the installed recorder; false (nothing recorded) without one.
+template <class A>
+inline bool record_launch(int kid, int variant, unsigned grid, unsigned block, size_t lds, const A& a) {
+  static_assert(sizeof(A) <= kRecArgBytes && std::is_trivially_copyable<A>::value, "recordable arguments");
+  Recorder* R = current_recorder();
+  if (!R) return false;
+  R->recs.emplace_back();
+  LaunchRec& L = R->recs.back();
+  L.kid = kid;
+  L.variant = variant;
+  L.grid = grid;
+  L.block = block;
+  L.lds = (unsigned)lds;
+  L.nbytes = sizeof(A);
+  std::memcpy(L.args, &a, sizeof(A));
+  return true;
+}
+
+// The batched forms: `count` members; start (device memory, 2 count + 1 ints)
+// holds the workgroup offsets start[0..count], member i owning workgroups
+// start[i] .. start[i+1] - 1 of the `total` launched, then each member's slot
+// in `tab`: member i's argument block is tab + start[count + 1 + i] * kRecArgBytes.
+hipError_t launch_rowops_batched(int kid, int variant, const void* tab, const int* start, int count, unsigned total,
+                                 unsigned block, size_t lds, hipStream_t st);
+hipError_t launch_de_batched(int kid, int variant, const void* tab, const int* start, int count, unsigned total,
+                             unsigned block, size_t lds, hipStream_t st);
+
+// The member of workgroup b: the last i with start[i] <= b (start has count + 1 entries).
+__device__ __forceinline__ int batch_member(const int* __restrict__ start, int count, int b) {
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 // r x r (up to 2r x 2r) FP64 matrices of one step, device-resident.
 struct SmallState {
@@ -93,10 +181,10 @@ struct RowArgs {
   double* out2 = nullptr;
   int64_t ldo2 = 0;
   double* pk_f64 = nullptr;   // row-major [i][pk_cols]
-  float* pk_f32 = nullptr;    // row-major [i][pk_cols]
+  float* pk_f32 = nullptr;    // pk_layout 0: row-major [i][pk_cols]; 1: pass-T order (see pk_f32_offset)
   uint16_t* pk_hi = nullptr;  // fragment order of the BF16 passes (see pk_offset)
   uint16_t* pk_lo = nullptr;
-  int pk_layout = 0;          // 0: pass-N order (16-row blocks), 1: pass-T order (64-row blocks)
+  int pk_layout = 0;          // 0: pass-N order (BF16: 16-row blocks), 1: pass-T order (BF16: 64-row blocks)
   int pk_cols = 0;
   // reduction C = out' * W  (W == nullptr: out' * out; gram_w: out' * [out | W])
   int reduce = 0;
@@ -134,6 +222,41 @@ struct SvdArgs {
   double* work = nullptr;  // global ke x ke scratch, used when it does not fit in LDS
 };
 hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st);
+
+// The QR of a rank-deficient matrix.  CholeskyQR2 drops a pivot whose
+// residual is at the rounding level and leaves a zero column in Q there; the
+// unpivoted factorisation of the columns before it can then be ill-conditioned,
+// so that Q R reproduces X only to ~eps cond.  Householder QR (qr! in the
+// reference) returns an orthonormal Q and Q R = X to ~eps for any input.  When
+// Rinv1(p, p) or Rinv2(p, p) of the CholeskyQR2 is zero for some column p,
+// this recomputes the factorisation from X by classical Gram-Schmidt with
+// reorthogonalisation (each column projected out twice): a column whose
+// residual falls to the rounding level (|x| <= 16 k eps max_j |X(:, j)|) gets
+// R(j, j) = 0 and, afterwards, an orthonormal completion q = e_i - Q Q(i, :)'
+// for the row i of Q with the smallest norm, projected out twice and
+// normalised.  Q (and its packed copy for the next pass) is rewritten, and the
+// products formed from R: M = R - Q' W (mt: its transpose) and Z = R (zt: R').
+// One workgroup; with nothing dropped it returns at once.
+struct CompleteArgs {
+  const double* X = nullptr;  // rows x k, ld rows (unchanged)
+  double* Q = nullptr;        // rows x k, ld rows
+  int64_t rows = 0;
+  int k = 0;
+  const double* Rinv1 = nullptr;  // k x k, ld k
+  const double* Rinv2 = nullptr;
+  double* pk_f64 = nullptr;
+  float* pk_f32 = nullptr;
+  uint16_t* pk_hi = nullptr;
+  uint16_t* pk_lo = nullptr;
+  int pk_layout = 0, pk_cols = 0;
+  double* M = nullptr;
+  int ldm = 0, mt = 0;
+  const double* W = nullptr;
+  int64_t ldw = 0;
+  double* Z = nullptr;
+  int ldz = 0, zt = 0;
+};
+hipError_t launch_complete_columns(const CompleteArgs& A, hipStream_t st);
 // Reverse (adjoint) of the data-driven BUG step, r x r parts (one workgroup each).
 struct BugAdjSmall {
   double RK[kMaxRank * kMaxRank], RL[kMaxRank * kMaxRank];          // R of the QRs of K1 and L1
@@ -208,6 +331,11 @@ struct PassArgs {
 // reduction runs over `red_len`; returns the partial buffer element count.
 int64_t plan_pass(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int* splits,
                   int64_t* chunk);
+// The same with a given reduction chunk (rounded up to the 256-row quantum).
+int64_t plan_pass_chunk(int64_t out_rows, int64_t red_len, int rb, int64_t chunk_in, int* splits, int64_t* chunk);
+// The chunks the autotuner times: those for about 128 .. 3072 workgroups, distinct.
+int pass_chunk_candidates(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int64_t* chunks,
+                          int cap);
 hipError_t launch_passN(int mode, const PassArgs& a, hipStream_t st);
 hipError_t launch_passT(int mode, const PassArgs& a, hipStream_t st);
 
@@ -470,6 +598,14 @@ __host__ __device__ inline int64_t pk_offset(int layout, int64_t i, int c, int r
   }
   const int r = (int)(i & 63);
   return (((((i >> 6) * 4 + ((r & 31) >> 3)) * 2 + (r >> 5)) * rb) + c) * 8 + (r & 7);
+}
+
+// Element (row i, column c) of an FP32 B panel with rb columns: pass N reads it
+// row-major (layout 0); pass T (layout 1) reads, for the 32-row block of i, half h
+// = (i / 16) % 2 and column c, the 16 consecutive rows 16 h .. 16 h + 15.
+__host__ __device__ inline int64_t pk_f32_offset(int layout, int64_t i, int c, int rb) {
+  if (layout == 0) return i * rb + c;
+  return ((((i >> 5) * 2 + ((i >> 4) & 1)) * rb) + c) * 16 + (i & 15);
 }
 
 }  // namespace lri

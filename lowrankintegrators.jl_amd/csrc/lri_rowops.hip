@@ -543,10 +543,10 @@ __device__ void run_epilogue(const EpiArgs& E, double* C, int a, int b, double* 
 // (entry e, quarter g) adds partials g, g+4, ... with 16 loads in flight, and the
 // four quarters are combined in a fixed order (deterministic).
 __device__ __forceinline__ void reduce_entries(const double* __restrict__ gpart, int nb, int ab,
-                                               double* __restrict__ C) {
+                                               double* __restrict__ C, int bid) {
   __shared__ double q4[4][64];
   const int el = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int e = blockIdx.x * 64 + el;
+  const int e = bid * 64 + el;
   double s = 0.0;
   if (e < ab) {
     int k = g;
@@ -566,20 +566,50 @@ __device__ __forceinline__ void reduce_entries(const double* __restrict__ gpart,
 
 __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ gpart, int nb, int ab,
                                                      double* __restrict__ C) {
-  reduce_entries(gpart, nb, ab, C);
+  reduce_entries(gpart, nb, ab, C, blockIdx.x);
+}
+
+struct ReduceLaunch {
+  const double* gpart;
+  int nb, ab;
+  double* C;
+};
+__global__ __launch_bounds__(kThreads) void k_reduce_batch(const unsigned char* __restrict__ tab,
+                                                           const int* __restrict__ start, int count) {
+  const int b = blockIdx.x;
+  const int i = batch_member(start, count, b);
+  const ReduceLaunch& R = *reinterpret_cast<const ReduceLaunch*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
+  reduce_entries(R.gpart, R.nb, R.ab, R.C, b - start[i]);
 }
 
 // One workgroup: C (a x b, from k_reduce) into LDS, then the epilogue.  (A
 // variant where the last k_reduce workgroup ran the epilogue after a
 // device-scope release/acquire measured slower than this second launch.)
 template <int CK>
-__global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* __restrict__ Cg, int a, int b) {
+__device__ __forceinline__ void epilogue_body(const EpiArgs& E, const double* __restrict__ Cg, int a, int b) {
   extern __shared__ double lds[];
   double* C = lds;
   double* scr = lds + a * b;
   for (int e = threadIdx.x; e < a * b; e += kThreads) C[e] = Cg[e];
   __syncthreads();
   run_epilogue<CK>(E, C, a, b, scr);
+}
+template <int CK>
+__global__ __launch_bounds__(kThreads) void k_epilogue(EpiArgs E, const double* __restrict__ Cg, int a, int b) {
+  epilogue_body<CK>(E, Cg, a, b);
+}
+
+struct EpiLaunch {
+  EpiArgs E;
+  const double* Cg;
+  int a, b;
+};
+template <int CK>
+__global__ __launch_bounds__(kThreads) void k_epilogue_batch(const unsigned char* __restrict__ tab,
+                                                             const int* __restrict__ start, int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  const EpiLaunch& L = *reinterpret_cast<const EpiLaunch*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
+  epilogue_body<CK>(L.E, L.Cg, L.a, L.b);
 }
 
 
@@ -648,9 +678,10 @@ __host__ __device__ inline RowsPlan rows_plan(int ro, int ri, int rw, bool hasX,
   return p;
 }
 
-// CK > 0: with the prologue of RowArgs::pre (Cholesky variant CK)
+// CK > 0: with the prologue of RowArgs::pre (Cholesky variant CK).  bid / nblk:
+// this workgroup's index among the nblk of the launch (of its member, batched).
 template <typename PT, int NRB, int CK = 0>
-__global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
+__device__ __forceinline__ void rows_body(const RowArgs<PT>& A, int bid, int nblk) {
   extern __shared__ double lds[];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int ro = A.r_out, ri = A.r_in;
@@ -682,7 +713,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
     double* scr = Cp + pa * pb;
     for (int e = t; e < pa * pb; e += kThreads) Cp[e] = A.pre_C[e];
     __syncthreads();
-    run_epilogue<CK>(A.pre, Cp, pa, pb, scr, blockIdx.x == 0);
+    run_epilogue<CK>(A.pre, Cp, pa, pb, scr, bid == 0);
     __syncthreads();
     // Y = its R^{-1} (pa x pa), staged as below; the tile barrier publishes it
     const double* Ri = scr + pa * pa;
@@ -693,7 +724,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
     ys_staged = true;
   }
   const int64_t ntiles = (A.rows + rpb - 1) / rpb;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (int64_t tile = bid; tile < ntiles; tile += nblk) {
     const int64_t row0 = tile * rpb;
     const int nrows = (int)min((int64_t)rpb, A.rows - row0);
     const int64_t i = row0 + lrow;
@@ -823,7 +854,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
           if (A.out) A.out[i + c * A.ldo] = v[q];
           if (A.out2) A.out2[i + c * A.ldo2] = v[q];
           if (A.pk_f64) A.pk_f64[i * A.pk_cols + c] = v[q];
-          if (A.pk_f32) A.pk_f32[i * A.pk_cols + c] = (float)v[q];
+          if (A.pk_f32) A.pk_f32[pk_f32_offset(A.pk_layout, i, c, A.pk_cols)] = (float)v[q];
           if (A.pk_hi) {
             uint32_t hi, lo;
             split_bf16((float)v[q], hi, lo);
@@ -836,7 +867,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
       // zero the packed columns beyond the current rank
       for (int c = ro + cg; c < A.pk_cols; c += tpr) {
         if (A.pk_f64) A.pk_f64[i * A.pk_cols + c] = 0.0;
-        if (A.pk_f32) A.pk_f32[i * A.pk_cols + c] = 0.0f;
+        if (A.pk_f32) A.pk_f32[pk_f32_offset(A.pk_layout, i, c, A.pk_cols)] = 0.0f;
         if (A.pk_hi) {
           const int64_t off = pk_offset(A.pk_layout, i, c, A.pk_cols);
           A.pk_hi[off] = 0;
@@ -866,7 +897,7 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
     if (use_tile) __syncthreads();  // the next tile overwrites the LDS tile
   }
   if (!reduce) return;
-  double* gp = A.gpart + (int64_t)blockIdx.x * ro * (rg + rw);
+  double* gp = A.gpart + (int64_t)bid * ro * (rg + rw);
   const int nbi = p.ro16 / 16;
 #pragma unroll
   for (int b = 0; b < NRB; ++b) {
@@ -883,6 +914,20 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
       }
     }
   }
+}
+
+template <typename PT, int NRB, int CK = 0>
+__global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
+  rows_body<PT, NRB, CK>(A, blockIdx.x, gridDim.x);
+}
+
+template <typename PT, int NRB, int CK>
+__global__ __launch_bounds__(kThreads) void k_rows_batch(const unsigned char* __restrict__ tab,
+                                                         const int* __restrict__ start, int count) {
+  const int b = blockIdx.x;
+  const int i = batch_member(start, count, b);
+  const RowArgs<PT>& A = *reinterpret_cast<const RowArgs<PT>*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
+  rows_body<PT, NRB, CK>(A, b - start[i], start[i + 1] - start[i]);
 }
 
 template <typename PT>
@@ -914,45 +959,49 @@ hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
   int grid = (int)(ntiles < A.max_blocks ? ntiles : A.max_blocks);
   if (grid < 1) grid = 1;
   const int nrb = (p.nred + 3) / 4;
+  int nrbv = 0, ckv = 0;
   if (pre) {
-    if constexpr (std::is_same<PT, double>::value) {
-      if (nrb > 2) return hipErrorInvalidValue;
-      const int ck = A.pre_a <= 16 ? 16 : 32;
-      if (ck == 16 && nrb <= 1)
-        hipLaunchKernelGGL((k_rows<PT, 1, 16>), dim3(grid), dim3(kThreads), lds, st, A);
-      else if (ck == 16)
-        hipLaunchKernelGGL((k_rows<PT, 2, 16>), dim3(grid), dim3(kThreads), lds, st, A);
-      else if (nrb <= 1)
-        hipLaunchKernelGGL((k_rows<PT, 1, 32>), dim3(grid), dim3(kThreads), lds, st, A);
-      else
-        hipLaunchKernelGGL((k_rows<PT, 2, 32>), dim3(grid), dim3(kThreads), lds, st, A);
-    } else {
-      return hipErrorInvalidValue;
+    if (!std::is_same<PT, double>::value || nrb > 2) return hipErrorInvalidValue;
+    ckv = A.pre_a <= 16 ? 16 : 32;
+    nrbv = nrb <= 1 ? 1 : 2;
+  } else {
+    nrbv = nrb <= 1 ? 1 : (nrb <= 2 ? 2 : (nrb <= 4 ? 4 : (nrb <= 8 ? 8 : 16)));
+  }
+  const int kid = std::is_same<PT, double>::value ? KID_ROWS_D : KID_ROWS_F;
+  if (!record_launch(kid, nrbv * 100 + ckv, (unsigned)grid, kThreads, lds, A)) {
+    const dim3 g(grid), b(kThreads);
+    switch (nrbv * 100 + ckv) {
+      case 116: hipLaunchKernelGGL((k_rows<PT, 1, 16>), g, b, lds, st, A); break;
+      case 216: hipLaunchKernelGGL((k_rows<PT, 2, 16>), g, b, lds, st, A); break;
+      case 132: hipLaunchKernelGGL((k_rows<PT, 1, 32>), g, b, lds, st, A); break;
+      case 232: hipLaunchKernelGGL((k_rows<PT, 2, 32>), g, b, lds, st, A); break;
+      case 100: hipLaunchKernelGGL((k_rows<PT, 1>), g, b, lds, st, A); break;
+      case 200: hipLaunchKernelGGL((k_rows<PT, 2>), g, b, lds, st, A); break;
+      case 400: hipLaunchKernelGGL((k_rows<PT, 4>), g, b, lds, st, A); break;
+      case 800: hipLaunchKernelGGL((k_rows<PT, 8>), g, b, lds, st, A); break;
+      default: hipLaunchKernelGGL((k_rows<PT, 16>), g, b, lds, st, A); break;
     }
-  } else if (nrb <= 1)
-    hipLaunchKernelGGL((k_rows<PT, 1>), dim3(grid), dim3(kThreads), lds, st, A);
-  else if (nrb <= 2)
-    hipLaunchKernelGGL((k_rows<PT, 2>), dim3(grid), dim3(kThreads), lds, st, A);
-  else if (nrb <= 4)
-    hipLaunchKernelGGL((k_rows<PT, 4>), dim3(grid), dim3(kThreads), lds, st, A);
-  else if (nrb <= 8)
-    hipLaunchKernelGGL((k_rows<PT, 8>), dim3(grid), dim3(kThreads), lds, st, A);
-  else
-    hipLaunchKernelGGL((k_rows<PT, 16>), dim3(grid), dim3(kThreads), lds, st, A);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !A.reduce) return e;
   const int ab = A.r_out * rc;
   double* Cr = A.epi.small->Cred;
-  hipLaunchKernelGGL(k_reduce, dim3((ab + 63) / 64), dim3(kThreads), 0, st, (const double*)A.gpart, grid, ab, Cr);
+  const ReduceLaunch R{(const double*)A.gpart, grid, ab, Cr};
+  if (!record_launch(KID_REDUCE, 0, (unsigned)((ab + 63) / 64), kThreads, 0, R))
+    hipLaunchKernelGGL(k_reduce, dim3((ab + 63) / 64), dim3(kThreads), 0, st, R.gpart, R.nb, R.ab, R.C);
   e = hipGetLastError();
   if (e != hipSuccess || A.epi.kind == EPI_NONE || A.defer_epi) return e;
   const bool chol = A.epi.kind == EPI_CHOL1 || A.epi.kind == EPI_CHOL2 || A.epi.kind == EPI_CHOL3;
-  if (chol && A.r_out <= 16)
-    hipLaunchKernelGGL(k_epilogue<16>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rc);
-  else if (chol && A.r_out <= 32)
-    hipLaunchKernelGGL(k_epilogue<32>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rc);
-  else
-    hipLaunchKernelGGL(k_epilogue<64>, dim3(1), dim3(kThreads), epi_lds, st, A.epi, (const double*)Cr, A.r_out, rc);
+  const int ck = (chol && A.r_out <= 16) ? 16 : ((chol && A.r_out <= 32) ? 32 : 64);
+  const EpiLaunch L{A.epi, (const double*)Cr, A.r_out, rc};
+  if (!record_launch(KID_EPILOGUE, ck, 1, kThreads, epi_lds, L)) {
+    if (ck == 16)
+      hipLaunchKernelGGL(k_epilogue<16>, dim3(1), dim3(kThreads), epi_lds, st, L.E, L.Cg, L.a, L.b);
+    else if (ck == 32)
+      hipLaunchKernelGGL(k_epilogue<32>, dim3(1), dim3(kThreads), epi_lds, st, L.E, L.Cg, L.a, L.b);
+    else
+      hipLaunchKernelGGL(k_epilogue<64>, dim3(1), dim3(kThreads), epi_lds, st, L.E, L.Cg, L.a, L.b);
+  }
   return hipGetLastError();
 }
 
@@ -1509,9 +1558,32 @@ __global__ __launch_bounds__(1024) void k_jacobi_svd(SvdArgs P0, SvdArgs P1) {
   jacobi_svd_body(blockIdx.x == 0 ? P0 : P1, lds);
 }
 
+__device__ void jacobi_wave_dispatch(const SvdArgs& P);
+
 // one wavefront per decomposition, k <= 32 (workgroup b decomposes P_b)
 __global__ __launch_bounds__(64) void k_jacobi_wave(SvdArgs P0, SvdArgs P1) {
-  const SvdArgs& P = blockIdx.x == 0 ? P0 : P1;
+  jacobi_wave_dispatch(blockIdx.x == 0 ? P0 : P1);
+}
+
+// the batched forms: one record holds the one or two decompositions of a launch
+struct SvdPair {
+  SvdArgs P0, P1;
+};
+__global__ __launch_bounds__(1024) void k_jacobi_svd_batch(const unsigned char* __restrict__ tab,
+                                                           const int* __restrict__ start, int count) {
+  extern __shared__ double lds[];
+  const int i = batch_member(start, count, blockIdx.x);
+  const SvdPair& S = *reinterpret_cast<const SvdPair*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
+  jacobi_svd_body(blockIdx.x - start[i] == 0 ? S.P0 : S.P1, lds);
+}
+__global__ __launch_bounds__(64) void k_jacobi_wave_batch(const unsigned char* __restrict__ tab,
+                                                          const int* __restrict__ start, int count) {
+  const int i = batch_member(start, count, blockIdx.x);
+  const SvdPair& S = *reinterpret_cast<const SvdPair*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
+  jacobi_wave_dispatch(blockIdx.x - start[i] == 0 ? S.P0 : S.P1);
+}
+
+__device__ void jacobi_wave_dispatch(const SvdArgs& P) {
   const int np = ((P.k + 1) & ~1) / 2;
   if (np > 8)
     jacobi_wave_body<4>(P);
@@ -1546,10 +1618,13 @@ hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st) {
   size_t lds;
   hipError_t e = jacobi_plan(P, &lds);
   if (e != hipSuccess) return e;
-  if (P.k <= 32)
-    hipLaunchKernelGGL(k_jacobi_wave, dim3(1), dim3(64), 0, st, P, P);
-  else
+  const SvdPair S{P, P};
+  if (P.k <= 32) {
+    if (!record_launch(KID_JACOBI_WAVE, 0, 1, 64, 0, S))
+      hipLaunchKernelGGL(k_jacobi_wave, dim3(1), dim3(64), 0, st, P, P);
+  } else if (!record_launch(KID_JACOBI_WG, 0, 1, (unsigned)jacobi_threads(P.k), lds, S)) {
     hipLaunchKernelGGL(k_jacobi_svd, dim3(1), dim3(jacobi_threads(P.k)), lds, st, P, P);
+  }
   return hipGetLastError();
 }
 
@@ -1558,11 +1633,259 @@ hipError_t launch_jacobi_svd2(SvdArgs P0, SvdArgs P1, hipStream_t st) {
   hipError_t e = jacobi_plan(P0, &l0);
   if (e == hipSuccess) e = jacobi_plan(P1, &l1);
   if (e != hipSuccess) return e;
-  if (P0.k <= 32 && P1.k <= 32)
-    hipLaunchKernelGGL(k_jacobi_wave, dim3(2), dim3(64), 0, st, P0, P1);
-  else
-    hipLaunchKernelGGL(k_jacobi_svd, dim3(2), dim3(jacobi_threads(std::max(P0.k, P1.k))), l0 > l1 ? l0 : l1, st,
-                       P0, P1);
+  const SvdPair S{P0, P1};
+  if (P0.k <= 32 && P1.k <= 32) {
+    if (!record_launch(KID_JACOBI_WAVE, 0, 2, 64, 0, S))
+      hipLaunchKernelGGL(k_jacobi_wave, dim3(2), dim3(64), 0, st, P0, P1);
+  } else {
+    const unsigned th = (unsigned)jacobi_threads(std::max(P0.k, P1.k));
+    const size_t lds = l0 > l1 ? l0 : l1;
+    if (!record_launch(KID_JACOBI_WG, 0, 2, th, lds, S))
+      hipLaunchKernelGGL(k_jacobi_svd, dim3(2), dim3(th), lds, st, P0, P1);
+  }
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------------------------
+// orthonormal completion of dropped CholeskyQR columns (see CompleteArgs)
+// ----------------------------------------------------------------------------
+constexpr int kCompleteThreads = 256;
+
+// per thread v[0..7]: the workgroup sums into out[0..7] (LDS), visible to all after return
+__device__ __forceinline__ void complete_block_sum8(double (&v)[8], double* red, double* out) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) red[(threadIdx.x >> 6) * 8 + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int q = threadIdx.x;
+    out[q] = (red[q] + red[8 + q]) + (red[16 + q] + red[24 + q]);
+  }
+  __syncthreads();
+}
+
+// h[0..n) = Q(:, 0..n)' y over the rows (h in LDS)
+__device__ void complete_project(const double* __restrict__ Q, int64_t rows, int n, const double* __restrict__ y,
+                                 double* red, double* h) {
+  const int t = threadIdx.x;
+  for (int c0 = 0; c0 < n; c0 += 8) {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = t; i < rows; i += kCompleteThreads) {
+      const double yi = y[i];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (c0 + q < n) v[q] = fma(Q[i + (c0 + q) * rows], yi, v[q]);
+    }
+    complete_block_sum8(v, red, h + c0);
+  }
+}
+
+__global__ __launch_bounds__(kCompleteThreads) void k_complete_columns(CompleteArgs A) {
+  __shared__ double s_R[kMaxRank * kMaxRank];  // R (k x k, ld k)
+  __shared__ double s_h[kMaxAug + 8];
+  __shared__ double s_red[32];
+  __shared__ int s_dep[kMaxAug];
+  __shared__ int s_flag;
+  __shared__ double s_min[4];
+  __shared__ long long s_arg[4];
+  const int t = threadIdx.x;
+  const int k = A.k;
+  const int64_t rows = A.rows;
+  double* __restrict__ Q = A.Q;
+  if (t == 0) {
+    int nd = 0;
+    for (int p = 0; p < k; ++p) nd += (A.Rinv1[p + p * k] == 0.0 || A.Rinv2[p + p * k] == 0.0) ? 1 : 0;
+    s_flag = nd;
+  }
+  __syncthreads();
+  if (s_flag == 0) return;
+  for (int e = t; e < k * k; e += kCompleteThreads) s_R[e] = 0.0;
+  // the largest column norm of X, for the rounding-level threshold
+  double cmax = 0.0;
+  for (int c0 = 0; c0 < k; c0 += 8) {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = t; i < rows; i += kCompleteThreads)
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (c0 + q < k) v[q] = fma(A.X[i + (c0 + q) * rows], A.X[i + (c0 + q) * rows], v[q]);
+    complete_block_sum8(v, s_red, s_h);
+    for (int q = 0; q < 8 && c0 + q < k; ++q) cmax = fmax(cmax, s_h[q]);
+  }
+  const double tol = 16.0 * k * kEps * sqrt(cmax);
+  // classical Gram-Schmidt, each column projected out twice
+  for (int j = 0; j < k; ++j) {
+    double* __restrict__ qj = Q + (int64_t)j * rows;
+    for (int64_t i = t; i < rows; i += kCompleteThreads) qj[i] = A.X[i + (int64_t)j * rows];
+    __syncthreads();
+    for (int pass = 0; pass < 2 && j > 0; ++pass) {
+      complete_project(Q, rows, j, qj, s_red, s_h);
+      for (int64_t i = t; i < rows; i += kCompleteThreads) {
+        double v = qj[i];
+        for (int c = 0; c < j; ++c) v = fma(-Q[i + c * rows], s_h[c], v);
+        qj[i] = v;
+      }
+      if (t < j) s_R[t + j * k] += s_h[t];
+      __syncthreads();
+    }
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = t; i < rows; i += kCompleteThreads) v[0] = fma(qj[i], qj[i], v[0]);
+    complete_block_sum8(v, s_red, s_h);
+    const double nrm = sqrt(s_h[0]);
+    const bool dep = !(nrm > tol);
+    const double sc = dep ? 0.0 : 1.0 / nrm;
+    for (int64_t i = t; i < rows; i += kCompleteThreads) qj[i] *= sc;
+    if (t == 0) {
+      s_R[j + j * k] = dep ? 0.0 : nrm;
+      s_dep[j] = dep ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  // orthonormal completion of the dependent columns (their rows of R are zero)
+  for (int p = 0; p < k; ++p) {
+    if (!s_dep[p]) continue;
+    double* __restrict__ q = Q + (int64_t)p * rows;
+    double best = 1e300;
+    long long bi = 0;
+    for (int64_t i = t; i < rows; i += kCompleteThreads) {
+      double s2 = 0.0;
+      for (int c = 0; c < k; ++c) s2 = fma(Q[i + c * rows], Q[i + c * rows], s2);
+      if (s2 < best) {
+        best = s2;
+        bi = i;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off);
+      const long long oi = __shfl_xor(bi, off);
+      if (ob < best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if ((t & 63) == 0) {
+      s_min[t >> 6] = best;
+      s_arg[t >> 6] = bi;
+    }
+    __syncthreads();
+    if (t == 0)
+      for (int w = 1; w < 4; ++w)
+        if (s_min[w] < s_min[0] || (s_min[w] == s_min[0] && s_arg[w] < s_arg[0])) {
+          s_min[0] = s_min[w];
+          s_arg[0] = s_arg[w];
+        }
+    __syncthreads();
+    const int64_t is = s_arg[0];
+    if (t < k) s_h[t] = Q[is + t * rows];
+    __syncthreads();
+    for (int64_t i = t; i < rows; i += kCompleteThreads) {  // column p is zero and takes no part
+      double x = i == is ? 1.0 : 0.0;
+      for (int c = 0; c < k; ++c) x = fma(-Q[i + c * rows], s_h[c], x);
+      q[i] = x;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+      complete_project(Q, rows, k, q, s_red, s_h);
+      if (t == 0) s_h[p] = 0.0;
+      __syncthreads();
+      for (int64_t i = t; i < rows; i += kCompleteThreads) {
+        double x = q[i];
+        for (int c = 0; c < k; ++c) x = fma(-Q[i + c * rows], s_h[c], x);
+        q[i] = x;
+      }
+      __syncthreads();
+    }
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = t; i < rows; i += kCompleteThreads) v[0] = fma(q[i], q[i], v[0]);
+    complete_block_sum8(v, s_red, s_h);
+    const double inv = 1.0 / sqrt(s_h[0]);
+    for (int64_t i = t; i < rows; i += kCompleteThreads) q[i] *= inv;
+    __syncthreads();
+  }
+  // the packed copy of Q for the next pass
+  for (int64_t i = t; i < rows; i += kCompleteThreads)
+    for (int p = 0; p < k; ++p) {
+      const double x = Q[i + (int64_t)p * rows];
+      if (A.pk_f64) A.pk_f64[i * A.pk_cols + p] = x;
+      if (A.pk_f32) A.pk_f32[pk_f32_offset(A.pk_layout, i, p, A.pk_cols)] = (float)x;
+      if (A.pk_hi) {
+        uint32_t hi, lo;
+        split_bf16((float)x, hi, lo);
+        const int64_t off = pk_offset(A.pk_layout, i, p, A.pk_cols);
+        A.pk_hi[off] = (uint16_t)hi;
+        A.pk_lo[off] = (uint16_t)lo;
+      }
+    }
+  // the products of R: Z = R (zt: R'), M = R - Q' W (mt: its transpose)
+  if (A.Z)
+    for (int e = t; e < k * k; e += kCompleteThreads) {
+      const int i = e % k, j = e / k;
+      if (A.zt) A.Z[j + i * A.ldz] = s_R[e];
+      else A.Z[i + j * A.ldz] = s_R[e];
+    }
+  if (A.M)
+    for (int c = 0; c < k; ++c) {
+      complete_project(Q, rows, k, A.W + (int64_t)c * A.ldw, s_red, s_h);
+      if (t < k) {
+        const double x = s_R[t + c * k] - s_h[t];
+        if (A.mt) A.M[c + t * A.ldm] = x;
+        else A.M[t + c * A.ldm] = x;
+      }
+      __syncthreads();
+    }
+}
+
+hipError_t launch_complete_columns(const CompleteArgs& A, hipStream_t st) {
+  if (!A.X || !A.Q || A.rows < 1 || A.k < 1 || A.k > kMaxRank || !A.Rinv1 || !A.Rinv2 || (A.M && !A.W))
+    return hipErrorInvalidValue;
+  if (A.k > A.rows) return hipErrorInvalidValue;  // no room for an orthonormal completion
+  hipLaunchKernelGGL(k_complete_columns, dim3(1), dim3(kCompleteThreads), 0, st, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowops_batched(int kid, int variant, const void* tab, const int* start, int count, unsigned total,
+                                 unsigned block, size_t lds, hipStream_t st) {
+  const unsigned char* t = static_cast<const unsigned char*>(tab);
+  const dim3 g(total), b(block);
+  switch (kid) {
+    case KID_ROWS_D:
+    case KID_ROWS_F: {
+#define LRI_ROWS_BATCH(PT)                                                                                       \
+  switch (variant) {                                                                                             \
+    case 100: hipLaunchKernelGGL((k_rows_batch<PT, 1, 0>), g, b, lds, st, t, start, count); break;              \
+    case 200: hipLaunchKernelGGL((k_rows_batch<PT, 2, 0>), g, b, lds, st, t, start, count); break;              \
+    case 400: hipLaunchKernelGGL((k_rows_batch<PT, 4, 0>), g, b, lds, st, t, start, count); break;              \
+    case 800: hipLaunchKernelGGL((k_rows_batch<PT, 8, 0>), g, b, lds, st, t, start, count); break;              \
+    case 1600: hipLaunchKernelGGL((k_rows_batch<PT, 16, 0>), g, b, lds, st, t, start, count); break;            \
+    default: return hipErrorInvalidValue;                                                                        \
+  }
+      if (kid == KID_ROWS_F) {
+        LRI_ROWS_BATCH(float)
+      } else {
+        switch (variant) {
+          case 116: hipLaunchKernelGGL((k_rows_batch<double, 1, 16>), g, b, lds, st, t, start, count); break;
+          case 216: hipLaunchKernelGGL((k_rows_batch<double, 2, 16>), g, b, lds, st, t, start, count); break;
+          case 132: hipLaunchKernelGGL((k_rows_batch<double, 1, 32>), g, b, lds, st, t, start, count); break;
+          case 232: hipLaunchKernelGGL((k_rows_batch<double, 2, 32>), g, b, lds, st, t, start, count); break;
+          default: { LRI_ROWS_BATCH(double) }
+        }
+      }
+#undef LRI_ROWS_BATCH
+      break;
+    }
+    case KID_REDUCE: hipLaunchKernelGGL(k_reduce_batch, g, b, lds, st, t, start, count); break;
+    case KID_EPILOGUE:
+      if (variant == 16) hipLaunchKernelGGL(k_epilogue_batch<16>, g, b, lds, st, t, start, count);
+      else if (variant == 32) hipLaunchKernelGGL(k_epilogue_batch<32>, g, b, lds, st, t, start, count);
+      else hipLaunchKernelGGL(k_epilogue_batch<64>, g, b, lds, st, t, start, count);
+      break;
+    case KID_JACOBI_WAVE: hipLaunchKernelGGL(k_jacobi_wave_batch, g, b, lds, st, t, start, count); break;
+    case KID_JACOBI_WG: hipLaunchKernelGGL(k_jacobi_svd_batch, g, b, lds, st, t, start, count); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -302,56 +302,87 @@ __global__ __launch_bounds__(kPassThreads) void k_passT_f64(const T* __restrict_
 // FP32 passes (v_mfma_f32_32x32x2_f32) for F32 snapshots
 // ============================================================================
 
-// pass N: tile = 32*TM rows; an iteration covers 16 columns (8 k-steps of 2).
+typedef __attribute__((ext_vector_type(2))) float f32x2v_t;
+
+// TM consecutive floats of a column (rows r0 .. r0+TM-1 masked against rend) as
+// one vector load when aligned and in range.
+template <int TM>
+__device__ __forceinline__ void load_rows_f32(const float* __restrict__ p, int64_t off, int64_t r0, int64_t rend,
+                                              bool ok, bool vec, float (&v)[TM]) {
+  if (ok && vec && r0 + TM <= rend) {
+    if constexpr (TM == 4) {
+      const f32x4_t x = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p + off));
+      v[0] = x[0]; v[1] = x[1]; v[2] = x[2]; v[3] = x[3];
+      return;
+    } else if constexpr (TM == 2) {
+      const f32x2v_t x = __builtin_nontemporal_load(reinterpret_cast<const f32x2v_t*>(p + off));
+      v[0] = x[0]; v[1] = x[1];
+      return;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < TM; ++e) v[e] = (ok && r0 + e < rend) ? p[off + e] : 0.0f;
+}
+
+// pass N: tile = 32*TM rows.  Lane (lr = l%32, h = l/32) owns the TM consecutive
+// rows i0 + TM*lr .. i0 + TM*lr + TM-1 of a column (one 16-byte load for TM = 4)
+// and feeds row i0 + TM*lr + t to instruction t, so accumulator t holds the
+// output rows i0 + TM*d + t (d = 0..31).  A wave load instruction covers two
+// columns of 128*TM contiguous bytes.  An iteration covers 8 columns (4 k-steps
+// of 2); B (V) is row-major [j][rb], L2-resident.
 template <int TM, int CB>
 __global__ __launch_bounds__(kPassThreads) void k_passN_f32(const float* __restrict__ an,
                                                             const float* __restrict__ ap, int64_t lda, int64_t n,
                                                             int64_t m, const float* __restrict__ B, int rb,
-                                                            int64_t jchunk, float* __restrict__ part) {
+                                                            int64_t jchunk, float* __restrict__ part, int vec) {
   constexpr int NV = TM * CB * 16;
   __shared__ float red[3 * NV * 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lr = lane & 31, lk = lane >> 5;
+  const int lr = lane & 31, h = lane >> 5;
   const int64_t i0 = (int64_t)blockIdx.x * (32 * TM);
+  const int64_t r0 = i0 + TM * lr;
   const int s = blockIdx.y;
   const int64_t jb = (int64_t)s * jchunk;
   const int64_t je = min(m, jb + jchunk);
-  const WaveRange R = wave_range(jb, je, wave, 16);
+  const WaveRange R = wave_range(jb, je, wave, 8);
+  const bool v = vec != 0;
   f32x16_t acc[TM][CB];
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c) acc[t][c] = f32x16_t{};
-  float a[8][TM], b[8][CB], na[8][TM], nb[8][CB];
-  auto load = [&](int64_t j, float (&xa)[8][TM], float (&xb)[8][CB]) {
+  float xa[4][TM], xp[4][TM], b[4][CB], nxa[4][TM], nxp[4][TM], nb[4][CB];
+  auto load = [&](int64_t j, float (&A1)[4][TM], float (&A0)[4][TM], float (&Bv)[4][CB]) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int64_t jj = j + 2 * u + lk;
+    for (int u = 0; u < 4; ++u) {
+      const int64_t jj = j + 2 * u + h;
       const bool jok = jj < je;
+      load_rows_f32<TM>(an, jj * lda + r0, r0, n, jok, v, A1[u]);
+      load_rows_f32<TM>(ap, jj * lda + r0, r0, n, jok && ap != nullptr, v, A0[u]);  // no ap: A itself
 #pragma unroll
-      for (int c = 0; c < CB; ++c) xb[u][c] = jok ? B[jj * rb + c * 32 + lr] : 0.0f;
-#pragma unroll
-      for (int t = 0; t < TM; ++t) {
-        const int64_t ii = i0 + 32 * t + lr;
-        xa[u][t] = (jok && ii < n) ? an[jj * lda + ii] - (ap ? ap[jj * lda + ii] : 0.0f) : 0.0f;
-      }
+      for (int c = 0; c < CB; ++c) Bv[u][c] = jok ? B[jj * rb + c * 32 + lr] : 0.0f;
     }
   };
-  if (R.b < R.e) load(R.b, a, b);
-  for (int64_t j = R.b; j < R.e; j += 16) {
-    const bool more = j + 16 < R.e;
-    if (more) load(j + 16, na, nb);
+  if (R.b < R.e) load(R.b, xa, xp, b);
+  for (int64_t j = R.b; j < R.e; j += 8) {
+    const bool more = j + 8 < R.e;
+    if (more) load(j + 8, nxa, nxp, nb);
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int t = 0; t < TM; ++t)
+      for (int t = 0; t < TM; ++t) {
+        const float d = xa[u][t] - xp[u][t];
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f32(a[u][t], b[u][c], acc[t][c]);
+        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f32(d, b[u][c], acc[t][c]);
+      }
     if (more) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 4; ++u) {
 #pragma unroll
-        for (int t = 0; t < TM; ++t) a[u][t] = na[u][t];
+        for (int t = 0; t < TM; ++t) {
+          xa[u][t] = nxa[u][t];
+          xp[u][t] = nxp[u][t];
+        }
 #pragma unroll
         for (int c = 0; c < CB; ++c) b[u][c] = nb[u][c];
       }
@@ -363,98 +394,130 @@ __global__ __launch_bounds__(kPassThreads) void k_passN_f32(const float* __restr
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
-      for (int v = 0; v < 16; ++v) flat[(t * CB + c) * 16 + v] = acc[t][c][v];
+      for (int w = 0; w < 16; ++w) flat[(t * CB + c) * 16 + w] = acc[t][c][w];
   wg_sum<float, NV>(flat, red, wave, lane);
   if (wave != 0) return;
+  // instruction row d = 8 (w / 4) + 4 h + w % 4 of accumulator t is output row i0 + TM d + t
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t ii = i0 + 32 * t + 8 * (v / 4) + 4 * lk + (v % 4);
-        if (ii < n) part[((int64_t)s * rb + c * 32 + lr) * n + ii] = flat[(t * CB + c) * 16 + v];
+      for (int w = 0; w < 16; ++w) {
+        const int64_t ii = i0 + TM * (8 * (w / 4) + 4 * h + (w % 4)) + t;
+        if (ii < n) part[((int64_t)s * rb + c * 32 + lr) * n + ii] = flat[(t * CB + c) * 16 + w];
       }
 }
 
-// pass T: tile = 32*TJ columns; an iteration covers 16 rows (2 groups of 8,
-// each lane owning 4 consecutive rows of its column per group).
-template <int TJ, int CB>
-__global__ __launch_bounds__(kPassThreads) void k_passT_f32(const float* __restrict__ an,
-                                                            const float* __restrict__ ap, int64_t lda, int64_t n,
-                                                            int64_t m, const float* __restrict__ B, int rb,
-                                                            int64_t ichunk, float* __restrict__ part) {
-  constexpr int NV = TJ * CB * 16;
-  __shared__ float red[3 * NV * 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lr = lane & 31, lk = lane >> 5;
-  const int64_t j0 = (int64_t)blockIdx.x * (32 * TJ);
+// pass T, wide form: a workgroup covers 128 columns of dA (wave w the columns
+// 32w .. 32w+31), all four waves over the same rows, 32 rows per iteration.
+// Each wave reads its 32 x 32 tile of each snapshot with 8 lanes per column (128
+// contiguous bytes, 8 columns per wave instruction), stages the FP32 differences
+// column-major in its LDS region (pitch 36 floats) and reads back, as lane
+// (column lc, half h), the rows 16h .. 16h+15 of its column for k-steps
+// e = 0..15.  The B block of the 32 rows (pass-T order, see pk_f32_offset: the
+// 16 rows of (half h, column c) contiguous) is loaded once per workgroup into
+// double-buffered LDS, prefetched into registers one block ahead.  Every wave
+// owns its 32 output rows: no cross-wave reduction.
+constexpr int kF32Pitch = 36;
+
+template <int CB>
+__global__ __launch_bounds__(kPassThreads) void k_passT_f32_wide(const float* __restrict__ an,
+                                                                 const float* __restrict__ ap, int64_t lda,
+                                                                 int64_t n, int64_t m, const float* __restrict__ B,
+                                                                 int rb, int64_t ichunk, float* __restrict__ part,
+                                                                 int vec) {
+  constexpr int BQ = 32 * 32 * CB / 4 / kPassThreads;  // 16-byte pieces of a B block per thread
+  __shared__ __attribute__((aligned(16))) float sA[4][32 * kF32Pitch];
+  __shared__ f32x4_t sB[2][BQ * kPassThreads];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int lc = lane & 31, h = lane >> 5;
+  const int64_t j0 = (int64_t)blockIdx.x * 128 + 32 * wave;
   const int s = blockIdx.y;
   const int64_t ib = (int64_t)s * ichunk;
   const int64_t ie = min(n, ib + ichunk);
-  const WaveRange R = wave_range(ib, ie, wave, 16);
-  f32x16_t acc[TJ][CB];
+  const bool v = vec != 0;
+  float* tA = sA[wave];
+  f32x16_t acc[CB];
 #pragma unroll
-  for (int t = 0; t < TJ; ++t)
+  for (int c = 0; c < CB; ++c) acc[c] = f32x16_t{};
+  // load lane mapping: instruction p covers columns 8p .. 8p+7; lane -> column 8p + lane/8, rows 4*(lane%8)
+  const int lcol = lane >> 3, loct = lane & 7;
+  float xn[4][4], xp[4][4], nxn[4][4], nxp[4][4];
+  auto load = [&](int64_t i, float (&A1)[4][4], float (&A0)[4][4]) {
 #pragma unroll
-    for (int c = 0; c < CB; ++c) acc[t][c] = f32x16_t{};
-  float a[TJ][8], b[8][CB], na[TJ][8], nb[8][CB];
-  auto load = [&](int64_t i, float (&xa)[TJ][8], float (&xb)[8][CB]) {
-#pragma unroll
-    for (int t = 0; t < TJ; ++t) {
-      const int64_t jj = j0 + 32 * t + lr;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int64_t ii = i + 8 * (e / 4) + 4 * lk + (e % 4);
-        xa[t][e] = (jj < m && ii < ie) ? an[jj * lda + ii] - (ap ? ap[jj * lda + ii] : 0.0f) : 0.0f;
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int64_t ii = i + 8 * (e / 4) + 4 * lk + (e % 4);
-#pragma unroll
-      for (int c = 0; c < CB; ++c) xb[e][c] = ii < ie ? B[ii * rb + c * 32 + lr] : 0.0f;
+    for (int p = 0; p < 4; ++p) {
+      const int64_t jj = j0 + 8 * p + lcol;
+      const int64_t r0 = i + 4 * loct;
+      load_rows_f32<4>(an, jj * lda + r0, r0, ie, jj < m, v, A1[p]);
+      load_rows_f32<4>(ap, jj * lda + r0, r0, ie, jj < m && ap != nullptr, v, A0[p]);  // no ap: A itself
     }
   };
-  if (R.b < R.e) load(R.b, a, b);
-  for (int64_t i = R.b; i < R.e; i += 16) {
-    const bool more = i + 16 < R.e;
-    if (more) load(i + 16, na, nb);
+  auto loadB = [&](int64_t i, f32x4_t (&g)[BQ]) {
+    const f32x4_t* src = reinterpret_cast<const f32x4_t*>(B + (i >> 5) * 32 * rb);
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
+    for (int q = 0; q < BQ; ++q) g[q] = src[q * kPassThreads + t];
+  };
+  f32x4_t g[BQ];
+  int buf = 0;
+  if (ib < ie) {
+    load(ib, xn, xp);
+    loadB(ib, g);
 #pragma unroll
-      for (int t = 0; t < TJ; ++t)
+    for (int q = 0; q < BQ; ++q) sB[0][q * kPassThreads + t] = g[q];
+  }
+  __syncthreads();
+  for (int64_t i = ib; i < ie; i += 32) {
+    const bool more = i + 32 < ie;
+    if (more) {
+      load(i + 32, nxn, nxp);
+      loadB(i + 32, g);
+    }
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[t][c] = mfma_f32(a[t][e], b[e][c], acc[t][c]);
+    for (int p = 0; p < 4; ++p) {
+      const int col = 8 * p + lcol;
+      f32x4_t d;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) d[e] = xn[p][e] - xp[p][e];
+      *reinterpret_cast<f32x4_t*>(tA + col * kF32Pitch + 4 * loct) = d;
+    }
+    wave_lds_fence();
+    f32x4_t a4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a4[q] = *reinterpret_cast<const f32x4_t*>(tA + lc * kF32Pitch + 16 * h + 4 * q);
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const f32x4_t* bp = reinterpret_cast<const f32x4_t*>(&sB[buf][0]) + (h * rb + c * 32 + lc) * 4;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const f32x4_t b4 = bp[q];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[c] = mfma_f32(a4[q][e], b4[e], acc[c]);
+      }
+    }
     if (more) {
 #pragma unroll
-      for (int t = 0; t < TJ; ++t)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) a[t][e] = na[t][e];
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-#pragma unroll
-        for (int c = 0; c < CB; ++c) b[e][c] = nb[e][c];
+      for (int q = 0; q < BQ; ++q) sB[buf ^ 1][q * kPassThreads + t] = g[q];
     }
+    __syncthreads();  // sB[buf] read by all waves, sB[buf ^ 1] written; the A staging is free again
+    if (more) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          xn[p][e] = nxn[p][e];
+          xp[p][e] = nxp[p][e];
+        }
+    }
+    buf ^= 1;
   }
-  float flat[NV];
 #pragma unroll
-  for (int t = 0; t < TJ; ++t)
+  for (int c = 0; c < CB; ++c)
 #pragma unroll
-    for (int c = 0; c < CB; ++c)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) flat[(t * CB + c) * 16 + v] = acc[t][c][v];
-  wg_sum<float, NV>(flat, red, wave, lane);
-  if (wave != 0) return;
-#pragma unroll
-  for (int t = 0; t < TJ; ++t)
-#pragma unroll
-    for (int c = 0; c < CB; ++c)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int64_t jj = j0 + 32 * t + 8 * (v / 4) + 4 * lk + (v % 4);
-        if (jj < m) part[((int64_t)s * rb + c * 32 + lr) * m + jj] = flat[(t * CB + c) * 16 + v];
-      }
+    for (int w = 0; w < 16; ++w) {
+      const int64_t jj = j0 + 8 * (w / 4) + 4 * h + (w % 4);
+      if (jj < m) part[((int64_t)s * rb + c * 32 + lc) * m + jj] = acc[c][w];
+    }
 }
 
 // ============================================================================
@@ -1077,7 +1140,7 @@ static int tile_units(int mode, bool transposed, int rb) {
     return cb <= 2 ? 4 : (cb <= 4 ? 2 : 1);
   }
   int cb = rb / 32;
-  if (mode == PASS_F32) return cb <= 1 ? 2 : 1;
+  if (mode == PASS_F32) return transposed ? 4 : (cb <= 1 ? 4 : (cb == 2 ? 2 : 1));  // pass T: the wide form
   if (transposed) return cb <= 1 ? 4 : 1;  // rb = 32: the wide pass T (128 columns per workgroup)
   return cb <= 1 ? 4 : (cb == 2 ? 2 : 1);
 }
@@ -1086,28 +1149,61 @@ static int wg_span(int mode, bool transposed, int rb) {
   return (mode == PASS_F64 ? 16 : 32) * tile_units(mode, transposed, rb);
 }
 
-int64_t plan_pass(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int* splits,
-                  int64_t* chunk) {
+constexpr int64_t kChunkQuantum = 256;  // 4 waves x the largest per-wave iteration (64 rows)
+
+// The reduction chunk for about `target` workgroups, with at least 1024 of the
+// reduction range per split.
+static int64_t chunk_for_target(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb,
+                                int64_t target) {
   const int span = wg_span(mode, transposed, rb);
   const int64_t tiles = (out_rows + span - 1) / span;
-  const int64_t quantum = 256;  // 4 waves x the largest per-wave iteration (64 rows)
-  // aim for ~512 workgroups (2048 waves) with at least 1024 of the reduction range per split
-  int64_t want = (512 + tiles - 1) / tiles;
+  int64_t want = (target + tiles - 1) / tiles;
   const int64_t maxs = (red_len + 1023) / 1024;
   if (want > maxs) want = maxs;
   if (want < 1) want = 1;
-  int64_t ch = (red_len + want - 1) / want;
-  ch = (ch + quantum - 1) / quantum * quantum;
+  const int64_t ch = (red_len + want - 1) / want;
+  return (ch + kChunkQuantum - 1) / kChunkQuantum * kChunkQuantum;
+}
+
+int64_t plan_pass_chunk(int64_t out_rows, int64_t red_len, int rb, int64_t ch, int* splits, int64_t* chunk) {
+  ch = (ch + kChunkQuantum - 1) / kChunkQuantum * kChunkQuantum;
+  if (ch < kChunkQuantum) ch = kChunkQuantum;
   const int64_t s = (red_len + ch - 1) / ch;
   *splits = (int)s;
   *chunk = ch;
   return s * rb * out_rows;
 }
 
+// the default: ~512 workgroups (2048 waves)
+int64_t plan_pass(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int* splits,
+                  int64_t* chunk) {
+  return plan_pass_chunk(out_rows, red_len, rb, chunk_for_target(mode, transposed, out_rows, red_len, rb, 512),
+                         splits, chunk);
+}
+
+int pass_chunk_candidates(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int64_t* chunks,
+                          int cap) {
+  int k = 0;
+  for (int64_t target : {128, 192, 256, 384, 512, 640, 768, 1024, 1280, 1536, 2048, 3072}) {
+    const int64_t ch = chunk_for_target(mode, transposed, out_rows, red_len, rb, target);
+    bool dup = false;
+    for (int i = 0; i < k; ++i) dup = dup || chunks[i] == ch;
+    if (!dup && k < cap) chunks[k++] = ch;
+  }
+  return k;
+}
+
 // vector loads of 4 consecutive elements need 4-element-aligned columns
 static int vec_f64(const PassArgs& a, size_t esz) {
   const size_t al = esz == 8 ? 16 : 4 * esz;
   return (a.lda % 4 == 0) && ((uintptr_t)a.anext % al == 0) && ((uintptr_t)a.aprev % al == 0) ? 1 : 0;
+}
+
+// vector loads of TM consecutive floats need TM-aligned columns
+static int vec_f32(const PassArgs& a, int tm) {
+  if (tm <= 1) return 0;
+  const size_t al = 4 * (size_t)tm;
+  return (a.lda % tm == 0) && ((uintptr_t)a.anext % al == 0) && ((uintptr_t)a.aprev % al == 0) ? 1 : 0;
 }
 
 static bool vec_ok(const PassArgs& a) {
@@ -1139,12 +1235,18 @@ static hipError_t run_pass(int mode, bool transposed, const PassArgs& a, hipStre
     }
 #undef LRI_F64_LAUNCH
   } else if (mode == PASS_F32) {
-    if (transposed)
-      hipLaunchKernelGGL((k_passT_f32<TM, CB>), grid, blk, 0, st, (const float*)a.anext, (const float*)a.aprev,
-                         a.lda, a.n, a.m, (const float*)a.B, a.rb, a.chunk, (float*)a.part);
-    else
-      hipLaunchKernelGGL((k_passN_f32<TM, CB>), grid, blk, 0, st, (const float*)a.anext, (const float*)a.aprev,
-                         a.lda, a.n, a.m, (const float*)a.B, a.rb, a.chunk, (float*)a.part);
+    const float* an = (const float*)a.anext;
+    const float* ap = (const float*)a.aprev;
+    if (transposed) {
+      // tile = 128 columns whatever TM (the wide form)
+      const dim3 gw((unsigned)((a.m + 127) / 128), (unsigned)a.splits);
+      const int vw = vec_f32(a, 4);
+      hipLaunchKernelGGL((k_passT_f32_wide<CB>), gw, blk, 0, st, an, ap, a.lda, a.n, a.m, (const float*)a.B, a.rb,
+                         a.chunk, (float*)a.part, vw);
+    } else {
+      hipLaunchKernelGGL((k_passN_f32<TM, CB>), grid, blk, 0, st, an, ap, a.lda, a.n, a.m, (const float*)a.B, a.rb,
+                         a.chunk, (float*)a.part, vec_f32(a, TM));
+    }
   } else {
     const uint16_t* an = (const uint16_t*)a.anext;
     const uint16_t* ap = (const uint16_t*)a.aprev;
@@ -1180,6 +1282,7 @@ static hipError_t run_pass(int mode, bool transposed, const PassArgs& a, hipStre
 }
 
 static hipError_t launch_pass(int mode, bool transposed, const PassArgs& a, hipStream_t st) {
+  if (current_recorder()) return hipErrorNotSupported;  // the passes have no batched form
   if (mode == PASS_BF16X3 && a.dtype != LRI_BF16) return hipErrorInvalidValue;
   if (mode == PASS_F32 && a.dtype != LRI_F32) return hipErrorInvalidValue;
   const int tu = tile_units(mode, transposed, a.rb);

@@ -55,8 +55,10 @@ SIGNATURES = {
                                        _i64]),
     "lri_step_greedy": (_int, [_vp, _i64]),
     "lri_step_greedy_twofactor": (_int, [_vp, _i64]),
+    "lri_step_hybrid_greedy": (_int, [_vp, _i64, ctypes.c_double, _i64]),
     "lri_set_operator_lyapunov": (_int, [_vp, _dp, _dp, _int, _dp, _dp, _int, _i64, _dp, _i64, _dp, _i64]),
     "lri_set_operator_quadratic": (_int, [_vp, _dp, _dp, _dp, _int]),
+    "lri_set_resident_eigenbasis": (_int, [_vp, _int]),
     "lri_step_de_ksl": (_int, [_vp, _int, ctypes.c_double, _i64]),
     "lri_step_de_bug": (_int, [_vp, ctypes.c_double, _i64]),
     "lri_step_de_bug_ra": (_int, [_vp, ctypes.c_double, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
@@ -71,12 +73,20 @@ SIGNATURES = {
     "lri_batch_step_de_bug_ra": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_i64), ctypes.c_double, _i64,
                                         ctypes.POINTER(_i64)]),
     "lri_batch_use_graphs": (_int, [_vp, _int]),
+    "lri_batch_use_merged_launches": (_int, [_vp, _int]),
+    "lri_batch_launch_stats": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "lri_batch_member_status": (_int, [_vp, _i64, ctypes.POINTER(_int)]),
     "lri_batch_synchronize": (_int, [_vp]),
     "lri_synchronize": (_int, [_vp]),
+    "lri_trajectory_reserve": (_int, [_vp, _i64]),
+    "lri_trajectory_save": (_int, [_vp, _i64]),
+    "lri_trajectory_get": (_int, [_vp, _i64, ctypes.POINTER(_i64), _dp, _i64, _dp, _i64, _dp, _i64]),
     "lri_get_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "lri_kernel_timing": (_int, [_vp, _int]),
     "lri_kernel_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)]),
+    "lri_kernel_timing_sample": (_int, [_vp, _int]),
+    "lri_kernel_timing_samples": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_float), _i64, ctypes.POINTER(_i64)]),
+    "lri_autotune_passes": (_int, [_vp, _i64, _i64, _i64, ctypes.POINTER(ctypes.c_double)]),
     "lri_selftest_mfma": (_int, [_int, _vp, _vp, _vp]),
     "lri_stream_probe": (_int, [_int, _dp]),
     "lri_debug_product": (_int, [_vp, _int, _i64, _i64, _dp, _i64]),
@@ -279,6 +289,13 @@ class Context:
     def step_greedy_twofactor(self, kn: int):
         check(lib().lri_step_greedy_twofactor(self.h, int(kn)), self.h)
 
+    def set_resident_eigenbasis(self, on: bool):
+        """Keep the state in the operator's analytic eigenbasis (constant coefficients, linear)."""
+        check(lib().lri_set_resident_eigenbasis(self.h, int(bool(on))), self.h)
+
+    def step_hybrid_greedy(self, kn: int, dt: float, substeps: int):
+        check(lib().lri_step_hybrid_greedy(self.h, int(kn), float(dt), int(substeps)), self.h)
+
     # -- matrix ODE
     def set_operator_lyapunov(self, a_diag, a_off, a_periodic, b_diag, b_off, b_periodic, G=None, H=None):
         """F(Y) = A Y + Y B' + G H' with A, B symmetric tridiagonal (diagonal, off-diagonal)."""
@@ -379,6 +396,52 @@ class Context:
         cnt = _i64()
         check(lib().lri_kernel_timing_read(self.h, int(family), ctypes.byref(ms), ctypes.byref(cnt)), self.h)
         return float(ms.value), int(cnt.value)
+
+    def trajectory_reserve(self, slots: int):
+        """Device storage for ``slots`` saved states (replaces any earlier reservation)."""
+        check(lib().lri_trajectory_reserve(self.h, int(slots)), self.h)
+
+    def trajectory_save(self, slot: int):
+        """Queue a device copy of the current state into ``slot`` (no synchronisation)."""
+        check(lib().lri_trajectory_save(self.h, int(slot)), self.h)
+
+    def trajectory_get(self, slot: int):
+        """(U, S, V) saved in ``slot`` (synchronising)."""
+        r = _i64()
+        check(lib().lri_trajectory_get(self.h, int(slot), ctypes.byref(r), None, self.n, None, 1, None, self.m), self.h)
+        rr = int(r.value)
+        U = np.zeros((self.n, rr), order="F")
+        S = np.zeros((rr, rr), order="F")
+        V = np.zeros((self.m, rr), order="F")
+        check(lib().lri_trajectory_get(self.h, int(slot), ctypes.byref(r), _dptr(U), self.n, _dptr(S), rr, _dptr(V),
+                                       self.m), self.h)
+        return U, S, V
+
+    def timing_sample(self, every: int):
+        """Time one launch in ``every`` of each family."""
+        check(lib().lri_kernel_timing_sample(self.h, int(every)), self.h)
+
+    def timing_samples(self, family: int) -> np.ndarray:
+        """Per-launch durations (ms) of a family since timing was enabled."""
+        cnt = _i64()
+        check(lib().lri_kernel_timing_samples(self.h, int(family), None, 0, ctypes.byref(cnt)), self.h)
+        out = np.zeros(int(cnt.value), np.float32)
+        if out.size:
+            check(lib().lri_kernel_timing_samples(self.h, int(family), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                  out.size, ctypes.byref(cnt)), self.h)
+        return out.astype(np.float64)
+
+    def autotune_passes(self, kp: int, kn: int, reps: int = 20) -> dict:
+        """Time the candidate reduction chunks of pass N and pass T over A[kn] - A[kp] at the
+        current rank and precision and keep the fastest (lri_autotune_passes)."""
+        rep = np.zeros(8, np.float64)
+        check(lib().lri_autotune_passes(self.h, int(kp), int(kn), int(reps),
+                                        rep.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), self.h)
+        out = {}
+        for d, name in enumerate(("pass_n", "pass_t")):
+            c, s_, us, dus = rep[4 * d:4 * d + 4]
+            out[name] = {"chunk": int(c), "splits": int(s_), "us": float(us), "default_us": float(dus)}
+        return out
 
 
 def selftest_mfma(which: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
@@ -490,6 +553,17 @@ class Batch:
 
     def use_graphs(self, on: bool):
         _check_batch(lib().lri_batch_use_graphs(self.h, int(bool(on))), self.h)
+
+    def use_merged_launches(self, on: bool):
+        """Merged launches (default on): the members share one stream, and a rank-adaptive
+        step of members in their resident eigenbasis runs each stage as one launch over all."""
+        _check_batch(lib().lri_batch_use_merged_launches(self.h, int(bool(on))), self.h)
+
+    def launch_stats(self):
+        """(batch steps run as merged launches, kernel launches of the last of them)."""
+        a, b = _i64(), _i64()
+        _check_batch(lib().lri_batch_launch_stats(self.h, ctypes.byref(a), ctypes.byref(b)), self.h)
+        return a.value, b.value
 
     def synchronize(self):
         _check_batch(lib().lri_batch_synchronize(self.h), self.h)

@@ -7,6 +7,7 @@ Reference files followed:
 * DE sub-problem wiring, KSL ......... ``src/integrators/projector_splitting.jl:43-85,129-189``
 * DE sub-problem wiring, BUG ......... ``src/integrators/unconventional.jl:39-84,133-157``
 * DE sub-problem wiring, RA-BUG ...... ``src/integrators/rank_adaptive_unconventional.jl:47-92,194-233``
+* Greedy on a MatrixHybridProblem .... ``src/integrators/greedy_integrator.jl:41-46,61-82,106-111``
 * driver ............................. ``src/primitives.jl:68-80``
 
 The reference integrates each sub-problem (K, S, L) with an OrdinaryDiffEq
@@ -264,6 +265,62 @@ def rank_adaptive_bug_de_step_lowrank(op: LowRankLyapunov, U, S, V, t, dt, N, to
     Us, sig, Vst = np.linalg.svd(Sh)
     r1 = int(min(rmax, truncate_to_tolerance(sig, tol)))
     return Uh @ Us[:, :r1], np.diag(sig[:r1]), Vh @ Vst[:r1].T, r1
+
+
+# ----------------------------------------------------------------------------
+# the Greedy integrator on a MatrixHybridProblem (greedy_integrator.jl:41-82)
+# ----------------------------------------------------------------------------
+
+
+@dataclass
+class MatrixHybridProblem:
+    """``MatrixHybridProblem{y,f,u0,tspan}`` (primitives.jl:36-41): data y(t), and the
+    Z dynamics dZ/dt = f(Z, U, t) of a two-factor state U Z'.  Here f is given by
+    an operator F on n x m matrices, f(Z, U, t) = F(U Z', t)' U, the form of the
+    reference's own hybrid test (test/data_informed_approximation.jl:75,
+    FZ(Z, U, t) = burgers(U Z')' U)."""
+
+    y: Callable[[float], np.ndarray]
+    F: Callable[[np.ndarray, float], np.ndarray]
+    U0: np.ndarray
+    Z0: np.ndarray
+    tspan: tuple
+
+
+def greedy_hybrid_step(F, U, Z, X, t, dt, N):
+    """greedy_step!(::TwoFactorRepresentation, ..., ::Type{<:MatrixHybridProblem})
+    (greedy_integrator.jl:73-82): the Z integrator (an ODEProblem in Z with the
+    parameter U, :44) advances Z over [t, t + dt] with U held (here N classical
+    RK4 steps instead of Tsit5), then X = y(t + dt), XZ = X Z and U = Q P' from the
+    SVD XZ = Q Sigma P' (U is updated in place after Z, so the Z step sees the old U).
+    Returns (U1, Z1)."""
+    Z1 = rk4(_L_rhs(F, U), Z, t, dt, N)
+    Q, _, Pt = np.linalg.svd(X @ Z1, full_matrices=False)
+    return Q @ Pt, Z1
+
+
+def solve_hybrid(prob: MatrixHybridProblem, dt, substeps: int):
+    """``solve(prob::MatrixHybridProblem, GreedyIntegrator(), dt)`` (primitives.jl:68-76,
+    greedy_integrator.jl:61-71,106-111); returns the list of (U, Z) and of t."""
+    t0, tf = prob.tspan
+    if not tf > t0:
+        raise AssertionError("Integration in reverse time direction is not supported")
+    U, Z = prob.U0.copy(), prob.Z0.copy()
+    Ys: List[Any] = [None] * _init_sol_len(dt, t0, tf)
+    ts: List[Any] = list(np.linspace(t0, tf, len(Ys)))
+    Ys[0] = (U.copy(), Z.copy())
+    t, it, T = t0, 0, tf - t0
+    while (tf - t) / T > 1e-8:
+        U, Z = greedy_hybrid_step(prob.F, U, Z, np.asarray(prob.y(t + dt), dtype=np.float64), t, dt, substeps)
+        t = t + dt
+        it += 1
+        if it <= len(Ys) - 1:
+            Ys[it] = (U.copy(), Z.copy())
+            ts[it] = t
+        else:
+            Ys.append((U.copy(), Z.copy()))
+            ts.append(t)
+    return Ys, ts
 
 
 def solve_de(prob: MatrixDEProblem, alg, dt, substeps: int) -> DLRSolution:

@@ -182,3 +182,80 @@ def test_batch_cfg3_distribution(lri):
         e = rel_err(*got[i], Uo, So, Vo)
         say(f"  member {i} (n={n}, r0={r}) vs oracle: rank {So.shape[0]}, rel err {e:.2e}")
         assert got[i][1].shape == So.shape and e <= 1e-9
+
+
+def resident_batch(lri, probs, rcaps, skip=()):
+    b = lri.native.Batch([p[0] for p in probs], [p[0] for p in probs], rcaps)
+    for k, ((n, r, U, S, V), c) in enumerate(zip(probs, b.members)):
+        Lop = lri.laplacian_1d(n)
+        c.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False)
+        c.set_resident_eigenbasis(True)
+        if k not in skip:
+            c.set_state(U, S, V)
+    return b
+
+
+def test_merged_batch_matches_single_contexts(lri):
+    """Merged launches (every member in its resident eigenbasis): 40 members drawn like the
+    batched workload, three rank-adaptive steps at tol 1e-8 (ranks change), one member left
+    without a state (it reports LRI_EINVAL and the others advance).  Every other member
+    equals a resident context of its own bit for bit, and the step is a few hundred
+    launches for all members, not one sequence per member."""
+    count, dt, tol, rmax, steps, bad = 40, 1e-4, 1e-8, 64, 3, 7
+    probs = cfg3_problems(count, seed=5)
+    Ns = [lri.rk4_substeps(2.0 * lri.laplacian_1d(n).gershgorin(), dt) for n, *_ in probs]
+    say(f"case: merged batch, {count} members, ranks {[p[1] for p in probs]}, member {bad} without a state")
+    with resident_batch(lri, probs, [rmax] * count, skip=(bad,)) as b:
+        ranks = []
+        for _ in range(steps):
+            with pytest.raises(ValueError, match=f"problem {bad}: state not set"):
+                b.step_de_bug_ra(dt, Ns, tol, rmax)
+            st = b.member_status()
+            assert st[bad] == lri.native.EINVAL and not np.delete(st, bad).any()
+            ranks.append([c.rank() for c in b.members])
+        merged, launches = b.launch_stats()
+        got = [c.get_state() if k != bad else None for k, c in enumerate(b.members)]
+    say(f"  {merged} merged steps, {launches} launches in the last; ranks {ranks[-1]}")
+    assert merged == steps and 0 < launches < 8 * count
+    for i, (n, r, U, S, V) in enumerate(probs):
+        if i == bad:
+            continue
+        with lri.native.Context(n, n, rmax) as c:
+            Lop = lri.laplacian_1d(n)
+            c.set_operator_lyapunov(Lop.d, Lop.e, False, Lop.d, Lop.e, False)
+            c.set_resident_eigenbasis(True)
+            c.set_state(U, S, V)
+            rk = [c.step_de_bug_ra(dt, Ns[i], tol, rmax) for _ in range(steps)]
+            ref = c.get_state()
+        assert [x[i] for x in ranks] == rk, f"member {i}"
+        for a, b_ in zip(got[i], ref):
+            assert a.shape == b_.shape and np.array_equal(a, b_), f"member {i}"
+
+
+def test_merged_batch_graph_replays(lri):
+    """tol = 0 with each member's capacity at its rank: the ranks stay, the merged list
+    repeats and is replayed as a captured graph from the second step on.  Six steps with the
+    states read back between them must equal the same batch with direct launches, and the
+    merged batch must equal the batch without merged launches (one stream per member)."""
+    count, dt, steps = 24, 1e-4, 6
+    probs = cfg3_problems(count, seed=6)
+    rs = [p[1] for p in probs]
+    Ns = [lri.rk4_substeps(2.0 * lri.laplacian_1d(n).gershgorin(), dt) for n, *_ in probs]
+    say(f"case: merged graph replays, {count} members, fixed ranks {rs}, {steps} steps")
+    seqs = {}
+    for mode in ("graphs", "direct", "unmerged"):
+        with resident_batch(lri, probs, rs) as b:
+            b.use_graphs(mode != "direct")
+            b.use_merged_launches(mode != "unmerged")
+            seq = []
+            for _ in range(steps):
+                assert list(b.step_de_bug_ra(dt, Ns, 0.0, 64)) == rs
+                seq.append([c.get_state() for c in b.members])
+            merged, _ = b.launch_stats()
+            assert merged == (0 if mode == "unmerged" else steps)
+            seqs[mode] = seq
+    for mode in ("direct", "unmerged"):
+        for k in range(steps):
+            for i in range(count):
+                for a, b_ in zip(seqs["graphs"][k][i], seqs[mode][k][i]):
+                    assert np.array_equal(a, b_), f"{mode}: step {k}, member {i}"

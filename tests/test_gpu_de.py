@@ -65,7 +65,7 @@ def operator_pair(lri, n, m, q, periodic=False, variable=False, seed=0, sa=60.0,
     return dev, host
 
 
-def run_one(lri, name, dev, u, dt, N, tol=1e-3, rmax=64):
+def run_one(lri, name, dev, u, dt, N, tol=1e-3, rmax=64, resident=False):
     n, m, r = u.U.shape[0], u.V.shape[0], u.S.shape[0]
     rcap = min(2 * r, 64, n, m) if name == "ra" else r
     ctx = lri.native.Context(n, m, rcap)
@@ -74,6 +74,8 @@ def run_one(lri, name, dev, u, dt, N, tol=1e-3, rmax=64):
         q = dev.G.shape[1]
         ctx.set_operator_lyapunov(dev.A.d, dev.A.e, dev.A.periodic, dev.B.d, dev.B.e, dev.B.periodic,
                                   dev.G if q else None, dev.H if q else None)
+        if resident:
+            ctx.set_resident_eigenbasis(True)
         nat = lri.native
         if name == "primal":
             ctx.step_de_ksl(nat.PRIMAL, dt, N)
@@ -146,15 +148,19 @@ def test_scalar_operator_closed_form(lri, name):
 
 
 @pytest.mark.parametrize("name", ALGS)
-@pytest.mark.parametrize("periodic,q,variable", [(False, 1, False), (True, 2, True), (False, 0, True)])
-def test_one_step_vs_oracle(lri, name, periodic, q, variable):
+@pytest.mark.parametrize("periodic,q,variable,resident", [(False, 1, False, False), (True, 2, True, False),
+                                                          (False, 0, True, False), (False, 1, False, True),
+                                                          (True, 2, False, True)])
+def test_one_step_vs_oracle(lri, name, periodic, q, variable, resident):
+    """resident: the state kept in the operators' analytic eigenbasis (lri_set_resident_eigenbasis)."""
     n, m, r = 50, 37, 4
-    say(f"case: one DE step {name}, periodic={periodic} q={q} variable={variable}, n={n} m={m} r={r}")
+    say(f"case: one DE step {name}, periodic={periodic} q={q} variable={variable} resident={resident}, "
+        f"n={n} m={m} r={r}")
     dev, host = operator_pair(lri, n, m, q, periodic, variable, seed=1)
     u = random_factors(np.random.default_rng(2), n, m, r)
     dt = 0.05
     N = lri.rk4_substeps(dev.spectral_bound(), dt / 2 if name == "strang" else dt)
-    got = run_one(lri, name, dev, u, dt, N)
+    got = run_one(lri, name, dev, u, dt, N, resident=resident)
     ref = oracle_one(name, host, u, dt, N)
     assert got[1].shape == ref[1].shape
     assert_close(got, ref)
@@ -172,6 +178,50 @@ def test_tall_sizes(lri, n, m):
     got = run_one(lri, "bug", dev, u, dt, N)
     ref = oracle_one("bug", host, u, dt, N)
     assert_close(got, ref)
+
+
+def test_resident_eigenbasis_solve(lri):
+    """Ten BUG and rank-adaptive steps of the heat equation with forcing, n = m = 300, with the state
+    resident in the eigenbasis against the same steps in the original basis (1e-12), the oracle
+    (1e-10), a trajectory saved while resident, and leaving the eigenbasis for a data-path call."""
+    n = m = 300
+    r, steps, dt = 8, 10, 2e-4
+    dev, host = operator_pair(lri, n, m, 1, False, False, seed=8, sa=n * n, sb=m * m)
+    u = random_factors(np.random.default_rng(9), n, m, r, decay=0.5)
+    N = lri.rk4_substeps(dev.spectral_bound(), dt)
+    say(f"case: resident eigenbasis, BUG and RA, n={n} r={r}, {steps} steps of {N} RK4 sub-steps")
+    q = dev.G.shape[1]
+    for name in ("bug", "ra"):
+        got = {}
+        for resident in (False, True):
+            with lri.native.Context(n, m, 32) as ctx:
+                ctx.set_operator_lyapunov(dev.A.d, dev.A.e, dev.A.periodic, dev.B.d, dev.B.e, dev.B.periodic,
+                                          dev.G, dev.H)
+                if resident:
+                    ctx.set_resident_eigenbasis(True)
+                ctx.set_state(u.U, u.S, u.V)
+                ctx.trajectory_reserve(steps + 1)
+                for k in range(steps):
+                    if name == "bug":
+                        ctx.step_de_bug(dt, N)
+                    else:
+                        ctx.step_de_bug_ra(dt, N, 1e-6, 32)
+                    ctx.trajectory_save(k)
+                got[resident] = (ctx.get_state(), ctx.trajectory_get(steps // 2))
+                if resident:  # a data-path call leaves the eigenbasis; the state is unchanged
+                    ctx.set_resident_eigenbasis(False)
+                    assert rel_err(*ctx.get_state(), *got[True][0]) <= 1e-13
+        e = rel_err(*got[True][0], *got[False][0])
+        et = rel_err(*got[True][1], *got[False][1])
+        say(f"  {name}: resident vs original basis {e:.2e} (trajectory slot {et:.2e})")
+        assert e <= 1e-12 and et <= 1e-12
+        U, S, V = u.U, u.S, u.V
+        for k in range(steps):
+            if name == "bug":
+                U, S, V = dlra_de.bug_de_step(host, U, S, V, k * dt, dt, N)
+            else:
+                U, S, V, _ = dlra_de.rank_adaptive_bug_de_step(host, U, S, V, k * dt, dt, N, 1e-6, 32)
+        assert_close(got[True][0], (U, S, V))
 
 
 @pytest.mark.parametrize("name", ["bug", "primal"])

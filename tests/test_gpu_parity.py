@@ -148,24 +148,77 @@ def test_zero_increment_steps(lri, r):
         assert rel_err(U, S, V, u.U, u.S, u.V) <= 1e-13, kind
 
 
+def rank_deficient_case(r, seed=11, n=120, m=100):
+    """A state whose S has two zero singular values (K = U S has rank r - 2)."""
+    rng = np.random.default_rng(seed)
+    u = random_factors(rng, n, m, r, decay=0.05)
+    sig = np.linalg.svd(u.S, compute_uv=False)
+    sig[2] = sig[r - 3] = 0.0
+    Us, _ = np.linalg.qr(rng.standard_normal((r, r)))
+    Vs, _ = np.linalg.qr(rng.standard_normal((r, r)))
+    return type(u)(u.U, Us @ np.diag(sig) @ Vs.T, u.V), rng
+
+
 @pytest.mark.parametrize("r", [6, 24, 40])
-def test_rank_deficient_state_reports_breakdown(lri, r):
-    """A state whose S has two zero singular values and a zero increment: K =
-    U*S has two zero columns, so the first Cholesky factor drops two pivots and
-    the KSL step reports EBREAKDOWN (r = 6 and 24 run the factorisations inside
-    the row kernels, r = 40 as separate epilogue launches)."""
-    say(f"case: rank-deficient state r={r}")
-    rng = np.random.default_rng(11)
+def test_rank_deficient_zero_increment_returns_y0(lri, r):
+    """A rank-deficient state and a zero increment: K = U S drops two Cholesky pivots.
+    The reference's Householder qr! (projector_splitting.jl:137,149,174,186) returns an
+    orthonormal basis and the step returns Y0; so must the engine, whose dropped columns
+    are completed (r = 6 and 24 run the factorisations inside the row kernels, r = 40 as
+    separate epilogue launches)."""
+    say(f"case: rank-deficient state, zero increment, r={r}")
+    u, rng = rank_deficient_case(r)
+    A0 = rng.standard_normal((120, 100))
+    for kind in ("primal", "dual", "strang"):
+        snaps = [A0, A0.copy(), A0.copy()] if kind == "strang" else [A0, A0.copy()]
+        U, S, V = run_native_step(lri, kind, u, snaps)
+        e = rel_err(U, S, V, u.U, u.S, u.V)
+        say(f"  {kind}: rel err vs Y0 {e:.2e}, orthogonality {orth_defect(U):.1e} {orth_defect(V):.1e}")
+        assert e <= 1e-13, kind
+        assert orth_defect(U) <= 1e-13 and orth_defect(V) <= 1e-13, kind
+
+
+@pytest.mark.parametrize("r", [6, 24, 40])
+def test_rank_deficient_step_identities(lri, r):
+    """A rank-deficient state and an increment dA = U0 C V0' + 1e-2 (I - U0 U0') D (I - V0 V0')
+    whose C keeps K = U0 S0 + dA V0 (and L = V0 S0' + dA' U0) of rank r - 2, while its
+    other part reaches the completed directions.  The completion is a free choice
+    (Householder makes another), but every orthonormal completion gives a primal step with
+    U1 orthonormal, range(K) inside span(U1) and Y1 = K V0' + U1 U1' dA (I - V0 V0')
+    (projector_splitting.jl:129-152: S~ = R - U1' dA V0, L = V0 S~' + dA' U1, Y1 = U1 L');
+    the dual step (projector_splitting.jl:166-189) the mirror identity
+    Y1 = U0 L' + (I - U0 U0') dA V1 V1'."""
+    say(f"case: rank-deficient state, rank-deficient increment, r={r}")
+    rng = np.random.default_rng(12)
     u = random_factors(rng, 120, 100, r, decay=0.05)
     sig = np.linalg.svd(u.S, compute_uv=False)
     sig[2] = sig[r - 3] = 0.0
     Us, _ = np.linalg.qr(rng.standard_normal((r, r)))
     Vs, _ = np.linalg.qr(rng.standard_normal((r, r)))
-    u = type(u)(u.U, Us @ np.diag(sig) @ Vs.T, u.V)
+    U0, S0, V0 = u.U, Us @ np.diag(sig) @ Vs.T, u.V
+    c = 1e-2 * rng.uniform(0.5, 1.0, r)
+    c[2] = c[r - 3] = 0.0
+    PU, PV = np.eye(120) - U0 @ U0.T, np.eye(100) - V0 @ V0.T
+    dA = U0 @ (Us @ np.diag(c) @ Vs.T) @ V0.T + 1e-2 * PU @ rng.standard_normal((120, 100)) @ PV
     A0 = rng.standard_normal((120, 100))
-    for kind in ("primal", "dual"):
-        with pytest.raises(lri.native.BreakdownError):
-            run_native_step(lri, kind, u, [A0, A0.copy()])
+    A1 = A0 + dA
+    dA = A1 - A0  # the increment the engine sees
+    u = type(u)(U0, S0, V0)
+    K = U0 @ S0 + dA @ V0
+    assert np.linalg.matrix_rank(K) == r - 2
+    U, S, V = run_native_step(lri, "primal", u, [A0, A1])
+    Y1 = K @ V0.T + U @ (U.T @ dA) @ PV
+    e = np.linalg.norm(U @ S @ V.T - Y1) / np.linalg.norm(Y1)
+    say(f"  primal: identity residual {e:.2e}, orthogonality {orth_defect(U):.1e} {orth_defect(V):.1e}")
+    assert e <= 1e-12 and orth_defect(U) <= 1e-13 and orth_defect(V) <= 1e-13
+    assert np.linalg.norm(K - U @ (U.T @ K)) <= 1e-12 * np.linalg.norm(K)
+    U, S, V = run_native_step(lri, "dual", u, [A0, A1])
+    L = V0 @ S0.T + dA.T @ U0
+    Y1 = U0 @ L.T + PU @ dA @ V @ V.T
+    e = np.linalg.norm(U @ S @ V.T - Y1) / np.linalg.norm(Y1)
+    say(f"  dual: identity residual {e:.2e}, orthogonality {orth_defect(U):.1e} {orth_defect(V):.1e}")
+    assert e <= 1e-12 and orth_defect(U) <= 1e-13 and orth_defect(V) <= 1e-13
+    assert np.linalg.norm(L - V @ (V.T @ L)) <= 1e-12 * np.linalg.norm(L)
 
 
 # ------------------------------------------------------------------ FP64 parity, one step
@@ -279,8 +332,8 @@ def test_rank_adaptive_capacity_overflow(lri):
         with pytest.raises(lri.native.NotSupportedError):
             ctx.step_bug_ra(0, 1, 1e-14, 64)
         U, S, V = ctx.get_state()
-        assert S.shape[0] == r and np.array_equal(U, u.U) and np.array_equal(V, u.V)
-        assert rel_err(U, S, V, u.U, u.S, u.V) == 0.0
+        unchanged = S.shape[0] == r and np.array_equal(U, u.U) and np.array_equal(S, u.S) and np.array_equal(V, u.V)
+        assert unchanged
         assert ctx.step_bug_ra(0, 1, 1e-14, rcap) == rcap
         U, S, V = ctx.get_state()
     Uo, So, Vo, r1 = dlra.rank_adaptive_bug_step(u.U, u.S, u.V, A1 - A0, 1e-14, rcap)
@@ -374,6 +427,31 @@ def test_array_data_late_start(lri):
         assert len(got.Y) == len(ref.Y) == 7 and list(got.t) == list(ref.t)
         a, b = got.Y[-1], ref.Y[-1]
         assert rel_err(a.U, a.S, a.V, b.U, b.S, b.V) <= FP64_TOL, name
+
+
+def test_device_trajectory_roundtrip(lri):
+    """lri_trajectory_save / _get: each slot returns the state it was saved from, also
+    across rank changes of the rank-adaptive step, and an unsaved slot is refused."""
+    u, A0, A1 = one_step_case(16, 150, 120, 6, inc=1e-1)
+    A2 = A1 + 1e-1 * np.random.default_rng(17).standard_normal(A1.shape)
+    with lri.native.Context(150, 120, 24) as ctx:
+        ctx.set_state(u.U, u.S, u.V)
+        ctx.reserve(3, lri.native.F64)
+        for k, a in enumerate((A0, A1, A2)):
+            ctx.upload(k, lri.api._encode(a, "f64"))
+        ctx.trajectory_reserve(4)
+        ctx.trajectory_save(0)
+        want = [ctx.get_state()]
+        for k in range(2):
+            ctx.step_bug_ra(k, k + 1, 1e-12, 24)
+            ctx.trajectory_save(k + 1)
+            want.append(ctx.get_state())
+        assert [w[1].shape[0] for w in want] == [6, 12, 24]
+        for k, w in enumerate(want):
+            got = ctx.trajectory_get(k)
+            assert all(np.array_equal(a, b) for a, b in zip(got, w)), k
+        with pytest.raises(ValueError):
+            ctx.trajectory_get(3)
 
 
 # ------------------------------------------------------------------ mixed precision

@@ -217,3 +217,78 @@ def test_api_burgers_samples_match_the_oracle(lri):
     """The package's Burgers initial samples (used by bench.py) equal the oracle's
     restatement of data_agnostic_approximation.jl:35-44."""
     assert np.allclose(lri.burgers_samples(60, 5), dlra_de.burgers_initial(60, 5), rtol=0, atol=1e-15)
+
+
+# ------------------------------------------------------------------ the Greedy step on a hybrid problem
+
+
+def test_hybrid_step_zero_dynamics_is_the_polar_factor():
+    """With f = 0 the Z integrator leaves Z unchanged and U1 is the orthogonal polar
+    factor of X Z (greedy_integrator.jl:79-81), checked against scipy's polar."""
+    rng = np.random.default_rng(31)
+    n, m, r = 60, 45, 5
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    Z = rng.standard_normal((m, r))
+    X = rng.standard_normal((n, m))
+    U1, Z1 = dlra_de.greedy_hybrid_step(lambda Y, t: np.zeros_like(Y), U, Z, X, 0.0, 0.1, 3)
+    assert np.array_equal(Z1, Z)
+    W, P = sla.polar(X @ Z)
+    assert np.linalg.norm(U1 - W) <= 1e-12
+    H = U1.T @ (X @ Z)  # X Z = U1 H with H symmetric positive definite
+    assert np.linalg.norm(H - H.T) <= 1e-12 * np.linalg.norm(H) and np.linalg.eigvalsh(0.5 * (H + H.T)).min() > 0
+
+
+def test_hybrid_step_linear_dynamics_against_the_exponential():
+    """F(Y) = A Y + Y B' + g h': the Z ODE f(Z, U, t) = F(U Z')' U is linear,
+    Z' = B Z + Z (U'AU)' + h (g'U) (U'U = I), so Z1 = exp of the vectorised
+    system; the RK4 result converges to it (and the factored right-hand side of
+    the low-rank evaluation gives the same Z1)."""
+    rng = np.random.default_rng(32)
+    n, m, r = 30, 24, 4
+    A = -np.diag(rng.uniform(1, 3, n)) + 0.1 * rng.standard_normal((n, n))
+    B = -np.diag(rng.uniform(1, 3, m)) + 0.1 * rng.standard_normal((m, m))
+    g, h = rng.standard_normal((n, 1)), rng.standard_normal((m, 1))
+    F = dlra_de.LyapunovOperator(A, B, g, h)
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    Z = rng.standard_normal((m, r))
+    X = rng.standard_normal((n, m))
+    dt = 0.3
+    _, Z1 = dlra_de.greedy_hybrid_step(F, U, Z, X, 0.0, dt, 200)
+    At = U.T @ A @ U
+    c = h @ (g.T @ U)
+    # vec(B Z + Z At') = (I (x) B + At (x) I) vec(Z)  (column-major vec)
+    Lop = np.kron(np.eye(r), B) + np.kron(At, np.eye(m))
+    E = sla.expm(dt * Lop)
+    z0, cv = Z.flatten(order="F"), c.flatten(order="F")
+    z1 = E @ z0 + np.linalg.solve(Lop, (E - np.eye(m * r)) @ cv)
+    assert np.linalg.norm(Z1.flatten(order="F") - z1) <= 1e-10 * np.linalg.norm(z1)
+    lowrank = dlra_de.LowRankLyapunov(A, B, g, h)
+    Z1f = dlra_de.rk4(lowrank.L_rhs(U), Z, 0.0, dt, 200)
+    assert np.linalg.norm(Z1f - Z1) <= 1e-12 * np.linalg.norm(Z1)
+
+
+def test_hybrid_step_scalar_operator_closed_form():
+    """F(Y) = c Y: Z' = c Z, so N RK4 steps give R(c h)^N Z with R the RK4 stability
+    polynomial; a dropped or transposed factor of the right-hand side fails this."""
+    rng = np.random.default_rng(33)
+    n, m, r, c, dt, N = 20, 16, 3, -2.0, 0.4, 5
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    Z = rng.standard_normal((m, r))
+    _, Z1 = dlra_de.greedy_hybrid_step(lambda Y, t: c * Y, U, Z, rng.standard_normal((n, m)), 0.0, dt, N)
+    z = c * dt / N
+    R = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    assert np.linalg.norm(Z1 - R ** N * Z) <= 1e-14 * np.linalg.norm(Z)
+
+
+def test_solve_hybrid_driver():
+    """Step count and accumulated t of solve (primitives.jl:68-80) for the hybrid problem."""
+    rng = np.random.default_rng(34)
+    n, m, r = 12, 10, 2
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    Z = rng.standard_normal((m, r))
+    Xs = rng.standard_normal((n, m))
+    prob = dlra_de.MatrixHybridProblem(lambda t: Xs * (1 + t), lambda Y, t: -Y, U, Z, (0.0, 1.0))
+    Ys, ts = dlra_de.solve_hybrid(prob, 0.1, 2)
+    assert len(Ys) == 11 and abs(ts[-1] - 1.0) < 1e-12
+    U1, Z1 = Ys[-1]
+    assert np.linalg.norm(U1.T @ U1 - np.eye(r)) <= 1e-12
