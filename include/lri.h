@@ -164,6 +164,22 @@ int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* U
 int lri_step_ksl_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
                          int64_t ldsb, double* Vb, int64_t ldvb);
 
+/* The same for the dual KSL data step (projector_splitting.jl:166-189).  The
+ * dual step is the primal step of the transposed problem, dual(U, S, V; dA) =
+ * (V1', S1'', U1') for primal(V, S', U; dA') = (U1', S1', V1'): the reverse step
+ * is the primal one run on the context's transposed view (the same buffers,
+ * the passes over dA exchanged).  Four passes over the increment. */
+int lri_step_ksl_dual_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
+                              int64_t ldsb, double* Vb, int64_t ldvb);
+
+/* The same for the Strang KSL step (projector_splitting.jl:205-211) over
+ * A[k_prev] -> A[k_mid] (primal half) -> A[k_next] (dual half), as
+ * lri_step_ksl_strang: the primal half is recomputed from a device copy of the
+ * state, the dual half reversed at its result, then the primal half reversed.
+ * Ten passes over the two increments.  The state is left unchanged. */
+int lri_step_ksl_strang_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_mid, int64_t k_next, double* Ub,
+                                int64_t ldub, double* Sb, int64_t ldsb, double* Vb, int64_t ldvb);
+
 /* Reverse of the rank-adaptive BUG data step (rank_adaptive_unconventional.jl:182-233)
  * with the new rank r_new held fixed (the rank the forward step returned; the
  * recomputed step must reproduce it, else LRI_EINVAL): on entry the first

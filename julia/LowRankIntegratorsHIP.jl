@@ -552,7 +552,8 @@ end
 
 `LowRankIntegrators.solve(prob, alg, dt).Y[end]`, differentiable with respect
 to `prob.u0` through the `rrule` below: the engine's hand-written reverse
-steps (`lri_step_bug_adjoint`, `lri_step_ksl_adjoint`, `lri_step_bug_ra_adjoint`)
+steps (`lri_step_bug_adjoint`, `lri_step_ksl_adjoint`, `lri_step_ksl_dual_adjoint`,
+`lri_step_bug_ra_adjoint`)
 with checkpointed recomputation.  The reference has no adjoint (Julia AD would
 differentiate `unconventional_step!` and its counterparts).
 """
@@ -563,13 +564,16 @@ function forward_step!(ctx::HIPContext, inner, kp, kn)
         check(ctx.ptr, ccall((:lri_step_bug, libhip), Cint, (Ptr{Cvoid}, Int64, Int64), ctx.ptr, kp, kn))
     elseif inner isa ProjectorSplitting{PrimalLieTrotter}
         check(ctx.ptr, ccall((:lri_step_ksl, libhip), Cint, (Ptr{Cvoid}, Cint, Int64, Int64), ctx.ptr, LRI_PRIMAL, kp, kn))
+    elseif inner isa ProjectorSplitting{DualLieTrotter}
+        check(ctx.ptr, ccall((:lri_step_ksl, libhip), Cint, (Ptr{Cvoid}, Cint, Int64, Int64), ctx.ptr, LRI_DUAL, kp, kn))
     elseif inner isa RankAdaptiveUnconventionalAlgorithm
         r = Ref{Int64}(0)
         check(ctx.ptr, ccall((:lri_step_bug_ra, libhip), Cint, (Ptr{Cvoid}, Int64, Int64, Cdouble, Int64, Ref{Int64}),
                              ctx.ptr, kp, kn, inner.alg_params.tol, inner.alg_params.r_max, r))
     else
         throw(ArgumentError("the reverse step is built for UnconventionalAlgorithm, " *
-                            "ProjectorSplitting(PrimalLieTrotter()) and RankAdaptiveUnconventionalAlgorithm"))
+                            "ProjectorSplitting(PrimalLieTrotter()), ProjectorSplitting(DualLieTrotter()) " *
+                            "and RankAdaptiveUnconventionalAlgorithm"))
     end
 end
 
@@ -589,9 +593,12 @@ function reverse_step!(ctx::HIPContext, inner, kp, kn, Ub, Sb, Vb, r_new)
         return Ubw[:, 1:r], Sbw[1:r, 1:r], Vbw[:, 1:r]
     end
     Ub, Sb, Vb = Matrix{Float64}(Ub), Matrix{Float64}(Sb), Matrix{Float64}(Vb)
-    sym = inner isa UnconventionalAlgorithm ? :lri_step_bug_adjoint : :lri_step_ksl_adjoint
-    if sym === :lri_step_bug_adjoint
+    if inner isa UnconventionalAlgorithm
         GC.@preserve Ub Sb Vb check(ctx.ptr, ccall((:lri_step_bug_adjoint, libhip), Cint,
+            (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
+            ctx.ptr, kp, kn, Ub, size(Ub, 1), Sb, size(Sb, 1), Vb, size(Vb, 1)))
+    elseif inner isa ProjectorSplitting{DualLieTrotter}
+        GC.@preserve Ub Sb Vb check(ctx.ptr, ccall((:lri_step_ksl_dual_adjoint, libhip), Cint,
             (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64),
             ctx.ptr, kp, kn, Ub, size(Ub, 1), Sb, size(Sb, 1), Vb, size(Vb, 1)))
     else

@@ -892,8 +892,9 @@ def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, preci
 
     - array-form data (``MatrixDataProblem``), ``alg`` ``UnconventionalAlgorithm()``,
       ``RankAdaptiveUnconventionalAlgorithm(tol; rmax)`` (the ranks of the forward solve held
-      fixed) or ``ProjectorSplitting(PrimalLieTrotter())``: ``lri_step_bug_adjoint`` /
-      ``lri_step_bug_ra_adjoint`` / ``lri_step_ksl_adjoint``; returns ``(value, (Ub0, Sb0, Vb0))``;
+      fixed), ``ProjectorSplitting(PrimalLieTrotter())`` or ``ProjectorSplitting(DualLieTrotter())``:
+      ``lri_step_bug_adjoint`` / ``lri_step_bug_ra_adjoint`` / ``lri_step_ksl_adjoint`` /
+      ``lri_step_ksl_dual_adjoint``; returns ``(value, (Ub0, Sb0, Vb0))``;
     - a matrix ODE (``MatrixDEProblem`` with a ``QuadraticOperator``; D = 0 for a linear
       one) and ``UnconventionalAlgorithm()`` or ``ProjectorSplitting(PrimalLieTrotter())``:
       ``lri_step_de_bug_adjoint`` / ``lri_step_de_ksl_adjoint``, with ``dt`` and
@@ -909,13 +910,20 @@ def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, preci
     if isinstance(prob, MatrixDEProblem):
         return _solve_gradient_de(prob, alg, loss_grad, dt, checkpoint, substeps, device)
     ksl = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, PrimalLieTrotter)
+    dual = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, DualLieTrotter)
     ra = isinstance(alg, RankAdaptiveUnconventionalAlgorithm)
-    if not (ksl or ra or isinstance(alg, UnconventionalAlgorithm)):
-        raise TypeError("the adjoint is built for the unconventional (BUG), rank-adaptive BUG and primal KSL "
-                        "integrators")
+    if isinstance(alg, ProjectorSplitting) and isinstance(alg.order, Strang):
+        raise TypeError("array-form data has no half-step snapshot: the Strang reverse step is "
+                        "native.Context.step_ksl_strang_adjoint on function-form snapshots")
+    if not (ksl or dual or ra or isinstance(alg, UnconventionalAlgorithm)):
+        raise TypeError("the adjoint is built for the unconventional (BUG), rank-adaptive BUG and primal "
+                        "and dual KSL integrators")
     if ksl:
         fwd = lambda c, a, b: c.step_ksl(native.PRIMAL, a, b)  # noqa: E731
         rev = lambda c, a, b, *x: c.step_ksl_adjoint(a, b, *x)  # noqa: E731
+    elif dual:
+        fwd = lambda c, a, b: c.step_ksl(native.DUAL, a, b)  # noqa: E731
+        rev = lambda c, a, b, *x: c.step_ksl_dual_adjoint(a, b, *x)  # noqa: E731
     elif ra:
         fwd = lambda c, a, b: c.step_bug_ra(a, b, alg.tol, alg.rmax)  # noqa: E731
         rev = lambda c, a, b, *x: c.step_bug_ra_adjoint(a, b, alg.tol, alg.rmax, *x)  # noqa: E731

@@ -42,6 +42,18 @@ struct Failure {
     if (!(cond)) throw Failure{LRI_EINVAL, std::string(msg)}; \
   } while (0)
 
+// S(0:r, 0:r) = S(0:r, 0:r)' in place (ld lds); one workgroup, each pair swapped by one thread
+__global__ void k_transpose_sq(double* S, int lds, int r) {
+  for (int e = threadIdx.x; e < r * r; e += blockDim.x) {
+    const int i = e % r, j = e / r;
+    if (i < j) {
+      const double a = S[i + j * lds], b = S[j + i * lds];
+      S[i + j * lds] = b;
+      S[j + i * lds] = a;
+    }
+  }
+}
+
 // A = I (k x k, ld k)
 __global__ void k_eye(double* A, int k) {
   for (int e = threadIdx.x; e < k * k; e += blockDim.x) A[e] = (e % k == e / k) ? 1.0 : 0.0;
@@ -194,6 +206,34 @@ struct lri_ctx {
   int op_aper = 0, op_bper = 0, op_q = 0;
   DeSmall* de = nullptr;
   bool pack_enabled = true;
+  // Transposed view (swap_sides): the context seen as the transposed problem,
+  // n <-> m, U <-> V and every side's buffers exchanged, the data A' in place of
+  // A: a pass over dA in the view is the other direction's pass over the real
+  // snapshots.  The dual KSL step is the primal step of the transposed problem,
+  // so its adjoint runs the primal adjoint in this view.
+  bool tview = false;
+  void swap_sides() {
+    std::swap(n, m);
+    std::swap(U, V);
+    std::swap(U2, V2);
+    std::swap(K, L);
+    std::swap(K2, L2);
+    std::swap(P, Q);
+    std::swap(Uhat, Vhat);
+    std::swap(pkN_f64, pkT_f64);
+    std::swap(pkN_f32, pkT_f32);
+    std::swap(pkN_hi, pkT_hi);
+    std::swap(pkN_lo, pkT_lo);
+    std::swap(ldm_pk, ldn_pk);
+    std::swap(pkN_mode, pkT_mode);
+    std::swap(pkN_rb, pkT_rb);
+    std::swap(partN, partT);
+    tview = !tview;
+  }
+  void transpose_S() {
+    hipLaunchKernelGGL(k_transpose_sq, dim3(1), dim3(256), 0, stream, S, (int)rcap, (int)r);
+    HIPCHECK(hipGetLastError());
+  }
   // constant-coefficient operators: analytic eigenbasis Q (rows x rows), its
   // eigenvalues, and Q' G / Q' H, for the closed-form RK4 sub-steps (one per side;
   // side 1 may alias side 0)
@@ -335,11 +375,11 @@ struct lri_ctx {
       A.pk_f64 = forN ? pkN_f64 : pkT_f64;
     } else if (mode == PASS_F32) {
       A.pk_f32 = forN ? pkN_f32 : pkT_f32;
-      A.pk_layout = forN ? 0 : 1;
+      A.pk_layout = (forN != tview) ? 0 : 1;  // the layout of the real pass that reads it
     } else {
       A.pk_hi = forN ? pkN_hi : pkT_hi;
       A.pk_lo = forN ? pkN_lo : pkT_lo;
-      A.pk_layout = forN ? 0 : 1;
+      A.pk_layout = (forN != tview) ? 0 : 1;
     }
     if (forN) {
       pkN_mode = mode;
@@ -382,8 +422,8 @@ struct lri_ctx {
     a.aprev = kp >= 0 ? snaps[kp] : nullptr;
     a.dtype = sdtype;
     a.lda = sld;
-    a.n = n;
-    a.m = m;
+    a.n = tview ? m : n;  // the snapshots' own dimensions
+    a.m = tview ? n : m;
     a.rb = rb_for(mode, cols);
     if (!transposed) {
       REQUIRE(pkN_mode == mode && pkN_rb == a.rb, "internal: stale packed V panel");
@@ -408,22 +448,20 @@ struct lri_ctx {
     return a;
   }
   // The product passes with the currently packed B; they return the split count.
-  int passN(int64_t kp, int64_t kn, int64_t cols, void* part) {
+  // In the transposed view pass N over dA' is pass T over the real snapshots.
+  int pass_dir(bool transposed, int64_t kp, int64_t kn, int64_t cols, void* part) {
     const int mode = pass_mode();
-    PassArgs a = pass_args(false, kp, kn, cols, part);
-    int64_t need = plan(mode, false, n, m, a.rb, &a.splits, &a.chunk);
+    PassArgs a = pass_args(transposed, kp, kn, cols, part);
+    const bool real_t = transposed != tview;
+    int64_t need = real_t ? plan(mode, true, a.m, a.n, a.rb, &a.splits, &a.chunk)
+                          : plan(mode, false, a.n, a.m, a.rb, &a.splits, &a.chunk);
     REQUIRE((size_t)need * (mode == PASS_F64 ? 8 : 4) <= part_bytes, "internal: partial buffer too small");
-    timed(LRI_KERNEL_PASS_N, [&] { return launch_passN(mode, a, stream); });
+    timed(transposed ? LRI_KERNEL_PASS_T : LRI_KERNEL_PASS_N,
+          [&] { return real_t ? launch_passT(mode, a, stream) : launch_passN(mode, a, stream); });
     return a.splits;
   }
-  int passT(int64_t kp, int64_t kn, int64_t cols, void* part) {
-    const int mode = pass_mode();
-    PassArgs a = pass_args(true, kp, kn, cols, part);
-    int64_t need = plan(mode, true, m, n, a.rb, &a.splits, &a.chunk);
-    REQUIRE((size_t)need * (mode == PASS_F64 ? 8 : 4) <= part_bytes, "internal: partial buffer too small");
-    timed(LRI_KERNEL_PASS_T, [&] { return launch_passT(mode, a, stream); });
-    return a.splits;
-  }
+  int passN(int64_t kp, int64_t kn, int64_t cols, void* part) { return pass_dir(false, kp, kn, cols, part); }
+  int passT(int64_t kp, int64_t kn, int64_t cols, void* part) { return pass_dir(true, kp, kn, cols, part); }
 
   // The pass autotuner: for pass N and pass T over dA = A[kn] - A[kp] at the
   // current rank and precision, time every candidate reduction chunk
@@ -639,11 +677,11 @@ struct lri_ctx {
         C.pk_f64 = pk_forN ? pkN_f64 : pkT_f64;
       } else if (mode == PASS_F32) {
         C.pk_f32 = pk_forN ? pkN_f32 : pkT_f32;
-        C.pk_layout = pk_forN ? 0 : 1;
+        C.pk_layout = (pk_forN != tview) ? 0 : 1;
       } else {
         C.pk_hi = pk_forN ? pkN_hi : pkT_hi;
         C.pk_lo = pk_forN ? pkN_lo : pkT_lo;
-        C.pk_layout = pk_forN ? 0 : 1;
+        C.pk_layout = (pk_forN != tview) ? 0 : 1;
       }
     }
     C.M = M;
@@ -1435,6 +1473,78 @@ struct lri_ctx {
       rows(A);
     }
     timed(LRI_KERNEL_SMALL, [&] { return launch_ksl_adj_small(adj, rr, 2, stream); });  // Sb0 = CK'
+    pkN_mode = pkT_mode = -1;
+  }
+
+  // The BUG or primal KSL reverse step with host cotangents: those of (U1, S1,
+  // V1) in, those of (U0, S0, V0) out (the state is unchanged).
+  void data_adjoint(bool ksl, int64_t kp, int64_t kn, double* Ub, int64_t ldub, double* Sb, int64_t ldsb, double* Vb,
+                    int64_t ldvb) {
+    const size_t d = sizeof(double);
+    double* ubuf = ksl ? P + n * r : P;
+    HIPCHECK(hipMemcpy2DAsync(ubuf, d * n, Ub, d * ldub, d * n, r, hipMemcpyDefault, stream));
+    HIPCHECK(hipMemcpy2DAsync(Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyDefault, stream));
+    HIPCHECK(hipMemcpy2DAsync(adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyDefault, stream));
+    if (ksl)
+      ksl_adjoint(kp, kn);
+    else
+      bug_adjoint(kp, kn);
+    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ubuf, d * n, d * n, r, hipMemcpyDefault, stream));
+    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, Q, d * m, d * m, r, hipMemcpyDefault, stream));
+    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, adj->Sb0, d * r, d * r, r, hipMemcpyDefault, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+
+  // dual_LT_step! (projector_splitting.jl:166-189) is primal_LT_step! of the
+  // transposed problem: dual(U, S, V; dA) = (V1', S1'', U1') for
+  // primal(V, S', U; dA') = (U1', S1', V1').  Its reverse step is the primal one
+  // run in the transposed view, with the U and V cotangents exchanged and the S
+  // cotangents transposed on the way in and out.
+  void ksl_dual_adjoint(int64_t kp, int64_t kn, double* Ub, int64_t ldub, double* Sb, int64_t ldsb, double* Vb,
+                        int64_t ldvb) {
+    // Sb may be host or device memory: it is transposed through a host copy
+    const int rr = (int)r;
+    const size_t d = sizeof(double);
+    std::vector<double> sb0((size_t)rr * rr), sbt((size_t)rr * rr);
+    HIPCHECK(hipMemcpy2D(sb0.data(), d * rr, Sb, d * ldsb, d * rr, rr, hipMemcpyDefault));
+    for (int j = 0; j < rr; ++j)
+      for (int i = 0; i < rr; ++i) sbt[i + (size_t)j * rr] = sb0[j + (size_t)i * rr];
+    transpose_S();
+    swap_sides();
+    try {
+      data_adjoint(true, kp, kn, Vb, ldvb, sbt.data(), rr, Ub, ldub);
+    } catch (...) {
+      swap_sides();
+      try {
+        transpose_S();
+      } catch (...) {
+      }
+      throw;
+    }
+    swap_sides();
+    transpose_S();
+    for (int j = 0; j < rr; ++j)
+      for (int i = 0; i < rr; ++i) sb0[i + (size_t)j * rr] = sbt[j + (size_t)i * rr];
+    HIPCHECK(hipMemcpy2D(Sb, d * ldsb, sb0.data(), d * rr, d * rr, rr, hipMemcpyDefault));
+  }
+
+  // a device copy of the state (U, S, V, rank), for reverse steps that recompute a forward half
+  double* backup = nullptr;
+  int64_t backup_r = 0;
+  void save_state() {
+    if (!backup) HIPCHECK(hipMalloc((void**)&backup, sizeof(double) * (size_t)traj_stride()));
+    const size_t d = sizeof(double);
+    HIPCHECK(hipMemcpyAsync(backup, U, d * n * rcap, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(backup + n * rcap, S, d * rcap * rcap, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(backup + n * rcap + rcap * rcap, V, d * m * rcap, hipMemcpyDeviceToDevice, stream));
+    backup_r = r;
+  }
+  void restore_state() {
+    const size_t d = sizeof(double);
+    HIPCHECK(hipMemcpyAsync(U, backup, d * n * rcap, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(S, backup + n * rcap, d * rcap * rcap, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(V, backup + n * rcap + rcap * rcap, d * m * rcap, hipMemcpyDeviceToDevice, stream));
+    r = backup_r;
     pkN_mode = pkT_mode = -1;
   }
 
@@ -3285,6 +3395,7 @@ void free_ctx(lri_ctx* c) {
   if (c->qadj) (void)hipFree(c->qadj);
   if (c->qadj_rows) (void)hipFree(c->qadj_rows);
   if (c->traj) (void)hipFree(c->traj);
+  if (c->backup) (void)hipFree(c->backup);
   for (auto& g : c->ra_graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& t : c->pending) {
     (void)hipEventDestroy(t.a);
@@ -3912,30 +4023,22 @@ int lri_step_bug(lri_ctx* ctx, int64_t k_prev, int64_t k_next) {
 namespace {
 // the host side of the reverse steps: cotangents in, step, cotangents out; the
 // KSL step keeps dA V0 in P[:, :r], so its cotangent of U sits in P[:, r:2r]
+void data_adjoint_checks(lri_ctx* ctx, double* Ub, int64_t ldub, double* Sb, int64_t ldsb, double* Vb, int64_t ldvb) {
+  REQUIRE(Ub && Sb && Vb, "Ub, Sb and Vb are required");
+  REQUIRE(ldub >= ctx->n && ldvb >= ctx->m && ldsb >= ctx->r, "leading dimensions");
+  if (!ctx->adj) {
+    HIPCHECK(hipMalloc((void**)&ctx->adj, sizeof(BugAdjSmall)));
+    HIPCHECK(hipMemset(ctx->adj, 0, sizeof(BugAdjSmall)));
+  }
+}
+
 int adjoint_entry(lri_ctx* ctx, bool ksl, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
                   int64_t ldsb, double* Vb, int64_t ldvb) {
   return guarded(ctx, [&] {
     ctx->leave_resident();
     ctx->check_step(k_prev, k_next);
-    REQUIRE(Ub && Sb && Vb, "Ub, Sb and Vb are required");
-    const int64_t r = ctx->r, n = ctx->n, m = ctx->m;
-    REQUIRE(ldub >= n && ldvb >= m && ldsb >= r, "leading dimensions");
-    if (!ctx->adj) {
-      HIPCHECK(hipMalloc((void**)&ctx->adj, sizeof(BugAdjSmall)));
-      HIPCHECK(hipMemset(ctx->adj, 0, sizeof(BugAdjSmall)));
-    }
-    const size_t d = sizeof(double);
-    double* ubuf = ksl ? ctx->P + n * r : ctx->P;
-    HIPCHECK(hipMemcpy2DAsync(ubuf, d * n, Ub, d * ldub, d * n, r, hipMemcpyDefault, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(ctx->Q, d * m, Vb, d * ldvb, d * m, r, hipMemcpyDefault, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(ctx->adj->Sb1, d * r, Sb, d * ldsb, d * r, r, hipMemcpyDefault, ctx->stream));
-    if (ksl)
-      ctx->ksl_adjoint(k_prev, k_next);
-    else
-      ctx->bug_adjoint(k_prev, k_next);
-    HIPCHECK(hipMemcpy2DAsync(Ub, d * ldub, ubuf, d * n, d * n, r, hipMemcpyDefault, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(Vb, d * ldvb, ctx->Q, d * m, d * m, r, hipMemcpyDefault, ctx->stream));
-    HIPCHECK(hipMemcpy2DAsync(Sb, d * ldsb, ctx->adj->Sb0, d * r, d * r, r, hipMemcpyDefault, ctx->stream));
+    data_adjoint_checks(ctx, Ub, ldub, Sb, ldsb, Vb, ldvb);
+    ctx->data_adjoint(ksl, k_prev, k_next, Ub, ldub, Sb, ldsb, Vb, ldvb);
     return ctx->read_flags();
   });
 }
@@ -3949,6 +4052,47 @@ int lri_step_bug_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* U
 int lri_step_ksl_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
                          int64_t ldsb, double* Vb, int64_t ldvb) {
   return adjoint_entry(ctx, true, k_prev, k_next, Ub, ldub, Sb, ldsb, Vb, ldvb);
+}
+
+int lri_step_ksl_dual_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double* Ub, int64_t ldub, double* Sb,
+                              int64_t ldsb, double* Vb, int64_t ldvb) {
+  return guarded(ctx, [&] {
+    ctx->leave_resident();
+    ctx->check_step(k_prev, k_next);
+    data_adjoint_checks(ctx, Ub, ldub, Sb, ldsb, Vb, ldvb);
+    ctx->ksl_dual_adjoint(k_prev, k_next, Ub, ldub, Sb, ldsb, Vb, ldvb);
+    return ctx->read_flags();
+  });
+}
+
+int lri_step_ksl_strang_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_mid, int64_t k_next, double* Ub, int64_t ldub,
+                                double* Sb, int64_t ldsb, double* Vb, int64_t ldvb) {
+  return guarded(ctx, [&] {
+    ctx->leave_resident();
+    ctx->check_step(k_prev, k_mid);
+    ctx->check_step(k_mid, k_next);
+    data_adjoint_checks(ctx, Ub, ldub, Sb, ldsb, Vb, ldvb);
+    // the Strang step is primal(A[k_mid] - A[k_prev]) then dual(A[k_next] - A[k_mid])
+    // (projector_splitting.jl:205-211): the state after the first half is
+    // recomputed from a copy of the step's start, the dual half reversed there,
+    // then the primal half reversed at the start
+    ctx->save_state();
+    ctx->ksl_primal(k_prev, k_mid);
+    const int st = ctx->read_flags();
+    if (st != LRI_OK) {
+      ctx->restore_state();
+      return st;
+    }
+    try {
+      ctx->ksl_dual_adjoint(k_mid, k_next, Ub, ldub, Sb, ldsb, Vb, ldvb);
+    } catch (...) {
+      ctx->restore_state();
+      throw;
+    }
+    ctx->restore_state();
+    ctx->data_adjoint(true, k_prev, k_mid, Ub, ldub, Sb, ldsb, Vb, ldvb);
+    return ctx->read_flags();
+  });
 }
 
 int lri_step_bug_ra_adjoint(lri_ctx* ctx, int64_t k_prev, int64_t k_next, double tol, int64_t rmax, int64_t r_new,

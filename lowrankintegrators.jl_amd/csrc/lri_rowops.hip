@@ -680,8 +680,10 @@ __host__ __device__ inline RowsPlan rows_plan(int ro, int ri, int rw, bool hasX,
 
 // CK > 0: with the prologue of RowArgs::pre (Cholesky variant CK).  bid / nblk:
 // this workgroup's index among the nblk of the launch (of its member, batched).
+// The arguments are taken by value: bound to a reference into the kernel's
+// argument segment, the Cholesky variants spill ~470 bytes per lane to scratch.
 template <typename PT, int NRB, int CK = 0>
-__device__ __forceinline__ void rows_body(const RowArgs<PT>& A, int bid, int nblk) {
+__device__ __forceinline__ void rows_body(const RowArgs<PT> A, int bid, int nblk) {
   extern __shared__ double lds[];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int ro = A.r_out, ri = A.r_in;
@@ -1689,20 +1691,14 @@ __global__ __launch_bounds__(kCompleteThreads) void k_complete_columns(CompleteA
   __shared__ double s_h[kMaxAug + 8];
   __shared__ double s_red[32];
   __shared__ int s_dep[kMaxAug];
-  __shared__ int s_flag;
   __shared__ double s_min[4];
   __shared__ long long s_arg[4];
   const int t = threadIdx.x;
   const int k = A.k;
   const int64_t rows = A.rows;
   double* __restrict__ Q = A.Q;
-  if (t == 0) {
-    int nd = 0;
-    for (int p = 0; p < k; ++p) nd += (A.Rinv1[p + p * k] == 0.0 || A.Rinv2[p + p * k] == 0.0) ? 1 : 0;
-    s_flag = nd;
-  }
-  __syncthreads();
-  if (s_flag == 0) return;
+  // one diagonal entry per thread: the common case (nothing dropped) costs one load
+  if (!__syncthreads_or(t < k && (A.Rinv1[t + t * k] == 0.0 || A.Rinv2[t + t * k] == 0.0))) return;
   for (int e = t; e < k * k; e += kCompleteThreads) s_R[e] = 0.0;
   // the largest column norm of X, for the rounding-level threshold
   double cmax = 0.0;

@@ -51,6 +51,8 @@ SIGNATURES = {
     "lri_step_bug_ra": (_int, [_vp, _i64, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
     "lri_step_bug_adjoint": (_int, [_vp, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
     "lri_step_ksl_adjoint": (_int, [_vp, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
+    "lri_step_ksl_dual_adjoint": (_int, [_vp, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
+    "lri_step_ksl_strang_adjoint": (_int, [_vp, _i64, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]),
     "lri_step_bug_ra_adjoint": (_int, [_vp, _i64, _i64, ctypes.c_double, _i64, _i64, _dp, _i64, _dp, _i64, _dp,
                                        _i64]),
     "lri_step_greedy": (_int, [_vp, _i64]),
@@ -248,12 +250,23 @@ class Context:
         """Cotangents of the primal KSL step's inputs from those of its result (host arrays)."""
         return self._adjoint(lib().lri_step_ksl_adjoint, kp, kn, Ub, Sb, Vb)
 
+    def step_ksl_dual_adjoint(self, kp: int, kn: int, Ub, Sb, Vb):
+        """Cotangents of the dual KSL step's inputs from those of its result (host arrays)."""
+        return self._adjoint(lib().lri_step_ksl_dual_adjoint, kp, kn, Ub, Sb, Vb)
+
+    def step_ksl_strang_adjoint(self, kp: int, km: int, kn: int, Ub, Sb, Vb):
+        """Cotangents of the Strang KSL step's inputs (A[kp] -> A[km] -> A[kn]) from
+        those of its result (host arrays)."""
+        fn = lib().lri_step_ksl_strang_adjoint
+        return self._adjoint(lambda h, a, b, *x: fn(h, a, int(km), b, *x), kp, kn, Ub, Sb, Vb)
+
     def step_adjoint_ptr(self, kind: str, kp: int, kn: int, ub: int, ldub: int, sb: int, ldsb: int, vb: int,
                          ldvb: int):
-        """The BUG (kind "bug") or primal KSL ("ksl") reverse step on cotangents at raw
+        """The BUG (kind "bug"), primal KSL ("ksl") or dual KSL ("ksl_dual") reverse step on cotangents at raw
         addresses -- device memory (e.g. torch tensors' data_ptr()) or host -- column-major
         with the given leading dimensions, overwritten in place."""
-        fn = {"bug": lib().lri_step_bug_adjoint, "ksl": lib().lri_step_ksl_adjoint}[kind]
+        fn = {"bug": lib().lri_step_bug_adjoint, "ksl": lib().lri_step_ksl_adjoint,
+              "ksl_dual": lib().lri_step_ksl_dual_adjoint}[kind]
         P = ctypes.POINTER(ctypes.c_double)
         check(fn(self.h, int(kp), int(kn), ctypes.cast(ub, P), int(ldub), ctypes.cast(sb, P), int(ldsb),
                  ctypes.cast(vb, P), int(ldvb)), self.h)

@@ -70,6 +70,42 @@ def ksl_primal_step_vjp(U, S, V, dA, Ub1, Sb1, Vb1):
     return Ub0, Sb0, Vb0, dAb
 
 
+def ksl_dual_step_vjp(U, S, V, dA, Ub1, Sb1, Vb1):
+    """Cotangents (Ub0, Sb0, Vb0, dAb) of the dual KSL data step
+    (projector_splitting.jl:166-189):
+        Qd = dA' U, L = V S' + Qd -> (V1, R);  St = R' - Qd' V1;
+        K = U St + dA V1 -> (U1, R2);          S1 = R2."""
+    Qd = dA.T @ U
+    V1, R = _qr(V @ S.T + Qd)
+    St = R.T - Qd.T @ V1
+    U1, R2 = _qr(U @ St + dA @ V1)
+    Kb = qr_vjp_full(U1, R2, Ub1, Sb1)
+    Ub0 = Kb @ St.T
+    Stb = U.T @ Kb
+    dAb = Kb @ V1.T
+    V1b = Vb1 + dA.T @ Kb - Qd @ Stb
+    Qdb = -V1 @ Stb.T
+    Lb = qr_vjp_full(V1, R, V1b, Stb.T)
+    Vb0 = Lb @ S
+    Sb0 = Lb.T @ V
+    Qdb = Qdb + Lb
+    dAb = dAb + U @ Qdb.T
+    Ub0 = Ub0 + dA @ Qdb
+    return Ub0, Sb0, Vb0, dAb
+
+
+def ksl_strang_step_vjp(U, S, V, dA1, dA2, Ub1, Sb1, Vb1):
+    """Cotangents (Ub0, Sb0, Vb0, dAb1, dAb2) of the Strang KSL data step
+    (projector_splitting.jl:205-211): the primal half step on dA1, then the
+    dual half step on dA2 from its result."""
+    from .dlra import ksl_primal_step
+
+    Um, Sm, Vm = ksl_primal_step(U, S, V, dA1)
+    Ubm, Sbm, Vbm, dAb2 = ksl_dual_step_vjp(Um, Sm, Vm, dA2, Ub1, Sb1, Vb1)
+    Ub0, Sb0, Vb0, dAb1 = ksl_primal_step_vjp(U, S, V, dA1, Ubm, Sbm, Vbm)
+    return Ub0, Sb0, Vb0, dAb1, dAb2
+
+
 def ksl_primal_step_jvp(U, S, V, dA, Ud, Sd, Vd, dAd):
     """Forward-mode tangents (U1d, S1d, V1d) of the primal KSL data step."""
     P = dA @ V
@@ -147,14 +183,18 @@ def bug_step_jvp(U, S, V, dA, Ud, Sd, Vd, dAd):
 
 
 def bug_solve_gradient(U0, S0, V0, snaps, loss_grad, checkpoint=20, ksl=False):
-    """Gradient of loss(U_T, S_T, V_T) after the BUG (ksl: primal KSL) steps over
-    snaps[0..T] with respect to (U0, S0, V0): checkpoints every `checkpoint` steps
-    on the way forward, recomputation inside each segment on the way back.
-    loss_grad(U, S, V) -> (value, Ub, Sb, Vb)."""
-    from .dlra import bug_step, ksl_primal_step
+    """Gradient of loss(U_T, S_T, V_T) after the BUG (ksl True or "primal": primal
+    KSL; "dual": dual KSL) steps over snaps[0..T] with respect to (U0, S0, V0):
+    checkpoints every `checkpoint` steps on the way forward, recomputation inside
+    each segment on the way back.  loss_grad(U, S, V) -> (value, Ub, Sb, Vb)."""
+    from .dlra import bug_step, ksl_dual_step, ksl_primal_step
 
-    step = ksl_primal_step if ksl else bug_step
-    vjp = ksl_primal_step_vjp if ksl else bug_step_vjp
+    if ksl == "dual":
+        step, vjp = ksl_dual_step, ksl_dual_step_vjp
+    elif ksl:
+        step, vjp = ksl_primal_step, ksl_primal_step_vjp
+    else:
+        step, vjp = bug_step, bug_step_vjp
 
     T = len(snaps) - 1
     cps = {0: (U0, S0, V0)}

@@ -128,6 +128,59 @@ def test_ksl_primal_vjp_against_finite_differences_and_forward_mode():
     assert abs(lhs - an) < 1e-11 * max(1.0, abs(lhs))
 
 
+def _fd_check(rng, step, x, g, yb, e=1e-6):
+    """Central difference of <yb, step(x)> along a random direction, and the
+    same derivative from the cotangents g of x."""
+    D = _dirs(rng, *[a.shape for a in x])
+
+    def f(*y):
+        return sum(np.sum(b * o) for b, o in zip(yb, step(*y)))
+
+    fd = (f(*[a + e * d for a, d in zip(x, D)]) - f(*[a - e * d for a, d in zip(x, D)])) / (2 * e)
+    an = sum(np.sum(a * d) for a, d in zip(g, D))
+    return fd, an
+
+
+def test_ksl_dual_vjp_against_finite_differences_and_transposition():
+    rng = np.random.default_rng(23)
+    n, m, r = 35, 27, 4
+    u = random_factors(rng, n, m, r, decay=0.5)
+    dA = 1e-1 * rng.standard_normal((n, m))
+    yb = _dirs(rng, (n, r), (r, r), (m, r))
+    g = adjoint.ksl_dual_step_vjp(u.U, u.S, u.V, dA, *yb)
+    fd, an = _fd_check(rng, dlra.ksl_dual_step, (u.U, u.S, u.V, dA), g, yb)
+    assert abs(fd - an) < 1e-6 * max(1.0, abs(fd))
+    # the dual step is the primal step of the transposed problem:
+    # dual(U, S, V; dA) = (V1', S1'', U1') for primal(V, S', U; dA') = (U1', S1', V1')
+    d1 = dlra.ksl_dual_step(u.U, u.S, u.V, dA)
+    p1 = dlra.ksl_primal_step(u.V, u.S.T, u.U, dA.T)
+    assert np.allclose(d1[0], p1[2], atol=1e-13) and np.allclose(d1[1], p1[1].T, atol=1e-13)
+    assert np.allclose(d1[2], p1[0], atol=1e-13)
+    gp = adjoint.ksl_primal_step_vjp(u.V, u.S.T, u.U, dA.T, yb[2], yb[1].T, yb[0])
+    for a, b in zip(g, (gp[2], gp[1].T, gp[0], gp[3].T)):
+        assert np.linalg.norm(a - b) <= 1e-12 * max(1.0, np.linalg.norm(b))
+
+
+def test_ksl_strang_vjp_against_finite_differences():
+    rng = np.random.default_rng(24)
+    n, m, r = 33, 29, 3
+    u = random_factors(rng, n, m, r, decay=0.5)
+    dA1, dA2 = 1e-1 * rng.standard_normal((n, m)), 1e-1 * rng.standard_normal((n, m))
+    yb = _dirs(rng, (n, r), (r, r), (m, r))
+    g = adjoint.ksl_strang_step_vjp(u.U, u.S, u.V, dA1, dA2, *yb)
+
+    def strang(U, S, V, a1, a2):
+        return dlra.ksl_dual_step(*dlra.ksl_primal_step(U, S, V, a1), a2)
+
+    fd, an = _fd_check(rng, strang, (u.U, u.S, u.V, dA1, dA2), g, yb)
+    assert abs(fd - an) < 1e-6 * max(1.0, abs(fd))
+    # the Strang step with a zero second increment is the primal step, whose
+    # dual half then only re-orthogonalises: its gradient in dA2 still flows
+    gz = adjoint.ksl_strang_step_vjp(u.U, u.S, u.V, dA1, 0 * dA2, *yb)
+    fd, an = _fd_check(rng, strang, (u.U, u.S, u.V, dA1, 0 * dA2), gz, yb)
+    assert abs(fd - an) < 1e-6 * max(1.0, abs(fd))
+
+
 def test_svd_vjp_matches_finite_differences():
     rng = np.random.default_rng(31)
     q, k = 8, 3
