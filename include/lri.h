@@ -452,6 +452,32 @@ int lri_stream_probe(int pattern, double* tbps);
  * (data_integrator.jl:14-16) that every step is built from. */
 int lri_debug_product(lri_ctx* ctx, int which, int64_t k_prev, int64_t k_next, double* out, int64_t ldo);
 
+/* ---- allocation-free product ----------------------------------------------
+ *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the product pass of lri_debug_product without a context,
+ * on caller-owned device arrays and asynchronously on `stream` (a hipStream_t;
+ * NULL: the default stream), allocating nothing.
+ *
+ *   which = 0:  out (n x r, ld ldo) = dA * B,   B m x r (ld ldb)
+ *   which = 1:  out (m x r, ld ldo) = dA' * B,  B n x r (ld ldb)
+ *
+ * with dA = A_next - A_prev, both n x m column-major (ld lda) of type dtype,
+ * in the arithmetic `precision` selects (as lri_set_precision).  B and out are
+ * FP64.  Every intermediate (the packed B operand and the split partials of
+ * the pass) lives in a static device buffer of LRI_NOALLOC_SCRATCH_BYTES that
+ * each GPU the library runs on holds until the process ends; a shape that
+ * needs more (e.g. n = m = 65536 at r = 64 in FP64) returns LRI_EINVAL.  All
+ * calls on one GPU share that buffer: calls that overlap on one GPU (two
+ * streams, two host threads) race.  lri_noalloc_last_error() gives the message
+ * of the calling thread's last failure. */
+#define LRI_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+int lri_product_noalloc(int which, int dtype, int precision, int64_t n, int64_t m, const void* a_prev,
+                        const void* a_next, int64_t lda, const double* B, int64_t ldb, int64_t r, double* out,
+                        int64_t ldo, void* stream);
+const char* lri_noalloc_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblri_hip.so")
-SOURCES = ["lri_tsmm.hip", "lri_rowops.hip", "lri_de.hip", "lri_quad.hip", "lri_quad_adj.hip", "lri_engine.hip"]
+SOURCES = ["lri_tsmm.hip", "lri_rowops.hip", "lri_de.hip", "lri_quad.hip", "lri_quad_adj.hip", "lri_engine.hip", "lri_noalloc.hip"]
 
 ARCH = "gfx950"
 FLAGS = [

@@ -92,6 +92,8 @@ SIGNATURES = {
     "lri_selftest_mfma": (_int, [_int, _vp, _vp, _vp]),
     "lri_stream_probe": (_int, [_int, _dp]),
     "lri_debug_product": (_int, [_vp, _int, _i64, _i64, _dp, _i64]),
+    "lri_product_noalloc": (_int, [_int, _int, _int, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp]),
+    "lri_noalloc_last_error": (ctypes.c_char_p, []),
 }
 
 _LIB = None
@@ -580,6 +582,19 @@ class Batch:
 
     def synchronize(self):
         _check_batch(lib().lri_batch_synchronize(self.h), self.h)
+
+
+def product_noalloc(which: int, dtype: int, precision: int, n: int, m: int, a_prev: int, a_next: int, lda: int,
+                    B: int, ldb: int, r: int, out: int, ldo: int, stream: int = 0) -> None:
+    """lri_product_noalloc on device pointers (asynchronous on ``stream``; allocates nothing)."""
+    L = lib()
+    st = L.lri_product_noalloc(int(which), int(dtype), int(precision), int(n), int(m), _vp(a_prev), _vp(a_next),
+                               int(lda), _vp(B), int(ldb), int(r), _vp(out), int(ldo), _vp(stream))
+    if st != OK:
+        msg = (L.lri_noalloc_last_error() or b"").decode(errors="replace") or L.lri_status_string(st).decode()
+        if st == EINVAL:
+            raise ValueError(msg)
+        raise LRIError(st, msg)
 
 
 def stream_probe(pattern: int) -> float:
