@@ -70,182 +70,123 @@ __device__ __forceinline__ int chol_scale_exp(double md) {
   return ex & ~1;
 }
 
-// Upper Cholesky G = R'R (k <= K <= 32) and R^{-1}, computed by wave 0 alone
-// with every column in registers.  Lane j < 32 holds column j of the trailing
-// matrix, a[t] = G(t, j), and lane 32 + j column j of B = L^{-1} (L = R', B
-// starts as I), a[t] = B(t, j); the pivot loop is unrolled over K, so every
-// register index is static and step p touches only rows t > p.  One
-// right-looking sweep yields R and R^{-T} together, with no back substitution,
-// and one FMA per row updates both halves: a[t] -= R(p, t) m, with m = R(p, j)
-// in the first half and m = B(p, j) in the second.  The multipliers R(p, t)
-// go through LDS: one store per lane, then all the reads of the step issue
-// back to back into vector registers (readlane broadcasts serialise on the
-// scalar registers, of which these kernels are short).  The pivot of step p+1
-// is taken as soon as lane p+1 has applied step p to its own diagonal.  G is
-// read by rows (it is symmetric), which keeps the LDS reads free of bank
-// conflicts.  A pivot <= tau = 10 eps max_i G(i,i) marks column p as
+// Upper Cholesky G = R'R (k <= K <= 64) and R^{-1}, right-looking on the
+// augmented system [W | B], W the scaled G and B = L^{-1} (L = R', B starting
+// as I): one elimination sweep yields R and R^{-T} together, with no back
+// substitution.  G is scaled by 2^-e (exactly; max_i G(i,i) 2^-e in [0.5, 2))
+// so that every pivot stays a normal number, and R and R^{-1} are unscaled at
+// the end.  A pivot <= tau = 10 eps max_i G(i,i) (scaled) marks column p as
 // numerically dependent: dropped[p] = 1, R(p,p) = 1, row p of R is zero and
 // row and column p of R^{-1} are zero (so Q gets a zero column there).  R and
-// Rinv are written to LDS (column-major, ld k).  Returns the number of dropped
-// columns; sets *nonfinite on NaN/Inf.  Called by all threads.
+// Rinv are written to LDS (column-major, ld k).  Returns the number of
+// dropped columns; sets *nonfinite on NaN/Inf.  Called by all threads.
+//
+// The rows of [W | B] are spread over the first four waves and the columns over the
+// lanes: lane j holds column j of W and of B, wave w the rows t = w, w + 4,
+// ... in a register window (slot s: row 4 (q + s) + w while pivots 4q..4q+3
+// run; the window shifts by one slot after each group of four).  Per pivot p
+// the owning wave (p % 4) forms the pivot row, R(p, :) and L^{-1}(p, :), into
+// one of two LDS row buffers, a barrier publishes it, and every wave updates
+// its rows with K/4 FMAs per half.  Finished rows read zero multipliers, so no
+// row needs a mask.  The pivot loop is not unrolled and has no integer
+// division: a pivot is about a barrier, two LDS round trips and the
+// reciprocal square root (~920 cycles at k = 32, against ~1270 for a sweep
+// by one wave unrolled over the pivots).  G is not modified.
 template <int K>
-__device__ int chol_inv_unrolled(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv,
-                                 int k, unsigned char* dropped, int* nonfinite) {
-  static_assert(K <= 32, "one wave: K columns of G and K of L^{-1}");
+__device__ int chol_inv_waves(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv, int k,
+                              unsigned char* dropped, int* nonfinite) {
+  static_assert(K == 16 || K == 32 || K == 64, "K: 16, 32 or 64");
+  constexpr int NS = K / 4;
+  __shared__ double s_rowR[2][128];  // R(p, j) by column j; [K, 128) stays zero (rows past the window)
+  __shared__ double s_rowB[2][64];   // L^{-1}(p, j)
   __shared__ int s_nd, s_nf;
-  __shared__ double s_row[64];  // the multipliers of the current pivot, broadcast to the wave
-  if (threadIdx.x < 64) {
-    const int l = threadIdx.x, h = l >> 5, j = l & 31;
-    double md = 0.0;
-    int nf = 0;
-    if (h == 0 && j < k) {
-      const double dj = G[j + j * k];
-      nf = !isfinite(dj);
-      md = dj;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      md = fmax(md, __shfl_xor(md, off));
-      nf |= __shfl_xor(nf, off);
-    }
-    const int e2 = chol_scale_exp(md), eh = e2 / 2;
-    const double tau = 10.0 * kEps * ldexp(md, -e2);
-    double a[K];
+  const int t = threadIdx.x, wv = t >> 6, j = t & 63;
+  for (int e = t; e < 2 * 128; e += blockDim.x) (&s_rowR[0][0])[e] = 0.0;
+  // the scale, found by every wave
+  double md = j < k ? G[j + j * k] : 0.0;
+  int nf0 = (j < k) && !isfinite(md);
+  for (int off = 32; off > 0; off >>= 1) {
+    md = fmax(md, __shfl_xor(md, off));
+    nf0 |= __shfl_xor(nf0, off);
+  }
+  const int e2 = chol_scale_exp(md), eh = e2 / 2;
+  const double tau = 10.0 * kEps * ldexp(md, -e2);
+  if (t == 0) {
+    s_nd = 0;
+    s_nf = nf0;
+  }
+  double a[NS], b[NS];
 #pragma unroll
-    for (int t = 0; t < K; ++t) {
-      if (h == 0)
-        a[t] = (t < k && j < k && t <= j) ? ldexp(G[j + t * k], -e2) : 0.0;
-      else
-        a[t] = (t == j && j < k) ? 1.0 : 0.0;
-    }
-    int nd = 0;
-    bool mydrop = false;
-    double dnext = readlane_f64(a[0], 0);
+  for (int s = 0; s < NS; ++s) {
+    const int tt = 4 * s + wv;
+    a[s] = (wv < 4 && tt < k && j < k && tt <= j) ? ldexp(G[tt + j * k], -e2) : 0.0;
+    b[s] = (wv < 4 && tt == j && j < k) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int q = 0; 4 * q < k; ++q) {
 #pragma unroll
-    for (int p = 0; p < K; ++p) {
+    for (int wp = 0; wp < 4; ++wp) {
+      const int p = 4 * q + wp;
       if (p < k) {
-        const double d = dnext;
-        const bool fin = isfinite(d);
-        const bool keep = (d > tau) && fin;
-        nf |= !fin;
-        const double inv = keep ? rsqrt_nr(d) : 1.0;
-        nd += keep ? 0 : 1;
-        mydrop = mydrop || (j == p && !keep);
-        // first half: R(p, j); second half: (L^{-1})(p, j) = R^{-1}(j, p)
-        const double m = (keep && j < k && (h == 1 || j > p)) ? a[p] * inv : 0.0;
-        if (p + 1 < K) dnext = readlane_f64(__builtin_fma(-m, m, a[p + 1]), p + 1);
-        if (j < k) {
-          if (h == 0)
-            R[p + j * k] = (j == p) ? (keep ? ldexp(d * inv, eh) : 1.0) : (j > p ? ldexp(m, eh) : 0.0);
-          else
-            Rinv[j + p * k] = mydrop ? 0.0 : ldexp(m, -eh);
+        if (wv == wp) {
+          const double d = readlane_f64(a[0], p);
+          const bool fin = isfinite(d);
+          const bool keep = (d > tau) && fin;
+          const double inv = keep ? rsqrt_nr(d) : 1.0;
+          const double mR = (keep && j > p && j < k) ? a[0] * inv : 0.0;
+          const double mB = (keep && j < k) ? b[0] * inv : 0.0;
+          s_rowR[buf][j] = mR;
+          s_rowB[buf][j] = mB;
+          if (j < k) {
+            R[p + j * k] = j == p ? (keep ? d * inv : 1.0) : mR;
+            Rinv[j + p * k] = mB;
+          }
+          if (j == p) {
+            dropped[p] = keep ? 0 : 1;
+            s_nd += keep ? 0 : 1;
+            s_nf |= !fin;
+          }
         }
-        if (h == 0 && j == p) dropped[p] = keep ? 0 : 1;
-        // rows t > p: G(t, j) -= R(p,t) R(p,j), B(t, j) -= R(p,t) (L^{-1})(p,j)
-        // (entries of the first half below the diagonal are never read, so they
-        // need no mask; lanes t >= k hold R(p,t) = 0)
-        s_row[l] = m;
-        wave_lds_sync();
-        double rt[K];
+        __syncthreads();
+        if (wv < 4) {
+          const double mRj = s_rowR[buf][j], mBj = s_rowB[buf][j];
 #pragma unroll
-        for (int t = p + 1; t < K; ++t) rt[t] = s_row[t];
-#pragma unroll
-        for (int t = p + 1; t < K; ++t) a[t] = __builtin_fma(-rt[t], m, a[t]);
-        wave_lds_sync();
+          for (int s = 0; s < NS; ++s) {
+            const double rt = s_rowR[buf][4 * (q + s) + wv];
+            a[s] = __builtin_fma(-rt, mRj, a[s]);
+            b[s] = __builtin_fma(-rt, mBj, b[s]);
+          }
+        }
+        buf ^= 1;
       }
     }
-    if (l == 0) {
-      s_nd = nd;
-      s_nf = nf;
+#pragma unroll
+    for (int s = 0; s + 1 < NS; ++s) {
+      a[s] = a[s + 1];
+      b[s] = b[s + 1];
     }
+    a[NS - 1] = 0.0;
+    b[NS - 1] = 0.0;
   }
   __syncthreads();
-  if (threadIdx.x == 0 && s_nf) *nonfinite = 1;
+  // unscale; the rows of R^{-1} of dropped columns are zero (lanes: rows, waves: columns)
+  const int nw = blockDim.x >> 6;
+  if (j < k)
+    for (int c = wv; c < k; c += nw) {
+      R[j + c * k] = ldexp(R[j + c * k], eh);
+      Rinv[j + c * k] = dropped[j] ? 0.0 : ldexp(Rinv[j + c * k], -eh);
+    }
+  if (t == 0 && s_nf) *nonfinite = 1;
   __syncthreads();
   return s_nd;
 }
 
-// The same sweep for K = 64 with a runtime pivot loop (full unrolling would
-// not fit the register file): the windows g[t] = G(p + t, j) and w[t] shift by
-// one row per pivot so that register indices stay static, and the multipliers
-// are read from a zeroed LDS row of 2K entries by absolute column.
-template <int K>
-__device__ int chol_inv_window(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv,
-                               int k, unsigned char* dropped, int* nonfinite) {
-  __shared__ int s_nd, s_nf;
-  __shared__ double s_r[2 * K];  // R(p, c) by column c; [k, 2K) stays zero
-  if (threadIdx.x < 64) {
-    const int j = threadIdx.x;
-    for (int e = j; e < 2 * K; e += 64) s_r[e] = 0.0;
-    double md = 0.0;
-    int nf = 0;
-    if (j < k) {
-      const double dj = G[j + j * k];
-      nf = !isfinite(dj);
-      md = dj;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      md = fmax(md, __shfl_xor(md, off));
-      nf |= __shfl_xor(nf, off);
-    }
-    const int e2 = chol_scale_exp(md), eh = e2 / 2;
-    const double tau = 10.0 * kEps * ldexp(md, -e2);
-    double g[K], w[K];
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-      g[t] = (t < k && j < k && t <= j) ? ldexp(G[j + t * k], -e2) : 0.0;  // G symmetric: read by rows
-      w[t] = (t == j && j < k) ? 1.0 : 0.0;
-    }
-    int nd = 0;
-    bool mydrop = false;
-    wave_lds_sync();
-    double dnext = readlane_f64(g[0], 0);
-    for (int p = 0; p < k; ++p) {
-      const double d = dnext;
-      const bool fin = isfinite(d);
-      const bool keep = (d > tau) && fin;
-      nf |= !fin;
-      const double inv = keep ? rsqrt_nr(d) : 1.0;
-      nd += keep ? 0 : 1;
-      mydrop = mydrop || (j == p && !keep);
-      const double rpj = (keep && j > p && j < k) ? g[0] * inv : 0.0;
-      const double xpj = (keep && j < k) ? w[0] * inv : 0.0;
-      dnext = readlane_f64(__builtin_fma(-rpj, rpj, g[1]), (p + 1) & 63);
-      if (j < k) {
-        R[p + j * k] = (j == p) ? (keep ? ldexp(d * inv, eh) : 1.0) : (j > p ? ldexp(rpj, eh) : 0.0);
-        Rinv[j + p * k] = mydrop ? 0.0 : ldexp(xpj, -eh);
-      }
-      if (j == p) dropped[p] = keep ? 0 : 1;
-      if (j < 2 * K) s_r[j] = rpj;
-      wave_lds_sync();
-#pragma unroll
-      for (int t = 1; t < K; ++t) {
-        const double rpi = s_r[p + t];
-        g[t - 1] = __builtin_fma(-rpi, rpj, g[t]);
-        w[t - 1] = __builtin_fma(-rpi, xpj, w[t]);
-      }
-      g[K - 1] = 0.0;
-      w[K - 1] = 0.0;
-      wave_lds_sync();
-    }
-    if (j == 0) {
-      s_nd = nd;
-      s_nf = nf;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && s_nf) *nonfinite = 1;
-  __syncthreads();
-  return s_nd;
-}
-
-// CK: the Cholesky variant compiled into an epilogue kernel (16, 32: unrolled
-// for k <= CK; 64: runtime loop).  Each kernel instance carries one variant,
-// so the register allocation of one does not burden the others.
+// CK: the Cholesky variant compiled into an epilogue kernel (16, 32, 64: the
+// bound on k), all in the four-wave form.
 template <int CK>
 __device__ int chol_inv(const double* G, double* R, double* Rinv, int k, unsigned char* dropped, int* nonfinite) {
-  if (CK == 16) return chol_inv_unrolled<16>(G, R, Rinv, k, dropped, nonfinite);
-  if (CK == 32) return chol_inv_unrolled<32>(G, R, Rinv, k, dropped, nonfinite);
-  return chol_inv_window<64>(G, R, Rinv, k, dropped, nonfinite);
+  return chol_inv_waves<CK>(G, R, Rinv, k, dropped, nonfinite);
 }
 
 // C = op(A) * op(B), k x k, column-major, on the FP64 matrix cores (16 x 16
@@ -736,22 +677,36 @@ __device__ __forceinline__ void rows_body(const RowArgs<PT> A, int bid, int nblk
     for (int q = 0; q < 8; ++q) v[q] = 0.0;
     if (ok) {
       if (A.parts) {
+        // the partials were written by the previous launch, mostly into other
+        // XCDs' L2: every load is a fabric round trip, so the loads of all 8
+        // columns and of PC splits at a time are issued together (addresses
+        // clamped into the array, the extra values discarded), then summed in
+        // split order
+        constexpr int PC = (sizeof(PT) == 4 ? 8 : 4) / (NRB >= 16 ? 2 : 1);  // (NRB 16 has no registers to spare)
+        const int64_t sstride = (int64_t)A.rb * A.rows;
+        const int cmax = ro - 1;
+        for (int sp0 = 0; sp0 < A.nsplit; sp0 += PC) {
+          PT w[8][PC];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int c = 8 * cg + q;
-          if (c < ro) {
-            double sum = 0.0;
-            int sp = 0;
-            for (; sp + 4 <= A.nsplit; sp += 4) {
-              double w[4];
+          for (int q = 0; q < 8; ++q) {
+            const int c = min(8 * cg + q, cmax);
 #pragma unroll
-              for (int u = 0; u < 4; ++u) w[u] = (double)A.parts[((int64_t)(sp + u) * A.rb + c) * A.rows + i];
-#pragma unroll
-              for (int u = 0; u < 4; ++u) sum += w[u];
+            for (int u = 0; u < PC; ++u) {
+              const int sp = min(sp0 + u, A.nsplit - 1);
+              w[q][u] = A.parts[sp * sstride + (int64_t)c * A.rows + i];
             }
-            for (; sp < A.nsplit; ++sp) sum += (double)A.parts[((int64_t)sp * A.rb + c) * A.rows + i];
-            v[q] = sum;
-            if (A.sum_out) A.sum_out[i + c * A.ld_sum] = sum;
+          }
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int u = 0; u < PC; ++u)
+              if (sp0 + u < A.nsplit) v[q] += (double)w[q][u];
+        }
+        if (A.sum_out) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int c = 8 * cg + q;
+            if (c < ro) A.sum_out[i + c * A.ld_sum] = v[q];
           }
         }
       }
@@ -1089,7 +1044,7 @@ __device__ __forceinline__ double group_sum(double x) {
 // rows ln, ln + G, ... of the pair in each lane, the group sums through DPP,
 // no workgroup barriers (LDS operations of one wavefront complete in order).
 template <int G>
-__device__ void jacobi_wave_body(const SvdArgs& P) {
+__device__ void jacobi_wave_body(const SvdArgs P) {
   constexpr int R = (128 / (G * G)) > 0 ? 128 / (G * G) : 1;  // rows per lane: ke <= 128 / G
   constexpr int LDA = 33;                                       // odd column stride against bank conflicts
   __shared__ double A[LDA * 32];
@@ -1191,7 +1146,7 @@ __device__ __forceinline__ double group_sum_rt(double x, int g) {
 // The workgroup form (k <= 128, blockDim.x threads): gsz lanes per column pair,
 // a workgroup barrier per round.  A (k x ke) in LDS at an odd stride; V in
 // LDS when it fits, else in P.work.
-__device__ void jacobi_svd_body(const SvdArgs& P, double* lds) {
+__device__ void jacobi_svd_body(const SvdArgs P, double* lds) {
   const int k = P.k;
   const int ke = (k + 1) & ~1;
   const int nt = blockDim.x, t = threadIdx.x;
@@ -1560,7 +1515,7 @@ __global__ __launch_bounds__(1024) void k_jacobi_svd(SvdArgs P0, SvdArgs P1) {
   jacobi_svd_body(blockIdx.x == 0 ? P0 : P1, lds);
 }
 
-__device__ void jacobi_wave_dispatch(const SvdArgs& P);
+__device__ void jacobi_wave_dispatch(const SvdArgs P);
 
 // one wavefront per decomposition, k <= 32 (workgroup b decomposes P_b)
 __global__ __launch_bounds__(64) void k_jacobi_wave(SvdArgs P0, SvdArgs P1) {
@@ -1585,7 +1540,7 @@ __global__ __launch_bounds__(64) void k_jacobi_wave_batch(const unsigned char* _
   jacobi_wave_dispatch(blockIdx.x - start[i] == 0 ? S.P0 : S.P1);
 }
 
-__device__ void jacobi_wave_dispatch(const SvdArgs& P) {
+__device__ void jacobi_wave_dispatch(const SvdArgs P) {
   const int np = ((P.k + 1) & ~1) / 2;
   if (np > 8)
     jacobi_wave_body<4>(P);
