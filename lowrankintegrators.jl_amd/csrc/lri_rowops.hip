@@ -81,27 +81,28 @@ __device__ __forceinline__ int chol_scale_exp(double md) {
 // Rinv are written to LDS (column-major, ld k).  Returns the number of
 // dropped columns; sets *nonfinite on NaN/Inf.  Called by all threads.
 //
-// The rows of [W | B] are spread over the first four waves and the columns over the
-// lanes: lane j holds column j of W and of B, wave w the rows t = w, w + 4,
-// ... in a register window (slot s: row 4 (q + s) + w while pivots 4q..4q+3
-// run; the window shifts by one slot after each group of four).  Per pivot p
-// the owning wave (p % 4) forms the pivot row, R(p, :) and L^{-1}(p, :), into
-// one of two LDS row buffers, a barrier publishes it, and every wave updates
-// its rows with K/4 FMAs per half.  Finished rows read zero multipliers, so no
-// row needs a mask.  The pivot loop is not unrolled and has no integer
-// division: a pivot is about a barrier, two LDS round trips and the
-// reciprocal square root (~920 cycles at k = 32, against ~1270 for a sweep
-// by one wave unrolled over the pivots).  G is not modified.
+// The rows of [W | B] are spread over the first four waves in blocks of four
+// (wave w: row blocks w, w + 4, ...; at group g of 16 pivots, slot s of a
+// wave's register window holds row 16 (g + s / 4) + 4 w + s % 4, and the
+// window shifts by one block after each group) and the columns over the lanes
+// (lane j: column j of W and of B).  The wave owning a block factors its four
+// pivots in registers, the pivot rows exchanged by readlane, writes the four
+// rows R(p, :) and L^{-1}(p, :) into one of two LDS buffers, and after one
+// barrier every other wave applies them to its rows (K/4 slots, 2 FMAs each).
+// Finished rows read zero multipliers, so no row needs a mask, and the group
+// loop is not unrolled.  At k = 32 this takes ~26K cycles, against ~35K with
+// one barrier per pivot and ~40K for one wave unrolled over the pivots (LDS
+// broadcasts).  G is not modified.
 template <int K>
 __device__ int chol_inv_waves(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv, int k,
                               unsigned char* dropped, int* nonfinite) {
   static_assert(K == 16 || K == 32 || K == 64, "K: 16, 32 or 64");
   constexpr int NS = K / 4;
-  __shared__ double s_rowR[2][128];  // R(p, j) by column j; [K, 128) stays zero (rows past the window)
-  __shared__ double s_rowB[2][64];   // L^{-1}(p, j)
-  __shared__ int s_nd, s_nf;
+  __shared__ double s_rowR[2][4][128];  // R(p, j) by column j; [K, 128) stays zero (rows past the window)
+  __shared__ double s_rowB[2][4][64];   // L^{-1}(p, j)
+  __shared__ int s_nfw[4];
   const int t = threadIdx.x, wv = t >> 6, j = t & 63;
-  for (int e = t; e < 2 * 128; e += blockDim.x) (&s_rowR[0][0])[e] = 0.0;
+  for (int e = t; e < 2 * 4 * 128; e += blockDim.x) (&s_rowR[0][0][0])[e] = 0.0;
   // the scale, found by every wave
   double md = j < k ? G[j + j * k] : 0.0;
   int nf0 = (j < k) && !isfinite(md);
@@ -111,64 +112,85 @@ __device__ int chol_inv_waves(const double* __restrict__ G, double* __restrict__
   }
   const int e2 = chol_scale_exp(md), eh = e2 / 2;
   const double tau = 10.0 * kEps * ldexp(md, -e2);
-  if (t == 0) {
-    s_nd = 0;
-    s_nf = nf0;
-  }
   double a[NS], b[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const int tt = 4 * s + wv;
+    const int tt = 16 * (s / 4) + 4 * wv + (s & 3);
     a[s] = (wv < 4 && tt < k && j < k && tt <= j) ? ldexp(G[tt + j * k], -e2) : 0.0;
     b[s] = (wv < 4 && tt == j && j < k) ? 1.0 : 0.0;
   }
+  int nfl = nf0;
   __syncthreads();
   int buf = 0;
-  for (int q = 0; 4 * q < k; ++q) {
+  for (int g = 0; 16 * g < k; ++g) {
 #pragma unroll
     for (int wp = 0; wp < 4; ++wp) {
-      const int p = 4 * q + wp;
-      if (p < k) {
+      const int p0 = 16 * g + 4 * wp;
+      if (p0 < k) {
         if (wv == wp) {
-          const double d = readlane_f64(a[0], p);
-          const bool fin = isfinite(d);
-          const bool keep = (d > tau) && fin;
-          const double inv = keep ? rsqrt_nr(d) : 1.0;
-          const double mR = (keep && j > p && j < k) ? a[0] * inv : 0.0;
-          const double mB = (keep && j < k) ? b[0] * inv : 0.0;
-          s_rowR[buf][j] = mR;
-          s_rowB[buf][j] = mB;
-          if (j < k) {
-            R[p + j * k] = j == p ? (keep ? d * inv : 1.0) : mR;
-            Rinv[j + p * k] = mB;
-          }
-          if (j == p) {
-            dropped[p] = keep ? 0 : 1;
-            s_nd += keep ? 0 : 1;
-            s_nf |= !fin;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int p = p0 + i;
+            if (p < k) {
+              const double d = readlane_f64(a[i], p);
+              const bool fin = isfinite(d);
+              const bool keep = (d > tau) && fin;
+              const double inv0 = rsqrt_nr(keep ? d : 1.0);
+              const double inv = keep ? inv0 : 1.0;
+              const double mR = (keep && j > p && j < k) ? a[i] * inv : 0.0;
+              const double mB = (keep && j < k) ? b[i] * inv : 0.0;
+#pragma unroll
+              for (int i2 = i + 1; i2 < 4; ++i2) {
+                const double rt = readlane_f64(mR, (p0 + i2) & 63);
+                a[i2] = __builtin_fma(-rt, mR, a[i2]);
+                b[i2] = __builtin_fma(-rt, mB, b[i2]);
+              }
+              s_rowR[buf][i][j] = mR;
+              s_rowB[buf][i][j] = mB;
+              if (j < k) {
+                R[p + j * k] = j == p ? (keep ? d * inv : 1.0) : mR;
+                Rinv[j + p * k] = mB;
+              }
+              if (j == p) dropped[p] = keep ? 0 : 1;
+              nfl |= !fin;
+            } else {
+              s_rowR[buf][i][j] = 0.0;
+              s_rowB[buf][i][j] = 0.0;
+            }
           }
         }
         __syncthreads();
         if (wv < 4) {
-          const double mRj = s_rowR[buf][j], mBj = s_rowB[buf][j];
+          // the owner's own block is done; its later blocks start at slot 4
+          const int s0 = wv == wp ? 4 : 0;
 #pragma unroll
-          for (int s = 0; s < NS; ++s) {
-            const double rt = s_rowR[buf][4 * (q + s) + wv];
-            a[s] = __builtin_fma(-rt, mRj, a[s]);
-            b[s] = __builtin_fma(-rt, mBj, b[s]);
+          for (int i = 0; i < 4; ++i) {
+            const double mRj = s_rowR[buf][i][j], mBj = s_rowB[buf][i][j];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+              if (s >= s0) {
+                const double rt = s_rowR[buf][i][16 * (g + s / 4) + 4 * wv + (s & 3)];
+                a[s] = __builtin_fma(-rt, mRj, a[s]);
+                b[s] = __builtin_fma(-rt, mBj, b[s]);
+              }
+            }
           }
         }
         buf ^= 1;
       }
     }
 #pragma unroll
-    for (int s = 0; s + 1 < NS; ++s) {
-      a[s] = a[s + 1];
-      b[s] = b[s + 1];
+    for (int s = 0; s + 4 < NS; ++s) {
+      a[s] = a[s + 4];
+      b[s] = b[s + 4];
     }
-    a[NS - 1] = 0.0;
-    b[NS - 1] = 0.0;
+#pragma unroll
+    for (int s = (NS > 4 ? NS - 4 : 0); s < NS; ++s) {
+      a[s] = 0.0;
+      b[s] = 0.0;
+    }
   }
+  if (j == 0 && wv < 4) s_nfw[wv] = nfl;
   __syncthreads();
   // unscale; the rows of R^{-1} of dropped columns are zero (lanes: rows, waves: columns)
   const int nw = blockDim.x >> 6;
@@ -177,9 +199,10 @@ __device__ int chol_inv_waves(const double* __restrict__ G, double* __restrict__
       R[j + c * k] = ldexp(R[j + c * k], eh);
       Rinv[j + c * k] = dropped[j] ? 0.0 : ldexp(Rinv[j + c * k], -eh);
     }
-  if (t == 0 && s_nf) *nonfinite = 1;
+  const int nd = __popcll(__ballot(j < k && dropped[j]));
+  if (t == 0 && (s_nfw[0] | s_nfw[1] | s_nfw[2] | s_nfw[3])) *nonfinite = 1;
   __syncthreads();
-  return s_nd;
+  return nd;
 }
 
 // CK: the Cholesky variant compiled into an epilogue kernel (16, 32, 64: the
@@ -911,7 +934,8 @@ hipError_t launch_rows(RowArgs<PT> A, hipStream_t st) {
   }
   const size_t lds = sizeof(double) * (p.lds_doubles + pre_doubles);
   const size_t epi_lds = sizeof(double) * ((size_t)A.r_out * rc + epi_scratch_doubles(A.epi.kind, A.r_out, rc));
-  if (lds > 150 * 1024 || epi_lds > 150 * 1024) return hipErrorInvalidValue;  // room for static shared variables
+  // room for the static shared variables (the Cholesky row buffers: 12.3 KB)
+  if (lds > 147 * 1024 || epi_lds > 147 * 1024) return hipErrorInvalidValue;
   const int64_t ntiles = (A.rows + p.rpb - 1) / p.rpb;
   int grid = (int)(ntiles < A.max_blocks ? ntiles : A.max_blocks);
   if (grid < 1) grid = 1;
