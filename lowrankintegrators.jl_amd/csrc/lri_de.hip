@@ -499,7 +499,7 @@ hipError_t launch_spectral_flow(double* Z, int64_t n, int k, const double* lam, 
 // ----------------------------------------------------------------------------
 // Symmetric k x k eigen-decomposition (k <= 128) as the one-sided Jacobi SVD of
 // M + sigma I with sigma = max_i sum_j |M_ij| >= spectral radius, so the
-// shifted matrix is positive semidefinite: its right singular vectors are
+// shifted matrix is positive semidefinite: its singular vectors are
 // eigenvectors of M and its singular values minus sigma the eigenvalues.
 // ----------------------------------------------------------------------------
 
@@ -583,8 +583,13 @@ hipError_t launch_sym_eig(const SymEigArgs* P, int count, hipStream_t st) {
     sv[b].S = p.work;
     sv[b].lds = p.k;
     sv[b].k = p.k;
-    sv[b].U = p.Uscr;
-    sv[b].V = p.W;
+    // the eigenvectors are the orthonormal factor: the accumulated rotations,
+    // which are the left singular vectors when the sweeps run QR-preconditioned
+    // (k <= 64) and the right ones otherwise (the other factor has zero columns
+    // where M + sigma I is singular)
+    const bool pre = p.k <= 64;
+    sv[b].U = pre ? p.W : p.Uscr;
+    sv[b].V = pre ? p.Uscr : p.W;
     sv[b].sigma = p.ev;
     sv[b].r_out = p.r_scr;
     sv[b].tol = 0.0;
