@@ -3433,7 +3433,7 @@ struct MergedPhase {
     unsigned block, total;
     size_t lds;
     int count;
-    size_t meta_off;  // its workgroup offsets and slots in `meta` (2 count + 1 ints)
+    size_t meta_off;  // its workgroup offsets, slots and workgroup members in `meta` (2 count + 1 + total ints)
   };
   int count = 0;
   int cap = 0;
@@ -3528,6 +3528,8 @@ struct MergedPhase {
         }
         meta_vec.push_back((int)acc);
         for (int i : g.members) meta_vec.push_back(i * cap + (int)p);
+        for (size_t m = 0; m < g.members.size(); ++m)  // the member of every workgroup
+          meta_vec.insert(meta_vec.end(), recs[g.members[m]][p].grid, (int)m);
         ln.total = acc;
         launches.push_back(ln);
       }

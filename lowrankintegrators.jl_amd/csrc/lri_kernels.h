@@ -82,15 +82,12 @@ hipError_t launch_rowops_batched(int kid, int variant, const void* tab, const in
 hipError_t launch_de_batched(int kid, int variant, const void* tab, const int* start, int count, unsigned total,
                              unsigned block, size_t lds, hipStream_t st);
 
-// The member of workgroup b: the last i with start[i] <= b (start has count + 1 entries).
+// The member of workgroup b, the last i with start[i] <= b.  A merged launch's
+// table holds the workgroup offsets start[0..count], the members' argument
+// slots start[count + 1 + i] and then the member of every workgroup, so this
+// is one load (a binary search would be nine dependent ones per workgroup).
 __device__ __forceinline__ int batch_member(const int* __restrict__ start, int count, int b) {
-  int lo = 0, hi = count;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (start[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+  return start[2 * count + 1 + b];
 }
 
 // r x r (up to 2r x 2r) FP64 matrices of one step, device-resident.
