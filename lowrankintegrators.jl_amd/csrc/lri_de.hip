@@ -584,12 +584,10 @@ hipError_t launch_sym_eig(const SymEigArgs* P, int count, hipStream_t st) {
     sv[b].lds = p.k;
     sv[b].k = p.k;
     // the eigenvectors are the orthonormal factor: the accumulated rotations,
-    // which are the left singular vectors when the sweeps run QR-preconditioned
-    // (k <= 64) and the right ones otherwise (the other factor has zero columns
-    // where M + sigma I is singular)
-    const bool pre = p.k <= 64;
-    sv[b].U = pre ? p.W : p.Uscr;
-    sv[b].V = pre ? p.Uscr : p.W;
+    // which the QR-preconditioned sweeps return as the left singular vectors
+    // (the right ones have zero columns where M + sigma I is singular)
+    sv[b].U = p.W;
+    sv[b].V = p.Uscr;
     sv[b].sigma = p.ev;
     sv[b].r_out = p.r_scr;
     sv[b].tol = 0.0;

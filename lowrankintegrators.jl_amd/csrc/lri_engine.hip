@@ -133,7 +133,7 @@ struct DeSmall {
   double Snew[kMaxAug * kMaxAug];
   double ework[2][kMaxAug * kMaxAug + 2];  // shifted matrix + shift, per concurrent eigenproblem
   double Uscr[2][kMaxAug * kMaxAug];
-  double svd2[kMaxAug * kMaxAug];          // global Jacobi work of the second eigenproblem (k > 64)
+  double svd2[kSvdWorkDoubles];            // global Jacobi work of the second eigenproblem (k > 64)
   double swork[2 * kMaxAug * kMaxAug + 2 * kMaxAug * kMaxForce];
   int rscr[2];
 };
@@ -3845,7 +3845,7 @@ int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
     alloc((void**)&c->gpart, d * 256 * kMaxAug * kMaxAug);
     alloc((void**)&c->flags, 64);
     alloc((void**)&c->small, sizeof(SmallState));
-    alloc((void**)&c->svd_work, d * kMaxAug * kMaxAug);
+    alloc((void**)&c->svd_work, d * kSvdWorkDoubles);
     HIPCHECK(hipHostMalloc((void**)&c->r_host, sizeof(int), hipHostMallocDefault));
     *c->r_host = 0;
     HIPCHECK(hipDeviceSynchronize());

@@ -216,8 +216,11 @@ struct SvdArgs {
   int* r_out = nullptr;
   double tol = 0.0;
   int64_t rmax = 0;
-  double* work = nullptr;  // global ke x ke scratch, used when it does not fit in LDS
+  double* work = nullptr;  // global scratch of kSvdWorkDoubles for k > 64 (V, the QR preconditioner)
 };
+// the global Jacobi work of one decomposition with k > 64: V (ke x ke), the
+// preconditioner's reflectors (k x k) and tau (k)
+constexpr int kSvdWorkDoubles = 2 * kMaxAug * kMaxAug + kMaxAug;
 hipError_t launch_jacobi_svd(SvdArgs P, hipStream_t st);
 
 // The QR of a rank-deficient matrix.  CholeskyQR2 drops a pivot whose
@@ -574,7 +577,7 @@ hipError_t launch_spectral_flow(double* Z, int64_t n, int k, const double* lam, 
                                 const double* coef, int q, double h, int64_t N, hipStream_t st);
 
 // Symmetric eigen-decomposition M = W diag(ev) W' of a k x k matrix (k <= kMaxAug).
-// work: k*k + 2 doubles; svd_work: kMaxAug^2 doubles; Uscr: k*k doubles; r_scr: one int.
+// work: k*k + 2 doubles; svd_work: kSvdWorkDoubles; Uscr: k*k doubles; r_scr: one int.
 struct SymEigArgs {
   const double* M = nullptr;
   int ldm = 0, k = 0;

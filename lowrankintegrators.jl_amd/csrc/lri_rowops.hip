@@ -1088,22 +1088,25 @@ __device__ __forceinline__ double group_sum_rt(double x, int g) {
 // accumulated rotations, W = Ur Sigma), S = (Q J) Sigma Ur': the left
 // singular vectors are Q J and the right ones the normalised columns of W.
 //
-// Householder QR of A (k x k, ld lda) in LDS, k <= 64, nt threads (a multiple
-// of 64): reflector j (H_j = I - tau_j v v', v_j = 1, as LAPACK's dlarfg) in
+// Householder QR of A (k x k, ld lda), k <= 128, nt threads (a multiple of
+// 64): reflector j (H_j = I - tau_j v v', v_j = 1, as LAPACK's dlarfg) in
 // column j of H (k x k, ld k), tau_j in tau; on return A holds R' (lower
-// triangular).  Wave 0 forms each reflector (one row per lane); nt / 64
-// threads per column apply it.
+// triangular).  Wave 0 forms each reflector (rows lane and lane + 64);
+// nt / 64 (k <= 64) or nt / 128 threads per column apply it.  A, H and tau may
+// be in LDS or in global memory (the workgroup barriers order both).
 __device__ void jacobi_qr_precondition(double* A, int lda, int k, double* H, double* tau, int nt) {
   __shared__ double s_hv[2];  // beta (the new diagonal), tau of the current reflector
   const int t = threadIdx.x;
-  const int gq = nt >> 6;          // threads per column
+  const int cap = k > 64 ? 128 : 64;
+  const int gq = nt / cap;  // threads per column
   const int c = t / gq, ln = t % gq;
   for (int j = 0; j < k; ++j) {
     if (t < 64) {
-      const int i = t;
-      const double x = (i < k) ? A[i + j * lda] : 0.0;
-      const double a = readlane_f64(x, j);
-      double s2 = (i > j && i < k) ? x * x : 0.0;
+      const int i0 = t, i1 = t + 64;
+      const double x0 = i0 < k ? A[i0 + j * lda] : 0.0;
+      const double x1 = i1 < k ? A[i1 + j * lda] : 0.0;
+      const double a = j < 64 ? readlane_f64(x0, j) : readlane_f64(x1, j - 64);
+      double s2 = (i0 > j && i0 < k ? x0 * x0 : 0.0) + (i1 > j && i1 < k ? x1 * x1 : 0.0);
       s2 = group_sum<64>(s2);
       double beta = a, tj = 0.0, scale = 0.0;
       if (s2 > 0.0) {
@@ -1111,8 +1114,9 @@ __device__ void jacobi_qr_precondition(double* A, int lda, int k, double* H, dou
         tj = (beta - a) / beta;
         scale = 1.0 / (a - beta);
       }
-      if (i < k) H[i + j * k] = i == j ? 1.0 : (i > j ? x * scale : 0.0);
-      if (i == 0) {
+      if (i0 < k) H[i0 + j * k] = i0 == j ? 1.0 : (i0 > j ? x0 * scale : 0.0);
+      if (i1 < k) H[i1 + j * k] = i1 == j ? 1.0 : (i1 > j ? x1 * scale : 0.0);
+      if (t == 0) {
         s_hv[0] = beta;
         s_hv[1] = tj;
         tau[j] = tj;
@@ -1143,11 +1147,12 @@ __device__ void jacobi_qr_precondition(double* A, int lda, int k, double* H, dou
   __syncthreads();
 }
 
-// X <- Q X = H_0 H_1 ... H_{k-2} X for the reflectors of jacobi_qr_precondition,
-// X k rows x ncols (ld ldx, ncols <= 64), nt threads
+// X <- Q X = H_0 H_1 ... H_{k-1} X for the reflectors of jacobi_qr_precondition,
+// X k rows x ncols (ld ldx, ncols <= 128), nt threads
 __device__ void jacobi_apply_q(const double* H, const double* tau, int k, double* X, int ldx, int ncols, int nt) {
   const int t = threadIdx.x;
-  const int gq = nt >> 6;
+  const int cap = ncols > 64 ? 128 : 64;
+  const int gq = nt / cap;
   const int c = t / gq, ln = t % gq;
   for (int j = k - 1; j >= 0; --j) {
     const double tj = tau[j];
@@ -1266,8 +1271,9 @@ __device__ void jacobi_svd_body(const SvdArgs P, double* lds) {
   double* A = lds;                      // k rows x ke columns, ld lda
   double* sig = A + (size_t)lda * ke;   // ke
   double* Vm = P.work ? P.work : sig + ke;  // ke x ke, ld ldv
-  const bool pre = k <= 64;             // QR preconditioning (V is then in LDS)
-  double* Hq = sig + ke + (size_t)ldv * ke;  // k x k reflectors, then 64 tau
+  // the QR preconditioner's reflectors (k x k) and tau (k): after V, in LDS or
+  // in the global work buffer
+  double* Hq = Vm + (size_t)ldv * ke;
   double* tq = Hq + (size_t)k * k;
   for (int e = t; e < ke * k; e += nt) {
     const int i = e % k, j = e / k;
@@ -1278,7 +1284,7 @@ __device__ void jacobi_svd_body(const SvdArgs P, double* lds) {
     Vm[i + j * ldv] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
-  if (pre) jacobi_qr_precondition(A, lda, k, Hq, tq, nt);  // A <- R' of S = Q R
+  jacobi_qr_precondition(A, lda, k, Hq, tq, nt);  // A <- R' of S = Q R
   const int npairs = ke / 2;
   int gsz = 1;
   while (gsz * 2 * npairs <= nt && gsz < 64) gsz *= 2;
@@ -1338,8 +1344,8 @@ __device__ void jacobi_svd_body(const SvdArgs P, double* lds) {
     __syncthreads();
     if (!any) break;
   }
-  if (pre) jacobi_apply_q(Hq, tq, k, Vm, ldv, ke, nt);  // Vm <- Q J
-  jacobi_finish(P, A, lda, Vm, ldv, sig, nt, pre);
+  jacobi_apply_q(Hq, tq, k, Vm, ldv, ke, nt);  // Vm <- Q J
+  jacobi_finish(P, A, lda, Vm, ldv, sig, nt, true);
 }
 
 __global__ __launch_bounds__(kThreads) void k_small_abt(const double* __restrict__ A, const double* __restrict__ B,
@@ -1680,14 +1686,14 @@ __device__ void jacobi_wave_dispatch(const SvdArgs P) {
 static hipError_t jacobi_plan(SvdArgs& P, size_t* lds) {
   if (P.k < 1 || P.k > kMaxAug) return hipErrorInvalidValue;
   const int ke = (P.k + 1) & ~1;
-  *lds = sizeof(double) * ((size_t)(P.k | 1) * ke + ke);
-  if (*lds + sizeof(double) * (size_t)(ke | 1) * ke <= 150 * 1024) {
-    *lds += sizeof(double) * (size_t)(ke | 1) * ke;
+  *lds = sizeof(double) * ((size_t)(P.k | 1) * ke + ke);  // A and sigma
+  if (P.k <= 64) {
+    // V, the QR preconditioner's reflectors and tau in LDS as well
+    *lds += sizeof(double) * ((size_t)(ke | 1) * ke + (size_t)P.k * P.k + P.k);
     P.work = nullptr;
   } else if (!P.work) {
-    return hipErrorInvalidValue;
+    return hipErrorInvalidValue;  // V, reflectors and tau in the global work buffer (kSvdWorkDoubles)
   }
-  if (P.k <= 64) *lds += sizeof(double) * ((size_t)P.k * P.k + 64);  // the QR preconditioner
   return hipSuccess;
 }
 
