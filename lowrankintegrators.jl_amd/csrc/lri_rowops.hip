@@ -646,7 +646,10 @@ __host__ __device__ inline RowsPlan rows_plan(int ro, int ri, int rw, bool hasX,
 // this workgroup's index among the nblk of the launch (of its member, batched).
 // The arguments are taken by value: bound to a reference into the kernel's
 // argument segment, the Cholesky variants spill ~470 bytes per lane to scratch.
-template <typename PT, int NRB, int CK = 0>
+// LEAN: fewer loads in flight per thread (2 splits of the partials, 4 X values
+// ahead of the Y staging), so that the body fits 256 registers and two
+// workgroups share a CU; the arithmetic, and so the result, is the same.
+template <typename PT, int NRB, int CK = 0, bool LEAN = false>
 __device__ __forceinline__ void rows_body(const RowArgs<PT> A, int bid, int nblk) {
   extern __shared__ double lds[];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -705,7 +708,7 @@ __device__ __forceinline__ void rows_body(const RowArgs<PT> A, int bid, int nblk
         // columns and of PC splits at a time are issued together (addresses
         // clamped into the array, the extra values discarded), then summed in
         // split order
-        constexpr int PC = (sizeof(PT) == 4 ? 8 : 4) / (NRB >= 16 ? 2 : 1);  // (NRB 16 has no registers to spare)
+        constexpr int PC = LEAN ? 2 : (sizeof(PT) == 4 ? 8 : 4) / (NRB >= 16 ? 2 : 1);  // (NRB 16 has no registers to spare)
         const int64_t sstride = (int64_t)A.rb * A.rows;
         const int cmax = ro - 1;
         for (int sp0 = 0; sp0 < A.nsplit; sp0 += PC) {
@@ -762,12 +765,13 @@ __device__ __forceinline__ void rows_body(const RowArgs<PT> A, int bid, int nblk
         }
       }
     }
-    // the first 16 of this thread's X values, loaded before Y is staged so that
+    // the first XR of this thread's X values, loaded before Y is staged so that
     // both loads are in flight together
-    double xr[16];
+    constexpr int XR = LEAN ? 4 : 16;
+    double xr[XR];
     if (hasX) {
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
+      for (int u = 0; u < XR; ++u) {
         const int d = cg + u * tpr;
         xr[u] = (ok && d < ri) ? A.X[i + d * A.ldx] : 0.0;
       }
@@ -793,11 +797,11 @@ __device__ __forceinline__ void rows_body(const RowArgs<PT> A, int bid, int nblk
       for (int c = 8 * tpr + cg; c < p.ro16; c += tpr) Ot[lrow * p.ldot + c] = 0.0;
       if (hasX) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
+        for (int u = 0; u < XR; ++u) {
           const int d = cg + u * tpr;
           if (d < p.ri4) Xt[lrow * p.ldxt + d] = xr[u];
         }
-        for (int d = cg + 16 * tpr; d < p.ri4; d += tpr) Xt[lrow * p.ldxt + d] = (ok && d < ri) ? A.X[i + d * A.ldx] : 0.0;
+        for (int d = cg + XR * tpr; d < p.ri4; d += tpr) Xt[lrow * p.ldxt + d] = (ok && d < ri) ? A.X[i + d * A.ldx] : 0.0;
       }
       if (hasW && reduce)
         for (int c = cg; c < p.rw16; c += tpr) Wt[lrow * p.ldwt + c] = (ok && c < rw) ? A.W[i + c * A.ldw] : 0.0;
@@ -901,13 +905,16 @@ __global__ __launch_bounds__(kThreads) void k_rows(RowArgs<PT> A) {
   rows_body<PT, NRB, CK>(A, blockIdx.x, gridDim.x);
 }
 
+// The batched form: merged launches cover many narrow members with one tile per
+// workgroup, so latency hiding matters more than loads in flight per thread:
+// the lean body at two workgroups per CU for NRB <= 2 without a prologue.
 template <typename PT, int NRB, int CK>
-__global__ __launch_bounds__(kThreads) void k_rows_batch(const unsigned char* __restrict__ tab,
-                                                         const int* __restrict__ start, int count) {
+__global__ __launch_bounds__(kThreads, (NRB <= 2 && CK == 0) ? 2 : 1) void k_rows_batch(
+    const unsigned char* __restrict__ tab, const int* __restrict__ start, int count) {
   const int b = blockIdx.x;
   const int i = batch_member(start, count, b);
   const RowArgs<PT>& A = *reinterpret_cast<const RowArgs<PT>*>(tab + (size_t)start[count + 1 + i] * kRecArgBytes);
-  rows_body<PT, NRB, CK>(A, b - start[i], start[i + 1] - start[i]);
+  rows_body<PT, NRB, CK, (NRB <= 2 && CK == 0)>(A, b - start[i], start[i + 1] - start[i]);
 }
 
 template <typename PT>
