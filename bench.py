@@ -893,6 +893,7 @@ def main():
     ap.add_argument("--ring", type=int, default=8, help="FP64/FP32 ring length (snapshots)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP64/FP32 forms")
+    ap.add_argument("--no-solve", action="store_true", help="skip the solve() lines beside the headline")
     ap.add_argument("--no-check", action="store_true", help="skip the error check")
     ap.add_argument("--no-tune", action="store_true", help="keep the default pass plans (no autotuning)")
     ap.add_argument("--no-events", action="store_true",
@@ -1075,6 +1076,26 @@ def main():
         out["rel_err_vs_fp64"] = check[0]
         out["fp64_products_on_bf16_steps_per_s"] = check[1]
     ctx.close()
+
+    if not args.no_solve:
+        # the reference's own driver: solve() with update_sol! semantics (primitives.jl:68-90), the
+        # whole call timed (context set-up, the 199 steps, the states copied out at the end)
+        cols = [a.t() for a in snaps]  # column-major n x n views of the row-major A_k' tensors
+        prob = lri.MatrixDataProblem(cols, lri.SVDLikeRepresentation(U0, S0, V0))
+        alg = lri.ProjectorSplitting(lri.PrimalLieTrotter())
+        lri.solve(lri.MatrixDataProblem(cols[:4], lri.SVDLikeRepresentation(U0, S0, V0)), alg,
+                  precision="mixed", save_everystep=False)  # warm-up
+        solve_lines = {}
+        for every in (True, False):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sol = lri.solve(prob, alg, precision="mixed", save_everystep=every)
+            el_s = time.perf_counter() - t0
+            key = "save_everystep" if every else "final_state_only"
+            solve_lines[key] = {"steps_per_s": npairs / el_s, "wall_s": el_s, "states_returned": len(sol.Y)}
+            log(f"solve(): {npairs / el_s:.1f} steps/s with {key} ({el_s:.3f} s for the call)")
+            del sol
+        out["solve"] = solve_lines
 
     if not args.no_forms:
         R = min(args.ring, N)

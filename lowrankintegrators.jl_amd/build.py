@@ -48,13 +48,25 @@ def _run(cmd):
     return out
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(OBJDIR, exist_ok=True)
-    cc = hipcc()
+# The host code again with AddressSanitizer and UBSan (host pass only; the
+# device code is unchanged), as liblri_hip_san.so: tests/test_sanitize_host.py
+# links a C program against it and drives the C ABI's argument validation and
+# error paths on the CPU.  The sanitizer runtime comes from that program.
+LIB_SAN = os.path.join(HERE, "liblri_hip_san.so")
+SAN_FLAGS = [
+    "-Xarch_host", "-fsanitize=address",
+    "-Xarch_host", "-fsanitize=undefined",
+    "-Xarch_host", "-fno-sanitize-recover=all",
+    "-Xarch_host", "-fno-omit-frame-pointer",
+]
+
+
+def _build_variant(cc, flags, objdir, lib, verbose):
+    os.makedirs(objdir, exist_ok=True)
 
     def compile_one(src):
-        obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
-        out = _run([cc, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        out = _run([cc, *flags, "-c", os.path.join(CSRC, src), "-o", obj])
         return obj, out
 
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -62,11 +74,19 @@ def build(verbose: bool = True) -> str:
     for _, out in results:
         if out.strip() and verbose:
             print(out)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *[o for o, _ in results]])
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", LIB)
+        print("built", lib)
+    return lib
+
+
+def build(verbose: bool = True, sanitized: bool = True) -> str:
+    cc = hipcc()
+    _build_variant(cc, FLAGS, OBJDIR, LIB, verbose)
+    if sanitized:
+        _build_variant(cc, FLAGS + SAN_FLAGS, os.path.join(OBJDIR, "san"), LIB_SAN, verbose)
     return LIB
 
 

@@ -3774,6 +3774,12 @@ const char* lri_last_error(const lri_ctx* ctx) {
   return ctx->err.c_str();
 }
 
+namespace {
+bool shape_ok(int64_t n, int64_t m, int64_t rcap) {
+  return n >= 1 && m >= 1 && rcap >= 1 && rcap <= LRI_MAX_RANK && rcap <= n && rcap <= m;
+}
+}  // namespace
+
 int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
   g_create_error.clear();
   if (!out) {
@@ -3781,7 +3787,7 @@ int lri_create(lri_ctx** out, int device, int64_t n, int64_t m, int64_t rcap) {
     return LRI_EINVAL;
   }
   *out = nullptr;
-  if (n < 1 || m < 1 || rcap < 1 || rcap > LRI_MAX_RANK || rcap > n || rcap > m) {
+  if (!shape_ok(n, m, rcap)) {
     g_create_error = "need n, m >= 1 and 1 <= rcap <= min(n, m, LRI_MAX_RANK)";
     return LRI_EINVAL;
   }
@@ -4361,6 +4367,11 @@ int lri_batch_create(lri_batch** out, int device, int64_t count, const int64_t* 
     return LRI_EINVAL;
   }
   *out = nullptr;
+  for (int64_t i = 0; i < count; ++i)  // every member's shape before any device work
+    if (!shape_ok(n[i], m[i], rcap[i])) {
+      g_create_error = "problem " + std::to_string(i) + ": need n, m >= 1 and 1 <= rcap <= min(n, m, LRI_MAX_RANK)";
+      return LRI_EINVAL;
+    }
   lri_batch* b = nullptr;
   try {
     b = new lri_batch();
