@@ -378,6 +378,15 @@ int lri_trajectory_save(lri_ctx* ctx, int64_t slot);
 int lri_trajectory_get(lri_ctx* ctx, int64_t slot, int64_t* r_out, double* U, int64_t ldu, double* S, int64_t lds,
                        double* V, int64_t ldv);
 
+/* Slots slot0 .. slot0+count-1 in one copy (synchronising), in their device
+ * layout: slot i at host + i * (n*rcap + rcap*rcap + m*rcap) holds U (n x rcap,
+ * ld n), S (rcap x rcap, ld rcap) and V (m x rcap, ld m), of which the first
+ * ranks[i] columns (and the leading ranks[i] x ranks[i] block of S) are the
+ * state.  Slots saved in the resident eigenbasis are rejected (LRI_EINVAL; use
+ * lri_trajectory_get).  One transfer instead of three synchronising copies per
+ * state, for DLRSolution.Y after a solve with save_everystep. */
+int lri_trajectory_get_range(lri_ctx* ctx, int64_t slot0, int64_t count, int64_t* ranks, double* host);
+
 /* Wait for all queued work; returns a latched device-side status, if any. */
 int lri_synchronize(lri_ctx* ctx);
 

@@ -85,12 +85,13 @@ mutable struct HIPContext
     n::Int
     m::Int
     rcap::Int
+    resident::Bool             # lri_set_resident_eigenbasis is on
     function HIPContext(n::Integer, m::Integer, rcap::Integer; device::Integer = 0)
         out = Ref{Ptr{Cvoid}}(C_NULL)
         st = ccall((:lri_create, libhip), Cint, (Ref{Ptr{Cvoid}}, Cint, Int64, Int64, Int64),
                    out, device, n, m, rcap)
         check(C_NULL, st)
-        c = new(out[], Int(n), Int(m), Int(rcap))
+        c = new(out[], Int(n), Int(m), Int(rcap), false)
         finalizer(destroy!, c)
         return c
     end
@@ -261,6 +262,8 @@ use_eigenbasis!(c::AnyContext, F::QuadraticOperator) = nothing
 function use_eigenbasis!(c::AnyContext, F::LyapunovOperator)
     (is_constant(F.ad, F.ae) && is_constant(F.bd, F.be)) || return nothing
     check(c.ptr, ccall((:lri_set_resident_eigenbasis, libhip), Cint, (Ptr{Cvoid}, Cint), c.ptr, Cint(1)))
+    c isa HIPContext && (c.resident = true)
+    return nothing
 end
 
 # Gershgorin bound of the operator's linear part, for the default RK4 sub-step count
@@ -511,6 +514,32 @@ function trajectory_get(c::HIPCache, slot::Integer)
 end
 
 """
+    trajectory_get_range(c::HIPCache, slot0, count)
+
+The states of slots `slot0 .. slot0+count-1` from one copy (`lri_trajectory_get_range`):
+U and V are views into one host buffer in the slots' device layout
+(`n*rcap + rcap*rcap + m*rcap` doubles per slot), S a copy of its leading block.
+"""
+function trajectory_get_range(c::HIPCache, slot0::Integer, count::Integer)
+    ctx = c.ctx
+    n, m, rc = ctx.n, ctx.m, ctx.rcap
+    stride = n * rc + rc * rc + m * rc
+    buf = Vector{Float64}(undef, max(count, 1) * stride)
+    ranks = zeros(Int64, max(count, 1))
+    check(ctx.ptr, ccall((:lri_trajectory_get_range, libhip), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Ptr{Float64}), ctx.ptr, slot0, count, ranks, buf))
+    out = Any[]
+    for i in 1:count
+        o, r = (i - 1) * stride, Int(ranks[i])
+        U = reshape(view(buf, o+1:o+n*r), n, r)
+        S = reshape(buf[o+n*rc+1:o+n*rc+rc*rc], rc, rc)[1:r, 1:r]
+        V = reshape(view(buf, o+n*rc+rc*rc+1:o+n*rc+rc*rc+m*r), m, r)
+        push!(out, c.twofactor ? TwoFactorRepresentation(U, V) : SVDLikeRepresentation(U, S, V))
+    end
+    return out
+end
+
+"""
     update_sol!(integrator::DLRIntegrator{<:Any,<:Any,<:HIPAlgorithm})
 
 `update_sol!` (primitives.jl:82-90) with the deep copy queued on the device
@@ -536,8 +565,23 @@ function update_sol!(integrator::DLRIntegrator{<:Any,<:Any,<:HIPAlgorithm})
     end
     t0, tf = c.tspan
     if !((tf - integrator.t) / (tf - t0) > 1e-8)
-        for (idx, slot) in sort!(collect(c.traj))
-            sol.Y[idx] = trajectory_get(c, slot)
+        items = sort!(collect(c.traj))
+        if c.ctx.resident  # slots saved in the eigenbasis are transformed back one at a time
+            for (idx, slot) in items
+                sol.Y[idx] = trajectory_get(c, slot)
+            end
+        else  # runs of consecutive slots, one copy each
+            i = 1
+            while i <= length(items)
+                j = i
+                while j < length(items) && items[j+1][2] == items[j][2] + 1
+                    j += 1
+                end
+                for (t, u) in zip(i:j, trajectory_get_range(c, items[i][2], j - i + 1))
+                    sol.Y[items[t][1]] = u
+                end
+                i = j + 1
+            end
         end
         empty!(c.traj)
         sync_state!(integrator)

@@ -767,9 +767,25 @@ def update_sol(integrator: DLRIntegrator, force: bool = False) -> None:
 def fetch_sol(integrator: DLRIntegrator) -> DLRSolution:
     """Copy the states that ``update_sol`` left in the device trajectory into ``sol.Y``."""
     traj = getattr(integrator, "_traj", None) or {}
-    for k, slot in sorted(traj.items()):
-        U, S, V = integrator.ctx.trajectory_get(slot)
+    ctx = integrator.ctx
+
+    def put(k, U, S, V):
         integrator.sol.Y[k] = TwoFactorRepresentation(U, V) if integrator.twofactor else SVDLikeRepresentation(U, S, V)
+
+    items = sorted(traj.items())
+    if getattr(ctx, "resident", False):
+        # slots saved in the eigenbasis are transformed back one at a time
+        for k, slot in items:
+            put(k, *ctx.trajectory_get(slot))
+    else:
+        i = 0
+        while i < len(items):  # runs of consecutive slots, one copy each
+            j = i + 1
+            while j < len(items) and items[j][1] == items[j - 1][1] + 1:
+                j += 1
+            for (k, _), usv in zip(items[i:j], ctx.trajectory_get_range(items[i][1], j - i)):
+                put(k, *usv)
+            i = j
     traj.clear()
     return integrator.sol
 

@@ -4697,6 +4697,26 @@ int lri_trajectory_get(lri_ctx* ctx, int64_t slot, int64_t* r_out, double* U, in
   });
 }
 
+int lri_trajectory_get_range(lri_ctx* ctx, int64_t slot0, int64_t count, int64_t* ranks, double* host) {
+  return guarded<true>(ctx, [&] {
+    REQUIRE(count >= 0 && slot0 >= 0 && slot0 + count <= ctx->traj_slots,
+            "trajectory slots out of range (lri_trajectory_reserve)");
+    REQUIRE(count == 0 || (ranks && host), "ranks and host are required");
+    const int64_t stride = ctx->traj_stride();
+    for (int64_t i = 0; i < count; ++i) {
+      const size_t k = (size_t)(slot0 + i);
+      REQUIRE(ctx->traj_rank[k] >= 1, "a trajectory slot of the range was never saved");
+      REQUIRE(!ctx->traj_resident[k], "a slot of the range was saved in the resident eigenbasis: use lri_trajectory_get");
+      ranks[i] = ctx->traj_rank[k];
+    }
+    const int st = ctx->read_flags();
+    if (count > 0)
+      HIPCHECK(hipMemcpy(host, ctx->traj + slot0 * stride, sizeof(double) * (size_t)(count * stride),
+                         hipMemcpyDeviceToHost));
+    return st;
+  });
+}
+
 int lri_synchronize(lri_ctx* ctx) {
   return guarded<true>(ctx, [&] { return ctx->read_flags(); });
 }

@@ -83,6 +83,7 @@ SIGNATURES = {
     "lri_trajectory_reserve": (_int, [_vp, _i64]),
     "lri_trajectory_save": (_int, [_vp, _i64]),
     "lri_trajectory_get": (_int, [_vp, _i64, ctypes.POINTER(_i64), _dp, _i64, _dp, _i64, _dp, _i64]),
+    "lri_trajectory_get_range": (_int, [_vp, _i64, _i64, ctypes.POINTER(_i64), _dp]),
     "lri_get_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "lri_kernel_timing": (_int, [_vp, _int]),
     "lri_kernel_timing_read": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i64)]),
@@ -307,6 +308,7 @@ class Context:
     def set_resident_eigenbasis(self, on: bool):
         """Keep the state in the operator's analytic eigenbasis (constant coefficients, linear)."""
         check(lib().lri_set_resident_eigenbasis(self.h, int(bool(on))), self.h)
+        self.resident = bool(on)
 
     def step_hybrid_greedy(self, kn: int, dt: float, substeps: int):
         check(lib().lri_step_hybrid_greedy(self.h, int(kn), float(dt), int(substeps)), self.h)
@@ -431,6 +433,26 @@ class Context:
         check(lib().lri_trajectory_get(self.h, int(slot), ctypes.byref(r), _dptr(U), self.n, _dptr(S), rr, _dptr(V),
                                        self.m), self.h)
         return U, S, V
+
+    def trajectory_get_range(self, slot0: int, count: int):
+        """The states of slots slot0 .. slot0+count-1 in one copy: a list of (U, S, V),
+        U and V views into one host buffer (their slots' leading columns), S a copy of the
+        leading block."""
+        n, m = self.n, self.m
+        rc = self.rcap
+        stride = n * rc + rc * rc + m * rc
+        buf = np.empty(max(count, 1) * stride)
+        ranks = np.zeros(max(count, 1), dtype=np.int64)
+        check(lib().lri_trajectory_get_range(self.h, int(slot0), int(count),
+                                             ranks.ctypes.data_as(ctypes.POINTER(_i64)), _dptr(buf)), self.h)
+        out = []
+        for i in range(count):
+            o, r = i * stride, int(ranks[i])
+            U = buf[o:o + n * r].reshape((n, r), order="F")
+            S = np.array(buf[o + n * rc:o + n * rc + rc * rc].reshape((rc, rc), order="F")[:r, :r], order="F")
+            V = buf[o + n * rc + rc * rc:o + n * rc + rc * rc + m * r].reshape((m, r), order="F")
+            out.append((U, S, V))
+        return out
 
     def timing_sample(self, every: int):
         """Time one launch in ``every`` of each family."""

@@ -214,6 +214,12 @@ def test_resident_eigenbasis_solve(lri):
                         ctx.step_de_bug_ra(dt, N, 1e-6, 32)
                     ctx.trajectory_save(k)
                 got[resident] = (ctx.get_state(), ctx.trajectory_get(steps // 2))
+                if resident:  # the one-copy range refuses slots saved in the eigenbasis
+                    with pytest.raises(ValueError):
+                        ctx.trajectory_get_range(0, 2)
+                else:
+                    rng = ctx.trajectory_get_range(steps // 2, 1)[0]
+                    assert all(np.array_equal(a, b) for a, b in zip(rng, got[False][1]))
                 if resident:  # a data-path call leaves the eigenbasis; the state is unchanged
                     ctx.set_resident_eigenbasis(False)
                     assert rel_err(*ctx.get_state(), *got[True][0]) <= 1e-13

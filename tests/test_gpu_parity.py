@@ -450,8 +450,18 @@ def test_device_trajectory_roundtrip(lri):
         for k, w in enumerate(want):
             got = ctx.trajectory_get(k)
             assert all(np.array_equal(a, b) for a, b in zip(got, w)), k
+        # lri_trajectory_get_range: one copy of a run of slots, the same bits
+        for slot0, count in ((0, 3), (1, 2), (2, 1), (0, 0)):
+            got = ctx.trajectory_get_range(slot0, count)
+            assert len(got) == count
+            for k, g in enumerate(got):
+                assert all(np.array_equal(a, b) for a, b in zip(g, want[slot0 + k])), (slot0, k)
         with pytest.raises(ValueError):
             ctx.trajectory_get(3)
+        with pytest.raises(ValueError):
+            ctx.trajectory_get_range(2, 2)  # slot 3 never saved
+        with pytest.raises(ValueError):
+            ctx.trajectory_get_range(3, 2)  # past the reservation
 
 
 # ------------------------------------------------------------------ mixed precision
