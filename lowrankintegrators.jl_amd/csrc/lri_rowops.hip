@@ -205,11 +205,192 @@ __device__ int chol_inv_waves(const double* __restrict__ G, double* __restrict__
   return nd;
 }
 
+// The same factorisation (same scaling, drop rule and outputs) for k <= K <= 32
+// on one wave, blocked by four pivots, with the trailing updates on the FP64
+// matrix cores.  The whole scaled G and the running B = L^{-1} stay in
+// registers in the v_mfma_f64_16x16x4_f64 result layout: 16 x 16 tiles (T1, T2),
+// lane l and item u holding row 16 T1 + 4 u + l / 16, column 16 T2 + l % 16.
+// Since G is kept symmetric, the panel of pivot block b (columns 4b .. 4b+3)
+// in the operand layout of that instruction (lane l: row 16 t + l % 16, column
+// 4b + l / 16) is item b % 4 of tile (b / 4, t) at the same lane: no data moves
+// to form it.  Per block:
+//  * the 4 x 4 diagonal block D is read by readlane (lane 16 q + 4 (b % 4) + a
+//    holds D(a, q)) and factored in uniform registers with the drop rule
+//    (keep_q = pivot > tau; inv_q = 1/sqrt(pivot)), giving the multipliers
+//    Lb(a, q) and the upper triangular M with L(i, 4b + q) = sum_p A(i, 4b + p) M(p, q):
+//    M(:, q) = keep_q inv_q (e_q - sum_{p<q} M(:, p) Lb(q, p));
+//  * the panel L = A M takes four cross-lane reads per tile (ds_bpermute);
+//  * B's block row becomes M' times itself (rows 4b + a live in item b % 4 of
+//    tiles (b / 4, .) at lanes with l / 16 = a);
+//  * G -= P P' and B -= P B(block row) on the matrix cores, P the panel with
+//    its rows <= 4b + 3 zeroed (finished rows and columns stay untouched).
+// The four pivots of a block form the only sequential chain: ~1,000 cycles per
+// block against ~770 per pivot for the row-wise forms.  All threads call it;
+// the first wave computes, the others wait at the final barrier.
+template <int K>
+__device__ int chol_inv_mfma(const double* __restrict__ G, double* __restrict__ R, double* __restrict__ Rinv, int k,
+                             unsigned char* dropped, int* nonfinite) {
+  static_assert(K == 16 || K == 32, "K: 16 or 32");
+  constexpr int NT = K / 16, NB = K / 4;
+  const int t = threadIdx.x, j = t & 63;
+  if (t < 64) {
+    const int lr = j & 15, lk = j >> 4;
+    double md = j < k ? G[j + j * k] : 0.0;
+    int nf0 = (j < k) && !isfinite(md);
+    for (int off = 32; off > 0; off >>= 1) {
+      md = fmax(md, __shfl_xor(md, off));
+      nf0 |= __shfl_xor(nf0, off);
+    }
+    const int e2 = chol_scale_exp(md), eh = e2 / 2;
+    const double tau = 10.0 * kEps * ldexp(md, -e2);
+    f64x4_t g[NT][NT], x[NT][NT];
+#pragma unroll
+    for (int T1 = 0; T1 < NT; ++T1)
+#pragma unroll
+      for (int T2 = 0; T2 < NT; ++T2)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int row = 16 * T1 + 4 * u + lk, col = 16 * T2 + lr;
+          const bool in = row < k && col < k;
+          const int lo = row < col ? row : col, hi = row < col ? col : row;  // the upper triangle
+          const double v = G[in ? lo + hi * k : 0];
+          g[T1][T2][u] = in ? ldexp(v, -e2) : 0.0;
+          x[T1][T2][u] = row == col ? 1.0 : 0.0;
+        }
+    // one-hot weights of this lane's block column q = l / 16 (selects of uniform
+    // values by lane, as multiply-adds: lane-dependent selects compile to branches)
+    double w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = lk == q ? 1.0 : 0.0;
+    // cross-lane read addresses: lane 16 p + l % 16
+    int src[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) src[p] = (16 * p + lr) << 2;
+    auto bperm = [](int addr, double v) {
+      const uint64_t u = __builtin_bit_cast(uint64_t, v);
+      const uint32_t lo = __builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)u);
+      const uint32_t hi = __builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(u >> 32));
+      return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+    };
+    // pivots past k are zero and count as dropped: no mask is needed in the sweep
+    uint32_t dm = 0;
+    int nfv = 0;
+    double rR[NB][NT];
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) {
+      const int T = kb >> 2, u0 = kb & 3;
+      double P[NT];
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt) P[tt] = g[T][tt][u0];
+      double W[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q <= a; ++q) W[a][q] = readlane_f64(P[T], 16 * q + 4 * u0 + a);
+      // the block's four pivots; a dropped pivot gets inv = 0, which zeroes its
+      // multipliers and its column of M
+      double Lb[4][4], inv[4], dq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double d = W[q][q];
+        const bool keep = d > tau && d <= 1.7976931348623157e308;
+        nfv |= !(fabs(d) <= 1.7976931348623157e308);
+        dm |= keep ? 0u : 1u << (4 * kb + q);
+        inv[q] = keep ? rsqrt_nr(d) : 0.0;
+        dq[q] = keep ? 0.0 : 1.0;
+#pragma unroll
+        for (int a = q + 1; a < 4; ++a) Lb[a][q] = W[a][q] * inv[q];
+#pragma unroll
+        for (int a = q + 1; a < 4; ++a)
+#pragma unroll
+          for (int b = q + 1; b <= a; ++b) W[a][b] = __builtin_fma(-Lb[a][q], Lb[b][q], W[a][b]);
+      }
+      double M[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double c[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] = i == q ? 1.0 : 0.0;
+#pragma unroll
+        for (int p = 0; p < q; ++p)
+#pragma unroll
+          for (int i = 0; i <= p; ++i) c[i] = __builtin_fma(-M[i][p], Lb[q][p], c[i]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) M[i][q] = i <= q ? c[i] * inv[q] : 0.0;
+      }
+      double mq[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        double v = M[p][3] * w[3];
+#pragma unroll
+        for (int q = 2; q >= p; --q) v = __builtin_fma(M[p][q], w[q], v);
+        mq[p] = v;
+      }
+      const double drq = __builtin_fma(dq[0], w[0], __builtin_fma(dq[1], w[1], __builtin_fma(dq[2], w[2], dq[3] * w[3])));
+      double Pm[NT];
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt) {
+        double acc = bperm(src[0], P[tt]) * mq[0];
+#pragma unroll
+        for (int p = 1; p < 4; ++p) acc = __builtin_fma(bperm(src[p], P[tt]), mq[p], acc);
+        // R(p, row) = L(row, p), p = 4 kb + l / 16: the panel at and below the
+        // block's diagonal (1 on the diagonal of a dropped pivot), zero above
+        const int row = 16 * tt + lr, a = row - 4 * kb;
+        const double rv = a > lk ? acc : 0.0;
+        rR[kb][tt] = a == lk ? acc + drq : rv;
+        Pm[tt] = row >= 4 * kb + 4 ? acc : 0.0;
+      }
+      double xn[NT];
+#pragma unroll
+      for (int t2 = 0; t2 < NT; ++t2) {
+        double acc = bperm(src[0], x[T][t2][u0]) * mq[0];
+#pragma unroll
+        for (int b = 1; b < 4; ++b) acc = __builtin_fma(bperm(src[b], x[T][t2][u0]), mq[b], acc);
+        xn[t2] = acc;
+      }
+#pragma unroll
+      for (int t2 = 0; t2 < NT; ++t2) x[T][t2][u0] = xn[t2];
+#pragma unroll
+      for (int t1 = T; t1 < NT; ++t1) {
+#pragma unroll
+        for (int t2 = T; t2 < NT; ++t2)
+          g[t1][t2] = __builtin_amdgcn_mfma_f64_16x16x4f64(-Pm[t1], Pm[t2], g[t1][t2], 0, 0, 0);
+#pragma unroll
+        for (int t2 = 0; t2 <= T; ++t2)
+          x[t1][t2] = __builtin_amdgcn_mfma_f64_16x16x4f64(-Pm[t1], xn[t2], x[t1][t2], 0, 0, 0);
+      }
+    }
+    // R unscaled; R^{-1} = B' unscaled, the rows of dropped columns zero
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb)
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt) {
+        const int p = 4 * kb + lk, row = 16 * tt + lr;
+        if (p < k && row < k) R[p + row * k] = ldexp(rR[kb][tt], eh);
+      }
+#pragma unroll
+    for (int T1 = 0; T1 < NT; ++T1)
+#pragma unroll
+      for (int T2 = 0; T2 < NT; ++T2)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int c = 16 * T1 + 4 * u + lk, jj = 16 * T2 + lr;
+          if (c < k && jj < k) Rinv[jj + c * k] = ((dm >> jj) & 1) ? 0.0 : ldexp(x[T1][T2][u], -eh);
+        }
+    if (j < k) dropped[j] = (dm >> j) & 1;
+    // a non-finite pivot of a row past k cannot occur (those rows are zero)
+    if (t == 0 && (nfv | nf0)) *nonfinite = 1;
+  }
+  __syncthreads();
+  return __popcll(__ballot(j < k && dropped[j]));
+}
+
 // CK: the Cholesky variant compiled into an epilogue kernel (16, 32, 64: the
-// bound on k), all in the four-wave form.
+// bound on k): the one-wave matrix-core form up to 32, the four-wave form above.
 template <int CK>
 __device__ int chol_inv(const double* G, double* R, double* Rinv, int k, unsigned char* dropped, int* nonfinite) {
-  return chol_inv_waves<CK>(G, R, Rinv, k, dropped, nonfinite);
+  if constexpr (CK <= 32) return chol_inv_mfma<CK>(G, R, Rinv, k, dropped, nonfinite);
+  else return chol_inv_waves<CK>(G, R, Rinv, k, dropped, nonfinite);
 }
 
 // C = op(A) * op(B), k x k, column-major, on the FP64 matrix cores (16 x 16
