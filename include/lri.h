@@ -309,6 +309,22 @@ int lri_step_de_bug_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* U
 int lri_step_de_ksl_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double* Ub, int64_t ldub, double* Sb,
                             int64_t ldsb, double* Vb, int64_t ldvb, double* scale_bar);
 
+/* Reverse of lri_step_de_bug_ra (rank_adaptive_unconventional.jl:194-233 with the
+ * sub-problem right-hand sides of :59-86) with the new rank r_new held fixed (the
+ * rank the forward step returned; the recomputed step must reproduce it, else
+ * LRI_EINVAL): on entry the first r_new columns of Ub (n rows) and Vb (m rows)
+ * and the r_new x r_new Sb hold the cotangents of the result (U1, S1 = diag(sigma),
+ * V1) -- only Sb's diagonal enters -- on exit the first r columns and the r x r
+ * Sb hold those of (U0, S0, V0); the arrays hold max(r, r_new) columns.
+ * scale_bar as for lri_step_de_bug_adjoint.  The step is recomputed up to the
+ * SVD of the 2r x 2r core (the K and L sub-step starts kept), then the SVD rule
+ * (distinct singular values), the RK4 reverse of the 2r S sub-step, the
+ * augmented bases' QR rule and the K and L reverses.  Needs the quadratic
+ * operator and 2 r <= LRI_QUAD_MAX_RANK.  The state is left unchanged. */
+int lri_step_de_bug_ra_adjoint(lri_ctx* ctx, double dt, int64_t substeps, double tol, int64_t rmax, int64_t r_new,
+                               double* Ub, int64_t ldub, double* Sb, int64_t ldsb, double* Vb, int64_t ldvb,
+                               double* scale_bar);
+
 /* ---- batches of independent problems -------------------------------------
  *
  * Many small solves of uneven n, m and rank (the reference runs them one
@@ -428,11 +444,12 @@ int lri_kernel_timing_samples(lri_ctx* ctx, int family, float* ms, int64_t cap, 
  * result per chunk; the default chunk aims at about 512 workgroups.  This
  * times every candidate chunk (those for about 128 .. 3072 workgroups) of
  * pass N and pass T over dA = A[kn] - A[kp] (kp < 0: A[kn]) at the context's
- * current rank and precision, `reps` launches each after an untimed one, and
- * keeps the fastest for this context's passes of that shape (the partial
- * buffers grow if a candidate needs it).  report (8 doubles): for pass N then
- * pass T, the chosen chunk, its split count, its mean time (us) and the mean
- * time of the default chunk (us).  A chosen chunk changes the order in which
+ * current rank and precision, in three interleaved rounds of `reps` launches
+ * each (after an untimed one), and keeps the candidate with the lowest median
+ * for this context's passes of that shape; the default chunk stays unless the
+ * best is at least 1 % faster (the partial buffers grow if a candidate needs
+ * it).  report (8 doubles): for pass N then pass T, the chosen chunk, its split
+ * count, its median time per launch (us) and that of the default chunk (us).  A chosen chunk changes the order in which
  * the partials are summed, so results move within the precision's rounding.
  * No counterpart in the reference (its products are BLAS calls). */
 int lri_autotune_passes(lri_ctx* ctx, int64_t kp, int64_t kn, int64_t reps, double* report);

@@ -708,17 +708,20 @@ end
 
 Gradient of `loss_grad(U, S, V) -> (value, Ub, Sb, Vb)` at the final state of
 `solve(prob, alg, dt)` for a matrix ODE with a `QuadraticOperator` (D = 0 for a
-linear one) and `UnconventionalAlgorithm()` or `ProjectorSplitting(PrimalLieTrotter())`:
+linear one) and `UnconventionalAlgorithm()`, `RankAdaptiveUnconventionalAlgorithm`
+(the ranks of the forward solve held fixed) or `ProjectorSplitting(PrimalLieTrotter())`:
 returns `(value, (Ub0, Sb0, Vb0), (abar, bbar))`, abar and bbar the derivatives
 along the scales of the operator's A and B (A = ν Laplacian: dL/dν = abar / ν),
-through `lri_step_de_bug_adjoint` / `lri_step_de_ksl_adjoint` with checkpointed
-recomputation.
+through `lri_step_de_bug_adjoint` / `lri_step_de_bug_ra_adjoint` /
+`lri_step_de_ksl_adjoint` with checkpointed recomputation.
 """
 function solve_gradient(prob::MatrixDEProblem, alg::HIPAlgorithm, dt, loss_grad; checkpoint::Integer = 20)
     inner = alg.alg
     ksl = inner isa ProjectorSplitting{PrimalLieTrotter}
-    (ksl || inner isa UnconventionalAlgorithm) ||
-        throw(ArgumentError("the matrix-ODE adjoint is built for UnconventionalAlgorithm and primal KSL"))
+    ra = inner isa RankAdaptiveUnconventionalAlgorithm
+    (ksl || ra || inner isa UnconventionalAlgorithm) ||
+        throw(ArgumentError("the matrix-ODE adjoint is built for UnconventionalAlgorithm, " *
+                            "RankAdaptiveUnconventionalAlgorithm and primal KSL"))
     prob.f isa QuadraticOperator || throw(ArgumentError("the matrix-ODE adjoint takes a QuadraticOperator"))
     c = alg_cache(prob, alg, deepcopy(prob.u0), dt)
     ctx, N = c.ctx, c.substeps
@@ -728,9 +731,20 @@ function solve_gradient(prob::MatrixDEProblem, alg::HIPAlgorithm, dt, loss_grad;
         t += dt
         steps += 1
     end
-    fwd!() = ksl ? check(ctx.ptr, ccall((:lri_step_de_ksl, libhip), Cint, (Ptr{Cvoid}, Cint, Cdouble, Int64),
-                                        ctx.ptr, LRI_PRIMAL, dt, N)) :
-                   check(ctx.ptr, ccall((:lri_step_de_bug, libhip), Cint, (Ptr{Cvoid}, Cdouble, Int64), ctx.ptr, dt, N))
+    function fwd!()
+        if ksl
+            check(ctx.ptr, ccall((:lri_step_de_ksl, libhip), Cint, (Ptr{Cvoid}, Cint, Cdouble, Int64),
+                                 ctx.ptr, LRI_PRIMAL, dt, N))
+        elseif ra
+            rn = Ref{Int64}(0)
+            check(ctx.ptr, ccall((:lri_step_de_bug_ra, libhip), Cint,
+                                 (Ptr{Cvoid}, Cdouble, Int64, Cdouble, Int64, Ref{Int64}),
+                                 ctx.ptr, dt, N, inner.alg_params.tol, inner.alg_params.r_max, rn))
+        else
+            check(ctx.ptr, ccall((:lri_step_de_bug, libhip), Cint, (Ptr{Cvoid}, Cdouble, Int64), ctx.ptr, dt, N))
+        end
+        return nothing
+    end
     cps = Dict(0 => get_factors(ctx))
     for s in 0:steps-1
         fwd!()
@@ -749,7 +763,20 @@ function solve_gradient(prob::MatrixDEProblem, alg::HIPAlgorithm, dt, loss_grad;
         end
         for s in c1-1:-1:c0
             set_state!(ctx, states[s-c0+1]...)
-            if ksl
+            if ra  # the new rank r_new is the cotangents' width; the arrays hold max(r, r_new) columns
+                r, rn = rank_of(ctx), size(Ub, 2)
+                w = max(r, rn)
+                Ubw, Sbw, Vbw = zeros(ctx.n, w), zeros(w, w), zeros(ctx.m, w)
+                Ubw[:, 1:rn] .= Ub
+                Sbw[1:rn, 1:rn] .= Sb
+                Vbw[:, 1:rn] .= Vb
+                GC.@preserve Ubw Sbw Vbw scale check(ctx.ptr, ccall((:lri_step_de_bug_ra_adjoint, libhip), Cint,
+                    (Ptr{Cvoid}, Cdouble, Int64, Cdouble, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64,
+                     Ptr{Float64}, Int64, Ptr{Float64}),
+                    ctx.ptr, dt, N, inner.alg_params.tol, inner.alg_params.r_max, rn, Ubw, ctx.n, Sbw, w, Vbw, ctx.m,
+                    scale))
+                Ub, Sb, Vb = Ubw[:, 1:r], Sbw[1:r, 1:r], Vbw[:, 1:r]
+            elseif ksl
                 GC.@preserve Ub Sb Vb scale check(ctx.ptr, ccall((:lri_step_de_ksl_adjoint, libhip), Cint,
                     (Ptr{Cvoid}, Cdouble, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64,
                      Ptr{Float64}), ctx.ptr, dt, N, Ub, size(Ub, 1), Sb, size(Sb, 1), Vb, size(Vb, 1), scale))

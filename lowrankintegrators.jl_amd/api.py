@@ -912,8 +912,10 @@ def solve_gradient(prob, alg, loss_grad, dt=None, *, checkpoint: int = 20, preci
       ``lri_step_bug_adjoint`` / ``lri_step_bug_ra_adjoint`` / ``lri_step_ksl_adjoint`` /
       ``lri_step_ksl_dual_adjoint``; returns ``(value, (Ub0, Sb0, Vb0))``;
     - a matrix ODE (``MatrixDEProblem`` with a ``QuadraticOperator``; D = 0 for a linear
-      one) and ``UnconventionalAlgorithm()`` or ``ProjectorSplitting(PrimalLieTrotter())``:
-      ``lri_step_de_bug_adjoint`` / ``lri_step_de_ksl_adjoint``, with ``dt`` and
+      one) and ``UnconventionalAlgorithm()``, ``RankAdaptiveUnconventionalAlgorithm(tol; rmax)``
+      (2 rmax <= LRI_QUAD_MAX_RANK; the ranks of the forward solve held fixed) or
+      ``ProjectorSplitting(PrimalLieTrotter())``:
+      ``lri_step_de_bug_adjoint`` / ``lri_step_de_bug_ra_adjoint`` / ``lri_step_de_ksl_adjoint``, with ``dt`` and
       the RK4 ``substeps`` as ``solve``; returns ``(value, (Ub0, Sb0, Vb0), (abar, bbar))``,
       abar and bbar the derivatives along scales of the operator's A and B
       (A = nu Laplacian: dL/dnu = abar / nu).
@@ -1004,11 +1006,19 @@ def de_substeps(prob, dt, substeps: Optional[int] = None) -> int:
 
 def _solve_gradient_de(prob, alg, loss_grad, dt, checkpoint, substeps, device):
     ksl = isinstance(alg, ProjectorSplitting) and isinstance(alg.order, PrimalLieTrotter)
-    if not (ksl or isinstance(alg, UnconventionalAlgorithm)):
-        raise TypeError("the matrix-ODE adjoint is built for the unconventional (BUG) and primal KSL integrators")
-    fwd = (lambda c: c.step_de_ksl(native.PRIMAL, dt, N)) if ksl else (lambda c: c.step_de_bug(dt, N))  # noqa: E731
-    rev = (lambda c, *x: c.step_de_ksl_adjoint(dt, N, *x)) if ksl else (  # noqa: E731
-        lambda c, *x: c.step_de_bug_adjoint(dt, N, *x))
+    ra = isinstance(alg, RankAdaptiveUnconventionalAlgorithm)
+    if not (ksl or ra or isinstance(alg, UnconventionalAlgorithm)):
+        raise TypeError("the matrix-ODE adjoint is built for the unconventional (BUG), rank-adaptive BUG and "
+                        "primal KSL integrators")
+    if ksl:
+        fwd = lambda c: c.step_de_ksl(native.PRIMAL, dt, N)  # noqa: E731
+        rev = lambda c, *x: c.step_de_ksl_adjoint(dt, N, *x)  # noqa: E731
+    elif ra:  # the ranks of the forward solve held fixed
+        fwd = lambda c: c.step_de_bug_ra(dt, N, alg.tol, alg.rmax)  # noqa: E731
+        rev = lambda c, *x: c.step_de_bug_ra_adjoint(dt, N, alg.tol, alg.rmax, *x)  # noqa: E731
+    else:
+        fwd = lambda c: c.step_de_bug(dt, N)  # noqa: E731
+        rev = lambda c, *x: c.step_de_bug_adjoint(dt, N, *x)  # noqa: E731
     if not isinstance(prob.f, QuadraticOperator):
         raise TypeError("the matrix-ODE adjoint takes a QuadraticOperator (D = 0 for a linear right-hand side)")
     if dt is None:

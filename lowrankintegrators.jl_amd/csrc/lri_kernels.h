@@ -562,11 +562,12 @@ struct BasisVjpArgs {
 };
 hipError_t launch_basis_vjp(const BasisVjpArgs& Q, hipStream_t st);
 // out[0] = sumK[len-1] + <Gab0, Ga0> + <Gab1, Ga1>, out[1] = sumL[len-1] + <Gbb0, Gb0> + <Gbb1, Gb1>
-// (the derivatives along the scales of the two linear parts; k x k, ld k)
+// (the derivatives along the scales of the two linear parts; the first pair of
+// each sum k x k, the second k1 x k1, ld k and k1)
 hipError_t launch_quad_adj_scales(const double* sumK, const double* sumL, int len, const double* Gab0,
                                   const double* Ga0, const double* Gab1, const double* Ga1, const double* Gbb0,
-                                  const double* Gb0, const double* Gbb1, const double* Gb1, int k, double* out,
-                                  hipStream_t st);
+                                  const double* Gb0, const double* Gbb1, const double* Gb1, int k, int k1,
+                                  double* out, hipStream_t st);
 
 // Analytic eigenbasis of a constant symmetric tridiagonal d I + e (S + S')
 // (Dirichlet, or periodic with n >= 3): Q n x n column-major, lam n.

@@ -647,13 +647,18 @@ hipError_t launch_basis_vjp(const BasisVjpArgs& Q, hipStream_t st) {
 __global__ __launch_bounds__(256) void k_quad_adj_scales(const double* sumK, const double* sumL, int len,
                                                          const double* Gab0, const double* Ga0, const double* Gab1,
                                                          const double* Ga1, const double* Gbb0, const double* Gb0,
-                                                         const double* Gbb1, const double* Gb1, int k, double* out) {
+                                                         const double* Gbb1, const double* Gb1, int k, int k1,
+                                                         double* out) {
   __shared__ double ra[256], rb[256];
   const int t = threadIdx.x;
   double a = 0.0, b = 0.0;
   for (int e = t; e < k * k; e += blockDim.x) {
-    a += Gab0[e] * Ga0[e] + Gab1[e] * Ga1[e];
-    b += Gbb0[e] * Gb0[e] + Gbb1[e] * Gb1[e];
+    a += Gab0[e] * Ga0[e];
+    b += Gbb0[e] * Gb0[e];
+  }
+  for (int e = t; e < k1 * k1; e += blockDim.x) {
+    a += Gab1[e] * Ga1[e];
+    b += Gbb1[e] * Gb1[e];
   }
   ra[t] = a;
   rb[t] = b;
@@ -673,11 +678,11 @@ __global__ __launch_bounds__(256) void k_quad_adj_scales(const double* sumK, con
 
 hipError_t launch_quad_adj_scales(const double* sumK, const double* sumL, int len, const double* Gab0,
                                   const double* Ga0, const double* Gab1, const double* Ga1, const double* Gbb0,
-                                  const double* Gb0, const double* Gbb1, const double* Gb1, int k, double* out,
-                                  hipStream_t st) {
-  if (k < 1 || k > kQuadMaxRank || len < 1) return hipErrorInvalidValue;
+                                  const double* Gb0, const double* Gbb1, const double* Gb1, int k, int k1,
+                                  double* out, hipStream_t st) {
+  if (k < 1 || k > kQuadMaxRank || k1 < 1 || k1 > kQuadMaxRank || len < 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_quad_adj_scales, dim3(1), dim3(256), 0, st, sumK, sumL, len, Gab0, Ga0, Gab1, Ga1, Gbb0, Gb0,
-                     Gbb1, Gb1, k, out);
+                     Gbb1, Gb1, k, k1, out);
   return hipGetLastError();
 }
 

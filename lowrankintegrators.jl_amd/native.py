@@ -65,6 +65,8 @@ SIGNATURES = {
     "lri_step_de_bug": (_int, [_vp, ctypes.c_double, _i64]),
     "lri_step_de_bug_ra": (_int, [_vp, ctypes.c_double, _i64, ctypes.c_double, _i64, ctypes.POINTER(_i64)]),
     "lri_step_de_bug_adjoint": (_int, [_vp, ctypes.c_double, _i64, _dp, _i64, _dp, _i64, _dp, _i64, _dp]),
+    "lri_step_de_bug_ra_adjoint": (_int, [_vp, ctypes.c_double, _i64, ctypes.c_double, _i64, _i64, _dp, _i64, _dp,
+                                          _i64, _dp, _i64, _dp]),
     "lri_step_de_ksl_adjoint": (_int, [_vp, ctypes.c_double, _i64, _dp, _i64, _dp, _i64, _dp, _i64, _dp]),
     "lri_batch_create": (_int, [ctypes.POINTER(_vp), _int, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i64)]),
@@ -378,6 +380,25 @@ class Context:
         """The same for the primal KSL matrix-ODE step."""
         return self._de_adjoint(lib().lri_step_de_ksl_adjoint, dt, substeps, Ub, Sb, Vb)
 
+    def step_de_bug_ra_adjoint(self, dt: float, substeps: int, tol: float, rmax: int, Ub, Sb, Vb):
+        """Cotangents of the rank-adaptive matrix-ODE step's inputs (rank r) from those of its
+        result (rank r1 = Ub.shape[1], the rank the forward step returned), and the
+        derivatives (a, b) along the scales of the operator's linear parts."""
+        n, r1 = Ub.shape
+        m = Vb.shape[0]
+        r = self.rank()
+        w = max(r, r1)
+        U = np.zeros((n, w), order="F")
+        V = np.zeros((m, w), order="F")
+        S = np.zeros((w, w), order="F")
+        U[:, :r1] = Ub
+        V[:, :r1] = Vb
+        S[:r1, :r1] = Sb
+        sc = np.zeros(2)
+        check(lib().lri_step_de_bug_ra_adjoint(self.h, float(dt), int(substeps), float(tol), int(rmax), int(r1),
+                                               _dptr(U), n, _dptr(S), w, _dptr(V), m, _dptr(sc)), self.h)
+        return U[:, :r].copy(), S[:r, :r].copy(), V[:, :r].copy(), (float(sc[0]), float(sc[1]))
+
     def _de_adjoint(self, fn, dt, substeps, Ub, Sb, Vb):
         Ub = np.asfortranarray(Ub, dtype=np.float64).copy(order="F")
         Sb = np.asfortranarray(Sb, dtype=np.float64).copy(order="F")
@@ -470,7 +491,8 @@ class Context:
 
     def autotune_passes(self, kp: int, kn: int, reps: int = 20) -> dict:
         """Time the candidate reduction chunks of pass N and pass T over A[kn] - A[kp] at the
-        current rank and precision and keep the fastest (lri_autotune_passes)."""
+        current rank and precision (three interleaved rounds, the median per candidate) and
+        keep the fastest, the default unless beaten by 1 % (lri_autotune_passes)."""
         rep = np.zeros(8, np.float64)
         check(lib().lri_autotune_passes(self.h, int(kp), int(kn), int(reps),
                                         rep.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), self.h)

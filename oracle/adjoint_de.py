@@ -225,6 +225,115 @@ def bug_de_step_vjp(F: ScaledQuadratic, U0, S0, V0, dt, N, Ub1, Sb1, Vb1):
     return Ub0, Sb0, Vb0, acc["a"], acc["b"]
 
 
+def ra_bug_de_step(F: ScaledQuadratic, U0, S0, V0, dt, N, tol, rmax):
+    """The rank-adaptive matrix-ODE step (rank_adaptive_unconventional.jl:194-233 with
+    the sub-problems of :59-86) in the form its reverse differentiates: the augmented
+    bases Uhat = [U0 | Q((I - U0 U0') K1)], Vhat likewise (the same spans as the
+    reference's Q([K1 U0]), so U1 S1 V1' is the same), the S sub-step from
+    M S0 N' (M = Uhat'U0, N = Vhat'V0), then the SVD truncated to
+    min(rmax, the tolerance rank).  Returns (U1, S1, V1, r1)."""
+    from .adjoint import _complete
+    from .lra import truncate_to_tolerance
+
+    h = dt / N
+    fK, fL = _rhs(F, U0, V0)
+    K1 = rk4_states(fK, U0 @ S0, h, N)[-1]
+    L1 = rk4_states(fL, V0 @ S0.T, h, N)[-1]
+    Uh = np.hstack([U0, _complete(U0, K1)[0]])
+    Vh = np.hstack([V0, _complete(V0, L1)[0]])
+    M, Nn = Uh.T @ U0, Vh.T @ V0
+    fS = lambda S: Uh.T @ F(Uh @ S @ Vh.T) @ Vh  # noqa: E731
+    Us, sig, Vst = np.linalg.svd(rk4_states(fS, M @ S0 @ Nn.T, h, N)[-1])
+    r1 = int(min(rmax, truncate_to_tolerance(sig, tol)))
+    return Uh @ Us[:, :r1], np.diag(sig[:r1]), Vh @ Vst[:r1].T, r1
+
+
+def ra_bug_de_step_vjp(F: ScaledQuadratic, U0, S0, V0, dt, N, tol, rmax, Ub1, Sb1, Vb1):
+    """Cotangents (Ub0, Sb0, Vb0, abar, bbar) of ``ra_bug_de_step``'s inputs and of the
+    scales a, b of F, given those of its result (U1 n x r1, S1 = diag(sigma[:r1]),
+    V1 m x r1; only the diagonal of Sb1 enters), the rank r1 held fixed; also r1.
+    The reverse of the data step's truncation and completions (``adjoint.ra_bug_step_vjp``)
+    around the RK4 reverses of ``bug_de_step_vjp``, with the S sub-step in 2r:
+      Uhbar = Ub1 Us[:, :r1]' + (S sub-step cotangent of Uhat) + U0 Mb',
+      Sibar from the SVD rule through the S reverse, Mb = Sibar N S0', Nb = Sibar' M S0,
+      Ub0 = Uhbar[:, :r] + Uhat Mb + (completion reverse) + Kbar0 S0' + (L sub-step cotangent of U0),
+      Sb0 = M' Sibar N + U0'Kbar0 + Lbar0'V0."""
+    from .adjoint import _complete, svd_vjp
+    from .lra import truncate_to_tolerance
+
+    h = dt / N
+    r = U0.shape[1]
+    fK, fL = _rhs(F, U0, V0)
+    Ks = rk4_states(fK, U0 @ S0, h, N)
+    Ls = rk4_states(fL, V0 @ S0.T, h, N)
+    K1, L1 = Ks[-1], Ls[-1]
+    Qu, Ru, _ = _complete(U0, K1)
+    Qv, Rv, _ = _complete(V0, L1)
+    Uh = np.hstack([U0, Qu])
+    Vh = np.hstack([V0, Qv])
+    M, Nn = Uh.T @ U0, Vh.T @ V0
+    fS = lambda S: Uh.T @ F(Uh @ S @ Vh.T) @ Vh  # noqa: E731
+    Ss = rk4_states(fS, M @ S0 @ Nn.T, h, N)
+    Us, sig, Vst = np.linalg.svd(Ss[-1])
+    Vs = Vst.T
+    r1 = int(min(rmax, truncate_to_tolerance(sig, tol)))
+    assert Ub1.shape[1] == r1, (Ub1.shape, r1)
+    acc = {"Uh": Ub1 @ Us[:, :r1].T, "Vh": Vb1 @ Vs[:, :r1].T, "U0": np.zeros_like(U0), "V0": np.zeros_like(V0),
+           "a": 0.0, "b": 0.0}
+    Shb = svd_vjp(Us, sig, Vs, Uh.T @ Ub1, np.diag(Sb1).copy(), Vh.T @ Vb1)
+
+    def vjp_S(S, sb):
+        Y = Uh @ S @ Vh.T
+        Gb, da, db = F.vjp(Y, Uh @ sb @ Vh.T)
+        FY = F(Y)
+        acc["Uh"] += FY @ Vh @ sb.T + Gb @ Vh @ S.T
+        acc["Vh"] += FY.T @ Uh @ sb + Gb.T @ Uh @ S
+        acc["a"] += da
+        acc["b"] += db
+        return Uh.T @ Gb @ Vh
+
+    def vjp_K(K, kb):
+        Y = K @ V0.T
+        Gb, da, db = F.vjp(Y, kb @ V0.T)
+        acc["V0"] += F(Y).T @ kb + Gb.T @ K
+        acc["a"] += da
+        acc["b"] += db
+        return Gb @ V0
+
+    def vjp_L(L, lb):
+        Y = U0 @ L.T
+        Gb, da, db = F.vjp(Y, U0 @ lb.T)
+        acc["U0"] += F(Y) @ lb + Gb @ L
+        acc["a"] += da
+        acc["b"] += db
+        return Gb.T @ U0
+
+    Sib = rk4_vjp(fS, vjp_S, Ss, h, Shb)
+    Mb = Sib @ Nn @ S0.T
+    Nb = Sib.T @ M @ S0
+    Sb0 = M.T @ Sib @ Nn
+    Uhb = acc["Uh"] + U0 @ Mb.T
+    Vhb = acc["Vh"] + V0 @ Nb.T
+    Ub0 = Uhb[:, :r] + Uh @ Mb
+    Vb0 = Vhb[:, :r] + Vh @ Nb
+    # Q = Q((I - B B') X1): Zbar by the thin-QR rule, then Z = X1 - B (B'X1)
+    Xb = []
+    for B, X1, Qc, Rc, Qb in ((U0, K1, Qu, Ru, Uhb[:, r:]), (V0, L1, Qv, Rv, Vhb[:, r:])):
+        Zb = qr_vjp(Qc, Rc, Qb)
+        Xb.append(Zb - B @ (B.T @ Zb))
+        Bb = -Zb @ (X1.T @ B) - X1 @ (Zb.T @ B)
+        if B is U0:
+            Ub0 = Ub0 + Bb
+        else:
+            Vb0 = Vb0 + Bb
+    K0b = rk4_vjp(fK, vjp_K, Ks, h, Xb[0])
+    L0b = rk4_vjp(fL, vjp_L, Ls, h, Xb[1])
+    Ub0 = Ub0 + K0b @ S0.T + acc["U0"]
+    Vb0 = Vb0 + L0b @ S0 + acc["V0"]
+    Sb0 = Sb0 + U0.T @ K0b + L0b.T @ V0
+    return Ub0, Sb0, Vb0, acc["a"], acc["b"], r1
+
+
 def bug_de_step_jvp(F: ScaledQuadratic, U0, S0, V0, dt, N, Ud, Sd, Vd, ad=0.0, bd=0.0):
     """Forward-mode tangents (U1d, S1d, V1d) of the step along (Ud, Sd, Vd, ad, bd)."""
     h = dt / N
@@ -368,12 +477,20 @@ def bug_de_solve(F, U, S, V, dt, N, steps, ksl=False):
     return U, S, V
 
 
-def bug_de_solve_gradient(F: ScaledQuadratic, U0, S0, V0, dt, N, steps, loss_grad, checkpoint=20, ksl=False):
-    """Gradient of loss(U_T, S_T, V_T) after `steps` BUG (ksl: primal KSL) steps with
-    respect to (U0, S0, V0) and the scales (a, b), with checkpoints every `checkpoint`
-    steps and recomputation inside each segment.  Returns (value, Ub0, Sb0, Vb0, abar, bbar)."""
-    bug_de_step_ = ksl_primal_de_step if ksl else bug_de_step
-    bug_de_step_vjp_ = ksl_primal_de_step_vjp if ksl else bug_de_step_vjp
+def bug_de_solve_gradient(F: ScaledQuadratic, U0, S0, V0, dt, N, steps, loss_grad, checkpoint=20, ksl=False,
+                          ra=None):
+    """Gradient of loss(U_T, S_T, V_T) after `steps` BUG (ksl: primal KSL; ra = (tol, rmax):
+    rank-adaptive BUG, the ranks of the forward solve held) steps with respect to
+    (U0, S0, V0) and the scales (a, b), with checkpoints every `checkpoint` steps and
+    recomputation inside each segment.  Returns (value, Ub0, Sb0, Vb0, abar, bbar)."""
+    if ra is not None:
+        tol, rmax = ra
+        bug_de_step_ = lambda F_, U, S, V, dt_, N_: ra_bug_de_step(F_, U, S, V, dt_, N_, tol, rmax)[:3]  # noqa: E731
+        bug_de_step_vjp_ = lambda F_, U, S, V, dt_, N_, *c: ra_bug_de_step_vjp(  # noqa: E731
+            F_, U, S, V, dt_, N_, tol, rmax, *c)[:5]
+    else:
+        bug_de_step_ = ksl_primal_de_step if ksl else bug_de_step
+        bug_de_step_vjp_ = ksl_primal_de_step_vjp if ksl else bug_de_step_vjp
     cps = {0: (U0, S0, V0)}
     U, S, V = U0, S0, V0
     for k in range(steps):

@@ -60,6 +60,8 @@ int main(void) {
   expect(lri_step_ksl_adjoint(NULL, 0, 1, x, 4, x, 1, x, 4) == LRI_EINVAL, "step_ksl_adjoint(NULL)");
   expect(lri_step_ksl_dual_adjoint(NULL, 0, 1, x, 4, x, 1, x, 4) == LRI_EINVAL, "step_ksl_dual_adjoint(NULL)");
   expect(lri_trajectory_get_range(NULL, 0, 1, &r, x) == LRI_EINVAL, "trajectory_get_range(NULL)");
+  expect(lri_step_de_bug_ra_adjoint(NULL, 1e-3, 1, 1e-8, 4, 1, x, 4, x, 1, x, 4, NULL) == LRI_EINVAL,
+         "step_de_bug_ra_adjoint(NULL)");
   expect(lri_step_ksl_strang_adjoint(NULL, 0, 1, 2, x, 4, x, 1, x, 4) == LRI_EINVAL,
          "step_ksl_strang_adjoint(NULL)");
   expect(lri_step_greedy(NULL, 1) == LRI_EINVAL, "step_greedy(NULL)");

@@ -251,3 +251,77 @@ def test_ksl_checkpointed_burgers_gradient_vs_oracle(lri):
     say(f"  loss {val:.6e} (oracle {rv:.6e}); rel err dL/dxi {e_xi:.1e}, dL/dnu {e_nu:.1e}")
     assert abs(val - rv) <= 1e-10 * abs(rv)
     assert e_xi <= 1e-8 and e_nu <= 1e-8
+
+
+def ra_context(lri, dev, U, S, V):
+    """A context of capacity 2r (the rank-adaptive step's largest new rank)."""
+    n, m, r = U.shape[0], V.shape[0], S.shape[0]
+    ctx = lri.native.Context(n, m, 2 * r)
+    ctx.set_state(U, S, V)
+    q = dev.G.shape[1]
+    ctx.set_operator_lyapunov(dev.A.d, dev.A.e, dev.A.periodic, dev.B.d, dev.B.e, dev.B.periodic,
+                              dev.G if q else None, dev.H if q else None)
+    ctx.set_operator_quadratic(dev.D.dl, dev.D.dd, dev.D.du, dev.D.periodic)
+    return ctx
+
+
+@pytest.mark.parametrize("periodic,q,shape,dt", [(True, 1, (50, 37, 4), 0.01), (False, 0, (129, 70, 8), 0.03),
+                                                 (True, 2, (300, 200, 16), 0.05)])
+def test_ra_one_step_adjoint_vs_oracle(lri, periodic, q, shape, dt):
+    """lri_step_de_bug_ra_adjoint against oracle/adjoint_de.ra_bug_de_step_vjp for the
+    basis-free loss <C, U1 S1 V1'> + <c, sigma> (each side's cotangents from its own
+    factors, whose singular-vector signs may differ); the rank held at rmax = r + 2."""
+    n, m, r = shape
+    say(f"case: rank-adaptive matrix-ODE adjoint, one step, n={n} m={m} r={r} q={q} periodic={periodic}")
+    dev, host = random_operator(lri, n, m, q, periodic, seed=13)
+    rng = np.random.default_rng(14)
+    U, S, V = random_state(rng, n, m, r)
+    N = lri.rk4_substeps(dev.spectral_bound(2 * np.abs(U @ S @ V.T).max()), dt)
+    tol, rmax = 1e-12, r + 2
+    with ra_context(lri, dev, U, S, V) as ctx:
+        r1 = ctx.step_de_bug_ra(dt, N, tol, rmax)
+        U1e, S1e, V1e = ctx.get_state()
+    U1o, S1o, V1o, r1o = ad.ra_bug_de_step(host, U, S, V, dt, N, tol, rmax)
+    assert r1 == r1o == rmax
+    e_fwd = rel(U1e @ S1e @ V1e.T, U1o @ S1o @ V1o.T)
+    C = rng.standard_normal((n, m))
+    cs = rng.standard_normal(r1)
+
+    def cot(U1, S1, V1):
+        return C @ V1 @ S1.T, U1.T @ C @ V1 + np.diag(cs), C.T @ U1 @ S1
+
+    with ra_context(lri, dev, U, S, V) as ctx:
+        Ub, Sb, Vb, (a, b) = ctx.step_de_bug_ra_adjoint(dt, N, tol, rmax, *cot(U1e, S1e, V1e))
+        U2, S2, V2 = ctx.get_state()
+    assert np.array_equal(U2, U) and np.array_equal(S2, S) and np.array_equal(V2, V)  # state unchanged
+    ref = ad.ra_bug_de_step_vjp(host, U, S, V, dt, N, tol, rmax, *cot(U1o, S1o, V1o))
+    errs = [rel(tangential(U, Ub), tangential(U, ref[0])), rel(Sb, ref[1]),
+            rel(tangential(V, Vb), tangential(V, ref[2])), abs(a - ref[3]) / abs(ref[3]),
+            abs(b - ref[4]) / max(abs(ref[4]), 1e-300)]
+    say(f"  N={N} r1={r1}: forward {e_fwd:.1e}; rel err Ub {errs[0]:.1e} Sb {errs[1]:.1e} Vb {errs[2]:.1e} "
+        f"abar {errs[3]:.1e} bbar {errs[4]:.1e}")
+    assert e_fwd <= 1e-11
+    assert max(errs) <= 1e-8
+
+
+def test_ra_checkpointed_solve_gradient_vs_oracle(lri):
+    """solve_gradient with RankAdaptiveUnconventionalAlgorithm on a matrix ODE (the ranks
+    of the forward solve held fixed) against the oracle's checkpointed reverse."""
+    n, m, r, dt, steps = 60, 45, 3, 0.02, 5
+    tol, rmax = 1e-3, 6
+    dev, host = random_operator(lri, n, m, 1, True, seed=15)
+    U, S, V = random_state(np.random.default_rng(16), n, m, r)
+    prob = lri.MatrixDEProblem(dev, lri.SVDLikeRepresentation(U, S, V), (0.0, steps * dt))
+    N = lri.de_substeps(prob, dt)
+    say(f"case: rank-adaptive matrix-ODE solve gradient n={n} m={m} r0={r} rmax={rmax}, {steps} steps of {N} "
+        f"RK4 sub-steps, checkpoint 2")
+    val, (Ub, Sb, Vb), (a, b) = lri.solve_gradient(prob, lri.RankAdaptiveUnconventionalAlgorithm(tol, rmax=rmax),
+                                                   lri.mean_column_loss, dt, checkpoint=2)
+    rv, rUb, rSb, rVb, ra_, rb_ = ad.bug_de_solve_gradient(host, U, S, V, dt, N, steps, ad.mean_column_loss,
+                                                           checkpoint=2, ra=(tol, rmax))
+    errs = [rel(tangential(U, Ub), tangential(U, rUb)), rel(Sb, rSb), rel(tangential(V, Vb), tangential(V, rVb)),
+            abs(a - ra_) / abs(ra_), abs(b - rb_) / max(abs(rb_), 1e-300)]
+    say(f"  loss {val:.6e} (oracle {rv:.6e}); rel err Ub {errs[0]:.1e} Sb {errs[1]:.1e} Vb {errs[2]:.1e} "
+        f"abar {errs[3]:.1e} bbar {errs[4]:.1e}")
+    assert abs(val - rv) <= 1e-10 * abs(rv)
+    assert max(errs) <= 1e-8

@@ -128,7 +128,7 @@ def test_every_ccall_matches_the_header():
                      "lri_step_greedy", "lri_step_greedy_twofactor", "lri_set_operator_lyapunov",
                      "lri_set_operator_quadratic", "lri_step_de_ksl", "lri_step_de_bug", "lri_step_de_bug_ra",
                      "lri_step_bug_adjoint", "lri_step_ksl_adjoint", "lri_step_ksl_dual_adjoint", "lri_step_bug_ra_adjoint",
-                     "lri_step_de_bug_adjoint", "lri_step_de_ksl_adjoint", "lri_trajectory_reserve",
+                     "lri_step_de_bug_adjoint", "lri_step_de_bug_ra_adjoint", "lri_step_de_ksl_adjoint", "lri_trajectory_reserve",
                      "lri_trajectory_save", "lri_trajectory_get", "lri_trajectory_get_range", "lri_last_error"):
         assert required in seen, f"{required} is not called from the Julia module"
 

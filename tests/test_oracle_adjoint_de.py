@@ -193,3 +193,104 @@ def test_ksl_step_vjp_matches_finite_differences_and_forward_mode(periodic):
     assert abs(fa - abar) <= 1e-7 * max(1.0, abs(abar)) and abs(fb - bbar) <= 1e-7 * max(1.0, abs(bbar))
     J = ad.ksl_primal_de_step_jvp(F, U, S, V, dt, N, *dirs, 0.4, -0.2)
     assert abs(pair(C, J) - (an + 0.4 * abar - 0.2 * bbar)) <= 1e-11 * max(1.0, abs(an))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_ra_step_vjp_matches_finite_differences(periodic):
+    """Rank-adaptive step (rank_adaptive_unconventional.jl:194-233 with the DE
+    sub-problems): the augmented form [U0 | Q((I - U0 U0') K1)] gives the reference
+    ordering's U1 S1 V1' (dlra_de, Q([K1 U0])); its reverse against central
+    differences of the basis-free loss <C, U1 S1 V1'> + <c, sigma> in generic
+    directions and along the scales, and against the reference ordering along
+    tangents of the orthonormal bases; the rank is held by a clear gap."""
+    rng = np.random.default_rng(5 + periodic)
+    n, m, r, dt, N = 14, 11, 3, 0.03, 3
+    F = random_op(rng, n, m, periodic=periodic)
+    U, S, V = random_state(rng, n, m, r)
+    tol, rmax = 1e-10, 4
+    U1, S1, V1, r1 = ad.ra_bug_de_step(F, U, S, V, dt, N, tol, rmax)
+    from oracle import dlra_de
+    Ur, Sr, Vr, rr = dlra_de.rank_adaptive_bug_de_step(lambda Y, t: F(Y), U, S, V, 0.0, dt, N, tol, rmax)
+    assert r1 == rr == 4
+    assert np.abs(U1 @ S1 @ V1.T - Ur @ Sr @ Vr.T).max() <= 1e-14
+    C = rng.standard_normal((n, m))
+    cs = rng.standard_normal(r1)
+    Ub1, Vb1 = C @ V1 @ S1.T, C.T @ U1 @ S1
+    Sb1 = U1.T @ C @ V1 + np.diag(cs)
+    Ub0, Sb0, Vb0, abar, bbar, r1b = ad.ra_bug_de_step_vjp(F, U, S, V, dt, N, tol, rmax, Ub1, Sb1, Vb1)
+    assert r1b == r1
+
+    def loss(step, U_, S_, V_, a=1.0, b=1.0):
+        a_, b_, c_, q = step(F.scaled(a, b), U_, S_, V_)
+        assert q == r1
+        return float(np.sum(C * (a_ @ b_ @ c_.T))) + float(cs @ np.diag(b_))
+
+    aug = lambda F_, U_, S_, V_: ad.ra_bug_de_step(F_, U_, S_, V_, dt, N, tol, rmax)  # noqa: E731
+    ref = lambda F_, U_, S_, V_: dlra_de.rank_adaptive_bug_de_step(  # noqa: E731
+        lambda Y, t: F_(Y), U_, S_, V_, 0.0, dt, N, tol, rmax)
+    eps = 1e-6
+    x = (U, S, V)
+    D = [rng.standard_normal(a.shape) for a in x]
+    fd = (loss(aug, *(a + eps * d for a, d in zip(x, D))) - loss(aug, *(a - eps * d for a, d in zip(x, D)))) / (2 * eps)
+    an = pair((Ub0, Sb0, Vb0), D)
+    assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
+    fa = (loss(aug, *x, 1 + eps) - loss(aug, *x, 1 - eps)) / (2 * eps)
+    fb = (loss(aug, *x, 1, 1 + eps) - loss(aug, *x, 1, 1 - eps)) / (2 * eps)
+    assert abs(fa - abar) <= 1e-6 * max(1.0, abs(abar))
+    assert abs(fb - bbar) <= 1e-6 * max(1.0, abs(bbar))
+
+    def tangent(X):  # along the orthonormal bases: (I - XX') G + X (W - W')
+        G, W = rng.standard_normal(X.shape), rng.standard_normal((X.shape[1], X.shape[1]))
+        return G - X @ (X.T @ G) + X @ (W - W.T)
+
+    D = [tangent(U), rng.standard_normal(S.shape), tangent(V)]
+    fd = (loss(ref, *(a + eps * d for a, d in zip(x, D))) - loss(ref, *(a - eps * d for a, d in zip(x, D)))) / (2 * eps)
+    assert abs(fd - pair((Ub0, Sb0, Vb0), D)) <= 1e-5 * max(1.0, abs(fd))
+
+
+def test_ra_checkpointed_solve_gradient_matches_finite_differences():
+    """Several rank-adaptive steps (the rank grows from 3): the checkpointed reverse
+    against central differences of the forward solve, the ranks held by clear gaps."""
+    rng = np.random.default_rng(9)
+    n, m, r, dt, N, steps = 14, 11, 3, 0.02, 2, 4
+    F = random_op(rng, n, m, periodic=True)
+    U, S, V = random_state(rng, n, m, r)
+    tol, rmax = 1e-3, 5
+    lg = ad.mean_column_loss
+
+    def run(F_, x):
+        U_, S_, V_ = x
+        ranks = []
+        for _ in range(steps):
+            U_, S_, V_, r1 = ad.ra_bug_de_step(F_, U_, S_, V_, dt, N, tol, rmax)
+            ranks.append(r1)
+        return (U_, S_, V_), ranks
+
+    (UT, ST, VT), ranks = run(F, (U, S, V))
+    assert ranks[0] > r and len(set(ranks)) >= 1
+    val, Ub, Sb, Vb, abar, bbar = ad.bug_de_solve_gradient(F, U, S, V, dt, N, steps, lg, checkpoint=3,
+                                                           ra=(tol, rmax))
+    _, Ub2, Sb2, Vb2, abar2, _ = ad.bug_de_solve_gradient(F, U, S, V, dt, N, steps, lg, checkpoint=1,
+                                                          ra=(tol, rmax))
+    for a, b in ((Ub, Ub2), (Sb, Sb2), (Vb, Vb2)):
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+    assert val == pytest.approx(lg(UT, ST, VT)[0], rel=1e-14)
+
+    def loss(x, a=1.0):
+        y, rk = run(F.scaled(a, 1.0), x)
+        assert rk == ranks
+        return lg(*y)[0]
+
+    def tangent(X):  # along the orthonormal bases: (I - XX') G + X (W - W')
+        G, W = rng.standard_normal(X.shape), rng.standard_normal((X.shape[1], X.shape[1]))
+        return G - X @ (X.T @ G) + X @ (W - W.T)
+
+    # directions along the orthonormal bases (the composition of several steps is
+    # badly conditioned off them, where a solve never goes)
+    eps = 1e-6
+    for dirs in ([tangent(U), 0 * S, 0 * V], [0 * U, rng.standard_normal(S.shape), 0 * V], [0 * U, 0 * S, tangent(V)]):
+        fd = (loss([x + eps * d for x, d in zip((U, S, V), dirs)])
+              - loss([x - eps * d for x, d in zip((U, S, V), dirs)])) / (2 * eps)
+        assert abs(fd - pair((Ub, Sb, Vb), dirs)) <= 1e-4 * abs(fd) + 1e-9
+    fa = (loss((U, S, V), 1 + eps) - loss((U, S, V), 1 - eps)) / (2 * eps)
+    assert abs(fa - abar) <= 1e-4 * abs(fa) + 1e-9
