@@ -72,8 +72,29 @@ def expm_small(torch, X, order=10):
     return E
 
 
-def generate_snapshots(torch, n, r, count, seed, device):
-    """BF16 snapshots A_k (column-major n x n: stored as row-major A_k')."""
+def padded_ld(n, elem_bytes):
+    """A leading dimension for n-row columns that is not a multiple of 4 KB: a
+    power-of-two column pitch (n = 16384: 32 KB in BF16) makes some pairs of
+    snapshots stream at ~70 % of the others' rate in the passes, which read two
+    snapshots at the same offsets at once; 64 elements of padding per column
+    (128 B in BF16, never read) remove it."""
+    ld = (n + 63) // 64 * 64
+    return ld + 64 if (ld * elem_bytes) % 4096 == 0 else ld
+
+
+def padded_copy(torch, t, dtype, ld):
+    """t (a column-major n x n matrix stored as a row-major (n, n) tensor) as dtype in an
+    (n, ld) buffer; returns the (n, n) view (row stride ld)."""
+    buf = torch.empty((t.shape[0], ld), dtype=dtype, device=t.device)
+    view = buf[:, :t.shape[1]]
+    view.copy_(t)
+    return view
+
+
+def generate_snapshots(torch, n, r, count, seed, device, ld=None):
+    """BF16 snapshots A_k (column-major n x n: stored as row-major A_k'), each in an
+    (n, ld) buffer (ld >= n; the (n, n) views are returned, row stride ld)."""
+    ld = n if ld is None else ld
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
     dt = 1.0 / (count - 1)
@@ -94,7 +115,7 @@ def generate_snapshots(torch, n, r, count, seed, device):
         # A_k' = Va_k diag(sig) Ua_k' + e^t D_k,  D_k = C_k'
         lowrank = ((Va * sig) @ Ua.t()).float()
         A_t = lowrank.add_(D, alpha=math.exp(t))
-        snaps.append(A_t.to(torch.bfloat16))
+        snaps.append(A_t.to(torch.bfloat16) if ld == n else padded_copy(torch, A_t, torch.bfloat16, ld))
         del A_t, lowrank
         if k + 1 < count:
             # D_{k+1} = (E1 C_k E2')' = E2 D_k E1'
@@ -198,7 +219,7 @@ def run_adjoint(args, torch, lri, snaps, U0, S0, V0, n, r):
     Sb = rng.standard_normal((r, r))
     Vb = rng.standard_normal((n, r))
     ctx = lri.native.Context(n, n, r, 0)
-    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, n, owners=snaps)
+    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, snaps[0].stride(0), owners=snaps)
     ctx.set_precision(lri.native.PREC_MIXED)
     ctx.set_state(U0, S0, V0)
     steps = min(args.steps, 100)
@@ -312,7 +333,7 @@ def run_greedy(args, torch, lri, snaps, U0, S0, V0, n, r):
     """Greedy integrator (SVD-like state) over the BF16 snapshots: 3 single-snapshot passes per step."""
     N = len(snaps)
     ctx = lri.native.Context(n, n, r, 0)
-    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, n, owners=snaps)
+    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, snaps[0].stride(0), owners=snaps)
     ctx.set_precision(lri.native.PREC_MIXED)
     ctx.set_state(U0, S0, V0)
 
@@ -356,9 +377,10 @@ def run_greedy(args, torch, lri, snaps, U0, S0, V0, n, r):
         "dtype": "bf16",
         "data": "synthetic",
         "config": {"workload": f"GreedyIntegrator on a MatrixDataProblem, n=m={n}, r={r}, {N} BF16 snapshots "
-                               f"resident in HBM (X = A[k] itself), BF16x3 matrix-core products with FP32 "
-                               f"accumulation, FP64 shifted CholeskyQR3", "n": n, "m": n, "r": r, "snapshots": N,
-                   "snapshot_dtype": "bf16", "precision": "mixed", "parallelism": "single GPU"},
+                               f"resident in HBM (X = A[k] itself, column pitch {snaps[0].stride(0)} elements), "
+                               f"BF16x3 matrix-core products with FP32 accumulation, FP64 shifted CholeskyQR3",
+                   "n": n, "m": n, "r": r, "snapshots": N, "snapshot_dtype": "bf16",
+                   "snapshot_ld": snaps[0].stride(0), "precision": "mixed", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None, "kernel": dom,
                      "bytes_per_launch": bytes_per_launch},
@@ -895,6 +917,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the FP64/FP32 forms")
     ap.add_argument("--no-solve", action="store_true", help="skip the solve() lines beside the headline")
     ap.add_argument("--no-check", action="store_true", help="skip the error check")
+    ap.add_argument("--snap-pitch", choices=("padded", "n"), default="padded",
+                    help="snapshot leading dimension: n + 64 elements where n elements are a multiple of 4 KB "
+                         "(padded), or n")
     ap.add_argument("--no-tune", action="store_true", help="keep the default pass plans (no autotuning)")
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP-event kernel timing in the timed region (for external profilers)")
@@ -965,7 +990,8 @@ def main():
 
     log(f"generating {N} BF16 snapshots n=m={n} on the GPU")
     tg = time.perf_counter()
-    snaps = generate_snapshots(torch, n, r, N, args.seed, dev)
+    snap_ld = padded_ld(n, 2) if args.snap_pitch == "padded" else n
+    snaps = generate_snapshots(torch, n, r, N, args.seed, dev, ld=snap_ld)
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - tg
     log(f"generation {gen_s:.1f} s; randomized SVD of A_0")
@@ -979,7 +1005,7 @@ def main():
         return
 
     ctx = lri.native.Context(n, m, r, 0)
-    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, n, owners=snaps)
+    ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, snaps[0].stride(0), owners=snaps)
     ctx.set_precision(lri.native.PREC_MIXED)
     ctx.set_state(U0, S0, V0)
     tune = None
@@ -1061,10 +1087,10 @@ def main():
         "dtype": "bf16",
         "data": "synthetic",
         "config": {"workload": f"KSL primal Lie-Trotter, MatrixDataProblem, n=m={n}, r={r}, {N} BF16 snapshots "
-                               f"resident in HBM ({npairs} steps per sweep), BF16x3 matrix-core products with FP32 "
-                               f"accumulation, FP64 CholeskyQR2/S-step",
-                   "n": n, "m": m, "r": r, "snapshots": N, "snapshot_dtype": "bf16", "precision": "mixed",
-                   "parallelism": "single GPU"},
+                               f"resident in HBM ({npairs} steps per sweep, column pitch {snap_ld} elements), "
+                               f"BF16x3 matrix-core products with FP32 accumulation, FP64 CholeskyQR2/S-step",
+                   "n": n, "m": m, "r": r, "snapshots": N, "snapshot_dtype": "bf16", "snapshot_ld": snap_ld,
+                   "precision": "mixed", "parallelism": "single GPU"},
         "roofline": roofline,
         "kernels": fam,
         "value_without_kernel_events": value_noev,
@@ -1101,9 +1127,11 @@ def main():
         R = min(args.ring, N)
         for form, tdt, ldt, prec in (("fp64", torch.float64, lri.native.F64, lri.native.PREC_F64),
                                      ("fp32", torch.float32, lri.native.F32, lri.native.PREC_MIXED)):
-            ring = [snaps[k].to(tdt) for k in range(R)]
+            esz = 8 if form == "fp64" else 4
+            ld_f = padded_ld(n, esz) if args.snap_pitch == "padded" else n
+            ring = [padded_copy(torch, snaps[k], tdt, ld_f) for k in range(R)]
             c2 = lri.native.Context(n, m, r, 0)
-            c2.bind([a.data_ptr() for a in ring], ldt, n, owners=ring)
+            c2.bind([a.data_ptr() for a in ring], ldt, ld_f, owners=ring)
             c2.set_precision(prec)
             c2.set_state(U0, S0, V0)
             run_steps(c2, lri, 4, R - 1)
@@ -1115,7 +1143,6 @@ def main():
             pT = c2.timing_read(lri.native.KERNEL_PASS_T)
             c2.timing(False)
             out[f"{form}_steps_per_s"] = ns / el2
-            esz = 8 if form == "fp64" else 4
             avg = max(pN[0] / max(pN[1], 1), pT[0] / max(pT[1], 1))
             out[f"{form}_pass_GBs"] = 2.0 * n * m * esz / (avg * 1e-3) / 1e9
             if form == "fp32" and not args.no_check:

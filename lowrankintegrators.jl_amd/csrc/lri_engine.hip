@@ -4300,8 +4300,12 @@ int lri_snapshots_reserve(lri_ctx* ctx, int64_t count, int dtype) {
     ctx->owned.clear();
     ctx->snaps.clear();
     ctx->sdtype = dtype;
-    ctx->sld = ctx->n;
-    const size_t bytes = dtype_size(dtype) * ctx->n * ctx->m;
+    // a column pitch that is a multiple of 4 KB makes some pairs of snapshots
+    // stream markedly slower in the passes (which read two at the same offsets
+    // at once): 64 elements of padding, never read, remove it
+    const int64_t es = (int64_t)dtype_size(dtype);
+    ctx->sld = ctx->n + ((ctx->n * es) % 4096 == 0 ? 64 : 0);
+    const size_t bytes = dtype_size(dtype) * ctx->sld * ctx->m;
     for (int64_t k = 0; k < count; ++k) {
       void* p = nullptr;
       HIPCHECK(hipMalloc(&p, bytes));
@@ -4319,7 +4323,7 @@ int lri_snapshot_upload(lri_ctx* ctx, int64_t k, const void* host, int64_t ld) {
     REQUIRE(k >= 0 && k < (int64_t)ctx->owned.size(), "slot out of range");
     REQUIRE(host && ld >= ctx->n, "bad host pointer or leading dimension");
     const size_t es = dtype_size(ctx->sdtype);
-    HIPCHECK(hipMemcpy2DAsync(ctx->owned[k], es * ctx->n, host, es * ld, es * ctx->n, ctx->m,
+    HIPCHECK(hipMemcpy2DAsync(ctx->owned[k], es * ctx->sld, host, es * ld, es * ctx->n, ctx->m,
                               hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     return LRI_OK;

@@ -1142,7 +1142,10 @@ static int tile_units(int mode, bool transposed, int rb) {
   int cb = rb / 32;
   if (mode == PASS_F32) return transposed ? 4 : (cb <= 1 ? 4 : (cb == 2 ? 2 : 1));  // pass T: the wide form
   if (transposed) return cb <= 1 ? 4 : 1;  // rb = 32: the wide pass T (128 columns per workgroup)
-  return cb <= 1 ? 4 : (cb == 2 ? 2 : 1);
+  // rb = 32: 64-row tiles (128 VGPRs + 32 AGPRs, 24 KB of LDS: four waves per SIMD) keep
+  // twice the loads in flight per CU of the 128-row tiles (two waves per SIMD); at
+  // n = m = 16384 pass N took 175 us against 192-204 us
+  return cb <= 1 ? 2 : (cb == 2 ? 2 : 1);
 }
 
 static int wg_span(int mode, bool transposed, int rb) {
@@ -1174,10 +1177,13 @@ int64_t plan_pass_chunk(int64_t out_rows, int64_t red_len, int rb, int64_t ch, i
   return s * rb * out_rows;
 }
 
-// the default: ~512 workgroups (2048 waves)
+// the default: ~512 workgroups (2048 waves); ~768 for the BF16 pass N at rb <= 32,
+// whose 64-row workgroups run four waves per SIMD (at n = m = 16384: 3 splits, 170
+// us against 196 us for 2)
 int64_t plan_pass(int mode, bool transposed, int64_t out_rows, int64_t red_len, int rb, int* splits,
                   int64_t* chunk) {
-  return plan_pass_chunk(out_rows, red_len, rb, chunk_for_target(mode, transposed, out_rows, red_len, rb, 512),
+  const int64_t target = (mode == PASS_BF16X3 && !transposed && rb <= 32) ? 768 : 512;
+  return plan_pass_chunk(out_rows, red_len, rb, chunk_for_target(mode, transposed, out_rows, red_len, rb, target),
                          splits, chunk);
 }
 
