@@ -73,11 +73,11 @@ def expm_small(torch, X, order=10):
 
 
 def padded_ld(n, elem_bytes):
-    """A leading dimension for n-row columns that is not a multiple of 4 KB: a
-    power-of-two column pitch (n = 16384: 32 KB in BF16) makes some pairs of
-    snapshots stream at ~70 % of the others' rate in the passes, which read two
-    snapshots at the same offsets at once; 64 elements of padding per column
-    (128 B in BF16, never read) remove it."""
+    """A leading dimension for n-row BF16 columns that is not a multiple of 4 KB: a
+    power-of-two column pitch (n = 16384: 32 KB) makes some pairs of snapshots
+    stream at ~70 % of the others' rate in the passes, which read two snapshots
+    at the same offsets at once; 64 elements of padding per column (128 B, never
+    read) remove it."""
     ld = (n + 63) // 64 * 64
     return ld + 64 if (ld * elem_bytes) % 4096 == 0 else ld
 
@@ -1128,7 +1128,7 @@ def main():
         for form, tdt, ldt, prec in (("fp64", torch.float64, lri.native.F64, lri.native.PREC_F64),
                                      ("fp32", torch.float32, lri.native.F32, lri.native.PREC_MIXED)):
             esz = 8 if form == "fp64" else 4
-            ld_f = padded_ld(n, esz) if args.snap_pitch == "padded" else n
+            ld_f = n  # the FP32 and FP64 passes gain nothing from the padding (FP32 lost 3 %)
             ring = [padded_copy(torch, snaps[k], tdt, ld_f) for k in range(R)]
             c2 = lri.native.Context(n, m, r, 0)
             c2.bind([a.data_ptr() for a in ring], ldt, ld_f, owners=ring)

@@ -110,9 +110,9 @@ int lri_get_rank(lri_ctx* ctx, int64_t* r_out);
 
 /* ---- data (update_data!, data_integrator.jl:22-28) --------------------- */
 
-/* Allocate `count` library-owned snapshot slots of type dtype, column pitch n,
- * or n + 64 elements where n elements are a multiple of 4 KB (a pitch that is a
- * multiple of 4 KB makes some pairs of snapshots stream markedly slower in the
+/* Allocate `count` library-owned snapshot slots of type dtype, column pitch n;
+ * BF16 slots take n + 64 elements where n elements are a multiple of 4 KB (such
+ * a pitch makes some pairs of BF16 snapshots stream markedly slower in the
  * passes, which read two at the same offsets at once; lri_snapshots_bind takes
  * the caller's pitch, and the same padding is recommended there). */
 int lri_snapshots_reserve(lri_ctx* ctx, int64_t count, int dtype);
