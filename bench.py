@@ -336,6 +336,13 @@ def run_greedy(args, torch, lri, snaps, U0, S0, V0, n, r):
     ctx.bind([a.data_ptr() for a in snaps], lri.native.BF16, snaps[0].stride(0), owners=snaps)
     ctx.set_precision(lri.native.PREC_MIXED)
     ctx.set_state(U0, S0, V0)
+    tune = None
+    if not args.no_tune:
+        # the single-operand passes (over A[k] itself: kp = -1)
+        tune = ctx.autotune_passes(-1, 1, reps=10)
+        log("Greedy: autotuned passes: " + ", ".join(
+            f"{k} chunk {v['chunk']} ({v['splits']} splits) {v['us']:.1f} us (default {v['default_us']:.1f} us)"
+            for k, v in tune.items()))
 
     def sweep(nsteps):
         for i in range(nsteps):
@@ -385,6 +392,7 @@ def run_greedy(args, torch, lri, snaps, U0, S0, V0, n, r):
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None, "kernel": dom,
                      "bytes_per_launch": bytes_per_launch},
         "kernels": fam,
+        "autotune": tune,
     }
     if not args.no_check:
         ctx.set_state(U0, S0, V0)
