@@ -578,3 +578,45 @@ def test_bitwise_deterministic(lri):
     b = run_solve(lri, u0, snaps, "primal", "mixed", "bf16")
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_snapshot_pitch_is_transparent(lri):
+    """BF16 snapshots in the library's slots (their pitch padded to n + 64 where n
+    elements are a multiple of 4 KB), bound as device arrays at pitch n, and bound at
+    pitch n + 64: the same bits after three mixed-precision KSL steps."""
+    import torch
+
+    n, m, r = 2048, 600, 8
+    d = rotating_data(n, m, r, noise=1e-3, seed=41)
+    u0 = d.initial()
+    snaps = [bf16_round(d(t)) for t in np.arange(4) / 199.0]
+
+    def run(setup):
+        with lri.native.Context(n, m, r) as ctx:
+            ctx.set_precision(lri.native.PREC_MIXED)
+            ctx.set_state(u0.U, u0.S, u0.V)
+            setup(ctx)
+            for k in range(3):
+                ctx.step_ksl(lri.native.PRIMAL, k, k + 1)
+            return ctx.get_state()
+
+    def owned(ctx):
+        ctx.reserve(len(snaps), lri.native.BF16)
+        for k, a in enumerate(snaps):
+            ctx.upload(k, lri.api._encode(a, "bf16"))
+
+    def bound(pad):
+        def setup(ctx):
+            ts = []
+            for a in snaps:  # column-major n x m: a row-major (m, n + pad) buffer, its first n columns
+                t = torch.zeros((m, n + pad), dtype=torch.bfloat16, device="cuda")
+                t[:, :n] = torch.from_numpy(np.ascontiguousarray(a.T).astype(np.float32)).cuda().to(torch.bfloat16)
+                ts.append(t)
+            torch.cuda.synchronize()
+            ctx.bind([t.data_ptr() for t in ts], lri.native.BF16, n + pad, owners=ts)
+        return setup
+
+    say(f"case: BF16 snapshot pitch n={n} m={m} r={r}: owned slots, bound at pitch {n} and {n + 64}")
+    a, b, c = run(owned), run(bound(0)), run(bound(64))
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
