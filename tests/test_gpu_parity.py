@@ -583,13 +583,25 @@ def test_bitwise_deterministic(lri):
 def test_snapshot_pitch_is_transparent(lri):
     """BF16 snapshots in the library's slots (their pitch padded to n + 64 where n
     elements are a multiple of 4 KB), bound as device arrays at pitch n, and bound at
-    pitch n + 64: the same bits after three mixed-precision KSL steps."""
-    import torch
+    pitch n + 64: the same bits after three mixed-precision KSL steps.  The bound
+    arrays come from the HIP runtime the library itself uses (hipMalloc, hipMemcpy2D)."""
+    import ctypes
 
     n, m, r = 2048, 600, 8
     d = rotating_data(n, m, r, noise=1e-3, seed=41)
     u0 = d.initial()
-    snaps = [bf16_round(d(t)) for t in np.arange(4) / 199.0]
+    snaps = [lri.api._encode(bf16_round(d(t)), "bf16") for t in np.arange(4) / 199.0]  # uint16, n x m
+    try:
+        hip = ctypes.CDLL("libamdhip64.so.7")
+    except OSError:
+        hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    hip.hipMemcpy2D.argtypes = [vp, ctypes.c_size_t, vp, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                ctypes.c_int]
+    hip.hipFree.argtypes = [vp]
+    host_to_device = 1
+    allocs = []
 
     def run(setup):
         with lri.native.Context(n, m, r) as ctx:
@@ -603,20 +615,26 @@ def test_snapshot_pitch_is_transparent(lri):
     def owned(ctx):
         ctx.reserve(len(snaps), lri.native.BF16)
         for k, a in enumerate(snaps):
-            ctx.upload(k, lri.api._encode(a, "bf16"))
+            ctx.upload(k, a)
 
     def bound(pad):
         def setup(ctx):
-            ts = []
-            for a in snaps:  # column-major n x m: a row-major (m, n + pad) buffer, its first n columns
-                t = torch.zeros((m, n + pad), dtype=torch.bfloat16, device="cuda")
-                t[:, :n] = torch.from_numpy(np.ascontiguousarray(a.T).astype(np.float32)).cuda().to(torch.bfloat16)
-                ts.append(t)
-            torch.cuda.synchronize()
-            ctx.bind([t.data_ptr() for t in ts], lri.native.BF16, n + pad, owners=ts)
+            ld = n + pad
+            ptrs = []
+            for a in snaps:
+                p = vp()
+                assert hip.hipMalloc(ctypes.byref(p), 2 * ld * m) == 0
+                allocs.append(p.value)
+                assert hip.hipMemcpy2D(p, 2 * ld, a.ctypes.data, 2 * n, 2 * n, m, host_to_device) == 0
+                ptrs.append(p.value)
+            ctx.bind(ptrs, lri.native.BF16, ld)
         return setup
 
     say(f"case: BF16 snapshot pitch n={n} m={m} r={r}: owned slots, bound at pitch {n} and {n + 64}")
-    a, b, c = run(owned), run(bound(0)), run(bound(64))
+    try:
+        a, b, c = run(owned), run(bound(0)), run(bound(64))
+    finally:
+        for p in allocs:
+            hip.hipFree(vp(p))
     for x, y, z in zip(a, b, c):
         assert np.array_equal(x, y) and np.array_equal(x, z)
