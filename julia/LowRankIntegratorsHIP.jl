@@ -797,8 +797,9 @@ end
 
 `lri_autotune_passes`: time every candidate reduction chunk of the two
 streaming passes over A[kn] - A[kp] (0-based slots) at the context's rank and
-precision and keep the fastest for this context.  Returns, per pass, the
-chosen chunk, its split count, and its and the default chunk's mean time (us).
+precision (three interleaved rounds, the median per candidate) and keep the
+fastest for this context, the default unless beaten by 1 %.  Returns, per pass,
+the chosen chunk, its split count, and its and the default chunk's median time (us).
 """
 function autotune!(c::HIPContext, kp::Integer, kn::Integer; reps::Integer = 10)
     rep = zeros(8)
